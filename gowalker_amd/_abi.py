@@ -1,0 +1,121 @@
+"""ctypes declarations of libgw.so's C ABI (include/gw.h).
+
+The library is loaded from gowalker_amd/_build/libgw.so; there is no fallback: if it is
+missing, `lib()` raises so that no caller silently runs something else.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libgw.so")
+
+GW_OK = 0
+GW_ERR = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "STATE", -5: "BUDGET", -6: "IO", -7: "FAULT"}
+GW_NODE2VEC, GW_SOPR = 0, 1
+GW_OUT_VISITS, GW_OUT_HASH, GW_OUT_PATHS = 1, 2, 4
+GW_RUN_BUCKETS = 1
+
+
+class GwConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("reserved", C.c_uint32), ("cache_bytes", C.c_uint64)]
+
+
+class GwBlockDesc(C.Structure):
+    _fields_ = [("blk", C.c_uint32), ("start_vert", C.c_uint32), ("nverts", C.c_uint32), ("pad", C.c_uint32),
+                ("start_edge", C.c_uint64), ("nedges", C.c_uint64)]
+
+
+class GwWalkType(C.Structure):
+    _fields_ = [("algo", C.c_uint32), ("length", C.c_uint32), ("p", C.c_double), ("q", C.c_double),
+                ("alpha", C.c_double), ("term", C.c_double)]
+
+
+class GwWalkParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("ntypes", C.c_uint32), ("reserved", C.c_uint32), ("type", GwWalkType * 2)]
+
+
+class GwLaunch(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("waves_per_cu", C.c_uint32), ("variant", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class GwRunStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("trials", C.c_uint64), ("probes", C.c_uint64), ("mix_steps", C.c_uint64),
+                ("walkers_run", C.c_uint64), ("walkers_ended", C.c_uint64), ("walkers_left", C.c_uint64),
+                ("kernel_ms", C.c_double), ("filter_ms", C.c_double)]
+
+
+class GwCounters(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("trials", C.c_uint64), ("probes", C.c_uint64), ("mix_steps", C.c_uint64),
+                ("h2d_bytes", C.c_uint64), ("h2d_uploads", C.c_uint64), ("h2d_ms", C.c_double),
+                ("kernel_ms", C.c_double), ("hbm_bytes_held", C.c_uint64)]
+
+
+class GwDeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("cus", C.c_uint32), ("wave_size", C.c_uint32), ("hbm_bytes", C.c_uint64),
+                ("lds_per_cu", C.c_uint32), ("pad", C.c_uint32)]
+
+
+P = C.c_void_p
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+
+# name -> (argtypes); every entry returns gw_status unless listed in _RESTYPE
+SIGNATURES = {
+    "gw_create": [C.POINTER(GwConfig), C.POINTER(P)],
+    "gw_destroy": [P],
+    "gw_last_error": [P],
+    "gw_device_info_get": [P, C.POINTER(GwDeviceInfo)],
+    "gw_alloc_pinned": [C.c_size_t, C.POINTER(P)],
+    "gw_free_pinned": [P],
+    "gw_set_block_table": [P, C.POINTER(GwBlockDesc), C.c_uint32],
+    "gw_reserve_slots": [P, C.c_uint32],
+    "gw_slot_bytes": [P, u64p],
+    "gw_upload_block": [P, C.c_uint32, C.c_uint32, P, P],
+    "gw_evict": [P, C.c_uint32],
+    "gw_wait_uploads": [P],
+    "gw_map_host_block": [P, C.c_uint32, P, P],
+    "gw_unmap_host_blocks": [P],
+    "gw_set_walk": [P, C.POINTER(GwWalkParams)],
+    "gw_init_walkers": [P, C.c_uint64, C.c_uint32, P, C.c_uint32],
+    "gw_run": [P, C.POINTER(GwLaunch), C.POINTER(GwRunStats)],
+    "gw_bucket_counts": [P, P, C.c_uint64],
+    "gw_walkers_left": [P, u64p],
+    "gw_read_visits": [P, P, C.c_uint64],
+    "gw_read_hashes": [P, P, C.c_uint64],
+    "gw_read_state": [P, P, C.c_uint64],
+    "gw_read_paths": [P, P, C.c_uint64],
+    "gw_counters_get": [P, C.POINTER(GwCounters)],
+    "gw_split_blocks": [P, C.c_uint64, C.c_uint64, P, P, C.c_uint64, u64p],
+}
+_RESTYPE = {"gw_last_error": C.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """Load libgw.so (built by `make -C gowalker_amd`).  Raises if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgw.so not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                               "or `make -C gowalker_amd` (expected %s)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, C.c_int32)
+        _lib = L
+    return _lib
+
+
+class GwError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("gw status %d (%s): %s" % (status, GW_ERR.get(status, "?"), msg))
+        self.status = status
+
+
+def check(status, ctx=None):
+    if status != GW_OK:
+        msg = lib().gw_last_error(ctx)
+        raise GwError(status, msg.decode() if msg else "")
+    return status

@@ -1,0 +1,770 @@
+// gw_api.cpp -- the C ABI of libgw.so (declared in include/gw.h): device context,
+// block table, HBM slot cache, walker state and the run loop over the gfx950 kernels.
+//
+// Replaces gpu/memory.hpp (initwalk, initgpugraph, initgpucache, cudarand,
+// copycachetogpu, pipeline_gpugraph) and the GPU half of graph_engine::pipeline
+// (engine/engine.hpp:406-457).  No call exits the process; errors are returned.
+#include "gw.h"
+#include "gw_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace gw;
+
+struct gw_ctx {
+    int device = 0;
+    hipStream_t s_compute = nullptr, s_copy = nullptr, s_read = nullptr;
+    hipEvent_t ev_copy = nullptr, ev_f0 = nullptr, ev_f1 = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    hipEvent_t ev_c0 = nullptr, ev_c1 = nullptr;
+    uint32_t ncu = 0;
+    gw_device_info info{};
+    uint64_t cap = 0, held = 0;
+    std::string err;
+
+    // block table
+    uint32_t nblocks = 0;
+    uint64_t nverts_total = 0;
+    std::vector<gw_block_desc> desc;
+    std::vector<DevBlock> hblk;      // host mirror of the device table
+    std::vector<int32_t> blk_slot;   // slot holding each block, -1 if none
+    std::vector<const uint64_t*> map_beg;
+    std::vector<const uint32_t*> map_csr;
+    DevBlock* d_blocks = nullptr;
+    DevBlock* h_blocks_pinned = nullptr;
+    uint32_t* d_vcut = nullptr;
+    std::vector<uint32_t> vcut;
+
+    // slots
+    uint32_t nslots = 0;
+    uint64_t slot_beg_bytes = 0, slot_csr_bytes = 0;
+    std::vector<uint64_t*> slot_beg;
+    std::vector<uint32_t*> slot_csr;
+    std::vector<int32_t> slot_blk;
+
+    // walk configuration
+    bool walk_set = false;
+    gw_walk_params wp{};
+    DevType dt[2]{};
+
+    // walkers
+    uint64_t nwalkers = 0;
+    uint32_t outputs = 0;
+    uint32_t path_stride = 0;
+    uint32_t *d_cur = nullptr, *d_prev = nullptr, *d_meta = nullptr, *d_bpair = nullptr;
+    uint64_t* d_hash = nullptr;
+    uint32_t* d_paths = nullptr;
+    uint32_t* d_visits = nullptr;
+    uint32_t* d_work = nullptr;
+    uint32_t* d_bucket = nullptr;
+    uint32_t* d_misc = nullptr;            // [0] nwork, [1] queue
+    unsigned long long* d_cnt = nullptr;   // 8 counters
+    unsigned long long* h_cnt = nullptr;   // pinned readback
+    uint64_t walker_bytes = 0;
+
+    // cumulative counters
+    gw_counters tot{};
+
+    // fault injection (GW_FAULT=upload:N or alloc:N fails the N-th such call)
+    int fault_kind = 0;   // 1 upload, 2 alloc
+    long fault_at = 0, n_upload = 0, n_alloc = 0;
+};
+
+static std::string g_err;
+
+static gw_status fail(gw_ctx* c, gw_status st, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf; else g_err = buf;
+    return st;
+}
+
+#define HIPCHK(c, call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((c), GW_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                               \
+    } while (0)
+
+static gw_status dalloc(gw_ctx* c, void** p, uint64_t bytes, bool counted)
+{
+    *p = nullptr;
+    if (bytes == 0) return GW_OK;
+    if (c->fault_kind == 2 && ++c->n_alloc == c->fault_at)
+        return fail(c, GW_ERR_FAULT, "injected allocation failure (GW_FAULT=alloc:%ld)", c->fault_at);
+    if (counted && c->cap && c->held + bytes > c->cap)
+        return fail(c, GW_ERR_BUDGET, "HBM budget exceeded: %llu held + %llu requested > %llu cap",
+                    (unsigned long long)c->held, (unsigned long long)bytes, (unsigned long long)c->cap);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, GW_ERR_NOMEM, "hipMalloc(%llu) failed: %s", (unsigned long long)bytes, hipGetErrorString(e));
+    }
+    if (counted) c->held += bytes;
+    return GW_OK;
+}
+
+static void dfree(gw_ctx* c, void* p, uint64_t bytes, bool counted)
+{
+    if (!p) return;
+    (void)hipFree(p);
+    if (counted) c->held -= std::min(c->held, bytes);
+}
+
+static void free_walkers(gw_ctx* c)
+{
+    dfree(c, c->d_cur, c->nwalkers * 4, true);
+    dfree(c, c->d_prev, c->nwalkers * 4, true);
+    dfree(c, c->d_meta, c->nwalkers * 4, true);
+    dfree(c, c->d_bpair, c->nwalkers * 4, true);
+    dfree(c, c->d_work, c->nwalkers * 4, true);
+    dfree(c, c->d_hash, c->nwalkers * 8, true);
+    dfree(c, c->d_paths, c->nwalkers * (uint64_t)c->path_stride * 4, true);
+    dfree(c, c->d_visits, c->nverts_total * 4, true);
+    c->d_cur = c->d_prev = c->d_meta = c->d_bpair = c->d_work = c->d_paths = c->d_visits = nullptr;
+    c->d_hash = nullptr;
+    c->nwalkers = 0;
+    c->walker_bytes = 0;
+}
+
+static void free_slots(gw_ctx* c)
+{
+    for (uint32_t s = 0; s < c->nslots; ++s) {
+        dfree(c, c->slot_beg[s], c->slot_beg_bytes, true);
+        dfree(c, c->slot_csr[s], c->slot_csr_bytes, true);
+    }
+    c->slot_beg.clear();
+    c->slot_csr.clear();
+    c->slot_blk.clear();
+    c->nslots = 0;
+    for (auto& b : c->blk_slot) b = -1;
+}
+
+extern "C" {
+
+const char* gw_last_error(const gw_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+gw_status gw_create(const gw_config* cfg, gw_ctx** out)
+{
+    if (!cfg || !out) return fail(nullptr, GW_ERR_ARG, "gw_create: null argument");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(nullptr, GW_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (cfg->device < 0 || cfg->device >= n)
+        return fail(nullptr, GW_ERR_ARG, "device %d out of range [0,%d)", cfg->device, n);
+    gw_ctx* c = new gw_ctx();
+    c->device = cfg->device;
+    c->cap = cfg->cache_bytes;
+    if (const char* f = getenv("GW_FAULT")) {
+        if (!strncmp(f, "upload:", 7)) { c->fault_kind = 1; c->fault_at = atol(f + 7); }
+        else if (!strncmp(f, "alloc:", 6)) { c->fault_kind = 2; c->fault_at = atol(f + 6); }
+    }
+    auto bail = [&](gw_status st) { gw_destroy(c); return st; };
+    if (hipSetDevice(c->device) != hipSuccess) return bail(fail(nullptr, GW_ERR_HIP, "hipSetDevice failed"));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess)
+        return bail(fail(nullptr, GW_ERR_HIP, "hipGetDeviceProperties failed"));
+    c->ncu = (uint32_t)prop.multiProcessorCount;
+    snprintf(c->info.name, sizeof c->info.name, "%s", prop.gcnArchName);
+    c->info.cus = c->ncu;
+    c->info.wave_size = (uint32_t)prop.warpSize;
+    c->info.hbm_bytes = prop.totalGlobalMem;
+    c->info.lds_per_cu = (uint32_t)prop.maxSharedMemoryPerMultiProcessor;
+    if (hipStreamCreateWithFlags(&c->s_compute, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->s_copy, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->s_read, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(nullptr, GW_ERR_HIP, "stream creation failed"));
+    hipEvent_t* evs[] = {&c->ev_copy, &c->ev_f0, &c->ev_f1, &c->ev_k0, &c->ev_k1, &c->ev_c0, &c->ev_c1};
+    for (hipEvent_t* ev : evs)
+        if (hipEventCreate(ev) != hipSuccess) return bail(fail(nullptr, GW_ERR_HIP, "event creation failed"));
+    void* p = nullptr;
+    if (dalloc(c, &p, 4 * sizeof(uint32_t), false) != GW_OK) return bail(fail(nullptr, GW_ERR_NOMEM, "%s", c->err.c_str()));
+    c->d_misc = (uint32_t*)p;
+    if (dalloc(c, &p, 8 * sizeof(unsigned long long), false) != GW_OK)
+        return bail(fail(nullptr, GW_ERR_NOMEM, "%s", c->err.c_str()));
+    c->d_cnt = (unsigned long long*)p;
+    if (hipHostMalloc((void**)&c->h_cnt, 16 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(nullptr, GW_ERR_NOMEM, "pinned counter buffer"));
+    *out = c;
+    return GW_OK;
+}
+
+gw_status gw_destroy(gw_ctx* c)
+{
+    if (!c) return GW_OK;
+    (void)hipSetDevice(c->device);
+    if (c->s_compute) (void)hipStreamSynchronize(c->s_compute);
+    if (c->s_copy) (void)hipStreamSynchronize(c->s_copy);
+    free_walkers(c);
+    free_slots(c);
+    dfree(c, c->d_bucket, 0, false);
+    dfree(c, c->d_blocks, 0, false);
+    dfree(c, c->d_vcut, 0, false);
+    dfree(c, c->d_misc, 0, false);
+    dfree(c, c->d_cnt, 0, false);
+    if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+    if (c->h_blocks_pinned) (void)hipHostFree(c->h_blocks_pinned);
+    hipEvent_t evs[] = {c->ev_copy, c->ev_f0, c->ev_f1, c->ev_k0, c->ev_k1, c->ev_c0, c->ev_c1};
+    for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
+    if (c->s_compute) (void)hipStreamDestroy(c->s_compute);
+    if (c->s_copy) (void)hipStreamDestroy(c->s_copy);
+    if (c->s_read) (void)hipStreamDestroy(c->s_read);
+    delete c;
+    return GW_OK;
+}
+
+gw_status gw_device_info_get(gw_ctx* c, gw_device_info* out)
+{
+    if (!c || !out) return fail(c, GW_ERR_ARG, "null argument");
+    *out = c->info;
+    return GW_OK;
+}
+
+gw_status gw_alloc_pinned(size_t bytes, void** out)
+{
+    if (!out) return fail(nullptr, GW_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (bytes == 0) return GW_OK;
+    hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(nullptr, GW_ERR_NOMEM, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return GW_OK;
+}
+
+gw_status gw_free_pinned(void* p)
+{
+    if (p && hipHostFree(p) != hipSuccess) return fail(nullptr, GW_ERR_HIP, "hipHostFree failed");
+    return GW_OK;
+}
+
+gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
+{
+    if (!c || !b || nb == 0) return fail(c, GW_ERR_ARG, "gw_set_block_table: empty table");
+    if (nb > 65535) return fail(c, GW_ERR_ARG, "at most 65535 blocks are supported (got %u)", nb);
+    for (uint32_t i = 0; i < nb; ++i) {
+        if (b[i].blk != i) return fail(c, GW_ERR_ARG, "block %u has id %u (ids must be 0..n-1 in order)", i, b[i].blk);
+        if (i && b[i].start_vert != b[i - 1].start_vert + b[i - 1].nverts)
+            return fail(c, GW_ERR_ARG, "block %u does not start where block %u ends", i, i - 1);
+    }
+    if (c->nwalkers) return fail(c, GW_ERR_STATE, "gw_set_block_table after gw_init_walkers");
+    (void)hipSetDevice(c->device);
+    free_slots(c);
+    dfree(c, c->d_blocks, 0, false);
+    dfree(c, c->d_vcut, 0, false);
+    dfree(c, c->d_bucket, 0, false);
+    c->d_blocks = nullptr; c->d_vcut = nullptr; c->d_bucket = nullptr;
+    if (c->h_blocks_pinned) { (void)hipHostFree(c->h_blocks_pinned); c->h_blocks_pinned = nullptr; }
+    c->nblocks = nb;
+    c->desc.assign(b, b + nb);
+    c->hblk.assign(nb, DevBlock{});
+    c->blk_slot.assign(nb, -1);
+    c->map_beg.assign(nb, nullptr);
+    c->map_csr.assign(nb, nullptr);
+    c->vcut.resize(nb + 1);
+    for (uint32_t i = 0; i < nb; ++i) {
+        DevBlock& d = c->hblk[i];
+        d.start_vert = b[i].start_vert;
+        d.nverts = b[i].nverts;
+        d.start_edge = b[i].start_edge;
+        d.nedges = b[i].nedges;
+        c->vcut[i] = b[i].start_vert;
+    }
+    c->vcut[nb] = b[nb - 1].start_vert + b[nb - 1].nverts;
+    c->nverts_total = c->vcut[nb];
+    void* p;
+    gw_status st;
+    if ((st = dalloc(c, &p, sizeof(DevBlock) * nb, false)) != GW_OK) return st;
+    c->d_blocks = (DevBlock*)p;
+    if ((st = dalloc(c, &p, sizeof(uint32_t) * (nb + 1), false)) != GW_OK) return st;
+    c->d_vcut = (uint32_t*)p;
+    if ((st = dalloc(c, &p, sizeof(uint32_t) * (uint64_t)nb * nb, false)) != GW_OK) return st;
+    c->d_bucket = (uint32_t*)p;
+    HIPCHK(c, hipHostMalloc((void**)&c->h_blocks_pinned, sizeof(DevBlock) * nb, hipHostMallocDefault));
+    HIPCHK(c, hipMemcpy(c->d_vcut, c->vcut.data(), sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice));
+    return GW_OK;
+}
+
+gw_status gw_slot_bytes(gw_ctx* c, uint64_t* bytes)
+{
+    if (!c || !bytes) return fail(c, GW_ERR_ARG, "null argument");
+    if (!c->nblocks) return fail(c, GW_ERR_STATE, "no block table");
+    uint64_t mb = 0, me = 0;
+    for (auto& d : c->desc) { mb = std::max<uint64_t>(mb, (uint64_t)d.nverts + 1); me = std::max<uint64_t>(me, d.nedges); }
+    *bytes = ((mb * 8 + 255) & ~255ULL) + ((me * 4 + 16 + 255) & ~255ULL);
+    return GW_OK;
+}
+
+gw_status gw_reserve_slots(gw_ctx* c, uint32_t nslots)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (!c->nblocks) return fail(c, GW_ERR_STATE, "gw_reserve_slots before gw_set_block_table");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->s_copy));
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    free_slots(c);
+    for (auto& d : c->hblk) { d.resident = 0; d.beg = nullptr; d.csr = nullptr; }
+    uint64_t mb = 0, me = 0;
+    for (auto& d : c->desc) { mb = std::max<uint64_t>(mb, (uint64_t)d.nverts + 1); me = std::max<uint64_t>(me, d.nedges); }
+    c->slot_beg_bytes = (mb * 8 + 255) & ~255ULL;
+    c->slot_csr_bytes = (me * 4 + 16 + 255) & ~255ULL;   // 16 B tail pad
+    uint64_t need = (uint64_t)nslots * (c->slot_beg_bytes + c->slot_csr_bytes);
+    if (c->cap && c->held + need > c->cap)
+        return fail(c, GW_ERR_BUDGET, "%u slots of %llu B plus %llu B held exceed the %llu B cache budget", nslots,
+                    (unsigned long long)(c->slot_beg_bytes + c->slot_csr_bytes), (unsigned long long)c->held,
+                    (unsigned long long)c->cap);
+    c->slot_beg.assign(nslots, nullptr);
+    c->slot_csr.assign(nslots, nullptr);
+    c->slot_blk.assign(nslots, -1);
+    c->nslots = nslots;
+    for (uint32_t s = 0; s < nslots; ++s) {
+        void* p;
+        gw_status st;
+        if ((st = dalloc(c, &p, c->slot_beg_bytes, true)) != GW_OK) { free_slots(c); return st; }
+        c->slot_beg[s] = (uint64_t*)p;
+        if ((st = dalloc(c, &p, c->slot_csr_bytes, true)) != GW_OK) { free_slots(c); return st; }
+        c->slot_csr[s] = (uint32_t*)p;
+    }
+    // re-expose zero-copy mappings that exist
+    for (uint32_t i = 0; i < c->nblocks; ++i)
+        if (c->map_beg[i]) { c->hblk[i].beg = c->map_beg[i]; c->hblk[i].csr = c->map_csr[i]; c->hblk[i].resident = 1; }
+    return GW_OK;
+}
+
+static void block_nonresident(gw_ctx* c, uint32_t blk)
+{
+    DevBlock& d = c->hblk[blk];
+    c->blk_slot[blk] = -1;
+    if (c->map_beg[blk]) { d.beg = c->map_beg[blk]; d.csr = c->map_csr[blk]; d.resident = 1; }
+    else { d.beg = nullptr; d.csr = nullptr; d.resident = 0; }
+}
+
+gw_status gw_upload_block(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const uint32_t* csr)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (slot >= c->nslots) return fail(c, GW_ERR_ARG, "slot %u out of range (%u slots)", slot, c->nslots);
+    if (blk >= c->nblocks) return fail(c, GW_ERR_ARG, "block %u out of range (%u blocks)", blk, c->nblocks);
+    const gw_block_desc& d = c->desc[blk];
+    if (!beg || (d.nedges && !csr)) return fail(c, GW_ERR_ARG, "null block buffer");
+    if (c->fault_kind == 1 && ++c->n_upload == c->fault_at)
+        return fail(c, GW_ERR_FAULT, "injected upload failure (GW_FAULT=upload:%ld)", c->fault_at);
+    (void)hipSetDevice(c->device);
+    if (c->slot_blk[slot] >= 0) block_nonresident(c, (uint32_t)c->slot_blk[slot]);
+    if (c->blk_slot[blk] >= 0) { c->slot_blk[c->blk_slot[blk]] = -1; c->blk_slot[blk] = -1; }
+    // the slot may still be read by a kernel of the previous run on the compute stream
+    HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));
+    HIPCHK(c, hipStreamWaitEvent(c->s_copy, c->ev_k1, 0));
+    const uint64_t bb = ((uint64_t)d.nverts + 1) * 8, cb = d.nedges * 4;
+    HIPCHK(c, hipEventRecord(c->ev_c0, c->s_copy));
+    HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, c->s_copy));
+    if (cb) HIPCHK(c, hipMemcpyAsync(c->slot_csr[slot], csr, cb, hipMemcpyHostToDevice, c->s_copy));
+    HIPCHK(c, hipEventRecord(c->ev_c1, c->s_copy));
+    HIPCHK(c, hipEventRecord(c->ev_copy, c->s_copy));
+    c->slot_blk[slot] = (int32_t)blk;
+    c->blk_slot[blk] = (int32_t)slot;
+    DevBlock& h = c->hblk[blk];
+    h.beg = c->slot_beg[slot];
+    h.csr = c->slot_csr[slot];
+    h.resident = 1;
+    c->tot.h2d_bytes += bb + cb;
+    c->tot.h2d_uploads += 1;
+    // copy-engine time, accumulated when the copy completes (gw_wait_uploads / gw_run)
+    HIPCHK(c, hipEventSynchronize(c->ev_c1));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_c0, c->ev_c1) == hipSuccess) c->tot.h2d_ms += ms;
+    return GW_OK;
+}
+
+gw_status gw_evict(gw_ctx* c, uint32_t slot)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (slot >= c->nslots) return fail(c, GW_ERR_ARG, "slot %u out of range", slot);
+    if (c->slot_blk[slot] >= 0) block_nonresident(c, (uint32_t)c->slot_blk[slot]);
+    c->slot_blk[slot] = -1;
+    return GW_OK;
+}
+
+gw_status gw_wait_uploads(gw_ctx* c)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    HIPCHK(c, hipStreamSynchronize(c->s_copy));
+    return GW_OK;
+}
+
+gw_status gw_map_host_block(gw_ctx* c, uint32_t blk, const uint64_t* beg, const uint32_t* csr)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (blk >= c->nblocks) return fail(c, GW_ERR_ARG, "block %u out of range", blk);
+    void* db = nullptr;
+    void* dc = nullptr;
+    HIPCHK(c, hipHostGetDevicePointer(&db, (void*)beg, 0));
+    if (c->desc[blk].nedges) HIPCHK(c, hipHostGetDevicePointer(&dc, (void*)csr, 0));
+    c->map_beg[blk] = (const uint64_t*)db;
+    c->map_csr[blk] = (const uint32_t*)dc;
+    if (c->blk_slot[blk] < 0) block_nonresident(c, blk);
+    return GW_OK;
+}
+
+gw_status gw_unmap_host_blocks(gw_ctx* c)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    for (uint32_t i = 0; i < c->nblocks; ++i) {
+        c->map_beg[i] = nullptr;
+        c->map_csr[i] = nullptr;
+        if (c->blk_slot[i] < 0) block_nonresident(c, i);
+    }
+    return GW_OK;
+}
+
+static uint64_t thr(double w, double m)
+{
+    if (w >= m) return 1ULL << 32;
+    return (uint64_t)std::floor(w / m * 4294967296.0);
+}
+
+gw_status gw_set_walk(gw_ctx* c, const gw_walk_params* wp)
+{
+    if (!c || !wp) return fail(c, GW_ERR_ARG, "null argument");
+    if (wp->ntypes < 1 || wp->ntypes > 2) return fail(c, GW_ERR_ARG, "ntypes must be 1 or 2");
+    for (uint32_t t = 0; t < wp->ntypes; ++t) {
+        const gw_walk_type& ty = wp->type[t];
+        DevType& d = c->dt[t];
+        d = DevType{};
+        if (ty.algo > 1) return fail(c, GW_ERR_ARG, "unknown algorithm %u", ty.algo);
+        if (ty.length > kMaxLength) return fail(c, GW_ERR_ARG, "length %u exceeds %u", ty.length, kMaxLength);
+        if (!(ty.term >= 0.0 && ty.term < 1.0)) return fail(c, GW_ERR_ARG, "term must be in [0, 1)");
+        d.algo = ty.algo;
+        d.length = ty.length;
+        d.t_term = ty.term > 0.0 ? (uint64_t)std::floor(ty.term * 4294967296.0) : 0;
+        if (ty.algo == GW_NODE2VEC) {
+            if (!(ty.p >= 1.0 / 65536 && ty.p <= 65536 && ty.q >= 1.0 / 65536 && ty.q <= 65536))
+                return fail(c, GW_ERR_ARG, "p and q must lie in [2^-16, 2^16]");
+            const double ip = 1.0 / ty.p, iq = 1.0 / ty.q;
+            const double m = std::max(std::max(ip, 1.0), iq);
+            d.t_p = thr(ip, m);
+            d.t_one = thr(1.0, m);
+            d.t_q = thr(iq, m);
+            d.t_min = std::min(std::min(d.t_p, d.t_one), d.t_q);
+        } else {
+            if (!(ty.alpha >= 0.0 && ty.alpha < 1.0)) return fail(c, GW_ERR_ARG, "alpha must be in [0, 1)");
+            const long long a = std::llround(ty.alpha * 65536.0);
+            if (a >= 65536) return fail(c, GW_ERR_ARG, "alpha rounds to 1 in 16-bit fixed point");
+            d.alpha_a = (uint32_t)a;
+            d.alpha_b = 65536u - (uint32_t)a;
+        }
+    }
+    if (wp->ntypes == 1) c->dt[1] = c->dt[0];
+    c->wp = *wp;
+    c->walk_set = true;
+    return GW_OK;
+}
+
+gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* sources, uint32_t outputs)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (!c->nblocks) return fail(c, GW_ERR_STATE, "gw_init_walkers before gw_set_block_table");
+    if (!c->walk_set) return fail(c, GW_ERR_STATE, "gw_init_walkers before gw_set_walk");
+    if (n == 0 || n > 0xFFFFFFFFull) return fail(c, GW_ERR_ARG, "walker count must be in [1, 2^32)");
+    if (!sources) {
+        if (wpv == 0) return fail(c, GW_ERR_ARG, "walkers_per_vertex must be >= 1");
+        if ((n - 1) / wpv >= c->nverts_total)
+            return fail(c, GW_ERR_ARG, "%llu walkers at %u per vertex exceed %llu vertices", (unsigned long long)n, wpv,
+                        (unsigned long long)c->nverts_total);
+    } else {
+        for (uint64_t i = 0; i < n; ++i)
+            if (sources[i] >= c->nverts_total)
+                return fail(c, GW_ERR_ARG, "source %u of walker %llu is not a vertex", sources[i], (unsigned long long)i);
+    }
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    free_walkers(c);
+    const uint32_t maxlen = std::max(c->dt[0].length, c->wp.ntypes == 2 ? c->dt[1].length : 0u);
+    c->path_stride = (outputs & GW_OUT_PATHS) ? maxlen + 1 : 0;
+    void* p;
+    gw_status st;
+    c->nwalkers = n;
+    uint32_t** u32s[] = {&c->d_cur, &c->d_prev, &c->d_meta, &c->d_bpair, &c->d_work};
+    for (uint32_t** q : u32s) {
+        if ((st = dalloc(c, &p, n * 4, true)) != GW_OK) { free_walkers(c); return st; }
+        *q = (uint32_t*)p;
+    }
+    if (outputs & GW_OUT_HASH) {
+        if ((st = dalloc(c, &p, n * 8, true)) != GW_OK) { free_walkers(c); return st; }
+        c->d_hash = (uint64_t*)p;
+    }
+    if (outputs & GW_OUT_PATHS) {
+        if ((st = dalloc(c, &p, n * (uint64_t)c->path_stride * 4, true)) != GW_OK) { free_walkers(c); return st; }
+        c->d_paths = (uint32_t*)p;
+    }
+    if (outputs & GW_OUT_VISITS) {
+        if ((st = dalloc(c, &p, c->nverts_total * 4, true)) != GW_OK) { free_walkers(c); return st; }
+        c->d_visits = (uint32_t*)p;
+        HIPCHK(c, launch_fill_u32(c->d_visits, 0, c->nverts_total, c->s_compute));
+    }
+    c->outputs = outputs;
+    uint32_t* d_src = nullptr;
+    if (sources) {
+        HIPCHK(c, hipMalloc((void**)&d_src, n * 4));
+        HIPCHK(c, hipMemcpy(d_src, sources, n * 4, hipMemcpyHostToDevice));
+    }
+    InitArgs a{};
+    a.vcut = c->d_vcut;
+    a.nblocks = c->nblocks;
+    a.ntypes = c->wp.ntypes;
+    a.len0 = c->dt[0].length;
+    a.len1 = c->dt[1].length;
+    a.nwalkers = n;
+    a.wpv = wpv ? wpv : 1;
+    a.sources = d_src;
+    a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta; a.bpair = c->d_bpair;
+    a.hash = c->d_hash; a.paths = c->d_paths; a.path_stride = c->path_stride;
+    hipError_t e = launch_init(a, c->s_compute);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s_compute);
+    if (d_src) (void)hipFree(d_src);
+    if (e != hipSuccess) return fail(c, GW_ERR_HIP, "walker init kernel failed: %s", hipGetErrorString(e));
+    return GW_OK;
+}
+
+// Worklist + (optional) bucket pass on the compute stream; returns nwork and left.
+static gw_status run_filter(gw_ctx* c, bool buckets, bool worklist, uint32_t* nwork, uint64_t* left, float* ms)
+{
+    HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
+    HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
+    if (buckets) HIPCHK(c, launch_fill_u32(c->d_bucket, 0, (uint64_t)c->nblocks * c->nblocks, c->s_compute));
+    FilterArgs f{};
+    f.blocks = c->d_blocks;
+    f.nblocks = c->nblocks;
+    f.nwalkers = c->nwalkers;
+    f.meta = c->d_meta;
+    f.bpair = c->d_bpair;
+    f.worklist = c->d_work;   // a bucket-only pass still compacts; the list is ignored
+    f.nwork = c->d_misc;
+    f.bucket = buckets ? c->d_bucket : nullptr;
+    f.left = c->d_cnt + 6;
+    HIPCHK(c, hipEventRecord(c->ev_f0, c->s_compute));
+    (void)worklist;
+    HIPCHK(c, launch_filter(f, c->ncu * 8, c->s_compute));
+    HIPCHK(c, hipEventRecord(c->ev_f1, c->s_compute));
+    HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_misc, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->s_compute));
+    HIPCHK(c, hipMemcpyAsync(c->h_cnt + 4, c->d_cnt + 6, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->s_compute));
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    *nwork = ((uint32_t*)c->h_cnt)[0];
+    *left = c->h_cnt[4];
+    HIPCHK(c, hipEventElapsedTime(ms, c->ev_f0, c->ev_f1));
+    return GW_OK;
+}
+
+static gw_status push_block_table(gw_ctx* c)
+{
+    HIPCHK(c, hipStreamWaitEvent(c->s_compute, c->ev_copy, 0));
+    memcpy(c->h_blocks_pinned, c->hblk.data(), sizeof(DevBlock) * c->nblocks);
+    HIPCHK(c, hipMemcpyAsync(c->d_blocks, c->h_blocks_pinned, sizeof(DevBlock) * c->nblocks, hipMemcpyHostToDevice,
+                             c->s_compute));
+    return GW_OK;
+}
+
+gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (!c->nwalkers) return fail(c, GW_ERR_STATE, "gw_run before gw_init_walkers");
+    (void)hipSetDevice(c->device);
+    gw_launch l = L ? *L : gw_launch{};
+    if (l.block == 0) l.block = 256;
+    if (l.block % 64 || l.block > 1024) return fail(c, GW_ERR_ARG, "block must be a multiple of 64 up to 1024");
+    if (l.waves_per_cu == 0) l.waves_per_cu = 16;
+    gw_status st;
+    if ((st = push_block_table(c)) != GW_OK) return st;
+    uint32_t nwork = 0;
+    uint64_t left = 0;
+    float fms = 0.f, kms = 0.f;
+    if ((st = run_filter(c, (l.flags & GW_RUN_BUCKETS) != 0, true, &nwork, &left, &fms)) != GW_OK) return st;
+    HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
+    if (nwork) {
+        WalkArgs a{};
+        a.blocks = c->d_blocks;
+        a.vcut = c->d_vcut;
+        a.nblocks = c->nblocks;
+        a.ntypes = c->wp.ntypes;
+        a.seed_lo = (uint32_t)c->wp.seed;
+        a.seed_hi = (uint32_t)(c->wp.seed >> 32);
+        a.type0 = c->dt[0];
+        a.type1 = c->dt[1];
+        a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta; a.bpair = c->d_bpair;
+        a.hash = c->d_hash; a.paths = c->d_paths; a.path_stride = c->path_stride; a.visits = c->d_visits;
+        a.worklist = c->d_work;
+        a.nwork = nwork;
+        a.queue = c->d_misc + 1;
+        a.counters = c->d_cnt;
+        uint64_t lanes = (uint64_t)c->ncu * l.waves_per_cu * 64;
+        uint64_t grid = lanes / l.block;
+        const uint64_t need = ((uint64_t)nwork + l.block - 1) / l.block;
+        if (grid > need) grid = need;
+        if (grid == 0) grid = 1;
+        const bool one = c->nblocks == 1;
+        HIPCHK(c, hipEventRecord(c->ev_k0, c->s_compute));
+        HIPCHK(c, launch_walk(a, one, l.variant, (uint32_t)grid, l.block, c->s_compute));
+        HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));
+        HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 c->s_compute));
+        HIPCHK(c, hipStreamSynchronize(c->s_compute));
+        HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
+    } else {
+        memset(c->h_cnt, 0, 8 * sizeof(unsigned long long));
+    }
+    gw_run_stats s{};
+    s.steps = c->h_cnt[0];
+    s.trials = c->h_cnt[1];
+    s.probes = c->h_cnt[2];
+    s.mix_steps = c->h_cnt[3];
+    s.walkers_ended = c->h_cnt[4];
+    s.walkers_run = nwork;
+    s.walkers_left = left - s.walkers_ended;
+    s.kernel_ms = kms;
+    s.filter_ms = fms;
+    c->tot.steps += s.steps;
+    c->tot.trials += s.trials;
+    c->tot.probes += s.probes;
+    c->tot.mix_steps += s.mix_steps;
+    c->tot.kernel_ms += kms;
+    if (stats) *stats = s;
+    return GW_OK;
+}
+
+gw_status gw_bucket_counts(gw_ctx* c, uint64_t* W, uint64_t n)
+{
+    if (!c || !W) return fail(c, GW_ERR_ARG, "null argument");
+    if (!c->nwalkers) return fail(c, GW_ERR_STATE, "no walkers");
+    const uint64_t nn = (uint64_t)c->nblocks * c->nblocks;
+    if (n < nn) return fail(c, GW_ERR_ARG, "bucket buffer holds %llu < %llu entries", (unsigned long long)n,
+                            (unsigned long long)nn);
+    (void)hipSetDevice(c->device);
+    gw_status st;
+    if ((st = push_block_table(c)) != GW_OK) return st;
+    uint32_t nwork;
+    uint64_t left;
+    float ms;
+    if ((st = run_filter(c, true, false, &nwork, &left, &ms)) != GW_OK) return st;
+    std::vector<uint32_t> tmp(nn);
+    HIPCHK(c, hipMemcpy(tmp.data(), c->d_bucket, nn * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < nn; ++i) W[i] = tmp[i];
+    return GW_OK;
+}
+
+gw_status gw_walkers_left(gw_ctx* c, uint64_t* left)
+{
+    if (!c || !left) return fail(c, GW_ERR_ARG, "null argument");
+    if (!c->nwalkers) return fail(c, GW_ERR_STATE, "no walkers");
+    gw_status st;
+    if ((st = push_block_table(c)) != GW_OK) return st;
+    uint32_t nwork;
+    float ms;
+    return run_filter(c, false, false, &nwork, left, &ms);
+}
+
+static gw_status readback(gw_ctx* c, void* dst, const void* src, uint64_t bytes, const char* what)
+{
+    if (!src) return fail(c, GW_ERR_STATE, "%s were not requested in gw_init_walkers", what);
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    HIPCHK(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GW_OK;
+}
+
+gw_status gw_read_visits(gw_ctx* c, uint32_t* out, uint64_t n)
+{
+    if (!c || !out) return fail(c, GW_ERR_ARG, "null argument");
+    if (n < c->nverts_total) return fail(c, GW_ERR_ARG, "visit buffer too small");
+    return readback(c, out, c->d_visits, c->nverts_total * 4, "visits");
+}
+
+gw_status gw_read_hashes(gw_ctx* c, uint64_t* out, uint64_t n)
+{
+    if (!c || !out) return fail(c, GW_ERR_ARG, "null argument");
+    if (n < c->nwalkers) return fail(c, GW_ERR_ARG, "hash buffer too small");
+    return readback(c, out, c->d_hash, c->nwalkers * 8, "hashes");
+}
+
+gw_status gw_read_paths(gw_ctx* c, uint32_t* out, uint64_t n)
+{
+    if (!c || !out) return fail(c, GW_ERR_ARG, "null argument");
+    if (n < c->nwalkers * (uint64_t)c->path_stride) return fail(c, GW_ERR_ARG, "path buffer too small");
+    return readback(c, out, c->d_paths, c->nwalkers * (uint64_t)c->path_stride * 4, "paths");
+}
+
+gw_status gw_read_state(gw_ctx* c, uint32_t* out4, uint64_t n)
+{
+    if (!c || !out4) return fail(c, GW_ERR_ARG, "null argument");
+    if (n < c->nwalkers) return fail(c, GW_ERR_ARG, "state buffer too small");
+    std::vector<uint32_t> cur(c->nwalkers), prev(c->nwalkers), meta(c->nwalkers);
+    gw_status st;
+    if ((st = readback(c, cur.data(), c->d_cur, c->nwalkers * 4, "state")) != GW_OK) return st;
+    if ((st = readback(c, prev.data(), c->d_prev, c->nwalkers * 4, "state")) != GW_OK) return st;
+    if ((st = readback(c, meta.data(), c->d_meta, c->nwalkers * 4, "state")) != GW_OK) return st;
+    for (uint64_t i = 0; i < c->nwalkers; ++i) {
+        out4[4 * i + 0] = cur[i];
+        out4[4 * i + 1] = prev[i];
+        out4[4 * i + 2] = meta[i] & kHopMask;
+        out4[4 * i + 3] = (meta[i] >> kStatusShift) & 0xFu;
+    }
+    return GW_OK;
+}
+
+gw_status gw_counters_get(gw_ctx* c, gw_counters* out)
+{
+    if (!c || !out) return fail(c, GW_ERR_ARG, "null argument");
+    *out = c->tot;
+    out->hbm_bytes_held = c->held;
+    return GW_OK;
+}
+
+// split_blocks, preprocess/split.hpp:15-87: walk the .beg offsets once; cut before a
+// vertex whose edges would overflow blocksize/4 edges (at v-1), or when the block
+// reaches blocksize/8 vertices (at v); the last cut point is entries-1.
+gw_status gw_split_blocks(const uint64_t* beg, uint64_t entries, uint64_t block_size, uint32_t* vblocks,
+                          uint64_t* eblocks, uint64_t cap, uint64_t* ncuts)
+{
+    if (!beg || !vblocks || !eblocks || !ncuts || entries < 1 || block_size < 8)
+        return fail(nullptr, GW_ERR_ARG, "gw_split_blocks: bad arguments");
+    if (entries - 1 > 0xFFFFFFFFull) return fail(nullptr, GW_ERR_ARG, "too many vertices for u32 ids");
+    const uint64_t max_e = block_size / 4, max_v = block_size / 8;
+    uint64_t n = 0;
+    uint32_t cur = 0;
+    uint64_t rd = 0;
+    auto push = [&](uint32_t v, uint64_t e) -> bool {
+        if (n >= cap) return false;
+        vblocks[n] = v; eblocks[n] = e; ++n;
+        return true;
+    };
+    if (!push(0, 0)) return fail(nullptr, GW_ERR_ARG, "cut buffer too small");
+    for (uint64_t v = 0; v < entries; ++v) {
+        if (beg[v] - rd > max_e) {
+            cur = (uint32_t)(v - 1);
+            rd = beg[v - 1];
+            if (!push(cur, rd)) return fail(nullptr, GW_ERR_ARG, "cut buffer too small");
+        } else if ((uint32_t)v - cur >= max_v) {
+            cur = (uint32_t)v;
+            rd = beg[v];
+            if (!push(cur, rd)) return fail(nullptr, GW_ERR_ARG, "cut buffer too small");
+        }
+    }
+    if (!push((uint32_t)(entries - 1), beg[entries - 1])) return fail(nullptr, GW_ERR_ARG, "cut buffer too small");
+    *ncuts = n;
+    return GW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// gw_internal.h -- structures shared by the C-ABI layer (gw_api.cpp) and the gfx950
+// kernels (gw_walk.hip, gw_gen.hip).  Not part of the public ABI.
+#pragma once
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+namespace gw {
+
+constexpr uint64_t kFnvOffset = 1469598103934665603ULL;
+constexpr uint64_t kFnvPrime = 1099511628211ULL;
+constexpr uint32_t kMaxTrials = 1u << 24;   // per step; the walk ends with status 4 beyond it
+constexpr uint32_t kTagStep = 0, kTagTerm = 1;
+
+// walker meta word: hop in bits 0..19, status in 20..23, type in 24
+constexpr uint32_t kHopMask = 0xFFFFFu;
+constexpr uint32_t kStatusShift = 20, kTypeShift = 24;
+constexpr uint32_t kMaxLength = kHopMask;
+enum : uint32_t { kActive = 0, kDoneLength = 1, kDoneTerm = 2, kDoneDeadEnd = 3, kDoneTrialCap = 4 };
+
+// One block as the kernels see it: the block's own beg (nverts+1 global offsets) and
+// csr arrays, wherever they live (an HBM slot, a caller's device graph, or pinned host
+// memory for the zero-copy tail).  resident == 0 means the pointers must not be read.
+struct DevBlock {
+    const uint64_t* beg;
+    const uint32_t* csr;
+    uint64_t start_edge;
+    uint64_t nedges;      // possibly truncated (giant single-vertex block)
+    uint32_t start_vert;
+    uint32_t nverts;
+    uint32_t resident;
+    uint32_t pad;
+};
+
+// Per walker type, in the integer form the step logic uses.
+struct DevType {
+    uint64_t t_p, t_one, t_q, t_min;   // node2vec acceptance thresholds, each <= 2^32
+    uint64_t t_term;                   // floor(term * 2^32), 0 = no geometric termination
+    uint32_t algo;                     // 0 node2vec, 1 SOPR/SOSR
+    uint32_t length;
+    uint32_t alpha_a, alpha_b;         // A = round(alpha * 2^16), B = 2^16 - A
+};
+
+struct WalkArgs {
+    const DevBlock* blocks;
+    const uint32_t* vcut;     // nblocks+1 vertex cut points
+    uint32_t nblocks;
+    uint32_t ntypes;
+    uint32_t seed_lo, seed_hi;
+    DevType type0, type1;
+    // walker state (SoA)
+    uint32_t* cur;
+    uint32_t* prev;
+    uint32_t* meta;
+    uint32_t* bpair;          // cur block in bits 0..15, prev block in 16..31
+    uint64_t* hash;           // optional
+    uint32_t* paths;          // optional, path_stride entries per walker
+    uint32_t path_stride;
+    uint32_t* visits;         // optional
+    // work
+    const uint32_t* worklist; // walker ids; nullptr = identity
+    uint32_t nwork;
+    uint32_t* queue;          // work-fetch counter, zero at launch
+    unsigned long long* counters;  // steps, trials, probes, mix, ended, left
+};
+
+struct InitArgs {
+    const uint32_t* vcut;
+    uint32_t nblocks;
+    uint32_t ntypes;
+    uint32_t len0, len1;
+    uint64_t nwalkers;
+    uint32_t wpv;
+    const uint32_t* sources;  // device copy, optional
+    uint32_t* cur;
+    uint32_t* prev;
+    uint32_t* meta;
+    uint32_t* bpair;
+    uint64_t* hash;
+    uint32_t* paths;
+    uint32_t path_stride;
+};
+
+struct FilterArgs {
+    const DevBlock* blocks;
+    uint32_t nblocks;
+    uint64_t nwalkers;
+    const uint32_t* meta;
+    const uint32_t* bpair;
+    uint32_t* worklist;
+    uint32_t* nwork;          // append counter
+    uint32_t* bucket;         // optional nblocks*nblocks counters
+    unsigned long long* left; // unfinished walkers
+};
+
+// launchers (gw_walk.hip)
+hipError_t launch_init(const InitArgs& a, hipStream_t s);
+hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s);
+hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid,
+                       uint32_t block, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+
+}  // namespace gw

@@ -1,0 +1,429 @@
+// gw_walk.hip -- gfx950 walker-step kernels for GOWalker's second-order walks.
+//
+// Replaces gpu/gpuwalk.hpp: gpu_update/gpu_run (:87-205, Node2vec), SOPR_gpu_update /
+// SOPR_gpu_run (:207-319), prefix/insertglobal/walkswap (:321-372, here a worklist
+// filter with wavefront ballot compaction), and the walker generators of
+// engine/walk.hpp:32-95.
+//
+// Step semantics (identical, bit for bit, to the CPU restatement used as the oracle):
+//   * Philox-4x32-10 keyed by the seed; counter (walker, hop, k, tag).  Trial k of a
+//     step draws block k/2 with tag 0; even k uses (.x pos, .y accept), odd k uses
+//     (.z, .w).  The geometric termination draw is block 0 with tag 1 (.x).
+//   * pos = (u32 * deg) >> 32.
+//   * Node2vec (gpuwalk.hpp:118-150): accept iff accept-word < T(class), classes
+//     x == prev (1/p), x in N(prev) (1), other (1/q), T = floor(w/max * 2^32); the
+//     smallest T is tested before the candidate is classified.
+//   * SOPR (gpuwalk.hpp:241-266): P(x) ~ (1-a)/dv + a/du [x in N(prev), x != prev] with
+//     a = A/2^16, sampled by envelope rejection or, when the envelope is loose, by an
+//     exact two-component mixture after counting |N(cur) & N(prev)| (see sopr_step).
+//   * Membership: textbook binary search over prev's sorted adjacency
+//     (engine/engine.hpp:224-243, not the defective device search of gpuwalk.hpp:26-48).
+#include "gw_internal.h"
+
+namespace gw {
+
+__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                        uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+typedef unsigned __int128 u128;
+
+struct Cnt {
+    uint64_t steps, trials, probes, mix, ended, left;
+};
+
+__device__ __forceinline__ uint32_t bitlen64(uint64_t n) { return n ? 64u - (uint32_t)__clzll(n) : 0u; }
+
+// engine/engine.hpp:224-243
+__device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t n, uint32_t x, uint64_t& probes)
+{
+    int64_t lo = 0, hi = (int64_t)n - 1;
+    while (lo <= hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        const uint32_t m = a[mid];
+        ++probes;
+        if (m == x) return true;
+        if (m < x) lo = mid + 1; else hi = mid - 1;
+    }
+    return false;
+}
+
+__device__ __forceinline__ uint64_t lower_bound(const uint32_t* __restrict__ a, uint64_t lo, uint64_t hi,
+                                                uint32_t x, uint64_t& probes)
+{
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        ++probes;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t upper_bound(const uint32_t* __restrict__ a, uint64_t lo, uint64_t hi,
+                                                uint32_t x, uint64_t& probes)
+{
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        ++probes;
+        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Distinct values y of N(u) (y != u) in ascending order, matched against their runs in
+// N(v).  select == ~0: returns the number c of matching positions of N(v); otherwise
+// returns the index in N(v) of the select-th matching position.
+__device__ uint64_t intersect(const uint32_t* __restrict__ nu, uint64_t du,
+                              const uint32_t* __restrict__ nv, uint64_t dv,
+                              uint32_t u, uint64_t select, uint64_t& probes)
+{
+    uint64_t c = 0, base = 0;
+    uint32_t last = 0;
+    bool have_last = false;
+    for (uint64_t i = 0; i < du && base < dv; ++i) {
+        const uint32_t y = nu[i];
+        ++probes;
+        if ((have_last && y == last) || y == u) { last = y; have_last = true; continue; }
+        last = y; have_last = true;
+        const uint64_t lb = lower_bound(nv, base, dv, y, probes);
+        if (lb < dv) {
+            ++probes;
+            if (nv[lb] == y) {
+                const uint64_t ub = upper_bound(nv, lb, dv, y, probes);
+                const uint64_t cnt = ub - lb;
+                if (select != ~0ULL && select < c + cnt) return lb + (select - c);
+                c += cnt;
+                base = ub;
+                continue;
+            }
+        }
+        base = lb;
+    }
+    return c;
+}
+
+__device__ __forceinline__ bool sopr_use_mixture(uint64_t du, uint64_t dv, uint32_t a, uint32_t b)
+{
+    const uint64_t bdu = (uint64_t)b * du;
+    const uint64_t num = bdu + (uint64_t)a * dv;
+    const uint64_t e = (num + bdu - 1) / bdu;
+    return du * (4ull * bitlen64(dv) + 2) < e * (1 + bitlen64(du));
+}
+
+__device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64_t dv,
+                                         const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
+                                         const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
+                                         uint32_t& x, Cnt& c)
+{
+    uint4 d = make_uint4(0, 0, 0, 0);
+    for (uint32_t k = 0; k < kMaxTrials; ++k) {
+        if (!(k & 1)) d = philox(w, hop, k >> 1, kTagStep, k0, k1);
+        const uint32_t rp = (k & 1) ? d.z : d.x;
+        const uint64_t ra = (k & 1) ? d.w : d.y;
+        ++c.trials;
+        x = nv[((uint64_t)rp * dv) >> 32];
+        if (ra < tp.t_min) return true;
+        uint64_t t;
+        if (x == prev) t = tp.t_p;
+        else t = member(nu, du, x, c.probes) ? tp.t_one : tp.t_q;
+        if (ra < t) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint64_t dv,
+                                          const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
+                                          const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
+                                          uint32_t& x, Cnt& c)
+{
+    const uint64_t A = tp.alpha_a, B = tp.alpha_b;
+    const u128 num32 = (u128)(B * du) << 32;
+    if (sopr_use_mixture(du, dv, tp.alpha_a, tp.alpha_b)) {
+        ++c.mix;
+        ++c.trials;
+        const uint4 d = philox(w, hop, 0, kTagStep, k0, k1);
+        const uint64_t cnt = intersect(nu, du, nv, dv, prev, ~0ULL, c.probes);
+        if (cnt == 0 || (u128)d.x * (B * du + A * cnt) < num32) {
+            x = nv[((uint64_t)d.y * dv) >> 32];
+        } else {
+            const uint64_t k = ((uint64_t)d.z * cnt) >> 32;
+            x = nv[intersect(nu, du, nv, dv, prev, k, c.probes)];
+        }
+        return true;
+    }
+    const uint64_t den = B * du + A * dv;
+    uint4 d = make_uint4(0, 0, 0, 0);
+    for (uint32_t k = 0; k < kMaxTrials; ++k) {
+        if (!(k & 1)) d = philox(w, hop, k >> 1, kTagStep, k0, k1);
+        const uint32_t rp = (k & 1) ? d.z : d.x;
+        const uint64_t ra = (k & 1) ? d.w : d.y;
+        ++c.trials;
+        x = nv[((uint64_t)rp * dv) >> 32];
+        if ((u128)ra * den < num32) return true;
+        if (x != prev && member(nu, du, x, c.probes)) return true;
+    }
+    return false;
+}
+
+// vertex -> block: the block whose range [vcut[b], vcut[b+1]) holds v; empty blocks
+// (duplicate cut points) never match, as in getblk (gpuwalk.hpp:69-76).
+__device__ __forceinline__ uint32_t find_block(const uint32_t* __restrict__ vcut, uint32_t nblocks, uint32_t v)
+{
+    uint32_t lo = 0, hi = nblocks;   // largest b in [0, nblocks) with vcut[b] <= v
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (vcut[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+struct Row {
+    const uint32_t* adj;
+    uint64_t deg;
+};
+
+__device__ __forceinline__ Row load_row(const DevBlock& b, uint32_t v)
+{
+    const uint64_t* bp = b.beg + (v - b.start_vert);
+    uint64_t t = bp[1] - b.start_edge;
+    uint64_t h = bp[0] - b.start_edge;
+    if (t > b.nedges) t = b.nedges;
+    if (h > t) h = t;
+    return Row{ b.csr + h, t - h };
+}
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Persistent step kernel.  Each lane owns one walker at a time and takes one whole step
+// per loop iteration; a lane whose walker ends or leaves the resident set takes the
+// next one from the work queue (one atomic per wavefront), so a wavefront never waits
+// for its longest walker -- only for its slowest step.
+template <bool ONE>
+__global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a)
+{
+    Cnt c = {0, 0, 0, 0, 0, 0};
+    const uint32_t lane = __lane_id();
+    DevBlock one;
+    if (ONE) one = a.blocks[0];
+    bool have = false, exhausted = false;
+    uint32_t w = 0, cur = 0, prev = 0, hop = 0, type = 0, bc = 0, bp = 0;
+    uint64_t hash = 0;
+    Row rv{nullptr, 0}, ru{nullptr, 0};
+    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+
+    for (;;) {
+        const bool need = !have && !exhausted;
+        const uint64_t mask = __ballot(need);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(a.queue, (uint32_t)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const uint32_t idx = base + lane_rank(mask);
+                if (idx < a.nwork) {
+                    w = a.worklist ? a.worklist[idx] : idx;
+                    cur = a.cur[w];
+                    prev = a.prev[w];
+                    const uint32_t m = a.meta[w];
+                    hop = m & kHopMask;
+                    type = (m >> kTypeShift) & 1u;
+                    const uint32_t pr = a.bpair[w];
+                    bc = pr & 0xFFFFu;
+                    bp = pr >> 16;
+                    if (a.hash) hash = a.hash[w];
+                    if (ONE) {
+                        rv = load_row(one, cur);
+                        ru = load_row(one, prev);
+                    } else {
+                        rv = load_row(a.blocks[bc], cur);
+                        ru = load_row(a.blocks[bp], prev);
+                    }
+                    have = true;
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__any(have)) break;
+        if (!have) continue;
+
+        const DevType& tp = type ? a.type1 : a.type0;
+        uint32_t status = kActive;
+        bool release = false;
+        if (tp.t_term) {
+            const uint4 d = philox(w, hop, 0, kTagTerm, k0, k1);
+            if ((uint64_t)d.x < tp.t_term) status = kDoneTerm;
+        }
+        if (status == kActive && rv.deg == 0) status = kDoneDeadEnd;
+        if (status == kActive) {
+            uint32_t x = 0;
+            const bool ok = tp.algo == 0 ? n2v_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c)
+                                         : sopr_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c);
+            if (!ok) {
+                status = kDoneTrialCap;
+            } else {
+                prev = cur;
+                cur = x;
+                ++hop;
+                ++c.steps;
+                hash = (hash ^ x) * kFnvPrime;
+                if (a.visits) atomicAdd(a.visits + x, 1u);
+                if (a.paths) a.paths[(uint64_t)w * a.path_stride + hop] = x;
+                ru = rv;
+                bp = bc;
+                if (hop >= tp.length) {
+                    status = kDoneLength;
+                } else if (ONE) {
+                    rv = load_row(one, cur);
+                } else {
+                    const DevBlock& cb = a.blocks[bc];
+                    if (cur - cb.start_vert >= cb.nverts) bc = find_block(a.vcut, a.nblocks, cur);
+                    const DevBlock& nb = a.blocks[bc];
+                    if (nb.resident) rv = load_row(nb, cur);
+                    else release = true;
+                }
+            }
+        }
+        if (status != kActive || release) {
+            a.cur[w] = cur;
+            a.prev[w] = prev;
+            a.meta[w] = hop | (status << kStatusShift) | (type << kTypeShift);
+            a.bpair[w] = bc | (bp << 16);
+            if (a.hash) a.hash[w] = hash;
+            if (status != kActive) ++c.ended; else ++c.left;
+            have = false;
+        }
+    }
+    const uint64_t s0 = wave_sum(c.steps), s1 = wave_sum(c.trials), s2 = wave_sum(c.probes);
+    const uint64_t s3 = wave_sum(c.mix), s4 = wave_sum(c.ended), s5 = wave_sum(c.left);
+    if (lane == 0) {
+        atomicAdd(a.counters + 0, (unsigned long long)s0);
+        atomicAdd(a.counters + 1, (unsigned long long)s1);
+        atomicAdd(a.counters + 2, (unsigned long long)s2);
+        atomicAdd(a.counters + 3, (unsigned long long)s3);
+        atomicAdd(a.counters + 4, (unsigned long long)s4);
+        atomicAdd(a.counters + 5, (unsigned long long)s5);
+    }
+}
+
+// Walker generation (engine/walk.hpp:32-49): walker w at sources[w] or w / wpv, with
+// source = previous = current and hop 0; type w % 2 in a mixed batch.
+__global__ void __launch_bounds__(256) init_kernel(InitArgs a)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.nwalkers; w += stride) {
+        const uint32_t src = a.sources ? a.sources[w] : (uint32_t)(w / a.wpv);
+        const uint32_t type = a.ntypes == 2 ? (uint32_t)(w & 1) : 0u;
+        const uint32_t len = type ? a.len1 : a.len0;
+        const uint32_t b = find_block(a.vcut, a.nblocks, src);
+        a.cur[w] = src;
+        a.prev[w] = src;
+        a.meta[w] = (len == 0 ? (kDoneLength << kStatusShift) : 0u) | (type << kTypeShift);
+        a.bpair[w] = b | (b << 16);
+        if (a.hash) a.hash[w] = (kFnvOffset ^ src) * kFnvPrime;
+        if (a.paths) {
+            uint32_t* p = a.paths + w * a.path_stride;
+            p[0] = src;
+            for (uint32_t i = 1; i < a.path_stride; ++i) p[i] = 0xFFFFFFFFu;
+        }
+    }
+}
+
+// Worklist of unfinished walkers whose (prev, cur) blocks are both resident, compacted
+// with a wavefront ballot + one atomic per wavefront; optionally the bucket matrix of
+// all unfinished walkers (the counts prefix() scans in gpuwalk.hpp:321-338).
+__global__ void __launch_bounds__(256) filter_kernel(FilterArgs a)
+{
+    const uint32_t lane = __lane_id();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t left = 0;
+    const uint64_t n_pad = (a.nwalkers + 63) & ~63ULL;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_pad; w += stride) {
+        bool take = false;
+        if (w < a.nwalkers) {
+            const uint32_t m = a.meta[w];
+            if (((m >> kStatusShift) & 0xFu) == kActive) {
+                ++left;
+                const uint32_t pr = a.bpair[w];
+                const uint32_t bc = pr & 0xFFFFu, bp = pr >> 16;
+                take = a.blocks[bc].resident && a.blocks[bp].resident;
+                if (a.bucket) atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
+            }
+        }
+        const uint64_t mask = __ballot(take);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(a.nwork, (uint32_t)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (take) a.worklist[base + lane_rank(mask)] = (uint32_t)w;
+        }
+    }
+    left = wave_sum(left);
+    if (lane == 0 && left) atomicAdd(a.left, (unsigned long long)left);
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_init(const InitArgs& a, hipStream_t s)
+{
+    uint64_t blocks = (a.nwalkers + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(init_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s)
+{
+    if (a.nwalkers == 0) return hipSuccess;
+    uint64_t need = (a.nwalkers + 255) / 256;
+    if (grid == 0 || grid > need) grid = (uint32_t)(need < 65536 ? need : 65536);
+    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid,
+                       uint32_t block, hipStream_t s)
+{
+    (void)variant;
+    if (a.nwork == 0) return hipSuccess;
+    if (one_block) hipLaunchKernelGGL(walk_kernel<true>, dim3(grid), dim3(block), 0, s, a);
+    else hipLaunchKernelGGL(walk_kernel<false>, dim3(grid), dim3(block), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace gw

@@ -1,0 +1,198 @@
+"""Python view of libgw.so for tests and benchmarks.
+
+`Context` wraps one gw_ctx.  `Blocks` cuts an in-memory CSR graph into the reference's
+blocks (split_blocks of preprocess/split.hpp:15-87 through gw_split_blocks, with the
+giant single-vertex truncation of engine/cache.hpp:170-172).  Everything computes in
+libgw.so; this module only moves numpy arrays across the C ABI.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._abi import (GW_NODE2VEC, GW_OUT_HASH, GW_OUT_PATHS, GW_OUT_VISITS, GW_RUN_BUCKETS, GW_SOPR, GwBlockDesc,
+                   GwConfig, GwCounters, GwDeviceInfo, GwLaunch, GwRunStats, GwWalkParams, check, lib)
+
+NODE2VEC, SOPR = GW_NODE2VEC, GW_SOPR
+OUT_VISITS, OUT_HASH, OUT_PATHS = GW_OUT_VISITS, GW_OUT_HASH, GW_OUT_PATHS
+RUN_BUCKETS = GW_RUN_BUCKETS
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Blocks:
+    """Block table of a CSR graph for a given block size in bytes."""
+
+    def __init__(self, beg, csr, block_bytes):
+        self.beg = np.ascontiguousarray(beg, dtype=np.uint64)
+        self.csr = np.ascontiguousarray(csr, dtype=np.uint32)
+        entries = len(self.beg)
+        cap = entries + 2
+        vb = np.zeros(cap, dtype=np.uint32)
+        eb = np.zeros(cap, dtype=np.uint64)
+        n = C.c_uint64(0)
+        check(lib().gw_split_blocks(_p(self.beg), entries, block_bytes, _p(vb), _p(eb), cap, C.byref(n)))
+        self.vcut = vb[:n.value].copy()
+        self.ecut = eb[:n.value].copy()
+        self.nblocks = n.value - 1
+        maxedges = block_bytes // 4
+        self.desc = (GwBlockDesc * self.nblocks)()
+        for b in range(self.nblocks):
+            d = self.desc[b]
+            d.blk = b
+            d.start_vert = int(self.vcut[b])
+            d.nverts = int(self.vcut[b + 1] - self.vcut[b])
+            d.start_edge = int(self.ecut[b])
+            ne = int(self.ecut[b + 1] - self.ecut[b])
+            if d.nverts == 1 and ne > maxedges:
+                ne = maxedges
+            d.nedges = ne
+
+    def arrays(self, b):
+        d = self.desc[b]
+        beg = self.beg[d.start_vert:d.start_vert + d.nverts + 1]
+        csr = self.csr[d.start_edge:d.start_edge + d.nedges]
+        return np.ascontiguousarray(beg), np.ascontiguousarray(csr)
+
+    def block_of(self, v):
+        return int(np.searchsorted(self.vcut[:-1], v, side="right") - 1)
+
+
+def walk_params(types, seed=42):
+    """types: list of dicts {algo, length, p, q, alpha, term} (1 or 2 entries)."""
+    wp = GwWalkParams()
+    wp.seed = seed
+    wp.ntypes = len(types)
+    for i, t in enumerate(types):
+        wt = wp.type[i]
+        wt.algo = t.get("algo", NODE2VEC)
+        wt.length = t["length"]
+        wt.p = t.get("p", 1.0)
+        wt.q = t.get("q", 1.0)
+        wt.alpha = t.get("alpha", 0.2)
+        wt.term = t.get("term", 0.0)
+    return wp
+
+
+class Context:
+    def __init__(self, device=0, cache_bytes=0):
+        cfg = GwConfig(device, 0, cache_bytes)
+        h = C.c_void_p()
+        check(lib().gw_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.blocks = None
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib().gw_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        return check(st, self.h)
+
+    def device_info(self):
+        i = GwDeviceInfo()
+        self._chk(lib().gw_device_info_get(self.h, C.byref(i)))
+        return dict(name=i.name.decode(), cus=i.cus, wave_size=i.wave_size, hbm_bytes=i.hbm_bytes,
+                    lds_per_cu=i.lds_per_cu)
+
+    def set_blocks(self, blocks):
+        self.blocks = blocks
+        self._chk(lib().gw_set_block_table(self.h, blocks.desc, blocks.nblocks))
+
+    def slot_bytes(self):
+        v = C.c_uint64(0)
+        self._chk(lib().gw_slot_bytes(self.h, C.byref(v)))
+        return v.value
+
+    def reserve_slots(self, n):
+        self._chk(lib().gw_reserve_slots(self.h, n))
+
+    def upload(self, slot, blk):
+        beg, csr = self.blocks.arrays(blk)
+        self._keep.append((beg, csr))
+        self._chk(lib().gw_upload_block(self.h, slot, blk, _p(beg), _p(csr)))
+
+    def evict(self, slot):
+        self._chk(lib().gw_evict(self.h, slot))
+
+    def wait_uploads(self):
+        self._chk(lib().gw_wait_uploads(self.h))
+        self._keep = []
+
+    def load_all(self):
+        """Reserve one slot per block and upload every block (fully resident graph)."""
+        nb = self.blocks.nblocks
+        self.reserve_slots(nb)
+        for b in range(nb):
+            self.upload(b, b)
+        self.wait_uploads()
+
+    def set_walk(self, types, seed=42):
+        wp = walk_params(types, seed)
+        self._chk(lib().gw_set_walk(self.h, C.byref(wp)))
+        self.types = types
+
+    def init_walkers(self, n, wpv=1, sources=None, outputs=OUT_HASH):
+        src = None if sources is None else np.ascontiguousarray(sources, dtype=np.uint32)
+        self._chk(lib().gw_init_walkers(self.h, n, wpv, _p(src), outputs))
+        self.nwalkers = n
+        self.outputs = outputs
+
+    def run(self, block=0, waves_per_cu=0, variant=0, buckets=False):
+        L = GwLaunch(block, waves_per_cu, variant, RUN_BUCKETS if buckets else 0)
+        s = GwRunStats()
+        self._chk(lib().gw_run(self.h, C.byref(L), C.byref(s)))
+        return {f: getattr(s, f) for f, _ in GwRunStats._fields_}
+
+    def bucket_counts(self):
+        nb = self.blocks.nblocks
+        W = np.zeros(nb * nb, dtype=np.uint64)
+        self._chk(lib().gw_bucket_counts(self.h, _p(W), nb * nb))
+        return W.reshape(nb, nb)
+
+    def walkers_left(self):
+        v = C.c_uint64(0)
+        self._chk(lib().gw_walkers_left(self.h, C.byref(v)))
+        return v.value
+
+    def visits(self):
+        n = int(self.blocks.vcut[-1])
+        out = np.zeros(n, dtype=np.uint32)
+        self._chk(lib().gw_read_visits(self.h, _p(out), n))
+        return out
+
+    def hashes(self):
+        out = np.zeros(self.nwalkers, dtype=np.uint64)
+        self._chk(lib().gw_read_hashes(self.h, _p(out), self.nwalkers))
+        return out
+
+    def state(self):
+        out = np.zeros(self.nwalkers * 4, dtype=np.uint32)
+        self._chk(lib().gw_read_state(self.h, _p(out), self.nwalkers))
+        return out.reshape(self.nwalkers, 4)
+
+    def paths(self):
+        L = max(t["length"] for t in self.types)
+        out = np.zeros(self.nwalkers * (L + 1), dtype=np.uint32)
+        self._chk(lib().gw_read_paths(self.h, _p(out), out.size))
+        return out.reshape(self.nwalkers, L + 1)
+
+    def counters(self):
+        c = GwCounters()
+        self._chk(lib().gw_counters_get(self.h, C.byref(c)))
+        return {f: getattr(c, f) for f, _ in GwCounters._fields_}

@@ -1,0 +1,197 @@
+/*
+ * gw.h -- C ABI of libgw.so, the MI355X (gfx950) walk engine behind GOWalker's host side.
+ *
+ * The reference has no library boundary: its host C++ (apps/, engine/, preprocess/)
+ * calls the CUDA helpers of gpu/memory.hpp and the kernels of gpu/gpuwalk.hpp directly.
+ * Each entry point below replaces one of those calls; the replaced reference interface
+ * is cited on every declaration.
+ *
+ * Conventions
+ *   - Every call returns gw_status (GW_OK == 0).  Nothing in the library exits or
+ *     throws; gw_last_error() describes the last failure of a context.
+ *   - The library owns all device memory and its three streams (compute, copy,
+ *     readback -- the reference's update / graph / back streams, api/types.hpp:174-187).
+ *   - A context is used by one host thread at a time.
+ *   - Host buffers handed to gw_upload_block must stay valid until gw_wait_uploads()
+ *     returns; pinned buffers (gw_alloc_pinned) make the copy asynchronous.
+ *   - Integer widths follow api/types.hpp:12-20: vertex ids u32, edge offsets u64,
+ *     block ids u32.
+ */
+#ifndef GW_H
+#define GW_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t gw_status;
+enum {
+    GW_OK = 0,
+    GW_ERR_ARG = -1,      /* invalid argument */
+    GW_ERR_HIP = -2,      /* HIP runtime failure */
+    GW_ERR_NOMEM = -3,    /* device or pinned allocation failed */
+    GW_ERR_STATE = -4,    /* call out of order (e.g. run before init) */
+    GW_ERR_BUDGET = -5,   /* HBM cache budget exceeded */
+    GW_ERR_IO = -6,       /* file read/write failure */
+    GW_ERR_FAULT = -7     /* failure injected through GW_FAULT (testing) */
+};
+
+enum { GW_NODE2VEC = 0, GW_SOPR = 1 };   /* AlgorithmType, api/types.hpp:27 (SOSR = SOPR step) */
+
+typedef struct gw_ctx gw_ctx;
+
+typedef struct {
+    int32_t device;          /* HIP device ordinal (apps/Node2vec.cu:24-30 picks 0) */
+    uint32_t reserved;
+    uint64_t cache_bytes;    /* HBM budget for graph slots + walker state; 0 = unlimited
+                                (the `memory` option, apps/Node2vec.cu:39) */
+} gw_config;
+
+/* Block descriptor, mirrors gpu_graph_block (api/types.hpp:159-166) without the
+ * cache-index/status fields, which the library tracks itself. */
+typedef struct {
+    uint32_t blk;
+    uint32_t start_vert;
+    uint32_t nverts;
+    uint32_t pad;
+    uint64_t start_edge;
+    uint64_t nedges;
+} gw_block_desc;
+
+/* One walker type.  Node2vec uses p, q; SOPR/SOSR uses alpha.  term is the geometric
+ * termination probability per step (0 reproduces the reference's fixed length). */
+typedef struct {
+    uint32_t algo;           /* GW_NODE2VEC or GW_SOPR */
+    uint32_t length;         /* hop cap, the `length` option */
+    double p, q, alpha, term;
+} gw_walk_type;
+
+typedef struct {
+    uint64_t seed;           /* Philox key */
+    uint32_t ntypes;         /* 1, or 2 for a mixed batch (walker w has type w % 2) */
+    uint32_t reserved;
+    gw_walk_type type[2];
+} gw_walk_params;
+
+/* Output selection for gw_init_walkers */
+enum {
+    GW_OUT_VISITS = 1u << 0, /* per-vertex arrival counts (u32) */
+    GW_OUT_HASH = 1u << 1,   /* per-walker 64-bit path hash */
+    GW_OUT_PATHS = 1u << 2   /* full paths (length+1 u32 per walker; small runs only) */
+};
+
+/* Launch shape of the step kernel (all shapes give bit-identical walks). */
+typedef struct {
+    uint32_t block;          /* threads per workgroup (64..1024, multiple of 64); 0 = tuned */
+    uint32_t waves_per_cu;   /* persistent grid = CUs * waves_per_cu * 64 / block; 0 = tuned */
+    uint32_t variant;        /* 0 = per-step lane loop, 1 = flattened per-probe loop */
+    uint32_t flags;          /* GW_RUN_* */
+} gw_launch;
+
+enum {
+    GW_RUN_BUCKETS = 1u << 0 /* also recount the (prev_blk, cur_blk) bucket matrix */
+};
+
+typedef struct {
+    uint64_t steps;          /* successful transitions */
+    uint64_t trials;         /* candidate draws */
+    uint64_t probes;         /* adjacency reads for membership / intersection */
+    uint64_t mix_steps;      /* SOPR steps taken by the mixture sampler */
+    uint64_t walkers_run;    /* walkers whose (prev, cur) pair was resident */
+    uint64_t walkers_ended;  /* walkers that finished during this run */
+    uint64_t walkers_left;   /* unfinished walkers after this run */
+    double kernel_ms;        /* step kernel time (events on the compute stream) */
+    double filter_ms;        /* worklist/bucket kernel time */
+} gw_run_stats;
+
+typedef struct {
+    uint64_t steps, trials, probes, mix_steps;
+    uint64_t h2d_bytes;      /* block upload bytes: (nverts+1)*8 + nedges*4 per upload */
+    uint64_t h2d_uploads;
+    double h2d_ms;           /* copy-stream busy time of block uploads */
+    double kernel_ms;
+    uint64_t hbm_bytes_held; /* bytes counted against cache_bytes */
+} gw_counters;
+
+typedef struct {
+    char name[64];
+    uint32_t cus;
+    uint32_t wave_size;
+    uint64_t hbm_bytes;
+    uint32_t lds_per_cu;
+    uint32_t pad;
+} gw_device_info;
+
+/* ---- context: replaces graph_engine's GPU init (engine/engine.hpp:79-92) and the
+ *      gpu_stream constructor (api/types.hpp:180-186) ---- */
+gw_status gw_create(const gw_config* cfg, gw_ctx** out);
+gw_status gw_destroy(gw_ctx* ctx);
+const char* gw_last_error(const gw_ctx* ctx);
+gw_status gw_device_info_get(gw_ctx* ctx, gw_device_info* out);
+
+/* ---- pinned host memory: replaces cudaHostAlloc in cache_block::cudamalloc
+ *      (engine/cache.hpp:106-120) and graph_buffer::alloc (api/graph_buffer.hpp:26-40) ---- */
+gw_status gw_alloc_pinned(size_t bytes, void** out);
+gw_status gw_free_pinned(void* p);
+
+/* ---- block table: replaces initgpugraph (gpu/memory.hpp:28-49) ---- */
+gw_status gw_set_block_table(gw_ctx* ctx, const gw_block_desc* blocks, uint32_t nblocks);
+
+/* ---- HBM block cache: replaces initgpucache / initcacheblock (gpu/memory.hpp:51-107).
+ *      Slots are sized for the largest block; fails with GW_ERR_BUDGET when
+ *      nslots slots plus the walker state exceed cache_bytes. ---- */
+gw_status gw_reserve_slots(gw_ctx* ctx, uint32_t nslots);
+gw_status gw_slot_bytes(gw_ctx* ctx, uint64_t* bytes_per_slot);
+
+/* Asynchronous upload of block `blk` into `slot` on the copy stream: replaces
+ * copycachetogpu + pipeline_gpugraph (gpu/memory.hpp:124-139, :209-224).  beg holds
+ * nverts+1 global offsets, csr holds nedges targets (the block's own arrays). */
+gw_status gw_upload_block(gw_ctx* ctx, uint32_t slot, uint32_t blk,
+                          const uint64_t* beg, const uint32_t* csr);
+/* Marks a slot free and its block non-resident: replaces the INACTIVE update of
+ * GOwalker_scheduler_t::copyblocks (engine/schedule.hpp:181-183). */
+gw_status gw_evict(gw_ctx* ctx, uint32_t slot);
+gw_status gw_wait_uploads(gw_ctx* ctx);
+/* Makes block `blk` readable from pinned host memory (zero-copy), replacing the
+ * mapped d_map table (engine/engine.hpp:85-92) used by zerocopy_update. */
+gw_status gw_map_host_block(gw_ctx* ctx, uint32_t blk, const uint64_t* beg, const uint32_t* csr);
+gw_status gw_unmap_host_blocks(gw_ctx* ctx);
+
+/* ---- walk configuration and walkers: replaces gpu_createwalk and the generate
+ *      kernels (engine/walk.hpp:32-95, :272-389) and cudarand (gpu/memory.hpp:109-122;
+ *      Philox needs no state) ---- */
+gw_status gw_set_walk(gw_ctx* ctx, const gw_walk_params* params);
+/* Walker w starts at sources[w], or at w / walkers_per_vertex when sources is NULL. */
+gw_status gw_init_walkers(gw_ctx* ctx, uint64_t nwalkers, uint32_t walkers_per_vertex,
+                          const uint32_t* sources, uint32_t outputs);
+
+/* Runs every unfinished walker whose (prev, cur) blocks are both resident until it
+ * finishes or steps into a non-resident block: replaces the engine.hpp:418-448 loop
+ * with gpu_run/SOPR_gpu_run, prefix, insertglobal and walkswap (gpu/gpuwalk.hpp:193-372). */
+gw_status gw_run(gw_ctx* ctx, const gw_launch* launch, gw_run_stats* stats);
+
+/* Bucket matrix W[prev_blk * nblocks + cur_blk] of unfinished walkers, as of the last
+ * gw_run with GW_RUN_BUCKETS (or gw_init_walkers): replaces copy_back's read of
+ * block_offset (engine/walk.hpp:488-498) and walk_block::block_numwalks. */
+gw_status gw_bucket_counts(gw_ctx* ctx, uint64_t* W, uint64_t n);
+gw_status gw_walkers_left(gw_ctx* ctx, uint64_t* left);
+
+/* ---- outputs ---- */
+gw_status gw_read_visits(gw_ctx* ctx, uint32_t* out, uint64_t n);
+gw_status gw_read_hashes(gw_ctx* ctx, uint64_t* out, uint64_t n);
+/* cur, prev, hop, status (1 length cap, 2 terminated, 3 dead end, 4 trial cap) */
+gw_status gw_read_state(gw_ctx* ctx, uint32_t* out4, uint64_t n);
+gw_status gw_read_paths(gw_ctx* ctx, uint32_t* out, uint64_t n);
+gw_status gw_counters_get(gw_ctx* ctx, gw_counters* out);
+
+/* ---- host-side block format helpers (no GPU needed) ---- */
+/* split_blocks (preprocess/split.hpp:15-87) over an in-memory .beg array. */
+gw_status gw_split_blocks(const uint64_t* beg, uint64_t entries, uint64_t block_size,
+                          uint32_t* vblocks, uint64_t* eblocks, uint64_t cap, uint64_t* ncuts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,131 @@
+"""Bit-exact parity of the gfx950 walk kernels (through libgw.so's C ABI) with the CPU
+oracle: same graph, same seed => identical paths, path hashes, final states, per-vertex
+visit counts and step/trial/probe counters."""
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import graphs as G
+from gowalker_amd.engine import NODE2VEC, OUT_HASH, OUT_PATHS, OUT_VISITS, SOPR, Blocks, Context
+
+
+def oracle_types(types):
+    ts = []
+    for t in types:
+        if t.get("algo", NODE2VEC) == NODE2VEC:
+            ts.append(O.type_cfg(O.N2V, t["length"], p=t["p"], q=t["q"], term=t.get("term", 0.0)))
+        else:
+            ts.append(O.type_cfg(O.SOPR, t["length"], alpha=t.get("alpha", 0.2), term=t.get("term", 0.0)))
+    return ts
+
+
+def gpu_run(beg, csr, types, seed, n, wpv=1, block_bytes=1 << 30, nslots=None, sources=None, launch=None):
+    bl = Blocks(beg, csr, block_bytes)
+    with Context() as ctx:
+        ctx.set_blocks(bl)
+        if nslots is None:
+            ctx.load_all()
+        ctx.set_walk(types, seed)
+        ctx.init_walkers(n, wpv, sources=sources, outputs=OUT_HASH | OUT_PATHS | OUT_VISITS)
+        tot = dict(steps=0, trials=0, probes=0, mix_steps=0)
+        runs = 0
+        if nslots is None:
+            st = ctx.run(**(launch or {}))
+            runs = 1
+            for k in tot:
+                tot[k] += st[k]
+            assert st["walkers_left"] == 0
+        else:
+            # out-of-core rehearsal: resident window of nslots blocks slides over the
+            # bucket matrix until every walker is done (walks are schedule-independent)
+            ctx.reserve_slots(nslots)
+            while True:
+                W = ctx.bucket_counts()
+                if W.sum() == 0:
+                    break
+                a, b = np.unravel_index(np.argmax(W), W.shape)
+                want = [int(a)] if a == b else [int(a), int(b)]
+                score = W.sum(axis=0) + W.sum(axis=1)
+                for x in np.argsort(-score, kind="stable"):
+                    if len(want) >= nslots:
+                        break
+                    if int(x) not in want:
+                        want.append(int(x))
+                for s, blk in enumerate(want):
+                    ctx.upload(s, blk)
+                ctx.wait_uploads()
+                st = ctx.run(**(launch or {}))
+                runs += 1
+                for k in tot:
+                    tot[k] += st[k]
+                assert st["walkers_run"] > 0
+        out = dict(paths=ctx.paths(), hashes=ctx.hashes(), state=ctx.state(), visits=ctx.visits(), totals=tot,
+                   runs=runs, counters=ctx.counters())
+    return out
+
+
+def check_case(beg, csr, types, seed, n, wpv=1, **kw):
+    g = gpu_run(beg, csr, types, seed, n, wpv, **kw)
+    ts = oracle_types(types)
+    cfg = O.walk_cfg(seed, *ts)
+    o = O.run(beg, csr, cfg, n, wpv=wpv, sources=kw.get("sources"), ntypes=len(types), paths=True, visits=True)
+    print("case", types, "n", n, "gpu totals", g["totals"], "oracle totals", o["totals"], "runs", g["runs"])
+    sys.stdout.flush()
+    np.testing.assert_array_equal(g["state"], o["state"])
+    np.testing.assert_array_equal(g["paths"], o["paths"])
+    np.testing.assert_array_equal(g["hashes"], o["hashes"])
+    np.testing.assert_array_equal(g["visits"], o["visits"])
+    assert g["totals"] == o["totals"]
+    return g, o
+
+
+pytestmark = pytest.mark.gpu
+
+
+def test_n2v_single_block(gpu):
+    beg, csr = G.random_undirected(2000, 12000, seed=3)
+    check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=2.0, q=0.5)], 42, 2000)
+
+
+def test_n2v_multi_block_resident(gpu):
+    beg, csr = G.rmat(10, 8000, seed=5, permute=True)
+    g, o = check_case(beg, csr, [dict(algo=NODE2VEC, length=25, p=0.5, q=2.0)], 7, 1024, block_bytes=4096)
+    assert o["totals"]["probes"] > 0
+
+
+def test_sopr_mixture_and_rejection(gpu):
+    beg, csr = G.power_law_directed(3000, 40000, seed=2)
+    g, o = check_case(beg, csr, [dict(algo=SOPR, length=30, alpha=0.2, term=0.15)], 42, 6000, wpv=2)
+    assert 0 < o["totals"]["mix_steps"] < o["totals"]["steps"]
+
+
+def test_mixed_batch(gpu):
+    beg, csr = G.rmat(11, 20000, seed=9, undirected=False)
+    types = [dict(algo=NODE2VEC, length=20, p=1.0, q=0.25), dict(algo=SOPR, length=20, alpha=0.2, term=0.2)]
+    check_case(beg, csr, types, 11, 4096, wpv=2, block_bytes=8192)
+
+
+@pytest.mark.parametrize("name", ["triangle", "star", "path", "bridge", "dag_dead_end"])
+def test_tiny_graphs(gpu, name):
+    beg, csr = getattr(G, name)()
+    n = len(beg) - 1
+    check_case(beg, csr, [dict(algo=NODE2VEC, length=12, p=0.5, q=2.0)], 1, n * 8, wpv=8)
+    check_case(beg, csr, [dict(algo=SOPR, length=12, alpha=0.3, term=0.1)], 2, n * 8, wpv=8)
+
+
+def test_out_of_core_window(gpu):
+    beg, csr = G.rmat(10, 8000, seed=4, permute=True)
+    g, o = check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 5, 1024, block_bytes=4096,
+                      nslots=3)
+    assert g["runs"] > 1
+    assert g["counters"]["h2d_bytes"] > 0
+
+
+def test_sources_and_launch_shapes(gpu):
+    beg, csr = G.random_undirected(500, 3000, seed=8)
+    src = np.random.default_rng(0).integers(0, 500, 3000).astype(np.uint32)
+    types = [dict(algo=NODE2VEC, length=15, p=2.0, q=0.5)]
+    for launch in (dict(block=64, waves_per_cu=1), dict(block=1024, waves_per_cu=32)):
+        check_case(beg, csr, types, 3, 3000, sources=src, launch=launch)
