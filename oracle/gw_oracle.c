@@ -83,14 +83,18 @@ static inline void draw(uint64_t seed, uint32_t w, uint32_t hop, uint32_t k, uin
     orc_philox4x32_10(ctr, key, out);
 }
 
-/* RandNum, util/util.hpp:229-241 (splitmix64 seeding), :262-271 (xoroshiro128+). */
+/* RandNum, util/util.hpp:229-241 (splitmix64 seeding), :262-271 (xoroshiro128+).
+ * The reference holds z in a signed long long (`LL z`), so its right shifts are
+ * arithmetic; that differs from textbook splitmix64 and is reproduced here. */
+static inline int64_t asr(int64_t z, int k) { return z >> k; }   /* gcc: arithmetic */
+
 void orc_randnum_init(orc_randnum* r, uint64_t seed)
 {
     for (int i = 0; i < 2; i++) {
-        uint64_t z = (seed += 0x9E3779B97F4A7C15ULL);
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-        if (i == 0) r->s0 = z ^ (z >> 31); else r->s1 = z ^ (z >> 31);
+        int64_t z = (int64_t)(seed += 0x9E3779B97F4A7C15ULL);
+        z = (int64_t)((uint64_t)(z ^ asr(z, 30)) * 0xBF58476D1CE4E5B9ULL);
+        z = (int64_t)((uint64_t)(z ^ asr(z, 27)) * 0x94D049BB133111EBULL);
+        if (i == 0) r->s0 = (uint64_t)(z ^ asr(z, 31)); else r->s1 = (uint64_t)(z ^ asr(z, 31));
     }
 }
 
@@ -438,6 +442,19 @@ static uint64_t ref_walk(const orc_graph* g, const orc_refcfg* c, uint32_t sourc
     return moves;
 }
 
+/* Per-walker RandNum seed: the reference seeds per thread (time(NULL) + tid,
+ * engine/engine.hpp:98-101); here each walker gets its own stream, decorrelated across
+ * run seeds by a 64-bit finaliser (MurmurHash3 fmix64). */
+static uint64_t fmix64(uint64_t k)
+{
+    k ^= k >> 33; k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL;
+    k ^= k >> 33;
+    return k;
+}
+
+static uint64_t walker_seed(uint64_t seed, uint64_t w) { return fmix64(seed) ^ fmix64(w + 0x632BE59BD9B4E019ULL); }
+
 static double now_s(void)
 {
     struct timespec ts;
@@ -462,7 +479,7 @@ uint64_t orc_ref_run(const orc_graph* g, const orc_refcfg* cfg, uint64_t w0, uin
         uint64_t w = w0 + (uint64_t)i;
         uint32_t src = sources ? sources[w] : (uint32_t)(w / wpv);
         orc_randnum rn;
-        orc_randnum_init(&rn, cfg->seed + w);
+        orc_randnum_init(&rn, walker_seed(cfg->seed, w));
         moves += ref_walk(g, cfg, src, &rn, visits, &tr);
         fin++;
         if (max_seconds > 0.0 && (i & 255) == 0 && now_s() - t0 > max_seconds)

@@ -107,7 +107,7 @@ typedef struct {
     uint32_t length;
     double term;          /* build addition; 0 reproduces the reference */
     int literal;          /* 1: type-punned inlineRandPair; 0: fixed 52-bit uniform */
-    uint64_t seed;        /* walker w uses RandNum(seed + w) */
+    uint64_t seed;        /* walker w uses RandNum(fmix64(seed) ^ fmix64(w + c)) */
 } orc_refcfg;
 
 /* Runs walkers [w0, w0+nw) to completion; returns real moves (the reference's bogus
