@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Walk-throughput benchmark of the MI355X walk engine (one GPU).
+
+Default workload (SOPR second-order PageRank walks, fully HBM-resident):
+  directed Chung-Lu power-law graph, 41.6M vertices, 1.47B arcs, out-degree exponent 2.2,
+  max out-degree 3M (generated on the GPU, seed 42); 4 walkers per vertex (166.4M walkers),
+  alpha 0.2, termination probability 0.15 per step, length cap 30, per-vertex visit counts.
+One timed step = initialise every walker and run all of them to completion.
+
+Prints one JSON line: walk steps/s plus the roofline of the step kernel (algorithmic bytes
+per launch over its HIP-event time vs the 8 TB/s HBM peak) and a CPU baseline (the
+reference's CPU step logic transcribed to C, on a bounded sample, all host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "sopr_chung_lu": dict(
+        workload="SOPR, directed Chung-Lu 41.6M V / 1.47B arcs (gamma 2.2, max out-degree 3M), "
+                 "4 walkers/vertex, alpha 0.2, term 0.15, length cap 30, fully HBM-resident",
+        graph=dict(kind="chung_lu", nverts=41_600_000, nedges=1_470_000_000, gamma=2.2, max_degree=3_000_000),
+        types=[dict(algo=1, length=30, alpha=0.2, term=0.15)], wpv=4),
+    "node2vec_rmat20": dict(
+        workload="Node2vec, RMAT scale 20 (2^20 V, 16M edges symmetrized, dedup, no self-loops), "
+                 "p 2, q 0.5, 1 walker/vertex, length 80, single resident block",
+        graph=dict(kind="rmat", scale=20, nedges=16 * 1024 * 1024, undirected=True, permute=False),
+        types=[dict(algo=0, length=80, p=2.0, q=0.5)], wpv=1),
+}
+
+
+def algorithmic_bytes(st, nwalkers, visits):
+    """Bytes the step kernel must move by the algorithm's definition:
+    per walker: worklist id 4 + state read 16 + state write 16 + the two adjacency rows
+    (beg[v], beg[v+1], beg[u], beg[u+1]) 32; per step: next row 16 (+ visit RMW 8);
+    per trial: candidate 4; per probe: 4."""
+    b = nwalkers * (4 + 16 + 16 + 32) + st["steps"] * (16 + (8 if visits else 0))
+    b += st["trials"] * 4 + st["probes"] * 4
+    return b
+
+
+def build_graph(ctx, g, seed):
+    from gowalker_amd import engine as E
+    if g["kind"] == "chung_lu":
+        return E.gen_chung_lu(ctx, g["nverts"], g["nedges"], g["gamma"], g["max_degree"], seed, pinned=True)
+    return E.gen_rmat(ctx, g["scale"], g["nedges"], seed=seed, undirected=g["undirected"], permute=g["permute"],
+                      pinned=True)
+
+
+def cpu_baseline(beg, csr, cfg, seconds, threads):
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import numpy as np
+
+    import _oracle as O
+    t = cfg["types"][0]
+    n = len(beg) - 1
+    rng = np.random.default_rng(7)
+    sample = 4_000_000
+    src = rng.integers(0, n, sample).astype(np.uint32)
+    algo = O.N2V if t["algo"] == 0 else O.SOPR
+    t0 = time.perf_counter()
+    r = O.ref_run(beg, csr, algo, t["length"], p=t.get("p", 1.0), q=t.get("q", 1.0), alpha=t.get("alpha", 0.2),
+                  term=t.get("term", 0.0), literal=False, nwalkers=sample, sources=src, max_seconds=seconds,
+                  nthreads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=r["moves"] / dt, unit="steps/s", cores=threads, kind="port",
+                sample="%d of %d random-source walkers run to completion in %.1f s (%d moves); C transcription of "
+                       "the reference CPU step (engine/engine.hpp:158-404, fixed-mode RandNum), OpenMP"
+                       % (r["done"], sample, dt, r["moves"]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="sopr_chung_lu", choices=sorted(CONFIGS))
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has a single MI355X and this engine runs one "
+              "process per GPU; multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        sys.exit(2)
+
+    from gowalker_amd import engine as E
+    cfg = CONFIGS[args.config]
+    ctx = E.Context(0)
+    t0 = time.perf_counter()
+    beg, csr = build_graph(ctx, cfg["graph"], args.seed)
+    gen_s = time.perf_counter() - t0
+    nverts = beg.n - 1
+    narcs = csr.n
+    print("[bench] graph %s: %d vertices, %d arcs, generated in %.2f s" % (args.config, nverts, narcs, gen_s),
+          file=sys.stderr, flush=True)
+    block_bytes = 1 << 34   # one block holds the whole graph (fully resident)
+    blocks = E.Blocks(beg.a, csr.a, block_bytes)
+    ctx.set_blocks(blocks)
+    t0 = time.perf_counter()
+    ctx.load_all()
+    load_s = time.perf_counter() - t0
+    h2d_graph = ctx.counters()["h2d_bytes"]
+    ctx.set_walk(cfg["types"], seed=args.seed)
+    nwalkers = nverts * cfg["wpv"]
+    visits = cfg["types"][0]["algo"] == 1
+    outputs = E.OUT_VISITS if visits else E.OUT_HASH
+    launch = dict(block=args.block, waves_per_cu=args.waves_per_cu)
+
+    def one_step():
+        ctx.init_walkers(nwalkers, cfg["wpv"], outputs=outputs)
+        st = ctx.run(**launch)
+        assert st["walkers_left"] == 0, st
+        return st
+
+    for i in range(args.warmup):
+        st = one_step()
+        print("[bench] warmup %d: %d steps, kernel %.1f ms" % (i, st["steps"], st["kernel_ms"]), file=sys.stderr,
+              flush=True)
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        stats.append(one_step())
+    dt = time.perf_counter() - t0
+    steps = sum(s["steps"] for s in stats)
+    kms = [s["kernel_ms"] for s in stats]
+    st0 = stats[0]
+    for s in stats[1:]:
+        assert s["steps"] == st0["steps"] and s["trials"] == st0["trials"], "walks differ between steps"
+    avg_kms = sum(kms) / len(kms)
+    abytes = algorithmic_bytes(st0, nwalkers, visits)
+    achieved = abytes / (avg_kms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        try:
+            tj = json.load(open(args.traffic_json))
+            traffic = tj.get(args.config)
+        except Exception:
+            traffic = None
+    line = {
+        "metric": "walk steps/s on 1 MI355X",
+        "value": steps / dt,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (generated on the GPU, seed %d)" % args.seed,
+        "config": {"workload": cfg["workload"], "nverts": nverts, "narcs": narcs, "walkers": nwalkers,
+                   "walk_steps_per_pass": st0["steps"], "trials_per_pass": st0["trials"],
+                   "probes_per_pass": st0["probes"], "mixture_steps_per_pass": st0["mix_steps"],
+                   "h2d_gb": 0.0, "graph_h2d_gb_untimed": h2d_graph / 1e9, "graph_load_s": load_s,
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "walk_kernel<true>", "kernel_ms": avg_kms, "algorithmic_bytes": abytes},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0))
+        line["cpu_baseline"] = cpu_baseline(beg.a, csr.a, cfg, args.cpu_seconds, threads)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    beg.close()
+    csr.close()
+
+
+if __name__ == "__main__":
+    main()

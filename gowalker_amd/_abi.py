@@ -13,7 +13,8 @@ GW_OK = 0
 GW_ERR = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "STATE", -5: "BUDGET", -6: "IO", -7: "FAULT"}
 GW_NODE2VEC, GW_SOPR = 0, 1
 GW_OUT_VISITS, GW_OUT_HASH, GW_OUT_PATHS = 1, 2, 4
-GW_RUN_BUCKETS = 1
+GW_RUN_BUCKETS, GW_RUN_TRACE = 1, 2
+GW_GEN_UNDIRECTED, GW_GEN_PERMUTE, GW_GEN_DEDUP = 1, 2, 4
 
 
 class GwConfig(C.Structure):
@@ -86,6 +87,22 @@ SIGNATURES = {
     "gw_read_paths": [P, P, C.c_uint64],
     "gw_counters_get": [P, C.POINTER(GwCounters)],
     "gw_split_blocks": [P, C.c_uint64, C.c_uint64, P, P, C.c_uint64, u64p],
+    "gw_gen_rmat": [P, C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
+                    C.POINTER(P)],
+    "gw_gen_chung_lu": [P, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64, C.POINTER(P)],
+    "gw_graph_size": [P, u64p, u64p],
+    "gw_graph_read": [P, P, P, P],
+    "gw_graph_free": [P, P],
+    "gw_trace_read": [P, P, C.c_uint64],
+    "gw_block_features": [P, P, P, C.c_uint64],
+    "gw_mlp_create": [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(P)],
+    "gw_mlp_destroy": [P, P],
+    "gw_mlp_num_params": [P, u32p],
+    "gw_mlp_set_params": [P, P, P],
+    "gw_mlp_get_params": [P, P, P],
+    "gw_mlp_forward": [P, P, P, C.c_uint32, P],
+    "gw_mlp_backward": [P, P, P, P, C.c_uint32, P, P],
+    "gw_mlp_step": [P, P, P, P, C.c_uint32, C.c_float, P],
 }
 _RESTYPE = {"gw_last_error": C.c_char_p}
 
@@ -119,3 +136,46 @@ def check(status, ctx=None):
         msg = lib().gw_last_error(ctx)
         raise GwError(status, msg.decode() if msg else "")
     return status
+
+
+# ---- libgwhost.so (include/gwh.h): the C++ out-of-core engine ----
+HOST_LIB_PATH = os.path.join(_HERE, "_build", "libgwhost.so")
+
+
+class GwhEngineOpts(C.Structure):
+    _fields_ = [("memory_bytes", C.c_uint64), ("block_bytes", C.c_uint64), ("zero_threshold", C.c_uint64),
+                ("scheduler", C.c_uint32), ("verbose", C.c_uint32), ("launch", GwLaunch), ("device", C.c_int32),
+                ("reserved", C.c_uint32)]
+
+
+class GwhReport(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("trials", C.c_uint64), ("probes", C.c_uint64), ("mix_steps", C.c_uint64),
+                ("h2d_bytes", C.c_uint64), ("uploads", C.c_uint64), ("iterations", C.c_uint64),
+                ("zero_copy_runs", C.c_uint64), ("zero_copy_steps", C.c_uint64), ("wall_s", C.c_double),
+                ("kernel_ms", C.c_double), ("filter_ms", C.c_double), ("h2d_ms", C.c_double),
+                ("nslots", C.c_uint32), ("nblocks", C.c_uint32), ("learned_decisions", C.c_uint64),
+                ("fallbacks", C.c_uint64), ("final_loss", C.c_float), ("pad", C.c_uint32)]
+
+
+HOST_SIGNATURES = {
+    "gwh_run": [P, P, C.c_uint64, C.POINTER(GwhEngineOpts), C.POINTER(GwWalkParams), C.c_uint64, C.c_uint32, P,
+                C.POINTER(GwhReport), P, P],
+    "gwh_last_error": [],
+}
+
+_hlib = None
+
+
+def host_lib():
+    global _hlib
+    if _hlib is None:
+        lib()
+        if not os.path.exists(HOST_LIB_PATH):
+            raise RuntimeError("libgwhost.so not built (expected %s)" % HOST_LIB_PATH)
+        L = C.CDLL(HOST_LIB_PATH)
+        for name, args in HOST_SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_char_p if name == "gwh_last_error" else C.c_int32
+        _hlib = L
+    return _hlib

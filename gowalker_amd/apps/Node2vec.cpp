@@ -1,0 +1,4 @@
+// Node2vec entry point (apps/Node2vec.cu): options, graph load, engine run, metrics report.
+#include "app.hpp"
+
+int main(int argc, const char** argv) { return gwh::app_main(argc, argv, gwh::App::Node2vec); }

@@ -1,0 +1,4 @@
+// SOSR entry point (apps/SOSR.cu): options, graph load, engine run, metrics report.
+#include "app.hpp"
+
+int main(int argc, const char** argv) { return gwh::app_main(argc, argv, gwh::App::SOSR); }

@@ -63,6 +63,9 @@ struct gw_ctx {
     uint32_t* d_visits = nullptr;
     uint32_t* d_work = nullptr;
     uint32_t* d_bucket = nullptr;
+    unsigned long long* d_trace = nullptr; // per-block steps of the last traced run
+    float* d_feat = nullptr;               // nblocks x kNumFeatures
+    float* d_age = nullptr;
     uint32_t* d_misc = nullptr;            // [0] nwork, [1] queue
     unsigned long long* d_cnt = nullptr;   // 8 counters
     unsigned long long* h_cnt = nullptr;   // pinned readback
@@ -151,6 +154,15 @@ static void free_slots(gw_ctx* c)
     for (auto& b : c->blk_slot) b = -1;
 }
 
+namespace gw {
+gw_status gen_fail(gw_ctx* ctx, gw_status st, const char* msg) { return fail(ctx, st, "%s", msg); }
+hipStream_t ctx_stream(gw_ctx* ctx)
+{
+    (void)hipSetDevice(ctx->device);
+    return ctx->s_compute;
+}
+}  // namespace gw
+
 extern "C" {
 
 const char* gw_last_error(const gw_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
@@ -211,6 +223,9 @@ gw_status gw_destroy(gw_ctx* c)
     free_walkers(c);
     free_slots(c);
     dfree(c, c->d_bucket, 0, false);
+    dfree(c, c->d_trace, 0, false);
+    dfree(c, c->d_feat, 0, false);
+    dfree(c, c->d_age, 0, false);
     dfree(c, c->d_blocks, 0, false);
     dfree(c, c->d_vcut, 0, false);
     dfree(c, c->d_misc, 0, false);
@@ -267,7 +282,11 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     dfree(c, c->d_blocks, 0, false);
     dfree(c, c->d_vcut, 0, false);
     dfree(c, c->d_bucket, 0, false);
+    dfree(c, c->d_trace, 0, false);
+    dfree(c, c->d_feat, 0, false);
+    dfree(c, c->d_age, 0, false);
     c->d_blocks = nullptr; c->d_vcut = nullptr; c->d_bucket = nullptr;
+    c->d_trace = nullptr; c->d_feat = nullptr; c->d_age = nullptr;
     if (c->h_blocks_pinned) { (void)hipHostFree(c->h_blocks_pinned); c->h_blocks_pinned = nullptr; }
     c->nblocks = nb;
     c->desc.assign(b, b + nb);
@@ -294,6 +313,13 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     c->d_vcut = (uint32_t*)p;
     if ((st = dalloc(c, &p, sizeof(uint32_t) * (uint64_t)nb * nb, false)) != GW_OK) return st;
     c->d_bucket = (uint32_t*)p;
+    if ((st = dalloc(c, &p, sizeof(unsigned long long) * nb, false)) != GW_OK) return st;
+    c->d_trace = (unsigned long long*)p;
+    HIPCHK(c, hipMemset(c->d_trace, 0, sizeof(unsigned long long) * nb));
+    if ((st = dalloc(c, &p, sizeof(float) * nb * kNumFeatures, false)) != GW_OK) return st;
+    c->d_feat = (float*)p;
+    if ((st = dalloc(c, &p, sizeof(float) * nb, false)) != GW_OK) return st;
+    c->d_age = (float*)p;
     HIPCHK(c, hipHostMalloc((void**)&c->h_blocks_pinned, sizeof(DevBlock) * nb, hipHostMallocDefault));
     HIPCHK(c, hipMemcpy(c->d_vcut, c->vcut.data(), sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice));
     return GW_OK;
@@ -492,30 +518,34 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
     }
     (void)hipSetDevice(c->device);
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
-    free_walkers(c);
     const uint32_t maxlen = std::max(c->dt[0].length, c->wp.ntypes == 2 ? c->dt[1].length : 0u);
-    c->path_stride = (outputs & GW_OUT_PATHS) ? maxlen + 1 : 0;
-    void* p;
-    gw_status st;
-    c->nwalkers = n;
-    uint32_t** u32s[] = {&c->d_cur, &c->d_prev, &c->d_meta, &c->d_bpair, &c->d_work};
-    for (uint32_t** q : u32s) {
-        if ((st = dalloc(c, &p, n * 4, true)) != GW_OK) { free_walkers(c); return st; }
-        *q = (uint32_t*)p;
+    const uint32_t stride = (outputs & GW_OUT_PATHS) ? maxlen + 1 : 0;
+    // a re-initialisation with the same shape reuses the device buffers
+    if (!(c->nwalkers == n && c->outputs == outputs && c->path_stride == stride)) {
+        free_walkers(c);
+        c->path_stride = stride;
+        void* p;
+        gw_status st;
+        c->nwalkers = n;
+        uint32_t** u32s[] = {&c->d_cur, &c->d_prev, &c->d_meta, &c->d_bpair, &c->d_work};
+        for (uint32_t** q : u32s) {
+            if ((st = dalloc(c, &p, n * 4, true)) != GW_OK) { free_walkers(c); return st; }
+            *q = (uint32_t*)p;
+        }
+        if (outputs & GW_OUT_HASH) {
+            if ((st = dalloc(c, &p, n * 8, true)) != GW_OK) { free_walkers(c); return st; }
+            c->d_hash = (uint64_t*)p;
+        }
+        if (outputs & GW_OUT_PATHS) {
+            if ((st = dalloc(c, &p, n * (uint64_t)stride * 4, true)) != GW_OK) { free_walkers(c); return st; }
+            c->d_paths = (uint32_t*)p;
+        }
+        if (outputs & GW_OUT_VISITS) {
+            if ((st = dalloc(c, &p, c->nverts_total * 4, true)) != GW_OK) { free_walkers(c); return st; }
+            c->d_visits = (uint32_t*)p;
+        }
     }
-    if (outputs & GW_OUT_HASH) {
-        if ((st = dalloc(c, &p, n * 8, true)) != GW_OK) { free_walkers(c); return st; }
-        c->d_hash = (uint64_t*)p;
-    }
-    if (outputs & GW_OUT_PATHS) {
-        if ((st = dalloc(c, &p, n * (uint64_t)c->path_stride * 4, true)) != GW_OK) { free_walkers(c); return st; }
-        c->d_paths = (uint32_t*)p;
-    }
-    if (outputs & GW_OUT_VISITS) {
-        if ((st = dalloc(c, &p, c->nverts_total * 4, true)) != GW_OK) { free_walkers(c); return st; }
-        c->d_visits = (uint32_t*)p;
-        HIPCHK(c, launch_fill_u32(c->d_visits, 0, c->nverts_total, c->s_compute));
-    }
+    if (c->d_visits) HIPCHK(c, launch_fill_u32(c->d_visits, 0, c->nverts_total, c->s_compute));
     c->outputs = outputs;
     uint32_t* d_src = nullptr;
     if (sources) {
@@ -610,6 +640,10 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
         a.nwork = nwork;
         a.queue = c->d_misc + 1;
         a.counters = c->d_cnt;
+        if (l.flags & GW_RUN_TRACE) {
+            HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
+            a.block_steps = c->d_trace;
+        }
         uint64_t lanes = (uint64_t)c->ncu * l.waves_per_cu * 64;
         uint64_t grid = lanes / l.block;
         const uint64_t need = ((uint64_t)nwork + l.block - 1) / l.block;
@@ -662,6 +696,37 @@ gw_status gw_bucket_counts(gw_ctx* c, uint64_t* W, uint64_t n)
     std::vector<uint32_t> tmp(nn);
     HIPCHK(c, hipMemcpy(tmp.data(), c->d_bucket, nn * 4, hipMemcpyDeviceToHost));
     for (uint64_t i = 0; i < nn; ++i) W[i] = tmp[i];
+    return GW_OK;
+}
+
+gw_status gw_trace_read(gw_ctx* c, uint64_t* block_steps, uint64_t n)
+{
+    if (!c || !block_steps) return fail(c, GW_ERR_ARG, "null argument");
+    if (!c->nblocks) return fail(c, GW_ERR_STATE, "no block table");
+    if (n < c->nblocks) return fail(c, GW_ERR_ARG, "trace buffer too small");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    HIPCHK(c, hipMemcpy(block_steps, c->d_trace, c->nblocks * 8ull, hipMemcpyDeviceToHost));
+    return GW_OK;
+}
+
+gw_status gw_block_features(gw_ctx* c, const float* age, float* feats, uint64_t n)
+{
+    if (!c || !feats) return fail(c, GW_ERR_ARG, "null argument");
+    if (!c->nwalkers) return fail(c, GW_ERR_STATE, "no walkers");
+    if (n < (uint64_t)c->nblocks * kNumFeatures) return fail(c, GW_ERR_ARG, "feature buffer too small");
+    (void)hipSetDevice(c->device);
+    gw_status st;
+    if ((st = push_block_table(c)) != GW_OK) return st;
+    uint32_t nwork;
+    uint64_t left;
+    float ms;
+    if ((st = run_filter(c, true, false, &nwork, &left, &ms)) != GW_OK) return st;
+    if (age) HIPCHK(c, hipMemcpyAsync(c->d_age, age, c->nblocks * 4ull, hipMemcpyHostToDevice, c->s_compute));
+    HIPCHK(c, launch_features(c->d_bucket, c->nblocks, c->d_blocks, age ? c->d_age : nullptr, c->d_feat, c->s_compute));
+    HIPCHK(c, hipMemcpyAsync(feats, c->d_feat, (uint64_t)c->nblocks * kNumFeatures * 4, hipMemcpyDeviceToHost,
+                             c->s_compute));
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
     return GW_OK;
 }
 

@@ -18,23 +18,9 @@
 //     exact two-component mixture after counting |N(cur) & N(prev)| (see sopr_step).
 //   * Membership: textbook binary search over prev's sorted adjacency
 //     (engine/engine.hpp:224-243, not the defective device search of gpuwalk.hpp:26-48).
-#include "gw_internal.h"
+#include "gw_device.h"
 
 namespace gw {
-
-__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                        uint32_t k0, uint32_t k1)
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
-    }
-    return make_uint4(c0, c1, c2, c3);
-}
 
 typedef unsigned __int128 u128;
 
@@ -202,17 +188,6 @@ __device__ __forceinline__ Row load_row(const DevBlock& b, uint32_t v)
     return Row{ b.csr + h, t - h };
 }
 
-__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-__device__ __forceinline__ uint64_t wave_sum(uint64_t v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // Persistent step kernel.  Each lane owns one walker at a time and takes one whole step
 // per loop iteration; a lane whose walker ends or leaves the resident set takes the
 // next one from the work queue (one atomic per wavefront), so a wavefront never waits
@@ -220,6 +195,13 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v)
 template <bool ONE>
 __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a)
 {
+    // block-access trace: steps per current block, histogrammed in LDS
+    __shared__ uint32_t trace[kTraceLdsBlocks];
+    const bool lds_trace = a.block_steps && a.nblocks <= kTraceLdsBlocks;
+    if (a.block_steps) {
+        for (uint32_t i = threadIdx.x; i < kTraceLdsBlocks; i += blockDim.x) trace[i] = 0;
+        __syncthreads();
+    }
     Cnt c = {0, 0, 0, 0, 0, 0};
     const uint32_t lane = __lane_id();
     DevBlock one;
@@ -282,6 +264,10 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a)
             if (!ok) {
                 status = kDoneTrialCap;
             } else {
+                if (a.block_steps) {
+                    if (lds_trace) atomicAdd(trace + bc, 1u);
+                    else atomicAdd(a.block_steps + bc, 1ull);
+                }
                 prev = cur;
                 cur = x;
                 ++hop;
@@ -313,6 +299,11 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a)
             if (status != kActive) ++c.ended; else ++c.left;
             have = false;
         }
+    }
+    if (lds_trace) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < a.nblocks; i += blockDim.x)
+            if (trace[i]) atomicAdd(a.block_steps + i, (unsigned long long)trace[i]);
     }
     const uint64_t s0 = wave_sum(c.steps), s1 = wave_sum(c.trials), s2 = wave_sum(c.probes);
     const uint64_t s3 = wave_sum(c.mix), s4 = wave_sum(c.ended), s5 = wave_sum(c.left);
@@ -381,6 +372,57 @@ __global__ void __launch_bounds__(256) filter_kernel(FilterArgs a)
     }
     left = wave_sum(left);
     if (lane == 0 && left) atomicAdd(a.left, (unsigned long long)left);
+}
+
+// Per-block features of the learned scheduler, one thread per block b, from the bucket
+// matrix W (prev-major) of unfinished walkers and the resident set:
+//  0 log1p(row sum W[b,:])            1 log1p(column sum W[:,b])      2 log1p(W[b,b])
+//  3 log1p(sum over resident j != b of W[b,j] + W[j,b])               4 resident flag
+//  5 log1p(min over resident j != b of W[b,j] + W[j,b])  (0 if none)
+//  6 log1p(plain add score: feature 3 + W[b,b])       7 age (iterations since last use) / 64
+//  8 log1p(nedges) / 24    9 log1p(nverts) / 24    10 log1p(mean degree) / 8
+// 11 fraction of all unfinished walkers touching b
+__global__ void __launch_bounds__(256) features_kernel(const uint32_t* __restrict__ W, uint32_t nb,
+                                                        const DevBlock* __restrict__ blocks,
+                                                        const float* __restrict__ age, float* __restrict__ f)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double row = 0, col = 0, res = 0, total = 0;
+    double mn = -1;
+    for (uint32_t j = 0; j < nb; ++j) {
+        const double r = W[(uint64_t)b * nb + j], c = W[(uint64_t)j * nb + b];
+        row += r;
+        col += c;
+        if (j != b && blocks[j].resident) {
+            res += r + c;
+            if (mn < 0 || r + c < mn) mn = r + c;
+        }
+    }
+    for (uint64_t i = 0; i < (uint64_t)nb * nb; ++i) total += W[i];
+    const double diag = W[(uint64_t)b * nb + b];
+    const DevBlock& d = blocks[b];
+    float* o = f + (uint64_t)b * kNumFeatures;
+    o[0] = (float)log1p(row);
+    o[1] = (float)log1p(col);
+    o[2] = (float)log1p(diag);
+    o[3] = (float)log1p(res);
+    o[4] = d.resident ? 1.f : 0.f;
+    o[5] = (float)log1p(mn < 0 ? 0.0 : mn);
+    o[6] = (float)log1p(res + diag);
+    o[7] = age ? age[b] / 64.f : 0.f;
+    o[8] = (float)(log1p((double)d.nedges) / 24.0);
+    o[9] = (float)(log1p((double)d.nverts) / 24.0);
+    o[10] = (float)(log1p(d.nverts ? (double)d.nedges / d.nverts : 0.0) / 8.0);
+    o[11] = total > 0 ? (float)((row + col - diag) / total) : 0.f;
+}
+
+hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age,
+                           float* feats, hipStream_t s)
+{
+    hipLaunchKernelGGL(features_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, s, bucket, nblocks, blocks, age,
+                       feats);
+    return hipGetLastError();
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n)

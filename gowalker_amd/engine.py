@@ -196,3 +196,89 @@ class Context:
         c = GwCounters()
         self._chk(lib().gw_counters_get(self.h, C.byref(c)))
         return {f: getattr(c, f) for f, _ in GwCounters._fields_}
+
+
+class PinnedArray:
+    """numpy view of page-locked host memory from gw_alloc_pinned (freed on close)."""
+
+    def __init__(self, n, dtype):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        p = C.c_void_p()
+        check(lib().gw_alloc_pinned(max(1, self.n * self.dtype.itemsize), C.byref(p)))
+        self.ptr = p
+        buf = (C.c_char * max(1, self.n * self.dtype.itemsize)).from_address(p.value)
+        self.a = np.frombuffer(buf, dtype=self.dtype, count=self.n)
+
+    def close(self):
+        if self.ptr:
+            self.a = None
+            lib().gw_free_pinned(self.ptr)
+            self.ptr = None
+
+
+def _read_graph(ctx, g, pinned):
+    nv, ne = C.c_uint64(), C.c_uint64()
+    check(lib().gw_graph_size(g, C.byref(nv), C.byref(ne)))
+    if pinned:
+        beg, csr = PinnedArray(nv.value + 1, np.uint64), PinnedArray(ne.value, np.uint32)
+        check(lib().gw_graph_read(ctx.h, g, _p(beg.a), _p(csr.a)), ctx.h)
+        return beg, csr
+    beg = np.zeros(nv.value + 1, dtype=np.uint64)
+    csr = np.zeros(ne.value, dtype=np.uint32)
+    check(lib().gw_graph_read(ctx.h, g, _p(beg), _p(csr)), ctx.h)
+    return beg, csr
+
+
+def gen_rmat(ctx, scale, nedges, a=0.57, b=0.19, c=0.19, seed=42, undirected=True, permute=False, dedup=True,
+             pinned=False):
+    """RMAT graph generated on the GPU, returned as host CSR arrays (numpy or PinnedArray)."""
+    from ._abi import GW_GEN_DEDUP, GW_GEN_PERMUTE, GW_GEN_UNDIRECTED
+    flags = (GW_GEN_UNDIRECTED if undirected else 0) | (GW_GEN_PERMUTE if permute else 0) | \
+        (GW_GEN_DEDUP if dedup else 0)
+    g = C.c_void_p()
+    check(lib().gw_gen_rmat(ctx.h, scale, nedges, a, b, c, seed, flags, C.byref(g)), ctx.h)
+    try:
+        return _read_graph(ctx, g, pinned)
+    finally:
+        lib().gw_graph_free(ctx.h, g)
+
+
+def gen_chung_lu(ctx, nverts, nedges, gamma=2.2, max_degree=3000000, seed=42, pinned=False):
+    """Directed Chung-Lu power-law graph generated on the GPU, as host CSR arrays."""
+    g = C.c_void_p()
+    check(lib().gw_gen_chung_lu(ctx.h, nverts, nedges, gamma, max_degree, seed, C.byref(g)), ctx.h)
+    try:
+        return _read_graph(ctx, g, pinned)
+    finally:
+        lib().gw_graph_free(ctx.h, g)
+
+
+SCHED_PLAIN, SCHED_LEARNED = 0, 1
+
+
+def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory_bytes=0, block_bytes=256 << 20,
+                    zero_threshold=0, scheduler=SCHED_PLAIN, hashes=False, visits=False, verbose=False, launch=None):
+    """Runs the C++ out-of-core engine (libgwhost.so) over a host CSR graph; beg/csr should
+    be pinned (PinnedArray.a) for the zero-copy tail.  Returns (report dict, hashes, visits)."""
+    from ._abi import GwhEngineOpts, GwhReport, host_lib
+    L = host_lib()
+    o = GwhEngineOpts()
+    o.memory_bytes = memory_bytes
+    o.block_bytes = block_bytes
+    o.zero_threshold = zero_threshold
+    o.scheduler = scheduler
+    o.verbose = 1 if verbose else 0
+    if launch:
+        o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 0), 0)
+    wp = walk_params(types, seed)
+    rep = GwhReport()
+    h = np.zeros(nwalkers, dtype=np.uint64) if hashes else None
+    v = np.zeros(len(beg) - 1, dtype=np.uint32) if visits else None
+    src = None if sources is None else np.ascontiguousarray(sources, dtype=np.uint32)
+    st = L.gwh_run(_p(beg), _p(csr), len(beg), C.byref(o), C.byref(wp), nwalkers, wpv, _p(src), C.byref(rep),
+                   _p(h), _p(v))
+    if st != 0:
+        from ._abi import GwError
+        raise GwError(st, L.gwh_last_error().decode())
+    return {f: getattr(rep, f) for f, _ in GwhReport._fields_ if f != "pad"}, h, v

@@ -1,0 +1,186 @@
+// engine.hpp -- the out-of-core walk driver: GOWalker's graph_engine::pipe_run and
+// pipeline (engine/engine.hpp:406-638) on top of libgw.so.
+//
+// Per iteration: read the bucket matrix of unfinished walkers from the device, let the
+// scheduler change at most one resident block, place the wanted blocks into HBM slots
+// (copyblocks, engine/schedule.hpp:144-197), upload what is missing on the copy stream,
+// and run every walker whose (prev, cur) blocks are resident until it finishes or steps
+// out.  When the scheduler finds nothing worth loading, the remaining walkers finish by
+// reading non-resident blocks from pinned host memory (the zero-copy tail of
+// engine/engine.hpp:459-515, 593-634).  Walks do not depend on the schedule: every draw
+// is keyed on (seed, walker, hop, draw), so only bytes moved and time change.
+#pragma once
+#include <chrono>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gw.h"
+#include "schedule.hpp"
+
+namespace gwh {
+
+struct GwError : std::runtime_error {
+    gw_status status;
+    GwError(gw_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+inline void ck(gw_status s, gw_ctx* c, const char* what)
+{
+    if (s != GW_OK) throw GwError(s, std::string(what) + ": " + gw_last_error(c));
+}
+
+struct EngineConfig {
+    int device = 0;
+    uint64_t memory_bytes = 0;      // HBM budget for slots + walker state (0: every block fits)
+    uint64_t zero_threshold = 0;
+    bool zero_copy_tail = true;
+    gw_launch launch{0, 0, 0, 0};
+    bool verbose = false;
+};
+
+struct WalkConfig {
+    gw_walk_params params{};
+    uint64_t nwalkers = 0;
+    uint32_t wpv = 1;
+    const uint32_t* sources = nullptr;
+    uint32_t outputs = 0;
+};
+
+struct RunReport {
+    uint64_t steps = 0, trials = 0, probes = 0, mix_steps = 0;
+    uint64_t h2d_bytes = 0, uploads = 0, iterations = 0;
+    uint64_t zero_copy_runs = 0, zero_copy_steps = 0;
+    double wall_s = 0, kernel_ms = 0, filter_ms = 0, h2d_ms = 0;
+    uint32_t nslots = 0, nblocks = 0;
+};
+
+class Engine {
+public:
+    // beg/csr: the whole graph in host memory (pinned for the zero-copy tail); blocks:
+    // the block table over it.
+    Engine(const EngineConfig& cfg, const uint64_t* beg, const uint32_t* csr, std::vector<gw_block_desc> blocks)
+        : cfg_(cfg), beg_(beg), csr_(csr), blocks_(std::move(blocks))
+    {
+        gw_config gc{cfg.device, 0, cfg.memory_bytes};
+        ck(gw_create(&gc, &ctx_), nullptr, "gw_create");
+        ck(gw_set_block_table(ctx_, blocks_.data(), (uint32_t)blocks_.size()), ctx_, "gw_set_block_table");
+    }
+    ~Engine() { gw_destroy(ctx_); }
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    gw_ctx* ctx() { return ctx_; }
+    uint32_t nblocks() const { return (uint32_t)blocks_.size(); }
+
+    // Number of HBM slots that fit next to the walker state under the budget.
+    uint32_t slots_for_budget()
+    {
+        const uint32_t nb = nblocks();
+        if (!cfg_.memory_bytes) return nb;
+        gw_counters c;
+        ck(gw_counters_get(ctx_, &c), ctx_, "gw_counters_get");
+        uint64_t per = 0;
+        ck(gw_slot_bytes(ctx_, &per), ctx_, "gw_slot_bytes");
+        uint64_t free_b = cfg_.memory_bytes > c.hbm_bytes_held ? cfg_.memory_bytes - c.hbm_bytes_held : 0;
+        uint64_t n = free_b / per;
+        return (uint32_t)(n < nb ? n : nb);
+    }
+
+    RunReport run(const WalkConfig& wc, Scheduler& sched)
+    {
+        auto t0 = std::chrono::steady_clock::now();
+        RunReport r;
+        const uint32_t nb = nblocks();
+        r.nblocks = nb;
+        gw_counters c0;
+        ck(gw_set_walk(ctx_, &wc.params), ctx_, "gw_set_walk");
+        ck(gw_init_walkers(ctx_, wc.nwalkers, wc.wpv, wc.sources, wc.outputs), ctx_, "gw_init_walkers");
+        const uint32_t want = slots_for_budget();
+        if (want == 0) throw GwError(GW_ERR_BUDGET, "the memory budget leaves no room for a single block slot");
+        if (want != nslots_) {
+            ck(gw_reserve_slots(ctx_, want), ctx_, "gw_reserve_slots");
+            nslots_ = want;
+        } else {
+            for (uint32_t s = 0; s < nslots_; ++s) ck(gw_evict(ctx_, s), ctx_, "gw_evict");
+        }
+        r.nslots = nslots_;
+        slots_.reset(nslots_, nb);
+        sched.buckets.clear();
+        ck(gw_counters_get(ctx_, &c0), ctx_, "gw_counters_get");
+        std::vector<uint64_t> W((uint64_t)nb * nb);
+        for (uint64_t it = 0;; ++it) {
+            ck(gw_bucket_counts(ctx_, W.data(), W.size()), ctx_, "gw_bucket_counts");
+            uint64_t left = 0;
+            for (uint64_t x : W) left += x;
+            if (!left) break;
+            SchedInput in{W.data(), nb, nslots_, cfg_.zero_threshold, blocks_.data(), it};
+            IterationOutcome out;
+            out.iteration = it;
+            gw_run_stats st{};
+            gw_launch launch = cfg_.launch;
+            launch.flags |= sched.run_flags();
+            if (sched.schedule(in)) {
+                uint64_t bytes = 0;
+                for (uint32_t b : sched.buckets) {
+                    int s = slots_.place(b, sched.buckets);
+                    if (s == -2) throw GwError(GW_ERR_STATE, "scheduler wants more blocks than slots");
+                    if (s < 0) continue;
+                    const gw_block_desc& d = blocks_[b];
+                    ck(gw_upload_block(ctx_, (uint32_t)s, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,
+                       "gw_upload_block");
+                    bytes += ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4;
+                    out.loaded.push_back(b);
+                    r.uploads++;
+                }
+                ck(gw_run(ctx_, &launch, &st), ctx_, "gw_run");
+                out.h2d_bytes = bytes;
+            } else {
+                if (!cfg_.zero_copy_tail)
+                    throw GwError(GW_ERR_STATE, "walkers remain but no block is worth loading (zero-copy tail off)");
+                for (uint32_t b = 0; b < nb; ++b) {
+                    const gw_block_desc& d = blocks_[b];
+                    ck(gw_map_host_block(ctx_, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,
+                       "gw_map_host_block");
+                }
+                ck(gw_run(ctx_, &launch, &st), ctx_, "gw_run (zero-copy)");
+                ck(gw_unmap_host_blocks(ctx_), ctx_, "gw_unmap_host_blocks");
+                r.zero_copy_runs++;
+                r.zero_copy_steps += st.steps;
+            }
+            out.steps = st.steps;
+            out.kernel_ms = st.kernel_ms;
+            sched.feedback(in, out);
+            r.iterations++;
+            r.steps += st.steps;
+            r.trials += st.trials;
+            r.probes += st.probes;
+            r.mix_steps += st.mix_steps;
+            r.kernel_ms += st.kernel_ms;
+            r.filter_ms += st.filter_ms;
+            if (cfg_.verbose)
+                std::fprintf(stderr, "[engine] it %llu: %s, %zu resident, steps %llu, walkers run %llu, left %llu\n",
+                             (unsigned long long)it, out.loaded.empty() && st.walkers_run ? "no upload" : "upload",
+                             sched.buckets.size(), (unsigned long long)st.steps, (unsigned long long)st.walkers_run,
+                             (unsigned long long)st.walkers_left);
+        }
+        gw_counters c1;
+        ck(gw_counters_get(ctx_, &c1), ctx_, "gw_counters_get");
+        r.h2d_bytes = c1.h2d_bytes - c0.h2d_bytes;
+        r.h2d_ms = c1.h2d_ms - c0.h2d_ms;
+        r.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return r;
+    }
+
+private:
+    EngineConfig cfg_;
+    const uint64_t* beg_;
+    const uint32_t* csr_;
+    std::vector<gw_block_desc> blocks_;
+    gw_ctx* ctx_ = nullptr;
+    uint32_t nslots_ = 0;
+    SlotMap slots_;
+};
+
+}  // namespace gwh

@@ -1,0 +1,87 @@
+// gwh.cpp -- libgwhost.so: the C entry of include/gwh.h over the C++ engine.
+#include "gwh.h"
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "format.hpp"
+#include "learned.hpp"
+#include "schedule.hpp"
+
+static std::string g_err;
+
+extern "C" const char* gwh_last_error(void) { return g_err.c_str(); }
+
+extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t entries, const gwh_engine_opts* opts,
+                             const gw_walk_params* params, uint64_t nwalkers, uint32_t wpv, const uint32_t* sources,
+                             gwh_report* report, uint64_t* hashes, uint32_t* visits)
+{
+    using namespace gwh;
+    if (!beg || !csr || entries < 2 || !opts || !params || !report || opts->block_bytes < 64) {
+        g_err = "gwh_run: bad arguments";
+        return GW_ERR_ARG;
+    }
+    try {
+        std::vector<uint32_t> vb(entries + 2);
+        std::vector<uint64_t> eb(entries + 2);
+        uint64_t n = 0;
+        ck(gw_split_blocks(beg, entries, opts->block_bytes, vb.data(), eb.data(), vb.size(), &n), nullptr,
+           "gw_split_blocks");
+        vb.resize(n);
+        eb.resize(n);
+        EngineConfig ec;
+        ec.device = opts->device;
+        ec.memory_bytes = opts->memory_bytes;
+        ec.zero_threshold = opts->zero_threshold;
+        ec.launch = opts->launch;
+        ec.verbose = opts->verbose != 0;
+        Engine engine(ec, beg, csr, make_block_table(vb, eb, opts->block_bytes));
+        WalkConfig wc;
+        wc.params = *params;
+        wc.nwalkers = nwalkers;
+        wc.wpv = wpv;
+        wc.sources = sources;
+        wc.outputs = (hashes ? (uint32_t)GW_OUT_HASH : 0u) | (visits ? (uint32_t)GW_OUT_VISITS : 0u);
+        std::unique_ptr<Scheduler> sched;
+        LearnedScheduler* ls = nullptr;
+        if (opts->scheduler == GWH_SCHED_LEARNED) {
+            ls = new LearnedScheduler(engine.ctx(), LearnedConfig());
+            sched.reset(ls);
+        } else {
+            sched.reset(new PlainScheduler());
+        }
+        RunReport r = engine.run(wc, *sched);
+        *report = gwh_report{};
+        report->steps = r.steps;
+        report->trials = r.trials;
+        report->probes = r.probes;
+        report->mix_steps = r.mix_steps;
+        report->h2d_bytes = r.h2d_bytes;
+        report->uploads = r.uploads;
+        report->iterations = r.iterations;
+        report->zero_copy_runs = r.zero_copy_runs;
+        report->zero_copy_steps = r.zero_copy_steps;
+        report->wall_s = r.wall_s;
+        report->kernel_ms = r.kernel_ms;
+        report->filter_ms = r.filter_ms;
+        report->h2d_ms = r.h2d_ms;
+        report->nslots = r.nslots;
+        report->nblocks = r.nblocks;
+        if (ls) {
+            report->learned_decisions = ls->learned_decisions();
+            report->fallbacks = ls->fallbacks();
+            report->final_loss = ls->last_loss();
+        }
+        if (hashes) ck(gw_read_hashes(engine.ctx(), hashes, nwalkers), engine.ctx(), "gw_read_hashes");
+        if (visits) ck(gw_read_visits(engine.ctx(), visits, entries - 1), engine.ctx(), "gw_read_visits");
+    } catch (const GwError& e) {
+        g_err = e.what();
+        return e.status;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return GW_ERR_IO;
+    }
+    return GW_OK;
+}

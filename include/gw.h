@@ -91,7 +91,8 @@ typedef struct {
 } gw_launch;
 
 enum {
-    GW_RUN_BUCKETS = 1u << 0 /* also recount the (prev_blk, cur_blk) bucket matrix */
+    GW_RUN_BUCKETS = 1u << 0, /* also recount the (prev_blk, cur_blk) bucket matrix */
+    GW_RUN_TRACE = 1u << 1    /* record steps per current block (gw_trace_read) */
 };
 
 typedef struct {
@@ -178,6 +179,34 @@ gw_status gw_run(gw_ctx* ctx, const gw_launch* launch, gw_run_stats* stats);
 gw_status gw_bucket_counts(gw_ctx* ctx, uint64_t* W, uint64_t n);
 gw_status gw_walkers_left(gw_ctx* ctx, uint64_t* left);
 
+/* ---- learned block scheduler support (no reference counterpart: the reference's
+ *      GOwalker_scheduler_t, engine/schedule.hpp:70-138, is the plain rule this
+ *      scheduler falls back to) ---- */
+/* Block-access trace of the last gw_run with GW_RUN_TRACE: walk steps taken while each
+ * block held the walker's current vertex. */
+gw_status gw_trace_read(gw_ctx* ctx, uint64_t* block_steps, uint64_t n);
+/* 12 features per block (nblocks x 12, row major) from the current bucket matrix, the
+ * resident set and a caller-kept age per block (iterations since last use; may be NULL). */
+gw_status gw_block_features(gw_ctx* ctx, const float* age, float* feats, uint64_t n);
+
+/* MLP in -> h1 -> h2 -> 1 (ReLU), BF16 weights, FP32 accumulation, trained on the GPU
+ * with Adam on FP32 master weights.  Parameters are exchanged as fp32 in the flat order
+ * W1 (h1 x in), b1, W2 (h2 x h1), b2, w3 (h2), b3. */
+typedef struct gw_mlp gw_mlp;
+gw_status gw_mlp_create(gw_ctx* ctx, uint32_t in, uint32_t h1, uint32_t h2, uint64_t seed, gw_mlp** out);
+gw_status gw_mlp_destroy(gw_ctx* ctx, gw_mlp* m);
+gw_status gw_mlp_num_params(const gw_mlp* m, uint32_t* n);
+gw_status gw_mlp_set_params(gw_ctx* ctx, gw_mlp* m, const float* params);
+gw_status gw_mlp_get_params(gw_ctx* ctx, const gw_mlp* m, float* params);
+/* x: n x in (host, row major); y: n outputs */
+gw_status gw_mlp_forward(gw_ctx* ctx, gw_mlp* m, const float* x, uint32_t n, float* y);
+/* mean-squared-error gradient of every parameter and the loss, for targets t */
+gw_status gw_mlp_backward(gw_ctx* ctx, gw_mlp* m, const float* x, const float* t, uint32_t n,
+                          float* grad, float* loss);
+/* one Adam step (lr) on the batch; loss receives the batch loss before the step */
+gw_status gw_mlp_step(gw_ctx* ctx, gw_mlp* m, const float* x, const float* t, uint32_t n,
+                      float lr, float* loss);
+
 /* ---- outputs ---- */
 gw_status gw_read_visits(gw_ctx* ctx, uint32_t* out, uint64_t n);
 gw_status gw_read_hashes(gw_ctx* ctx, uint64_t* out, uint64_t n);
@@ -185,6 +214,25 @@ gw_status gw_read_hashes(gw_ctx* ctx, uint64_t* out, uint64_t n);
 gw_status gw_read_state(gw_ctx* ctx, uint32_t* out4, uint64_t n);
 gw_status gw_read_paths(gw_ctx* ctx, uint32_t* out, uint64_t n);
 gw_status gw_counters_get(gw_ctx* ctx, gw_counters* out);
+
+/* ---- synthetic graphs generated in HBM (the reference downloads real datasets,
+ *      script/download_*.sh; its text->CSR converter cannot run as shipped,
+ *      api/constants.hpp:5 + preprocess/graph_converter.hpp:227-231).  The result is the
+ *      reference's CSR layout: u64 offsets indexed by vertex id (nverts+1), u32 sorted
+ *      adjacency. ---- */
+typedef struct gw_graph gw_graph;
+enum { GW_GEN_UNDIRECTED = 1u << 0, GW_GEN_PERMUTE = 1u << 1, GW_GEN_DEDUP = 1u << 2 };
+/* RMAT with 2^scale ids and nedges generated edges (self-loops dropped). */
+gw_status gw_gen_rmat(gw_ctx* ctx, uint32_t scale, uint64_t nedges, double a, double b, double c,
+                      uint64_t seed, uint32_t flags, gw_graph** out);
+/* Directed Chung-Lu power law: out-degree exponent gamma, vertex 0 has max_degree
+ * out-edges, targets drawn by weight (multi-edges kept, lists sorted). */
+gw_status gw_gen_chung_lu(gw_ctx* ctx, uint64_t nverts, uint64_t nedges, double gamma,
+                          uint64_t max_degree, uint64_t seed, gw_graph** out);
+gw_status gw_graph_size(const gw_graph* g, uint64_t* nverts, uint64_t* nedges);
+/* Copies the CSR to host memory (beg: nverts+1, csr: nedges). */
+gw_status gw_graph_read(gw_ctx* ctx, const gw_graph* g, uint64_t* beg, uint32_t* csr);
+gw_status gw_graph_free(gw_ctx* ctx, gw_graph* g);
 
 /* ---- host-side block format helpers (no GPU needed) ---- */
 /* split_blocks (preprocess/split.hpp:15-87) over an in-memory .beg array. */

@@ -1,0 +1,49 @@
+/*
+ * gwh.h -- C entry to the host-side engine (libgwhost.so): GOWalker's graph_engine
+ * (engine/engine.hpp:54-638) driving libgw.so over an in-memory CSR graph, for callers
+ * that are not C++ (the reference has no such entry; its apps call the engine directly,
+ * apps/Node2vec.cu:173-176).
+ */
+#ifndef GWH_H
+#define GWH_H
+#include "gw.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { GWH_SCHED_PLAIN = 0, GWH_SCHED_LEARNED = 1 };
+
+typedef struct {
+    uint64_t memory_bytes;     /* HBM budget (slots + walker state); 0 = every block resident */
+    uint64_t block_bytes;      /* blocksize in bytes */
+    uint64_t zero_threshold;   /* walkers below which the run ends with zero-copy reads */
+    uint32_t scheduler;        /* GWH_SCHED_* */
+    uint32_t verbose;
+    gw_launch launch;
+    int32_t device;
+    uint32_t reserved;
+} gwh_engine_opts;
+
+typedef struct {
+    uint64_t steps, trials, probes, mix_steps;
+    uint64_t h2d_bytes, uploads, iterations, zero_copy_runs, zero_copy_steps;
+    double wall_s, kernel_ms, filter_ms, h2d_ms;
+    uint32_t nslots, nblocks;
+    uint64_t learned_decisions, fallbacks;
+    float final_loss;
+    uint32_t pad;
+} gwh_report;
+
+/* Runs walkers to completion out of core: beg has `entries` u64 offsets (ids
+ * 0..entries-2), csr the u32 targets; both in pinned memory (gw_alloc_pinned) for the
+ * zero-copy tail.  hashes (nwalkers) and visits (entries-1) are optional outputs. */
+gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t entries, const gwh_engine_opts* opts,
+                  const gw_walk_params* params, uint64_t nwalkers, uint32_t walkers_per_vertex,
+                  const uint32_t* sources, gwh_report* report, uint64_t* hashes, uint32_t* visits);
+const char* gwh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
