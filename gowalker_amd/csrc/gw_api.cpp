@@ -616,7 +616,7 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
     gw_launch l = L ? *L : gw_launch{};
     if (l.block == 0) l.block = 256;
     if (l.block % 64 || l.block > 1024) return fail(c, GW_ERR_ARG, "block must be a multiple of 64 up to 1024");
-    if (l.waves_per_cu == 0) l.waves_per_cu = 16;
+
     gw_status st;
     if ((st = push_block_table(c)) != GW_OK) return st;
     uint32_t nwork = 0;
@@ -644,14 +644,16 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
             HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
             a.block_steps = c->d_trace;
         }
-        uint64_t lanes = (uint64_t)c->ncu * l.waves_per_cu * 64;
-        uint64_t grid = lanes / l.block;
+        // persistent grid: as many workgroups as the CUs hold at once (occupancy), or
+        // the caller's waves per CU
+        const bool one_blk = c->nblocks == 1;
+        uint64_t grid = l.waves_per_cu ? (uint64_t)c->ncu * l.waves_per_cu * 64 / l.block
+                                       : (uint64_t)c->ncu * walk_blocks_per_cu(one_blk, l.block);
         const uint64_t need = ((uint64_t)nwork + l.block - 1) / l.block;
         if (grid > need) grid = need;
         if (grid == 0) grid = 1;
-        const bool one = c->nblocks == 1;
         HIPCHK(c, hipEventRecord(c->ev_k0, c->s_compute));
-        HIPCHK(c, launch_walk(a, one, l.variant, (uint32_t)grid, l.block, c->s_compute));
+        HIPCHK(c, launch_walk(a, one_blk, l.variant, (uint32_t)grid, l.block, c->s_compute));
         HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));
         HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                  c->s_compute));

@@ -99,6 +99,7 @@ hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s);
 hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid,
                        uint32_t block, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+uint32_t walk_blocks_per_cu(bool one_block, uint32_t block);   // resident workgroups per CU
 hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age,
                            float* feats, hipStream_t s);
 constexpr uint32_t kTraceLdsBlocks = 2048;   // per-workgroup LDS histogram size of the trace

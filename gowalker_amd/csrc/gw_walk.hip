@@ -459,6 +459,14 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
     return hipGetLastError();
 }
 
+uint32_t walk_blocks_per_cu(bool one_block, uint32_t block)
+{
+    int n = 0;
+    hipError_t e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true>, (int)block, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false>, (int)block, 0);
+    return (e == hipSuccess && n > 0) ? (uint32_t)n : 1u;
+}
+
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s)
 {
     if (n == 0) return hipSuccess;

@@ -67,16 +67,9 @@ public:
         int best = -1;
         for (uint32_t c = 0; c < nb; ++c) {
             if (cached[c]) continue;
-            uint32_t pos;
-            std::vector<uint32_t> C = buckets;
-            if (C.size() >= in.ncblock) C.clear();   // gain against any resident partner
-            uint64_t gain = PlainScheduler::score(in, buckets.size() >= in.ncblock ? buckets : C, c, &pos);
-            if (buckets.size() >= in.ncblock) {
-                // with a full cache a candidate must have walkers paired with some resident block
-                gain = 0;
-                for (uint32_t j : buckets) gain += in.W[(uint64_t)c * nb + j] + in.W[(uint64_t)j * nb + c];
-                gain += in.W[(uint64_t)c * nb + c];
-            }
+            // walkers that could run at once if c joined the resident set
+            uint64_t gain = in.W[(uint64_t)c * nb + c];
+            for (uint32_t j : buckets) gain += in.W[(uint64_t)c * nb + j] + in.W[(uint64_t)j * nb + c];
             if (gain == 0 || gain <= in.zero_threshold) continue;
             if (best < 0 || pred_[c] > pred_[best]) best = (int)c;
         }

@@ -90,8 +90,8 @@ def ptr(a):
 
 
 def philox(ctr, key):
-    c = np.asarray(ctr, dtype=np.uint32)
-    k = np.asarray(key, dtype=np.uint32)
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
     out = np.zeros(4, dtype=np.uint32)
     lib().orc_philox4x32_10(ptr(c), ptr(k), ptr(out))
     return out

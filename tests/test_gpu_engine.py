@@ -1,0 +1,131 @@
+"""The C++ out-of-core engine (libgwhost.so: scheduler, slot cache, H2D uploads, zero-copy
+tail) over libgw.so: walks are identical to the oracle's whatever the cache budget and
+scheduler; bytes moved follow the block sizes; failures come back as status codes."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import graphs as G
+from gowalker_amd import engine as E
+from gowalker_amd._abi import GwError
+
+pytestmark = pytest.mark.gpu
+
+
+def pinned_graph(beg, csr):
+    b = E.PinnedArray(len(beg), np.uint64)
+    c = E.PinnedArray(len(csr), np.uint32)
+    b.a[:] = beg
+    c.a[:] = csr
+    return b, c
+
+
+def oracle_hashes(beg, csr, types, seed, n, wpv):
+    t = types[0]
+    if t["algo"] == E.NODE2VEC:
+        tc = O.type_cfg(O.N2V, t["length"], p=t["p"], q=t["q"], term=t.get("term", 0.0))
+    else:
+        tc = O.type_cfg(O.SOPR, t["length"], alpha=t["alpha"], term=t.get("term", 0.0))
+    return O.run(beg, csr, O.walk_cfg(seed, tc), n, wpv=wpv, visits=True)
+
+
+@pytest.mark.parametrize("sched", [E.SCHED_PLAIN, E.SCHED_LEARNED])
+@pytest.mark.parametrize("memory_blocks", [3, 6, 100])
+def test_out_of_core_matches_oracle(gpu, sched, memory_blocks):
+    beg, csr = G.rmat(13, 60000, seed=3, permute=True)
+    n = len(beg) - 1
+    block_bytes = 16384
+    b, c = pinned_graph(beg, csr)
+    types = [dict(algo=E.NODE2VEC, length=20, p=0.5, q=2.0)]
+    # budget = walker state (5 u32 + a u64 hash per walker, a u32 visit count per vertex)
+    # + `memory_blocks` slots
+    with E.Context() as ctx:
+        ctx.set_blocks(E.Blocks(beg, csr, block_bytes))
+        slot = ctx.slot_bytes()
+    walker_bytes = n * (5 * 4 + 8) + n * 4
+    mem = walker_bytes + memory_blocks * slot + 4096
+    rep, h, v = E.run_out_of_core(b.a, c.a, types, 9, n, memory_bytes=mem, block_bytes=block_bytes,
+                                  scheduler=sched, hashes=True, visits=True)
+    o = oracle_hashes(beg, csr, types, 9, n, 1)
+    print(rep)
+    np.testing.assert_array_equal(h, o["hashes"])
+    np.testing.assert_array_equal(v, o["visits"])
+    assert rep["steps"] == o["totals"]["steps"]
+    assert rep["nslots"] == min(memory_blocks, rep["nblocks"])
+    assert rep["h2d_bytes"] > 0 and rep["uploads"] >= min(memory_blocks, rep["nblocks"])
+    b.close()
+    c.close()
+
+
+def test_zero_copy_tail_and_sopr(gpu):
+    beg, csr = G.power_law_directed(20000, 200000, seed=5)
+    n = len(beg) - 1
+    b, c = pinned_graph(beg, csr)
+    types = [dict(algo=E.SOPR, length=25, alpha=0.2, term=0.1)]
+    rep, h, v = E.run_out_of_core(b.a, c.a, types, 4, n * 2, wpv=2, memory_bytes=0, block_bytes=8192,
+                                  zero_threshold=400, hashes=True, visits=True)
+    o = oracle_hashes(beg, csr, types, 4, n * 2, 2)
+    np.testing.assert_array_equal(h, o["hashes"])
+    np.testing.assert_array_equal(v, o["visits"])
+    assert rep["zero_copy_runs"] >= 1
+    b.close()
+    c.close()
+
+
+def test_budget_too_small_is_an_error(gpu):
+    beg, csr = G.rmat(10, 5000, seed=1)
+    b, c = pinned_graph(beg, csr)
+    with pytest.raises(GwError) as e:
+        E.run_out_of_core(b.a, c.a, [dict(algo=E.NODE2VEC, length=5, p=1.0, q=1.0)], 1, 1024,
+                          memory_bytes=1024, block_bytes=4096)
+    assert "budget" in str(e.value).lower()
+    b.close()
+    c.close()
+
+
+def test_injected_upload_failure_returns_status(gpu):
+    # the failure is injected from the host (GW_FAULT); the library returns a status and
+    # the process carries on -- run in a child so the environment stays clean
+    code = (
+        "import sys; sys.path[:0]=[%r, %r];"
+        "import numpy as np, graphs as G; from gowalker_amd import engine as E; from gowalker_amd._abi import GwError;"
+        "beg,csr=G.rmat(10,5000,seed=1); b=E.PinnedArray(len(beg),np.uint64); c=E.PinnedArray(len(csr),np.uint32);"
+        "b.a[:]=beg; c.a[:]=csr\n"
+        "try:\n E.run_out_of_core(b.a,c.a,[dict(algo=0,length=5,p=1.0,q=1.0)],1,1024,block_bytes=4096)\n"
+        "except GwError as e:\n print('STATUS', e.status, e); sys.exit(0)\n"
+        "sys.exit(3)\n") % (os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                            os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, GW_FAULT="upload:3")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "STATUS -7" in r.stdout
+
+
+def test_apps_end_to_end(gpu, tmp_path):
+    """gw_generate writes the on-disk format; Node2vec runs on it out of core and its path
+    hashes equal the oracle's on the same files."""
+    build = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gowalker_amd", "_build")
+    base = str(tmp_path / "g")
+    subprocess.run([os.path.join(build, "gw_generate"), base, "rmat", "scale", "16", "edges", "400000", "undirected",
+                    "dedup", "permute", "seed", "3"], check=True, timeout=120)
+    hf = str(tmp_path / "h.bin")
+    r = subprocess.run([os.path.join(build, "Node2vec"), base + ".txt", "length", "15", "walkpersource", "2",
+                        "memory", "1", "blocksize", "1", "p", "0.5", "q", "2", "gpu", "cpu", "GOWalker", "seed", "5",
+                        "hashes", hf, "metrics.reporter.filename", str(tmp_path / "m.txt")],
+                       capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    beg = np.fromfile(base + ".beg", dtype=np.uint64)
+    csr = np.fromfile(base + ".csr", dtype=np.uint32)
+    o = O.run(beg, csr, O.walk_cfg(5, O.type_cfg(O.N2V, 15, p=0.5, q=2.0)), (len(beg) - 1) * 2, wpv=2)
+    np.testing.assert_array_equal(np.fromfile(hf, dtype=np.uint64), o["hashes"])
+    m = open(str(tmp_path / "m.txt")).read()
+    assert "0_Total_time=" in m and "walk_steps_per_s=" in m
+    # the reference requires one scheduler flag
+    r2 = subprocess.run([os.path.join(build, "Node2vec"), base + ".txt", "length", "5"], capture_output=True,
+                        text=True, timeout=60, cwd=str(tmp_path))
+    assert "Please select one walker from GOWalker and SOWalker!" in r2.stdout
