@@ -28,6 +28,12 @@ CONFIGS = {
                  "4 walkers/vertex, alpha 0.2, term 0.15, length cap 30, fully HBM-resident",
         graph=dict(kind="chung_lu", nverts=41_600_000, nedges=1_470_000_000, gamma=2.2, max_degree=3_000_000),
         types=[dict(algo=1, length=30, alpha=0.2, term=0.15)], wpv=4),
+    "node2vec_rmat26_ooc": dict(
+        workload="Node2vec out of core, RMAT scale 26 (67.1M ids) with 1.8B generated edges symmetrized, dedup, "
+                 "random id permutation, 256 MB blocks, HBM cap 16 GB, p 0.5, q 2, 1 walker/vertex, length 40; "
+                 "plain vs learned block scheduler",
+        graph=dict(kind="rmat", scale=26, nedges=1_800_000_000, undirected=True, permute=True),
+        types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=256, memory_gb=16),
     "node2vec_rmat20": dict(
         workload="Node2vec, RMAT scale 20 (2^20 V, 16M edges symmetrized, dedup, no self-loops), "
                  "p 2, q 0.5, 1 walker/vertex, length 80, single resident block",
@@ -76,6 +82,53 @@ def cpu_baseline(beg, csr, cfg, seconds, threads):
                        % (r["done"], sample, dt, r["moves"]))
 
 
+def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
+    """Out-of-core configuration: the C++ engine with the plain and the learned scheduler."""
+    from gowalker_amd import engine as E
+    ctx.close()
+    mem = int((args.memory_gb if args.memory_gb else cfg["memory_gb"]) * (1 << 30))
+    res = {}
+    hashes = {}
+    for name, sched in (("plain", E.SCHED_PLAIN), ("learned", E.SCHED_LEARNED)):
+        reps = []
+        for i in range(args.warmup + args.steps):
+            rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nverts * cfg["wpv"], wpv=cfg["wpv"],
+                                          memory_bytes=mem, block_bytes=cfg["block_mb"] << 20,
+                                          scheduler=sched, hashes=(i == 0))
+            if i == 0:
+                hashes[name] = h
+            print("[bench] %s run %d: %.2f s, %d steps, %.2f GB H2D, %d iterations, %d slots / %d blocks" %
+                  (name, i, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["iterations"], rep["nslots"],
+                   rep["nblocks"]), file=sys.stderr, flush=True)
+            if i >= args.warmup:
+                reps.append(rep)
+        wall = sum(r["wall_s"] for r in reps)
+        res[name] = dict(steps_per_s=sum(r["steps"] for r in reps) / wall, h2d_gb=reps[0]["h2d_bytes"] / 1e9,
+                         ms_per_run=wall * 1e3 / len(reps), iterations=reps[0]["iterations"],
+                         uploads=reps[0]["uploads"], zero_copy_steps=reps[0]["zero_copy_steps"],
+                         h2d_gbs=(reps[0]["h2d_bytes"] / 1e9) / (reps[0]["h2d_ms"] / 1e3) if reps[0]["h2d_ms"] else None,
+                         kernel_ms=reps[0]["kernel_ms"], learned_decisions=reps[0]["learned_decisions"],
+                         fallbacks=reps[0]["fallbacks"], nslots=reps[0]["nslots"], nblocks=reps[0]["nblocks"],
+                         steps=reps[0]["steps"])
+    identical = bool((hashes["plain"] == hashes["learned"]).all())
+    line = {
+        "metric": "walk steps/s on 1 MI355X + H2D GB, learned vs plain block scheduler",
+        "value": res["learned"]["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["learned"]["ms_per_run"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (generated on the GPU, seed %d)" % args.seed,
+        "config": {"workload": cfg["workload"], "nverts": nverts, "narcs": narcs, "memory_bytes": mem,
+                   "walks_identical": identical, "schedulers": res,
+                   "learned_vs_plain_steps_per_s": res["learned"]["steps_per_s"] / res["plain"]["steps_per_s"],
+                   "learned_vs_plain_h2d": res["learned"]["h2d_gb"] / res["plain"]["h2d_gb"]
+                   if res["plain"]["h2d_gb"] else None, "graph_gen_s": gen_s, "parallelism": "1 GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +141,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
+    ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X and this engine runs one "
@@ -104,6 +158,11 @@ def main():
     narcs = csr.n
     print("[bench] graph %s: %d vertices, %d arcs, generated in %.2f s" % (args.config, nverts, narcs, gen_s),
           file=sys.stderr, flush=True)
+    if cfg.get("out_of_core"):
+        run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s)
+        beg.close()
+        csr.close()
+        return
     block_bytes = 1 << 34   # one block holds the whole graph (fully resident)
     blocks = E.Blocks(beg.a, csr.a, block_bytes)
     ctx.set_blocks(blocks)

@@ -1,0 +1,128 @@
+// Unit tests of the host layer that need no GPU: option parsing (api/cmdopts.hpp
+// semantics), the on-disk format (names, meta, block files, truncation), the plain
+// scheduler against the oracle's known answers, and slot placement.
+#include <cassert>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "cmdopts.hpp"
+#include "format.hpp"
+#include "schedule.hpp"
+
+using namespace gwh;
+
+#define CHECK(c)                                                             \
+    do {                                                                     \
+        if (!(c)) {                                                          \
+            std::fprintf(stderr, "%s:%d: CHECK failed: %s\n", __FILE__, __LINE__, #c); \
+            std::exit(1);                                                    \
+        }                                                                    \
+    } while (0)
+
+static void test_options(const std::string& dir)
+{
+    std::string cnf = dir + "/conf";
+    ::mkdir(cnf.c_str(), 0755);
+    FILE* f = std::fopen((cnf + "/randgraph.cnf").c_str(), "w");
+    std::fprintf(f, "# comment\nlength = 77\nq=3.5\n");
+    std::fclose(f);
+    setenv("RANDGRAPH_ROOT", dir.c_str(), 1);
+    const char* argv[] = {"Node2vec", "g.txt", "length", "40", "p", "0.5", "gpu", "length", "41", "--blocksize=256", "zero"};
+    Options o;
+    o.set_argc(11, argv);
+    CHECK(o.get_int("length", 30) == 41);          // last pair wins (backwards scan)
+    CHECK(o.get_float("p", 1.0f) == 0.5f);
+    CHECK(o.get_float("q", 2.0f) == 3.5f);         // config file when argv has no pair
+    CHECK(o.get_long("blocksize", 1024) == 256);   // --key=value
+    CHECK(o.get_int("memory", 4) == 4);            // default
+    CHECK(o.get_bool("gpu") && o.get_bool("zero") && !o.get_bool("cpu"));
+    // a name as the last token has no value: not a pair (the scan starts at argc-2)
+    const char* argv2[] = {"SOPR", "g.txt", "alpha"};
+    Options o2;
+    o2.set_argc(3, argv2);
+    CHECK(o2.get_float("alpha", 0.2f) == 0.2f);
+    CHECK(o2.get_bool("alpha"));
+}
+
+static void test_names()
+{
+    CHECK(remove_extension("/data/tw.txt") == "/data/tw");
+    CHECK(remove_extension("noext") == "noext");
+    CHECK(vert_blocks_name("/data/tw", 256ull << 20) == "/data/sowalker_256/tw_256MB.vert.blocks");
+    CHECK(edge_blocks_name("tw", 1ull << 30) == "./sowalker_1024/tw_1024MB.edge.blocks");
+    CHECK(beg_name("/d/g") == "/d/g.beg" && csr_name("/d/g") == "/d/g.csr" && meta_name("/d/g") == "/d/g.meta");
+}
+
+static void test_format(const std::string& dir)
+{
+    const std::string base = dir + "/g";
+    // giant vertex 0 (20 edges) then two small vertices, 32-byte blocks (8 edges, 4 verts)
+    std::vector<uint64_t> beg = {0, 20, 21, 22};
+    Meta m{3, 22, 0};
+    write_meta(base, m);
+    Meta r = read_meta(base);
+    CHECK(r.nvertices == 3 && r.nedges == 22 && r.min_vert == 0);
+    CHECK(file_size(meta_name(base)) == 16);
+    ::mkdir((dir + "/sowalker_0").c_str(), 0755);
+    write_block_files(base, beg.data(), beg.size(), 32);
+    auto vb = read_vector<uint32_t>(vert_blocks_name(base, 32));
+    auto eb = read_vector<uint64_t>(edge_blocks_name(base, 32));
+    CHECK((vb == std::vector<uint32_t>{0, 0, 1, 3}));
+    CHECK((eb == std::vector<uint64_t>{0, 0, 20, 22}));
+    auto t = make_block_table(vb, eb, 32);
+    CHECK(t.size() == 3 && t[0].nverts == 0 && t[1].nverts == 1 && t[1].nedges == 8 && t[2].nedges == 2);
+    // a truncated file is an error, not an assert
+    write_all(base + ".short", "abc", 3);
+    bool threw = false;
+    try {
+        read_vector<uint64_t>(base + ".short");
+    } catch (const IoError&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        read_meta(dir + "/missing");
+    } catch (const IoError&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+static void test_scheduler()
+{
+    // same known answers as tests/test_oracle_format.py
+    std::vector<uint64_t> W(9, 0);
+    W[1 * 3 + 0] = 5; W[0 * 3 + 1] = 2; W[2 * 3 + 2] = 6; W[2 * 3 + 0] = 1;
+    PlainScheduler s;
+    s.buckets = {0};
+    SchedInput in{W.data(), 3, 2, 0, nullptr, 0};
+    CHECK(s.schedule(in) && (s.buckets == std::vector<uint32_t>{0, 1}));
+    s.buckets = {0};
+    SchedInput in7{W.data(), 3, 2, 7, nullptr, 0};
+    CHECK(!s.schedule(in7) && (s.buckets == std::vector<uint32_t>{0}));
+    std::vector<uint64_t> W2(16, 0);
+    W2[2 * 4 + 0] = 3; W2[0 * 4 + 2] = 1; W2[2 * 4 + 1] = 10;
+    s.buckets = {0, 1};
+    SchedInput inf{W2.data(), 4, 2, 0, nullptr, 0};
+    CHECK(s.schedule(inf) && (s.buckets == std::vector<uint32_t>{2, 1}));
+    // slot placement: resident blocks stay, new ones take free then unwanted slots
+    SlotMap sm;
+    sm.reset(2, 4);
+    CHECK(sm.place(0, {0, 1}) == 0 && sm.place(1, {0, 1}) == 1 && sm.place(1, {0, 1}) == -1);
+    CHECK(sm.place(2, {2, 1}) == 0 && sm.blk_slot[0] == -1 && sm.blk_slot[2] == 0);
+    CHECK(sm.place(3, {2, 1}) == -2);
+}
+
+int main(int argc, char** argv)
+{
+    std::string dir = argc > 1 ? argv[1] : "/tmp";
+    test_options(dir);
+    test_names();
+    test_format(dir);
+    test_scheduler();
+    std::printf("host_unit: all passed\n");
+    return 0;
+}

@@ -66,7 +66,8 @@ def test_zero_copy_tail_and_sopr(gpu):
     n = len(beg) - 1
     b, c = pinned_graph(beg, csr)
     types = [dict(algo=E.SOPR, length=25, alpha=0.2, term=0.1)]
-    rep, h, v = E.run_out_of_core(b.a, c.a, types, 4, n * 2, wpv=2, memory_bytes=0, block_bytes=8192,
+    # 64 KB blocks hold 16384 edges: no vertex is truncated (see test_giant_vertex_truncation)
+    rep, h, v = E.run_out_of_core(b.a, c.a, types, 4, n * 2, wpv=2, memory_bytes=0, block_bytes=65536,
                                   zero_threshold=400, hashes=True, visits=True)
     o = oracle_hashes(beg, csr, types, 4, n * 2, 2)
     np.testing.assert_array_equal(h, o["hashes"])
@@ -129,3 +130,35 @@ def test_apps_end_to_end(gpu, tmp_path):
     r2 = subprocess.run([os.path.join(build, "Node2vec"), base + ".txt", "length", "5"], capture_output=True,
                         text=True, timeout=60, cwd=str(tmp_path))
     assert "Please select one walker from GOWalker and SOWalker!" in r2.stdout
+
+
+def effective_graph(beg, csr, blocks):
+    """The graph the engine walks: a single-vertex block longer than blocksize/4 edges is
+    cut to its first blocksize/4 targets (engine/cache.hpp:170-172)."""
+    deg = np.diff(beg.astype(np.int64))
+    for i in range(blocks.nblocks):
+        d = blocks.desc[i]
+        if d.nverts == 1:
+            deg[d.start_vert] = d.nedges
+    nb = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    out = np.concatenate([csr[int(beg[v]):int(beg[v]) + int(deg[v])] for v in range(len(deg))])
+    return nb, out.astype(np.uint32)
+
+
+def test_giant_vertex_truncation(gpu):
+    beg, csr = G.power_law_directed(20000, 200000, seed=5)
+    n = len(beg) - 1
+    bl = E.Blocks(beg, csr, 8192)
+    giants = [i for i in range(bl.nblocks) if bl.desc[i].nverts == 1 and
+              bl.ecut[i + 1] - bl.ecut[i] > 2048]
+    assert giants, "the test graph must have a vertex longer than a block"
+    eb, ec = effective_graph(beg, csr, bl)
+    b, c = pinned_graph(beg, csr)
+    types = [dict(algo=E.SOPR, length=25, alpha=0.2, term=0.1)]
+    rep, h, v = E.run_out_of_core(b.a, c.a, types, 4, n * 2, wpv=2, memory_bytes=0, block_bytes=8192,
+                                  zero_threshold=400, hashes=True, visits=True)
+    o = oracle_hashes(eb, ec, types, 4, n * 2, 2)
+    np.testing.assert_array_equal(h, o["hashes"])
+    np.testing.assert_array_equal(v, o["visits"])
+    b.close()
+    c.close()
