@@ -24,8 +24,9 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     "sopr_chung_lu": dict(
-        workload="SOPR, directed Chung-Lu 41.6M V / 1.47B arcs (gamma 2.2, max out-degree 3M), "
-                 "4 walkers/vertex, alpha 0.2, term 0.15, length cap 30, fully HBM-resident",
+        workload="SOPR, directed Chung-Lu 41.6M V / ~1.47B arcs (gamma 2.2, max out-degree ~3M, duplicate "
+                 "targets and self-loops removed), 4 walkers/vertex, alpha 0.2, term 0.15, length cap 30, "
+                 "fully HBM-resident",
         graph=dict(kind="chung_lu", nverts=41_600_000, nedges=1_470_000_000, gamma=2.2, max_degree=3_000_000),
         types=[dict(algo=1, length=30, alpha=0.2, term=0.15)], wpv=4),
     "node2vec_rmat26_ooc": dict(
@@ -140,6 +141,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=1, help="0: per-step lane loop, 1: flattened state machine")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
     args = ap.parse_args()
@@ -174,7 +176,7 @@ def main():
     nwalkers = nverts * cfg["wpv"]
     visits = cfg["types"][0]["algo"] == 1
     outputs = E.OUT_VISITS if visits else E.OUT_HASH
-    launch = dict(block=args.block, waves_per_cu=args.waves_per_cu)
+    launch = dict(block=args.block, waves_per_cu=args.waves_per_cu, variant=args.variant)
 
     def one_step():
         ctx.init_walkers(nwalkers, cfg["wpv"], outputs=outputs)
@@ -226,7 +228,8 @@ def main():
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "walk_kernel<true>", "kernel_ms": avg_kms, "algorithmic_bytes": abytes},
+                     "kernel": "walk_flat_kernel<true>" if args.variant == 1 else "walk_kernel<true>",
+                     "kernel_ms": avg_kms, "algorithmic_bytes": abytes},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

@@ -28,7 +28,8 @@ class GwBlockDesc(C.Structure):
 
 class GwWalkType(C.Structure):
     _fields_ = [("algo", C.c_uint32), ("length", C.c_uint32), ("p", C.c_double), ("q", C.c_double),
-                ("alpha", C.c_double), ("term", C.c_double)]
+                ("alpha", C.c_double), ("term", C.c_double), ("sopr_sampler", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class GwWalkParams(C.Structure):
@@ -89,7 +90,7 @@ SIGNATURES = {
     "gw_split_blocks": [P, C.c_uint64, C.c_uint64, P, P, C.c_uint64, u64p],
     "gw_gen_rmat": [P, C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
                     C.POINTER(P)],
-    "gw_gen_chung_lu": [P, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64, C.POINTER(P)],
+    "gw_gen_chung_lu": [P, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64, C.c_uint32, C.POINTER(P)],
     "gw_graph_size": [P, u64p, u64p],
     "gw_graph_read": [P, P, P, P],
     "gw_graph_free": [P, P],

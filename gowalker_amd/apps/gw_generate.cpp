@@ -3,7 +3,7 @@
 // in for the reference's text->CSR converter, which cannot run as shipped
 // (api/constants.hpp:5 makes EDGE_SIZE 0; preprocess/graph_converter.hpp:227-231).
 //   gw_generate <base> rmat scale 20 edges 16777216 undirected dedup [permute] seed 42 blocksize 1024
-//   gw_generate <base> chunglu nverts 41600000 edges 1470000000 gamma 2.2 maxdeg 3000000 seed 42
+//   gw_generate <base> chunglu nverts 41600000 edges 1470000000 gamma 2.2 maxdeg 3000000 seed 42 [multi]
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -37,7 +37,7 @@ int main(int argc, const char** argv)
                          o.get_float("b", 0.19f), o.get_float("c", 0.19f), seed, flags, &g);
     } else if (kind == "chunglu") {
         st = gw_gen_chung_lu(ctx, o.get_long("nverts", 1000000), o.get_long("edges", 10000000), o.get_float("gamma", 2.2f),
-                             o.get_long("maxdeg", 100000), seed, &g);
+                             o.get_long("maxdeg", 100000), seed, o.get_bool("multi") ? 0u : (uint32_t)GW_GEN_DEDUP, &g);
     } else {
         std::fprintf(stderr, "unknown generator %s\n", kind.c_str());
         gw_destroy(ctx);

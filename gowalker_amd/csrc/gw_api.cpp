@@ -492,6 +492,8 @@ gw_status gw_set_walk(gw_ctx* c, const gw_walk_params* wp)
             if (a >= 65536) return fail(c, GW_ERR_ARG, "alpha rounds to 1 in 16-bit fixed point");
             d.alpha_a = (uint32_t)a;
             d.alpha_b = 65536u - (uint32_t)a;
+            if (ty.sopr_sampler > 1) return fail(c, GW_ERR_ARG, "unknown SOPR sampler %u", ty.sopr_sampler);
+            d.sampler = ty.sopr_sampler;
         }
     }
     if (wp->ntypes == 1) c->dt[1] = c->dt[0];
@@ -616,13 +618,25 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
     gw_launch l = L ? *L : gw_launch{};
     if (l.block == 0) l.block = 256;
     if (l.block % 64 || l.block > 1024) return fail(c, GW_ERR_ARG, "block must be a multiple of 64 up to 1024");
+    if (l.variant > 1) return fail(c, GW_ERR_ARG, "unknown kernel variant %u", l.variant);
+    if (l.variant == 1 && l.block > 256) return fail(c, GW_ERR_ARG, "the flattened kernel runs at most 256 threads per workgroup");
 
     gw_status st;
     if ((st = push_block_table(c)) != GW_OK) return st;
     uint32_t nwork = 0;
     uint64_t left = 0;
     float fms = 0.f, kms = 0.f;
-    if ((st = run_filter(c, (l.flags & GW_RUN_BUCKETS) != 0, true, &nwork, &left, &fms)) != GW_OK) return st;
+    // with every block readable the whole walker array is the worklist (the kernel skips
+    // finished walkers), so no compaction pass is needed
+    bool all_res = true;
+    for (uint32_t b = 0; b < c->nblocks && all_res; ++b) all_res = c->hblk[b].resident != 0;
+    const bool identity = all_res && !(l.flags & GW_RUN_BUCKETS);
+    if (identity) {
+        HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
+        nwork = (uint32_t)c->nwalkers;
+    } else {
+        if ((st = run_filter(c, (l.flags & GW_RUN_BUCKETS) != 0, true, &nwork, &left, &fms)) != GW_OK) return st;
+    }
     HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
     if (nwork) {
         WalkArgs a{};
@@ -636,7 +650,7 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
         a.type1 = c->dt[1];
         a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta; a.bpair = c->d_bpair;
         a.hash = c->d_hash; a.paths = c->d_paths; a.path_stride = c->path_stride; a.visits = c->d_visits;
-        a.worklist = c->d_work;
+        a.worklist = identity ? nullptr : c->d_work;
         a.nwork = nwork;
         a.queue = c->d_misc + 1;
         a.counters = c->d_cnt;
@@ -644,11 +658,16 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
             HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
             a.block_steps = c->d_trace;
         }
+        // the flattened variant reads u32s through aligned 8-byte words, which is only
+        // safe inside padded HBM slots: zero-copy (host-mapped) blocks use variant 0
+        if (l.variant == 1)
+            for (uint32_t b = 0; b < c->nblocks; ++b)
+                if (c->hblk[b].resident && c->blk_slot[b] < 0) { l.variant = 0; break; }
         // persistent grid: as many workgroups as the CUs hold at once (occupancy), or
         // the caller's waves per CU
         const bool one_blk = c->nblocks == 1;
         uint64_t grid = l.waves_per_cu ? (uint64_t)c->ncu * l.waves_per_cu * 64 / l.block
-                                       : (uint64_t)c->ncu * walk_blocks_per_cu(one_blk, l.block);
+                                       : (uint64_t)c->ncu * walk_blocks_per_cu(one_blk, l.variant, l.block);
         const uint64_t need = ((uint64_t)nwork + l.block - 1) / l.block;
         if (grid > need) grid = need;
         if (grid == 0) grid = 1;
@@ -668,8 +687,8 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
     s.probes = c->h_cnt[2];
     s.mix_steps = c->h_cnt[3];
     s.walkers_ended = c->h_cnt[4];
-    s.walkers_run = nwork;
-    s.walkers_left = left - s.walkers_ended;
+    s.walkers_run = c->h_cnt[4] + c->h_cnt[5];
+    s.walkers_left = identity ? c->h_cnt[5] : left - s.walkers_ended;
     s.kernel_ms = kms;
     s.filter_ms = fms;
     c->tot.steps += s.steps;

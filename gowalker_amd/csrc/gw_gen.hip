@@ -210,6 +210,47 @@ __global__ void __launch_bounds__(256) cl_sortfix_kernel(uint64_t n, const uint6
     }
 }
 
+// duplicate / self-loop removal over sorted lists, one wavefront per vertex
+__device__ __forceinline__ bool cl_keep(const uint32_t* csr, uint64_t h, uint64_t k, uint64_t v)
+{
+    const uint32_t x = csr[h + k];
+    return x != (uint32_t)v && (k == 0 || x != csr[h + k - 1]);
+}
+
+__global__ void __launch_bounds__(256) cl_uniq_count_kernel(uint64_t n, const uint64_t* beg, const uint32_t* csr,
+                                                             uint32_t* deg)
+{
+    const uint32_t lane = __lane_id();
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t i = wave; i < n; i += nwaves) {
+        const uint64_t h = beg[i], d = beg[i + 1] - h;
+        uint64_t cnt = 0;
+        for (uint64_t k = lane; k < d; k += 64) cnt += cl_keep(csr, h, k, i);
+        cnt = wave_sum(cnt);
+        if (lane == 0) deg[i] = (uint32_t)cnt;
+    }
+}
+
+__global__ void __launch_bounds__(256) cl_uniq_write_kernel(uint64_t n, const uint64_t* beg, const uint32_t* csr,
+                                                             const uint64_t* beg2, uint32_t* csr2)
+{
+    const uint32_t lane = __lane_id();
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t i = wave; i < n; i += nwaves) {
+        const uint64_t h = beg[i], d = beg[i + 1] - h;
+        uint64_t out = beg2[i];
+        for (uint64_t k0 = 0; k0 < d; k0 += 64) {
+            const uint64_t k = k0 + lane;
+            const bool keep = k < d && cl_keep(csr, h, k, i);
+            const uint64_t m = __ballot(keep);
+            if (keep) csr2[out + lane_rank(m)] = csr[h + k];
+            out += __popcll(m);
+        }
+    }
+}
+
 __global__ void u32_to_u64_kernel(const uint32_t* in, uint64_t* out, uint64_t n)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -237,6 +278,8 @@ hipStream_t ctx_stream(gw_ctx* ctx);
 }
 
 using namespace gw;
+
+extern "C" gw_status gw_graph_free(gw_ctx* ctx, gw_graph* g);
 
 #define GCHK(call)                                                           \
     do {                                                                     \
@@ -408,8 +451,60 @@ done:
     return st;
 }
 
+static gw_status chung_lu_once(gw_ctx* ctx, uint64_t nverts, uint64_t nedges, double gamma, uint64_t max_degree,
+                               uint64_t seed, gw_graph** out);
+
+// With GW_GEN_DEDUP, duplicate targets and self-loops are removed; the edge count drawn
+// is raised once by the observed duplicate rate so that about `nedges` arcs remain.
 extern "C" gw_status gw_gen_chung_lu(gw_ctx* ctx, uint64_t nverts, uint64_t nedges, double gamma,
-                                     uint64_t max_degree, uint64_t seed, gw_graph** out)
+                                     uint64_t max_degree, uint64_t seed, uint32_t flags, gw_graph** out)
+{
+    if (!(flags & GW_GEN_DEDUP)) return chung_lu_once(ctx, nverts, nedges, gamma, max_degree, seed, out);
+    if (!ctx || !out) return gen_fail(ctx, GW_ERR_ARG, "null argument");
+    gw_status st = GW_OK;
+    hipStream_t s = ctx_stream(ctx);
+    uint64_t draw = nedges;
+    gw_graph* g = nullptr;
+    gw_graph* u = nullptr;
+    uint32_t* deg = nullptr;
+    for (int pass = 0; pass < 2; ++pass) {
+        if ((st = chung_lu_once(ctx, nverts, draw, gamma, max_degree, seed, &g)) != GW_OK) goto done;
+        GCHK(hipMalloc(&deg, nverts * 4));
+        hipLaunchKernelGGL(cl_uniq_count_kernel, dim3(grid_for(nverts * 64, 256, 1 << 20)), dim3(256), 0, s, nverts,
+                           g->beg, g->csr, deg);
+        GCHK(hipGetLastError());
+        u = new gw_graph();
+        GCHK(hipMalloc(&u->beg, (nverts + 1) * 8));
+        GCHK(degrees_to_offsets(deg, u->beg, nverts, s));
+        GCHK(hipMemcpy(&u->nedges, u->beg + nverts, 8, hipMemcpyDeviceToHost));
+        u->nverts = nverts;
+        GCHK(hipMalloc(&u->csr, u->nedges * 4 + 64));
+        hipLaunchKernelGGL(cl_uniq_write_kernel, dim3(grid_for(nverts * 64, 256, 1 << 20)), dim3(256), 0, s, nverts,
+                           g->beg, g->csr, u->beg, u->csr);
+        GCHK(hipGetLastError());
+        GCHK(hipStreamSynchronize(s));
+        (void)hipFree(deg);
+        deg = nullptr;
+        const uint64_t kept = u->nedges;
+        gw_graph_free(ctx, g);
+        g = nullptr;
+        if (pass == 1 || kept == 0 || kept >= nedges) break;
+        const double scale = (double)nedges / (double)kept;
+        draw = (uint64_t)((double)draw * scale);
+        gw_graph_free(ctx, u);
+        u = nullptr;
+    }
+    *out = u;
+    u = nullptr;
+done:
+    if (deg) (void)hipFree(deg);
+    if (g) gw_graph_free(ctx, g);
+    if (u) gw_graph_free(ctx, u);
+    return st;
+}
+
+static gw_status chung_lu_once(gw_ctx* ctx, uint64_t nverts, uint64_t nedges, double gamma, uint64_t max_degree,
+                               uint64_t seed, gw_graph** out)
 {
     gw_status st = GW_OK;
     if (!ctx || !out) return gen_fail(ctx, GW_ERR_ARG, "null argument");

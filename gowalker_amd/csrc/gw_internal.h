@@ -38,6 +38,8 @@ struct DevType {
     uint32_t algo;                     // 0 node2vec, 1 SOPR/SOSR
     uint32_t length;
     uint32_t alpha_a, alpha_b;         // A = round(alpha * 2^16), B = 2^16 - A
+    uint32_t sampler;                  // SOPR: 0 proposal, 1 envelope/mixture
+    uint32_t pad;
 };
 
 struct WalkArgs {
@@ -57,7 +59,7 @@ struct WalkArgs {
     uint32_t path_stride;
     uint32_t* visits;         // optional
     // work
-    const uint32_t* worklist; // walker ids; nullptr = identity
+    const uint32_t* worklist; // walker ids; nullptr = identity (finished walkers are skipped)
     uint32_t nwork;
     uint32_t* queue;          // work-fetch counter, zero at launch
     unsigned long long* counters;  // steps, trials, probes, mix, ended, left
@@ -99,10 +101,11 @@ hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s);
 hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid,
                        uint32_t block, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
-uint32_t walk_blocks_per_cu(bool one_block, uint32_t block);   // resident workgroups per CU
+uint32_t walk_blocks_per_cu(bool one_block, uint32_t variant, uint32_t block);   // resident workgroups per CU
 hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age,
                            float* feats, hipStream_t s);
 constexpr uint32_t kTraceLdsBlocks = 2048;   // per-workgroup LDS histogram size of the trace
 constexpr uint32_t kNumFeatures = 12;
+constexpr uint32_t kFilterLdsBuckets = 12288;  // bucket matrices up to 110 x 110 blocks count in LDS
 
 }  // namespace gw

@@ -132,6 +132,25 @@ __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint6
                                           const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
                                           uint32_t& x, Cnt& c)
 {
+    if (tp.sampler == 0) {
+        // proposal rejection: N(cur) with probability 1-a, else a prev-neighbour that is
+        // also a cur-neighbour (duplicate-free lists)
+        const uint64_t tb = (uint64_t)tp.alpha_b << 16;
+        for (uint32_t k = 0; k < kMaxTrials; ++k) {
+            const uint4 d = philox(w, hop, k, kTagStep, k0, k1);
+            ++c.trials;
+            if ((uint64_t)d.x < tb) {
+                x = nv[((uint64_t)d.y * dv) >> 32];
+                return true;
+            }
+            const uint32_t y = nu[((uint64_t)d.y * du) >> 32];
+            if (y != prev && member(nv, dv, y, c.probes)) {
+                x = y;
+                return true;
+            }
+        }
+        return false;
+    }
     const uint64_t A = tp.alpha_a, B = tp.alpha_b;
     const u128 num32 = (u128)(B * du) << 32;
     if (sopr_use_mixture(du, dv, tp.alpha_a, tp.alpha_b)) {
@@ -188,6 +207,35 @@ __device__ __forceinline__ Row load_row(const DevBlock& b, uint32_t v)
     return Row{ b.csr + h, t - h };
 }
 
+// Wave-local work queue: a wavefront takes kChunk walker indices with one atomic and
+// hands them to its lanes as they free up, so the shared counter sees one atomic per
+// kChunk walkers instead of one per walker.  `next`/`end` stay wavefront-uniform.
+constexpr uint32_t kChunk = 256;
+
+struct WaveQueue {
+    uint32_t next, end;
+};
+
+__device__ __forceinline__ uint32_t wave_take(WaveQueue& q, uint64_t mask, uint32_t* counter, uint32_t lane)
+{
+    const uint32_t cnt = (uint32_t)__popcll(mask);
+    const uint32_t rank = lane_rank(mask);
+    const uint32_t avail = q.end - q.next;
+    if (avail >= cnt) {
+        const uint32_t idx = q.next + rank;
+        q.next += cnt;
+        return idx;
+    }
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t nb = 0;
+    if ((int)lane == leader) nb = atomicAdd(counter, kChunk);
+    nb = __shfl(nb, leader, 64);
+    const uint32_t idx = rank < avail ? q.next + rank : nb + (rank - avail);
+    q.next = nb + (cnt - avail);
+    q.end = nb + kChunk;
+    return idx;
+}
+
 // Persistent step kernel.  Each lane owns one walker at a time and takes one whole step
 // per loop iteration; a lane whose walker ends or leaves the resident set takes the
 // next one from the work queue (one atomic per wavefront), so a wavefront never waits
@@ -211,22 +259,21 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a)
     uint64_t hash = 0;
     Row rv{nullptr, 0}, ru{nullptr, 0};
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    WaveQueue q{0, 0};
 
     for (;;) {
         const bool need = !have && !exhausted;
         const uint64_t mask = __ballot(need);
         if (mask) {
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(a.queue, (uint32_t)__popcll(mask));
-            base = __shfl(base, leader, 64);
+            const uint32_t idx = wave_take(q, mask, a.queue, lane);
             if (need) {
-                const uint32_t idx = base + lane_rank(mask);
-                if (idx < a.nwork) {
+                const uint32_t m = idx < a.nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
+                if (idx >= a.nwork) {
+                    exhausted = true;
+                } else if (((m >> kStatusShift) & 0xFu) == kActive) {
                     w = a.worklist ? a.worklist[idx] : idx;
                     cur = a.cur[w];
                     prev = a.prev[w];
-                    const uint32_t m = a.meta[w];
                     hop = m & kHopMask;
                     type = (m >> kTypeShift) & 1u;
                     const uint32_t pr = a.bpair[w];
@@ -241,12 +288,10 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a)
                         ru = load_row(a.blocks[bp], prev);
                     }
                     have = true;
-                } else {
-                    exhausted = true;
                 }
             }
         }
-        if (!__any(have)) break;
+        if (!__any(have || !exhausted)) break;
         if (!have) continue;
 
         const DevType& tp = type ? a.type1 : a.type0;
@@ -317,6 +362,434 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a)
     }
 }
 
+// ------------------------------------------------------------------------------------
+// Flattened step kernel (launch variant 1).  The same walk, restated as a per-lane state
+// machine in which every loop iteration performs exactly one memory round trip for every
+// busy lane: all lanes' next addresses are computed, one load instruction (two for an
+// adjacency row) goes out for the whole wavefront, and each lane consumes its value and
+// advances through compute-only states until it needs the next load.  A lane inside a
+// long membership search or SOPR intersection therefore no longer holds the other 63
+// lanes of its wavefront at the end of every step.  Trials, probes and draws are those of
+// the textbook loops above, so results and counters are bit-identical to variant 0.
+// Only used when every resident block lives in an HBM slot: a u32 is read through the
+// 8-byte-aligned word that holds it, which stays inside a slot's padded buffer.
+// ------------------------------------------------------------------------------------
+enum : uint32_t {
+    PH_IDLE = 0,
+    // load states
+    PH_ROW, PH_CAND, PH_PROBE, PH_MIX_U, PH_MIX_LB, PH_MIX_EQ, PH_MIX_UB, PH_PICK, PH_PROP_U, PH_PPROBE,
+    // compute-only states
+    PH_STEP, PH_TRIAL, PH_PTRIAL, PH_MIX_NEXT, PH_AFTER_LB, PH_AFTER_UB, PH_MOVE, PH_END
+};
+
+struct FlatLane {
+    uint32_t phase, w, cur, prev, hop, type, bc, bp, status;
+    bool release;
+    uint64_t hash;
+    const uint32_t* nv;
+    const uint32_t* nu;
+    uint64_t dv, du;
+    // trial / search state
+    uint32_t k, x, y, last;
+    uint4 d;
+    uint64_t ra;
+    int64_t lo, hi, mid;
+    uint64_t i, base, lb, cnt, sel, c, pos;
+    bool have_last, select;
+    // pending load
+    const void* addr;
+};
+
+__device__ __forceinline__ uint32_t word32(uint64_t v, const void* exact)
+{
+    return ((uintptr_t)exact & 4) ? (uint32_t)(v >> 32) : (uint32_t)v;
+}
+
+template <bool ONE>
+__device__ __forceinline__ void flat_advance(FlatLane& L, const WalkArgs& a, const DevBlock& one, Cnt& c,
+                                             uint32_t* trace, bool lds_trace)
+{
+    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    for (;;) {
+        const DevType& tp = L.type ? a.type1 : a.type0;
+        switch (L.phase) {
+        case PH_STEP: {
+            if (tp.t_term) {
+                const uint4 t = philox(L.w, L.hop, 0, kTagTerm, k0, k1);
+                if ((uint64_t)t.x < tp.t_term) { L.status = kDoneTerm; L.phase = PH_END; break; }
+            }
+            if (L.dv == 0) { L.status = kDoneDeadEnd; L.phase = PH_END; break; }
+            if (tp.algo == 1 && tp.sampler == 0) {
+                L.k = 0;
+                L.phase = PH_PTRIAL;
+                break;
+            }
+            if (tp.algo == 1 && sopr_use_mixture(L.du, L.dv, tp.alpha_a, tp.alpha_b)) {
+                ++c.mix;
+                ++c.trials;
+                L.d = philox(L.w, L.hop, 0, kTagStep, k0, k1);
+                L.select = false;
+                L.i = 0; L.base = 0; L.c = 0; L.have_last = false;
+                L.phase = PH_MIX_NEXT;
+            } else {
+                L.k = 0;
+                L.phase = PH_TRIAL;
+            }
+            break;
+        }
+        case PH_TRIAL: {
+            if (L.k >= kMaxTrials) { L.status = kDoneTrialCap; L.phase = PH_END; break; }
+            if (!(L.k & 1)) L.d = philox(L.w, L.hop, L.k >> 1, kTagStep, k0, k1);
+            const uint32_t rp = (L.k & 1) ? L.d.z : L.d.x;
+            L.ra = (L.k & 1) ? L.d.w : L.d.y;
+            ++c.trials;
+            L.addr = L.nv + (((uint64_t)rp * L.dv) >> 32);
+            L.phase = PH_CAND;
+            return;
+        }
+        case PH_PTRIAL: {
+            if (L.k >= kMaxTrials) { L.status = kDoneTrialCap; L.phase = PH_END; break; }
+            L.d = philox(L.w, L.hop, L.k, kTagStep, k0, k1);
+            ++c.trials;
+            if ((uint64_t)L.d.x < ((uint64_t)tp.alpha_b << 16)) {
+                L.addr = L.nv + (((uint64_t)L.d.y * L.dv) >> 32);
+                L.phase = PH_PICK;
+            } else {
+                L.addr = L.nu + (((uint64_t)L.d.y * L.du) >> 32);
+                L.phase = PH_PROP_U;
+            }
+            return;
+        }
+        case PH_MIX_NEXT: {
+            if (L.i < L.du && L.base < L.dv) {
+                L.addr = L.nu + L.i;
+                ++c.probes;
+                L.phase = PH_MIX_U;
+                return;
+            }
+            if (!L.select) {
+                L.cnt = L.c;
+                const uint64_t A = tp.alpha_a, B = tp.alpha_b;
+                if (L.cnt == 0 || (u128)L.d.x * (B * L.du + A * L.cnt) < ((u128)(B * L.du) << 32)) {
+                    L.addr = L.nv + (((uint64_t)L.d.y * L.dv) >> 32);
+                    L.phase = PH_PICK;
+                    return;
+                }
+                L.sel = ((uint64_t)L.d.z * L.cnt) >> 32;
+                L.select = true;
+                L.i = 0; L.base = 0; L.c = 0; L.have_last = false;
+                break;
+            }
+            // unreachable: the selection pass always finds sel < cnt
+            L.status = kDoneTrialCap;
+            L.phase = PH_END;
+            break;
+        }
+        case PH_AFTER_LB: {
+            if (L.lb < L.dv) {
+                L.addr = L.nv + L.lb;
+                ++c.probes;
+                L.phase = PH_MIX_EQ;
+                return;
+            }
+            L.base = L.lb;
+            ++L.i;
+            L.phase = PH_MIX_NEXT;
+            break;
+        }
+        case PH_AFTER_UB: {
+            const uint64_t ub = (uint64_t)L.lo, cv = ub - L.lb;
+            if (L.select && L.sel < L.c + cv) {
+                L.addr = L.nv + L.lb + (L.sel - L.c);
+                L.phase = PH_PICK;
+                return;
+            }
+            L.c += cv;
+            L.base = ub;
+            ++L.i;
+            L.phase = PH_MIX_NEXT;
+            break;
+        }
+        case PH_MOVE: {
+            if (a.block_steps) {
+                if (lds_trace) atomicAdd(trace + L.bc, 1u);
+                else atomicAdd(a.block_steps + L.bc, 1ull);
+            }
+            L.prev = L.cur;
+            L.cur = L.x;
+            ++L.hop;
+            ++c.steps;
+            L.hash = (L.hash ^ L.x) * kFnvPrime;
+            if (a.visits) atomicAdd(a.visits + L.x, 1u);
+            if (a.paths) a.paths[(uint64_t)L.w * a.path_stride + L.hop] = L.x;
+            L.nu = L.nv;
+            L.du = L.dv;
+            L.bp = L.bc;
+            if (L.hop >= tp.length) { L.status = kDoneLength; L.phase = PH_END; break; }
+            const DevBlock* nb = &one;
+            if (!ONE) {
+                const DevBlock& cb = a.blocks[L.bc];
+                if (L.cur - cb.start_vert >= cb.nverts) L.bc = find_block(a.vcut, a.nblocks, L.cur);
+                nb = &a.blocks[L.bc];
+                if (!nb->resident) { L.release = true; L.phase = PH_END; break; }
+            }
+            L.addr = nb->beg + (L.cur - nb->start_vert);
+            L.phase = PH_ROW;
+            return;
+        }
+        case PH_END: {
+            a.cur[L.w] = L.cur;
+            a.prev[L.w] = L.prev;
+            a.meta[L.w] = L.hop | (L.status << kStatusShift) | (L.type << kTypeShift);
+            a.bpair[L.w] = L.bc | (L.bp << 16);
+            if (a.hash) a.hash[L.w] = L.hash;
+            if (L.status != kActive) ++c.ended; else ++c.left;
+            L.phase = PH_IDLE;
+            return;
+        }
+        default:
+            return;   // a load state or idle
+        }
+    }
+}
+
+template <bool ONE>
+__device__ __forceinline__ void flat_consume(FlatLane& L, uint64_t v0, uint64_t v1, const WalkArgs& a,
+                                             const DevBlock& one, Cnt& c)
+{
+    const DevType& tp = L.type ? a.type1 : a.type0;
+    switch (L.phase) {
+    case PH_ROW: {
+        const DevBlock& b = ONE ? one : a.blocks[L.bc];
+        uint64_t t = v1 - b.start_edge, h = v0 - b.start_edge;
+        if (t > b.nedges) t = b.nedges;
+        if (h > t) h = t;
+        L.nv = b.csr + h;
+        L.dv = t - h;
+        L.phase = PH_STEP;
+        break;
+    }
+    case PH_CAND: {
+        L.x = word32(v0, L.addr);
+        if (tp.algo == 0) {
+            if (L.ra < tp.t_min) { L.phase = PH_MOVE; break; }
+            if (L.x == L.prev) {
+                if (L.ra < tp.t_p) { L.phase = PH_MOVE; break; }
+                ++L.k;
+                L.phase = PH_TRIAL;
+                break;
+            }
+        } else {
+            const uint64_t A = tp.alpha_a, B = tp.alpha_b;
+            if ((u128)L.ra * (B * L.du + A * L.dv) < ((u128)(B * L.du) << 32)) { L.phase = PH_MOVE; break; }
+            if (L.x == L.prev) { ++L.k; L.phase = PH_TRIAL; break; }
+        }
+        L.lo = 0;
+        L.hi = (int64_t)L.du - 1;
+        if (L.lo > L.hi) {
+            // empty prev list: not a member
+            const bool acc = tp.algo == 0 ? L.ra < tp.t_q : false;
+            if (acc) L.phase = PH_MOVE; else { ++L.k; L.phase = PH_TRIAL; }
+            break;
+        }
+        L.mid = L.lo + ((L.hi - L.lo) >> 1);
+        L.addr = L.nu + L.mid;
+        ++c.probes;
+        L.phase = PH_PROBE;
+        return;
+    }
+    case PH_PROBE: {
+        const uint32_t m = word32(v0, L.addr);
+        bool done = false, found = false;
+        if (m == L.x) { done = true; found = true; }
+        else {
+            if (m < L.x) L.lo = L.mid + 1; else L.hi = L.mid - 1;
+            if (L.lo > L.hi) done = true;
+        }
+        if (!done) {
+            L.mid = L.lo + ((L.hi - L.lo) >> 1);
+            L.addr = L.nu + L.mid;
+            ++c.probes;
+            return;
+        }
+        bool acc;
+        if (tp.algo == 0) acc = L.ra < (found ? tp.t_one : tp.t_q);
+        else acc = found;
+        if (acc) L.phase = PH_MOVE; else { ++L.k; L.phase = PH_TRIAL; }
+        break;
+    }
+    case PH_MIX_U: {
+        const uint32_t y = word32(v0, L.addr);
+        if ((L.have_last && y == L.last) || y == L.prev) {
+            L.last = y; L.have_last = true; ++L.i;
+            L.phase = PH_MIX_NEXT;
+            break;
+        }
+        L.last = y; L.have_last = true; L.y = y;
+        L.lo = (int64_t)L.base;
+        L.hi = (int64_t)L.dv;
+        if (L.lo < L.hi) {
+            L.mid = L.lo + ((L.hi - L.lo) >> 1);
+            L.addr = L.nv + L.mid;
+            ++c.probes;
+            L.phase = PH_MIX_LB;
+            return;
+        }
+        L.lb = (uint64_t)L.lo;
+        L.phase = PH_AFTER_LB;
+        break;
+    }
+    case PH_MIX_LB: {
+        const uint32_t m = word32(v0, L.addr);
+        if (m < L.y) L.lo = L.mid + 1; else L.hi = L.mid;
+        if (L.lo < L.hi) {
+            L.mid = L.lo + ((L.hi - L.lo) >> 1);
+            L.addr = L.nv + L.mid;
+            ++c.probes;
+            return;
+        }
+        L.lb = (uint64_t)L.lo;
+        L.phase = PH_AFTER_LB;
+        break;
+    }
+    case PH_MIX_EQ: {
+        const uint32_t m = word32(v0, L.addr);
+        if (m == L.y) {
+            L.lo = (int64_t)L.lb;
+            L.hi = (int64_t)L.dv;
+            L.mid = L.lo + ((L.hi - L.lo) >> 1);
+            L.addr = L.nv + L.mid;
+            ++c.probes;
+            L.phase = PH_MIX_UB;
+            return;
+        }
+        L.base = L.lb;
+        ++L.i;
+        L.phase = PH_MIX_NEXT;
+        break;
+    }
+    case PH_MIX_UB: {
+        const uint32_t m = word32(v0, L.addr);
+        if (m <= L.y) L.lo = L.mid + 1; else L.hi = L.mid;
+        if (L.lo < L.hi) {
+            L.mid = L.lo + ((L.hi - L.lo) >> 1);
+            L.addr = L.nv + L.mid;
+            ++c.probes;
+            return;
+        }
+        L.phase = PH_AFTER_UB;
+        break;
+    }
+    case PH_PICK: {
+        L.x = word32(v0, L.addr);
+        L.phase = PH_MOVE;
+        break;
+    }
+    case PH_PROP_U: {
+        L.y = word32(v0, L.addr);
+        if (L.y == L.prev) { ++L.k; L.phase = PH_PTRIAL; break; }
+        L.lo = 0;
+        L.hi = (int64_t)L.dv - 1;
+        L.mid = L.lo + ((L.hi - L.lo) >> 1);
+        L.addr = L.nv + L.mid;
+        ++c.probes;
+        L.phase = PH_PPROBE;
+        return;
+    }
+    case PH_PPROBE: {
+        const uint32_t m = word32(v0, L.addr);
+        if (m == L.y) { L.x = L.y; L.phase = PH_MOVE; break; }
+        if (m < L.y) L.lo = L.mid + 1; else L.hi = L.mid - 1;
+        if (L.lo <= L.hi) {
+            L.mid = L.lo + ((L.hi - L.lo) >> 1);
+            L.addr = L.nv + L.mid;
+            ++c.probes;
+            return;
+        }
+        ++L.k;
+        L.phase = PH_PTRIAL;
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+template <bool ONE>
+__global__ void __launch_bounds__(256) walk_flat_kernel(WalkArgs a)
+{
+    __shared__ uint32_t trace[kTraceLdsBlocks];
+    const bool lds_trace = a.block_steps && a.nblocks <= kTraceLdsBlocks;
+    if (a.block_steps) {
+        for (uint32_t i = threadIdx.x; i < kTraceLdsBlocks; i += blockDim.x) trace[i] = 0;
+        __syncthreads();
+    }
+    Cnt c = {0, 0, 0, 0, 0, 0};
+    const uint32_t lane = __lane_id();
+    DevBlock one;
+    if (ONE) one = a.blocks[0];
+    FlatLane L;
+    L.phase = PH_IDLE;
+    L.release = false;
+    bool exhausted = false;
+    WaveQueue q{0, 0};
+    for (;;) {
+        const bool need = L.phase == PH_IDLE && !exhausted;
+        const uint64_t mask = __ballot(need);
+        if (mask) {
+            const uint32_t idx = wave_take(q, mask, a.queue, lane);
+            if (need) {
+                const uint32_t m = idx < a.nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
+                if (idx >= a.nwork) {
+                    exhausted = true;
+                } else if (((m >> kStatusShift) & 0xFu) == kActive) {
+                    L.w = a.worklist ? a.worklist[idx] : idx;
+                    L.cur = a.cur[L.w];
+                    L.prev = a.prev[L.w];
+                    L.hop = m & kHopMask;
+                    L.type = (m >> kTypeShift) & 1u;
+                    const uint32_t pr = a.bpair[L.w];
+                    L.bc = pr & 0xFFFFu;
+                    L.bp = pr >> 16;
+                    L.hash = a.hash ? a.hash[L.w] : 0;
+                    const Row rv = ONE ? load_row(one, L.cur) : load_row(a.blocks[L.bc], L.cur);
+                    const Row ru = ONE ? load_row(one, L.prev) : load_row(a.blocks[L.bp], L.prev);
+                    L.nv = rv.adj; L.dv = rv.deg; L.nu = ru.adj; L.du = ru.deg;
+                    L.status = kActive;
+                    L.release = false;
+                    L.phase = PH_STEP;
+                    flat_advance<ONE>(L, a, one, c, trace, lds_trace);
+                }
+            }
+        }
+        const bool busy = L.phase != PH_IDLE;
+        if (!__any(busy || !exhausted)) break;
+        if (!busy) continue;
+        // one round trip: an 8-byte aligned word (adjacency rows take both beg words)
+        const uint64_t* p0 = (const uint64_t*)((uintptr_t)L.addr & ~(uintptr_t)7);
+        const uint64_t v0 = *p0;
+        uint64_t v1 = 0;
+        if (L.phase == PH_ROW) v1 = *((const uint64_t*)L.addr + 1);
+        flat_consume<ONE>(L, v0, v1, a, one, c);
+        flat_advance<ONE>(L, a, one, c, trace, lds_trace);
+    }
+    if (lds_trace) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < a.nblocks; i += blockDim.x)
+            if (trace[i]) atomicAdd(a.block_steps + i, (unsigned long long)trace[i]);
+    }
+    const uint64_t s0 = wave_sum(c.steps), s1 = wave_sum(c.trials), s2 = wave_sum(c.probes);
+    const uint64_t s3 = wave_sum(c.mix), s4 = wave_sum(c.ended), s5 = wave_sum(c.left);
+    if (lane == 0) {
+        atomicAdd(a.counters + 0, (unsigned long long)s0);
+        atomicAdd(a.counters + 1, (unsigned long long)s1);
+        atomicAdd(a.counters + 2, (unsigned long long)s2);
+        atomicAdd(a.counters + 3, (unsigned long long)s3);
+        atomicAdd(a.counters + 4, (unsigned long long)s4);
+        atomicAdd(a.counters + 5, (unsigned long long)s5);
+    }
+}
+
 // Walker generation (engine/walk.hpp:32-49): walker w at sources[w] or w / wpv, with
 // source = previous = current and hop 0; type w % 2 in a mixed batch.
 __global__ void __launch_bounds__(256) init_kernel(InitArgs a)
@@ -345,6 +818,14 @@ __global__ void __launch_bounds__(256) init_kernel(InitArgs a)
 // all unfinished walkers (the counts prefix() scans in gpuwalk.hpp:321-338).
 __global__ void __launch_bounds__(256) filter_kernel(FilterArgs a)
 {
+    // bucket counts go through an LDS histogram when the matrix fits (dynamic LDS)
+    extern __shared__ uint32_t hist[];
+    const uint32_t nbb = a.nblocks * a.nblocks;
+    const bool lds_hist = a.bucket && nbb <= kFilterLdsBuckets;
+    if (lds_hist) {
+        for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+    }
     const uint32_t lane = __lane_id();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint64_t left = 0;
@@ -358,7 +839,8 @@ __global__ void __launch_bounds__(256) filter_kernel(FilterArgs a)
                 const uint32_t pr = a.bpair[w];
                 const uint32_t bc = pr & 0xFFFFu, bp = pr >> 16;
                 take = a.blocks[bc].resident && a.blocks[bp].resident;
-                if (a.bucket) atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
+                if (lds_hist) atomicAdd(hist + bp * a.nblocks + bc, 1u);
+                else if (a.bucket) atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
             }
         }
         const uint64_t mask = __ballot(take);
@@ -372,6 +854,11 @@ __global__ void __launch_bounds__(256) filter_kernel(FilterArgs a)
     }
     left = wave_sum(left);
     if (lane == 0 && left) atomicAdd(a.left, (unsigned long long)left);
+    if (lds_hist) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x)
+            if (hist[i]) atomicAdd(a.bucket + i, hist[i]);
+    }
 }
 
 // Per-block features of the learned scheduler, one thread per block b, from the bucket
@@ -445,25 +932,36 @@ hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s)
     if (a.nwalkers == 0) return hipSuccess;
     uint64_t need = (a.nwalkers + 255) / 256;
     if (grid == 0 || grid > need) grid = (uint32_t)(need < 65536 ? need : 65536);
-    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), 0, s, a);
+    const uint32_t nbb = a.nblocks * a.nblocks;
+    const size_t lds = (a.bucket && nbb <= kFilterLdsBuckets) ? nbb * 4 : 0;
+    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid,
                        uint32_t block, hipStream_t s)
 {
-    (void)variant;
     if (a.nwork == 0) return hipSuccess;
-    if (one_block) hipLaunchKernelGGL(walk_kernel<true>, dim3(grid), dim3(block), 0, s, a);
-    else hipLaunchKernelGGL(walk_kernel<false>, dim3(grid), dim3(block), 0, s, a);
+    if (variant == 1) {
+        if (one_block) hipLaunchKernelGGL(walk_flat_kernel<true>, dim3(grid), dim3(block), 0, s, a);
+        else hipLaunchKernelGGL(walk_flat_kernel<false>, dim3(grid), dim3(block), 0, s, a);
+    } else {
+        if (one_block) hipLaunchKernelGGL(walk_kernel<true>, dim3(grid), dim3(block), 0, s, a);
+        else hipLaunchKernelGGL(walk_kernel<false>, dim3(grid), dim3(block), 0, s, a);
+    }
     return hipGetLastError();
 }
 
-uint32_t walk_blocks_per_cu(bool one_block, uint32_t block)
+uint32_t walk_blocks_per_cu(bool one_block, uint32_t variant, uint32_t block)
 {
     int n = 0;
-    hipError_t e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true>, (int)block, 0)
-                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false>, (int)block, 0);
+    hipError_t e;
+    if (variant == 1)
+        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<true>, (int)block, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<false>, (int)block, 0);
+    else
+        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true>, (int)block, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false>, (int)block, 0);
     return (e == hipSuccess && n > 0) ? (uint32_t)n : 1u;
 }
 

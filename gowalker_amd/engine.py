@@ -72,6 +72,7 @@ def walk_params(types, seed=42):
         wt.q = t.get("q", 1.0)
         wt.alpha = t.get("alpha", 0.2)
         wt.term = t.get("term", 0.0)
+        wt.sopr_sampler = t.get("sampler", 0)
     return wp
 
 
@@ -244,10 +245,12 @@ def gen_rmat(ctx, scale, nedges, a=0.57, b=0.19, c=0.19, seed=42, undirected=Tru
         lib().gw_graph_free(ctx.h, g)
 
 
-def gen_chung_lu(ctx, nverts, nedges, gamma=2.2, max_degree=3000000, seed=42, pinned=False):
+def gen_chung_lu(ctx, nverts, nedges, gamma=2.2, max_degree=3000000, seed=42, dedup=True, pinned=False):
     """Directed Chung-Lu power-law graph generated on the GPU, as host CSR arrays."""
+    from ._abi import GW_GEN_DEDUP
     g = C.c_void_p()
-    check(lib().gw_gen_chung_lu(ctx.h, nverts, nedges, gamma, max_degree, seed, C.byref(g)), ctx.h)
+    check(lib().gw_gen_chung_lu(ctx.h, nverts, nedges, gamma, max_degree, seed, GW_GEN_DEDUP if dedup else 0,
+                                C.byref(g)), ctx.h)
     try:
         return _read_graph(ctx, g, pinned)
     finally:

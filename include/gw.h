@@ -66,7 +66,12 @@ typedef struct {
     uint32_t algo;           /* GW_NODE2VEC or GW_SOPR */
     uint32_t length;         /* hop cap, the `length` option */
     double p, q, alpha, term;
+    uint32_t sopr_sampler;   /* GW_SOPR_PROPOSAL (default; duplicate-free adjacency lists) or
+                                GW_SOPR_ENVELOPE (exact on any lists, slower on hubs) */
+    uint32_t reserved;
 } gw_walk_type;
+
+enum { GW_SOPR_PROPOSAL = 0, GW_SOPR_ENVELOPE = 1 };
 
 typedef struct {
     uint64_t seed;           /* Philox key */
@@ -225,10 +230,11 @@ enum { GW_GEN_UNDIRECTED = 1u << 0, GW_GEN_PERMUTE = 1u << 1, GW_GEN_DEDUP = 1u 
 /* RMAT with 2^scale ids and nedges generated edges (self-loops dropped). */
 gw_status gw_gen_rmat(gw_ctx* ctx, uint32_t scale, uint64_t nedges, double a, double b, double c,
                       uint64_t seed, uint32_t flags, gw_graph** out);
-/* Directed Chung-Lu power law: out-degree exponent gamma, vertex 0 has max_degree
- * out-edges, targets drawn by weight (multi-edges kept, lists sorted). */
+/* Directed Chung-Lu power law: out-degree exponent gamma, vertex 0 draws max_degree
+ * out-edges, targets drawn by weight, lists sorted.  GW_GEN_DEDUP removes duplicate
+ * targets and self-loops (drawing ~nedges/(1 - duplicate rate) so ~nedges remain). */
 gw_status gw_gen_chung_lu(gw_ctx* ctx, uint64_t nverts, uint64_t nedges, double gamma,
-                          uint64_t max_degree, uint64_t seed, gw_graph** out);
+                          uint64_t max_degree, uint64_t seed, uint32_t flags, gw_graph** out);
 gw_status gw_graph_size(const gw_graph* g, uint64_t* nverts, uint64_t* nedges);
 /* Copies the CSR to host memory (beg: nverts+1, csr: nedges). */
 gw_status gw_graph_read(gw_ctx* ctx, const gw_graph* g, uint64_t* beg, uint32_t* csr);

@@ -40,6 +40,12 @@
  *     take N(cur)[(d.y*dv)>>32], else take the k-th matching position, k = (d.z*c)>>32.
  *     Mode: mixture iff du*(4*L(dv)+2) < E*(1+L(du)) where L(n) = bit length of n and
  *     E = ceil((B*du + A*dv) / (B*du)) bounds the rejection trial count.
+ *     (Sampler 1.)  Sampler 0, the default, is a proposal rejection that needs
+ *     duplicate-free adjacency lists: attempt k draws d = philox(seed, (w, hop, k, 0));
+ *     if d.x < B << 16 (probability 1-a) take N(cur)[(d.y*dv)>>32]; otherwise
+ *     y = N(prev)[(d.y*du)>>32] is accepted iff y != prev and y is in N(cur) (textbook
+ *     search), else the attempt is retried.  Every accepted x has mass
+ *     (1-a)/dv + a/du [x in N(prev), x != prev], i.e. the same law, at ~1/(1-a) attempts.
  */
 #include "gw_oracle.h"
 #include <string.h>
@@ -278,6 +284,23 @@ static void walk_impl(const orc_graph* g, const orc_walkcfg* cfg, uint32_t wid, 
                 else if (member(csr + u_head, du, x, &probes)) t = tc->t_one;
                 else t = tc->t_q;
                 if (ra < t) { ok = 1; break; }
+            }
+        } else if (tc->sopr_sampler == 0) {
+            const uint64_t tb = (uint64_t)tc->alpha_b << 16;
+            for (uint32_t k = 0; k < ORC_MAX_TRIALS; k++) {
+                draw(cfg->seed, wid, hop, k, 0, d);
+                trials++;
+                if ((uint64_t)d[0] < tb) {
+                    x = csr[v_head + (((uint64_t)d[1] * dv) >> 32)];
+                    ok = 1;
+                    break;
+                }
+                uint32_t y = csr[u_head + (((uint64_t)d[1] * du) >> 32)];
+                if (y != prev && member(csr + v_head, dv, y, &probes)) {
+                    x = y;
+                    ok = 1;
+                    break;
+                }
             }
         } else {
             uint64_t A = tc->alpha_a, B = tc->alpha_b;

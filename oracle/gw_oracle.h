@@ -62,6 +62,8 @@ typedef struct {
     uint64_t t_p, t_one, t_q, t_min;  /* node2vec integer thresholds, each in [0, 2^32] */
     uint32_t alpha_a, alpha_b;        /* SOPR fixed-point alpha: A = round(alpha*2^16), B = 2^16-A */
     uint64_t t_term;        /* geometric termination threshold floor(term*2^32); 0 = none */
+    uint32_t sopr_sampler;  /* 0: proposal (duplicate-free lists), 1: envelope/mixture (any lists) */
+    uint32_t reserved;
 } orc_typecfg;
 
 typedef struct {

@@ -20,7 +20,7 @@ class Graph(C.Structure):
 class TypeCfg(C.Structure):
     _fields_ = [("algo", C.c_uint32), ("length", C.c_uint32), ("t_p", C.c_uint64), ("t_one", C.c_uint64),
                 ("t_q", C.c_uint64), ("t_min", C.c_uint64), ("alpha_a", C.c_uint32), ("alpha_b", C.c_uint32),
-                ("t_term", C.c_uint64)]
+                ("t_term", C.c_uint64), ("sopr_sampler", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class WalkCfg(C.Structure):
@@ -105,10 +105,11 @@ def make_graph(beg, csr):
     return g
 
 
-def type_cfg(algo, length, p=1.0, q=1.0, alpha=0.2, term=0.0):
+def type_cfg(algo, length, p=1.0, q=1.0, alpha=0.2, term=0.0, sampler=0):
     t = TypeCfg()
     t.algo = algo
     t.length = length
+    t.sopr_sampler = sampler
     if algo == N2V:
         lib().orc_n2v_thresholds(p, q, C.byref(t))
     else:

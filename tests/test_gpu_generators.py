@@ -41,7 +41,7 @@ def test_rmat_multi_part_sort(gpu):
 def test_chung_lu_invariants(gpu):
     n, m, gamma, dmax = 200_000, 3_000_000, 2.2, 40_000
     with E.Context() as ctx:
-        beg, csr = E.gen_chung_lu(ctx, n, m, gamma, dmax, seed=42)
+        beg, csr = E.gen_chung_lu(ctx, n, m, gamma, dmax, seed=42, dedup=False)
     d = np.diff(beg.astype(np.int64))
     assert beg[-1] == m == len(csr)
     assert abs(d[0] - dmax) <= 1 and d.argmax() == 0
@@ -55,3 +55,15 @@ def test_chung_lu_invariants(gpu):
     top = int(n * 0.01)
     indeg = np.bincount(csr, minlength=n)
     assert math.isclose(indeg[:top].sum() / m, d[:top].sum() / m, rel_tol=0.05)
+
+
+def test_chung_lu_dedup(gpu):
+    n, m = 200_000, 3_000_000
+    with E.Context() as ctx:
+        beg, csr = E.gen_chung_lu(ctx, n, m, 2.2, 40_000, seed=42, dedup=True)
+    d = np.diff(beg.astype(np.int64))
+    assert abs(int(beg[-1]) - m) < 0.03 * m
+    src = np.repeat(np.arange(n, dtype=np.int64), d)
+    same = src[1:] == src[:-1]
+    assert (csr[1:][same] > csr[:-1][same]).all()     # strictly increasing: no duplicates
+    assert not (csr.astype(np.int64) == src).any()    # no self-loops

@@ -17,7 +17,8 @@ def oracle_types(types):
         if t.get("algo", NODE2VEC) == NODE2VEC:
             ts.append(O.type_cfg(O.N2V, t["length"], p=t["p"], q=t["q"], term=t.get("term", 0.0)))
         else:
-            ts.append(O.type_cfg(O.SOPR, t["length"], alpha=t.get("alpha", 0.2), term=t.get("term", 0.0)))
+            ts.append(O.type_cfg(O.SOPR, t["length"], alpha=t.get("alpha", 0.2), term=t.get("term", 0.0),
+                                 sampler=t.get("sampler", 0)))
     return ts
 
 
@@ -95,30 +96,54 @@ def test_n2v_multi_block_resident(gpu):
     assert o["totals"]["probes"] > 0
 
 
-def test_sopr_mixture_and_rejection(gpu):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_sopr_envelope_sampler(gpu, variant):
+    # multigraph lists; the envelope sampler switches between rejection and the mixture
     beg, csr = G.power_law_directed(3000, 40000, seed=2)
-    g, o = check_case(beg, csr, [dict(algo=SOPR, length=30, alpha=0.2, term=0.15)], 42, 6000, wpv=2)
+    g, o = check_case(beg, csr, [dict(algo=SOPR, length=30, alpha=0.2, term=0.15, sampler=1)], 42, 6000, wpv=2,
+                      launch=dict(variant=variant))
     assert 0 < o["totals"]["mix_steps"] < o["totals"]["steps"]
 
 
-def test_mixed_batch(gpu):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_sopr_proposal_sampler(gpu, variant):
+    beg, csr = G.power_law_directed(3000, 40000, seed=2, multi=False)
+    g, o = check_case(beg, csr, [dict(algo=SOPR, length=30, alpha=0.2, term=0.15)], 42, 6000, wpv=2,
+                      launch=dict(variant=variant))
+    assert o["totals"]["mix_steps"] == 0 and o["totals"]["probes"] > 0
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_node2vec_both_variants(gpu, variant):
+    beg, csr = G.rmat(12, 30000, seed=6, permute=True)
+    check_case(beg, csr, [dict(algo=NODE2VEC, length=30, p=0.5, q=2.0)], 8, 4096, block_bytes=8192,
+               launch=dict(variant=variant))
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_mixed_batch(gpu, variant):
     beg, csr = G.rmat(11, 20000, seed=9, undirected=False)
     types = [dict(algo=NODE2VEC, length=20, p=1.0, q=0.25), dict(algo=SOPR, length=20, alpha=0.2, term=0.2)]
-    check_case(beg, csr, types, 11, 4096, wpv=2, block_bytes=8192)
+    check_case(beg, csr, types, 11, 4096, wpv=2, block_bytes=8192, launch=dict(variant=variant))
 
 
 @pytest.mark.parametrize("name", ["triangle", "star", "path", "bridge", "dag_dead_end"])
 def test_tiny_graphs(gpu, name):
     beg, csr = getattr(G, name)()
     n = len(beg) - 1
-    check_case(beg, csr, [dict(algo=NODE2VEC, length=12, p=0.5, q=2.0)], 1, n * 8, wpv=8)
-    check_case(beg, csr, [dict(algo=SOPR, length=12, alpha=0.3, term=0.1)], 2, n * 8, wpv=8)
+    for variant in (0, 1):
+        L = dict(variant=variant)
+        check_case(beg, csr, [dict(algo=NODE2VEC, length=12, p=0.5, q=2.0)], 1, n * 8, wpv=8, launch=L)
+        check_case(beg, csr, [dict(algo=SOPR, length=12, alpha=0.3, term=0.1)], 2, n * 8, wpv=8, launch=L)
+        check_case(beg, csr, [dict(algo=SOPR, length=12, alpha=0.3, term=0.1, sampler=1)], 3, n * 8, wpv=8,
+                   launch=L)
 
 
-def test_out_of_core_window(gpu):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_out_of_core_window(gpu, variant):
     beg, csr = G.rmat(10, 8000, seed=4, permute=True)
     g, o = check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 5, 1024, block_bytes=4096,
-                      nslots=3)
+                      nslots=3, launch=dict(variant=variant))
     assert g["runs"] > 1
     assert g["counters"]["h2d_bytes"] > 0
 
@@ -127,5 +152,6 @@ def test_sources_and_launch_shapes(gpu):
     beg, csr = G.random_undirected(500, 3000, seed=8)
     src = np.random.default_rng(0).integers(0, 500, 3000).astype(np.uint32)
     types = [dict(algo=NODE2VEC, length=15, p=2.0, q=0.5)]
-    for launch in (dict(block=64, waves_per_cu=1), dict(block=1024, waves_per_cu=32)):
+    for launch in (dict(block=64, waves_per_cu=1), dict(block=1024, waves_per_cu=32),
+                   dict(block=64, waves_per_cu=3, variant=1), dict(block=256, variant=1)):
         check_case(beg, csr, types, 3, 3000, sources=src, launch=launch)

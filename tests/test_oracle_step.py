@@ -89,6 +89,28 @@ def sopr_law(beg, csr, u, v, alpha, use_mixture):
     return law
 
 
+def sopr_proposal_law(beg, csr, u, v, alpha):
+    """Exact law of the proposal sampler: an attempt takes position i of N(v) with
+    probability T_B c_i(dv) / 2^64 (T_B = B 2^16), or proposes position p of N(u) with
+    probability (2^32 - T_B) c_p(du) / 2^64 and keeps it iff its value is in N(v) and
+    is not u."""
+    A = int(round(alpha * 65536))
+    B = 65536 - A
+    tb = B << 16
+    nu = nbrs(beg, csr, u)
+    nv = nbrs(beg, csr, v)
+    sv = set(nv)
+    cv, cu = pos_counts(len(nv)), pos_counts(len(nu))
+    w = {}
+    for i, x in enumerate(nv):
+        w[x] = w.get(x, 0) + tb * cv[i]
+    for p, y in enumerate(nu):
+        if y != u and y in sv:
+            w[y] = w.get(y, 0) + (TWO32 - tb) * cu[p]
+    tot = sum(w.values())
+    return {x: Fraction(a, tot) for x, a in w.items()}
+
+
 def joint_law(beg, csr, s, step_law):
     """Law of (x1, x2) for a walker starting at s (prev = cur = s)."""
     first = step_law(s, s)
@@ -166,6 +188,29 @@ def hub_graph(hub_deg=2000):
     return G.from_edges(n, src, dst)
 
 
+@pytest.mark.parametrize("graph", ["hub", "bridge"])
+def test_sopr_proposal_two_step_law(graph):
+    beg, csr = hub_graph(2000) if graph == "hub" else G.bridge(5)
+    s = 0 if graph == "hub" else 4
+    alpha = 0.2
+    cfg = O.walk_cfg(78, O.type_cfg(O.SOPR, 2, alpha=alpha, sampler=0))
+    emp = empirical_pairs(beg, csr, cfg, s, N)
+    law = joint_law(beg, csr, s, lambda u, v: sopr_proposal_law(beg, csr, u, v, alpha))
+    assert chi2_pvalue(law, emp, N) > 1e-3
+    # on duplicate-free lists it is the reference's law (1-a)/dv + a/du [x in N(u), x != u]
+    A = Fraction(int(round(alpha * 65536)), 65536)
+    for u, v in [(s, s)] + [(s, x) for x in nbrs(beg, csr, s)][:5]:
+        nu = set(nbrs(beg, csr, u))
+        nv = nbrs(beg, csr, v)
+        wt = {x: (1 - A) / len(nv) + (A / len(nu) if (x in nu and x != u) else 0) for x in nv}
+        tot = sum(wt.values())
+        pl = sopr_proposal_law(beg, csr, u, v, alpha)
+        for x in nv:
+            assert abs(float(pl[x] - wt[x] / tot)) < 1e-7
+    bad = joint_law(beg, csr, s, lambda u, v: sopr_law(beg, csr, u, v, 0.0, False))
+    assert chi2_pvalue(bad, emp, N) < 1e-6
+
+
 @pytest.mark.parametrize("case", ["mixture", "rejection"])
 def test_sopr_two_step_law(case):
     if case == "mixture":
@@ -175,7 +220,7 @@ def test_sopr_two_step_law(case):
         beg, csr = G.bridge(5)
         s = 4
     alpha = 0.2
-    cfg = O.walk_cfg(77, O.type_cfg(O.SOPR, 2, alpha=alpha))
+    cfg = O.walk_cfg(77, O.type_cfg(O.SOPR, 2, alpha=alpha, sampler=1))
     emp = empirical_pairs(beg, csr, cfg, s, N)
     mix_for = lambda u, v: bool(O.lib().orc_sopr_use_mixture(int(beg[u + 1] - beg[u]), int(beg[v + 1] - beg[v]),
                                                              int(round(alpha * 65536)), 65536 - int(round(alpha * 65536))))
@@ -275,14 +320,16 @@ def test_reference_transcription_fixed_mode_matches_build_distribution():
     print("literal-mode L1 vs build: %.4f (noise floor %.4f)" % (l1(build, lit), l1(ref_a, ref_b)))
 
 
-def test_reference_transcription_sopr_matches_build_distribution():
-    beg, csr = G.power_law_directed(150, 1500, seed=4)
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_reference_transcription_sopr_matches_build_distribution(sampler):
+    beg, csr = G.power_law_directed(150, 1500, seed=4, multi=False)
     nv = len(beg) - 1
     wpv, L = 100, 10
-    cfg = O.walk_cfg(42, O.type_cfg(O.SOPR, L, alpha=0.2))
+    cfg = O.walk_cfg(42, O.type_cfg(O.SOPR, L, alpha=0.2, sampler=sampler))
     o = O.run(beg, csr, cfg, nv * wpv, wpv=wpv, visits=True, hashes=False, state=False)
     build = o["visits"]
-    assert o["totals"]["mix_steps"] > 0
+    if sampler == 0:
+        assert o["totals"]["mix_steps"] == 0
     ref_a = O.ref_run(beg, csr, O.SOPR, L, alpha=0.2, wpv=wpv, seed=1, visits=True)["visits"]
     ref_b = O.ref_run(beg, csr, O.SOPR, L, alpha=0.2, wpv=wpv, seed=2, visits=True)["visits"]
     keep = (build + ref_a) > 10
