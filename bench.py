@@ -35,6 +35,12 @@ CONFIGS = {
                  "plain vs learned block scheduler",
         graph=dict(kind="rmat", scale=26, nedges=1_800_000_000, undirected=True, permute=True),
         types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=256, memory_gb=16),
+    "node2vec_rmat24_ooc": dict(
+        workload="Node2vec out of core (reduced), RMAT scale 24 with 400M generated edges symmetrized, dedup, "
+                 "random id permutation, 64 MB blocks, HBM cap 3 GB, p 0.5, q 2, 1 walker/vertex, length 40; "
+                 "plain vs learned block scheduler",
+        graph=dict(kind="rmat", scale=24, nedges=400_000_000, undirected=True, permute=True),
+        types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=64, memory_gb=3),
     "node2vec_rmat20": dict(
         workload="Node2vec, RMAT scale 20 (2^20 V, 16M edges symmetrized, dedup, no self-loops), "
                  "p 2, q 0.5, 1 walker/vertex, length 80, single resident block",
@@ -141,7 +147,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--variant", type=int, default=1, help="0: per-step lane loop, 1: flattened state machine")
+    ap.add_argument("--variant", type=int, default=0, help="0: per-step lane loop, 1: flattened state machine")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
     args = ap.parse_args()

@@ -671,6 +671,12 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
         const uint64_t need = ((uint64_t)nwork + l.block - 1) / l.block;
         if (grid > need) grid = need;
         if (grid == 0) grid = 1;
+        {
+            const uint64_t waves = grid * (l.block / 64);
+            uint64_t ch = (uint64_t)nwork / (waves * 8 + 1);
+            ch = ch < 64 ? 64 : (ch > 1024 ? 1024 : ch);
+            a.chunk = (uint32_t)(ch / 64 * 64);
+        }
         HIPCHK(c, hipEventRecord(c->ev_k0, c->s_compute));
         HIPCHK(c, launch_walk(a, one_blk, l.variant, (uint32_t)grid, l.block, c->s_compute));
         HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));

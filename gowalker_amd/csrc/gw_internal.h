@@ -62,6 +62,7 @@ struct WalkArgs {
     const uint32_t* worklist; // walker ids; nullptr = identity (finished walkers are skipped)
     uint32_t nwork;
     uint32_t* queue;          // work-fetch counter, zero at launch
+    uint32_t chunk;           // walkers a wavefront takes per queue atomic (multiple of 64)
     unsigned long long* counters;  // steps, trials, probes, mix, ended, left
     unsigned long long* block_steps;  // optional: steps per current block (access trace)
 };

@@ -24,14 +24,16 @@ namespace gw {
 
 typedef unsigned __int128 u128;
 
+// per-lane counters (u32: a lane's share of one launch stays far below 2^32); summed in
+// u64 across the wavefront at the end
 struct Cnt {
-    uint64_t steps, trials, probes, mix, ended, left;
+    uint32_t steps, trials, probes, mix, ended, left;
 };
 
 __device__ __forceinline__ uint32_t bitlen64(uint64_t n) { return n ? 64u - (uint32_t)__clzll(n) : 0u; }
 
 // engine/engine.hpp:224-243
-__device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t n, uint32_t x, uint64_t& probes)
+__device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t n, uint32_t x, uint32_t& probes)
 {
     int64_t lo = 0, hi = (int64_t)n - 1;
     while (lo <= hi) {
@@ -45,7 +47,7 @@ __device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t 
 }
 
 __device__ __forceinline__ uint64_t lower_bound(const uint32_t* __restrict__ a, uint64_t lo, uint64_t hi,
-                                                uint32_t x, uint64_t& probes)
+                                                uint32_t x, uint32_t& probes)
 {
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
@@ -56,7 +58,7 @@ __device__ __forceinline__ uint64_t lower_bound(const uint32_t* __restrict__ a, 
 }
 
 __device__ __forceinline__ uint64_t upper_bound(const uint32_t* __restrict__ a, uint64_t lo, uint64_t hi,
-                                                uint32_t x, uint64_t& probes)
+                                                uint32_t x, uint32_t& probes)
 {
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
@@ -71,7 +73,7 @@ __device__ __forceinline__ uint64_t upper_bound(const uint32_t* __restrict__ a, 
 // returns the index in N(v) of the select-th matching position.
 __device__ uint64_t intersect(const uint32_t* __restrict__ nu, uint64_t du,
                               const uint32_t* __restrict__ nv, uint64_t dv,
-                              uint32_t u, uint64_t select, uint64_t& probes)
+                              uint32_t u, uint64_t select, uint32_t& probes)
 {
     uint64_t c = 0, base = 0;
     uint32_t last = 0;
@@ -207,16 +209,16 @@ __device__ __forceinline__ Row load_row(const DevBlock& b, uint32_t v)
     return Row{ b.csr + h, t - h };
 }
 
-// Wave-local work queue: a wavefront takes kChunk walker indices with one atomic and
+// Wave-local work queue: a wavefront takes a chunk of walker indices with one atomic and
 // hands them to its lanes as they free up, so the shared counter sees one atomic per
-// kChunk walkers instead of one per walker.  `next`/`end` stay wavefront-uniform.
-constexpr uint32_t kChunk = 256;
-
+// chunk instead of one per walker.  The host sizes the chunk (64..1024) so that every
+// wavefront still gets several chunks.  `next`/`end` stay wavefront-uniform.
 struct WaveQueue {
     uint32_t next, end;
 };
 
-__device__ __forceinline__ uint32_t wave_take(WaveQueue& q, uint64_t mask, uint32_t* counter, uint32_t lane)
+__device__ __forceinline__ uint32_t wave_take(WaveQueue& q, uint64_t mask, uint32_t* counter, uint32_t lane,
+                                              uint32_t kChunk)
 {
     const uint32_t cnt = (uint32_t)__popcll(mask);
     const uint32_t rank = lane_rank(mask);
@@ -241,7 +243,7 @@ __device__ __forceinline__ uint32_t wave_take(WaveQueue& q, uint64_t mask, uint3
 // next one from the work queue (one atomic per wavefront), so a wavefront never waits
 // for its longest walker -- only for its slowest step.
 template <bool ONE>
-__global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_kernel(WalkArgs a)
 {
     // block-access trace: steps per current block, histogrammed in LDS
     __shared__ uint32_t trace[kTraceLdsBlocks];
@@ -265,7 +267,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(WalkArgs a)
         const bool need = !have && !exhausted;
         const uint64_t mask = __ballot(need);
         if (mask) {
-            const uint32_t idx = wave_take(q, mask, a.queue, lane);
+            const uint32_t idx = wave_take(q, mask, a.queue, lane, a.chunk);
             if (need) {
                 const uint32_t m = idx < a.nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
                 if (idx >= a.nwork) {
@@ -737,7 +739,7 @@ __global__ void __launch_bounds__(256) walk_flat_kernel(WalkArgs a)
         const bool need = L.phase == PH_IDLE && !exhausted;
         const uint64_t mask = __ballot(need);
         if (mask) {
-            const uint32_t idx = wave_take(q, mask, a.queue, lane);
+            const uint32_t idx = wave_take(q, mask, a.queue, lane, a.chunk);
             if (need) {
                 const uint32_t m = idx < a.nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
                 if (idx >= a.nwork) {

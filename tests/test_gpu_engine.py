@@ -62,13 +62,18 @@ def test_out_of_core_matches_oracle(gpu, sched, memory_blocks):
 
 
 def test_zero_copy_tail_and_sopr(gpu):
-    beg, csr = G.power_law_directed(20000, 200000, seed=5)
+    beg, csr = G.power_law_directed(20000, 200000, seed=5, multi=False)
     n = len(beg) - 1
     b, c = pinned_graph(beg, csr)
     types = [dict(algo=E.SOPR, length=25, alpha=0.2, term=0.1)]
-    # 64 KB blocks hold 16384 edges: no vertex is truncated (see test_giant_vertex_truncation)
-    rep, h, v = E.run_out_of_core(b.a, c.a, types, 4, n * 2, wpv=2, memory_bytes=0, block_bytes=65536,
-                                  zero_threshold=400, hashes=True, visits=True)
+    # 64 KB blocks hold 16384 edges: no vertex is truncated (see test_giant_vertex_truncation);
+    # a budget of 3 slots keeps blocks out, and a high zero_threshold ends the run by zero-copy
+    with E.Context() as ctx:
+        ctx.set_blocks(E.Blocks(beg, csr, 65536))
+        slot = ctx.slot_bytes()
+    mem = (n * 2) * (5 * 4 + 8) + n * 4 + 3 * slot + 4096
+    rep, h, v = E.run_out_of_core(b.a, c.a, types, 4, n * 2, wpv=2, memory_bytes=mem, block_bytes=65536,
+                                  zero_threshold=2000, hashes=True, visits=True)
     o = oracle_hashes(beg, csr, types, 4, n * 2, 2)
     np.testing.assert_array_equal(h, o["hashes"])
     np.testing.assert_array_equal(v, o["visits"])
