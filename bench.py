@@ -150,6 +150,8 @@ def main():
     ap.add_argument("--variant", type=int, default=0, help="0: per-step lane loop, 1: flattened state machine")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="time every launch shape on this many walkers (untimed) and run with the fastest")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X and this engine runs one "
@@ -189,6 +191,15 @@ def main():
         st = ctx.run(**launch)
         assert st["walkers_left"] == 0, st
         return st
+
+    tuned = None
+    if args.autotune:
+        ctx.init_walkers(nwalkers, cfg["wpv"], outputs=outputs)
+        tuned, tms = ctx.autotune(min(args.autotune, nwalkers))
+        launch = dict(block=0)
+        args.variant = tuned["variant"]
+        print("[bench] autotune: %s (%.2f ms on %d walkers)" % (tuned, tms, min(args.autotune, nwalkers)),
+              file=sys.stderr, flush=True)
 
     for i in range(args.warmup):
         st = one_step()
@@ -231,7 +242,7 @@ def main():
                    "walk_steps_per_pass": st0["steps"], "trials_per_pass": st0["trials"],
                    "probes_per_pass": st0["probes"], "mixture_steps_per_pass": st0["mix_steps"],
                    "h2d_gb": 0.0, "graph_h2d_gb_untimed": h2d_graph / 1e9, "graph_load_s": load_s,
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU", "launch": tuned or launch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "walk_flat_kernel<true>" if args.variant == 1 else "walk_kernel<true>",

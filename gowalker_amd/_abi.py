@@ -80,6 +80,7 @@ SIGNATURES = {
     "gw_set_walk": [P, C.POINTER(GwWalkParams)],
     "gw_init_walkers": [P, C.c_uint64, C.c_uint32, P, C.c_uint32],
     "gw_run": [P, C.POINTER(GwLaunch), C.POINTER(GwRunStats)],
+    "gw_autotune": [P, C.c_uint64, C.POINTER(GwLaunch), C.POINTER(C.c_double)],
     "gw_bucket_counts": [P, P, C.c_uint64],
     "gw_walkers_left": [P, u64p],
     "gw_read_visits": [P, P, C.c_uint64],

@@ -74,6 +74,10 @@ struct gw_ctx {
     // cumulative counters
     gw_counters tot{};
 
+    // launch shape chosen by gw_autotune
+    bool have_tuned = false;
+    gw_launch tuned{256, 0, 0, 0};
+
     // fault injection (GW_FAULT=upload:N or alloc:N fails the N-th such call)
     int fault_kind = 0;   // 1 upload, 2 alloc
     long fault_at = 0, n_upload = 0, n_alloc = 0;
@@ -610,13 +614,25 @@ static gw_status push_block_table(gw_ctx* c)
     return GW_OK;
 }
 
+static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t cap);
+
 gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
 {
     if (!c) return fail(c, GW_ERR_ARG, "null context");
     if (!c->nwalkers) return fail(c, GW_ERR_STATE, "gw_run before gw_init_walkers");
     (void)hipSetDevice(c->device);
     gw_launch l = L ? *L : gw_launch{};
-    if (l.block == 0) l.block = 256;
+    if (l.block == 0) {   // shape chosen by gw_autotune, else the default
+        const uint32_t flags = l.flags;
+        l = c->have_tuned ? c->tuned : gw_launch{256, 0, 0, 0};
+        l.flags = flags;
+    }
+    return run_impl(c, l, stats, 0);
+}
+
+// cap > 0 limits the run to the first `cap` work items (used by the autotuner)
+static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t cap)
+{
     if (l.block % 64 || l.block > 1024) return fail(c, GW_ERR_ARG, "block must be a multiple of 64 up to 1024");
     if (l.variant > 1) return fail(c, GW_ERR_ARG, "unknown kernel variant %u", l.variant);
     if (l.variant == 1 && l.block > 256) return fail(c, GW_ERR_ARG, "the flattened kernel runs at most 256 threads per workgroup");
@@ -637,6 +653,7 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
     } else {
         if ((st = run_filter(c, (l.flags & GW_RUN_BUCKETS) != 0, true, &nwork, &left, &fms)) != GW_OK) return st;
     }
+    if (cap && nwork > cap) nwork = (uint32_t)cap;
     HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
     if (nwork) {
         WalkArgs a{};
@@ -703,6 +720,63 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
     c->tot.mix_steps += s.mix_steps;
     c->tot.kernel_ms += kms;
     if (stats) *stats = s;
+    return GW_OK;
+}
+
+// Launch-shape autotuner: times the step kernel on the first `sample` work items for
+// every candidate (kernel variant x workgroup size), restoring walker state, visit counts
+// and counters after each trial, and keeps the fastest for later gw_run calls that pass
+// block == 0.  Walks are bit-identical for every shape; only time changes.
+gw_status gw_autotune(gw_ctx* c, uint64_t sample, gw_launch* best, double* best_ms)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (!c->nwalkers) return fail(c, GW_ERR_STATE, "gw_autotune before gw_init_walkers");
+    if (c->d_paths) return fail(c, GW_ERR_STATE, "gw_autotune cannot restore path output");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    const uint64_t n = c->nwalkers;
+    struct Snap { void* dev; void* save; uint64_t bytes; };
+    std::vector<Snap> snaps = {{c->d_cur, nullptr, n * 4}, {c->d_prev, nullptr, n * 4}, {c->d_meta, nullptr, n * 4},
+                               {c->d_bpair, nullptr, n * 4}};
+    if (c->d_hash) snaps.push_back({c->d_hash, nullptr, n * 8});
+    if (c->d_visits) snaps.push_back({c->d_visits, nullptr, c->nverts_total * 4});
+    gw_status st = GW_OK;
+    auto cleanup = [&]() { for (auto& x : snaps) if (x.save) (void)hipFree(x.save); };
+    for (auto& x : snaps) {
+        if (hipMalloc(&x.save, x.bytes) != hipSuccess) { cleanup(); return fail(c, GW_ERR_NOMEM, "autotune snapshot"); }
+        if (hipMemcpyAsync(x.save, x.dev, x.bytes, hipMemcpyDeviceToDevice, c->s_compute) != hipSuccess) {
+            cleanup();
+            return fail(c, GW_ERR_HIP, "autotune snapshot copy");
+        }
+    }
+    auto restore = [&]() -> bool {
+        for (auto& x : snaps)
+            if (hipMemcpyAsync(x.dev, x.save, x.bytes, hipMemcpyDeviceToDevice, c->s_compute) != hipSuccess) return false;
+        return hipStreamSynchronize(c->s_compute) == hipSuccess;
+    };
+    const gw_counters tot0 = c->tot;
+    static const gw_launch cand[] = {{64, 0, 0, 0},  {128, 0, 0, 0}, {256, 0, 0, 0}, {512, 0, 0, 0},
+                                     {1024, 0, 0, 0}, {64, 0, 1, 0},  {128, 0, 1, 0}, {256, 0, 1, 0}};
+    double bms = 1e300;
+    gw_launch bl = cand[2];
+    for (const gw_launch& L : cand) {
+        double t = 1e300;
+        for (int rep = 0; rep < 2 && st == GW_OK; ++rep) {
+            gw_run_stats rs{};
+            if ((st = run_impl(c, L, &rs, sample)) != GW_OK) break;
+            t = std::min(t, rs.kernel_ms);
+            if (!restore()) st = fail(c, GW_ERR_HIP, "autotune restore");
+        }
+        if (st != GW_OK) break;
+        if (t < bms) { bms = t; bl = L; }
+    }
+    c->tot = tot0;
+    cleanup();
+    if (st != GW_OK) return st;
+    c->tuned = bl;
+    c->have_tuned = true;
+    if (best) *best = bl;
+    if (best_ms) *best_ms = bms;
     return GW_OK;
 }
 

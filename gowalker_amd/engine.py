@@ -160,6 +160,13 @@ class Context:
         self._chk(lib().gw_run(self.h, C.byref(L), C.byref(s)))
         return {f: getattr(s, f) for f, _ in GwRunStats._fields_}
 
+    def autotune(self, sample):
+        """Pick the fastest launch shape on the first `sample` walkers (state restored)."""
+        best = GwLaunch()
+        ms = C.c_double()
+        self._chk(lib().gw_autotune(self.h, sample, C.byref(best), C.byref(ms)))
+        return dict(block=best.block, waves_per_cu=best.waves_per_cu, variant=best.variant), ms.value
+
     def bucket_counts(self):
         nb = self.blocks.nblocks
         W = np.zeros(nb * nb, dtype=np.uint64)

@@ -178,6 +178,12 @@ gw_status gw_init_walkers(gw_ctx* ctx, uint64_t nwalkers, uint32_t walkers_per_v
  * with gpu_run/SOPR_gpu_run, prefix, insertglobal and walkswap (gpu/gpuwalk.hpp:193-372). */
 gw_status gw_run(gw_ctx* ctx, const gw_launch* launch, gw_run_stats* stats);
 
+/* Launch-shape autotuner (no reference counterpart; the reference hard-codes 512 x 512,
+ * apps/Node2vec.cu:47-48): times the step kernel on the first `sample` work items for each
+ * candidate kernel variant and workgroup size, restoring walker state after every trial,
+ * and makes the fastest the shape of later gw_run calls that pass block == 0. */
+gw_status gw_autotune(gw_ctx* ctx, uint64_t sample, gw_launch* best, double* best_ms);
+
 /* Bucket matrix W[prev_blk * nblocks + cur_blk] of unfinished walkers, as of the last
  * gw_run with GW_RUN_BUCKETS (or gw_init_walkers): replaces copy_back's read of
  * block_offset (engine/walk.hpp:488-498) and walk_block::block_numwalks. */

@@ -155,3 +155,28 @@ def test_sources_and_launch_shapes(gpu):
     for launch in (dict(block=64, waves_per_cu=1), dict(block=1024, waves_per_cu=32),
                    dict(block=64, waves_per_cu=3, variant=1), dict(block=256, variant=1)):
         check_case(beg, csr, types, 3, 3000, sources=src, launch=launch)
+
+
+def test_autotune_keeps_walks_bit_identical(gpu):
+    # the tuner runs every candidate shape on a prefix of the walkers and restores their
+    # state; the walks that follow with the tuned shape must still equal the oracle's
+    beg, csr = G.rmat(12, 30000, seed=12, permute=True)
+    types = [dict(algo=NODE2VEC, length=25, p=0.5, q=2.0)]
+    n = 8192
+    bl = Blocks(beg, csr, 1 << 30)
+    with Context() as ctx:
+        ctx.set_blocks(bl)
+        ctx.load_all()
+        ctx.set_walk(types, 21)
+        ctx.init_walkers(n, 1, outputs=OUT_HASH | OUT_VISITS)
+        best, ms = ctx.autotune(2048)
+        print("autotune best", best, "ms", ms)
+        assert best["block"] in (64, 128, 256, 512, 1024) and ms > 0
+        assert ctx.counters()["steps"] == 0
+        st = ctx.run(block=0)
+        assert st["walkers_left"] == 0
+        hashes, state, visits = ctx.hashes(), ctx.state(), ctx.visits()
+    o = O.run(beg, csr, O.walk_cfg(21, *oracle_types(types)), n, visits=True)
+    np.testing.assert_array_equal(state, o["state"])
+    np.testing.assert_array_equal(hashes, o["hashes"])
+    np.testing.assert_array_equal(visits, o["visits"])
