@@ -59,6 +59,14 @@ def algorithmic_bytes(st, nwalkers, visits):
     return b
 
 
+def sector_bytes(st, nwalkers, visits):
+    """The same accesses priced at the 64-byte granularity of a random HBM touch: every
+    row lookup, candidate and probe is one sector, a visit RMW two (read + write-back);
+    walker state streams (40 B per walker)."""
+    touches = st["steps"] * (1 + (2 if visits else 0)) + st["trials"] + st["probes"] + nwalkers * 2
+    return touches * 64 + nwalkers * 40
+
+
 def build_graph(ctx, g, seed):
     from gowalker_amd import engine as E
     if g["kind"] == "chung_lu":
@@ -246,7 +254,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "walk_flat_kernel<true>" if args.variant == 1 else "walk_kernel<true>",
-                     "kernel_ms": avg_kms, "algorithmic_bytes": abytes},
+                     "kernel_ms": avg_kms, "algorithmic_bytes": abytes,
+                     "sector_bytes": sector_bytes(st0, nwalkers, visits)},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

@@ -168,7 +168,7 @@ def test_autotune_keeps_walks_bit_identical(gpu):
         ctx.set_blocks(bl)
         ctx.load_all()
         ctx.set_walk(types, 21)
-        ctx.init_walkers(n, 1, outputs=OUT_HASH | OUT_VISITS)
+        ctx.init_walkers(n, 2, outputs=OUT_HASH | OUT_VISITS)
         best, ms = ctx.autotune(2048)
         print("autotune best", best, "ms", ms)
         assert best["block"] in (64, 128, 256, 512, 1024) and ms > 0
@@ -176,7 +176,7 @@ def test_autotune_keeps_walks_bit_identical(gpu):
         st = ctx.run(block=0)
         assert st["walkers_left"] == 0
         hashes, state, visits = ctx.hashes(), ctx.state(), ctx.visits()
-    o = O.run(beg, csr, O.walk_cfg(21, *oracle_types(types)), n, visits=True)
+    o = O.run(beg, csr, O.walk_cfg(21, *oracle_types(types)), n, wpv=2, visits=True)
     np.testing.assert_array_equal(state, o["state"])
     np.testing.assert_array_equal(hashes, o["hashes"])
     np.testing.assert_array_equal(visits, o["visits"])
