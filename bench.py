@@ -31,16 +31,16 @@ CONFIGS = {
         types=[dict(algo=1, length=30, alpha=0.2, term=0.15)], wpv=4),
     "node2vec_rmat26_ooc": dict(
         workload="Node2vec out of core, RMAT scale 26 (67.1M ids) with 1.8B generated edges symmetrized, dedup, "
-                 "random id permutation, 256 MB blocks, HBM cap 16 GB, p 0.5, q 2, 1 walker/vertex, length 40; "
-                 "plain vs learned block scheduler",
+                 "random id permutation, 256 MB blocks, HBM cap 16 GB (and 8 GB), p 0.5, q 2, 1 walker/vertex, "
+                 "length 40; plain vs learned block scheduler",
         graph=dict(kind="rmat", scale=26, nedges=1_800_000_000, undirected=True, permute=True),
-        types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=256, memory_gb=16),
+        types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=256, memory_gb=[16, 8]),
     "node2vec_rmat24_ooc": dict(
         workload="Node2vec out of core (reduced), RMAT scale 24 with 400M generated edges symmetrized, dedup, "
                  "random id permutation, 64 MB blocks, HBM cap 3 GB, p 0.5, q 2, 1 walker/vertex, length 40; "
                  "plain vs learned block scheduler",
         graph=dict(kind="rmat", scale=24, nedges=400_000_000, undirected=True, permute=True),
-        types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=64, memory_gb=3),
+        types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=64, memory_gb=[3]),
     "node2vec_rmat20": dict(
         workload="Node2vec, RMAT scale 20 (2^20 V, 16M edges symmetrized, dedup, no self-loops), "
                  "p 2, q 0.5, 1 walker/vertex, length 80, single resident block",
@@ -97,50 +97,94 @@ def cpu_baseline(beg, csr, cfg, seconds, threads):
                        % (r["done"], sample, dt, r["moves"]))
 
 
+PCIE_PEAK_GBS = 64.0   # PCIe Gen5 x16, one direction
+
+
+def ooc_schedulers(E, block_bytes):
+    # plain: the reference's max_choose_blocks without the optional zero-copy tail;
+    # plain_zero: the same with the reference's `zero` option (threshold blocksize/128/24
+    # walkers, apps/Node2vec.cu:100-104); learned: the MLP scheduler
+    return (("plain", E.SCHED_PLAIN, 0), ("plain_zero", E.SCHED_PLAIN, block_bytes // 128 // 24),
+            ("learned", E.SCHED_LEARNED, 0))
+
+
 def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
-    """Out-of-core configuration: the C++ engine with the plain and the learned scheduler."""
+    """Out-of-core configuration: the C++ engine under each scheduler, at the stated HBM cap
+    and at the tighter caps listed with it.  One timed step = one full walk of every walker."""
     from gowalker_amd import engine as E
     ctx.close()
-    mem = int((args.memory_gb if args.memory_gb else cfg["memory_gb"]) * (1 << 30))
-    res = {}
+    caps = [args.memory_gb] if args.memory_gb else cfg["memory_gb"]
+    block_bytes = cfg["block_mb"] << 20
+    nwalkers = nverts * cfg["wpv"]
+    per_cap = []
     hashes = {}
-    for name, sched in (("plain", E.SCHED_PLAIN), ("learned", E.SCHED_LEARNED)):
-        reps = []
-        for i in range(args.warmup + args.steps):
-            rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nverts * cfg["wpv"], wpv=cfg["wpv"],
-                                          memory_bytes=mem, block_bytes=cfg["block_mb"] << 20,
-                                          scheduler=sched, hashes=(i == 0))
-            if i == 0:
-                hashes[name] = h
-            print("[bench] %s run %d: %.2f s, %d steps, %.2f GB H2D, %d iterations, %d slots / %d blocks" %
-                  (name, i, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["iterations"], rep["nslots"],
-                   rep["nblocks"]), file=sys.stderr, flush=True)
-            if i >= args.warmup:
-                reps.append(rep)
-        wall = sum(r["wall_s"] for r in reps)
-        res[name] = dict(steps_per_s=sum(r["steps"] for r in reps) / wall, h2d_gb=reps[0]["h2d_bytes"] / 1e9,
-                         ms_per_run=wall * 1e3 / len(reps), iterations=reps[0]["iterations"],
-                         uploads=reps[0]["uploads"], zero_copy_steps=reps[0]["zero_copy_steps"],
-                         h2d_gbs=(reps[0]["h2d_bytes"] / 1e9) / (reps[0]["h2d_ms"] / 1e3) if reps[0]["h2d_ms"] else None,
-                         kernel_ms=reps[0]["kernel_ms"], learned_decisions=reps[0]["learned_decisions"],
-                         fallbacks=reps[0]["fallbacks"], nslots=reps[0]["nslots"], nblocks=reps[0]["nblocks"],
-                         steps=reps[0]["steps"])
-    identical = bool((hashes["plain"] == hashes["learned"]).all())
+    learned_reps = None
+    for cap_gb in caps:
+        mem = int(cap_gb * (1 << 30))
+        res = {}
+        for name, sched, zt in ooc_schedulers(E, block_bytes):
+            reps = []
+            for i in range(args.warmup + args.steps):
+                first = not hashes.get((cap_gb, name)) and i == 0
+                rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
+                                              memory_bytes=mem, block_bytes=block_bytes, zero_threshold=zt,
+                                              scheduler=sched, hashes=first)
+                if first:
+                    hashes[(cap_gb, name)] = h
+                print("[bench] cap %g GB %s run %d: %.3f s, %d steps, %.2f GB H2D (%d uploads + %d staged), "
+                      "%.3f GB zero-copy, %d iterations, kernel %.0f ms, copy %.0f ms, sched %.0f ms, %d slots / %d blocks"
+                      % (cap_gb, name, i, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["uploads"],
+                         rep["staged"], rep["zc_touches"] * 64 / 1e9, rep["iterations"], rep["kernel_ms"],
+                         rep["h2d_ms"], rep["sched_ms"], rep["nslots"], rep["nblocks"]), file=sys.stderr, flush=True)
+                if i >= args.warmup:
+                    reps.append(rep)
+            r0 = reps[0]
+            wall = sum(r["wall_s"] for r in reps)
+            zc_gb = r0["zc_touches"] * 64 / 1e9
+            res[name] = dict(steps_per_s=sum(r["steps"] for r in reps) / wall, ms_per_run=wall * 1e3 / len(reps),
+                             h2d_gb=r0["h2d_bytes"] / 1e9, zero_copy_gb_est=zc_gb,
+                             pcie_gb_total=r0["h2d_bytes"] / 1e9 + zc_gb,
+                             h2d_gbs=(r0["h2d_bytes"] / 1e9) / (r0["h2d_ms"] / 1e3) if r0["h2d_ms"] else None,
+                             iterations=r0["iterations"], uploads=r0["uploads"], staged=r0["staged"],
+                             zero_copy_steps=r0["zero_copy_steps"], kernel_ms=r0["kernel_ms"],
+                             copy_ms=r0["h2d_ms"], sched_ms=r0["sched_ms"], learned_decisions=r0["learned_decisions"],
+                             tail_decisions=r0["tail_decisions"], fallbacks=r0["fallbacks"], nslots=r0["nslots"],
+                             nblocks=r0["nblocks"], steps=r0["steps"], zero_threshold=zt)
+            if name == "learned" and learned_reps is None:
+                learned_reps = reps
+        ref = hashes[(cap_gb, "plain")]
+        ident = all(bool((hashes[(cap_gb, n)] == ref).all()) for n, _, _ in ooc_schedulers(E, block_bytes))
+        L, P = res["learned"], res["plain"]
+        per_cap.append({"memory_gb": cap_gb, "walks_identical": ident, "schedulers": res,
+                        "learned_vs_plain_steps_per_s": L["steps_per_s"] / P["steps_per_s"],
+                        "learned_vs_plain_pcie_bytes": L["pcie_gb_total"] / P["pcie_gb_total"],
+                        "learned_vs_plain_zero_steps_per_s": L["steps_per_s"] / res["plain_zero"]["steps_per_s"],
+                        "learned_vs_plain_zero_pcie_bytes": L["pcie_gb_total"] / res["plain_zero"]["pcie_gb_total"]})
+    head = per_cap[0]["schedulers"]["learned"]
+    r0 = learned_reps[0]
+    abytes = algorithmic_bytes(r0, nwalkers, False)
+    achieved = abytes / (r0["kernel_ms"] * 1e-3) / 1e9
     line = {
         "metric": "walk steps/s on 1 MI355X + H2D GB, learned vs plain block scheduler",
-        "value": res["learned"]["steps_per_s"],
+        "value": head["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": res["learned"]["ms_per_run"],
+        "ms_per_step": head["ms_per_run"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic (generated on the GPU, seed %d)" % args.seed,
-        "config": {"workload": cfg["workload"], "nverts": nverts, "narcs": narcs, "memory_bytes": mem,
-                   "walks_identical": identical, "schedulers": res,
-                   "learned_vs_plain_steps_per_s": res["learned"]["steps_per_s"] / res["plain"]["steps_per_s"],
-                   "learned_vs_plain_h2d": res["learned"]["h2d_gb"] / res["plain"]["h2d_gb"]
-                   if res["plain"]["h2d_gb"] else None, "graph_gen_s": gen_s, "parallelism": "1 GPU"},
-        "roofline": None, "cpu_baseline": None,
+        "config": {"workload": cfg["workload"], "nverts": nverts, "narcs": narcs, "walkers": nwalkers,
+                   "caps": per_cap, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS,
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "walk_kernel<false>",
+                     "kernel_ms": r0["kernel_ms"], "algorithmic_bytes": abytes,
+                     "sector_bytes": sector_bytes(r0, nwalkers, False),
+                     "note": "all step-kernel launches of one learned-scheduler walk at the first cap, summed"},
+        "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0))
+        line["cpu_baseline"] = cpu_baseline(beg.a, csr.a, cfg, args.cpu_seconds, threads)
     print(json.dumps(line), flush=True)
 
 

@@ -43,13 +43,13 @@ class GwLaunch(C.Structure):
 class GwRunStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("trials", C.c_uint64), ("probes", C.c_uint64), ("mix_steps", C.c_uint64),
                 ("walkers_run", C.c_uint64), ("walkers_ended", C.c_uint64), ("walkers_left", C.c_uint64),
-                ("kernel_ms", C.c_double), ("filter_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("filter_ms", C.c_double), ("zc_touches", C.c_uint64)]
 
 
 class GwCounters(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("trials", C.c_uint64), ("probes", C.c_uint64), ("mix_steps", C.c_uint64),
                 ("h2d_bytes", C.c_uint64), ("h2d_uploads", C.c_uint64), ("h2d_ms", C.c_double),
-                ("kernel_ms", C.c_double), ("hbm_bytes_held", C.c_uint64)]
+                ("kernel_ms", C.c_double), ("hbm_bytes_held", C.c_uint64), ("zc_touches", C.c_uint64)]
 
 
 class GwDeviceInfo(C.Structure):
@@ -79,6 +79,9 @@ SIGNATURES = {
     "gw_unmap_host_blocks": [P],
     "gw_set_walk": [P, C.POINTER(GwWalkParams)],
     "gw_init_walkers": [P, C.c_uint64, C.c_uint32, P, C.c_uint32],
+    "gw_stage_block": [P, C.c_uint32, C.c_uint32, P, P],
+    "gw_commit_block": [P, C.c_uint32],
+    "gw_staged_block": [P, C.c_uint32, C.POINTER(C.c_int32), u32p],
     "gw_run": [P, C.POINTER(GwLaunch), C.POINTER(GwRunStats)],
     "gw_autotune": [P, C.c_uint64, C.POINTER(GwLaunch), C.POINTER(C.c_double)],
     "gw_bucket_counts": [P, P, C.c_uint64],
@@ -156,7 +159,9 @@ class GwhReport(C.Structure):
                 ("zero_copy_runs", C.c_uint64), ("zero_copy_steps", C.c_uint64), ("wall_s", C.c_double),
                 ("kernel_ms", C.c_double), ("filter_ms", C.c_double), ("h2d_ms", C.c_double),
                 ("nslots", C.c_uint32), ("nblocks", C.c_uint32), ("learned_decisions", C.c_uint64),
-                ("fallbacks", C.c_uint64), ("final_loss", C.c_float), ("pad", C.c_uint32)]
+                ("fallbacks", C.c_uint64), ("final_loss", C.c_float), ("pad", C.c_uint32),
+                ("staged", C.c_uint64), ("zc_touches", C.c_uint64), ("tail_decisions", C.c_uint64),
+                ("sched_ms", C.c_double)]
 
 
 HOST_SIGNATURES = {

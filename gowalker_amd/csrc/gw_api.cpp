@@ -47,6 +47,10 @@ struct gw_ctx {
     std::vector<uint64_t*> slot_beg;
     std::vector<uint32_t*> slot_csr;
     std::vector<int32_t> slot_blk;
+    std::vector<int32_t> slot_staged;     // block copied by gw_stage_block, not yet committed
+    std::vector<hipEvent_t> slot_ev0, slot_ev1;   // start / end of the slot's last copy
+    std::vector<uint8_t> slot_wait;       // a run must wait for the slot's copy
+    std::vector<uint8_t> slot_timed;      // copy time not yet added to tot.h2d_ms
 
     // walk configuration
     bool walk_set = false;
@@ -70,6 +74,11 @@ struct gw_ctx {
     unsigned long long* d_cnt = nullptr;   // 8 counters
     unsigned long long* h_cnt = nullptr;   // pinned readback
     uint64_t walker_bytes = 0;
+
+    // bucket matrix of the walkers left by the last run (GW_RUN_BUCKETS) or pass
+    bool w_valid = false;
+    uint32_t* h_bucket = nullptr;          // pinned nblocks^2
+    uint64_t unfinished = 0;               // unfinished walkers
 
     // cumulative counters
     gw_counters tot{};
@@ -145,15 +154,39 @@ static void free_walkers(gw_ctx* c)
     c->walker_bytes = 0;
 }
 
+// Adds the copy-engine time of a slot's last copy to tot.h2d_ms (waiting for the copy
+// when `wait`; otherwise only if it has completed).
+static void collect_copy_time(gw_ctx* c, uint32_t s, bool wait)
+{
+    if (!c->slot_timed[s]) return;
+    if (wait) (void)hipEventSynchronize(c->slot_ev1[s]);
+    else if (hipEventQuery(c->slot_ev1[s]) != hipSuccess) return;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->slot_ev0[s], c->slot_ev1[s]) == hipSuccess) c->tot.h2d_ms += ms;
+    c->slot_timed[s] = 0;
+}
+
+static gw_status ensure_buckets(gw_ctx* c);
+static gw_status push_block_table(gw_ctx* c);
+
 static void free_slots(gw_ctx* c)
 {
+    if (c->s_copy) (void)hipStreamSynchronize(c->s_copy);
     for (uint32_t s = 0; s < c->nslots; ++s) {
+        collect_copy_time(c, s, true);
         dfree(c, c->slot_beg[s], c->slot_beg_bytes, true);
         dfree(c, c->slot_csr[s], c->slot_csr_bytes, true);
+        (void)hipEventDestroy(c->slot_ev0[s]);
+        (void)hipEventDestroy(c->slot_ev1[s]);
     }
     c->slot_beg.clear();
     c->slot_csr.clear();
     c->slot_blk.clear();
+    c->slot_staged.clear();
+    c->slot_ev0.clear();
+    c->slot_ev1.clear();
+    c->slot_wait.clear();
+    c->slot_timed.clear();
     c->nslots = 0;
     for (auto& b : c->blk_slot) b = -1;
 }
@@ -236,6 +269,7 @@ gw_status gw_destroy(gw_ctx* c)
     dfree(c, c->d_cnt, 0, false);
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
     if (c->h_blocks_pinned) (void)hipHostFree(c->h_blocks_pinned);
+    if (c->h_bucket) (void)hipHostFree(c->h_bucket);
     hipEvent_t evs[] = {c->ev_copy, c->ev_f0, c->ev_f1, c->ev_k0, c->ev_k1, c->ev_c0, c->ev_c1};
     for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
     if (c->s_compute) (void)hipStreamDestroy(c->s_compute);
@@ -292,6 +326,8 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     c->d_blocks = nullptr; c->d_vcut = nullptr; c->d_bucket = nullptr;
     c->d_trace = nullptr; c->d_feat = nullptr; c->d_age = nullptr;
     if (c->h_blocks_pinned) { (void)hipHostFree(c->h_blocks_pinned); c->h_blocks_pinned = nullptr; }
+    if (c->h_bucket) { (void)hipHostFree(c->h_bucket); c->h_bucket = nullptr; }
+    c->w_valid = false;
     c->nblocks = nb;
     c->desc.assign(b, b + nb);
     c->hblk.assign(nb, DevBlock{});
@@ -325,6 +361,7 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     if ((st = dalloc(c, &p, sizeof(float) * nb, false)) != GW_OK) return st;
     c->d_age = (float*)p;
     HIPCHK(c, hipHostMalloc((void**)&c->h_blocks_pinned, sizeof(DevBlock) * nb, hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_bucket, sizeof(uint32_t) * (uint64_t)nb * nb, hipHostMallocDefault));
     HIPCHK(c, hipMemcpy(c->d_vcut, c->vcut.data(), sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice));
     return GW_OK;
 }
@@ -360,10 +397,19 @@ gw_status gw_reserve_slots(gw_ctx* c, uint32_t nslots)
     c->slot_beg.assign(nslots, nullptr);
     c->slot_csr.assign(nslots, nullptr);
     c->slot_blk.assign(nslots, -1);
+    c->slot_staged.assign(nslots, -1);
+    c->slot_ev0.assign(nslots, nullptr);
+    c->slot_ev1.assign(nslots, nullptr);
+    c->slot_wait.assign(nslots, 0);
+    c->slot_timed.assign(nslots, 0);
     c->nslots = nslots;
     for (uint32_t s = 0; s < nslots; ++s) {
         void* p;
         gw_status st;
+        if (hipEventCreate(&c->slot_ev0[s]) != hipSuccess || hipEventCreate(&c->slot_ev1[s]) != hipSuccess) {
+            free_slots(c);
+            return fail(c, GW_ERR_HIP, "hipEventCreate for slot %u", s);
+        }
         if ((st = dalloc(c, &p, c->slot_beg_bytes, true)) != GW_OK) { free_slots(c); return st; }
         c->slot_beg[s] = (uint64_t*)p;
         if ((st = dalloc(c, &p, c->slot_csr_bytes, true)) != GW_OK) { free_slots(c); return st; }
@@ -371,7 +417,7 @@ gw_status gw_reserve_slots(gw_ctx* c, uint32_t nslots)
     }
     // re-expose zero-copy mappings that exist
     for (uint32_t i = 0; i < c->nblocks; ++i)
-        if (c->map_beg[i]) { c->hblk[i].beg = c->map_beg[i]; c->hblk[i].csr = c->map_csr[i]; c->hblk[i].resident = 1; }
+        if (c->map_beg[i]) { c->hblk[i].beg = c->map_beg[i]; c->hblk[i].csr = c->map_csr[i]; c->hblk[i].resident = kMapped; }
     return GW_OK;
 }
 
@@ -379,43 +425,105 @@ static void block_nonresident(gw_ctx* c, uint32_t blk)
 {
     DevBlock& d = c->hblk[blk];
     c->blk_slot[blk] = -1;
-    if (c->map_beg[blk]) { d.beg = c->map_beg[blk]; d.csr = c->map_csr[blk]; d.resident = 1; }
+    if (c->map_beg[blk]) { d.beg = c->map_beg[blk]; d.csr = c->map_csr[blk]; d.resident = kMapped; }
     else { d.beg = nullptr; d.csr = nullptr; d.resident = 0; }
 }
 
-gw_status gw_upload_block(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const uint32_t* csr)
+// Starts the copy of block `blk` into `slot` on the copy stream (no host wait).  The
+// slot's previous block stops being readable at once; the copy waits for kernels already
+// enqueued on the compute stream, which may still read that block.
+static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const uint32_t* csr)
 {
-    if (!c) return fail(c, GW_ERR_ARG, "null context");
-    if (slot >= c->nslots) return fail(c, GW_ERR_ARG, "slot %u out of range (%u slots)", slot, c->nslots);
-    if (blk >= c->nblocks) return fail(c, GW_ERR_ARG, "block %u out of range (%u blocks)", blk, c->nblocks);
     const gw_block_desc& d = c->desc[blk];
     if (!beg || (d.nedges && !csr)) return fail(c, GW_ERR_ARG, "null block buffer");
     if (c->fault_kind == 1 && ++c->n_upload == c->fault_at)
         return fail(c, GW_ERR_FAULT, "injected upload failure (GW_FAULT=upload:%ld)", c->fault_at);
     (void)hipSetDevice(c->device);
     if (c->slot_blk[slot] >= 0) block_nonresident(c, (uint32_t)c->slot_blk[slot]);
-    if (c->blk_slot[blk] >= 0) { c->slot_blk[c->blk_slot[blk]] = -1; c->blk_slot[blk] = -1; }
-    // the slot may still be read by a kernel of the previous run on the compute stream
-    HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));
-    HIPCHK(c, hipStreamWaitEvent(c->s_copy, c->ev_k1, 0));
+    c->slot_blk[slot] = -1;
+    c->slot_staged[slot] = -1;
+    collect_copy_time(c, slot, true);   // the slot's events are about to be re-recorded
+    HIPCHK(c, hipEventRecord(c->ev_c0, c->s_compute));
+    HIPCHK(c, hipStreamWaitEvent(c->s_copy, c->ev_c0, 0));
     const uint64_t bb = ((uint64_t)d.nverts + 1) * 8, cb = d.nedges * 4;
-    HIPCHK(c, hipEventRecord(c->ev_c0, c->s_copy));
+    HIPCHK(c, hipEventRecord(c->slot_ev0[slot], c->s_copy));
     HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, c->s_copy));
     if (cb) HIPCHK(c, hipMemcpyAsync(c->slot_csr[slot], csr, cb, hipMemcpyHostToDevice, c->s_copy));
-    HIPCHK(c, hipEventRecord(c->ev_c1, c->s_copy));
+    HIPCHK(c, hipEventRecord(c->slot_ev1[slot], c->s_copy));
     HIPCHK(c, hipEventRecord(c->ev_copy, c->s_copy));
+    c->slot_timed[slot] = 1;
+    c->tot.h2d_bytes += bb + cb;
+    c->tot.h2d_uploads += 1;
+    return GW_OK;
+}
+
+static void make_resident(gw_ctx* c, uint32_t slot, uint32_t blk)
+{
     c->slot_blk[slot] = (int32_t)blk;
     c->blk_slot[blk] = (int32_t)slot;
+    c->slot_wait[slot] = 1;   // the next run's kernels wait for the copy (device-side)
     DevBlock& h = c->hblk[blk];
     h.beg = c->slot_beg[slot];
     h.csr = c->slot_csr[slot];
-    h.resident = 1;
-    c->tot.h2d_bytes += bb + cb;
-    c->tot.h2d_uploads += 1;
-    // copy-engine time, accumulated when the copy completes (gw_wait_uploads / gw_run)
-    HIPCHK(c, hipEventSynchronize(c->ev_c1));
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c->ev_c0, c->ev_c1) == hipSuccess) c->tot.h2d_ms += ms;
+    h.resident = kInSlot;
+}
+
+static gw_status check_slot_blk(gw_ctx* c, uint32_t slot, uint32_t blk)
+{
+    if (slot >= c->nslots) return fail(c, GW_ERR_ARG, "slot %u out of range (%u slots)", slot, c->nslots);
+    if (blk >= c->nblocks) return fail(c, GW_ERR_ARG, "block %u out of range (%u blocks)", blk, c->nblocks);
+    return GW_OK;
+}
+
+gw_status gw_upload_block(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const uint32_t* csr)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    gw_status st;
+    if ((st = check_slot_blk(c, slot, blk)) != GW_OK) return st;
+    if (c->blk_slot[blk] >= 0) {   // moving a resident block: its old slot becomes free
+        c->slot_blk[c->blk_slot[blk]] = -1;
+        block_nonresident(c, blk);
+    }
+    if ((st = start_copy(c, slot, blk, beg, csr)) != GW_OK) return st;
+    make_resident(c, slot, blk);
+    return GW_OK;
+}
+
+gw_status gw_stage_block(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const uint32_t* csr)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    gw_status st;
+    if ((st = check_slot_blk(c, slot, blk)) != GW_OK) return st;
+    if (c->blk_slot[blk] >= 0) return fail(c, GW_ERR_ARG, "block %u is already resident (slot %d)", blk, c->blk_slot[blk]);
+    for (uint32_t s = 0; s < c->nslots; ++s)
+        if (s != slot && c->slot_staged[s] == (int32_t)blk) c->slot_staged[s] = -1;   // newest staging wins
+    if ((st = start_copy(c, slot, blk, beg, csr)) != GW_OK) return st;
+    c->slot_staged[slot] = (int32_t)blk;
+    return GW_OK;
+}
+
+gw_status gw_commit_block(gw_ctx* c, uint32_t slot)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (slot >= c->nslots) return fail(c, GW_ERR_ARG, "slot %u out of range (%u slots)", slot, c->nslots);
+    const int32_t blk = c->slot_staged[slot];
+    if (blk < 0) return fail(c, GW_ERR_STATE, "slot %u holds no staged block", slot);
+    if (c->blk_slot[blk] >= 0) return fail(c, GW_ERR_STATE, "block %d became resident in slot %d meanwhile", blk, c->blk_slot[blk]);
+    c->slot_staged[slot] = -1;
+    make_resident(c, slot, (uint32_t)blk);
+    return GW_OK;
+}
+
+gw_status gw_staged_block(gw_ctx* c, uint32_t slot, int32_t* blk, uint32_t* ready)
+{
+    if (!c || !blk) return fail(c, GW_ERR_ARG, "null argument");
+    if (slot >= c->nslots) return fail(c, GW_ERR_ARG, "slot %u out of range (%u slots)", slot, c->nslots);
+    *blk = c->slot_staged[slot];
+    if (ready) {
+        const hipError_t e = *blk >= 0 ? hipEventQuery(c->slot_ev1[slot]) : hipSuccess;
+        if (e != hipSuccess && e != hipErrorNotReady) return fail(c, GW_ERR_HIP, "copy event: %s", hipGetErrorString(e));
+        *ready = e == hipSuccess ? 1u : 0u;
+    }
     return GW_OK;
 }
 
@@ -425,6 +533,7 @@ gw_status gw_evict(gw_ctx* c, uint32_t slot)
     if (slot >= c->nslots) return fail(c, GW_ERR_ARG, "slot %u out of range", slot);
     if (c->slot_blk[slot] >= 0) block_nonresident(c, (uint32_t)c->slot_blk[slot]);
     c->slot_blk[slot] = -1;
+    c->slot_staged[slot] = -1;
     return GW_OK;
 }
 
@@ -432,6 +541,7 @@ gw_status gw_wait_uploads(gw_ctx* c)
 {
     if (!c) return fail(c, GW_ERR_ARG, "null context");
     HIPCHK(c, hipStreamSynchronize(c->s_copy));
+    for (uint32_t s = 0; s < c->nslots; ++s) collect_copy_time(c, s, true);
     return GW_OK;
 }
 
@@ -573,41 +683,60 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
     if (e == hipSuccess) e = hipStreamSynchronize(c->s_compute);
     if (d_src) (void)hipFree(d_src);
     if (e != hipSuccess) return fail(c, GW_ERR_HIP, "walker init kernel failed: %s", hipGetErrorString(e));
-    return GW_OK;
+    c->w_valid = false;
+    return ensure_buckets(c);   // initial bucket matrix and unfinished count
 }
 
 // Worklist + (optional) bucket pass on the compute stream; returns nwork and left.
-static gw_status run_filter(gw_ctx* c, bool buckets, bool worklist, uint32_t* nwork, uint64_t* left, float* ms)
+// Enqueues the bucket pass over all walkers on the compute stream: the matrix of
+// (prev_blk, cur_blk) counts of unfinished walkers into d_bucket and, on the host, into
+// the pinned h_bucket (valid after the stream is synchronized).
+static gw_status enqueue_bucket_pass(gw_ctx* c)
 {
+    const uint64_t nn = (uint64_t)c->nblocks * c->nblocks;
     HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
-    HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
-    if (buckets) HIPCHK(c, launch_fill_u32(c->d_bucket, 0, (uint64_t)c->nblocks * c->nblocks, c->s_compute));
+    HIPCHK(c, launch_fill_u32((uint32_t*)(c->d_cnt + 6), 0, 2, c->s_compute));
+    HIPCHK(c, launch_fill_u32(c->d_bucket, 0, nn, c->s_compute));
     FilterArgs f{};
     f.blocks = c->d_blocks;
     f.nblocks = c->nblocks;
     f.nwalkers = c->nwalkers;
     f.meta = c->d_meta;
     f.bpair = c->d_bpair;
-    f.worklist = c->d_work;   // a bucket-only pass still compacts; the list is ignored
+    f.worklist = c->d_work;   // the pass also compacts the runnable walkers; unused here
     f.nwork = c->d_misc;
-    f.bucket = buckets ? c->d_bucket : nullptr;
+    f.bucket = c->d_bucket;
     f.left = c->d_cnt + 6;
     HIPCHK(c, hipEventRecord(c->ev_f0, c->s_compute));
-    (void)worklist;
     HIPCHK(c, launch_filter(f, c->ncu * 8, c->s_compute));
     HIPCHK(c, hipEventRecord(c->ev_f1, c->s_compute));
-    HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_misc, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->s_compute));
-    HIPCHK(c, hipMemcpyAsync(c->h_cnt + 4, c->d_cnt + 6, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->s_compute));
+    HIPCHK(c, hipMemcpyAsync(c->h_bucket, c->d_bucket, nn * 4, hipMemcpyDeviceToHost, c->s_compute));
+    HIPCHK(c, hipMemcpyAsync(c->h_cnt + 8, c->d_cnt + 6, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->s_compute));
+    return GW_OK;
+}
+
+// Makes the cached bucket matrix current (one pass unless the last run left it valid).
+static gw_status ensure_buckets(gw_ctx* c)
+{
+    if (c->w_valid) return GW_OK;
+    gw_status st;
+    if ((st = push_block_table(c)) != GW_OK) return st;
+    if ((st = enqueue_bucket_pass(c)) != GW_OK) return st;
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
-    *nwork = ((uint32_t*)c->h_cnt)[0];
-    *left = c->h_cnt[4];
-    HIPCHK(c, hipEventElapsedTime(ms, c->ev_f0, c->ev_f1));
+    c->unfinished = c->h_cnt[8];
+    c->w_valid = true;
     return GW_OK;
 }
 
 static gw_status push_block_table(gw_ctx* c)
 {
-    HIPCHK(c, hipStreamWaitEvent(c->s_compute, c->ev_copy, 0));
+    // kernels read resident slots only after their copies (staged copies are not waited)
+    for (uint32_t s = 0; s < c->nslots; ++s)
+        if (c->slot_wait[s]) {
+            HIPCHK(c, hipStreamWaitEvent(c->s_compute, c->slot_ev1[s], 0));
+            c->slot_wait[s] = 0;
+        }
     memcpy(c->h_blocks_pinned, c->hblk.data(), sizeof(DevBlock) * c->nblocks);
     HIPCHK(c, hipMemcpyAsync(c->d_blocks, c->h_blocks_pinned, sizeof(DevBlock) * c->nblocks, hipMemcpyHostToDevice,
                              c->s_compute));
@@ -639,71 +768,78 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
 
     gw_status st;
     if ((st = push_block_table(c)) != GW_OK) return st;
-    uint32_t nwork = 0;
-    uint64_t left = 0;
+    c->w_valid = false;
+    // The whole walker array is the work list: the kernel skips finished walkers and,
+    // unless every block is readable, walkers whose (prev, cur) blocks are not both
+    // resident -- a coalesced 8-byte read per walker instead of a compaction pass and a
+    // host round trip.
+    bool all_res = true, mapped = false;
+    for (uint32_t b = 0; b < c->nblocks; ++b) {
+        all_res = all_res && c->hblk[b].resident != kNotResident;
+        mapped = mapped || c->hblk[b].resident == kMapped;
+    }
+    uint64_t nwork = c->nwalkers;
+    if (cap && nwork > cap) nwork = cap;
     float fms = 0.f, kms = 0.f;
-    // with every block readable the whole walker array is the worklist (the kernel skips
-    // finished walkers), so no compaction pass is needed
-    bool all_res = true;
-    for (uint32_t b = 0; b < c->nblocks && all_res; ++b) all_res = c->hblk[b].resident != 0;
-    const bool identity = all_res && !(l.flags & GW_RUN_BUCKETS);
-    if (identity) {
-        HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
-        nwork = (uint32_t)c->nwalkers;
-    } else {
-        if ((st = run_filter(c, (l.flags & GW_RUN_BUCKETS) != 0, true, &nwork, &left, &fms)) != GW_OK) return st;
-    }
-    if (cap && nwork > cap) nwork = (uint32_t)cap;
+    HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
     HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
-    if (nwork) {
-        WalkArgs a{};
-        a.blocks = c->d_blocks;
-        a.vcut = c->d_vcut;
-        a.nblocks = c->nblocks;
-        a.ntypes = c->wp.ntypes;
-        a.seed_lo = (uint32_t)c->wp.seed;
-        a.seed_hi = (uint32_t)(c->wp.seed >> 32);
-        a.type0 = c->dt[0];
-        a.type1 = c->dt[1];
-        a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta; a.bpair = c->d_bpair;
-        a.hash = c->d_hash; a.paths = c->d_paths; a.path_stride = c->path_stride; a.visits = c->d_visits;
-        a.worklist = identity ? nullptr : c->d_work;
-        a.nwork = nwork;
-        a.queue = c->d_misc + 1;
-        a.counters = c->d_cnt;
-        if (l.flags & GW_RUN_TRACE) {
-            HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
-            a.block_steps = c->d_trace;
-        }
-        // the flattened variant reads u32s through aligned 8-byte words, which is only
-        // safe inside padded HBM slots: zero-copy (host-mapped) blocks use variant 0
-        if (l.variant == 1)
-            for (uint32_t b = 0; b < c->nblocks; ++b)
-                if (c->hblk[b].resident && c->blk_slot[b] < 0) { l.variant = 0; break; }
-        // persistent grid: as many workgroups as the CUs hold at once (occupancy), or
-        // the caller's waves per CU
-        const bool one_blk = c->nblocks == 1;
-        uint64_t grid = l.waves_per_cu ? (uint64_t)c->ncu * l.waves_per_cu * 64 / l.block
-                                       : (uint64_t)c->ncu * walk_blocks_per_cu(one_blk, l.variant, l.block);
-        const uint64_t need = ((uint64_t)nwork + l.block - 1) / l.block;
-        if (grid > need) grid = need;
-        if (grid == 0) grid = 1;
-        {
-            const uint64_t waves = grid * (l.block / 64);
-            uint64_t ch = (uint64_t)nwork / (waves * 8 + 1);
-            ch = ch < 64 ? 64 : (ch > 1024 ? 1024 : ch);
-            a.chunk = (uint32_t)(ch / 64 * 64);
-        }
-        HIPCHK(c, hipEventRecord(c->ev_k0, c->s_compute));
-        HIPCHK(c, launch_walk(a, one_blk, l.variant, (uint32_t)grid, l.block, c->s_compute));
-        HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));
-        HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                 c->s_compute));
-        HIPCHK(c, hipStreamSynchronize(c->s_compute));
-        HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
-    } else {
-        memset(c->h_cnt, 0, 8 * sizeof(unsigned long long));
+    WalkArgs a{};
+    a.blocks = c->d_blocks;
+    a.vcut = c->d_vcut;
+    a.nblocks = c->nblocks;
+    a.ntypes = c->wp.ntypes;
+    a.seed_lo = (uint32_t)c->wp.seed;
+    a.seed_hi = (uint32_t)(c->wp.seed >> 32);
+    a.type0 = c->dt[0];
+    a.type1 = c->dt[1];
+    a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta; a.bpair = c->d_bpair;
+    a.hash = c->d_hash; a.paths = c->d_paths; a.path_stride = c->path_stride; a.visits = c->d_visits;
+    a.worklist = nullptr;
+    a.scan = all_res ? 0u : 1u;
+    a.zc_count = mapped ? 1u : 0u;
+    a.nwork = (uint32_t)nwork;
+    a.queue = c->d_misc + 1;
+    a.counters = c->d_cnt;
+    if (l.flags & GW_RUN_TRACE) {
+        HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
+        a.block_steps = c->d_trace;
     }
+    // the flattened variant reads u32s through aligned 8-byte words, which is only
+    // safe inside padded HBM slots: zero-copy (host-mapped) blocks use variant 0
+    if (l.variant == 1)
+        for (uint32_t b = 0; b < c->nblocks; ++b)
+            if (c->hblk[b].resident == kMapped) { l.variant = 0; break; }
+    // persistent grid: as many workgroups as the CUs hold at once (occupancy), or the
+    // caller's waves per CU
+    const bool one_blk = c->nblocks == 1;
+    uint64_t grid = l.waves_per_cu ? (uint64_t)c->ncu * l.waves_per_cu * 64 / l.block
+                                   : (uint64_t)c->ncu * walk_blocks_per_cu(one_blk, l.variant, l.block);
+    const uint64_t need = (nwork + l.block - 1) / l.block;
+    if (grid > need) grid = need;
+    if (grid == 0) grid = 1;
+    {
+        const uint64_t waves = grid * (l.block / 64);
+        uint64_t ch = nwork / (waves * 8 + 1);
+        ch = ch < 64 ? 64 : (ch > 1024 ? 1024 : ch);
+        a.chunk = (uint32_t)(ch / 64 * 64);
+    }
+    HIPCHK(c, hipEventRecord(c->ev_k0, c->s_compute));
+    HIPCHK(c, launch_walk(a, one_blk, l.variant, (uint32_t)grid, l.block, c->s_compute));
+    HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));
+    HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->s_compute));
+    const bool buckets = (l.flags & GW_RUN_BUCKETS) != 0;
+    if (buckets) {
+        // bucket matrix of the walkers left, read back with the counters (one host wait)
+        if ((st = enqueue_bucket_pass(c)) != GW_OK) return st;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
+    if (buckets) {
+        HIPCHK(c, hipEventElapsedTime(&fms, c->ev_f0, c->ev_f1));
+        c->w_valid = true;
+    }
+    for (uint32_t sl = 0; sl < c->nslots; ++sl) collect_copy_time(c, sl, false);
     gw_run_stats s{};
     s.steps = c->h_cnt[0];
     s.trials = c->h_cnt[1];
@@ -711,9 +847,12 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     s.mix_steps = c->h_cnt[3];
     s.walkers_ended = c->h_cnt[4];
     s.walkers_run = c->h_cnt[4] + c->h_cnt[5];
-    s.walkers_left = identity ? c->h_cnt[5] : left - s.walkers_ended;
+    c->unfinished = buckets ? c->h_cnt[8] : c->unfinished - std::min<uint64_t>(c->unfinished, s.walkers_ended);
+    s.walkers_left = c->unfinished;
     s.kernel_ms = kms;
     s.filter_ms = fms;
+    s.zc_touches = c->h_cnt[7];
+    c->tot.zc_touches += s.zc_touches;
     c->tot.steps += s.steps;
     c->tot.trials += s.trials;
     c->tot.probes += s.probes;
@@ -755,6 +894,7 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, gw_launch* best, double* best_
         return hipStreamSynchronize(c->s_compute) == hipSuccess;
     };
     const gw_counters tot0 = c->tot;
+    const uint64_t unfinished0 = c->unfinished;
     static const gw_launch cand[] = {{64, 0, 0, 0},  {128, 0, 0, 0}, {256, 0, 0, 0}, {512, 0, 0, 0},
                                      {1024, 0, 0, 0}, {64, 0, 1, 0},  {128, 0, 1, 0}, {256, 0, 1, 0}};
     double bms = 1e300;
@@ -765,12 +905,15 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, gw_launch* best, double* best_
             gw_run_stats rs{};
             if ((st = run_impl(c, L, &rs, sample)) != GW_OK) break;
             t = std::min(t, rs.kernel_ms);
+            c->unfinished = unfinished0;
             if (!restore()) st = fail(c, GW_ERR_HIP, "autotune restore");
         }
         if (st != GW_OK) break;
         if (t < bms) { bms = t; bl = L; }
     }
     c->tot = tot0;
+    c->unfinished = unfinished0;
+    c->w_valid = false;
     cleanup();
     if (st != GW_OK) return st;
     c->tuned = bl;
@@ -789,14 +932,8 @@ gw_status gw_bucket_counts(gw_ctx* c, uint64_t* W, uint64_t n)
                             (unsigned long long)nn);
     (void)hipSetDevice(c->device);
     gw_status st;
-    if ((st = push_block_table(c)) != GW_OK) return st;
-    uint32_t nwork;
-    uint64_t left;
-    float ms;
-    if ((st = run_filter(c, true, false, &nwork, &left, &ms)) != GW_OK) return st;
-    std::vector<uint32_t> tmp(nn);
-    HIPCHK(c, hipMemcpy(tmp.data(), c->d_bucket, nn * 4, hipMemcpyDeviceToHost));
-    for (uint64_t i = 0; i < nn; ++i) W[i] = tmp[i];
+    if ((st = ensure_buckets(c)) != GW_OK) return st;
+    for (uint64_t i = 0; i < nn; ++i) W[i] = c->h_bucket[i];
     return GW_OK;
 }
 
@@ -818,11 +955,8 @@ gw_status gw_block_features(gw_ctx* c, const float* age, float* feats, uint64_t 
     if (n < (uint64_t)c->nblocks * kNumFeatures) return fail(c, GW_ERR_ARG, "feature buffer too small");
     (void)hipSetDevice(c->device);
     gw_status st;
+    if ((st = ensure_buckets(c)) != GW_OK) return st;
     if ((st = push_block_table(c)) != GW_OK) return st;
-    uint32_t nwork;
-    uint64_t left;
-    float ms;
-    if ((st = run_filter(c, true, false, &nwork, &left, &ms)) != GW_OK) return st;
     if (age) HIPCHK(c, hipMemcpyAsync(c->d_age, age, c->nblocks * 4ull, hipMemcpyHostToDevice, c->s_compute));
     HIPCHK(c, launch_features(c->d_bucket, c->nblocks, c->d_blocks, age ? c->d_age : nullptr, c->d_feat, c->s_compute));
     HIPCHK(c, hipMemcpyAsync(feats, c->d_feat, (uint64_t)c->nblocks * kNumFeatures * 4, hipMemcpyDeviceToHost,
@@ -835,11 +969,11 @@ gw_status gw_walkers_left(gw_ctx* c, uint64_t* left)
 {
     if (!c || !left) return fail(c, GW_ERR_ARG, "null argument");
     if (!c->nwalkers) return fail(c, GW_ERR_STATE, "no walkers");
+    (void)hipSetDevice(c->device);
     gw_status st;
-    if ((st = push_block_table(c)) != GW_OK) return st;
-    uint32_t nwork;
-    float ms;
-    return run_filter(c, false, false, &nwork, left, &ms);
+    if ((st = ensure_buckets(c)) != GW_OK) return st;
+    *left = c->unfinished;
+    return GW_OK;
 }
 
 static gw_status readback(gw_ctx* c, void* dst, const void* src, uint64_t bytes, const char* what)

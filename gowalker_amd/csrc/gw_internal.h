@@ -16,6 +16,8 @@ constexpr uint32_t kHopMask = 0xFFFFFu;
 constexpr uint32_t kStatusShift = 20, kTypeShift = 24;
 constexpr uint32_t kMaxLength = kHopMask;
 enum : uint32_t { kActive = 0, kDoneLength = 1, kDoneTerm = 2, kDoneDeadEnd = 3, kDoneTrialCap = 4 };
+// DevBlock::resident: 0 not readable, 1 in an HBM slot, 2 host-mapped (zero-copy reads)
+enum : uint32_t { kNotResident = 0, kInSlot = 1, kMapped = 2 };
 
 // One block as the kernels see it: the block's own beg (nverts+1 global offsets) and
 // csr arrays, wherever they live (an HBM slot, a caller's device graph, or pinned host
@@ -61,6 +63,8 @@ struct WalkArgs {
     // work
     const uint32_t* worklist; // walker ids; nullptr = identity (finished walkers are skipped)
     uint32_t nwork;
+    uint32_t scan;            // identity list: also skip walkers whose blocks are not both resident
+    uint32_t zc_count;        // some block is host-mapped: count zero-copy reads (counters[7])
     uint32_t* queue;          // work-fetch counter, zero at launch
     uint32_t chunk;           // walkers a wavefront takes per queue atomic (multiple of 64)
     unsigned long long* counters;  // steps, trials, probes, mix, ended, left

@@ -27,7 +27,7 @@ typedef unsigned __int128 u128;
 // per-lane counters (u32: a lane's share of one launch stays far below 2^32); summed in
 // u64 across the wavefront at the end
 struct Cnt {
-    uint32_t steps, trials, probes, mix, ended, left;
+    uint32_t steps, trials, probes, mix, ended, left, zc;
 };
 
 __device__ __forceinline__ uint32_t bitlen64(uint64_t n) { return n ? 64u - (uint32_t)__clzll(n) : 0u; }
@@ -209,6 +209,12 @@ __device__ __forceinline__ Row load_row(const DevBlock& b, uint32_t v)
     return Row{ b.csr + h, t - h };
 }
 
+// Both blocks of a walker's (prev, cur) pair are readable (HBM slot or host-mapped).
+__device__ __forceinline__ bool runnable(const DevBlock* __restrict__ blocks, uint32_t pair)
+{
+    return blocks[pair & 0xFFFFu].resident && blocks[pair >> 16].resident;
+}
+
 // Wave-local work queue: a wavefront takes a chunk of walker indices with one atomic and
 // hands them to its lanes as they free up, so the shared counter sees one atomic per
 // chunk instead of one per walker.  The host sizes the chunk (64..1024) so that every
@@ -252,7 +258,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
         for (uint32_t i = threadIdx.x; i < kTraceLdsBlocks; i += blockDim.x) trace[i] = 0;
         __syncthreads();
     }
-    Cnt c = {0, 0, 0, 0, 0, 0};
+    Cnt c = {0, 0, 0, 0, 0, 0, 0};
     const uint32_t lane = __lane_id();
     DevBlock one;
     if (ONE) one = a.blocks[0];
@@ -272,7 +278,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
                 const uint32_t m = idx < a.nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
                 if (idx >= a.nwork) {
                     exhausted = true;
-                } else if (((m >> kStatusShift) & 0xFu) == kActive) {
+                } else if (((m >> kStatusShift) & 0xFu) == kActive &&
+                           (!a.scan || runnable(a.blocks, a.bpair[a.worklist ? a.worklist[idx] : idx]))) {
                     w = a.worklist ? a.worklist[idx] : idx;
                     cur = a.cur[w];
                     prev = a.prev[w];
@@ -306,8 +313,15 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
         if (status == kActive && rv.deg == 0) status = kDoneDeadEnd;
         if (status == kActive) {
             uint32_t x = 0;
+            const uint32_t t0 = c.trials, p0 = c.probes;
             const bool ok = tp.algo == 0 ? n2v_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c)
                                          : sopr_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c);
+            if (a.zc_count) {
+                // candidates come from cur's row, membership probes from prev's row
+                const bool mv = (ONE ? one.resident : a.blocks[bc].resident) == kMapped;
+                const bool mu = (ONE ? one.resident : a.blocks[bp].resident) == kMapped;
+                c.zc += (mv ? 1u + (c.trials - t0) : 0u) + (mu ? c.probes - p0 : 0u);
+            }
             if (!ok) {
                 status = kDoneTrialCap;
             } else {
@@ -354,6 +368,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
     }
     const uint64_t s0 = wave_sum(c.steps), s1 = wave_sum(c.trials), s2 = wave_sum(c.probes);
     const uint64_t s3 = wave_sum(c.mix), s4 = wave_sum(c.ended), s5 = wave_sum(c.left);
+    const uint64_t s7 = a.zc_count ? wave_sum(c.zc) : 0;
     if (lane == 0) {
         atomicAdd(a.counters + 0, (unsigned long long)s0);
         atomicAdd(a.counters + 1, (unsigned long long)s1);
@@ -361,6 +376,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
         atomicAdd(a.counters + 3, (unsigned long long)s3);
         atomicAdd(a.counters + 4, (unsigned long long)s4);
         atomicAdd(a.counters + 5, (unsigned long long)s5);
+        if (s7) atomicAdd(a.counters + 7, (unsigned long long)s7);
     }
 }
 
@@ -726,7 +742,7 @@ __global__ void __launch_bounds__(256) walk_flat_kernel(WalkArgs a)
         for (uint32_t i = threadIdx.x; i < kTraceLdsBlocks; i += blockDim.x) trace[i] = 0;
         __syncthreads();
     }
-    Cnt c = {0, 0, 0, 0, 0, 0};
+    Cnt c = {0, 0, 0, 0, 0, 0, 0};
     const uint32_t lane = __lane_id();
     DevBlock one;
     if (ONE) one = a.blocks[0];
@@ -744,7 +760,8 @@ __global__ void __launch_bounds__(256) walk_flat_kernel(WalkArgs a)
                 const uint32_t m = idx < a.nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
                 if (idx >= a.nwork) {
                     exhausted = true;
-                } else if (((m >> kStatusShift) & 0xFu) == kActive) {
+                } else if (((m >> kStatusShift) & 0xFu) == kActive &&
+                           (!a.scan || runnable(a.blocks, a.bpair[a.worklist ? a.worklist[idx] : idx]))) {
                     L.w = a.worklist ? a.worklist[idx] : idx;
                     L.cur = a.cur[L.w];
                     L.prev = a.prev[L.w];
@@ -883,7 +900,7 @@ __global__ void __launch_bounds__(256) features_kernel(const uint32_t* __restric
         const double r = W[(uint64_t)b * nb + j], c = W[(uint64_t)j * nb + b];
         row += r;
         col += c;
-        if (j != b && blocks[j].resident) {
+        if (j != b && blocks[j].resident == kInSlot) {
             res += r + c;
             if (mn < 0 || r + c < mn) mn = r + c;
         }
@@ -896,7 +913,7 @@ __global__ void __launch_bounds__(256) features_kernel(const uint32_t* __restric
     o[1] = (float)log1p(col);
     o[2] = (float)log1p(diag);
     o[3] = (float)log1p(res);
-    o[4] = d.resident ? 1.f : 0.f;
+    o[4] = d.resident == kInSlot ? 1.f : 0.f;
     o[5] = (float)log1p(mn < 0 ? 0.0 : mn);
     o[6] = (float)log1p(res + diag);
     o[7] = age ? age[b] / 64.f : 0.f;

@@ -1,11 +1,12 @@
 // engine.hpp -- the out-of-core walk driver: GOWalker's graph_engine::pipe_run and
 // pipeline (engine/engine.hpp:406-638) on top of libgw.so.
 //
-// Per iteration: read the bucket matrix of unfinished walkers from the device, let the
-// scheduler change at most one resident block, place the wanted blocks into HBM slots
-// (copyblocks, engine/schedule.hpp:144-197), upload what is missing on the copy stream,
-// and run every walker whose (prev, cur) blocks are resident until it finishes or steps
-// out.  When the scheduler finds nothing worth loading, the remaining walkers finish by
+// Per iteration: the bucket matrix of unfinished walkers (left by the previous run, no
+// extra pass), the scheduler's choice of resident blocks, placement into HBM slots
+// (copyblocks, engine/schedule.hpp:144-197), uploads on the copy stream, optional
+// prefetch of further blocks into free slots (copies that overlap the run), and one run
+// of every walker whose (prev, cur) blocks are resident until it finishes or steps out.
+// When the scheduler finds nothing worth loading, the remaining walkers finish by
 // reading non-resident blocks from pinned host memory (the zero-copy tail of
 // engine/engine.hpp:459-515, 593-634).  Walks do not depend on the schedule: every draw
 // is keyed on (seed, walker, hop, draw), so only bytes moved and time change.
@@ -50,9 +51,9 @@ struct WalkConfig {
 
 struct RunReport {
     uint64_t steps = 0, trials = 0, probes = 0, mix_steps = 0;
-    uint64_t h2d_bytes = 0, uploads = 0, iterations = 0;
-    uint64_t zero_copy_runs = 0, zero_copy_steps = 0;
-    double wall_s = 0, kernel_ms = 0, filter_ms = 0, h2d_ms = 0;
+    uint64_t h2d_bytes = 0, uploads = 0, staged = 0, iterations = 0;
+    uint64_t zero_copy_runs = 0, zero_copy_steps = 0, zc_touches = 0;
+    double wall_s = 0, kernel_ms = 0, filter_ms = 0, h2d_ms = 0, sched_ms = 0;
     uint32_t nslots = 0, nblocks = 0;
 };
 
@@ -90,7 +91,8 @@ public:
 
     RunReport run(const WalkConfig& wc, Scheduler& sched)
     {
-        auto t0 = std::chrono::steady_clock::now();
+        using clk = std::chrono::steady_clock;
+        auto t0 = clk::now();
         RunReport r;
         const uint32_t nb = nblocks();
         r.nblocks = nb;
@@ -110,19 +112,36 @@ public:
         sched.buckets.clear();
         ck(gw_counters_get(ctx_, &c0), ctx_, "gw_counters_get");
         std::vector<uint64_t> W((uint64_t)nb * nb);
+        std::vector<uint8_t> staged(nb, 0);
+        std::vector<uint32_t> pf;
         for (uint64_t it = 0;; ++it) {
             ck(gw_bucket_counts(ctx_, W.data(), W.size()), ctx_, "gw_bucket_counts");
             uint64_t left = 0;
             for (uint64_t x : W) left += x;
             if (!left) break;
-            SchedInput in{W.data(), nb, nslots_, cfg_.zero_threshold, blocks_.data(), it};
+            auto ts = clk::now();
+            for (uint32_t s = 0; s < nslots_; ++s) {
+                const int b = slots_.slot_stg[s];
+                if (b < 0) continue;
+                int32_t sb = -1;
+                uint32_t ready = 0;
+                ck(gw_staged_block(ctx_, s, &sb, &ready), ctx_, "gw_staged_block");
+                staged[b] = ready ? 2 : 1;
+            }
+            SchedInput in{W.data(), nb, nslots_, cfg_.zero_threshold, blocks_.data(), it, staged.data()};
             IterationOutcome out;
             out.iteration = it;
             gw_run_stats st{};
             gw_launch launch = cfg_.launch;
-            launch.flags |= sched.run_flags();
+            launch.flags |= sched.run_flags() | GW_RUN_BUCKETS;
             if (sched.schedule(in)) {
                 uint64_t bytes = 0;
+                // staged copies the scheduler wants become resident (the run waits for them)
+                for (uint32_t b : sched.buckets)
+                    if (slots_.blk_stg[b] >= 0) {
+                        ck(gw_commit_block(ctx_, (uint32_t)slots_.blk_stg[b]), ctx_, "gw_commit_block");
+                        slots_.commit(b);
+                    }
                 for (uint32_t b : sched.buckets) {
                     int s = slots_.place(b, sched.buckets);
                     if (s == -2) throw GwError(GW_ERR_STATE, "scheduler wants more blocks than slots");
@@ -134,9 +153,27 @@ public:
                     out.loaded.push_back(b);
                     r.uploads++;
                 }
+                // prefetch into slots that hold neither a wanted nor a staged block
+                pf.clear();
+                std::vector<uint32_t> keep = sched.buckets;
+                for (uint32_t b = 0; b < nb; ++b)
+                    if (slots_.blk_stg[b] >= 0) keep.push_back(b);
+                sched.prefetch(in, slots_.free_slots(keep), pf);
+                for (uint32_t b : pf) {
+                    const int s = slots_.stage(b, keep);
+                    if (s < 0) continue;
+                    keep.push_back(b);
+                    const gw_block_desc& d = blocks_[b];
+                    ck(gw_stage_block(ctx_, (uint32_t)s, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,
+                       "gw_stage_block");
+                    r.staged++;
+                }
+                for (uint32_t b = 0; b < nb; ++b) staged[b] = slots_.blk_stg[b] >= 0 ? 1 : 0;
+                r.sched_ms += std::chrono::duration<double, std::milli>(clk::now() - ts).count();
                 ck(gw_run(ctx_, &launch, &st), ctx_, "gw_run");
                 out.h2d_bytes = bytes;
             } else {
+                r.sched_ms += std::chrono::duration<double, std::milli>(clk::now() - ts).count();
                 if (!cfg_.zero_copy_tail)
                     throw GwError(GW_ERR_STATE, "walkers remain but no block is worth loading (zero-copy tail off)");
                 for (uint32_t b = 0; b < nb; ++b) {
@@ -148,10 +185,13 @@ public:
                 ck(gw_unmap_host_blocks(ctx_), ctx_, "gw_unmap_host_blocks");
                 r.zero_copy_runs++;
                 r.zero_copy_steps += st.steps;
+                r.zc_touches += st.zc_touches;
             }
             out.steps = st.steps;
             out.kernel_ms = st.kernel_ms;
+            auto tf = clk::now();
             sched.feedback(in, out);
+            r.sched_ms += std::chrono::duration<double, std::milli>(clk::now() - tf).count();
             r.iterations++;
             r.steps += st.steps;
             r.trials += st.trials;
@@ -160,16 +200,18 @@ public:
             r.kernel_ms += st.kernel_ms;
             r.filter_ms += st.filter_ms;
             if (cfg_.verbose)
-                std::fprintf(stderr, "[engine] it %llu: %s, %zu resident, steps %llu, walkers run %llu, left %llu\n",
-                             (unsigned long long)it, out.loaded.empty() && st.walkers_run ? "no upload" : "upload",
-                             sched.buckets.size(), (unsigned long long)st.steps, (unsigned long long)st.walkers_run,
-                             (unsigned long long)st.walkers_left);
+                std::fprintf(stderr, "[engine] it %llu: %zu resident, %zu loaded, %zu staged, steps %llu, walkers run %llu, "
+                             "left %llu, kernel %.2f ms\n",
+                             (unsigned long long)it, sched.buckets.size(), out.loaded.size(), pf.size(),
+                             (unsigned long long)st.steps, (unsigned long long)st.walkers_run,
+                             (unsigned long long)st.walkers_left, st.kernel_ms);
         }
+        ck(gw_wait_uploads(ctx_), ctx_, "gw_wait_uploads");
         gw_counters c1;
         ck(gw_counters_get(ctx_, &c1), ctx_, "gw_counters_get");
         r.h2d_bytes = c1.h2d_bytes - c0.h2d_bytes;
         r.h2d_ms = c1.h2d_ms - c0.h2d_ms;
-        r.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        r.wall_s = std::chrono::duration<double>(clk::now() - t0).count();
         return r;
     }
 

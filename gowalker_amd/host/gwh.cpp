@@ -69,10 +69,14 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         report->h2d_ms = r.h2d_ms;
         report->nslots = r.nslots;
         report->nblocks = r.nblocks;
+        report->staged = r.staged;
+        report->zc_touches = r.zc_touches;
+        report->sched_ms = r.sched_ms;
         if (ls) {
             report->learned_decisions = ls->learned_decisions();
             report->fallbacks = ls->fallbacks();
             report->final_loss = ls->last_loss();
+            report->tail_decisions = ls->tail_decisions();
         }
         if (hashes) ck(gw_read_hashes(engine.ctx(), hashes, nwalkers), engine.ctx(), "gw_read_hashes");
         if (visits) ck(gw_read_visits(engine.ctx(), visits, entries - 1), engine.ctx(), "gw_read_visits");

@@ -1,17 +1,23 @@
 // learned.hpp -- learned block scheduler: a 12-64-64-1 MLP (BF16 weights, FP32
-// accumulation, trained on the GPU by libgw.so's kernels) scores blocks by the walk steps
-// they will serve in the next iteration if resident.  It is trained online on the
-// block-access trace the engine records (steps per block per iteration, gw_trace_read).
+// accumulation, trained on the GPU by libgw.so's kernels) predicts, for every block, the
+// walk steps it serves in the next run if it is resident (log1p scale).  It is trained
+// online on the block-access trace the engine records (steps per current block per run,
+// gw_trace_read), with the features each block had when the decision was taken.
 //
-// Decision, given the scores of every block:
-//   load  c* = the non-resident block with the highest score among those that have
-//              walkers able to run right now (plain add score > zero_threshold);
-//   evict j* = (cache full) the resident block with the lowest score; swap only when
-//              score(c*) > score(j*).
+// Decisions, given the predictions y(b):
+//   fill   (free slots left): commit staged blocks whose copies are done, else the best
+//          staged one (the run waits for it), else upload the best candidate; then stage
+//          the next-best blocks into the free slots so their copies overlap the run;
+//   swap   (cache full): load the candidate c with the highest y among blocks whose
+//          walkers could run now, evict the resident block with the lowest y;
+//   tail   when y(c) says the upload would serve fewer steps than its bytes buy in
+//          zero-copy reads (steps x zc_bytes_per_step < block bytes), finish the walk by
+//          zero-copy instead of uploading (the reference's fixed walker-count threshold,
+//          apps/Node2vec.cu:100-104, is the plain rule's version of this decision).
 // The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback and judge: it
 // decides while the model has too few samples, and it takes over for a cool-down period
-// whenever the learned decisions' realised steps per uploaded byte fall below the plain
-// decisions' running average.  Walks stay bit-identical either way (counter-based RNG).
+// whenever learned swaps realise fewer steps per uploaded byte than plain swaps.  Walks
+// stay bit-identical either way (counter-based RNG).
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -26,13 +32,15 @@ namespace gwh {
 struct LearnedConfig {
     uint32_t h1 = 64, h2 = 64;
     uint64_t seed = 7;
-    uint32_t min_samples = 256;    // plain decisions until this many trace samples
-    uint32_t train_every = 2;      // iterations between training rounds
-    uint32_t epochs = 16;          // Adam steps per training round
-    uint32_t batch = 1024;
-    uint32_t max_samples = 16384;  // ring buffer of trace samples
+    uint32_t min_samples = 96;      // plain swap/tail decisions until this many trace samples
+    uint32_t train_steps = 4;       // Adam steps after every iteration
+    uint32_t batch = 512;
+    uint32_t max_samples = 16384;   // ring buffer of trace samples
     float lr = 2e-3f;
-    double judge_ratio = 0.8;      // learned yield/byte below ratio x plain's => fall back
+    double zc_bytes_per_step = 320; // PCIe bytes of one zero-copy step (about five 64 B reads)
+    double tail_margin = 1.0;       // zero-copy once steps x zc bytes x margin < block bytes
+    uint32_t max_stage = 64;        // blocks staged per iteration
+    double judge_ratio = 0.8;       // learned swap yield/byte below ratio x plain's => fall back
     uint32_t cooldown = 8;
 };
 
@@ -53,39 +61,60 @@ public:
     {
         const uint32_t nb = in.nblocks;
         if (age_.size() != nb) age_.assign(nb, 64.f);
-        feats_.assign((size_t)nb * kF, 0.f);
-        if (gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) != GW_OK) return plain_decide(in);
-        last_decision_learned_ = false;
-        if (!mlp_ || samples_ < cfg_.min_samples || cooldown_ > 0) {
+        prepare(in);
+        last_kind_ = kPlain;
+        std::vector<char> cached(nb, 0);
+        for (uint32_t b : buckets) cached[b] = 1;
+
+        if (buckets.size() < in.ncblock) {   // ---- fill: free slots remain
+            bool took = false;
+            for (uint32_t b = 0; b < nb && buckets.size() < in.ncblock; ++b)
+                if (!cached[b] && in.staged && in.staged[b] == 2) { buckets.push_back(b); cached[b] = 1; took = true; }
+            if (!took && in.staged) {
+                int best = -1;
+                for (uint32_t b = 0; b < nb; ++b)
+                    if (!cached[b] && in.staged[b] == 1 && (best < 0 || rank_[b] > rank_[best])) best = (int)b;
+                if (best >= 0) { buckets.push_back((uint32_t)best); cached[best] = 1; took = true; }
+            }
+            if (took) { last_kind_ = kFill; return true; }
+            const int c = best_candidate(cached);
+            if (c < 0) return plain_decide(in);
+            if (tail_now(in, (uint32_t)c)) return false;
+            buckets.push_back((uint32_t)c);
+            last_kind_ = kFill;
+            return true;
+        }
+
+        // ---- swap: the cache is full
+        if (!trained() || cooldown_ > 0) {
             if (cooldown_ > 0) --cooldown_;
             return plain_decide(in);
         }
-        pred_.assign(nb, 0.f);
-        if (gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) != GW_OK) return plain_decide(in);
-        std::vector<char> cached(nb, 0);
-        for (uint32_t b : buckets) cached[b] = 1;
-        int best = -1;
-        for (uint32_t c = 0; c < nb; ++c) {
-            if (cached[c]) continue;
-            // walkers that could run at once if c joined the resident set
-            uint64_t gain = in.W[(uint64_t)c * nb + c];
-            for (uint32_t j : buckets) gain += in.W[(uint64_t)c * nb + j] + in.W[(uint64_t)j * nb + c];
-            if (gain == 0 || gain <= in.zero_threshold) continue;
-            if (best < 0 || pred_[c] > pred_[best]) best = (int)c;
-        }
-        if (best < 0) return plain_decide(in);
-        if (buckets.size() < in.ncblock) {
-            buckets.push_back((uint32_t)best);
-        } else {
-            int victim = -1;
-            for (uint32_t p = 0; p < buckets.size(); ++p)
-                if (victim < 0 || pred_[buckets[p]] < pred_[buckets[victim]]) victim = (int)p;
-            if (!(pred_[best] > pred_[buckets[victim]])) return plain_decide(in);
-            buckets[victim] = (uint32_t)best;
-        }
-        last_decision_learned_ = true;
+        const int c = best_candidate(cached);
+        if (c < 0) return plain_decide(in);
+        if (tail_now(in, (uint32_t)c)) return false;
+        int victim = -1;
+        for (uint32_t p = 0; p < buckets.size(); ++p)
+            if (victim < 0 || rank_[buckets[p]] < rank_[buckets[victim]]) victim = (int)p;
+        if (!(rank_[c] > rank_[buckets[victim]])) return plain_decide(in);
+        buckets[victim] = (uint32_t)c;
+        last_kind_ = kLearnedSwap;
         ++learned_decisions_;
         return true;
+    }
+
+    void prefetch(const SchedInput& in, uint32_t free_slots, std::vector<uint32_t>& out) override
+    {
+        if (!free_slots || buckets.size() >= in.ncblock) return;
+        const uint32_t nb = in.nblocks;
+        std::vector<char> skip(nb, 0);
+        for (uint32_t b : buckets) skip[b] = 1;
+        std::vector<uint32_t> cand;
+        for (uint32_t b = 0; b < nb; ++b)
+            if (!skip[b] && !(in.staged && in.staged[b]) && demand_[b] > 0) cand.push_back(b);
+        std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return rank_[a] > rank_[b]; });
+        const uint32_t room = std::min<uint32_t>({free_slots, cfg_.max_stage, (uint32_t)(in.ncblock - buckets.size())});
+        for (uint32_t i = 0; i < cand.size() && i < room; ++i) out.push_back(cand[i]);
     }
 
     void feedback(const SchedInput& in, const IterationOutcome& out) override
@@ -93,19 +122,15 @@ public:
         const uint32_t nb = in.nblocks;
         std::vector<uint64_t> trace(nb, 0);
         if (gw_trace_read(ctx_, trace.data(), nb) != GW_OK) return;
-        // samples: (features at decision time, log1p(steps served while resident)) for
-        // every block that was resident during the iteration
-        std::vector<char> res(nb, 0);
-        for (uint32_t b : buckets) res[b] = 1;
-        for (uint32_t b = 0; b < nb; ++b) {
-            if (!res[b] || feats_.size() < (size_t)(b + 1) * kF) continue;
-            add_sample(&feats_[(size_t)b * kF], std::log1p((float)trace[b]));
-        }
+        // samples: (features at decision time, log1p(steps served)) for every block that
+        // was resident during the run
+        for (uint32_t b : buckets)
+            if (feats_.size() >= (size_t)(b + 1) * kF) add_sample(&feats_[(size_t)b * kF], std::log1p((float)trace[b]));
         for (uint32_t b = 0; b < nb; ++b) age_[b] = trace[b] ? 0.f : std::min(age_[b] + 1.f, 1024.f);
-        // judge: realised steps per uploaded byte, learned vs plain decisions
-        if (out.h2d_bytes) {
+        // judge: realised steps per uploaded byte of learned swaps vs plain swaps
+        if (out.h2d_bytes && (last_kind_ == kLearnedSwap || last_kind_ == kPlain)) {
             const double y = (double)out.steps / (double)out.h2d_bytes;
-            if (last_decision_learned_) {
+            if (last_kind_ == kLearnedSwap) {
                 learned_yield_ = learned_n_ ? 0.8 * learned_yield_ + 0.2 * y : y;
                 ++learned_n_;
                 if (plain_n_ >= 4 && learned_n_ >= 4 && learned_yield_ < cfg_.judge_ratio * plain_yield_) {
@@ -118,24 +143,79 @@ public:
                 ++plain_n_;
             }
         }
-        if (mlp_ && ++since_train_ >= cfg_.train_every && samples_ >= std::min<uint32_t>(cfg_.min_samples, 64)) {
-            since_train_ = 0;
-            train();
-        }
+        if (mlp_ && samples_ >= 32) train();
     }
 
     uint64_t learned_decisions() const { return learned_decisions_; }
     uint64_t fallbacks() const { return fallbacks_; }
+    uint64_t tail_decisions() const { return tail_decisions_; }
     float last_loss() const { return last_loss_; }
 
 private:
     static constexpr uint32_t kF = 12;
+    enum Kind { kPlain, kFill, kLearnedSwap };
+
+    bool trained() const { return mlp_ && samples_ >= cfg_.min_samples; }
+
+    // features, walker demand and the ranking score of every block
+    void prepare(const SchedInput& in)
+    {
+        const uint32_t nb = in.nblocks;
+        const uint64_t n = nb;
+        feats_.assign((size_t)nb * kF, 0.f);
+        have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
+        std::vector<char> cached(nb, 0);
+        for (uint32_t b : buckets) cached[b] = 1;
+        add_.assign(nb, 0);
+        demand_.assign(nb, 0);
+        for (uint32_t b = 0; b < nb; ++b) {
+            uint64_t a = in.W[b * n + b], d = in.W[b * n + b];
+            for (uint32_t j = 0; j < nb; ++j) {
+                if (j == b) continue;
+                const uint64_t x = in.W[b * n + j] + in.W[(uint64_t)j * n + b];
+                d += x;
+                if (cached[j]) a += x;
+            }
+            add_[b] = a;
+            demand_[b] = d;
+        }
+        pred_.assign(nb, 0.f);
+        use_model_ = trained() && have_feats_ && gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
+        rank_.resize(nb);
+        // before the model is trained: walkers that need the block, log scale
+        for (uint32_t b = 0; b < nb; ++b) rank_[b] = use_model_ ? pred_[b] : std::log1p((float)demand_[b]);
+    }
+
+    // the non-resident block with the best rank among those whose walkers could run now
+    int best_candidate(const std::vector<char>& cached) const
+    {
+        int best = -1;
+        for (uint32_t c = 0; c < add_.size(); ++c) {
+            if (cached[c] || add_[c] == 0) continue;
+            if (best < 0 || rank_[c] > rank_[best]) best = (int)c;
+        }
+        return best;
+    }
+
+    bool tail_now(const SchedInput& in, uint32_t c)
+    {
+        if (!use_model_ || cooldown_ > 0) return false;
+        const gw_block_desc& d = in.blocks[c];
+        const double bytes = ((double)d.nverts + 1) * 8 + (double)d.nedges * 4;
+        const double steps = std::expm1((double)std::max(0.f, pred_[c]));
+        if (steps * cfg_.zc_bytes_per_step * cfg_.tail_margin < bytes) {
+            ++tail_decisions_;
+            return true;
+        }
+        return false;
+    }
 
     bool plain_decide(const SchedInput& in)
     {
         plain_.buckets = buckets;
         bool r = plain_.schedule(in);
         buckets = plain_.buckets;
+        last_kind_ = kPlain;
         return r;
     }
 
@@ -158,7 +238,7 @@ private:
         if (!n) return;
         const uint32_t bs = std::min(cfg_.batch, n);
         std::vector<float> bx((size_t)bs * kF), by(bs);
-        for (uint32_t e = 0; e < cfg_.epochs; ++e) {
+        for (uint32_t e = 0; e < cfg_.train_steps; ++e) {
             for (uint32_t i = 0; i < bs; ++i) {
                 rng_ = rng_ * 6364136223846793005ULL + 1442695040888963407ULL;
                 const uint32_t k = (uint32_t)((rng_ >> 33) % n);
@@ -175,11 +255,13 @@ private:
     LearnedConfig cfg_;
     gw_mlp* mlp_ = nullptr;
     PlainScheduler plain_;
-    std::vector<float> age_, feats_, pred_, xs_, ys_;
+    std::vector<float> age_, feats_, pred_, rank_, xs_, ys_;
+    std::vector<uint64_t> add_, demand_;
+    bool have_feats_ = false, use_model_ = false;
     uint64_t samples_ = 0, ring_ = 0, rng_ = 0x853c49e6748fea9bULL;
-    uint32_t since_train_ = 0, cooldown_ = 0;
-    bool last_decision_learned_ = false;
-    uint64_t learned_decisions_ = 0, fallbacks_ = 0, learned_n_ = 0, plain_n_ = 0;
+    uint32_t cooldown_ = 0;
+    Kind last_kind_ = kPlain;
+    uint64_t learned_decisions_ = 0, fallbacks_ = 0, tail_decisions_ = 0, learned_n_ = 0, plain_n_ = 0;
     double learned_yield_ = 0, plain_yield_ = 0;
     float last_loss_ = 0.f;
 };

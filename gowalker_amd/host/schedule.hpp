@@ -21,6 +21,7 @@ struct SchedInput {
     uint64_t zero_threshold;
     const gw_block_desc* blocks;
     uint64_t iteration;
+    const uint8_t* staged = nullptr;   // per block: 0 not staged, 1 copy in flight, 2 copy done
 };
 
 // What one iteration achieved; fed back to schedulers that learn.
@@ -35,7 +36,15 @@ struct IterationOutcome {
 class Scheduler {
 public:
     virtual ~Scheduler() {}
+    // Sets `buckets`, the blocks resident for the next run; false = nothing is worth
+    // loading (the engine then finishes the walk by zero-copy reads).
     virtual bool schedule(const SchedInput& in) = 0;
+    // Blocks to stage into free slots while the run goes on (copy overlapped with the walk;
+    // a later schedule() commits them by putting them in `buckets`).
+    virtual void prefetch(const SchedInput& in, uint32_t free_slots, std::vector<uint32_t>& out)
+    {
+        (void)in; (void)free_slots; (void)out;
+    }
     virtual void feedback(const SchedInput& in, const IterationOutcome& out) { (void)in; (void)out; }
     virtual const char* name() const = 0;
     virtual uint32_t run_flags() const { return 0; }   // extra GW_RUN_* flags the scheduler needs
@@ -96,36 +105,85 @@ public:
 
 // Slot assignment of GOwalker_scheduler_t::copyblocks (engine/schedule.hpp:144-197): a
 // block already in a slot stays there; otherwise it takes the first free slot, else the
-// first slot whose block is no longer wanted.
+// first slot whose block is no longer wanted.  Slots may also hold a staged (prefetched,
+// not yet resident) block; such a slot counts as free once its block is not wanted.
 class SlotMap {
 public:
     void reset(uint32_t nslots, uint32_t nblocks)
     {
         slot_blk.assign(nslots, -1);
+        slot_stg.assign(nslots, -1);
         blk_slot.assign(nblocks, -1);
+        blk_stg.assign(nblocks, -1);
     }
-    // returns the slot to upload into, or -1 if the block is already resident
+    // returns the slot to upload into, -1 if the block is already resident, -2 if every
+    // slot holds a wanted block
     int place(uint32_t blk, const std::vector<uint32_t>& wanted)
     {
         if (blk_slot[blk] >= 0) return -1;
-        for (size_t s = 0; s < slot_blk.size(); ++s)
-            if (slot_blk[s] < 0) return assign((int)s, blk);
-        for (size_t s = 0; s < slot_blk.size(); ++s) {
-            if (std::find(wanted.begin(), wanted.end(), (uint32_t)slot_blk[s]) == wanted.end()) {
-                blk_slot[slot_blk[s]] = -1;
-                return assign((int)s, blk);
-            }
-        }
-        return -2;   // no slot available: more wanted blocks than slots
+        const int s = pick(wanted);
+        if (s < 0) return -2;
+        assign((uint32_t)s, blk);
+        return s;
     }
-    std::vector<int> slot_blk, blk_slot;
-
-private:
-    int assign(int s, uint32_t blk)
+    // slot to stage `blk` into (free or unwanted), or -1
+    int stage(uint32_t blk, const std::vector<uint32_t>& wanted)
     {
+        if (blk_slot[blk] >= 0 || blk_stg[blk] >= 0) return -1;
+        const int s = pick(wanted);
+        if (s < 0) return -1;
+        clear(s);
+        slot_stg[s] = (int)blk;
+        blk_stg[blk] = s;
+        return s;
+    }
+    // the staged copy of blk becomes resident in its slot
+    void commit(uint32_t blk)
+    {
+        const int s = blk_stg[blk];
+        slot_stg[s] = -1;
+        blk_stg[blk] = -1;
         slot_blk[s] = (int)blk;
         blk_slot[blk] = s;
-        return s;
+    }
+    uint32_t free_slots(const std::vector<uint32_t>& wanted) const
+    {
+        uint32_t n = 0;
+        for (size_t s = 0; s < slot_blk.size(); ++s) n += usable((int)s, wanted) ? 1u : 0u;
+        return n;
+    }
+    std::vector<int> slot_blk, slot_stg, blk_slot, blk_stg;
+
+private:
+    static bool in(const std::vector<uint32_t>& v, int b) { return b >= 0 && std::find(v.begin(), v.end(), (uint32_t)b) != v.end(); }
+    bool usable(int s, const std::vector<uint32_t>& wanted) const
+    {
+        return !in(wanted, slot_blk[s]) && !in(wanted, slot_stg[s]);
+    }
+    int pick(const std::vector<uint32_t>& wanted) const
+    {
+        // first empty slot, then the first slot with an unwanted staged block, then the
+        // first slot whose resident block is unwanted
+        for (size_t s = 0; s < slot_blk.size(); ++s)
+            if (slot_blk[s] < 0 && slot_stg[s] < 0) return (int)s;
+        for (size_t s = 0; s < slot_blk.size(); ++s)
+            if (slot_blk[s] < 0 && slot_stg[s] >= 0 && !in(wanted, slot_stg[s])) return (int)s;
+        for (size_t s = 0; s < slot_blk.size(); ++s)
+            if (slot_blk[s] >= 0 && !in(wanted, slot_blk[s])) return (int)s;
+        return -1;
+    }
+    void clear(int s)
+    {
+        if (slot_blk[s] >= 0) blk_slot[slot_blk[s]] = -1;
+        if (slot_stg[s] >= 0) blk_stg[slot_stg[s]] = -1;
+        slot_blk[s] = -1;
+        slot_stg[s] = -1;
+    }
+    void assign(uint32_t s, uint32_t blk)
+    {
+        clear((int)s);
+        slot_blk[s] = (int)blk;
+        blk_slot[blk] = (int)s;
     }
 };
 

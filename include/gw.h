@@ -96,7 +96,8 @@ typedef struct {
 } gw_launch;
 
 enum {
-    GW_RUN_BUCKETS = 1u << 0, /* also recount the (prev_blk, cur_blk) bucket matrix */
+    GW_RUN_BUCKETS = 1u << 0, /* after the run, recount the (prev_blk, cur_blk) bucket matrix of
+                                 the walkers left, so gw_bucket_counts needs no extra pass */
     GW_RUN_TRACE = 1u << 1    /* record steps per current block (gw_trace_read) */
 };
 
@@ -109,7 +110,9 @@ typedef struct {
     uint64_t walkers_ended;  /* walkers that finished during this run */
     uint64_t walkers_left;   /* unfinished walkers after this run */
     double kernel_ms;        /* step kernel time (events on the compute stream) */
-    double filter_ms;        /* worklist/bucket kernel time */
+    double filter_ms;        /* bucket pass time (GW_RUN_BUCKETS) */
+    uint64_t zc_touches;     /* reads that went to host-mapped blocks (row, candidate, probe);
+                                about 64 B of PCIe traffic each */
 } gw_run_stats;
 
 typedef struct {
@@ -119,6 +122,7 @@ typedef struct {
     double h2d_ms;           /* copy-stream busy time of block uploads */
     double kernel_ms;
     uint64_t hbm_bytes_held; /* bytes counted against cache_bytes */
+    uint64_t zc_touches;     /* zero-copy reads of host-mapped blocks */
 } gw_counters;
 
 typedef struct {
@@ -153,9 +157,21 @@ gw_status gw_slot_bytes(gw_ctx* ctx, uint64_t* bytes_per_slot);
 
 /* Asynchronous upload of block `blk` into `slot` on the copy stream: replaces
  * copycachetogpu + pipeline_gpugraph (gpu/memory.hpp:124-139, :209-224).  beg holds
- * nverts+1 global offsets, csr holds nedges targets (the block's own arrays). */
+ * nverts+1 global offsets, csr holds nedges targets (the block's own arrays; pinned
+ * memory, valid until the copy is done).  The host does not wait: the block is resident
+ * for the next gw_run, whose kernels wait for the copy on the device. */
 gw_status gw_upload_block(gw_ctx* ctx, uint32_t slot, uint32_t blk,
                           const uint64_t* beg, const uint32_t* csr);
+/* Prefetch into a staging slot (the reference copies synchronously before each run,
+ * engine/engine.hpp:415; this is the double buffer that overlaps the copy with the walk):
+ * starts the copy of `blk` into `slot` at once; the block stays invisible to gw_run (which
+ * does not wait for it) until gw_commit_block(slot).  The slot's previous block stops
+ * being resident immediately.  gw_staged_block reports the staged block of a slot (-1:
+ * none) and, if `ready` is not NULL, whether its copy has completed. */
+gw_status gw_stage_block(gw_ctx* ctx, uint32_t slot, uint32_t blk,
+                         const uint64_t* beg, const uint32_t* csr);
+gw_status gw_commit_block(gw_ctx* ctx, uint32_t slot);
+gw_status gw_staged_block(gw_ctx* ctx, uint32_t slot, int32_t* blk, uint32_t* ready);
 /* Marks a slot free and its block non-resident: replaces the INACTIVE update of
  * GOwalker_scheduler_t::copyblocks (engine/schedule.hpp:181-183). */
 gw_status gw_evict(gw_ctx* ctx, uint32_t slot);
@@ -184,9 +200,10 @@ gw_status gw_run(gw_ctx* ctx, const gw_launch* launch, gw_run_stats* stats);
  * and makes the fastest the shape of later gw_run calls that pass block == 0. */
 gw_status gw_autotune(gw_ctx* ctx, uint64_t sample, gw_launch* best, double* best_ms);
 
-/* Bucket matrix W[prev_blk * nblocks + cur_blk] of unfinished walkers, as of the last
- * gw_run with GW_RUN_BUCKETS (or gw_init_walkers): replaces copy_back's read of
- * block_offset (engine/walk.hpp:488-498) and walk_block::block_numwalks. */
+/* Bucket matrix W[prev_blk * nblocks + cur_blk] of unfinished walkers: replaces
+ * copy_back's read of block_offset (engine/walk.hpp:488-498) and
+ * walk_block::block_numwalks.  Free after gw_init_walkers or a gw_run with
+ * GW_RUN_BUCKETS; otherwise one pass over the walkers. */
 gw_status gw_bucket_counts(gw_ctx* ctx, uint64_t* W, uint64_t n);
 gw_status gw_walkers_left(gw_ctx* ctx, uint64_t* left);
 

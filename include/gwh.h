@@ -33,6 +33,10 @@ typedef struct {
     uint64_t learned_decisions, fallbacks;
     float final_loss;
     uint32_t pad;
+    uint64_t staged;           /* blocks prefetched into free slots (their bytes are in h2d_bytes) */
+    uint64_t zc_touches;       /* zero-copy reads of host-mapped blocks (about 64 B of PCIe each) */
+    uint64_t tail_decisions;   /* learned: zero-copy chosen over an upload */
+    double sched_ms;           /* host time spent scheduling and placing blocks */
 } gwh_report;
 
 /* Runs walkers to completion out of core: beg has `entries` u64 offsets (ids

@@ -114,6 +114,19 @@ static void test_scheduler()
     CHECK(sm.place(0, {0, 1}) == 0 && sm.place(1, {0, 1}) == 1 && sm.place(1, {0, 1}) == -1);
     CHECK(sm.place(2, {2, 1}) == 0 && sm.blk_slot[0] == -1 && sm.blk_slot[2] == 0);
     CHECK(sm.place(3, {2, 1}) == -2);
+    // staging: a prefetched block takes a free slot, is not resident until committed, and
+    // its slot is reusable once the block is no longer wanted
+    SlotMap st;
+    st.reset(3, 6);
+    CHECK(st.place(0, {0}) == 0);
+    CHECK(st.stage(1, {0}) == 1 && st.stage(2, {0, 1}) == 2 && st.stage(3, {0, 1, 2}) == -1);
+    CHECK(st.blk_slot[1] == -1 && st.blk_stg[1] == 1 && st.free_slots({0}) == 2 && st.free_slots({0, 1, 2}) == 0);
+    CHECK(st.stage(1, {0}) == -1);   // already staged
+    st.commit(1);
+    CHECK(st.blk_slot[1] == 1 && st.blk_stg[1] == -1 && st.slot_stg[1] == -1);
+    // wanted {0, 1, 4}: block 4 goes to the slot of the unwanted staged block 2
+    CHECK(st.place(4, {0, 1, 4}) == 2 && st.blk_stg[2] == -1 && st.blk_slot[4] == 2);
+    CHECK(st.free_slots({0, 1, 4}) == 0 && st.place(5, {0, 1, 4, 5}) == -2);
 }
 
 int main(int argc, char** argv)
