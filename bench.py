@@ -125,7 +125,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
         for name, sched, zt in ooc_schedulers(E, block_bytes):
             reps = []
             for i in range(args.warmup + args.steps):
-                first = not hashes.get((cap_gb, name)) and i == 0
+                first = (cap_gb, name) not in hashes
                 rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
                                               memory_bytes=mem, block_bytes=block_bytes, zero_threshold=zt,
                                               scheduler=sched, hashes=first)

@@ -80,6 +80,7 @@ SIGNATURES = {
     "gw_set_walk": [P, C.POINTER(GwWalkParams)],
     "gw_init_walkers": [P, C.c_uint64, C.c_uint32, P, C.c_uint32],
     "gw_stage_block": [P, C.c_uint32, C.c_uint32, P, P],
+    "gw_pending_work": [P, u64p, u64p],
     "gw_commit_block": [P, C.c_uint32],
     "gw_staged_block": [P, C.c_uint32, C.POINTER(C.c_int32), u32p],
     "gw_run": [P, C.POINTER(GwLaunch), C.POINTER(GwRunStats)],

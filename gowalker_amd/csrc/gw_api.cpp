@@ -71,7 +71,7 @@ struct gw_ctx {
     float* d_feat = nullptr;               // nblocks x kNumFeatures
     float* d_age = nullptr;
     uint32_t* d_misc = nullptr;            // [0] nwork, [1] queue
-    unsigned long long* d_cnt = nullptr;   // 8 counters
+    unsigned long long* d_cnt = nullptr;   // [0..5] run counters, [6] left, [7] zero-copy reads, [8] hops
     unsigned long long* h_cnt = nullptr;   // pinned readback
     uint64_t walker_bytes = 0;
 
@@ -79,6 +79,7 @@ struct gw_ctx {
     bool w_valid = false;
     uint32_t* h_bucket = nullptr;          // pinned nblocks^2
     uint64_t unfinished = 0;               // unfinished walkers
+    uint64_t pending_hops = 0;             // their remaining hop caps (as of the last bucket pass)
 
     // cumulative counters
     gw_counters tot{};
@@ -242,7 +243,7 @@ gw_status gw_create(const gw_config* cfg, gw_ctx** out)
     void* p = nullptr;
     if (dalloc(c, &p, 4 * sizeof(uint32_t), false) != GW_OK) return bail(fail(nullptr, GW_ERR_NOMEM, "%s", c->err.c_str()));
     c->d_misc = (uint32_t*)p;
-    if (dalloc(c, &p, 8 * sizeof(unsigned long long), false) != GW_OK)
+    if (dalloc(c, &p, 16 * sizeof(unsigned long long), false) != GW_OK)
         return bail(fail(nullptr, GW_ERR_NOMEM, "%s", c->err.c_str()));
     c->d_cnt = (unsigned long long*)p;
     if (hipHostMalloc((void**)&c->h_cnt, 16 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
@@ -683,8 +684,13 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
     if (e == hipSuccess) e = hipStreamSynchronize(c->s_compute);
     if (d_src) (void)hipFree(d_src);
     if (e != hipSuccess) return fail(c, GW_ERR_HIP, "walker init kernel failed: %s", hipGetErrorString(e));
+    // walkers with a zero hop cap are done at once; the bucket matrix is computed on demand
+    const uint64_t n1 = c->wp.ntypes == 2 ? n / 2 : 0, n0 = n - n1;
+    const uint64_t l0 = c->dt[0].length, l1 = c->wp.ntypes == 2 ? c->dt[1].length : 0;
+    c->unfinished = (l0 ? n0 : 0) + (l1 ? n1 : 0);
+    c->pending_hops = n0 * l0 + n1 * l1;
     c->w_valid = false;
-    return ensure_buckets(c);   // initial bucket matrix and unfinished count
+    return GW_OK;
 }
 
 // Worklist + (optional) bucket pass on the compute stream; returns nwork and left.
@@ -694,8 +700,8 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
 static gw_status enqueue_bucket_pass(gw_ctx* c)
 {
     const uint64_t nn = (uint64_t)c->nblocks * c->nblocks;
-    HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
     HIPCHK(c, launch_fill_u32((uint32_t*)(c->d_cnt + 6), 0, 2, c->s_compute));
+    HIPCHK(c, launch_fill_u32((uint32_t*)(c->d_cnt + 8), 0, 2, c->s_compute));
     HIPCHK(c, launch_fill_u32(c->d_bucket, 0, nn, c->s_compute));
     FilterArgs f{};
     f.blocks = c->d_blocks;
@@ -703,15 +709,18 @@ static gw_status enqueue_bucket_pass(gw_ctx* c)
     f.nwalkers = c->nwalkers;
     f.meta = c->d_meta;
     f.bpair = c->d_bpair;
-    f.worklist = c->d_work;   // the pass also compacts the runnable walkers; unused here
-    f.nwork = c->d_misc;
     f.bucket = c->d_bucket;
     f.left = c->d_cnt + 6;
+    f.hops = c->d_cnt + 8;
+    f.len0 = c->dt[0].length;
+    f.len1 = c->dt[1].length;
     HIPCHK(c, hipEventRecord(c->ev_f0, c->s_compute));
-    HIPCHK(c, launch_filter(f, c->ncu * 8, c->s_compute));
+    HIPCHK(c, launch_filter(f, c->ncu * 2, c->s_compute));
     HIPCHK(c, hipEventRecord(c->ev_f1, c->s_compute));
     HIPCHK(c, hipMemcpyAsync(c->h_bucket, c->d_bucket, nn * 4, hipMemcpyDeviceToHost, c->s_compute));
     HIPCHK(c, hipMemcpyAsync(c->h_cnt + 8, c->d_cnt + 6, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->s_compute));
+    HIPCHK(c, hipMemcpyAsync(c->h_cnt + 9, c->d_cnt + 8, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                              c->s_compute));
     return GW_OK;
 }
@@ -725,6 +734,7 @@ static gw_status ensure_buckets(gw_ctx* c)
     if ((st = enqueue_bucket_pass(c)) != GW_OK) return st;
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
     c->unfinished = c->h_cnt[8];
+    c->pending_hops = c->h_cnt[9];
     c->w_valid = true;
     return GW_OK;
 }
@@ -783,6 +793,22 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     float fms = 0.f, kms = 0.f;
     HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
     HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
+    if (!all_res) {
+        // compact the runnable walkers on the device; the step kernel reads the count
+        // there, so no host round trip separates the two kernels
+        FilterArgs f{};
+        f.blocks = c->d_blocks;
+        f.nblocks = c->nblocks;
+        f.nwalkers = cap && cap < c->nwalkers ? cap : c->nwalkers;
+        f.meta = c->d_meta;
+        f.bpair = c->d_bpair;
+        f.worklist = c->d_work;
+        f.nwork = c->d_misc;
+        f.left = c->d_cnt + 6;
+        f.len0 = c->dt[0].length;
+        f.len1 = c->dt[1].length;
+        HIPCHK(c, launch_filter(f, c->ncu * 2, c->s_compute));
+    }
     WalkArgs a{};
     a.blocks = c->d_blocks;
     a.vcut = c->d_vcut;
@@ -794,8 +820,9 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     a.type1 = c->dt[1];
     a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta; a.bpair = c->d_bpair;
     a.hash = c->d_hash; a.paths = c->d_paths; a.path_stride = c->path_stride; a.visits = c->d_visits;
-    a.worklist = nullptr;
-    a.scan = all_res ? 0u : 1u;
+    a.worklist = all_res ? nullptr : c->d_work;
+    a.nwork_dev = all_res ? nullptr : c->d_misc;
+    a.scan = 0u;
     a.zc_count = mapped ? 1u : 0u;
     a.nwork = (uint32_t)nwork;
     a.queue = c->d_misc + 1;
@@ -814,7 +841,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     const bool one_blk = c->nblocks == 1;
     uint64_t grid = l.waves_per_cu ? (uint64_t)c->ncu * l.waves_per_cu * 64 / l.block
                                    : (uint64_t)c->ncu * walk_blocks_per_cu(one_blk, l.variant, l.block);
-    const uint64_t need = (nwork + l.block - 1) / l.block;
+    const uint64_t need = (nwork + l.block - 1) / l.block;   // an upper bound with a device-side count
     if (grid > need) grid = need;
     if (grid == 0) grid = 1;
     {
@@ -847,7 +874,12 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     s.mix_steps = c->h_cnt[3];
     s.walkers_ended = c->h_cnt[4];
     s.walkers_run = c->h_cnt[4] + c->h_cnt[5];
-    c->unfinished = buckets ? c->h_cnt[8] : c->unfinished - std::min<uint64_t>(c->unfinished, s.walkers_ended);
+    if (buckets) {
+        c->unfinished = c->h_cnt[8];
+        c->pending_hops = c->h_cnt[9];
+    } else {
+        c->unfinished -= std::min<uint64_t>(c->unfinished, s.walkers_ended);
+    }
     s.walkers_left = c->unfinished;
     s.kernel_ms = kms;
     s.filter_ms = fms;
@@ -958,10 +990,25 @@ gw_status gw_block_features(gw_ctx* c, const float* age, float* feats, uint64_t 
     if ((st = ensure_buckets(c)) != GW_OK) return st;
     if ((st = push_block_table(c)) != GW_OK) return st;
     if (age) HIPCHK(c, hipMemcpyAsync(c->d_age, age, c->nblocks * 4ull, hipMemcpyHostToDevice, c->s_compute));
-    HIPCHK(c, launch_features(c->d_bucket, c->nblocks, c->d_blocks, age ? c->d_age : nullptr, c->d_feat, c->s_compute));
+    const uint32_t maxlen = std::max(c->dt[0].length, c->wp.ntypes == 2 ? c->dt[1].length : 0u);
+    const float ahead = c->unfinished && maxlen ? (float)((double)c->pending_hops / ((double)c->unfinished * maxlen)) : 0.f;
+    HIPCHK(c, launch_features(c->d_bucket, c->nblocks, c->d_blocks, age ? c->d_age : nullptr, ahead, c->d_feat,
+                              c->s_compute));
     HIPCHK(c, hipMemcpyAsync(feats, c->d_feat, (uint64_t)c->nblocks * kNumFeatures * 4, hipMemcpyDeviceToHost,
                              c->s_compute));
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    return GW_OK;
+}
+
+gw_status gw_pending_work(gw_ctx* c, uint64_t* walkers, uint64_t* hops)
+{
+    if (!c || !walkers || !hops) return fail(c, GW_ERR_ARG, "null argument");
+    if (!c->nwalkers) return fail(c, GW_ERR_STATE, "no walkers");
+    (void)hipSetDevice(c->device);
+    gw_status st;
+    if ((st = ensure_buckets(c)) != GW_OK) return st;
+    *walkers = c->unfinished;
+    *hops = c->pending_hops;
     return GW_OK;
 }
 

@@ -63,6 +63,7 @@ struct WalkArgs {
     // work
     const uint32_t* worklist; // walker ids; nullptr = identity (finished walkers are skipped)
     uint32_t nwork;
+    const uint32_t* nwork_dev;  // if set, the worklist length is read here (written by the filter pass)
     uint32_t scan;            // identity list: also skip walkers whose blocks are not both resident
     uint32_t zc_count;        // some block is host-mapped: count zero-copy reads (counters[7])
     uint32_t* queue;          // work-fetch counter, zero at launch
@@ -94,10 +95,12 @@ struct FilterArgs {
     uint64_t nwalkers;
     const uint32_t* meta;
     const uint32_t* bpair;
-    uint32_t* worklist;
-    uint32_t* nwork;          // append counter
+    uint32_t* worklist;       // optional: ids of walkers whose (prev, cur) blocks are resident
+    uint32_t* nwork;          // append counter of the worklist
     uint32_t* bucket;         // optional nblocks*nblocks counters
     unsigned long long* left; // unfinished walkers
+    unsigned long long* hops; // optional: sum of the remaining hop caps of unfinished walkers
+    uint32_t len0, len1;      // hop caps of walker types 0 / 1
 };
 
 // launchers (gw_walk.hip)
@@ -107,7 +110,7 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
                        uint32_t block, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 uint32_t walk_blocks_per_cu(bool one_block, uint32_t variant, uint32_t block);   // resident workgroups per CU
-hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age,
+hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age, float ahead,
                            float* feats, hipStream_t s);
 constexpr uint32_t kTraceLdsBlocks = 2048;   // per-workgroup LDS histogram size of the trace
 constexpr uint32_t kNumFeatures = 12;

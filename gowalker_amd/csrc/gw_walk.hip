@@ -215,6 +215,23 @@ __device__ __forceinline__ bool runnable(const DevBlock* __restrict__ blocks, ui
     return blocks[pair & 0xFFFFu].resident && blocks[pair >> 16].resident;
 }
 
+// Worklist length and work-queue chunk: from the host, or, after a filter pass on the
+// same stream, from device memory (no host round trip); every wavefront derives the same
+// chunk (64..1024, a multiple of 64) so that each still gets several chunks.
+__device__ __forceinline__ void work_size(const WalkArgs& a, uint32_t& nwork, uint32_t& chunk)
+{
+    if (!a.nwork_dev) {
+        nwork = a.nwork;
+        chunk = a.chunk;
+        return;
+    }
+    nwork = *a.nwork_dev;
+    const uint64_t waves = (uint64_t)gridDim.x * blockDim.x / 64;
+    uint64_t ch = nwork / (waves * 8 + 1);
+    ch = ch < 64 ? 64 : (ch > 1024 ? 1024 : ch);
+    chunk = (uint32_t)(ch / 64 * 64);
+}
+
 // Wave-local work queue: a wavefront takes a chunk of walker indices with one atomic and
 // hands them to its lanes as they free up, so the shared counter sees one atomic per
 // chunk instead of one per walker.  The host sizes the chunk (64..1024) so that every
@@ -268,15 +285,17 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
     Row rv{nullptr, 0}, ru{nullptr, 0};
     const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
     WaveQueue q{0, 0};
+    uint32_t nwork, chunk;
+    work_size(a, nwork, chunk);
 
     for (;;) {
         const bool need = !have && !exhausted;
         const uint64_t mask = __ballot(need);
         if (mask) {
-            const uint32_t idx = wave_take(q, mask, a.queue, lane, a.chunk);
+            const uint32_t idx = wave_take(q, mask, a.queue, lane, chunk);
             if (need) {
-                const uint32_t m = idx < a.nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
-                if (idx >= a.nwork) {
+                const uint32_t m = idx < nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
+                if (idx >= nwork) {
                     exhausted = true;
                 } else if (((m >> kStatusShift) & 0xFu) == kActive &&
                            (!a.scan || runnable(a.blocks, a.bpair[a.worklist ? a.worklist[idx] : idx]))) {
@@ -751,14 +770,16 @@ __global__ void __launch_bounds__(256) walk_flat_kernel(WalkArgs a)
     L.release = false;
     bool exhausted = false;
     WaveQueue q{0, 0};
+    uint32_t nwork, chunk;
+    work_size(a, nwork, chunk);
     for (;;) {
         const bool need = L.phase == PH_IDLE && !exhausted;
         const uint64_t mask = __ballot(need);
         if (mask) {
-            const uint32_t idx = wave_take(q, mask, a.queue, lane, a.chunk);
+            const uint32_t idx = wave_take(q, mask, a.queue, lane, chunk);
             if (need) {
-                const uint32_t m = idx < a.nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
-                if (idx >= a.nwork) {
+                const uint32_t m = idx < nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
+                if (idx >= nwork) {
                     exhausted = true;
                 } else if (((m >> kStatusShift) & 0xFu) == kActive &&
                            (!a.scan || runnable(a.blocks, a.bpair[a.worklist ? a.worklist[idx] : idx]))) {
@@ -832,47 +853,78 @@ __global__ void __launch_bounds__(256) init_kernel(InitArgs a)
     }
 }
 
-// Worklist of unfinished walkers whose (prev, cur) blocks are both resident, compacted
-// with a wavefront ballot + one atomic per wavefront; optionally the bucket matrix of
-// all unfinished walkers (the counts prefix() scans in gpuwalk.hpp:321-338).
-__global__ void __launch_bounds__(256) filter_kernel(FilterArgs a)
+// The bucket matrix is the count matrix that prefix() scans in gpuwalk.hpp:321-338; the
+// worklist replaces insertglobal's scatter (gpuwalk.hpp:340-372).
+constexpr uint32_t kFilterThreads = 1024, kFilterItems = 4;
+
+// Bucket / worklist pass over all walkers.  A workgroup takes a tile of
+// kFilterThreads * kFilterItems walkers (coalesced: item k of thread t is walker
+// tile + k * kFilterThreads + t), counts the unfinished ones into the (prev_blk, cur_blk)
+// histogram (LDS when the matrix fits) and, when a worklist is requested, appends the
+// runnable ones with one atomic per tile.
+__global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
 {
-    // bucket counts go through an LDS histogram when the matrix fits (dynamic LDS)
     extern __shared__ uint32_t hist[];
+    __shared__ uint32_t wave_cnt[kFilterThreads / 64];
+    __shared__ uint32_t tile_base;
     const uint32_t nbb = a.nblocks * a.nblocks;
     const bool lds_hist = a.bucket && nbb <= kFilterLdsBuckets;
     if (lds_hist) {
         for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x) hist[i] = 0;
         __syncthreads();
     }
-    const uint32_t lane = __lane_id();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    uint64_t left = 0;
-    const uint64_t n_pad = (a.nwalkers + 63) & ~63ULL;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_pad; w += stride) {
-        bool take = false;
-        if (w < a.nwalkers) {
-            const uint32_t m = a.meta[w];
-            if (((m >> kStatusShift) & 0xFu) == kActive) {
-                ++left;
-                const uint32_t pr = a.bpair[w];
-                const uint32_t bc = pr & 0xFFFFu, bp = pr >> 16;
-                take = a.blocks[bc].resident && a.blocks[bp].resident;
-                if (lds_hist) atomicAdd(hist + bp * a.nblocks + bc, 1u);
-                else if (a.bucket) atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
+    const uint64_t tile = (uint64_t)blockDim.x * kFilterItems;
+    uint64_t left = 0, hops = 0;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < a.nwalkers; t0 += (uint64_t)gridDim.x * tile) {
+        uint64_t masks[kFilterItems];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kFilterItems; ++k) {
+            const uint64_t w = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
+            bool take = false;
+            if (w < a.nwalkers) {
+                const uint32_t m = a.meta[w];
+                if (((m >> kStatusShift) & 0xFu) == kActive) {
+                    ++left;
+                    const uint32_t len = ((m >> kTypeShift) & 1u) ? a.len1 : a.len0;
+                    const uint32_t hop = m & kHopMask;
+                    hops += len > hop ? len - hop : 0u;
+                    const uint32_t pr = a.bpair[w];
+                    const uint32_t bc = pr & 0xFFFFu, bp = pr >> 16;
+                    take = a.worklist && a.blocks[bc].resident && a.blocks[bp].resident;
+                    if (lds_hist) atomicAdd(hist + bp * a.nblocks + bc, 1u);
+                    else if (a.bucket) atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
+                }
             }
+            masks[k] = __ballot(take);
+            cnt += (uint32_t)__popcll(masks[k]);
         }
-        const uint64_t mask = __ballot(take);
-        if (mask) {
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(a.nwork, (uint32_t)__popcll(mask));
-            base = __shfl(base, leader, 64);
-            if (take) a.worklist[base + lane_rank(mask)] = (uint32_t)w;
+        if (a.worklist) {   // uniform: one atomic per tile, waves write at their offsets
+            if (lane == 0) wave_cnt[wave] = cnt;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t tot = 0;
+                for (uint32_t i = 0; i < nwaves; ++i) { const uint32_t c = wave_cnt[i]; wave_cnt[i] = tot; tot += c; }
+                tile_base = tot ? atomicAdd(a.nwork, tot) : 0u;
+            }
+            __syncthreads();
+            uint32_t off = tile_base + wave_cnt[wave];
+#pragma unroll
+            for (uint32_t k = 0; k < kFilterItems; ++k) {
+                if ((masks[k] >> lane) & 1ull)
+                    a.worklist[off + lane_rank(masks[k])] = (uint32_t)(t0 + (uint64_t)k * blockDim.x + threadIdx.x);
+                off += (uint32_t)__popcll(masks[k]);
+            }
+            __syncthreads();   // wave_cnt / tile_base are reused by the next tile
         }
     }
     left = wave_sum(left);
     if (lane == 0 && left) atomicAdd(a.left, (unsigned long long)left);
+    if (a.hops) {
+        hops = wave_sum(hops);
+        if (lane == 0 && hops) atomicAdd(a.hops, (unsigned long long)hops);
+    }
     if (lds_hist) {
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x)
@@ -890,7 +942,8 @@ __global__ void __launch_bounds__(256) filter_kernel(FilterArgs a)
 // 11 fraction of all unfinished walkers touching b
 __global__ void __launch_bounds__(256) features_kernel(const uint32_t* __restrict__ W, uint32_t nb,
                                                         const DevBlock* __restrict__ blocks,
-                                                        const float* __restrict__ age, float* __restrict__ f)
+                                                        const float* __restrict__ age, float ahead,
+                                                        float* __restrict__ f)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
@@ -918,16 +971,16 @@ __global__ void __launch_bounds__(256) features_kernel(const uint32_t* __restric
     o[6] = (float)log1p(res + diag);
     o[7] = age ? age[b] / 64.f : 0.f;
     o[8] = (float)(log1p((double)d.nedges) / 24.0);
-    o[9] = (float)(log1p((double)d.nverts) / 24.0);
+    o[9] = ahead;   // run phase: mean fraction of the hop cap still ahead of unfinished walkers
     o[10] = (float)(log1p(d.nverts ? (double)d.nedges / d.nverts : 0.0) / 8.0);
     o[11] = total > 0 ? (float)((row + col - diag) / total) : 0.f;
 }
 
-hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age,
+hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age, float ahead,
                            float* feats, hipStream_t s)
 {
     hipLaunchKernelGGL(features_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, s, bucket, nblocks, blocks, age,
-                       feats);
+                       ahead, feats);
     return hipGetLastError();
 }
 
@@ -949,11 +1002,11 @@ hipError_t launch_init(const InitArgs& a, hipStream_t s)
 hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s)
 {
     if (a.nwalkers == 0) return hipSuccess;
-    uint64_t need = (a.nwalkers + 255) / 256;
+    const uint64_t need = (a.nwalkers + kFilterThreads * kFilterItems - 1) / (kFilterThreads * kFilterItems);
     if (grid == 0 || grid > need) grid = (uint32_t)(need < 65536 ? need : 65536);
     const uint32_t nbb = a.nblocks * a.nblocks;
     const size_t lds = (a.bucket && nbb <= kFilterLdsBuckets) ? nbb * 4 : 0;
-    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(kFilterThreads), lds, s, a);
     return hipGetLastError();
 }
 

@@ -128,6 +128,20 @@ class Context:
         self._keep.append((beg, csr))
         self._chk(lib().gw_upload_block(self.h, slot, blk, _p(beg), _p(csr)))
 
+    def stage(self, slot, blk):
+        """Prefetch blk into slot; it becomes resident only at commit(slot)."""
+        beg, csr = self.blocks.arrays(blk)
+        self._keep.append((beg, csr))
+        self._chk(lib().gw_stage_block(self.h, slot, blk, _p(beg), _p(csr)))
+
+    def commit(self, slot):
+        self._chk(lib().gw_commit_block(self.h, slot))
+
+    def staged(self, slot):
+        b, r = C.c_int32(-1), C.c_uint32(0)
+        self._chk(lib().gw_staged_block(self.h, slot, C.byref(b), C.byref(r)))
+        return b.value, bool(r.value)
+
     def evict(self, slot):
         self._chk(lib().gw_evict(self.h, slot))
 
@@ -172,6 +186,19 @@ class Context:
         W = np.zeros(nb * nb, dtype=np.uint64)
         self._chk(lib().gw_bucket_counts(self.h, _p(W), nb * nb))
         return W.reshape(nb, nb)
+
+    def pending_work(self):
+        """(unfinished walkers, sum of their remaining hop caps)"""
+        w, h = C.c_uint64(0), C.c_uint64(0)
+        self._chk(lib().gw_pending_work(self.h, C.byref(w), C.byref(h)))
+        return w.value, h.value
+
+    def block_features(self, age=None):
+        nb = self.blocks.nblocks
+        f = np.zeros(nb * 12, dtype=np.float32)
+        a = None if age is None else np.ascontiguousarray(age, dtype=np.float32)
+        self._chk(lib().gw_block_features(self.h, _p(a), _p(f), f.size))
+        return f.reshape(nb, 12)
 
     def walkers_left(self):
         v = C.c_uint64(0)

@@ -1,23 +1,28 @@
 // learned.hpp -- learned block scheduler: a 12-64-64-1 MLP (BF16 weights, FP32
-// accumulation, trained on the GPU by libgw.so's kernels) predicts, for every block, the
-// walk steps it serves in the next run if it is resident (log1p scale).  It is trained
-// online on the block-access trace the engine records (steps per current block per run,
-// gw_trace_read), with the features each block had when the decision was taken.
+// accumulation, trained on the GPU by libgw.so's kernels) predicts the walk steps that
+// loading a block buys, i.e. the steps of the run that follows its upload (log1p scale),
+// from the block's 12 features at decision time (gw_block_features).  It is trained
+// online on what the engine records: every run that followed exactly one new block gives
+// one sample (that block's features, the run's steps); the block-access trace
+// (gw_trace_read) keeps the per-block ages.
 //
 // Decisions, given the predictions y(b):
 //   fill   (free slots left): commit staged blocks whose copies are done, else the best
 //          staged one (the run waits for it), else upload the best candidate; then stage
 //          the next-best blocks into the free slots so their copies overlap the run;
 //   swap   (cache full): load the candidate c with the highest y among blocks whose
-//          walkers could run now, evict the resident block with the lowest y;
-//   tail   when y(c) says the upload would serve fewer steps than its bytes buy in
-//          zero-copy reads (steps x zc_bytes_per_step < block bytes), finish the walk by
-//          zero-copy instead of uploading (the reference's fixed walker-count threshold,
+//          walkers could run now; evict the resident block the fewest waiting walkers
+//          need;
+//   tail   when y(c) says the upload would buy fewer steps than its bytes buy in
+//          zero-copy reads (steps x zc_bytes_per_step < block bytes) and all the work
+//          left is worth at most a few uploads in zero-copy bytes, finish the walk by
+//          zero-copy instead (the reference's fixed walker-count threshold,
 //          apps/Node2vec.cu:100-104, is the plain rule's version of this decision).
-// The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback and judge: it
-// decides while the model has too few samples, and it takes over for a cool-down period
-// whenever learned swaps realise fewer steps per uploaded byte than plain swaps.  Walks
-// stay bit-identical either way (counter-based RNG).
+// The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: it decides
+// while the model has too few samples, and it takes over for a cool-down period whenever
+// the model's recent predictions of the steps an upload buys are off by more than a
+// factor e on average (the judge).  Walks stay bit-identical either way (counter-based
+// RNG).
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -32,15 +37,16 @@ namespace gwh {
 struct LearnedConfig {
     uint32_t h1 = 64, h2 = 64;
     uint64_t seed = 7;
-    uint32_t min_samples = 96;      // plain swap/tail decisions until this many trace samples
+    uint32_t min_samples = 24;      // plain swap decisions until this many upload samples
     uint32_t train_steps = 4;       // Adam steps after every iteration
     uint32_t batch = 512;
     uint32_t max_samples = 16384;   // ring buffer of trace samples
     float lr = 2e-3f;
-    double zc_bytes_per_step = 320; // PCIe bytes of one zero-copy step (about five 64 B reads)
+    double zc_bytes_per_step = 360; // PCIe bytes of one zero-copy step (5.7 measured 64 B reads)
     double tail_margin = 1.0;       // zero-copy once steps x zc bytes x margin < block bytes
+    double tail_guard = 16;         // ... and the work left x zc bytes < guard x block bytes
     uint32_t max_stage = 64;        // blocks staged per iteration
-    double judge_ratio = 0.8;       // learned swap yield/byte below ratio x plain's => fall back
+    double judge_error = 1.0;       // mean |log error| of recent predictions above this => fall back
     uint32_t cooldown = 8;
 };
 
@@ -95,8 +101,7 @@ public:
         if (tail_now(in, (uint32_t)c)) return false;
         int victim = -1;
         for (uint32_t p = 0; p < buckets.size(); ++p)
-            if (victim < 0 || rank_[buckets[p]] < rank_[buckets[victim]]) victim = (int)p;
-        if (!(rank_[c] > rank_[buckets[victim]])) return plain_decide(in);
+            if (victim < 0 || demand_[buckets[p]] < demand_[buckets[victim]]) victim = (int)p;
         buckets[victim] = (uint32_t)c;
         last_kind_ = kLearnedSwap;
         ++learned_decisions_;
@@ -122,28 +127,27 @@ public:
         const uint32_t nb = in.nblocks;
         std::vector<uint64_t> trace(nb, 0);
         if (gw_trace_read(ctx_, trace.data(), nb) != GW_OK) return;
-        // samples: (features at decision time, log1p(steps served)) for every block that
-        // was resident during the run
+        // sample: (features of the one block that became resident, log1p(steps of the run))
+        std::vector<uint32_t> fresh;
         for (uint32_t b : buckets)
-            if (feats_.size() >= (size_t)(b + 1) * kF) add_sample(&feats_[(size_t)b * kF], std::log1p((float)trace[b]));
-        for (uint32_t b = 0; b < nb; ++b) age_[b] = trace[b] ? 0.f : std::min(age_[b] + 1.f, 1024.f);
-        // judge: realised steps per uploaded byte of learned swaps vs plain swaps
-        if (out.h2d_bytes && (last_kind_ == kLearnedSwap || last_kind_ == kPlain)) {
-            const double y = (double)out.steps / (double)out.h2d_bytes;
-            if (last_kind_ == kLearnedSwap) {
-                learned_yield_ = learned_n_ ? 0.8 * learned_yield_ + 0.2 * y : y;
-                ++learned_n_;
-                if (plain_n_ >= 4 && learned_n_ >= 4 && learned_yield_ < cfg_.judge_ratio * plain_yield_) {
+            if (!was_resident_[b]) fresh.push_back(b);
+        if (fresh.size() == 1 && feats_.size() >= (size_t)(fresh[0] + 1) * kF) {
+            const float y = std::log1p((float)out.steps);
+            add_sample(&feats_[(size_t)fresh[0] * kF], y);
+            // judge: how far the model's prediction for this upload was from what it bought
+            if (use_model_) {
+                const double e = std::fabs((double)pred_[fresh[0]] - (double)y);
+                err_ = err_n_ ? 0.8 * err_ + 0.2 * e : e;
+                ++err_n_;
+                if (last_kind_ == kLearnedSwap && err_n_ >= 4 && err_ > cfg_.judge_error) {
                     cooldown_ = cfg_.cooldown;
-                    learned_n_ = 0;
+                    err_n_ = 0;
                     ++fallbacks_;
                 }
-            } else {
-                plain_yield_ = plain_n_ ? 0.8 * plain_yield_ + 0.2 * y : y;
-                ++plain_n_;
             }
         }
-        if (mlp_ && samples_ >= 32) train();
+        for (uint32_t b = 0; b < nb; ++b) age_[b] = trace[b] ? 0.f : std::min(age_[b] + 1.f, 1024.f);
+        if (mlp_ && samples_ >= 8) train();
     }
 
     uint64_t learned_decisions() const { return learned_decisions_; }
@@ -166,6 +170,9 @@ private:
         have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
         std::vector<char> cached(nb, 0);
         for (uint32_t b : buckets) cached[b] = 1;
+        was_resident_ = cached;
+        uint64_t wl = 0;
+        if (gw_pending_work(ctx_, &wl, &pending_hops_) != GW_OK) pending_hops_ = ~0ULL;
         add_.assign(nb, 0);
         demand_.assign(nb, 0);
         for (uint32_t b = 0; b < nb; ++b) {
@@ -182,8 +189,10 @@ private:
         pred_.assign(nb, 0.f);
         use_model_ = trained() && have_feats_ && gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
         rank_.resize(nb);
-        // before the model is trained: walkers that need the block, log scale
-        for (uint32_t b = 0; b < nb; ++b) rank_[b] = use_model_ ? pred_[b] : std::log1p((float)demand_[b]);
+        // before the model is trained: the walkers a load would release (log scale), with
+        // the walkers that need the block at all as a tie-break
+        for (uint32_t b = 0; b < nb; ++b)
+            rank_[b] = use_model_ ? pred_[b] : std::log1p((float)add_[b]) + 1e-3f * std::log1p((float)demand_[b]);
     }
 
     // the non-resident block with the best rank among those whose walkers could run now
@@ -203,7 +212,8 @@ private:
         const gw_block_desc& d = in.blocks[c];
         const double bytes = ((double)d.nverts + 1) * 8 + (double)d.nedges * 4;
         const double steps = std::expm1((double)std::max(0.f, pred_[c]));
-        if (steps * cfg_.zc_bytes_per_step * cfg_.tail_margin < bytes) {
+        if (steps * cfg_.zc_bytes_per_step * cfg_.tail_margin < bytes &&
+            (double)pending_hops_ * cfg_.zc_bytes_per_step < cfg_.tail_guard * bytes) {
             ++tail_decisions_;
             return true;
         }
@@ -257,12 +267,14 @@ private:
     PlainScheduler plain_;
     std::vector<float> age_, feats_, pred_, rank_, xs_, ys_;
     std::vector<uint64_t> add_, demand_;
+    std::vector<char> was_resident_;
+    uint64_t pending_hops_ = 0;
     bool have_feats_ = false, use_model_ = false;
     uint64_t samples_ = 0, ring_ = 0, rng_ = 0x853c49e6748fea9bULL;
     uint32_t cooldown_ = 0;
     Kind last_kind_ = kPlain;
-    uint64_t learned_decisions_ = 0, fallbacks_ = 0, tail_decisions_ = 0, learned_n_ = 0, plain_n_ = 0;
-    double learned_yield_ = 0, plain_yield_ = 0;
+    uint64_t learned_decisions_ = 0, fallbacks_ = 0, tail_decisions_ = 0, err_n_ = 0;
+    double err_ = 0;
     float last_loss_ = 0.f;
 };
 

@@ -206,6 +206,9 @@ gw_status gw_autotune(gw_ctx* ctx, uint64_t sample, gw_launch* best, double* bes
  * GW_RUN_BUCKETS; otherwise one pass over the walkers. */
 gw_status gw_bucket_counts(gw_ctx* ctx, uint64_t* W, uint64_t n);
 gw_status gw_walkers_left(gw_ctx* ctx, uint64_t* left);
+/* Unfinished walkers and the sum of their remaining hop caps (the work left; an upper
+ * bound under geometric termination).  Free after a gw_run with GW_RUN_BUCKETS. */
+gw_status gw_pending_work(gw_ctx* ctx, uint64_t* walkers, uint64_t* hops);
 
 /* ---- learned block scheduler support (no reference counterpart: the reference's
  *      GOwalker_scheduler_t, engine/schedule.hpp:70-138, is the plain rule this
@@ -214,7 +217,12 @@ gw_status gw_walkers_left(gw_ctx* ctx, uint64_t* left);
  * block held the walker's current vertex. */
 gw_status gw_trace_read(gw_ctx* ctx, uint64_t* block_steps, uint64_t n);
 /* 12 features per block (nblocks x 12, row major) from the current bucket matrix, the
- * resident set and a caller-kept age per block (iterations since last use; may be NULL). */
+ * resident set, the work left and a caller-kept age per block (iterations since last
+ * use; may be NULL): log1p of the block's row, column and diagonal walker mass, of the
+ * mass paired with resident blocks and of its smallest pairing, residency, log1p of the
+ * walkers a load would release, age / 64, log1p(edges) / 24, the mean fraction of the
+ * hop cap still ahead of unfinished walkers, log1p(mean degree) / 8, and the block's
+ * share of all waiting walkers. */
 gw_status gw_block_features(gw_ctx* ctx, const float* age, float* feats, uint64_t n);
 
 /* MLP in -> h1 -> h2 -> 1 (ReLU), BF16 weights, FP32 accumulation, trained on the GPU
