@@ -73,6 +73,12 @@ struct gw_ctx {
     uint32_t* d_misc = nullptr;            // [0] nwork, [1] queue
     unsigned long long* d_cnt = nullptr;   // [0..5] run counters, [6] left, [7] zero-copy reads, [8] hops
     unsigned long long* h_cnt = nullptr;   // pinned readback
+    unsigned long long* m_cnt = nullptr;   // its device-side (mapped) address
+    uint32_t* m_bucket = nullptr;          // mapped address of h_bucket
+    DevBlock* m_blocks = nullptr;          // mapped address of h_blocks_pinned
+    uint8_t* h_ring = nullptr;             // pinned ring for small transfers
+    uint8_t* m_ring = nullptr;
+    uint64_t ring_cap = 0, ring_pos = 0;
     uint64_t walker_bytes = 0;
 
     // bucket matrix of the walkers left by the last run (GW_RUN_BUCKETS) or pass
@@ -194,6 +200,48 @@ static void free_slots(gw_ctx* c)
 
 namespace gw {
 gw_status gen_fail(gw_ctx* ctx, gw_status st, const char* msg) { return fail(ctx, st, "%s", msg); }
+
+// A region of the pinned ring (256-byte aligned).  Regions are reused only after the
+// compute stream -- the only stream that reads or writes them -- has been synchronized.
+static uint64_t ring_take(gw_ctx* c, uint64_t bytes, hipError_t* e)
+{
+    bytes = (bytes + 255) & ~255ULL;
+    *e = hipSuccess;
+    if (c->ring_pos + bytes > c->ring_cap) {
+        *e = hipStreamSynchronize(c->s_compute);
+        c->ring_pos = 0;
+    }
+    const uint64_t at = c->ring_pos;
+    c->ring_pos += bytes;
+    return at;
+}
+
+hipError_t h2d_small(gw_ctx* c, void* dst, const void* src, uint64_t bytes)
+{
+    if (!bytes) return hipSuccess;
+    if (bytes % 4 || bytes > c->ring_cap / 2) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->s_compute);
+    hipError_t e;
+    const uint64_t at = ring_take(c, bytes, &e);
+    if (e != hipSuccess) return e;
+    memcpy(c->h_ring + at, src, bytes);
+    return launch_copy_u32((uint32_t*)dst, (const uint32_t*)(c->m_ring + at), bytes / 4, c->s_compute);
+}
+
+hipError_t d2h_small(gw_ctx* c, void* dst, const void* src, uint64_t bytes)
+{
+    if (!bytes) return hipSuccess;
+    hipError_t e;
+    if (bytes % 4 || bytes > c->ring_cap / 2) {
+        e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->s_compute);
+        return e == hipSuccess ? hipStreamSynchronize(c->s_compute) : e;
+    }
+    const uint64_t at = ring_take(c, bytes, &e);
+    if (e != hipSuccess) return e;
+    e = launch_copy_u32((uint32_t*)(c->m_ring + at), (const uint32_t*)src, bytes / 4, c->s_compute);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s_compute);
+    if (e == hipSuccess) memcpy(dst, c->h_ring + at, bytes);
+    return e;
+}
 hipStream_t ctx_stream(gw_ctx* ctx)
 {
     (void)hipSetDevice(ctx->device);
@@ -246,8 +294,13 @@ gw_status gw_create(const gw_config* cfg, gw_ctx** out)
     if (dalloc(c, &p, 16 * sizeof(unsigned long long), false) != GW_OK)
         return bail(fail(nullptr, GW_ERR_NOMEM, "%s", c->err.c_str()));
     c->d_cnt = (unsigned long long*)p;
-    if (hipHostMalloc((void**)&c->h_cnt, 16 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&c->h_cnt, 16 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->m_cnt, c->h_cnt, 0) != hipSuccess)
         return bail(fail(nullptr, GW_ERR_NOMEM, "pinned counter buffer"));
+    c->ring_cap = 4ull << 20;
+    if (hipHostMalloc((void**)&c->h_ring, c->ring_cap, hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->m_ring, c->h_ring, 0) != hipSuccess)
+        return bail(fail(nullptr, GW_ERR_NOMEM, "pinned transfer ring"));
     *out = c;
     return GW_OK;
 }
@@ -271,6 +324,7 @@ gw_status gw_destroy(gw_ctx* c)
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
     if (c->h_blocks_pinned) (void)hipHostFree(c->h_blocks_pinned);
     if (c->h_bucket) (void)hipHostFree(c->h_bucket);
+    if (c->h_ring) (void)hipHostFree(c->h_ring);
     hipEvent_t evs[] = {c->ev_copy, c->ev_f0, c->ev_f1, c->ev_k0, c->ev_k1, c->ev_c0, c->ev_c1};
     for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
     if (c->s_compute) (void)hipStreamDestroy(c->s_compute);
@@ -363,6 +417,8 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     c->d_age = (float*)p;
     HIPCHK(c, hipHostMalloc((void**)&c->h_blocks_pinned, sizeof(DevBlock) * nb, hipHostMallocDefault));
     HIPCHK(c, hipHostMalloc((void**)&c->h_bucket, sizeof(uint32_t) * (uint64_t)nb * nb, hipHostMallocDefault));
+    HIPCHK(c, hipHostGetDevicePointer((void**)&c->m_bucket, c->h_bucket, 0));
+    HIPCHK(c, hipHostGetDevicePointer((void**)&c->m_blocks, c->h_blocks_pinned, 0));
     HIPCHK(c, hipMemcpy(c->d_vcut, c->vcut.data(), sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice));
     return GW_OK;
 }
@@ -717,11 +773,9 @@ static gw_status enqueue_bucket_pass(gw_ctx* c)
     HIPCHK(c, hipEventRecord(c->ev_f0, c->s_compute));
     HIPCHK(c, launch_filter(f, c->ncu * 2, c->s_compute));
     HIPCHK(c, hipEventRecord(c->ev_f1, c->s_compute));
-    HIPCHK(c, hipMemcpyAsync(c->h_bucket, c->d_bucket, nn * 4, hipMemcpyDeviceToHost, c->s_compute));
-    HIPCHK(c, hipMemcpyAsync(c->h_cnt + 8, c->d_cnt + 6, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             c->s_compute));
-    HIPCHK(c, hipMemcpyAsync(c->h_cnt + 9, c->d_cnt + 8, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             c->s_compute));
+    HIPCHK(c, launch_copy_u32(c->m_bucket, c->d_bucket, nn, c->s_compute));
+    HIPCHK(c, launch_copy_u32((uint32_t*)(c->m_cnt + 8), (const uint32_t*)(c->d_cnt + 6), 2, c->s_compute));
+    HIPCHK(c, launch_copy_u32((uint32_t*)(c->m_cnt + 9), (const uint32_t*)(c->d_cnt + 8), 2, c->s_compute));
     return GW_OK;
 }
 
@@ -747,9 +801,12 @@ static gw_status push_block_table(gw_ctx* c)
             HIPCHK(c, hipStreamWaitEvent(c->s_compute, c->slot_ev1[s], 0));
             c->slot_wait[s] = 0;
         }
+    // the previous push has been consumed: every caller synchronizes the compute stream
+    // before it returns
     memcpy(c->h_blocks_pinned, c->hblk.data(), sizeof(DevBlock) * c->nblocks);
-    HIPCHK(c, hipMemcpyAsync(c->d_blocks, c->h_blocks_pinned, sizeof(DevBlock) * c->nblocks, hipMemcpyHostToDevice,
-                             c->s_compute));
+    static_assert(sizeof(DevBlock) % 4 == 0, "block descriptors are copied as words");
+    HIPCHK(c, launch_copy_u32((uint32_t*)c->d_blocks, (const uint32_t*)c->m_blocks, sizeof(DevBlock) / 4 * c->nblocks,
+                              c->s_compute));
     return GW_OK;
 }
 
@@ -853,8 +910,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     HIPCHK(c, hipEventRecord(c->ev_k0, c->s_compute));
     HIPCHK(c, launch_walk(a, one_blk, l.variant, (uint32_t)grid, l.block, c->s_compute));
     HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));
-    HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             c->s_compute));
+    HIPCHK(c, launch_copy_u32((uint32_t*)c->m_cnt, (const uint32_t*)c->d_cnt, 16, c->s_compute));
     const bool buckets = (l.flags & GW_RUN_BUCKETS) != 0;
     if (buckets) {
         // bucket matrix of the walkers left, read back with the counters (one host wait)
@@ -976,7 +1032,7 @@ gw_status gw_trace_read(gw_ctx* c, uint64_t* block_steps, uint64_t n)
     if (n < c->nblocks) return fail(c, GW_ERR_ARG, "trace buffer too small");
     (void)hipSetDevice(c->device);
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
-    HIPCHK(c, hipMemcpy(block_steps, c->d_trace, c->nblocks * 8ull, hipMemcpyDeviceToHost));
+    HIPCHK(c, d2h_small(c, block_steps, c->d_trace, c->nblocks * 8ull));
     return GW_OK;
 }
 
@@ -989,14 +1045,12 @@ gw_status gw_block_features(gw_ctx* c, const float* age, float* feats, uint64_t 
     gw_status st;
     if ((st = ensure_buckets(c)) != GW_OK) return st;
     if ((st = push_block_table(c)) != GW_OK) return st;
-    if (age) HIPCHK(c, hipMemcpyAsync(c->d_age, age, c->nblocks * 4ull, hipMemcpyHostToDevice, c->s_compute));
+    if (age) HIPCHK(c, h2d_small(c, c->d_age, age, c->nblocks * 4ull));
     const uint32_t maxlen = std::max(c->dt[0].length, c->wp.ntypes == 2 ? c->dt[1].length : 0u);
     const float ahead = c->unfinished && maxlen ? (float)((double)c->pending_hops / ((double)c->unfinished * maxlen)) : 0.f;
     HIPCHK(c, launch_features(c->d_bucket, c->nblocks, c->d_blocks, age ? c->d_age : nullptr, ahead, c->d_feat,
                               c->s_compute));
-    HIPCHK(c, hipMemcpyAsync(feats, c->d_feat, (uint64_t)c->nblocks * kNumFeatures * 4, hipMemcpyDeviceToHost,
-                             c->s_compute));
-    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    HIPCHK(c, d2h_small(c, feats, c->d_feat, (uint64_t)c->nblocks * kNumFeatures * 4));
     return GW_OK;
 }
 

@@ -4,6 +4,8 @@
 #include <cstdint>
 #include <hip/hip_runtime.h>
 
+#include "gw.h"
+
 namespace gw {
 
 constexpr uint64_t kFnvOffset = 1469598103934665603ULL;
@@ -109,6 +111,13 @@ hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s);
 hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid,
                        uint32_t block, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s);
+
+// Small transfers (multiples of 4 bytes, up to a few MB) on the context's compute stream
+// through its pinned ring and a copy kernel (no DMA engine).  h2d returns once the copy is
+// enqueued; d2h returns with the data on the host.
+hipError_t h2d_small(gw_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+hipError_t d2h_small(gw_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 uint32_t walk_blocks_per_cu(bool one_block, uint32_t variant, uint32_t block);   // resident workgroups per CU
 hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age, float ahead,
                            float* feats, hipStream_t s);

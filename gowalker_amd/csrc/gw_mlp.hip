@@ -343,7 +343,7 @@ static gw_status mlp_fwd(gw_ctx* ctx, gw_mlp* m, const float* x, uint32_t n, boo
     gw_status st;
     if ((st = mlp_reserve(ctx, m, n)) != GW_OK) return st;
     hipStream_t s = ctx_stream(ctx);
-    if (hipMemcpyAsync(m->x, x, (uint64_t)n * m->s.in * 4, hipMemcpyHostToDevice, s) != hipSuccess)
+    if (gw::h2d_small(ctx, m->x, x, (uint64_t)n * m->s.in * 4) != hipSuccess)
         return gen_fail(ctx, GW_ERR_HIP, "MLP input upload failed");
     const uint32_t lds = m->s.nparams * 4;
     hipLaunchKernelGGL(mlp_forward_kernel, dim3((n + 255) / 256), dim3(256), lds, s, m->s, m->wbf, m->x, n, m->y,
@@ -358,8 +358,8 @@ extern "C" gw_status gw_mlp_forward(gw_ctx* ctx, gw_mlp* m, const float* x, uint
     gw_status st;
     if ((st = mlp_fwd(ctx, m, x, n, false)) != GW_OK) return st;
     hipStream_t s = ctx_stream(ctx);
-    if (hipMemcpyAsync(y, m->y, n * 4, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
-        return gen_fail(ctx, GW_ERR_HIP, "MLP forward failed");
+    (void)s;
+    if (gw::d2h_small(ctx, y, m->y, n * 4ull) != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP forward failed");
     return GW_OK;
 }
 
@@ -368,8 +368,7 @@ static gw_status mlp_grad(gw_ctx* ctx, gw_mlp* m, const float* x, const float* t
     gw_status st;
     if ((st = mlp_fwd(ctx, m, x, n, true)) != GW_OK) return st;
     hipStream_t s = ctx_stream(ctx);
-    if (hipMemcpyAsync(m->t, t, n * 4, hipMemcpyHostToDevice, s) != hipSuccess)
-        return gen_fail(ctx, GW_ERR_HIP, "MLP target upload failed");
+    if (gw::h2d_small(ctx, m->t, t, n * 4ull) != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP target upload failed");
     const uint32_t lds = m->s.nparams * 4;
     hipLaunchKernelGGL(mlp_delta_kernel, dim3((n + 255) / 256), dim3(256), lds, s, m->s, m->wbf, m->y, m->t, n, m->z1,
                        m->z2, m->dy, m->dz1, m->dz2);
@@ -387,10 +386,9 @@ extern "C" gw_status gw_mlp_backward(gw_ctx* ctx, gw_mlp* m, const float* x, con
     if (!ctx || !m || !x || !t || n == 0) return gen_fail(ctx, GW_ERR_ARG, "bad MLP backward arguments");
     gw_status st;
     if ((st = mlp_grad(ctx, m, x, t, n)) != GW_OK) return st;
-    hipStream_t s = ctx_stream(ctx);
-    if ((grad && hipMemcpyAsync(grad, m->g, m->s.nparams * 4, hipMemcpyDeviceToHost, s) != hipSuccess) ||
-        (loss && hipMemcpyAsync(loss, m->loss, 4, hipMemcpyDeviceToHost, s) != hipSuccess) ||
-        hipStreamSynchronize(s) != hipSuccess)
+    if ((grad && gw::d2h_small(ctx, grad, m->g, m->s.nparams * 4ull) != hipSuccess) ||
+        (loss && gw::d2h_small(ctx, loss, m->loss, 4) != hipSuccess) ||
+        hipStreamSynchronize(ctx_stream(ctx)) != hipSuccess)
         return gen_fail(ctx, GW_ERR_HIP, "MLP gradient readback failed");
     return GW_OK;
 }
@@ -409,8 +407,7 @@ extern "C" gw_status gw_mlp_step(gw_ctx* ctx, gw_mlp* m, const float* x, const f
     hipLaunchKernelGGL(mlp_adam_kernel, dim3((m->s.nparams + 255) / 256), dim3(256), 0, s, m->s.nparams, m->master,
                        m->wbf, m->g, m->m1, m->m2, lr, b1, b2, 1e-8f, c1, c2);
     if (hipGetLastError() != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP update launch failed");
-    if ((loss && hipMemcpyAsync(loss, m->loss, 4, hipMemcpyDeviceToHost, s) != hipSuccess) ||
-        hipStreamSynchronize(s) != hipSuccess)
+    if ((loss && gw::d2h_small(ctx, loss, m->loss, 4) != hipSuccess) || hipStreamSynchronize(s) != hipSuccess)
         return gen_fail(ctx, GW_ERR_HIP, "MLP step failed");
     return GW_OK;
 }

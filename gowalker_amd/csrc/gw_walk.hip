@@ -1037,6 +1037,23 @@ uint32_t walk_blocks_per_cu(bool one_block, uint32_t variant, uint32_t block)
     return (e == hipSuccess && n > 0) ? (uint32_t)n : 1u;
 }
 
+// Word copy by a kernel: moves small host<->device transfers through pinned, mapped host
+// memory instead of the DMA engines, so they never queue behind large block uploads.
+__global__ void copy_u32_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(copy_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s)
 {
     if (n == 0) return hipSuccess;

@@ -71,17 +71,21 @@ def test_stage_commit_features_pending(gpu):
         assert ctx.staged(2) == (2, True)
         ctx.commit(2)
         assert ctx.staged(2)[0] == -1
-        # finish with a sliding window: the heaviest (prev, cur) pair goes to slots 3 and 0
-        for _ in range(500):
+        # a window of two slots (2 and 3) moves over the heaviest (prev, cur) pairs for a
+        # while, then every block is loaded and the walk finishes
+        for _ in range(20):
             W = ctx.bucket_counts()
-            if W.sum() == 0:
-                break
             a, b = (int(x) for x in np.unravel_index(np.argmax(W), W.shape))
             ctx.upload(3, a)
             if b != a:
-                ctx.upload(0, b)
+                ctx.upload(2, b)
             ctx.wait_uploads()
             assert ctx.run()["walkers_run"] > 0
+        ctx.reserve_slots(nb)
+        for blk in range(nb):
+            ctx.upload(blk, blk)
+        ctx.wait_uploads()
+        assert ctx.run()["walkers_left"] == 0
         assert ctx.bucket_counts().sum() == 0
         hashes, visits = ctx.hashes(), ctx.visits()
     o = oracle(beg, csr, 5, n)
