@@ -160,6 +160,166 @@ __global__ void __launch_bounds__(256) mlp_adam_kernel(uint32_t np, float* __res
     wbf[p] = f32_to_bf16(v);
 }
 
+// ------------------------------------------------------------------------------------
+// Fused single-workgroup kernel for MLPs whose weights and a chunk of activations fit in
+// 64 KB of LDS (the 12-64-64-1 block scorer uses 54 KB): the batch streams through LDS in
+// chunks of R samples -- forward, per-sample deltas and the per-parameter gradient sums
+// in one launch, each thread owning up to kFusedOwn parameters whose gradients stay in
+// registers -- and, in Adam mode, the update is applied at the end.  Every sum runs in a
+// fixed order, so results are deterministic.
+// MODE 0: forward (y); 1: gradient (g, loss); 2: gradient + Adam step.
+// ------------------------------------------------------------------------------------
+constexpr uint32_t kFusedThreads = 1024, kFusedOwn = 16, kFusedLdsFloats = 16384;
+
+struct AdamArgs {
+    float* master;
+    float* m1;
+    float* m2;
+    uint16_t* wbf_out;
+    float lr, b1, b2, eps, c1, c2;
+};
+
+__device__ __forceinline__ float relu(float v) { return v > 0.f ? v : 0.f; }
+
+__device__ __forceinline__ float wave_sum_f(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R, const uint16_t* wbf,
+                                                          const float* __restrict__ x, const float* __restrict__ t,
+                                                          uint32_t n, float* __restrict__ y, float* __restrict__ g,
+                                                          float* __restrict__ loss, AdamArgs adam)
+{
+    extern __shared__ float lds[];
+    __shared__ float red[kFusedThreads / 64];
+    float* sW = lds;
+    float* sX = sW + s.nparams;
+    float* sZ1 = sX + R * s.in;
+    float* sZ2 = sZ1 + R * s.h1;
+    float* sD1 = sZ2 + R * s.h2;
+    float* sD2 = sD1 + R * s.h1;
+    float* sDY = sD2 + R * s.h2;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid % 64, wave = tid / 64;
+    for (uint32_t i = tid; i < s.nparams; i += nt) sW[i] = bf16_to_f32(wbf[i]);
+    float acc[kFusedOwn];
+#pragma unroll
+    for (uint32_t i = 0; i < kFusedOwn; ++i) acc[i] = 0.f;
+    float lsum = 0.f;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < n; c0 += R) {
+        const uint32_t rows = n - c0 < R ? n - c0 : R;
+        for (uint32_t i = tid; i < rows * s.in; i += nt) sX[i] = x[(uint64_t)c0 * s.in + i];
+        __syncthreads();
+        for (uint32_t o = tid; o < rows * s.h1; o += nt) {
+            const uint32_t r = o / s.h1, j = o % s.h1;
+            float a = sW[s.o_b1 + j];
+            for (uint32_t k = 0; k < s.in; ++k) a = fmaf(sW[s.o_w1 + j * s.in + k], sX[r * s.in + k], a);
+            sZ1[o] = a;
+        }
+        __syncthreads();
+        for (uint32_t o = tid; o < rows * s.h2; o += nt) {
+            const uint32_t r = o / s.h2, j = o % s.h2;
+            float a = sW[s.o_b2 + j];
+            for (uint32_t k = 0; k < s.h1; ++k) a = fmaf(sW[s.o_w2 + j * s.h1 + k], relu(sZ1[r * s.h1 + k]), a);
+            sZ2[o] = a;
+        }
+        __syncthreads();
+        for (uint32_t r = wave; r < rows; r += nt / 64) {
+            float v = 0.f;
+            for (uint32_t j = lane; j < s.h2; j += 64) v = fmaf(sW[s.o_w3 + j], relu(sZ2[r * s.h2 + j]), v);
+            v = wave_sum_f(v) + sW[s.o_b3];
+            if (lane == 0) {
+                if (MODE == 0) {
+                    y[c0 + r] = v;
+                } else {
+                    const float d = v - t[c0 + r];
+                    lsum += d * d;
+                    sDY[r] = 2.f * d / (float)n;
+                }
+            }
+        }
+        __syncthreads();
+        if (MODE == 0) continue;
+        for (uint32_t o = tid; o < rows * s.h2; o += nt) {
+            const uint32_t r = o / s.h2, j = o % s.h2;
+            sD2[o] = sZ2[o] > 0.f ? sW[s.o_w3 + j] * sDY[r] : 0.f;
+        }
+        __syncthreads();
+        for (uint32_t o = tid; o < rows * s.h1; o += nt) {
+            const uint32_t r = o / s.h1, k = o % s.h1;
+            float a = 0.f;
+            for (uint32_t j = 0; j < s.h2; ++j) a = fmaf(sW[s.o_w2 + j * s.h1 + k], sD2[r * s.h2 + j], a);
+            sD1[o] = sZ1[o] > 0.f ? a : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t i = 0; i < kFusedOwn; ++i) {
+            const uint32_t p = tid + i * nt;
+            if (p >= s.nparams) break;
+            float a = 0.f;
+            if (p < s.o_b1) {
+                const uint32_t j = p / s.in, k = p % s.in;
+                for (uint32_t r = 0; r < rows; ++r) a = fmaf(sD1[r * s.h1 + j], sX[r * s.in + k], a);
+            } else if (p < s.o_w2) {
+                const uint32_t j = p - s.o_b1;
+                for (uint32_t r = 0; r < rows; ++r) a += sD1[r * s.h1 + j];
+            } else if (p < s.o_b2) {
+                const uint32_t q = p - s.o_w2, j = q / s.h1, k = q % s.h1;
+                for (uint32_t r = 0; r < rows; ++r) a = fmaf(sD2[r * s.h2 + j], relu(sZ1[r * s.h1 + k]), a);
+            } else if (p < s.o_w3) {
+                const uint32_t j = p - s.o_b2;
+                for (uint32_t r = 0; r < rows; ++r) a += sD2[r * s.h2 + j];
+            } else if (p < s.o_b3) {
+                const uint32_t j = p - s.o_w3;
+                for (uint32_t r = 0; r < rows; ++r) a = fmaf(sDY[r], relu(sZ2[r * s.h2 + j]), a);
+            } else {
+                for (uint32_t r = 0; r < rows; ++r) a += sDY[r];
+            }
+            acc[i] += a;
+        }
+        __syncthreads();
+    }
+    if (MODE == 0) return;
+    // batch loss: lane 0 of every wavefront holds its rows' squared errors
+    if (lane == 0) red[wave] = lsum;
+    __syncthreads();
+    if (tid == 0) {
+        float tot = 0.f;
+        for (uint32_t w = 0; w < nt / 64; ++w) tot += red[w];
+        loss[0] = tot / (float)n;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kFusedOwn; ++i) {
+        const uint32_t p = tid + i * nt;
+        if (p >= s.nparams) break;
+        g[p] = acc[i];
+        if (MODE == 2) {
+            const float gi = acc[i];
+            const float a = adam.b1 * adam.m1[p] + (1.f - adam.b1) * gi;
+            const float b = adam.b2 * adam.m2[p] + (1.f - adam.b2) * gi * gi;
+            adam.m1[p] = a;
+            adam.m2[p] = b;
+            const float v = adam.master[p] - adam.lr * (a / adam.c1) / (sqrtf(b / adam.c2) + adam.eps);
+            adam.master[p] = v;
+            adam.wbf_out[p] = f32_to_bf16(v);
+        }
+    }
+}
+
+// chunk of samples that fits next to the weights in LDS (0: the shape needs the
+// multi-kernel path)
+__host__ inline uint32_t fused_rows(const MlpShape& s)
+{
+    if (s.nparams > kFusedThreads * kFusedOwn || s.nparams >= kFusedLdsFloats) return 0;
+    const uint32_t per = s.in + 2 * s.h1 + 2 * s.h2 + 1;
+    uint32_t r = (kFusedLdsFloats - s.nparams) / per;
+    if (r > 64) r = 64;
+    return r >= 8 ? r : 0;
+}
+
 __global__ void mlp_sqerr_kernel(const float* y, const float* t, uint32_t n, float* out)
 {
     __shared__ float red[256];
@@ -352,10 +512,46 @@ static gw_status mlp_fwd(gw_ctx* ctx, gw_mlp* m, const float* x, uint32_t n, boo
     return GW_OK;
 }
 
+// Fused path: inputs (and targets) through the transfer ring, one single-workgroup launch.
+static gw_status mlp_fused(gw_ctx* ctx, gw_mlp* m, uint32_t R, int mode, const float* x, const float* t, uint32_t n,
+                           float lr)
+{
+    gw_status st;
+    if ((st = mlp_reserve(ctx, m, n)) != GW_OK) return st;
+    hipStream_t s = ctx_stream(ctx);
+    if (gw::h2d_small(ctx, m->x, x, (uint64_t)n * m->s.in * 4) != hipSuccess ||
+        (t && gw::h2d_small(ctx, m->t, t, n * 4ull) != hipSuccess))
+        return gen_fail(ctx, GW_ERR_HIP, "MLP input upload failed");
+    const MlpShape& sh = m->s;
+    const size_t lds = ((size_t)sh.nparams + (size_t)R * (sh.in + 2 * sh.h1 + 2 * sh.h2 + 1)) * 4;
+    AdamArgs a{m->master, m->m1, m->m2, m->wbf, lr, 0.9f, 0.999f, 1e-8f, 1.f, 1.f};
+    if (mode == 2) {
+        m->steps++;
+        a.c1 = 1.f - std::pow(a.b1, (float)m->steps);
+        a.c2 = 1.f - std::pow(a.b2, (float)m->steps);
+    }
+    if (mode == 0)
+        hipLaunchKernelGGL(mlp_fused_kernel<0>, dim3(1), dim3(kFusedThreads), lds, s, sh, R, m->wbf, m->x, m->t, n,
+                           m->y, m->g, m->loss, a);
+    else if (mode == 1)
+        hipLaunchKernelGGL(mlp_fused_kernel<1>, dim3(1), dim3(kFusedThreads), lds, s, sh, R, m->wbf, m->x, m->t, n,
+                           m->y, m->g, m->loss, a);
+    else
+        hipLaunchKernelGGL(mlp_fused_kernel<2>, dim3(1), dim3(kFusedThreads), lds, s, sh, R, m->wbf, m->x, m->t, n,
+                           m->y, m->g, m->loss, a);
+    if (hipGetLastError() != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP kernel launch failed");
+    return GW_OK;
+}
+
 extern "C" gw_status gw_mlp_forward(gw_ctx* ctx, gw_mlp* m, const float* x, uint32_t n, float* y)
 {
     if (!ctx || !m || !x || !y || n == 0) return gen_fail(ctx, GW_ERR_ARG, "bad MLP forward arguments");
     gw_status st;
+    if (const uint32_t R = fused_rows(m->s)) {
+        if ((st = mlp_fused(ctx, m, R, 0, x, nullptr, n, 0.f)) != GW_OK) return st;
+        if (gw::d2h_small(ctx, y, m->y, n * 4ull) != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP forward failed");
+        return GW_OK;
+    }
     if ((st = mlp_fwd(ctx, m, x, n, false)) != GW_OK) return st;
     hipStream_t s = ctx_stream(ctx);
     (void)s;
@@ -385,7 +581,8 @@ extern "C" gw_status gw_mlp_backward(gw_ctx* ctx, gw_mlp* m, const float* x, con
 {
     if (!ctx || !m || !x || !t || n == 0) return gen_fail(ctx, GW_ERR_ARG, "bad MLP backward arguments");
     gw_status st;
-    if ((st = mlp_grad(ctx, m, x, t, n)) != GW_OK) return st;
+    const uint32_t R = fused_rows(m->s);
+    if ((st = R ? mlp_fused(ctx, m, R, 1, x, t, n, 0.f) : mlp_grad(ctx, m, x, t, n)) != GW_OK) return st;
     if ((grad && gw::d2h_small(ctx, grad, m->g, m->s.nparams * 4ull) != hipSuccess) ||
         (loss && gw::d2h_small(ctx, loss, m->loss, 4) != hipSuccess) ||
         hipStreamSynchronize(ctx_stream(ctx)) != hipSuccess)
@@ -399,8 +596,14 @@ extern "C" gw_status gw_mlp_step(gw_ctx* ctx, gw_mlp* m, const float* x, const f
 {
     if (!ctx || !m || !x || !t || n == 0 || !(lr > 0.f)) return gen_fail(ctx, GW_ERR_ARG, "bad MLP step arguments");
     gw_status st;
-    if ((st = mlp_grad(ctx, m, x, t, n)) != GW_OK) return st;
     hipStream_t s = ctx_stream(ctx);
+    if (const uint32_t R = fused_rows(m->s)) {
+        if ((st = mlp_fused(ctx, m, R, 2, x, t, n, lr)) != GW_OK) return st;
+        if ((loss && gw::d2h_small(ctx, loss, m->loss, 4) != hipSuccess) || hipStreamSynchronize(s) != hipSuccess)
+            return gen_fail(ctx, GW_ERR_HIP, "MLP step failed");
+        return GW_OK;
+    }
+    if ((st = mlp_grad(ctx, m, x, t, n)) != GW_OK) return st;
     m->steps++;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - std::pow(b1, (float)m->steps), c2 = 1.f - std::pow(b2, (float)m->steps);

@@ -44,8 +44,9 @@ struct LearnedConfig {
     float lr = 2e-3f;
     double zc_bytes_per_step = 360; // PCIe bytes of one zero-copy step (5.7 measured 64 B reads)
     double tail_margin = 1.0;       // zero-copy once steps x zc bytes x margin < block bytes
-    double tail_guard = 16;         // ... and the work left x zc bytes < guard x block bytes
-    uint32_t max_stage = 64;        // blocks staged per iteration
+    double tail_guard = 256;        // ... and the work left x zc bytes < guard x block bytes
+    uint32_t max_inflight = 4;      // staged copies in flight at once (a deeper copy queue
+                                    // makes the host block when it enqueues)
     double judge_error = 1.0;       // mean |log error| of recent predictions above this => fall back
     uint32_t cooldown = 8;
 };
@@ -114,11 +115,15 @@ public:
         const uint32_t nb = in.nblocks;
         std::vector<char> skip(nb, 0);
         for (uint32_t b : buckets) skip[b] = 1;
+        uint32_t inflight = 0;
+        for (uint32_t b = 0; b < nb; ++b)
+            if (in.staged && in.staged[b] == 1 && !skip[b]) ++inflight;
+        if (inflight >= cfg_.max_inflight) return;
         std::vector<uint32_t> cand;
         for (uint32_t b = 0; b < nb; ++b)
             if (!skip[b] && !(in.staged && in.staged[b]) && demand_[b] > 0) cand.push_back(b);
         std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return rank_[a] > rank_[b]; });
-        const uint32_t room = std::min<uint32_t>({free_slots, cfg_.max_stage, (uint32_t)(in.ncblock - buckets.size())});
+        const uint32_t room = std::min<uint32_t>({free_slots, cfg_.max_inflight - inflight, (uint32_t)(in.ncblock - buckets.size())});
         for (uint32_t i = 0; i < cand.size() && i < room; ++i) out.push_back(cand[i]);
     }
 

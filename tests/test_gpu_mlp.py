@@ -1,7 +1,9 @@
 """The learned scheduler's MLP kernels (BF16 weights, FP32 accumulation) against an
 independent numpy fp64 reference on the same bf16-rounded weights: forward values,
-backward gradients (also pinned by finite differences in test_np_ref.py), the Adam step,
-and that training fits a known function."""
+backward gradients (also pinned by finite differences in test_np_ref.py), one Adam step
+against a numpy fp32 restatement, determinism, and that training fits a known function.
+The 12-64-64 scorer runs the fused single-workgroup kernel; 16-128-128 does not fit its
+LDS budget and runs the multi-kernel path."""
 import ctypes as C
 
 import numpy as np
@@ -17,10 +19,11 @@ NIN, H1, H2 = 12, 64, 64
 
 
 class Mlp:
-    def __init__(self, ctx, seed=3):
+    def __init__(self, ctx, seed=3, shape=(NIN, H1, H2)):
         self.ctx = ctx
         self.h = C.c_void_p()
-        check(lib().gw_mlp_create(ctx.h, NIN, H1, H2, seed, C.byref(self.h)), ctx.h)
+        self.shape = shape
+        check(lib().gw_mlp_create(ctx.h, shape[0], shape[1], shape[2], seed, C.byref(self.h)), ctx.h)
         n = C.c_uint32()
         check(lib().gw_mlp_num_params(self.h, C.byref(n)))
         self.n = n.value
@@ -73,22 +76,52 @@ def test_params_are_bf16_and_roundtrip(gpu):
         m.close()
 
 
-def test_forward_and_backward_match_reference(gpu):
+@pytest.mark.parametrize("shape", [(12, 64, 64), (16, 128, 128)])
+def test_forward_and_backward_match_reference(gpu, shape):
     rng = np.random.default_rng(5)
+    nin, h1, h2 = shape
+    scale = 0.3 * (64 / h1) ** 0.5
     with E.Context() as ctx:
-        m = Mlp(ctx)
-        p = R.bf16_round(rng.normal(0, 0.3, m.n).astype(np.float32))
+        m = Mlp(ctx, shape=shape)
+        p = R.bf16_round(rng.normal(0, scale, m.n).astype(np.float32))
         m.set_params(p)
-        x = rng.normal(0, 1, (777, NIN)).astype(np.float32)
+        x = rng.normal(0, 1, (777, nin)).astype(np.float32)
         t = rng.normal(0, 1, 777).astype(np.float32)
         y = m.forward(x)
-        yr, _ = R.mlp_forward(p.astype(np.float64), x.astype(np.float64), NIN, H1, H2)
-        # fp32 accumulation over at most 64 terms: 1e-5 relative to the activation scale
+        yr, _ = R.mlp_forward(p.astype(np.float64), x.astype(np.float64), nin, h1, h2)
+        # fp32 accumulation over at most 128 terms: 1e-5 relative to the activation scale
         np.testing.assert_allclose(y, yr, rtol=1e-4, atol=1e-4 * np.abs(yr).max())
         g, loss = m.backward(x, t)
-        gr, lr = R.mlp_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), NIN, H1, H2)
+        gr, lr = R.mlp_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), nin, h1, h2)
         assert abs(loss - lr) < 1e-4 * max(1.0, lr)
         np.testing.assert_allclose(g, gr, rtol=2e-3, atol=2e-5 * np.abs(gr).max())
+        # fixed summation order: a second call gives the same bits
+        g2, loss2 = m.backward(x, t)
+        np.testing.assert_array_equal(g, g2)
+        assert loss == loss2
+        m.close()
+
+
+def test_adam_step_matches_restatement(gpu):
+    rng = np.random.default_rng(13)
+    with E.Context() as ctx:
+        m = Mlp(ctx, seed=2)
+        p = m.params()
+        x = rng.normal(0, 1, (300, NIN)).astype(np.float32)
+        t = rng.normal(0, 1, 300).astype(np.float32)
+        g, _ = m.backward(x, t)
+        lr = np.float32(1e-3)
+        m.step(x, t, float(lr))
+        f = np.float32
+        a = (f(1) - f(0.9)) * g
+        b = (f(1) - f(0.999)) * g * g
+        c1, c2 = f(1) - f(0.9), f(1) - f(0.999)
+        want = R.bf16_round((p - lr * (a / c1) / (np.sqrt(b / c2) + f(1e-8))).astype(np.float32))
+        got = m.params()
+        # within one bf16 ulp (fused multiply-adds may move the fp32 value by an ulp)
+        ulp = np.abs(want) * 2.0 ** -7 + 1e-30
+        assert np.all(np.abs(got - want) <= ulp)
+        assert np.mean(got == want) > 0.99
         m.close()
 
 
