@@ -193,15 +193,18 @@ __global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R,
                                                           uint32_t n, float* __restrict__ y, float* __restrict__ g,
                                                           float* __restrict__ loss, AdamArgs adam)
 {
+    // LDS: weights, then the chunk's activations with odd row strides (in+1, h+1) so that
+    // lanes walking consecutive rows hit distinct banks; weight reads are broadcasts
     extern __shared__ float lds[];
     __shared__ float red[kFusedThreads / 64];
+    const uint32_t xs = s.in + 1, z1s = s.h1 + 1, z2s = s.h2 + 1;
     float* sW = lds;
     float* sX = sW + s.nparams;
-    float* sZ1 = sX + R * s.in;
-    float* sZ2 = sZ1 + R * s.h1;
-    float* sD1 = sZ2 + R * s.h2;
-    float* sD2 = sD1 + R * s.h1;
-    float* sDY = sD2 + R * s.h2;
+    float* sZ1 = sX + R * xs;
+    float* sZ2 = sZ1 + R * z1s;
+    float* sD1 = sZ2 + R * z2s;
+    float* sD2 = sD1 + R * z1s;
+    float* sDY = sD2 + R * z2s;
     const uint32_t tid = threadIdx.x, nt = blockDim.x, lane = tid % 64, wave = tid / 64;
     for (uint32_t i = tid; i < s.nparams; i += nt) sW[i] = bf16_to_f32(wbf[i]);
     float acc[kFusedOwn];
@@ -211,25 +214,28 @@ __global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R,
     __syncthreads();
     for (uint32_t c0 = 0; c0 < n; c0 += R) {
         const uint32_t rows = n - c0 < R ? n - c0 : R;
-        for (uint32_t i = tid; i < rows * s.in; i += nt) sX[i] = x[(uint64_t)c0 * s.in + i];
+        for (uint32_t i = tid; i < rows * s.in; i += nt) sX[(i / s.in) * xs + i % s.in] = x[(uint64_t)c0 * s.in + i];
         __syncthreads();
-        for (uint32_t o = tid; o < rows * s.h1; o += nt) {
-            const uint32_t r = o / s.h1, j = o % s.h1;
+        // outputs (r, j) with r fastest across lanes: weight row j is a broadcast
+        for (uint32_t o = tid; o < R * s.h1; o += nt) {
+            const uint32_t r = o % R, j = o / R;
+            if (r >= rows) continue;
             float a = sW[s.o_b1 + j];
-            for (uint32_t k = 0; k < s.in; ++k) a = fmaf(sW[s.o_w1 + j * s.in + k], sX[r * s.in + k], a);
-            sZ1[o] = a;
+            for (uint32_t k = 0; k < s.in; ++k) a = fmaf(sW[s.o_w1 + j * s.in + k], sX[r * xs + k], a);
+            sZ1[r * z1s + j] = a;
         }
         __syncthreads();
-        for (uint32_t o = tid; o < rows * s.h2; o += nt) {
-            const uint32_t r = o / s.h2, j = o % s.h2;
+        for (uint32_t o = tid; o < R * s.h2; o += nt) {
+            const uint32_t r = o % R, j = o / R;
+            if (r >= rows) continue;
             float a = sW[s.o_b2 + j];
-            for (uint32_t k = 0; k < s.h1; ++k) a = fmaf(sW[s.o_w2 + j * s.h1 + k], relu(sZ1[r * s.h1 + k]), a);
-            sZ2[o] = a;
+            for (uint32_t k = 0; k < s.h1; ++k) a = fmaf(sW[s.o_w2 + j * s.h1 + k], relu(sZ1[r * z1s + k]), a);
+            sZ2[r * z2s + j] = a;
         }
         __syncthreads();
         for (uint32_t r = wave; r < rows; r += nt / 64) {
             float v = 0.f;
-            for (uint32_t j = lane; j < s.h2; j += 64) v = fmaf(sW[s.o_w3 + j], relu(sZ2[r * s.h2 + j]), v);
+            for (uint32_t j = lane; j < s.h2; j += 64) v = fmaf(sW[s.o_w3 + j], relu(sZ2[r * z2s + j]), v);
             v = wave_sum_f(v) + sW[s.o_b3];
             if (lane == 0) {
                 if (MODE == 0) {
@@ -243,18 +249,21 @@ __global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R,
         }
         __syncthreads();
         if (MODE == 0) continue;
-        for (uint32_t o = tid; o < rows * s.h2; o += nt) {
-            const uint32_t r = o / s.h2, j = o % s.h2;
-            sD2[o] = sZ2[o] > 0.f ? sW[s.o_w3 + j] * sDY[r] : 0.f;
+        for (uint32_t o = tid; o < R * s.h2; o += nt) {
+            const uint32_t r = o % R, j = o / R;
+            if (r >= rows) continue;
+            sD2[r * z2s + j] = sZ2[r * z2s + j] > 0.f ? sW[s.o_w3 + j] * sDY[r] : 0.f;
         }
         __syncthreads();
-        for (uint32_t o = tid; o < rows * s.h1; o += nt) {
-            const uint32_t r = o / s.h1, k = o % s.h1;
+        for (uint32_t o = tid; o < R * s.h1; o += nt) {
+            const uint32_t r = o % R, k = o / R;
+            if (r >= rows) continue;
             float a = 0.f;
-            for (uint32_t j = 0; j < s.h2; ++j) a = fmaf(sW[s.o_w2 + j * s.h1 + k], sD2[r * s.h2 + j], a);
-            sD1[o] = sZ1[o] > 0.f ? a : 0.f;
+            for (uint32_t j = 0; j < s.h2; ++j) a = fmaf(sW[s.o_w2 + j * s.h1 + k], sD2[r * z2s + j], a);
+            sD1[r * z1s + k] = sZ1[r * z1s + k] > 0.f ? a : 0.f;
         }
         __syncthreads();
+        // gradient sums over the chunk's rows for the parameters this thread owns
 #pragma unroll
         for (uint32_t i = 0; i < kFusedOwn; ++i) {
             const uint32_t p = tid + i * nt;
@@ -262,19 +271,19 @@ __global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R,
             float a = 0.f;
             if (p < s.o_b1) {
                 const uint32_t j = p / s.in, k = p % s.in;
-                for (uint32_t r = 0; r < rows; ++r) a = fmaf(sD1[r * s.h1 + j], sX[r * s.in + k], a);
+                for (uint32_t r = 0; r < rows; ++r) a = fmaf(sD1[r * z1s + j], sX[r * xs + k], a);
             } else if (p < s.o_w2) {
                 const uint32_t j = p - s.o_b1;
-                for (uint32_t r = 0; r < rows; ++r) a += sD1[r * s.h1 + j];
+                for (uint32_t r = 0; r < rows; ++r) a += sD1[r * z1s + j];
             } else if (p < s.o_b2) {
                 const uint32_t q = p - s.o_w2, j = q / s.h1, k = q % s.h1;
-                for (uint32_t r = 0; r < rows; ++r) a = fmaf(sD2[r * s.h2 + j], relu(sZ1[r * s.h1 + k]), a);
+                for (uint32_t r = 0; r < rows; ++r) a = fmaf(sD2[r * z2s + j], relu(sZ1[r * z1s + k]), a);
             } else if (p < s.o_w3) {
                 const uint32_t j = p - s.o_b2;
-                for (uint32_t r = 0; r < rows; ++r) a += sD2[r * s.h2 + j];
+                for (uint32_t r = 0; r < rows; ++r) a += sD2[r * z2s + j];
             } else if (p < s.o_b3) {
                 const uint32_t j = p - s.o_w3;
-                for (uint32_t r = 0; r < rows; ++r) a = fmaf(sDY[r], relu(sZ2[r * s.h2 + j]), a);
+                for (uint32_t r = 0; r < rows; ++r) a = fmaf(sDY[r], relu(sZ2[r * z2s + j]), a);
             } else {
                 for (uint32_t r = 0; r < rows; ++r) a += sDY[r];
             }
@@ -314,7 +323,7 @@ __global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R,
 __host__ inline uint32_t fused_rows(const MlpShape& s)
 {
     if (s.nparams > kFusedThreads * kFusedOwn || s.nparams >= kFusedLdsFloats) return 0;
-    const uint32_t per = s.in + 2 * s.h1 + 2 * s.h2 + 1;
+    const uint32_t per = (s.in + 1) + 2 * (s.h1 + 1) + 2 * (s.h2 + 1) + 1;
     uint32_t r = (kFusedLdsFloats - s.nparams) / per;
     if (r > 64) r = 64;
     return r >= 8 ? r : 0;
@@ -523,7 +532,7 @@ static gw_status mlp_fused(gw_ctx* ctx, gw_mlp* m, uint32_t R, int mode, const f
         (t && gw::h2d_small(ctx, m->t, t, n * 4ull) != hipSuccess))
         return gen_fail(ctx, GW_ERR_HIP, "MLP input upload failed");
     const MlpShape& sh = m->s;
-    const size_t lds = ((size_t)sh.nparams + (size_t)R * (sh.in + 2 * sh.h1 + 2 * sh.h2 + 1)) * 4;
+    const size_t lds = ((size_t)sh.nparams + (size_t)R * ((sh.in + 1) + 2 * (sh.h1 + 1) + 2 * (sh.h2 + 1) + 1)) * 4;
     AdamArgs a{m->master, m->m1, m->m2, m->wbf, lr, 0.9f, 0.999f, 1e-8f, 1.f, 1.f};
     if (mode == 2) {
         m->steps++;

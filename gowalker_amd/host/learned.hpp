@@ -1,10 +1,10 @@
 // learned.hpp -- learned block scheduler: a 12-64-64-1 MLP (BF16 weights, FP32
 // accumulation, trained on the GPU by libgw.so's kernels) predicts the walk steps that
-// loading a block buys, i.e. the steps of the run that follows its upload (log1p scale),
-// from the block's 12 features at decision time (gw_block_features).  It is trained
-// online on what the engine records: every run that followed exactly one new block gives
-// one sample (that block's features, the run's steps); the block-access trace
-// (gw_trace_read) keeps the per-block ages.
+// loading a block buys over its stay in the cache (log1p scale), from the block's 12
+// features at decision time (gw_block_features).  It is trained online on what the
+// engine records: a load opens a sample (the features, the steps of the run that
+// followed it) and every later run adds the steps the block served as the walkers'
+// current block (the block-access trace, gw_trace_read) until its eviction closes it.
 //
 // Decisions, given the predictions y(b):
 //   fill   (free slots left): commit staged blocks whose copies are done, else the best
@@ -20,8 +20,8 @@
 //          apps/Node2vec.cu:100-104, is the plain rule's version of this decision).
 // The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: it decides
 // while the model has too few samples, and it takes over for a cool-down period whenever
-// the model's recent predictions of the steps an upload buys are off by more than a
-// factor e on average (the judge).  Walks stay bit-identical either way (counter-based
+// the model's recent predictions of the steps a load buys are off by more than a factor
+// e on average (the judge).  Walks stay bit-identical either way (counter-based
 // RNG).
 #pragma once
 #include <algorithm>
@@ -39,12 +39,12 @@ struct LearnedConfig {
     uint64_t seed = 7;
     uint32_t min_samples = 24;      // plain swap decisions until this many upload samples
     uint32_t train_steps = 4;       // Adam steps after every iteration
-    uint32_t batch = 512;
+    uint32_t batch = 256;
     uint32_t max_samples = 16384;   // ring buffer of trace samples
     float lr = 2e-3f;
     double zc_bytes_per_step = 360; // PCIe bytes of one zero-copy step (5.7 measured 64 B reads)
     double tail_margin = 1.0;       // zero-copy once steps x zc bytes x margin < block bytes
-    double tail_guard = 256;        // ... and the work left x zc bytes < guard x block bytes
+    double tail_guard = 64;         // ... and the work left x zc bytes < guard x block bytes
     uint32_t max_inflight = 4;      // staged copies in flight at once (a deeper copy queue
                                     // makes the host block when it enqueues)
     double judge_error = 1.0;       // mean |log error| of recent predictions above this => fall back
@@ -132,27 +132,36 @@ public:
         const uint32_t nb = in.nblocks;
         std::vector<uint64_t> trace(nb, 0);
         if (gw_trace_read(ctx_, trace.data(), nb) != GW_OK) return;
-        // sample: (features of the one block that became resident, log1p(steps of the run))
+        if (open_.size() != nb) open_.assign(nb, Open{});
+        std::vector<char> now(nb, 0);
+        for (uint32_t b : buckets) now[b] = 1;
+        // a block that left the resident set closes its sample: the steps it served from
+        // its upload to its eviction
+        for (uint32_t b = 0; b < nb; ++b)
+            if (open_[b].live && !now[b]) close_sample(b);
+        // blocks that became resident open one: (features when the load was decided, the
+        // run's steps if it was the only new block, else the steps it served as the
+        // walkers' current block)
         std::vector<uint32_t> fresh;
         for (uint32_t b : buckets)
             if (!was_resident_[b]) fresh.push_back(b);
-        if (fresh.size() == 1 && feats_.size() >= (size_t)(fresh[0] + 1) * kF) {
-            const float y = std::log1p((float)out.steps);
-            add_sample(&feats_[(size_t)fresh[0] * kF], y);
-            // judge: how far the model's prediction for this upload was from what it bought
-            if (use_model_) {
-                const double e = std::fabs((double)pred_[fresh[0]] - (double)y);
-                err_ = err_n_ ? 0.8 * err_ + 0.2 * e : e;
-                ++err_n_;
-                if (last_kind_ == kLearnedSwap && err_n_ >= 4 && err_ > cfg_.judge_error) {
-                    cooldown_ = cfg_.cooldown;
-                    err_n_ = 0;
-                    ++fallbacks_;
-                }
-            }
+        for (uint32_t b : fresh) {
+            if (feats_.size() < (size_t)(b + 1) * kF) continue;
+            Open& o = open_[b];
+            o.live = true;
+            o.f.assign(feats_.begin() + (size_t)b * kF, feats_.begin() + (size_t)(b + 1) * kF);
+            o.steps = fresh.size() == 1 ? (double)out.steps : (double)trace[b];
+            o.pred = use_model_ ? pred_[b] : -1.f;
         }
-        for (uint32_t b = 0; b < nb; ++b) age_[b] = trace[b] ? 0.f : std::min(age_[b] + 1.f, 1024.f);
-        if (mlp_ && samples_ >= 8) train();
+        for (uint32_t b = 0; b < nb; ++b) {
+            if (now[b] && open_[b].live && std::find(fresh.begin(), fresh.end(), b) == fresh.end())
+                open_[b].steps += (double)trace[b];
+            age_[b] = trace[b] ? 0.f : std::min(age_[b] + 1.f, 1024.f);
+        }
+        if (mlp_ && samples_ >= 8 && samples_ != trained_at_) {
+            trained_at_ = samples_;
+            train();
+        }
     }
 
     uint64_t learned_decisions() const { return learned_decisions_; }
@@ -165,6 +174,32 @@ private:
     enum Kind { kPlain, kFill, kLearnedSwap };
 
     bool trained() const { return mlp_ && samples_ >= cfg_.min_samples; }
+
+    struct Open {
+        bool live = false;
+        std::vector<float> f;
+        double steps = 0;
+        float pred = -1.f;
+    };
+
+    void close_sample(uint32_t b)
+    {
+        Open& o = open_[b];
+        o.live = false;
+        const float y = std::log1p((float)o.steps);
+        add_sample(o.f.data(), y);
+        // judge: how far the model's prediction for this load was from what it bought
+        if (o.pred >= 0.f) {
+            const double e = std::fabs((double)o.pred - (double)y);
+            err_ = err_n_ ? 0.8 * err_ + 0.2 * e : e;
+            ++err_n_;
+            if (err_n_ >= 4 && err_ > cfg_.judge_error) {
+                cooldown_ = cfg_.cooldown;
+                err_n_ = 0;
+                ++fallbacks_;
+            }
+        }
+    }
 
     // features, walker demand and the ranking score of every block
     void prepare(const SchedInput& in)
@@ -273,6 +308,8 @@ private:
     std::vector<float> age_, feats_, pred_, rank_, xs_, ys_;
     std::vector<uint64_t> add_, demand_;
     std::vector<char> was_resident_;
+    std::vector<Open> open_;
+    uint64_t trained_at_ = 0;
     uint64_t pending_hops_ = 0;
     bool have_feats_ = false, use_model_ = false;
     uint64_t samples_ = 0, ring_ = 0, rng_ = 0x853c49e6748fea9bULL;
