@@ -100,12 +100,16 @@ def cpu_baseline(beg, csr, cfg, seconds, threads):
 PCIE_PEAK_GBS = 64.0   # PCIe Gen5 x16, one direction
 
 
-def ooc_schedulers(E, block_bytes):
+def ooc_schedulers(E, block_bytes, ablation=False):
     # plain: the reference's max_choose_blocks without the optional zero-copy tail;
     # plain_zero: the same with the reference's `zero` option (threshold blocksize/128/24
-    # walkers, apps/Node2vec.cu:100-104); learned: the MLP scheduler
-    return (("plain", E.SCHED_PLAIN, 0), ("plain_zero", E.SCHED_PLAIN, block_bytes // 128 // 24),
-            ("learned", E.SCHED_LEARNED, 0))
+    # walkers, apps/Node2vec.cu:100-104); learned: the MLP scheduler (with --ablation also
+    # without its staging slot once the cache is full)
+    s = [("plain", E.SCHED_PLAIN, 0, 0), ("plain_zero", E.SCHED_PLAIN, block_bytes // 128 // 24, 0),
+         ("learned", E.SCHED_LEARNED, 0, 0)]
+    if ablation:
+        s.append(("learned_no_swap_prefetch", E.SCHED_LEARNED, 0, 1))
+    return s
 
 
 def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
@@ -122,13 +126,13 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
     for cap_gb in caps:
         mem = int(cap_gb * (1 << 30))
         res = {}
-        for name, sched, zt in ooc_schedulers(E, block_bytes):
+        for name, sched, zt, flags in ooc_schedulers(E, block_bytes, args.ablation):
             reps = []
             for i in range(args.warmup + args.steps):
                 first = (cap_gb, name) not in hashes
                 rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
                                               memory_bytes=mem, block_bytes=block_bytes, zero_threshold=zt,
-                                              scheduler=sched, hashes=first)
+                                              scheduler=sched, hashes=first, flags=flags)
                 if first:
                     hashes[(cap_gb, name)] = h
                 print("[bench] cap %g GB %s run %d: %.3f s, %d steps, %.2f GB H2D (%d uploads + %d staged), "
@@ -153,7 +157,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
             if name == "learned" and learned_reps is None:
                 learned_reps = reps
         ref = hashes[(cap_gb, "plain")]
-        ident = all(bool((hashes[(cap_gb, n)] == ref).all()) for n, _, _ in ooc_schedulers(E, block_bytes))
+        ident = all(bool((hashes[(cap_gb, n)] == ref).all()) for n, _, _, _ in ooc_schedulers(E, block_bytes, args.ablation))
         L, P = res["learned"], res["plain"]
         per_cap.append({"memory_gb": cap_gb, "walks_identical": ident, "schedulers": res,
                         "learned_vs_plain_steps_per_s": L["steps_per_s"] / P["steps_per_s"],
@@ -202,6 +206,7 @@ def main():
     ap.add_argument("--variant", type=int, default=0, help="0: per-step lane loop, 1: flattened state machine")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
+    ap.add_argument("--ablation", action="store_true", help="out-of-core: also run scheduler variants")
     ap.add_argument("--autotune", type=int, default=0,
                     help="time every launch shape on this many walkers (untimed) and run with the fastest")
     args = ap.parse_args()

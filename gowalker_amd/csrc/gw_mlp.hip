@@ -161,15 +161,16 @@ __global__ void __launch_bounds__(256) mlp_adam_kernel(uint32_t np, float* __res
 }
 
 // ------------------------------------------------------------------------------------
-// Fused single-workgroup kernel for MLPs whose weights and a chunk of activations fit in
-// 64 KB of LDS (the 12-64-64-1 block scorer uses 54 KB): the batch streams through LDS in
-// chunks of R samples -- forward, per-sample deltas and the per-parameter gradient sums
-// in one launch, each thread owning up to kFusedOwn parameters whose gradients stay in
-// registers -- and, in Adam mode, the update is applied at the end.  Every sum runs in a
-// fixed order, so results are deterministic.
-// MODE 0: forward (y); 1: gradient (g, loss); 2: gradient + Adam step.
+// Fused kernels for MLPs whose weights and a chunk of activations fit in 64 KB of LDS
+// (the 12-64-64-1 block scorer uses 54 KB).  Workgroup b takes the chunks b, b + G, ...
+// of R samples: forward, per-sample deltas and the per-parameter gradient sums of its
+// chunks in LDS, each thread owning up to kFusedOwn parameters whose partial gradients
+// stay in registers and are written to row b of a partial-gradient table.  A second,
+// single-workgroup kernel sums the G rows in order and applies Adam.  Every sum runs in
+// a fixed order, so results are deterministic.
+// MODE 0: forward (y); 1: partial gradients and squared errors.
 // ------------------------------------------------------------------------------------
-constexpr uint32_t kFusedThreads = 1024, kFusedOwn = 16, kFusedLdsFloats = 16384;
+constexpr uint32_t kFusedThreads = 1024, kFusedOwn = 16, kFusedLdsFloats = 16384, kFusedMaxGroups = 32;
 
 struct AdamArgs {
     float* master;
@@ -188,10 +189,10 @@ __device__ __forceinline__ float wave_sum_f(float v)
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R, const uint16_t* wbf,
+__global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R, const uint16_t* __restrict__ wbf,
                                                           const float* __restrict__ x, const float* __restrict__ t,
-                                                          uint32_t n, float* __restrict__ y, float* __restrict__ g,
-                                                          float* __restrict__ loss, AdamArgs adam)
+                                                          uint32_t n, float* __restrict__ y, float* __restrict__ gpart,
+                                                          float* __restrict__ lpart)
 {
     // LDS: weights, then the chunk's activations with odd row strides (in+1, h+1) so that
     // lanes walking consecutive rows hit distinct banks; weight reads are broadcasts
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R,
     for (uint32_t i = 0; i < kFusedOwn; ++i) acc[i] = 0.f;
     float lsum = 0.f;
     __syncthreads();
-    for (uint32_t c0 = 0; c0 < n; c0 += R) {
+    for (uint32_t c0 = blockIdx.x * R; c0 < n; c0 += gridDim.x * R) {
         const uint32_t rows = n - c0 < R ? n - c0 : R;
         for (uint32_t i = tid; i < rows * s.in; i += nt) sX[(i / s.in) * xs + i % s.in] = x[(uint64_t)c0 * s.in + i];
         __syncthreads();
@@ -292,21 +293,39 @@ __global__ void __launch_bounds__(1024) mlp_fused_kernel(MlpShape s, uint32_t R,
         __syncthreads();
     }
     if (MODE == 0) return;
-    // batch loss: lane 0 of every wavefront holds its rows' squared errors
+    // squared errors: lane 0 of every wavefront holds its rows'
     if (lane == 0) red[wave] = lsum;
     __syncthreads();
     if (tid == 0) {
         float tot = 0.f;
         for (uint32_t w = 0; w < nt / 64; ++w) tot += red[w];
-        loss[0] = tot / (float)n;
+        lpart[blockIdx.x] = tot;
     }
 #pragma unroll
     for (uint32_t i = 0; i < kFusedOwn; ++i) {
         const uint32_t p = tid + i * nt;
         if (p >= s.nparams) break;
-        g[p] = acc[i];
-        if (MODE == 2) {
-            const float gi = acc[i];
+        gpart[(uint64_t)blockIdx.x * s.nparams + p] = acc[i];
+    }
+}
+
+// Sums the G partial gradients in order into g and the loss; with Adam, updates the fp32
+// master weights and re-rounds the bf16 working copy.
+template <bool ADAM>
+__global__ void __launch_bounds__(1024) mlp_finish_kernel(uint32_t np, uint32_t G, const float* __restrict__ gpart,
+                                                           const float* __restrict__ lpart, uint32_t n,
+                                                           float* __restrict__ g, float* __restrict__ loss, AdamArgs adam)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        float tot = 0.f;
+        for (uint32_t b = 0; b < G; ++b) tot += lpart[b];
+        loss[0] = tot / (float)n;
+    }
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        float gi = 0.f;
+        for (uint32_t b = 0; b < G; ++b) gi += gpart[(uint64_t)b * np + p];
+        g[p] = gi;
+        if (ADAM) {
             const float a = adam.b1 * adam.m1[p] + (1.f - adam.b1) * gi;
             const float b = adam.b2 * adam.m2[p] + (1.f - adam.b2) * gi * gi;
             adam.m1[p] = a;
@@ -373,6 +392,7 @@ struct gw_mlp {
     float *m1 = nullptr, *m2 = nullptr, *g = nullptr;
     float *x = nullptr, *y = nullptr, *t = nullptr, *z1 = nullptr, *z2 = nullptr;
     float *dy = nullptr, *dz1 = nullptr, *dz2 = nullptr, *loss = nullptr;
+    float* gpart = nullptr;       // kFusedMaxGroups x nparams partial gradients (+ losses)
     uint64_t steps = 0;
 };
 
@@ -436,7 +456,8 @@ extern "C" gw_status gw_mlp_create(gw_ctx* ctx, uint32_t in, uint32_t h1, uint32
     }
     bool ok = hipMalloc(&m->wbf, np * 2) == hipSuccess && hipMalloc(&m->master, np * 4) == hipSuccess &&
               hipMalloc(&m->m1, np * 4) == hipSuccess && hipMalloc(&m->m2, np * 4) == hipSuccess &&
-              hipMalloc(&m->g, np * 4) == hipSuccess && hipMalloc(&m->loss, 16) == hipSuccess;
+              hipMalloc(&m->g, np * 4) == hipSuccess && hipMalloc(&m->loss, 16) == hipSuccess &&
+              hipMalloc(&m->gpart, ((size_t)kFusedMaxGroups * np + kFusedMaxGroups) * 4) == hipSuccess;
     ok = ok && hipMemcpy(m->wbf, wb.data(), np * 2, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(m->master, w.data(), np * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemset(m->m1, 0, np * 4) == hipSuccess && hipMemset(m->m2, 0, np * 4) == hipSuccess;
@@ -454,7 +475,7 @@ extern "C" gw_status gw_mlp_destroy(gw_ctx* ctx, gw_mlp* m)
     (void)ctx;
     if (!m) return GW_OK;
     mlp_free_scratch(m);
-    void* ps[] = {m->wbf, m->master, m->m1, m->m2, m->g, m->loss};
+    void* ps[] = {m->wbf, m->master, m->m1, m->m2, m->g, m->loss, m->gpart};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     delete m;
@@ -539,15 +560,23 @@ static gw_status mlp_fused(gw_ctx* ctx, gw_mlp* m, uint32_t R, int mode, const f
         a.c1 = 1.f - std::pow(a.b1, (float)m->steps);
         a.c2 = 1.f - std::pow(a.b2, (float)m->steps);
     }
-    if (mode == 0)
-        hipLaunchKernelGGL(mlp_fused_kernel<0>, dim3(1), dim3(kFusedThreads), lds, s, sh, R, m->wbf, m->x, m->t, n,
-                           m->y, m->g, m->loss, a);
-    else if (mode == 1)
-        hipLaunchKernelGGL(mlp_fused_kernel<1>, dim3(1), dim3(kFusedThreads), lds, s, sh, R, m->wbf, m->x, m->t, n,
-                           m->y, m->g, m->loss, a);
-    else
-        hipLaunchKernelGGL(mlp_fused_kernel<2>, dim3(1), dim3(kFusedThreads), lds, s, sh, R, m->wbf, m->x, m->t, n,
-                           m->y, m->g, m->loss, a);
+    uint32_t G = (n + R - 1) / R;
+    if (G > kFusedMaxGroups) G = kFusedMaxGroups;
+    float* lpart = m->gpart + (size_t)kFusedMaxGroups * sh.nparams;
+    if (mode == 0) {
+        hipLaunchKernelGGL(mlp_fused_kernel<0>, dim3(G), dim3(kFusedThreads), lds, s, sh, R, m->wbf, m->x, m->t, n,
+                           m->y, m->gpart, lpart);
+    } else {
+        hipLaunchKernelGGL(mlp_fused_kernel<1>, dim3(G), dim3(kFusedThreads), lds, s, sh, R, m->wbf, m->x, m->t, n,
+                           m->y, m->gpart, lpart);
+        const uint32_t fg = (sh.nparams + kFusedThreads - 1) / kFusedThreads;
+        if (mode == 1)
+            hipLaunchKernelGGL(mlp_finish_kernel<false>, dim3(fg), dim3(kFusedThreads), 0, s, sh.nparams, G, m->gpart,
+                               lpart, n, m->g, m->loss, a);
+        else
+            hipLaunchKernelGGL(mlp_finish_kernel<true>, dim3(fg), dim3(kFusedThreads), 0, s, sh.nparams, G, m->gpart,
+                               lpart, n, m->g, m->loss, a);
+    }
     if (hipGetLastError() != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP kernel launch failed");
     return GW_OK;
 }

@@ -295,7 +295,8 @@ SCHED_PLAIN, SCHED_LEARNED = 0, 1
 
 
 def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory_bytes=0, block_bytes=256 << 20,
-                    zero_threshold=0, scheduler=SCHED_PLAIN, hashes=False, visits=False, verbose=False, launch=None):
+                    zero_threshold=0, scheduler=SCHED_PLAIN, hashes=False, visits=False, verbose=False, launch=None,
+                    flags=0):
     """Runs the C++ out-of-core engine (libgwhost.so) over a host CSR graph; beg/csr should
     be pinned (PinnedArray.a) for the zero-copy tail.  Returns (report dict, hashes, visits)."""
     from ._abi import GwhEngineOpts, GwhReport, host_lib
@@ -306,6 +307,7 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
     o.zero_threshold = zero_threshold
     o.scheduler = scheduler
     o.verbose = 1 if verbose else 0
+    o.flags = flags
     if launch:
         o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 0), 0)
     wp = walk_params(types, seed)

@@ -153,13 +153,15 @@ public:
                     out.loaded.push_back(b);
                     r.uploads++;
                 }
-                // prefetch into slots that hold neither a wanted nor a staged block
+                // prefetch: staged blocks the scheduler still wants stay, new ones go into
+                // slots that hold neither a wanted block nor one of those
                 pf.clear();
+                sched.prefetch(in, nslots_ > sched.buckets.size() ? nslots_ - (uint32_t)sched.buckets.size() : 0u, pf);
                 std::vector<uint32_t> keep = sched.buckets;
-                for (uint32_t b = 0; b < nb; ++b)
+                for (uint32_t b : pf)
                     if (slots_.blk_stg[b] >= 0) keep.push_back(b);
-                sched.prefetch(in, slots_.free_slots(keep), pf);
                 for (uint32_t b : pf) {
+                    if (slots_.blk_stg[b] >= 0 || slots_.blk_slot[b] >= 0) continue;
                     const int s = slots_.stage(b, keep);
                     if (s < 0) continue;
                     keep.push_back(b);
@@ -188,6 +190,8 @@ public:
                 r.zc_touches += st.zc_touches;
             }
             out.steps = st.steps;
+            out.trials = st.trials;
+            out.probes = st.probes;
             out.kernel_ms = st.kernel_ms;
             auto tf = clk::now();
             sched.feedback(in, out);

@@ -47,7 +47,9 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         std::unique_ptr<Scheduler> sched;
         LearnedScheduler* ls = nullptr;
         if (opts->scheduler == GWH_SCHED_LEARNED) {
-            ls = new LearnedScheduler(engine.ctx(), LearnedConfig());
+            LearnedConfig lc;
+            lc.swap_prefetch = !(opts->flags & GWH_NO_SWAP_PREFETCH);
+            ls = new LearnedScheduler(engine.ctx(), lc);
             sched.reset(ls);
         } else {
             sched.reset(new PlainScheduler());

@@ -14,8 +14,9 @@
 //          walkers could run now; evict the resident block the fewest waiting walkers
 //          need;
 //   tail   when y(c) says the upload would buy fewer steps than its bytes buy in
-//          zero-copy reads (steps x zc_bytes_per_step < block bytes) and all the work
-//          left is worth at most a few uploads in zero-copy bytes, finish the walk by
+//          zero-copy reads (steps x reads per step x 64 B < block bytes, reads per step
+//          measured on the walk so far) and all the work left is worth at most
+//          tail_guard uploads in zero-copy bytes, finish the walk by
 //          zero-copy instead (the reference's fixed walker-count threshold,
 //          apps/Node2vec.cu:100-104, is the plain rule's version of this decision).
 // The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: it decides
@@ -42,11 +43,12 @@ struct LearnedConfig {
     uint32_t batch = 256;
     uint32_t max_samples = 16384;   // ring buffer of trace samples
     float lr = 2e-3f;
-    double zc_bytes_per_step = 360; // PCIe bytes of one zero-copy step (5.7 measured 64 B reads)
+    double zc_read_bytes = 64;      // PCIe bytes of one zero-copy read (row, candidate or probe)
     double tail_margin = 1.0;       // zero-copy once steps x zc bytes x margin < block bytes
     double tail_guard = 64;         // ... and the work left x zc bytes < guard x block bytes
     uint32_t max_inflight = 4;      // staged copies in flight at once (a deeper copy queue
                                     // makes the host block when it enqueues)
+    bool swap_prefetch = true;      // full cache: keep one slot for staging the next load
     double judge_error = 1.0;       // mean |log error| of recent predictions above this => fall back
     uint32_t cooldown = 8;
 };
@@ -72,10 +74,11 @@ public:
         last_kind_ = kPlain;
         std::vector<char> cached(nb, 0);
         for (uint32_t b : buckets) cached[b] = 1;
+        const uint32_t cap = capacity(in);
 
-        if (buckets.size() < in.ncblock) {   // ---- fill: free slots remain
+        if (buckets.size() < cap) {   // ---- fill: free slots remain
             bool took = false;
-            for (uint32_t b = 0; b < nb && buckets.size() < in.ncblock; ++b)
+            for (uint32_t b = 0; b < nb && buckets.size() < cap; ++b)
                 if (!cached[b] && in.staged && in.staged[b] == 2) { buckets.push_back(b); cached[b] = 1; took = true; }
             if (!took && in.staged) {
                 int best = -1;
@@ -109,22 +112,37 @@ public:
         return true;
     }
 
-    void prefetch(const SchedInput& in, uint32_t free_slots, std::vector<uint32_t>& out) override
+    void prefetch(const SchedInput& in, uint32_t spare, std::vector<uint32_t>& out) override
     {
-        if (!free_slots || buckets.size() >= in.ncblock) return;
         const uint32_t nb = in.nblocks;
+        const uint32_t cap = capacity(in);
+        if (!spare) return;
         std::vector<char> skip(nb, 0);
         for (uint32_t b : buckets) skip[b] = 1;
-        uint32_t inflight = 0;
-        for (uint32_t b = 0; b < nb; ++b)
-            if (in.staged && in.staged[b] == 1 && !skip[b]) ++inflight;
-        if (inflight >= cfg_.max_inflight) return;
         std::vector<uint32_t> cand;
         for (uint32_t b = 0; b < nb; ++b)
-            if (!skip[b] && !(in.staged && in.staged[b]) && demand_[b] > 0) cand.push_back(b);
+            if (!skip[b] && demand_[b] > 0) cand.push_back(b);
         std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return rank_[a] > rank_[b]; });
-        const uint32_t room = std::min<uint32_t>({free_slots, cfg_.max_inflight - inflight, (uint32_t)(in.ncblock - buckets.size())});
-        for (uint32_t i = 0; i < cand.size() && i < room; ++i) out.push_back(cand[i]);
+        if (buckets.size() < cap) {
+            // fill: keep the copies in flight, top up to max_inflight with the best blocks
+            uint32_t n = 0;
+            for (uint32_t b : cand)
+                if (in.staged && in.staged[b] == 1) { out.push_back(b); ++n; }
+            for (uint32_t b : cand) {
+                if (n >= cfg_.max_inflight || out.size() >= std::min<uint32_t>(spare, cap - (uint32_t)buckets.size())) break;
+                if (in.staged && in.staged[b]) continue;
+                out.push_back(b);
+                ++n;
+            }
+            return;
+        }
+        // swap: stage the best remaining block into the spare slot, unless the walk is
+        // near its zero-copy tail (a wasted copy would cost more than it saves)
+        if (!cfg_.swap_prefetch || !use_model_ || cooldown_ > 0 || cand.empty()) return;
+        const uint32_t p = cand[0];
+        const gw_block_desc& d = in.blocks[p];
+        const double bytes = ((double)d.nverts + 1) * 8 + (double)d.nedges * 4;
+        if (std::expm1((double)std::max(0.f, pred_[p])) * zc_bytes() > 2.0 * bytes) out.push_back(p);
     }
 
     void feedback(const SchedInput& in, const IterationOutcome& out) override
@@ -153,6 +171,13 @@ public:
             o.steps = fresh.size() == 1 ? (double)out.steps : (double)trace[b];
             o.pred = use_model_ ? pred_[b] : -1.f;
         }
+        // reads per step (row + candidates + probes) of the walks so far: a zero-copy step
+        // costs about that many PCIe reads
+        if (out.steps) {
+            const double r = 1.0 + (double)(out.trials + out.probes) / (double)out.steps;
+            reads_per_step_ = reads_n_ ? 0.9 * reads_per_step_ + 0.1 * r : r;
+            ++reads_n_;
+        }
         for (uint32_t b = 0; b < nb; ++b) {
             if (now[b] && open_[b].live && std::find(fresh.begin(), fresh.end(), b) == fresh.end())
                 open_[b].steps += (double)trace[b];
@@ -174,6 +199,15 @@ private:
     enum Kind { kPlain, kFill, kLearnedSwap };
 
     bool trained() const { return mlp_ && samples_ >= cfg_.min_samples; }
+
+    // resident blocks allowed: every slot, or all but the staging slot when the graph does
+    // not fit and the next load is prefetched
+    uint32_t capacity(const SchedInput& in) const
+    {
+        return cfg_.swap_prefetch && in.nblocks > in.ncblock && in.ncblock > 1 ? in.ncblock - 1 : in.ncblock;
+    }
+
+    double zc_bytes() const { return cfg_.zc_read_bytes * (reads_n_ ? reads_per_step_ : 6.0); }
 
     struct Open {
         bool live = false;
@@ -252,8 +286,8 @@ private:
         const gw_block_desc& d = in.blocks[c];
         const double bytes = ((double)d.nverts + 1) * 8 + (double)d.nedges * 4;
         const double steps = std::expm1((double)std::max(0.f, pred_[c]));
-        if (steps * cfg_.zc_bytes_per_step * cfg_.tail_margin < bytes &&
-            (double)pending_hops_ * cfg_.zc_bytes_per_step < cfg_.tail_guard * bytes) {
+        const double zc = zc_bytes();
+        if (steps * zc * cfg_.tail_margin < bytes && (double)pending_hops_ * zc < cfg_.tail_guard * bytes) {
             ++tail_decisions_;
             return true;
         }
@@ -263,7 +297,9 @@ private:
     bool plain_decide(const SchedInput& in)
     {
         plain_.buckets = buckets;
-        bool r = plain_.schedule(in);
+        SchedInput pin = in;
+        pin.ncblock = capacity(in);
+        bool r = plain_.schedule(pin);
         buckets = plain_.buckets;
         last_kind_ = kPlain;
         return r;
@@ -309,6 +345,8 @@ private:
     std::vector<uint64_t> add_, demand_;
     std::vector<char> was_resident_;
     std::vector<Open> open_;
+    double reads_per_step_ = 0;
+    uint64_t reads_n_ = 0;
     uint64_t trained_at_ = 0;
     uint64_t pending_hops_ = 0;
     bool have_feats_ = false, use_model_ = false;

@@ -28,6 +28,7 @@ struct SchedInput {
 struct IterationOutcome {
     uint64_t iteration;
     uint64_t steps;             // walk steps taken in the iteration
+    uint64_t trials, probes;    // candidate draws and adjacency probes of those steps
     uint64_t h2d_bytes;         // block bytes uploaded for it
     double kernel_ms;
     std::vector<uint32_t> loaded;   // blocks uploaded for it
@@ -39,11 +40,13 @@ public:
     // Sets `buckets`, the blocks resident for the next run; false = nothing is worth
     // loading (the engine then finishes the walk by zero-copy reads).
     virtual bool schedule(const SchedInput& in) = 0;
-    // Blocks to stage into free slots while the run goes on (copy overlapped with the walk;
-    // a later schedule() commits them by putting them in `buckets`).
-    virtual void prefetch(const SchedInput& in, uint32_t free_slots, std::vector<uint32_t>& out)
+    // The blocks that should be staged while the next run goes on (copies overlapped with
+    // the walk; a later schedule() commits them by putting them in `buckets`): staged
+    // blocks it leaves out may be overwritten, new ones are copied into slots that hold no
+    // wanted block.  At most `spare` = slots - |buckets| entries.
+    virtual void prefetch(const SchedInput& in, uint32_t spare, std::vector<uint32_t>& out)
     {
-        (void)in; (void)free_slots; (void)out;
+        (void)in; (void)spare; (void)out;
     }
     virtual void feedback(const SchedInput& in, const IterationOutcome& out) { (void)in; (void)out; }
     virtual const char* name() const = 0;

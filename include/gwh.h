@@ -13,6 +13,8 @@ extern "C" {
 #endif
 
 enum { GWH_SCHED_PLAIN = 0, GWH_SCHED_LEARNED = 1 };
+/* gwh_engine_opts.flags */
+enum { GWH_NO_SWAP_PREFETCH = 1u << 0 /* learned: no staging slot once the cache is full */ };
 
 typedef struct {
     uint64_t memory_bytes;     /* HBM budget (slots + walker state); 0 = every block resident */
@@ -22,7 +24,7 @@ typedef struct {
     uint32_t verbose;
     gw_launch launch;
     int32_t device;
-    uint32_t reserved;
+    uint32_t flags;            /* GWH_* flags */
 } gwh_engine_opts;
 
 typedef struct {
