@@ -104,11 +104,11 @@ def ooc_schedulers(E, block_bytes, ablation=False):
     # plain: the reference's max_choose_blocks without the optional zero-copy tail;
     # plain_zero: the same with the reference's `zero` option (threshold blocksize/128/24
     # walkers, apps/Node2vec.cu:100-104); learned: the MLP scheduler (with --ablation also
-    # without its staging slot once the cache is full)
+    # with a staging slot once the cache is full, and with next-run labels)
     s = [("plain", E.SCHED_PLAIN, 0, 0), ("plain_zero", E.SCHED_PLAIN, block_bytes // 128 // 24, 0),
          ("learned", E.SCHED_LEARNED, 0, 0)]
     if ablation:
-        s.append(("learned_no_swap_prefetch", E.SCHED_LEARNED, 0, 1))
+        s += [("learned_swap_prefetch", E.SCHED_LEARNED, 0, 1), ("learned_myopic", E.SCHED_LEARNED, 0, 2)]
     return s
 
 

@@ -48,7 +48,8 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         LearnedScheduler* ls = nullptr;
         if (opts->scheduler == GWH_SCHED_LEARNED) {
             LearnedConfig lc;
-            lc.swap_prefetch = !(opts->flags & GWH_NO_SWAP_PREFETCH);
+            lc.swap_prefetch = (opts->flags & GWH_SWAP_PREFETCH) != 0;
+            lc.myopic_label = (opts->flags & GWH_MYOPIC_LABEL) != 0;
             ls = new LearnedScheduler(engine.ctx(), lc);
             sched.reset(ls);
         } else {

@@ -48,7 +48,9 @@ struct LearnedConfig {
     double tail_guard = 64;         // ... and the work left x zc bytes < guard x block bytes
     uint32_t max_inflight = 4;      // staged copies in flight at once (a deeper copy queue
                                     // makes the host block when it enqueues)
-    bool swap_prefetch = true;      // full cache: keep one slot for staging the next load
+    bool swap_prefetch = false;     // full cache: keep one slot for staging the next load
+                                    // (overlaps every upload, costs a slot and missed copies)
+    bool myopic_label = false;      // label a load with the next run's steps only
     double judge_error = 1.0;       // mean |log error| of recent predictions above this => fall back
     uint32_t cooldown = 8;
 };
@@ -179,7 +181,8 @@ public:
             ++reads_n_;
         }
         for (uint32_t b = 0; b < nb; ++b) {
-            if (now[b] && open_[b].live && std::find(fresh.begin(), fresh.end(), b) == fresh.end())
+            if (now[b] && open_[b].live && !cfg_.myopic_label &&
+                std::find(fresh.begin(), fresh.end(), b) == fresh.end())
                 open_[b].steps += (double)trace[b];
             age_[b] = trace[b] ? 0.f : std::min(age_[b] + 1.f, 1024.f);
         }

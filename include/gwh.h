@@ -14,7 +14,10 @@ extern "C" {
 
 enum { GWH_SCHED_PLAIN = 0, GWH_SCHED_LEARNED = 1 };
 /* gwh_engine_opts.flags */
-enum { GWH_NO_SWAP_PREFETCH = 1u << 0 /* learned: no staging slot once the cache is full */ };
+enum {
+    GWH_SWAP_PREFETCH = 1u << 0, /* learned: keep a staging slot once the cache is full */
+    GWH_MYOPIC_LABEL = 1u << 1   /* learned: a load's label is the next run's steps only */
+};
 
 typedef struct {
     uint64_t memory_bytes;     /* HBM budget (slots + walker state); 0 = every block resident */
