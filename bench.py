@@ -1,11 +1,21 @@
 #!/usr/bin/env python3
 """Walk-throughput benchmark of the MI355X walk engine (one GPU).
 
-Default workload (SOPR second-order PageRank walks, fully HBM-resident):
-  directed Chung-Lu power-law graph, 41.6M vertices, 1.47B arcs, out-degree exponent 2.2,
-  max out-degree 3M (generated on the GPU, seed 42); 4 walkers per vertex (166.4M walkers),
-  alpha 0.2, termination probability 0.15 per step, length cap 30, per-vertex visit counts.
-One timed step = initialise every walker and run all of them to completion.
+Default workload (Node2vec out of core, learned vs plain block scheduler):
+  undirected RMAT graph, 2^26 ids with 1.8B generated edges (a,b,c,d = .57,.19,.19,.05)
+  symmetrized, deduplicated, ids randomly permuted (3.5B arcs, generated on the GPU, seed
+  42), cut into 256 MB blocks; HBM block cache capped at 16 GB (the stated size, where
+  the whole CSR fits) and at 8 GB (where blocks must be evicted); p 0.5, q 2, one walker
+  per vertex, length 40.  One timed step = one full walk of every walker through the
+  out-of-core engine (graph already in pinned host memory), for each scheduler.
+  value = steps/s of the learned scheduler at the first cap; the plain scheduler (the
+  reference's max_choose_blocks), the same with the reference's zero-copy tail, H2D and
+  zero-copy bytes are in config.caps.
+
+Other workloads (--config): sopr_chung_lu (SOPR on a directed Chung-Lu graph, 41.6M V /
+1.47B arcs, 4 walkers/vertex, term 0.15, fully HBM-resident), node2vec_rmat20 (the
+reference's CPU-runnable case, single resident block) and node2vec_rmat24_ooc (reduced
+out-of-core case).
 
 Prints one JSON line: walk steps/s plus the roofline of the step kernel (algorithmic bytes
 per launch over its HIP-event time vs the 8 TB/s HBM peak) and a CPU baseline (the
@@ -100,7 +110,7 @@ def cpu_baseline(beg, csr, cfg, seconds, threads):
 PCIE_PEAK_GBS = 64.0   # PCIe Gen5 x16, one direction
 
 
-def ooc_schedulers(E, block_bytes, ablation=False):
+def ooc_schedulers(E, block_bytes, ablation=False, only=None):
     # plain: the reference's max_choose_blocks without the optional zero-copy tail;
     # plain_zero: the same with the reference's `zero` option (threshold blocksize/128/24
     # walkers, apps/Node2vec.cu:100-104); learned: the MLP scheduler (with --ablation also
@@ -109,6 +119,8 @@ def ooc_schedulers(E, block_bytes, ablation=False):
          ("learned", E.SCHED_LEARNED, 0, 0)]
     if ablation:
         s += [("learned_swap_prefetch", E.SCHED_LEARNED, 0, 1), ("learned_myopic", E.SCHED_LEARNED, 0, 2)]
+    if only:
+        s = [x for x in s if x[0] in only]
     return s
 
 
@@ -126,7 +138,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
     for cap_gb in caps:
         mem = int(cap_gb * (1 << 30))
         res = {}
-        for name, sched, zt, flags in ooc_schedulers(E, block_bytes, args.ablation):
+        for name, sched, zt, flags in ooc_schedulers(E, block_bytes, args.ablation, args.schedulers):
             reps = []
             for i in range(args.warmup + args.steps):
                 first = (cap_gb, name) not in hashes
@@ -154,20 +166,29 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                              copy_ms=r0["h2d_ms"], sched_ms=r0["sched_ms"], learned_decisions=r0["learned_decisions"],
                              tail_decisions=r0["tail_decisions"], fallbacks=r0["fallbacks"], nslots=r0["nslots"],
                              nblocks=r0["nblocks"], steps=r0["steps"], zero_threshold=zt)
-            if name == "learned" and learned_reps is None:
+            if learned_reps is None and (name == "learned" or (args.schedulers and name == args.schedulers[0])):
                 learned_reps = reps
-        ref = hashes[(cap_gb, "plain")]
-        ident = all(bool((hashes[(cap_gb, n)] == ref).all()) for n, _, _, _ in ooc_schedulers(E, block_bytes, args.ablation))
-        L, P = res["learned"], res["plain"]
-        per_cap.append({"memory_gb": cap_gb, "walks_identical": ident, "schedulers": res,
-                        "learned_vs_plain_steps_per_s": L["steps_per_s"] / P["steps_per_s"],
-                        "learned_vs_plain_pcie_bytes": L["pcie_gb_total"] / P["pcie_gb_total"],
-                        "learned_vs_plain_zero_steps_per_s": L["steps_per_s"] / res["plain_zero"]["steps_per_s"],
-                        "learned_vs_plain_zero_pcie_bytes": L["pcie_gb_total"] / res["plain_zero"]["pcie_gb_total"]})
-    head = per_cap[0]["schedulers"]["learned"]
+        names = [n for n, _, _, _ in ooc_schedulers(E, block_bytes, args.ablation, args.schedulers)]
+        ref = hashes[(cap_gb, names[0])]
+        ident = all(bool((hashes[(cap_gb, n)] == ref).all()) for n in names)
+        entry = {"memory_gb": cap_gb, "walks_identical": ident, "schedulers": res}
+        for other in ("plain", "plain_zero"):
+            if "learned" in res and other in res:
+                entry["learned_vs_%s_steps_per_s" % other] = res["learned"]["steps_per_s"] / res[other]["steps_per_s"]
+                entry["learned_vs_%s_pcie_bytes" % other] = res["learned"]["pcie_gb_total"] / res[other]["pcie_gb_total"]
+        per_cap.append(entry)
+    lead = "learned" if "learned" in per_cap[0]["schedulers"] else names[0]
+    head = per_cap[0]["schedulers"][lead]
     r0 = learned_reps[0]
-    abytes = algorithmic_bytes(r0, nwalkers, False)
-    achieved = abytes / (r0["kernel_ms"] * 1e-3) / 1e9
+    launches = max(1, r0["iterations"])
+    abytes = algorithmic_bytes(r0, nwalkers, False) / launches      # per launch, averaged
+    achieved = abytes / (r0["kernel_ms"] / launches * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        try:
+            traffic = json.load(open(args.traffic_json)).get(args.config)
+        except Exception:
+            traffic = None
     line = {
         "metric": "walk steps/s on 1 MI355X + H2D GB, learned vs plain block scheduler",
         "value": head["steps_per_s"],
@@ -180,10 +201,12 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                    "caps": per_cap, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS,
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "walk_kernel<false>",
-                     "kernel_ms": r0["kernel_ms"], "algorithmic_bytes": abytes,
-                     "sector_bytes": sector_bytes(r0, nwalkers, False),
-                     "note": "all step-kernel launches of one learned-scheduler walk at the first cap, summed"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "walk_kernel<false>",
+                     "kernel_ms": r0["kernel_ms"] / launches, "algorithmic_bytes": abytes,
+                     "sector_bytes": sector_bytes(r0, nwalkers, False) / launches,
+                     "launches": launches,
+                     "note": "per launch, averaged over the step-kernel launches of one %s walk at %g GB" %
+                             (lead, per_cap[0]["memory_gb"])},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
@@ -197,7 +220,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="sopr_chung_lu", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="node2vec_rmat26_ooc", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -207,6 +230,8 @@ def main():
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
     ap.add_argument("--ablation", action="store_true", help="out-of-core: also run scheduler variants")
+    ap.add_argument("--schedulers", type=lambda s: s.split(","), default=None,
+                    help="out-of-core: comma list of schedulers to run (default plain,plain_zero,learned)")
     ap.add_argument("--autotune", type=int, default=0,
                     help="time every launch shape on this many walkers (untimed) and run with the fastest")
     args = ap.parse_args()

@@ -16,9 +16,9 @@
 //   tail   when y(c) says the upload would buy fewer steps than its bytes buy in
 //          zero-copy reads (steps x reads per step x 64 B < block bytes, reads per step
 //          measured on the walk so far) and all the work left is worth at most
-//          tail_guard uploads in zero-copy bytes, finish the walk by
-//          zero-copy instead (the reference's fixed walker-count threshold,
-//          apps/Node2vec.cu:100-104, is the plain rule's version of this decision).
+//          tail_guard uploads in zero-copy bytes, finish the walk by zero-copy instead;
+//          the reference's fixed walker-count threshold (apps/Node2vec.cu:100-104, the
+//          plain rule's version of this decision) is the latest point it switches at.
 // The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: it decides
 // while the model has too few samples, and it takes over for a cool-down period whenever
 // the model's recent predictions of the steps a load buys are off by more than a factor
@@ -283,11 +283,19 @@ private:
         return best;
     }
 
+    // the reference's zero-copy threshold for a block of this size, in waiting walkers
+    // (blocksize / 128 / 24, apps/Node2vec.cu:100-104): the learned tail never comes later
+    static double zero_floor(double bytes) { return bytes / 3072.0; }
+
     bool tail_now(const SchedInput& in, uint32_t c)
     {
-        if (!use_model_ || cooldown_ > 0) return false;
         const gw_block_desc& d = in.blocks[c];
         const double bytes = ((double)d.nverts + 1) * 8 + (double)d.nedges * 4;
+        if ((double)add_[c] <= zero_floor(bytes)) {
+            ++tail_decisions_;
+            return true;
+        }
+        if (!use_model_ || cooldown_ > 0) return false;
         const double steps = std::expm1((double)std::max(0.f, pred_[c]));
         const double zc = zc_bytes();
         if (steps * zc * cfg_.tail_margin < bytes && (double)pending_hops_ * zc < cfg_.tail_guard * bytes) {
@@ -302,6 +310,12 @@ private:
         plain_.buckets = buckets;
         SchedInput pin = in;
         pin.ncblock = capacity(in);
+        uint64_t mx = 0;
+        for (uint32_t b = 0; b < in.nblocks; ++b) {
+            const gw_block_desc& d = in.blocks[b];
+            mx = std::max<uint64_t>(mx, ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4);
+        }
+        pin.zero_threshold = std::max<uint64_t>(in.zero_threshold, (uint64_t)zero_floor((double)mx));
         bool r = plain_.schedule(pin);
         buckets = plain_.buckets;
         last_kind_ = kPlain;
