@@ -232,8 +232,9 @@ def main():
     ap.add_argument("--ablation", action="store_true", help="out-of-core: also run scheduler variants")
     ap.add_argument("--schedulers", type=lambda s: s.split(","), default=None,
                     help="out-of-core: comma list of schedulers to run (default plain,plain_zero,learned)")
-    ap.add_argument("--autotune", type=int, default=0,
-                    help="time every launch shape on this many walkers (untimed) and run with the fastest")
+    ap.add_argument("--autotune", type=int, default=4_000_000,
+                    help="resident workloads: time every launch shape and search order on this many walkers "
+                         "(untimed, state restored) and run with the fastest; 0 = fixed shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X and this engine runs one "
@@ -280,6 +281,7 @@ def main():
         tuned, tms = ctx.autotune(min(args.autotune, nwalkers))
         launch = dict(block=0)
         args.variant = tuned["variant"]
+        launch = dict(block=0)
         print("[bench] autotune: %s (%.2f ms on %d walkers)" % (tuned, tms, min(args.autotune, nwalkers)),
               file=sys.stderr, flush=True)
 

@@ -13,7 +13,7 @@ GW_OK = 0
 GW_ERR = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "STATE", -5: "BUDGET", -6: "IO", -7: "FAULT"}
 GW_NODE2VEC, GW_SOPR = 0, 1
 GW_OUT_VISITS, GW_OUT_HASH, GW_OUT_PATHS = 1, 2, 4
-GW_RUN_BUCKETS, GW_RUN_TRACE = 1, 2
+GW_RUN_BUCKETS, GW_RUN_TRACE, GW_RUN_INTERP = 1, 2, 4
 GW_GEN_UNDIRECTED, GW_GEN_PERMUTE, GW_GEN_DEDUP = 1, 2, 4
 
 
@@ -162,7 +162,7 @@ class GwhReport(C.Structure):
                 ("nslots", C.c_uint32), ("nblocks", C.c_uint32), ("learned_decisions", C.c_uint64),
                 ("fallbacks", C.c_uint64), ("final_loss", C.c_float), ("pad", C.c_uint32),
                 ("staged", C.c_uint64), ("zc_touches", C.c_uint64), ("tail_decisions", C.c_uint64),
-                ("sched_ms", C.c_double)]
+                ("sched_ms", C.c_double), ("tune_ms", C.c_double), ("tuned", GwLaunch)]
 
 
 HOST_SIGNATURES = {

@@ -818,10 +818,10 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
     if (!c->nwalkers) return fail(c, GW_ERR_STATE, "gw_run before gw_init_walkers");
     (void)hipSetDevice(c->device);
     gw_launch l = L ? *L : gw_launch{};
-    if (l.block == 0) {   // shape chosen by gw_autotune, else the default
+    if (l.block == 0) {   // shape and search order chosen by gw_autotune, else the default
         const uint32_t flags = l.flags;
         l = c->have_tuned ? c->tuned : gw_launch{256, 0, 0, 0};
-        l.flags = flags;
+        l.flags = flags | (l.flags & GW_RUN_INTERP);
     }
     return run_impl(c, l, stats, 0);
 }
@@ -881,6 +881,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     a.nwork_dev = all_res ? nullptr : c->d_misc;
     a.scan = 0u;
     a.zc_count = mapped ? 1u : 0u;
+    a.search_end = (l.flags & GW_RUN_INTERP) ? (uint32_t)c->nverts_total : 0u;
     a.nwork = (uint32_t)nwork;
     a.queue = c->d_misc + 1;
     a.counters = c->d_cnt;
@@ -890,6 +891,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     }
     // the flattened variant reads u32s through aligned 8-byte words, which is only
     // safe inside padded HBM slots: zero-copy (host-mapped) blocks use variant 0
+    if (l.variant == 1 && (l.flags & GW_RUN_INTERP)) l.variant = 0;   // bisection only in variant 1
     if (l.variant == 1)
         for (uint32_t b = 0; b < c->nblocks; ++b)
             if (c->hblk[b].resident == kMapped) { l.variant = 0; break; }
@@ -983,8 +985,11 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, gw_launch* best, double* best_
     };
     const gw_counters tot0 = c->tot;
     const uint64_t unfinished0 = c->unfinished;
-    static const gw_launch cand[] = {{64, 0, 0, 0},  {128, 0, 0, 0}, {256, 0, 0, 0}, {512, 0, 0, 0},
-                                     {1024, 0, 0, 0}, {64, 0, 1, 0},  {128, 0, 1, 0}, {256, 0, 1, 0}};
+    const uint32_t I = GW_RUN_INTERP;
+    static const gw_launch cand[] = {{64, 0, 0, 0},   {128, 0, 0, 0},  {256, 0, 0, 0},  {512, 0, 0, 0},
+                                     {1024, 0, 0, 0}, {64, 0, 1, 0},   {128, 0, 1, 0},  {256, 0, 1, 0},
+                                     {64, 0, 0, I},   {128, 0, 0, I},  {256, 0, 0, I},  {512, 0, 0, I},
+                                     {1024, 0, 0, I}};
     double bms = 1e300;
     gw_launch bl = cand[2];
     for (const gw_launch& L : cand) {

@@ -33,15 +33,40 @@ struct Cnt {
 __device__ __forceinline__ uint32_t bitlen64(uint64_t n) { return n ? 64u - (uint32_t)__clzll(n) : 0u; }
 
 // engine/engine.hpp:224-243
-__device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t n, uint32_t x, uint32_t& probes)
+// Membership of x in the sorted list a[0..n), probes counting the list reads.  With
+// id_end == 0 the probes bisect (the reference's search, engine/engine.hpp:224-243).
+// Otherwise every ids is below id_end and the next probe is interpolated from the value
+// bounds [vlo, vhi) of the remaining range -- a few probes when ids are spread evenly, as
+// in randomly permuted graphs -- with a bisection whenever a probe failed to halve the
+// range and for ranges of at most 16 entries (one 64-byte line), so no list takes more
+// than about twice the bisection's probes.  Both orders give the same answer.
+__device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t n, uint32_t x, uint32_t& probes,
+                                       uint32_t id_end)
 {
     int64_t lo = 0, hi = (int64_t)n - 1;
+    if (!id_end) {
+        while (lo <= hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            const uint32_t m = a[mid];
+            ++probes;
+            if (m == x) return true;
+            if (m < x) lo = mid + 1; else hi = mid - 1;
+        }
+        return false;
+    }
+    uint64_t vlo = 0, vhi = id_end;
+    bool bisect = false;
+    if ((uint64_t)x >= vhi) return false;
     while (lo <= hi) {
-        const int64_t mid = lo + ((hi - lo) >> 1);
+        const int64_t len = hi - lo + 1;
+        int64_t mid;
+        if (bisect || len <= 16 || vhi <= vlo + 1) mid = lo + ((hi - lo) >> 1);
+        else mid = lo + (int64_t)(((uint64_t)(x - vlo) * (uint64_t)len) / (vhi - vlo));
         const uint32_t m = a[mid];
         ++probes;
         if (m == x) return true;
-        if (m < x) lo = mid + 1; else hi = mid - 1;
+        if (m < x) { lo = mid + 1; vlo = (uint64_t)m + 1; } else { hi = mid - 1; vhi = m; }
+        bisect = (hi - lo + 1) * 2 > len;
     }
     return false;
 }
@@ -111,7 +136,7 @@ __device__ __forceinline__ bool sopr_use_mixture(uint64_t du, uint64_t dv, uint3
 __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64_t dv,
                                          const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
                                          const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
-                                         uint32_t& x, Cnt& c)
+                                         uint32_t& x, Cnt& c, uint32_t id_end)
 {
     uint4 d = make_uint4(0, 0, 0, 0);
     for (uint32_t k = 0; k < kMaxTrials; ++k) {
@@ -123,7 +148,7 @@ __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64
         if (ra < tp.t_min) return true;
         uint64_t t;
         if (x == prev) t = tp.t_p;
-        else t = member(nu, du, x, c.probes) ? tp.t_one : tp.t_q;
+        else t = member(nu, du, x, c.probes, id_end) ? tp.t_one : tp.t_q;
         if (ra < t) return true;
     }
     return false;
@@ -132,7 +157,7 @@ __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64
 __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint64_t dv,
                                           const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
                                           const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
-                                          uint32_t& x, Cnt& c)
+                                          uint32_t& x, Cnt& c, uint32_t id_end)
 {
     if (tp.sampler == 0) {
         // proposal rejection: N(cur) with probability 1-a, else a prev-neighbour that is
@@ -146,7 +171,7 @@ __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint6
                 return true;
             }
             const uint32_t y = nu[((uint64_t)d.y * du) >> 32];
-            if (y != prev && member(nv, dv, y, c.probes)) {
+            if (y != prev && member(nv, dv, y, c.probes, id_end)) {
                 x = y;
                 return true;
             }
@@ -177,7 +202,7 @@ __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint6
         ++c.trials;
         x = nv[((uint64_t)rp * dv) >> 32];
         if ((u128)ra * den < num32) return true;
-        if (x != prev && member(nu, du, x, c.probes)) return true;
+        if (x != prev && member(nu, du, x, c.probes, id_end)) return true;
     }
     return false;
 }
@@ -333,8 +358,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
         if (status == kActive) {
             uint32_t x = 0;
             const uint32_t t0 = c.trials, p0 = c.probes;
-            const bool ok = tp.algo == 0 ? n2v_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c)
-                                         : sopr_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c);
+            const bool ok = tp.algo == 0
+                                ? n2v_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c, a.search_end)
+                                : sopr_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c, a.search_end);
             if (a.zc_count) {
                 // candidates come from cur's row, membership probes from prev's row
                 const bool mv = (ONE ? one.resident : a.blocks[bc].resident) == kMapped;

@@ -37,7 +37,8 @@ struct EngineConfig {
     uint64_t memory_bytes = 0;      // HBM budget for slots + walker state (0: every block fits)
     uint64_t zero_threshold = 0;
     bool zero_copy_tail = true;
-    gw_launch launch{0, 0, 0, 0};
+    gw_launch launch{0, 0, 0, 0};   // block 0: autotuned once half the cache is resident
+    uint64_t autotune_sample = 1ull << 20;   // walker ids the autotuner runs each candidate on
     bool verbose = false;
 };
 
@@ -53,8 +54,9 @@ struct RunReport {
     uint64_t steps = 0, trials = 0, probes = 0, mix_steps = 0;
     uint64_t h2d_bytes = 0, uploads = 0, staged = 0, iterations = 0;
     uint64_t zero_copy_runs = 0, zero_copy_steps = 0, zc_touches = 0;
-    double wall_s = 0, kernel_ms = 0, filter_ms = 0, h2d_ms = 0, sched_ms = 0;
+    double wall_s = 0, kernel_ms = 0, filter_ms = 0, h2d_ms = 0, sched_ms = 0, tune_ms = 0;
     uint32_t nslots = 0, nblocks = 0;
+    gw_launch tuned{0, 0, 0, 0};    // launch the autotuner picked (block 0: none)
 };
 
 class Engine {
@@ -114,6 +116,7 @@ public:
         std::vector<uint64_t> W((uint64_t)nb * nb);
         std::vector<uint8_t> staged(nb, 0);
         std::vector<uint32_t> pf;
+        bool tuned = false;
         for (uint64_t it = 0;; ++it) {
             ck(gw_bucket_counts(ctx_, W.data(), W.size()), ctx_, "gw_bucket_counts");
             uint64_t left = 0;
@@ -172,6 +175,16 @@ public:
                 }
                 for (uint32_t b = 0; b < nb; ++b) staged[b] = slots_.blk_stg[b] >= 0 ? 1 : 0;
                 r.sched_ms += std::chrono::duration<double, std::milli>(clk::now() - ts).count();
+                if (!tuned && cfg_.launch.block == 0 && cfg_.autotune_sample &&
+                    2 * sched.buckets.size() >= std::min<size_t>(nslots_, nb)) {
+                    // launch shape and search order, timed on a prefix of the walkers while
+                    // half the cache is resident (walker state is restored afterwards)
+                    auto tt = clk::now();
+                    double ms = 0;
+                    if (gw_autotune(ctx_, cfg_.autotune_sample, &r.tuned, &ms) != GW_OK) r.tuned = gw_launch{0, 0, 0, 0};
+                    r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();
+                    tuned = true;
+                }
                 ck(gw_run(ctx_, &launch, &st), ctx_, "gw_run");
                 out.h2d_bytes = bytes;
             } else {

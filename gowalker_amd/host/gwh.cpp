@@ -75,6 +75,8 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         report->staged = r.staged;
         report->zc_touches = r.zc_touches;
         report->sched_ms = r.sched_ms;
+        report->tune_ms = r.tune_ms;
+        report->tuned = r.tuned;
         if (ls) {
             report->learned_decisions = ls->learned_decisions();
             report->fallbacks = ls->fallbacks();

@@ -98,7 +98,10 @@ typedef struct {
 enum {
     GW_RUN_BUCKETS = 1u << 0, /* after the run, recount the (prev_blk, cur_blk) bucket matrix of
                                  the walkers left, so gw_bucket_counts needs no extra pass */
-    GW_RUN_TRACE = 1u << 1    /* record steps per current block (gw_trace_read) */
+    GW_RUN_TRACE = 1u << 1,   /* record steps per current block (gw_trace_read) */
+    GW_RUN_INTERP = 1u << 2   /* membership tests by guarded interpolation search instead of
+                                 bisection: same walks, fewer reads on evenly spread ids
+                                 (randomly permuted graphs); step kernel variant 0 only */
 };
 
 typedef struct {
@@ -196,8 +199,9 @@ gw_status gw_run(gw_ctx* ctx, const gw_launch* launch, gw_run_stats* stats);
 
 /* Launch-shape autotuner (no reference counterpart; the reference hard-codes 512 x 512,
  * apps/Node2vec.cu:47-48): times the step kernel on the first `sample` work items for each
- * candidate kernel variant and workgroup size, restoring walker state after every trial,
- * and makes the fastest the shape of later gw_run calls that pass block == 0. */
+ * candidate kernel variant, workgroup size and membership search order (GW_RUN_INTERP),
+ * restoring walker state after every trial, and makes the fastest the shape of later
+ * gw_run calls that pass block == 0 (best->flags carries the search order). */
 gw_status gw_autotune(gw_ctx* ctx, uint64_t sample, gw_launch* best, double* best_ms);
 
 /* Bucket matrix W[prev_blk * nblocks + cur_blk] of unfinished walkers: replaces

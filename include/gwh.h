@@ -42,6 +42,8 @@ typedef struct {
     uint64_t zc_touches;       /* zero-copy reads of host-mapped blocks (about 64 B of PCIe each) */
     uint64_t tail_decisions;   /* learned: zero-copy chosen over an upload */
     double sched_ms;           /* host time spent scheduling and placing blocks */
+    double tune_ms;            /* autotuner time (launch shape and search order) */
+    gw_launch tuned;           /* what it picked (block 0: no tuning) */
 } gwh_report;
 
 /* Runs walkers to completion out of core: beg has `entries` u64 offsets (ids

@@ -197,6 +197,37 @@ static int member(const uint32_t* a, uint64_t n, uint32_t x, uint64_t* probes)
     return 0;
 }
 
+/* The same membership test with a different probe order (not in the reference; its
+ * answer equals member()'s for every sorted list, so walks do not change): the next probe
+ * is interpolated from the value bounds [vlo, vhi) of the remaining range, which lands
+ * near x in a few probes when ids are spread evenly (randomly permuted graphs); when a
+ * probe fails to halve the range the next one bisects, and ranges of at most 16 entries
+ * are bisected. */
+static int member_interp(const uint32_t* a, uint64_t n, uint32_t x, uint32_t id_end, uint64_t* probes)
+{
+    int64_t lo = 0, hi = (int64_t)n - 1;
+    uint64_t vlo = 0, vhi = id_end;
+    int bisect = 0;
+    if ((uint64_t)x >= vhi) return 0;
+    while (lo <= hi) {
+        const int64_t len = hi - lo + 1;
+        int64_t mid;
+        if (bisect || len <= 16 || vhi <= vlo + 1) mid = lo + ((hi - lo) >> 1);
+        else mid = lo + (int64_t)(((uint64_t)(x - vlo) * (uint64_t)len) / (vhi - vlo));
+        uint32_t m = a[mid];
+        (*probes)++;
+        if (m == x) return 1;
+        if (m < x) { lo = mid + 1; vlo = (uint64_t)m + 1; } else { hi = mid - 1; vhi = m; }
+        bisect = (hi - lo + 1) * 2 > len;
+    }
+    return 0;
+}
+
+int orc_member(const uint32_t* a, uint64_t n, uint32_t x, uint32_t search, uint32_t id_end, uint64_t* probes)
+{
+    return search ? member_interp(a, n, x, id_end, probes) : member(a, n, x, probes);
+}
+
 static uint64_t lower_bound(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t x, uint64_t* probes)
 {
     while (lo < hi) {
@@ -281,7 +312,7 @@ static void walk_impl(const orc_graph* g, const orc_walkcfg* cfg, uint32_t wid, 
                 if (ra < tc->t_min) { ok = 1; break; }
                 uint64_t t;
                 if (x == prev) t = tc->t_p;
-                else if (member(csr + u_head, du, x, &probes)) t = tc->t_one;
+                else if (orc_member(csr + u_head, du, x, tc->search, (uint32_t)g->nverts, &probes)) t = tc->t_one;
                 else t = tc->t_q;
                 if (ra < t) { ok = 1; break; }
             }
@@ -296,7 +327,7 @@ static void walk_impl(const orc_graph* g, const orc_walkcfg* cfg, uint32_t wid, 
                     break;
                 }
                 uint32_t y = csr[u_head + (((uint64_t)d[1] * du) >> 32)];
-                if (y != prev && member(csr + v_head, dv, y, &probes)) {
+                if (y != prev && orc_member(csr + v_head, dv, y, tc->search, (uint32_t)g->nverts, &probes)) {
                     x = y;
                     ok = 1;
                     break;
@@ -327,7 +358,7 @@ static void walk_impl(const orc_graph* g, const orc_walkcfg* cfg, uint32_t wid, 
                     trials++;
                     x = csr[v_head + (((uint64_t)rp * dv) >> 32)];
                     if ((u128)ra * den < num32) { ok = 1; break; }
-                    if (x != prev && member(csr + u_head, du, x, &probes)) { ok = 1; break; }
+                    if (x != prev && orc_member(csr + u_head, du, x, tc->search, (uint32_t)g->nverts, &probes)) { ok = 1; break; }
                 }
             }
         }

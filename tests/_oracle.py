@@ -20,7 +20,7 @@ class Graph(C.Structure):
 class TypeCfg(C.Structure):
     _fields_ = [("algo", C.c_uint32), ("length", C.c_uint32), ("t_p", C.c_uint64), ("t_one", C.c_uint64),
                 ("t_q", C.c_uint64), ("t_min", C.c_uint64), ("alpha_a", C.c_uint32), ("alpha_b", C.c_uint32),
-                ("t_term", C.c_uint64), ("sopr_sampler", C.c_uint32), ("reserved", C.c_uint32)]
+                ("t_term", C.c_uint64), ("sopr_sampler", C.c_uint32), ("search", C.c_uint32)]
 
 
 class WalkCfg(C.Structure):
@@ -81,6 +81,8 @@ def lib():
         L.orc_split_blocks.restype = C.c_size_t
         L.orc_max_choose.argtypes = [P, C.c_uint32, P, P, C.c_uint32, C.c_uint64]
         L.orc_max_choose.restype = C.c_int
+        L.orc_member.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
+        L.orc_member.restype = C.c_int
         _lib = L
     return _lib
 
@@ -105,11 +107,12 @@ def make_graph(beg, csr):
     return g
 
 
-def type_cfg(algo, length, p=1.0, q=1.0, alpha=0.2, term=0.0, sampler=0):
+def type_cfg(algo, length, p=1.0, q=1.0, alpha=0.2, term=0.0, sampler=0, search=0):
     t = TypeCfg()
     t.algo = algo
     t.length = length
     t.sopr_sampler = sampler
+    t.search = search
     if algo == N2V:
         lib().orc_n2v_thresholds(p, q, C.byref(t))
     else:
@@ -195,3 +198,12 @@ def max_choose(W, buckets, ncblock, zero_threshold=0):
     n = C.c_uint32(len(buckets))
     r = lib().orc_max_choose(ptr(W), nb, ptr(b), C.byref(n), ncblock, zero_threshold)
     return r, list(b[:n.value])
+
+
+def member(a, x, search=0, id_end=None):
+    """(found, probes) of the oracle's membership test on the sorted u32 list a."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    pr = C.c_uint64(0)
+    end = int(id_end if id_end is not None else (int(a.max()) + 1 if len(a) else 1))
+    r = lib().orc_member(a.ctypes.data, len(a), int(x), search, end, C.byref(pr))
+    return bool(r), pr.value

@@ -11,14 +11,14 @@ import graphs as G
 from gowalker_amd.engine import NODE2VEC, OUT_HASH, OUT_PATHS, OUT_VISITS, SOPR, Blocks, Context
 
 
-def oracle_types(types):
+def oracle_types(types, search=0):
     ts = []
     for t in types:
         if t.get("algo", NODE2VEC) == NODE2VEC:
-            ts.append(O.type_cfg(O.N2V, t["length"], p=t["p"], q=t["q"], term=t.get("term", 0.0)))
+            ts.append(O.type_cfg(O.N2V, t["length"], p=t["p"], q=t["q"], term=t.get("term", 0.0), search=search))
         else:
             ts.append(O.type_cfg(O.SOPR, t["length"], alpha=t.get("alpha", 0.2), term=t.get("term", 0.0),
-                                 sampler=t.get("sampler", 0)))
+                                 sampler=t.get("sampler", 0), search=search))
     return ts
 
 
@@ -69,7 +69,7 @@ def gpu_run(beg, csr, types, seed, n, wpv=1, block_bytes=1 << 30, nslots=None, s
 
 def check_case(beg, csr, types, seed, n, wpv=1, **kw):
     g = gpu_run(beg, csr, types, seed, n, wpv, **kw)
-    ts = oracle_types(types)
+    ts = oracle_types(types, search=1 if (kw.get("launch") or {}).get("interp") else 0)
     cfg = O.walk_cfg(seed, *ts)
     o = O.run(beg, csr, cfg, n, wpv=wpv, sources=kw.get("sources"), ntypes=len(types), paths=True, visits=True)
     print("case", types, "n", n, "gpu totals", g["totals"], "oracle totals", o["totals"], "runs", g["runs"])
@@ -180,3 +180,19 @@ def test_autotune_keeps_walks_bit_identical(gpu):
     np.testing.assert_array_equal(state, o["state"])
     np.testing.assert_array_equal(hashes, o["hashes"])
     np.testing.assert_array_equal(visits, o["visits"])
+
+
+@pytest.mark.parametrize("graph", ["permuted", "skewed"])
+def test_interpolation_search_order(gpu, graph):
+    # membership by interpolation: walks identical, probe counts equal the oracle's
+    # interpolation order; Node2vec, both SOPR samplers, resident and out-of-core windows
+    if graph == "permuted":
+        beg, csr = G.rmat(12, 40000, seed=31, permute=True)
+    else:
+        beg, csr = G.rmat(12, 40000, seed=31, permute=False)
+    L = dict(interp=True)
+    check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 3, 4096, launch=L)
+    check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1)], 4, 4096, launch=L)
+    check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1, sampler=1)], 5, 4096, launch=L)
+    check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 6, 2048, block_bytes=8192, nslots=3,
+               launch=L)
