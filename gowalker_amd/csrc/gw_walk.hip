@@ -33,18 +33,20 @@ struct Cnt {
 __device__ __forceinline__ uint32_t bitlen64(uint64_t n) { return n ? 64u - (uint32_t)__clzll(n) : 0u; }
 
 // engine/engine.hpp:224-243
-// Membership of x in the sorted list a[0..n), probes counting the list reads.  With
-// id_end == 0 the probes bisect (the reference's search, engine/engine.hpp:224-243).
-// Otherwise every ids is below id_end and the next probe is interpolated from the value
-// bounds [vlo, vhi) of the remaining range -- a few probes when ids are spread evenly, as
-// in randomly permuted graphs -- with a bisection whenever a probe failed to halve the
-// range and for ranges of at most 16 entries (one 64-byte line), so no list takes more
-// than about twice the bisection's probes.  Both orders give the same answer.
+// Membership of x in the sorted list a[0..n), probes counting the list reads.  INTERP =
+// false bisects (the reference's search, engine/engine.hpp:224-243).  INTERP = true
+// (every id below id_end) interpolates the next probe from the value bounds [vlo, vhi) of
+// the remaining range -- a few probes when ids are spread evenly, as in randomly
+// permuted graphs -- in exact fp32 arithmetic (the oracle computes the same positions),
+// and bisects whenever a probe failed to halve the range and inside ranges of at most 16
+// entries (one 64-byte line), so no list takes more than about twice the bisection's
+// probes.  Both orders give the same answer.
+template <bool INTERP>
 __device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t n, uint32_t x, uint32_t& probes,
                                        uint32_t id_end)
 {
     int64_t lo = 0, hi = (int64_t)n - 1;
-    if (!id_end) {
+    if (!INTERP) {
         while (lo <= hi) {
             const int64_t mid = lo + ((hi - lo) >> 1);
             const uint32_t m = a[mid];
@@ -54,19 +56,23 @@ __device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t 
         }
         return false;
     }
-    uint64_t vlo = 0, vhi = id_end;
+    uint32_t vlo = 0, vhi = id_end;   // a[lo..hi] within [vlo, vhi)
     bool bisect = false;
-    if ((uint64_t)x >= vhi) return false;
+    if (x >= vhi) return false;
     while (lo <= hi) {
-        const int64_t len = hi - lo + 1;
+        const uint32_t len = (uint32_t)(hi - lo + 1);
         int64_t mid;
-        if (bisect || len <= 16 || vhi <= vlo + 1) mid = lo + ((hi - lo) >> 1);
-        else mid = lo + (int64_t)(((uint64_t)(x - vlo) * (uint64_t)len) / (vhi - vlo));
+        if (bisect || len <= 16 || vhi - vlo < 2) {
+            mid = lo + ((hi - lo) >> 1);
+        } else {
+            uint32_t off = (uint32_t)((float)(x - vlo) * (float)len / (float)(vhi - vlo));
+            mid = lo + (off < len ? off : len - 1);
+        }
         const uint32_t m = a[mid];
         ++probes;
         if (m == x) return true;
-        if (m < x) { lo = mid + 1; vlo = (uint64_t)m + 1; } else { hi = mid - 1; vhi = m; }
-        bisect = (hi - lo + 1) * 2 > len;
+        if (m < x) { lo = mid + 1; vlo = m + 1; } else { hi = mid - 1; vhi = m; }
+        bisect = (uint64_t)(hi - lo + 1) * 2 > len;
     }
     return false;
 }
@@ -133,6 +139,7 @@ __device__ __forceinline__ bool sopr_use_mixture(uint64_t du, uint64_t dv, uint3
     return du * (4ull * bitlen64(dv) + 2) < e * (1 + bitlen64(du));
 }
 
+template <bool INTERP>
 __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64_t dv,
                                          const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
                                          const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
@@ -148,12 +155,13 @@ __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64
         if (ra < tp.t_min) return true;
         uint64_t t;
         if (x == prev) t = tp.t_p;
-        else t = member(nu, du, x, c.probes, id_end) ? tp.t_one : tp.t_q;
+        else t = member<INTERP>(nu, du, x, c.probes, id_end) ? tp.t_one : tp.t_q;
         if (ra < t) return true;
     }
     return false;
 }
 
+template <bool INTERP>
 __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint64_t dv,
                                           const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
                                           const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
@@ -171,7 +179,7 @@ __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint6
                 return true;
             }
             const uint32_t y = nu[((uint64_t)d.y * du) >> 32];
-            if (y != prev && member(nv, dv, y, c.probes, id_end)) {
+            if (y != prev && member<INTERP>(nv, dv, y, c.probes, id_end)) {
                 x = y;
                 return true;
             }
@@ -202,7 +210,7 @@ __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint6
         ++c.trials;
         x = nv[((uint64_t)rp * dv) >> 32];
         if ((u128)ra * den < num32) return true;
-        if (x != prev && member(nu, du, x, c.probes, id_end)) return true;
+        if (x != prev && member<INTERP>(nu, du, x, c.probes, id_end)) return true;
     }
     return false;
 }
@@ -290,7 +298,7 @@ __device__ __forceinline__ uint32_t wave_take(WaveQueue& q, uint64_t mask, uint3
 // per loop iteration; a lane whose walker ends or leaves the resident set takes the
 // next one from the work queue (one atomic per wavefront), so a wavefront never waits
 // for its longest walker -- only for its slowest step.
-template <bool ONE>
+template <bool ONE, bool INTERP>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_kernel(WalkArgs a)
 {
     // block-access trace: steps per current block, histogrammed in LDS
@@ -359,8 +367,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
             uint32_t x = 0;
             const uint32_t t0 = c.trials, p0 = c.probes;
             const bool ok = tp.algo == 0
-                                ? n2v_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c, a.search_end)
-                                : sopr_step(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c, a.search_end);
+                                ? n2v_step<INTERP>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
+                                                   a.search_end)
+                                : sopr_step<INTERP>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
+                                                    a.search_end);
             if (a.zc_count) {
                 // candidates come from cur's row, membership probes from prev's row
                 const bool mv = (ONE ? one.resident : a.blocks[bc].resident) == kMapped;
@@ -1044,8 +1054,11 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
         if (one_block) hipLaunchKernelGGL(walk_flat_kernel<true>, dim3(grid), dim3(block), 0, s, a);
         else hipLaunchKernelGGL(walk_flat_kernel<false>, dim3(grid), dim3(block), 0, s, a);
     } else {
-        if (one_block) hipLaunchKernelGGL(walk_kernel<true>, dim3(grid), dim3(block), 0, s, a);
-        else hipLaunchKernelGGL(walk_kernel<false>, dim3(grid), dim3(block), 0, s, a);
+        const bool in = a.search_end != 0;
+        if (one_block && !in) hipLaunchKernelGGL((walk_kernel<true, false>), dim3(grid), dim3(block), 0, s, a);
+        else if (one_block) hipLaunchKernelGGL((walk_kernel<true, true>), dim3(grid), dim3(block), 0, s, a);
+        else if (!in) hipLaunchKernelGGL((walk_kernel<false, false>), dim3(grid), dim3(block), 0, s, a);
+        else hipLaunchKernelGGL((walk_kernel<false, true>), dim3(grid), dim3(block), 0, s, a);
     }
     return hipGetLastError();
 }
@@ -1058,8 +1071,8 @@ uint32_t walk_blocks_per_cu(bool one_block, uint32_t variant, uint32_t block)
         e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<true>, (int)block, 0)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<false>, (int)block, 0);
     else
-        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true>, (int)block, 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false>, (int)block, 0);
+        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true, false>, (int)block, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, false>, (int)block, 0);
     return (e == hipSuccess && n > 0) ? (uint32_t)n : 1u;
 }
 

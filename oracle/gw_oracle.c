@@ -199,26 +199,31 @@ static int member(const uint32_t* a, uint64_t n, uint32_t x, uint64_t* probes)
 
 /* The same membership test with a different probe order (not in the reference; its
  * answer equals member()'s for every sorted list, so walks do not change): the next probe
- * is interpolated from the value bounds [vlo, vhi) of the remaining range, which lands
- * near x in a few probes when ids are spread evenly (randomly permuted graphs); when a
- * probe fails to halve the range the next one bisects, and ranges of at most 16 entries
- * are bisected. */
+ * is interpolated from the value bounds [vlo, vhi) of the remaining range in IEEE single
+ * precision (the kernel computes the same positions), which lands near x in a few probes
+ * when ids are spread evenly (randomly permuted graphs); when a probe fails to halve the
+ * range the next one bisects, and ranges of at most 16 entries are bisected. */
 static int member_interp(const uint32_t* a, uint64_t n, uint32_t x, uint32_t id_end, uint64_t* probes)
 {
     int64_t lo = 0, hi = (int64_t)n - 1;
-    uint64_t vlo = 0, vhi = id_end;
+    uint32_t vlo = 0, vhi = id_end;
     int bisect = 0;
-    if ((uint64_t)x >= vhi) return 0;
+    if (x >= vhi) return 0;
     while (lo <= hi) {
-        const int64_t len = hi - lo + 1;
+        const uint32_t len = (uint32_t)(hi - lo + 1);
         int64_t mid;
-        if (bisect || len <= 16 || vhi <= vlo + 1) mid = lo + ((hi - lo) >> 1);
-        else mid = lo + (int64_t)(((uint64_t)(x - vlo) * (uint64_t)len) / (vhi - vlo));
+        if (bisect || len <= 16 || vhi - vlo < 2) {
+            mid = lo + ((hi - lo) >> 1);
+        } else {
+            volatile float num = (float)(x - vlo) * (float)len;   /* no excess precision */
+            const uint32_t off = (uint32_t)(num / (float)(vhi - vlo));
+            mid = lo + (off < len ? off : len - 1);
+        }
         uint32_t m = a[mid];
         (*probes)++;
         if (m == x) return 1;
-        if (m < x) { lo = mid + 1; vlo = (uint64_t)m + 1; } else { hi = mid - 1; vhi = m; }
-        bisect = (hi - lo + 1) * 2 > len;
+        if (m < x) { lo = mid + 1; vlo = m + 1; } else { hi = mid - 1; vhi = m; }
+        bisect = (uint64_t)(hi - lo + 1) * 2 > len;
     }
     return 0;
 }
