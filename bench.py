@@ -148,10 +148,12 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                 if first:
                     hashes[(cap_gb, name)] = h
                 print("[bench] cap %g GB %s run %d: %.3f s, %d steps, %.2f GB H2D (%d uploads + %d staged), "
-                      "%.3f GB zero-copy, %d iterations, kernel %.0f ms, copy %.0f ms, sched %.0f ms, %d slots / %d blocks"
+                      "%.3f GB zero-copy, %d iterations, kernel %.0f ms, copy %.0f ms, sched %.0f ms, tune %.0f ms %s, "
+                      "%d slots / %d blocks"
                       % (cap_gb, name, i, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["uploads"],
                          rep["staged"], rep["zc_touches"] * 64 / 1e9, rep["iterations"], rep["kernel_ms"],
-                         rep["h2d_ms"], rep["sched_ms"], rep["nslots"], rep["nblocks"]), file=sys.stderr, flush=True)
+                         rep["h2d_ms"], rep["sched_ms"], rep["tune_ms"], rep["tuned"], rep["nslots"], rep["nblocks"]),
+                      file=sys.stderr, flush=True)
                 if i >= args.warmup:
                     reps.append(rep)
             r0 = reps[0]

@@ -84,7 +84,7 @@ SIGNATURES = {
     "gw_commit_block": [P, C.c_uint32],
     "gw_staged_block": [P, C.c_uint32, C.POINTER(C.c_int32), u32p],
     "gw_run": [P, C.POINTER(GwLaunch), C.POINTER(GwRunStats)],
-    "gw_autotune": [P, C.c_uint64, C.POINTER(GwLaunch), C.POINTER(C.c_double)],
+    "gw_autotune": [P, C.c_uint64, C.c_uint32, C.POINTER(GwLaunch), C.POINTER(C.c_double)],
     "gw_bucket_counts": [P, P, C.c_uint64],
     "gw_walkers_left": [P, u64p],
     "gw_read_visits": [P, P, C.c_uint64],

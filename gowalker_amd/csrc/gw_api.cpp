@@ -956,7 +956,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
 // every candidate (kernel variant x workgroup size), restoring walker state, visit counts
 // and counters after each trial, and keeps the fastest for later gw_run calls that pass
 // block == 0.  Walks are bit-identical for every shape; only time changes.
-gw_status gw_autotune(gw_ctx* c, uint64_t sample, gw_launch* best, double* best_ms)
+gw_status gw_autotune(gw_ctx* c, uint64_t sample, uint32_t what, gw_launch* best, double* best_ms)
 {
     if (!c) return fail(c, GW_ERR_ARG, "null context");
     if (!c->nwalkers) return fail(c, GW_ERR_STATE, "gw_autotune before gw_init_walkers");
@@ -993,6 +993,10 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, gw_launch* best, double* best_
     double bms = 1e300;
     gw_launch bl = cand[2];
     for (const gw_launch& L : cand) {
+        // candidates outside `what` are skipped; 256 threads of variant 0 always run
+        if (L.variant == 1 && !(what & GW_TUNE_VARIANT)) continue;
+        if (L.variant == 0 && L.block != 256 && !(what & GW_TUNE_BLOCK)) continue;
+        if ((L.flags & GW_RUN_INTERP) && !(what & GW_TUNE_SEARCH)) continue;
         double t = 1e300;
         for (int rep = 0; rep < 2 && st == GW_OK; ++rep) {
             gw_run_stats rs{};

@@ -174,11 +174,12 @@ class Context:
         self._chk(lib().gw_run(self.h, C.byref(L), C.byref(s)))
         return {f: getattr(s, f) for f, _ in GwRunStats._fields_}
 
-    def autotune(self, sample):
-        """Pick the fastest launch shape on the first `sample` walkers (state restored)."""
+    def autotune(self, sample, what=7):
+        """Pick the fastest launch shape / search order on the first `sample` walkers (state
+        restored); `what`: 1 workgroup sizes, 2 kernel variant 1, 4 search order."""
         best = GwLaunch()
         ms = C.c_double()
-        self._chk(lib().gw_autotune(self.h, sample, C.byref(best), C.byref(ms)))
+        self._chk(lib().gw_autotune(self.h, sample, what, C.byref(best), C.byref(ms)))
         return dict(block=best.block, waves_per_cu=best.waves_per_cu, variant=best.variant,
                     interp=bool(best.flags & GW_RUN_INTERP)), ms.value
 
@@ -310,7 +311,8 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
     o.verbose = 1 if verbose else 0
     o.flags = flags
     if launch:
-        o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 0), 0)
+        o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 0),
+                            GW_RUN_INTERP if launch.get("interp") else 0)
     wp = walk_params(types, seed)
     rep = GwhReport()
     h = np.zeros(nwalkers, dtype=np.uint64) if hashes else None

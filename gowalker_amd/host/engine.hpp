@@ -38,7 +38,9 @@ struct EngineConfig {
     uint64_t zero_threshold = 0;
     bool zero_copy_tail = true;
     gw_launch launch{0, 0, 0, 0};   // block 0: autotuned once half the cache is resident
-    uint64_t autotune_sample = 1ull << 20;   // walker ids the autotuner runs each candidate on
+    uint64_t autotune_sample = 1ull << 22;   // walker ids the autotuner runs each candidate on
+                                             // (membership search order only: a partial
+                                             // cache leaves too little work to rank shapes)
     bool verbose = false;
 };
 
@@ -181,7 +183,8 @@ public:
                     // half the cache is resident (walker state is restored afterwards)
                     auto tt = clk::now();
                     double ms = 0;
-                    if (gw_autotune(ctx_, cfg_.autotune_sample, &r.tuned, &ms) != GW_OK) r.tuned = gw_launch{0, 0, 0, 0};
+                    if (gw_autotune(ctx_, cfg_.autotune_sample, GW_TUNE_SEARCH, &r.tuned, &ms) != GW_OK)
+                        r.tuned = gw_launch{0, 0, 0, 0};
                     r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();
                     tuned = true;
                 }

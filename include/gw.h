@@ -199,10 +199,14 @@ gw_status gw_run(gw_ctx* ctx, const gw_launch* launch, gw_run_stats* stats);
 
 /* Launch-shape autotuner (no reference counterpart; the reference hard-codes 512 x 512,
  * apps/Node2vec.cu:47-48): times the step kernel on the first `sample` work items for each
- * candidate kernel variant, workgroup size and membership search order (GW_RUN_INTERP),
+ * candidate kernel variant, workgroup size and membership search order (GW_RUN_INTERP) that
+ * `what` selects (GW_TUNE_*; 256 threads, variant 0, bisection is always a candidate),
  * restoring walker state after every trial, and makes the fastest the shape of later
- * gw_run calls that pass block == 0 (best->flags carries the search order). */
-gw_status gw_autotune(gw_ctx* ctx, uint64_t sample, gw_launch* best, double* best_ms);
+ * gw_run calls that pass block == 0 (best->flags carries the search order).  Time it on
+ * a sample that keeps the GPU as busy as the real runs: small samples favour shapes
+ * that lose on full workloads. */
+enum { GW_TUNE_BLOCK = 1u << 0, GW_TUNE_VARIANT = 1u << 1, GW_TUNE_SEARCH = 1u << 2, GW_TUNE_ALL = 7u };
+gw_status gw_autotune(gw_ctx* ctx, uint64_t sample, uint32_t what, gw_launch* best, double* best_ms);
 
 /* Bucket matrix W[prev_blk * nblocks + cur_blk] of unfinished walkers: replaces
  * copy_back's read of block_offset (engine/walk.hpp:488-498) and
