@@ -37,10 +37,7 @@ struct EngineConfig {
     uint64_t memory_bytes = 0;      // HBM budget for slots + walker state (0: every block fits)
     uint64_t zero_threshold = 0;
     bool zero_copy_tail = true;
-    gw_launch launch{0, 0, 0, 0};   // block 0: autotuned once half the cache is resident
-    uint64_t autotune_sample = 1ull << 22;   // walker ids the autotuner runs each candidate on
-                                             // (membership search order only: a partial
-                                             // cache leaves too little work to rank shapes)
+    gw_launch launch{0, 0, 0, 0};   // block 0: 256 threads, search order from the probe model
     bool verbose = false;
 };
 
@@ -61,6 +58,70 @@ struct RunReport {
     gw_launch tuned{0, 0, 0, 0};    // launch the autotuner picked (block 0: none)
 };
 
+// Probes of a bisection and of the kernel's interpolation order (same estimate rule)
+// over `q` queries x in N(u), u the source of a random arc and x drawn from the list
+// of a random neighbour of u -- the membership tests Node2vec makes.  Interpolation
+// is preferred when it needs under 80% of the bisection's probes.
+inline bool prefer_interpolation(const uint64_t* beg, const uint32_t* csr, uint64_t nv, uint32_t q = 4096)
+{
+    const uint64_t ne = nv ? beg[nv] - beg[0] : 0;
+    if (!ne || nv > 0xFFFFFFFFull) return false;
+    uint64_t z = 0x9E3779B97F4A7C15ULL, pb = 0, pi = 0;
+    auto rnd = [&](uint64_t n) {
+        z += 0x9E3779B97F4A7C15ULL;
+        uint64_t x = z;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+        return (x ^ (x >> 31)) % n;
+    };
+    auto src = [&](uint64_t e) {   // vertex whose list holds arc e
+        uint64_t lo = 0, hi = nv;
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) / 2;
+            if (beg[mid] <= e) lo = mid; else hi = mid;
+        }
+        return lo;
+    };
+    for (uint32_t i = 0; i < q; ++i) {
+        const uint64_t e = beg[0] + rnd(ne);
+        const uint64_t u = src(e);
+        const uint32_t v = csr[e];
+        if (v >= nv) return false;
+        const uint64_t dv = beg[v + 1] - beg[v];
+        if (!dv) continue;
+        const uint32_t x = csr[beg[v] + rnd(dv)];
+        const uint32_t* a = csr + beg[u];
+        const int64_t n = (int64_t)(beg[u + 1] - beg[u]);
+        int64_t lo = 0, hi = n - 1;
+        while (lo <= hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            ++pb;
+            if (a[mid] == x) break;
+            if (a[mid] < x) lo = mid + 1; else hi = mid - 1;
+        }
+        lo = 0;
+        hi = n - 1;
+        uint32_t vlo = 0, vhi = (uint32_t)nv;
+        bool bisect = false;
+        while (lo <= hi) {
+            const uint32_t len = (uint32_t)(hi - lo + 1);
+            int64_t mid;
+            if (bisect || len <= 16 || vhi - vlo < 2) {
+                mid = lo + ((hi - lo) >> 1);
+            } else {
+                const uint32_t off = (uint32_t)((float)(x - vlo) * (float)len / (float)(vhi - vlo));
+                mid = lo + (off < len ? off : len - 1);
+            }
+            ++pi;
+            if (a[mid] == x) break;
+            if (a[mid] < x) { lo = mid + 1; vlo = a[mid] + 1; } else { hi = mid - 1; vhi = a[mid]; }
+            bisect = (uint64_t)(hi - lo + 1) * 2 > len;
+        }
+    }
+    return pb && pi * 5 < pb * 4;
+}
+
+
 class Engine {
 public:
     // beg/csr: the whole graph in host memory (pinned for the zero-copy tail); blocks:
@@ -77,6 +138,7 @@ public:
     Engine& operator=(const Engine&) = delete;
 
     gw_ctx* ctx() { return ctx_; }
+    uint64_t total_verts() const { return blocks_.empty() ? 0 : blocks_.back().start_vert + blocks_.back().nverts; }
     uint32_t nblocks() const { return (uint32_t)blocks_.size(); }
 
     // Number of HBM slots that fit next to the walker state under the budget.
@@ -118,7 +180,16 @@ public:
         std::vector<uint64_t> W((uint64_t)nb * nb);
         std::vector<uint8_t> staged(nb, 0);
         std::vector<uint32_t> pf;
-        bool tuned = false;
+        gw_launch base = cfg_.launch;
+        if (base.block == 0) {
+            // the membership search order is chosen before the first run from the probes
+            // both orders need on sampled Node2vec-style queries of this graph (timing on the
+            // device would need a resident cache that is not there yet)
+            auto tt = clk::now();
+            base = gw_launch{256, 0, 0, prefer_interpolation(beg_, csr_, total_verts()) ? (uint32_t)GW_RUN_INTERP : 0u};
+            r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();
+            r.tuned = base;
+        }
         for (uint64_t it = 0;; ++it) {
             ck(gw_bucket_counts(ctx_, W.data(), W.size()), ctx_, "gw_bucket_counts");
             uint64_t left = 0;
@@ -137,7 +208,7 @@ public:
             IterationOutcome out;
             out.iteration = it;
             gw_run_stats st{};
-            gw_launch launch = cfg_.launch;
+            gw_launch launch = base;
             launch.flags |= sched.run_flags() | GW_RUN_BUCKETS;
             if (sched.schedule(in)) {
                 uint64_t bytes = 0;
@@ -177,17 +248,6 @@ public:
                 }
                 for (uint32_t b = 0; b < nb; ++b) staged[b] = slots_.blk_stg[b] >= 0 ? 1 : 0;
                 r.sched_ms += std::chrono::duration<double, std::milli>(clk::now() - ts).count();
-                if (!tuned && cfg_.launch.block == 0 && cfg_.autotune_sample &&
-                    2 * sched.buckets.size() >= std::min<size_t>(nslots_, nb)) {
-                    // launch shape and search order, timed on a prefix of the walkers while
-                    // half the cache is resident (walker state is restored afterwards)
-                    auto tt = clk::now();
-                    double ms = 0;
-                    if (gw_autotune(ctx_, cfg_.autotune_sample, GW_TUNE_SEARCH, &r.tuned, &ms) != GW_OK)
-                        r.tuned = gw_launch{0, 0, 0, 0};
-                    r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();
-                    tuned = true;
-                }
                 ck(gw_run(ctx_, &launch, &st), ctx_, "gw_run");
                 out.h2d_bytes = bytes;
             } else {

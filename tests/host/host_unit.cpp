@@ -1,6 +1,7 @@
 // Unit tests of the host layer that need no GPU: option parsing (api/cmdopts.hpp
 // semantics), the on-disk format (names, meta, block files, truncation), the plain
 // scheduler against the oracle's known answers, and slot placement.
+#include <algorithm>
 #include <cassert>
 #include <cstdio>
 #include <cstdlib>
@@ -9,6 +10,7 @@
 
 #include "cmdopts.hpp"
 #include "format.hpp"
+#include "engine.hpp"
 #include "schedule.hpp"
 
 using namespace gwh;
@@ -129,6 +131,30 @@ static void test_scheduler()
     CHECK(st.free_slots({0, 1, 4}) == 0 && st.place(5, {0, 1, 4, 5}) == -2);
 }
 
+// the engine's search-order model: evenly spread ids favour interpolation, lists of
+// clustered small ids (an unpermuted power-law graph) do not
+static void test_search_order_model()
+{
+    const uint64_t n = 1 << 14, d = 1024;
+    std::vector<uint64_t> beg(n + 1);
+    std::vector<uint32_t> even(n * d), skew(n * d);
+    uint64_t z = 12345;
+    auto rnd = [&]() { z = z * 6364136223846793005ULL + 1442695040888963407ULL; return (uint32_t)(z >> 33); };
+    for (uint64_t v = 0; v < n; ++v) {
+        beg[v] = v * d;
+        for (uint64_t k = 0; k < d; ++k) {
+            even[v * d + k] = rnd() % n;
+            const uint32_t r = rnd() % 1000;
+            skew[v * d + k] = (uint32_t)((uint64_t)r * r * r % n / 997 / 997);   // clustered near 0
+        }
+        std::sort(even.begin() + v * d, even.begin() + (v + 1) * d);
+        std::sort(skew.begin() + v * d, skew.begin() + (v + 1) * d);
+    }
+    beg[n] = n * d;
+    CHECK(prefer_interpolation(beg.data(), even.data(), n));
+    CHECK(!prefer_interpolation(beg.data(), skew.data(), n));
+}
+
 int main(int argc, char** argv)
 {
     std::string dir = argc > 1 ? argv[1] : "/tmp";
@@ -136,6 +162,7 @@ int main(int argc, char** argv)
     test_names();
     test_format(dir);
     test_scheduler();
+    test_search_order_model();
     std::printf("host_unit: all passed\n");
     return 0;
 }
