@@ -203,7 +203,8 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                    "caps": per_cap, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS,
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "walk_kernel<false>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "walk_kernel<false, %s>" % ("true" if (r0.get("tuned") or {}).get("interp") else "false"),
                      "kernel_ms": r0["kernel_ms"] / launches, "algorithmic_bytes": abytes,
                      "sector_bytes": sector_bytes(r0, nwalkers, False) / launches,
                      "launches": launches,
@@ -331,7 +332,8 @@ def main():
                    "parallelism": "1 GPU", "launch": tuned or launch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "walk_flat_kernel<true>" if args.variant == 1 else "walk_kernel<true>",
+                     "kernel": "walk_flat_kernel<true>" if args.variant == 1 else
+                               "walk_kernel<true, %s>" % ("true" if (tuned or {}).get("interp") else "false"),
                      "kernel_ms": avg_kms, "algorithmic_bytes": abytes,
                      "sector_bytes": sector_bytes(st0, nwalkers, visits)},
         "cpu_baseline": None,
