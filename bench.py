@@ -51,6 +51,14 @@ CONFIGS = {
                  "plain vs learned block scheduler",
         graph=dict(kind="rmat", scale=24, nedges=400_000_000, undirected=True, permute=True),
         types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=64, memory_gb=[3]),
+    "mixed_rmat27_ooc": dict(
+        workload="Mixed batch out of core: RMAT scale 27 (134.2M ids) with 5.5B generated directed edges, dedup, "
+                 "ids in generation order (locality kept), 256 MB blocks, HBM cap 16 GB; 2 walkers/vertex: "
+                 "Node2vec p 1, q 0.25, length 80 and SOSR alpha 0.2, termination 0.2 (length cap 80) "
+                 "co-scheduled in one kernel",
+        graph=dict(kind="rmat", scale=27, nedges=5_500_000_000, undirected=False, permute=False),
+        types=[dict(algo=0, length=80, p=1.0, q=0.25), dict(algo=1, length=80, alpha=0.2, term=0.2)], wpv=2,
+        out_of_core=True, block_mb=256, memory_gb=[16]),
     "node2vec_rmat20": dict(
         workload="Node2vec, RMAT scale 20 (2^20 V, 16M edges symmetrized, dedup, no self-loops), "
                  "p 2, q 0.5, 1 walker/vertex, length 80, single resident block",
