@@ -291,7 +291,11 @@ private:
     {
         const gw_block_desc& d = in.blocks[c];
         const double bytes = ((double)d.nverts + 1) * 8 + (double)d.nedges * 4;
-        if ((double)add_[c] <= zero_floor(bytes)) {
+        // the reference's rule looks at the best walker count of all candidates
+        uint64_t best_add = 0;
+        for (uint32_t b = 0; b < in.nblocks; ++b)
+            if (!was_resident_[b]) best_add = std::max(best_add, add_[b]);
+        if ((double)best_add <= zero_floor(bytes)) {
             ++tail_decisions_;
             return true;
         }

@@ -213,9 +213,11 @@ gw_status gw_autotune(gw_ctx* ctx, uint64_t sample, uint32_t what, gw_launch* be
  * walk_block::block_numwalks.  Free after gw_init_walkers or a gw_run with
  * GW_RUN_BUCKETS; otherwise one pass over the walkers. */
 gw_status gw_bucket_counts(gw_ctx* ctx, uint64_t* W, uint64_t n);
+/* Unfinished walkers: replaces walk_manager->nwalks() (engine/engine.hpp:519). */
 gw_status gw_walkers_left(gw_ctx* ctx, uint64_t* left);
 /* Unfinished walkers and the sum of their remaining hop caps (the work left; an upper
- * bound under geometric termination).  Free after a gw_run with GW_RUN_BUCKETS. */
+ * bound under geometric termination); the hop sum has no reference counterpart and feeds
+ * the learned scheduler's zero-copy decision.  Free after a gw_run with GW_RUN_BUCKETS. */
 gw_status gw_pending_work(gw_ctx* ctx, uint64_t* walkers, uint64_t* hops);
 
 /* ---- learned block scheduler support (no reference counterpart: the reference's
