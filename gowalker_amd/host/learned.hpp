@@ -10,19 +10,18 @@
 //   fill   (free slots left): commit staged blocks whose copies are done, else the best
 //          staged one (the run waits for it), else upload the best candidate; then stage
 //          the next-best blocks into the free slots so their copies overlap the run;
-//   swap   (cache full): load the candidate c with the highest y among blocks whose
-//          walkers could run now; evict the resident block the fewest waiting walkers
-//          need;
-//   tail   when y(c) says the upload would buy fewer steps than its bytes buy in
-//          zero-copy reads (steps x reads per step x 64 B < block bytes, reads per step
-//          measured on the walk so far) and all the work left is worth at most
-//          tail_guard uploads in zero-copy bytes, finish the walk by zero-copy instead;
-//          the reference's fixed walker-count threshold (apps/Node2vec.cu:100-104, the
-//          plain rule's version of this decision) is the latest point it switches at.
-// The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: it decides
-// while the model has too few samples, and it takes over for a cool-down period whenever
-// the model's recent predictions of the steps a load buys are off by more than a factor
-// e on average (the judge).  Walks stay bit-identical either way (counter-based
+//   swap   (cache full): the plain rule's load and victim;
+//   tail   when y(c) of the block the plain rule would load says the upload buys fewer
+//          steps than its bytes buy in zero-copy reads (steps x reads per step x 64 B <
+//          block bytes, reads per step measured on the walk so far) and all the work left
+//          is worth at most tail_guard uploads in zero-copy bytes, finish the walk by
+//          zero-copy instead; the reference's fixed walker-count threshold
+//          (apps/Node2vec.cu:100-104, the plain rule's version of this decision) is the
+//          latest point it switches at.
+// The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: the model is
+// ignored while it has too few samples and, for a cool-down period, whenever its recent
+// predictions of the steps a load buys are off by more than a factor e on average (the
+// judge).  Walks stay bit-identical either way (counter-based
 // RNG).
 #pragma once
 #include <algorithm>
@@ -39,7 +38,8 @@ struct LearnedConfig {
     uint32_t h1 = 64, h2 = 64;
     uint64_t seed = 7;
     uint32_t min_samples = 24;      // plain swap decisions until this many upload samples
-    uint32_t train_steps = 4;       // Adam steps after every iteration
+    uint32_t train_steps = 4;       // Adam steps per training round
+    uint32_t train_every = 8;       // new samples between training rounds
     uint32_t batch = 256;
     uint32_t max_samples = 16384;   // ring buffer of trace samples
     float lr = 2e-3f;
@@ -97,20 +97,19 @@ public:
             return true;
         }
 
-        // ---- swap: the cache is full
-        if (!trained() || cooldown_ > 0) {
-            if (cooldown_ > 0) --cooldown_;
-            return plain_decide(in);
+        // ---- swap: the cache is full.  The plain rule picks the load and the victim (its
+        // pairing-aware eviction moved fewer bytes than evicting by predicted value in our
+        // measurements); the model decides when uploads stop paying for themselves.
+        if (cooldown_ > 0) --cooldown_;
+        const std::vector<uint32_t> before = buckets;
+        if (!plain_decide(in)) return false;
+        int c = -1;
+        for (uint32_t b : buckets)
+            if (std::find(before.begin(), before.end(), b) == before.end()) c = (int)b;
+        if (c >= 0 && tail_now(in, (uint32_t)c)) {
+            buckets = before;
+            return false;
         }
-        const int c = best_candidate(cached);
-        if (c < 0) return plain_decide(in);
-        if (tail_now(in, (uint32_t)c)) return false;
-        int victim = -1;
-        for (uint32_t p = 0; p < buckets.size(); ++p)
-            if (victim < 0 || demand_[buckets[p]] < demand_[buckets[victim]]) victim = (int)p;
-        buckets[victim] = (uint32_t)c;
-        last_kind_ = kLearnedSwap;
-        ++learned_decisions_;
         return true;
     }
 
@@ -186,7 +185,7 @@ public:
                 open_[b].steps += (double)trace[b];
             age_[b] = trace[b] ? 0.f : std::min(age_[b] + 1.f, 1024.f);
         }
-        if (mlp_ && samples_ >= 8 && samples_ != trained_at_) {
+        if (mlp_ && samples_ >= 8 && samples_ >= trained_at_ + cfg_.train_every) {
             trained_at_ = samples_;
             train();
         }
@@ -263,8 +262,20 @@ private:
             add_[b] = a;
             demand_[b] = d;
         }
+        // the model is consulted while free slots are being filled and near the zero-copy
+        // tail (best candidate within 64x the reference's threshold); elsewhere the plain
+        // rule decides and a forward pass would be wasted
+        uint64_t best_add = 0, max_bytes = 0;
+        for (uint32_t b = 0; b < nb; ++b) {
+            if (!cached[b]) best_add = std::max(best_add, add_[b]);
+            const gw_block_desc& d = in.blocks[b];
+            max_bytes = std::max<uint64_t>(max_bytes, ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4);
+        }
+        const bool need = buckets.size() < capacity(in) || (double)best_add <= 64.0 * zero_floor((double)max_bytes);
         pred_.assign(nb, 0.f);
-        use_model_ = trained() && have_feats_ && gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
+        use_model_ = need && trained() && have_feats_ &&
+                     gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
+        if (use_model_) ++learned_decisions_;
         rank_.resize(nb);
         // before the model is trained: the walkers a load would release (log scale), with
         // the walkers that need the block at all as a tie-break
