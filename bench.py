@@ -249,7 +249,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X and this engine runs one "
-              "process per GPU; multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+              "process per GPU; sharding by walker id (resident graphs) or by block (out of core) is not run "
+              "here.", file=sys.stderr)
         sys.exit(2)
 
     from gowalker_amd import engine as E
