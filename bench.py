@@ -248,9 +248,9 @@ def main():
                          "(untimed, state restored) and run with the fastest; 0 = fixed shape")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has a single MI355X and this engine runs one "
-              "process per GPU; sharding by walker id (resident graphs) or by block (out of core) is not run "
-              "here.", file=sys.stderr)
+        print("bench.py measures one MI355X (--gpus 1); the engine runs one process per GPU, and sharding by "
+              "walker id (resident graphs) or by block (out of core) is not run by this benchmark.",
+              file=sys.stderr)
         sys.exit(2)
 
     from gowalker_amd import engine as E
