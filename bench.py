@@ -104,15 +104,25 @@ def cpu_baseline(beg, csr, cfg, seconds, threads):
     sample = 4_000_000
     src = rng.integers(0, n, sample).astype(np.uint32)
     algo = O.N2V if t["algo"] == 0 else O.SOPR
-    t0 = time.perf_counter()
-    r = O.ref_run(beg, csr, algo, t["length"], p=t.get("p", 1.0), q=t.get("q", 1.0), alpha=t.get("alpha", 0.2),
-                  term=t.get("term", 0.0), literal=False, nwalkers=sample, sources=src, max_seconds=seconds,
-                  nthreads=threads)
-    dt = time.perf_counter() - t0
+    def timed(literal, nthreads, secs, nwalk):
+        t0 = time.perf_counter()
+        r = O.ref_run(beg, csr, algo, t["length"], p=t.get("p", 1.0), q=t.get("q", 1.0), alpha=t.get("alpha", 0.2),
+                      term=t.get("term", 0.0), literal=literal, nwalkers=nwalk, sources=src[:nwalk], max_seconds=secs,
+                      nthreads=nthreads)
+        return r, time.perf_counter() - t0
+
+    r, dt = timed(False, threads, seconds, sample)
+    r1, dt1 = timed(False, 1, seconds / 4, sample // 8)
+    rl, dtl = timed(True, threads, seconds / 4, sample)
     return dict(value=r["moves"] / dt, unit="steps/s", cores=threads, kind="port",
                 sample="%d of %d random-source walkers run to completion in %.1f s (%d moves); C transcription of "
                        "the reference CPU step (engine/engine.hpp:158-404, fixed-mode RandNum), OpenMP"
-                       % (r["done"], sample, dt, r["moves"]))
+                       % (r["done"], sample, dt, r["moves"]),
+                value_1core=r1["moves"] / dt1,
+                value_literal_randnum=rl["moves"] / dtl,
+                note="value_1core: same code on one thread (%d walkers); value_literal_randnum: the reference's "
+                     "type-punned inlineRandPair (util/util.hpp:291-301), nearly first-order sampling"
+                     % r1["done"])
 
 
 PCIE_PEAK_GBS = 64.0   # PCIe Gen5 x16, one direction
