@@ -821,7 +821,7 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
     if (l.block == 0) {   // shape and search order chosen by gw_autotune, else the default
         const uint32_t flags = l.flags;
         l = c->have_tuned ? c->tuned : gw_launch{256, 0, 0, 0};
-        l.flags = flags | (l.flags & (GW_RUN_INTERP | GW_RUN_SCAN));
+        l.flags = flags | (l.flags & GW_RUN_INTERP);
     }
     return run_impl(c, l, stats, 0);
 }
@@ -881,8 +881,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     a.nwork_dev = all_res ? nullptr : c->d_misc;
     a.scan = 0u;
     a.zc_count = mapped ? 1u : 0u;
-    a.search_order = (l.flags & GW_RUN_INTERP) ? 1u : ((l.flags & GW_RUN_SCAN) ? 2u : 0u);
-    a.search_end = (uint32_t)c->nverts_total;
+    a.search_end = (l.flags & GW_RUN_INTERP) ? (uint32_t)c->nverts_total : 0u;
     a.nwork = (uint32_t)nwork;
     a.queue = c->d_misc + 1;
     a.counters = c->d_cnt;
@@ -892,7 +891,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     }
     // the flattened variant reads u32s through aligned 8-byte words, which is only
     // safe inside padded HBM slots: zero-copy (host-mapped) blocks use variant 0
-    if (l.variant == 1 && (l.flags & (GW_RUN_INTERP | GW_RUN_SCAN))) l.variant = 0;   // variant 1 bisects only
+    if (l.variant == 1 && (l.flags & GW_RUN_INTERP)) l.variant = 0;   // bisection only in variant 1
     if (l.variant == 1)
         for (uint32_t b = 0; b < c->nblocks; ++b)
             if (c->hblk[b].resident == kMapped) { l.variant = 0; break; }
@@ -986,19 +985,18 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, uint32_t what, gw_launch* best
     };
     const gw_counters tot0 = c->tot;
     const uint64_t unfinished0 = c->unfinished;
-    const uint32_t I = GW_RUN_INTERP, S = GW_RUN_SCAN;
+    const uint32_t I = GW_RUN_INTERP;
     static const gw_launch cand[] = {{64, 0, 0, 0},   {128, 0, 0, 0},  {256, 0, 0, 0},  {512, 0, 0, 0},
                                      {1024, 0, 0, 0}, {64, 0, 1, 0},   {128, 0, 1, 0},  {256, 0, 1, 0},
                                      {64, 0, 0, I},   {128, 0, 0, I},  {256, 0, 0, I},  {512, 0, 0, I},
-                                     {1024, 0, 0, I}, {64, 0, 0, S},   {128, 0, 0, S},  {256, 0, 0, S},
-                                     {512, 0, 0, S},  {1024, 0, 0, S}};
+                                     {1024, 0, 0, I}};
     double bms = 1e300;
     gw_launch bl = cand[2];
     for (const gw_launch& L : cand) {
         // candidates outside `what` are skipped; 256 threads of variant 0 always run
         if (L.variant == 1 && !(what & GW_TUNE_VARIANT)) continue;
         if (L.variant == 0 && L.block != 256 && !(what & GW_TUNE_BLOCK)) continue;
-        if ((L.flags & (GW_RUN_INTERP | GW_RUN_SCAN)) && !(what & GW_TUNE_SEARCH)) continue;
+        if ((L.flags & GW_RUN_INTERP) && !(what & GW_TUNE_SEARCH)) continue;
         double t = 1e300;
         for (int rep = 0; rep < 2 && st == GW_OK; ++rep) {
             gw_run_stats rs{};

@@ -68,8 +68,7 @@ struct WalkArgs {
     const uint32_t* nwork_dev;  // if set, the worklist length is read here (written by the filter pass)
     uint32_t scan;            // identity list: also skip walkers whose blocks are not both resident
     uint32_t zc_count;        // some block is host-mapped: count zero-copy reads (counters[7])
-    uint32_t search_order;    // membership probe order: 0 bisection, 1 interpolation + scan, 2 bisection + scan
-    uint32_t search_end;      // order 1: every id is below it
+    uint32_t search_end;      // 0: membership by bisection; else interpolation, ids below it
     uint32_t* queue;          // work-fetch counter, zero at launch
     uint32_t chunk;           // walkers a wavefront takes per queue atomic (multiple of 64)
     unsigned long long* counters;  // steps, trials, probes, mix, ended, left

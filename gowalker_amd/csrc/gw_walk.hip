@@ -33,26 +33,20 @@ struct Cnt {
 __device__ __forceinline__ uint32_t bitlen64(uint64_t n) { return n ? 64u - (uint32_t)__clzll(n) : 0u; }
 
 // engine/engine.hpp:224-243
-// Membership of x in the sorted list a[0..n); `probes` counts round trips to the list.
-// Three probe orders, one answer (the textbook search's, engine/engine.hpp:224-243):
-//   ORDER 0  bisection, one read per probe (the reference's search);
-//   ORDER 2  bisection until at most kScan entries remain, which are then read together
-//            (independent loads within one or two 64-byte lines: one round trip, one
-//            probe);
-//   ORDER 1  as ORDER 2, but the next probe is interpolated from the value bounds
-//            [vlo, vhi) of the remaining range (every id below id_end) -- a few probes when
-//            ids are spread evenly, as in randomly permuted graphs -- in exact fp32
-//            arithmetic (the oracle computes the same positions), with a bisection
-//            whenever a probe failed to halve the range, so no list takes more than about
-//            twice the bisection's probes.
-constexpr uint32_t kScan = 8;
-
-template <int ORDER>
+// Membership of x in the sorted list a[0..n), probes counting the list reads.  INTERP =
+// false bisects (the reference's search, engine/engine.hpp:224-243).  INTERP = true
+// (every id below id_end) interpolates the next probe from the value bounds [vlo, vhi) of
+// the remaining range -- a few probes when ids are spread evenly, as in randomly
+// permuted graphs -- in exact fp32 arithmetic (the oracle computes the same positions),
+// and bisects whenever a probe failed to halve the range and inside ranges of at most 16
+// entries (one 64-byte line), so no list takes more than about twice the bisection's
+// probes.  Both orders give the same answer.
+template <bool INTERP>
 __device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t n, uint32_t x, uint32_t& probes,
                                        uint32_t id_end)
 {
     int64_t lo = 0, hi = (int64_t)n - 1;
-    if (ORDER == 0) {
+    if (!INTERP) {
         while (lo <= hi) {
             const int64_t mid = lo + ((hi - lo) >> 1);
             const uint32_t m = a[mid];
@@ -62,31 +56,23 @@ __device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t 
         }
         return false;
     }
-    uint32_t vlo = 0, vhi = ORDER == 1 ? id_end : 0u;   // a[lo..hi] within [vlo, vhi)
-    bool bisect = ORDER != 1;
-    if (ORDER == 1 && x >= vhi) return false;
+    uint32_t vlo = 0, vhi = id_end;   // a[lo..hi] within [vlo, vhi)
+    bool bisect = false;
+    if (x >= vhi) return false;
     while (lo <= hi) {
         const uint32_t len = (uint32_t)(hi - lo + 1);
-        if (len <= kScan) {
-            ++probes;
-            bool f = false;
-#pragma unroll
-            for (uint32_t i = 0; i < kScan; ++i)
-                if (i < len) f = f || a[lo + i] == x;
-            return f;
-        }
         int64_t mid;
-        if (bisect || vhi - vlo < 2) {
+        if (bisect || len <= 16 || vhi - vlo < 2) {
             mid = lo + ((hi - lo) >> 1);
         } else {
-            const uint32_t off = (uint32_t)((float)(x - vlo) * (float)len / (float)(vhi - vlo));
+            uint32_t off = (uint32_t)((float)(x - vlo) * (float)len / (float)(vhi - vlo));
             mid = lo + (off < len ? off : len - 1);
         }
         const uint32_t m = a[mid];
         ++probes;
         if (m == x) return true;
         if (m < x) { lo = mid + 1; vlo = m + 1; } else { hi = mid - 1; vhi = m; }
-        if (ORDER == 1) bisect = (uint64_t)(hi - lo + 1) * 2 > len;
+        bisect = (uint64_t)(hi - lo + 1) * 2 > len;
     }
     return false;
 }
@@ -153,7 +139,7 @@ __device__ __forceinline__ bool sopr_use_mixture(uint64_t du, uint64_t dv, uint3
     return du * (4ull * bitlen64(dv) + 2) < e * (1 + bitlen64(du));
 }
 
-template <int ORDER>
+template <bool INTERP>
 __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64_t dv,
                                          const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
                                          const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
@@ -169,13 +155,13 @@ __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64
         if (ra < tp.t_min) return true;
         uint64_t t;
         if (x == prev) t = tp.t_p;
-        else t = member<ORDER>(nu, du, x, c.probes, id_end) ? tp.t_one : tp.t_q;
+        else t = member<INTERP>(nu, du, x, c.probes, id_end) ? tp.t_one : tp.t_q;
         if (ra < t) return true;
     }
     return false;
 }
 
-template <int ORDER>
+template <bool INTERP>
 __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint64_t dv,
                                           const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
                                           const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
@@ -193,7 +179,7 @@ __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint6
                 return true;
             }
             const uint32_t y = nu[((uint64_t)d.y * du) >> 32];
-            if (y != prev && member<ORDER>(nv, dv, y, c.probes, id_end)) {
+            if (y != prev && member<INTERP>(nv, dv, y, c.probes, id_end)) {
                 x = y;
                 return true;
             }
@@ -224,7 +210,7 @@ __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint6
         ++c.trials;
         x = nv[((uint64_t)rp * dv) >> 32];
         if ((u128)ra * den < num32) return true;
-        if (x != prev && member<ORDER>(nu, du, x, c.probes, id_end)) return true;
+        if (x != prev && member<INTERP>(nu, du, x, c.probes, id_end)) return true;
     }
     return false;
 }
@@ -312,7 +298,7 @@ __device__ __forceinline__ uint32_t wave_take(WaveQueue& q, uint64_t mask, uint3
 // per loop iteration; a lane whose walker ends or leaves the resident set takes the
 // next one from the work queue (one atomic per wavefront), so a wavefront never waits
 // for its longest walker -- only for its slowest step.
-template <bool ONE, int ORDER>
+template <bool ONE, bool INTERP>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_kernel(WalkArgs a)
 {
     // block-access trace: steps per current block, histogrammed in LDS
@@ -381,9 +367,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
             uint32_t x = 0;
             const uint32_t t0 = c.trials, p0 = c.probes;
             const bool ok = tp.algo == 0
-                                ? n2v_step<ORDER>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
+                                ? n2v_step<INTERP>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
                                                    a.search_end)
-                                : sopr_step<ORDER>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
+                                : sopr_step<INTERP>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
                                                     a.search_end);
             if (a.zc_count) {
                 // candidates come from cur's row, membership probes from prev's row
@@ -1068,15 +1054,11 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
         if (one_block) hipLaunchKernelGGL(walk_flat_kernel<true>, dim3(grid), dim3(block), 0, s, a);
         else hipLaunchKernelGGL(walk_flat_kernel<false>, dim3(grid), dim3(block), 0, s, a);
     } else {
-        const dim3 g(grid), b(block);
-        switch ((one_block ? 3 : 0) + (a.search_order < 3 ? a.search_order : 0)) {
-        case 0: hipLaunchKernelGGL((walk_kernel<false, 0>), g, b, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((walk_kernel<false, 1>), g, b, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((walk_kernel<false, 2>), g, b, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((walk_kernel<true, 0>), g, b, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((walk_kernel<true, 1>), g, b, 0, s, a); break;
-        default: hipLaunchKernelGGL((walk_kernel<true, 2>), g, b, 0, s, a); break;
-        }
+        const bool in = a.search_end != 0;
+        if (one_block && !in) hipLaunchKernelGGL((walk_kernel<true, false>), dim3(grid), dim3(block), 0, s, a);
+        else if (one_block) hipLaunchKernelGGL((walk_kernel<true, true>), dim3(grid), dim3(block), 0, s, a);
+        else if (!in) hipLaunchKernelGGL((walk_kernel<false, false>), dim3(grid), dim3(block), 0, s, a);
+        else hipLaunchKernelGGL((walk_kernel<false, true>), dim3(grid), dim3(block), 0, s, a);
     }
     return hipGetLastError();
 }
@@ -1089,8 +1071,8 @@ uint32_t walk_blocks_per_cu(bool one_block, uint32_t variant, uint32_t block)
         e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<true>, (int)block, 0)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<false>, (int)block, 0);
     else
-        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true, 0>, (int)block, 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, 0>, (int)block, 0);
+        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true, false>, (int)block, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, false>, (int)block, 0);
     return (e == hipSuccess && n > 0) ? (uint32_t)n : 1u;
 }
 

@@ -9,7 +9,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import (GW_NODE2VEC, GW_OUT_HASH, GW_OUT_PATHS, GW_OUT_VISITS, GW_RUN_BUCKETS, GW_RUN_INTERP, GW_RUN_SCAN, GW_SOPR, GwBlockDesc,
+from ._abi import (GW_NODE2VEC, GW_OUT_HASH, GW_OUT_PATHS, GW_OUT_VISITS, GW_RUN_BUCKETS, GW_RUN_INTERP, GW_SOPR, GwBlockDesc,
                    GwConfig, GwCounters, GwDeviceInfo, GwLaunch, GwRunStats, GwWalkParams, check, lib)
 
 NODE2VEC, SOPR = GW_NODE2VEC, GW_SOPR
@@ -168,9 +168,8 @@ class Context:
         self.nwalkers = n
         self.outputs = outputs
 
-    def run(self, block=0, waves_per_cu=0, variant=0, buckets=False, interp=False, scan=False):
-        L = GwLaunch(block, waves_per_cu, variant, (RUN_BUCKETS if buckets else 0) | (GW_RUN_INTERP if interp else 0) |
-                     (GW_RUN_SCAN if scan else 0))
+    def run(self, block=0, waves_per_cu=0, variant=0, buckets=False, interp=False):
+        L = GwLaunch(block, waves_per_cu, variant, (RUN_BUCKETS if buckets else 0) | (GW_RUN_INTERP if interp else 0))
         s = GwRunStats()
         self._chk(lib().gw_run(self.h, C.byref(L), C.byref(s)))
         return {f: getattr(s, f) for f, _ in GwRunStats._fields_}
@@ -182,7 +181,7 @@ class Context:
         ms = C.c_double()
         self._chk(lib().gw_autotune(self.h, sample, what, C.byref(best), C.byref(ms)))
         return dict(block=best.block, waves_per_cu=best.waves_per_cu, variant=best.variant,
-                    interp=bool(best.flags & GW_RUN_INTERP), scan=bool(best.flags & GW_RUN_SCAN)), ms.value
+                    interp=bool(best.flags & GW_RUN_INTERP)), ms.value
 
     def bucket_counts(self):
         nb = self.blocks.nblocks
@@ -313,7 +312,7 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
     o.flags = flags
     if launch:
         o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 0),
-                            (GW_RUN_INTERP if launch.get("interp") else 0) | (GW_RUN_SCAN if launch.get("scan") else 0))
+                            GW_RUN_INTERP if launch.get("interp") else 0)
     wp = walk_params(types, seed)
     rep = GwhReport()
     h = np.zeros(nwalkers, dtype=np.uint64) if hashes else None
@@ -326,6 +325,5 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
         raise GwError(st, L.gwh_last_error().decode())
     out = {f: getattr(rep, f) for f, _ in GwhReport._fields_ if f not in ("pad", "tuned")}
     t = rep.tuned
-    out["tuned"] = dict(block=t.block, variant=t.variant, interp=bool(t.flags & GW_RUN_INTERP),
-                        scan=bool(t.flags & GW_RUN_SCAN)) if t.block else None
+    out["tuned"] = dict(block=t.block, variant=t.variant, interp=bool(t.flags & GW_RUN_INTERP)) if t.block else None
     return out, h, v

@@ -58,42 +58,15 @@ struct RunReport {
     gw_launch tuned{0, 0, 0, 0};    // launch the autotuner picked (block 0: none)
 };
 
-// Round trips a membership test makes in probe order 1 (interpolation) and 2
-// (bisection), both finishing with one read of the last <= 8 entries, as the step kernel
-// does.
-inline uint64_t member_round_trips(const uint32_t* a, int64_t n, uint32_t x, int order, uint32_t id_end)
-{
-    int64_t lo = 0, hi = n - 1;
-    uint32_t vlo = 0, vhi = order == 1 ? id_end : 0;
-    bool bisect = order != 1;
-    uint64_t trips = 0;
-    while (lo <= hi) {
-        const uint32_t len = (uint32_t)(hi - lo + 1);
-        if (len <= 8) return trips + 1;
-        int64_t mid;
-        if (bisect || vhi - vlo < 2) {
-            mid = lo + ((hi - lo) >> 1);
-        } else {
-            const uint32_t off = (uint32_t)((float)(x - vlo) * (float)len / (float)(vhi - vlo));
-            mid = lo + (off < len ? off : len - 1);
-        }
-        ++trips;
-        if (a[mid] == x) return trips;
-        if (a[mid] < x) { lo = mid + 1; vlo = a[mid] + 1; } else { hi = mid - 1; vhi = a[mid]; }
-        if (order == 1) bisect = (uint64_t)(hi - lo + 1) * 2 > len;
-    }
-    return trips;
-}
-
-// The membership probe order for a graph: over `q` queries x in N(u) -- u the source of
-// a random arc, x from the list of a random neighbour of u, the tests Node2vec makes --
-// interpolation (GW_RUN_INTERP) when it needs under 80% of the bisection's round trips,
-// else bisection (GW_RUN_SCAN); both finish with one read of the last <= 8 entries.
-inline uint32_t choose_search_order(const uint64_t* beg, const uint32_t* csr, uint64_t nv, uint32_t q = 4096)
+// Probes of a bisection and of the kernel's interpolation order (same estimate rule)
+// over `q` queries x in N(u), u the source of a random arc and x drawn from the list
+// of a random neighbour of u -- the membership tests Node2vec makes.  Interpolation
+// is preferred when it needs under 80% of the bisection's probes.
+inline bool prefer_interpolation(const uint64_t* beg, const uint32_t* csr, uint64_t nv, uint32_t q = 4096)
 {
     const uint64_t ne = nv ? beg[nv] - beg[0] : 0;
-    if (!ne || nv > 0xFFFFFFFFull) return GW_RUN_SCAN;
-    uint64_t z = 0x9E3779B97F4A7C15ULL, t1 = 0, t2 = 0;
+    if (!ne || nv > 0xFFFFFFFFull) return false;
+    uint64_t z = 0x9E3779B97F4A7C15ULL, pb = 0, pi = 0;
     auto rnd = [&](uint64_t n) {
         z += 0x9E3779B97F4A7C15ULL;
         uint64_t x = z;
@@ -113,16 +86,41 @@ inline uint32_t choose_search_order(const uint64_t* beg, const uint32_t* csr, ui
         const uint64_t e = beg[0] + rnd(ne);
         const uint64_t u = src(e);
         const uint32_t v = csr[e];
-        if (v >= nv) return GW_RUN_SCAN;
+        if (v >= nv) return false;
         const uint64_t dv = beg[v + 1] - beg[v];
         if (!dv) continue;
         const uint32_t x = csr[beg[v] + rnd(dv)];
+        const uint32_t* a = csr + beg[u];
         const int64_t n = (int64_t)(beg[u + 1] - beg[u]);
-        t1 += member_round_trips(csr + beg[u], n, x, 1, (uint32_t)nv);
-        t2 += member_round_trips(csr + beg[u], n, x, 2, (uint32_t)nv);
+        int64_t lo = 0, hi = n - 1;
+        while (lo <= hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            ++pb;
+            if (a[mid] == x) break;
+            if (a[mid] < x) lo = mid + 1; else hi = mid - 1;
+        }
+        lo = 0;
+        hi = n - 1;
+        uint32_t vlo = 0, vhi = (uint32_t)nv;
+        bool bisect = false;
+        while (lo <= hi) {
+            const uint32_t len = (uint32_t)(hi - lo + 1);
+            int64_t mid;
+            if (bisect || len <= 16 || vhi - vlo < 2) {
+                mid = lo + ((hi - lo) >> 1);
+            } else {
+                const uint32_t off = (uint32_t)((float)(x - vlo) * (float)len / (float)(vhi - vlo));
+                mid = lo + (off < len ? off : len - 1);
+            }
+            ++pi;
+            if (a[mid] == x) break;
+            if (a[mid] < x) { lo = mid + 1; vlo = a[mid] + 1; } else { hi = mid - 1; vhi = a[mid]; }
+            bisect = (uint64_t)(hi - lo + 1) * 2 > len;
+        }
     }
-    return t2 && t1 * 5 < t2 * 4 ? (uint32_t)GW_RUN_INTERP : (uint32_t)GW_RUN_SCAN;
+    return pb && pi * 5 < pb * 4;
 }
+
 
 class Engine {
 public:
@@ -188,7 +186,7 @@ public:
             // both orders need on sampled Node2vec-style queries of this graph (timing on the
             // device would need a resident cache that is not there yet)
             auto tt = clk::now();
-            base = gw_launch{256, 0, 0, choose_search_order(beg_, csr_, total_verts())};
+            base = gw_launch{256, 0, 0, prefer_interpolation(beg_, csr_, total_verts()) ? (uint32_t)GW_RUN_INTERP : 0u};
             r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();
             r.tuned = base;
         }

@@ -99,11 +99,9 @@ enum {
     GW_RUN_BUCKETS = 1u << 0, /* after the run, recount the (prev_blk, cur_blk) bucket matrix of
                                  the walkers left, so gw_bucket_counts needs no extra pass */
     GW_RUN_TRACE = 1u << 1,   /* record steps per current block (gw_trace_read) */
-    GW_RUN_INTERP = 1u << 2,  /* membership tests by guarded interpolation search, finishing
-                                 with one read of the last <= 16 entries: same walks, fewer
-                                 round trips on evenly spread ids (randomly permuted graphs) */
-    GW_RUN_SCAN = 1u << 3     /* membership tests by bisection finishing with one read of the
-                                 last <= 16 entries (the two flags: step kernel variant 0) */
+    GW_RUN_INTERP = 1u << 2   /* membership tests by guarded interpolation search instead of
+                                 bisection: same walks, fewer reads on evenly spread ids
+                                 (randomly permuted graphs); step kernel variant 0 only */
 };
 
 typedef struct {

@@ -197,33 +197,22 @@ static int member(const uint32_t* a, uint64_t n, uint32_t x, uint64_t* probes)
     return 0;
 }
 
-/* The same membership test with other probe orders (not in the reference; the answer
- * equals member()'s for every sorted list, so walks do not change; probes count round
- * trips to the list, as the kernels count them):
- *   order 2: bisection until at most ORC_SCAN entries remain, which are read together
- *            (one probe);
- *   order 1: as order 2, with the next probe interpolated from the value bounds
- *            [vlo, vhi) of the remaining range in IEEE single precision (the kernel
- *            computes the same positions), and a bisection whenever a probe failed to
- *            halve the range. */
-#define ORC_SCAN 8
-static int member_ordered(const uint32_t* a, uint64_t n, uint32_t x, uint32_t order, uint32_t id_end,
-                          uint64_t* probes)
+/* The same membership test with a different probe order (not in the reference; its
+ * answer equals member()'s for every sorted list, so walks do not change): the next probe
+ * is interpolated from the value bounds [vlo, vhi) of the remaining range in IEEE single
+ * precision (the kernel computes the same positions), which lands near x in a few probes
+ * when ids are spread evenly (randomly permuted graphs); when a probe fails to halve the
+ * range the next one bisects, and ranges of at most 16 entries are bisected. */
+static int member_interp(const uint32_t* a, uint64_t n, uint32_t x, uint32_t id_end, uint64_t* probes)
 {
     int64_t lo = 0, hi = (int64_t)n - 1;
-    uint32_t vlo = 0, vhi = order == 1 ? id_end : 0;
-    int bisect = order != 1;
-    if (order == 1 && x >= vhi) return 0;
+    uint32_t vlo = 0, vhi = id_end;
+    int bisect = 0;
+    if (x >= vhi) return 0;
     while (lo <= hi) {
         const uint32_t len = (uint32_t)(hi - lo + 1);
-        if (len <= ORC_SCAN) {
-            (*probes)++;
-            for (uint32_t i = 0; i < len; i++)
-                if (a[lo + i] == x) return 1;
-            return 0;
-        }
         int64_t mid;
-        if (bisect || vhi - vlo < 2) {
+        if (bisect || len <= 16 || vhi - vlo < 2) {
             mid = lo + ((hi - lo) >> 1);
         } else {
             volatile float num = (float)(x - vlo) * (float)len;   /* no excess precision */
@@ -234,14 +223,14 @@ static int member_ordered(const uint32_t* a, uint64_t n, uint32_t x, uint32_t or
         (*probes)++;
         if (m == x) return 1;
         if (m < x) { lo = mid + 1; vlo = m + 1; } else { hi = mid - 1; vhi = m; }
-        if (order == 1) bisect = (uint64_t)(hi - lo + 1) * 2 > len;
+        bisect = (uint64_t)(hi - lo + 1) * 2 > len;
     }
     return 0;
 }
 
 int orc_member(const uint32_t* a, uint64_t n, uint32_t x, uint32_t search, uint32_t id_end, uint64_t* probes)
 {
-    return search ? member_ordered(a, n, x, search, id_end, probes) : member(a, n, x, probes);
+    return search ? member_interp(a, n, x, id_end, probes) : member(a, n, x, probes);
 }
 
 static uint64_t lower_bound(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t x, uint64_t* probes)

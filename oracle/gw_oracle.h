@@ -63,9 +63,8 @@ typedef struct {
     uint32_t alpha_a, alpha_b;        /* SOPR fixed-point alpha: A = round(alpha*2^16), B = 2^16-A */
     uint64_t t_term;        /* geometric termination threshold floor(term*2^32); 0 = none */
     uint32_t sopr_sampler;  /* 0: proposal (duplicate-free lists), 1: envelope/mixture (any lists) */
-    uint32_t search;        /* membership probe order: 0 bisection (the reference's), 1 guarded
-                               interpolation + final scan, 2 bisection + final scan; same
-                               answers, different probe counts */
+    uint32_t search;        /* membership search order: 0 bisection (the reference's), 1 guarded
+                               interpolation; same answers, different probe counts */
 } orc_typecfg;
 
 typedef struct {

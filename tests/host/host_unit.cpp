@@ -151,14 +151,8 @@ static void test_search_order_model()
         std::sort(skew.begin() + v * d, skew.begin() + (v + 1) * d);
     }
     beg[n] = n * d;
-    CHECK(choose_search_order(beg.data(), even.data(), n) == GW_RUN_INTERP);
-    CHECK(choose_search_order(beg.data(), skew.data(), n) == GW_RUN_SCAN);
-    // round trips: a 1000-entry list, bisection to <= 8 entries then one read
-    std::vector<uint32_t> a(1000);
-    for (uint32_t i = 0; i < 1000; ++i) a[i] = 3 * i;
-    uint64_t t = member_round_trips(a.data(), 1000, 3 * 517, 2, 3000);
-    CHECK(t >= 1 && t <= 8);
-    CHECK(member_round_trips(a.data(), 1000, 3 * 517, 1, 3000) <= 2);   // evenly spread: direct hit
+    CHECK(prefer_interpolation(beg.data(), even.data(), n));
+    CHECK(!prefer_interpolation(beg.data(), skew.data(), n));
 }
 
 int main(int argc, char** argv)
