@@ -142,6 +142,14 @@ inline int app_main(int argc, const char** argv, App app)
         return 1;
     }
     try {
+        // preprocess/graph_converter.hpp:286-320: convert the text edge list unless its
+        // .beg/.csr/.meta exist (existing files are kept, as with `skip`; there is no
+        // interactive re-convert prompt)
+        if (!file_exists(beg_name(base)) || !file_exists(csr_name(base)) || !file_exists(meta_name(base))) {
+            if (input == base || !file_exists(input)) throw IoError("no preprocessed graph and no edge list " + input);
+            const Meta cm = convert_text(input);
+            if (verbose) std::printf("converted %s: %u vertices, %llu edges\n", input.c_str(), cm.nvertices, (unsigned long long)cm.nedges);
+        }
         Meta meta = read_meta(base);
         HostGraph g;
         g.load(base);

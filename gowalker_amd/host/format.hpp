@@ -15,7 +15,10 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cerrno>
+#include <cstdio>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -171,6 +174,78 @@ inline std::vector<gw_block_desc> make_block_table(const std::vector<uint32_t>& 
         if (d.nverts == 1 && d.nedges > maxedges) d.nedges = maxedges;
     }
     return t;
+}
+
+// Text edge list -> .beg/.csr/.meta with the reference converter's rules
+// (preprocess/graph_converter.hpp:286-384, :193-262): lines starting with '#' or '%' are
+// comments; fields are split on tabs, commas and spaces; ids below the first line's
+// source get empty lists (dealfirstline); self-loops are dropped; sources must not
+// decrease (the reference asserts; here an IoError names the line); duplicates are kept;
+// every id up to the largest one seen gets a (possibly empty) list, so .beg has
+// max_id + 2 entries; .meta holds max - min + 1, the edge count and the smallest id.
+// Lists are sorted, because the membership search needs sorted lists (the reference
+// sorts only with its `sorted` option, :444-447; its EDGE_SIZE of 0, api/constants.hpp:5,
+// makes the original stop before writing anything).
+inline Meta convert_text(const std::string& txt)
+{
+    FILE* f = std::fopen(txt.c_str(), "r");
+    if (!f) throw IoError("cannot open " + txt);
+    std::vector<uint64_t> beg{0};
+    std::vector<uint32_t> csr;
+    uint64_t cur = 0, lineno = 0, start = 0;
+    uint32_t maxv = 0, minv = 0xFFFFFFFFu;
+    bool first = true;
+    char line[1024];
+    auto close_list = [&]() {   // sort the current source's list and record its end
+        std::sort(csr.begin() + (ptrdiff_t)start, csr.end());
+        beg.push_back(csr.size());
+        start = csr.size();
+    };
+    while (std::fgets(line, sizeof line, f)) {
+        ++lineno;
+        if (line[0] == '#' || line[0] == '%') continue;
+        char* save = nullptr;
+        char* t1 = strtok_r(line, "\t, \r\n", &save);
+        char* t2 = t1 ? strtok_r(nullptr, "\t, \r\n", &save) : nullptr;
+        if (!t1 || !t2) {
+            std::fclose(f);
+            throw IoError(txt + ":" + std::to_string(lineno) + ": expected <from> <to>");
+        }
+        const uint32_t from = (uint32_t)std::strtoul(t1, nullptr, 10), to = (uint32_t)std::strtoul(t2, nullptr, 10);
+        if (first) {   // ids below the first source have empty lists
+            for (uint32_t v = 0; v < from; ++v) beg.push_back(0);
+            cur = from;
+            first = false;
+        }
+        if (from == to) continue;
+        if (from < cur) {
+            std::fclose(f);
+            throw IoError(txt + ":" + std::to_string(lineno) + ": sources must not decrease (sort the edge list)");
+        }
+        while (cur < from) {   // finish the current list and any empty ones in between
+            close_list();
+            ++cur;
+        }
+        csr.push_back(to);
+        maxv = std::max(maxv, std::max(from, to));
+        minv = std::min(minv, std::min(from, to));
+    }
+    std::fclose(f);
+    if (csr.empty()) throw IoError(txt + ": no edges");
+    close_list();
+    while (cur < maxv) {
+        ++cur;
+        beg.push_back(csr.size());
+    }
+    const std::string base = remove_extension(txt);
+    write_all(beg_name(base), beg.data(), beg.size() * 8);
+    write_all(csr_name(base), csr.data(), csr.size() * 4);
+    Meta m;
+    m.nvertices = maxv - minv + 1;
+    m.nedges = csr.size();
+    m.min_vert = minv;
+    write_meta(base, m);
+    return m;
 }
 
 // split_blocks over an in-memory .beg and write both block files.

@@ -1,6 +1,7 @@
 // Unit tests of the host layer that need no GPU: option parsing (api/cmdopts.hpp
-// semantics), the on-disk format (names, meta, block files, truncation), the plain
-// scheduler against the oracle's known answers, and slot placement.
+// semantics), the on-disk format (names, meta, block files, truncation), the text edge
+// list converter, the plain scheduler against the oracle's known answers, and slot
+// placement.
 #include <algorithm>
 #include <cassert>
 #include <cstdio>
@@ -93,6 +94,77 @@ static void test_format(const std::string& dir)
     CHECK(threw);
 }
 
+// preprocess/graph_converter.hpp:186-384 rules, worked by hand: comments skipped, ids
+// below the first source padded, self-loops dropped (the first line still pads), gaps
+// filled, lists sorted, duplicates kept, the largest target padded at the end
+static void test_convert(const std::string& dir)
+{
+    const std::string txt = dir + "/conv.txt";
+    FILE* f = std::fopen(txt.c_str(), "w");
+    std::fprintf(f, "# header\n%% other\n2 2\n2 5\n2,3\n4\t1\n4 1\n4 0\n4 7\n");
+    std::fclose(f);
+    Meta m = convert_text(txt);
+    CHECK(m.nvertices == 8 && m.nedges == 6 && m.min_vert == 0);
+    auto beg = read_vector<uint64_t>(dir + "/conv.beg");
+    auto csr = read_vector<uint32_t>(dir + "/conv.csr");
+    CHECK((beg == std::vector<uint64_t>{0, 0, 0, 2, 2, 6, 6, 6, 6}));
+    CHECK((csr == std::vector<uint32_t>{3, 5, 0, 1, 1, 7}));
+    Meta r = read_meta(dir + "/conv");
+    CHECK(r.nvertices == 8 && r.nedges == 6 && r.min_vert == 0);
+    // the smallest id counts only edges that are kept
+    f = std::fopen(txt.c_str(), "w");
+    std::fprintf(f, "1 1\n3 4\n3 5\n");
+    std::fclose(f);
+    m = convert_text(txt);
+    CHECK(m.nvertices == 3 && m.nedges == 2 && m.min_vert == 3);
+    CHECK((read_vector<uint64_t>(dir + "/conv.beg") == std::vector<uint64_t>{0, 0, 0, 0, 2, 2, 2}));
+    // a decreasing source is an error that names the line
+    f = std::fopen(txt.c_str(), "w");
+    std::fprintf(f, "3 1\n2 1\n");
+    std::fclose(f);
+    bool threw = false;
+    try {
+        convert_text(txt);
+    } catch (const IoError& e) {
+        threw = std::string(e.what()).find(":2:") != std::string::npos;
+    }
+    CHECK(threw);
+    // round trip: a random graph with empty lists and duplicate targets, written as text
+    // with each list reversed and self-loops added, converts back to the same .beg/.csr
+    uint64_t z = 99;
+    auto rnd = [&]() { z = z * 6364136223846793005ULL + 1442695040888963407ULL; return (uint32_t)(z >> 33); };
+    const uint32_t n = 3000;
+    std::vector<uint64_t> b{0};
+    std::vector<uint32_t> c;
+    for (uint32_t v = 0; v < n; ++v) {
+        const uint32_t d = rnd() % 4 == 0 ? 0 : rnd() % 40;
+        std::vector<uint32_t> l;
+        for (uint32_t k = 0; k < d; ++k) {
+            uint32_t t = rnd() % n;
+            if (t == v) t = (t + 1) % n;
+            l.push_back(t);
+        }
+        std::sort(l.begin(), l.end());
+        c.insert(c.end(), l.begin(), l.end());
+        b.push_back(c.size());
+    }
+    f = std::fopen(txt.c_str(), "w");
+    for (uint32_t v = 0; v < n; ++v) {
+        if (b[v + 1] > b[v]) std::fprintf(f, "%u %u\n", v, v);
+        for (uint64_t e = b[v + 1]; e > b[v]; --e) std::fprintf(f, "%u\t%u\n", v, c[e - 1]);
+    }
+    std::fclose(f);
+    m = convert_text(txt);
+    auto rb = read_vector<uint64_t>(dir + "/conv.beg");
+    auto rc = read_vector<uint32_t>(dir + "/conv.csr");
+    CHECK(rc == c && m.nedges == c.size());
+    // ids past the largest one seen (as source or target) have no entry
+    uint32_t top = 0;
+    for (uint32_t v = 0; v < n; ++v)
+        if (b[v + 1] > b[v]) top = std::max(top, std::max(v, *std::max_element(c.begin() + b[v], c.begin() + b[v + 1])));
+    CHECK(rb.size() == (size_t)top + 2 && std::equal(rb.begin(), rb.end(), b.begin()));
+}
+
 static void test_scheduler()
 {
     // same known answers as tests/test_oracle_format.py
@@ -161,6 +233,7 @@ int main(int argc, char** argv)
     test_options(dir);
     test_names();
     test_format(dir);
+    test_convert(dir);
     test_scheduler();
     test_search_order_model();
     std::printf("host_unit: all passed\n");

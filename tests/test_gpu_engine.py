@@ -132,6 +132,27 @@ def test_apps_end_to_end(gpu, tmp_path):
     np.testing.assert_array_equal(np.fromfile(hf, dtype=np.uint64), o["hashes"])
     m = open(str(tmp_path / "m.txt")).read()
     assert "0_Total_time=" in m and "walk_steps_per_s=" in m
+    # a text edge list is converted on first use (preprocess/graph_converter.hpp:286-384):
+    # each list written in reverse with a comment line and self-loops, it converts back to
+    # the generator's .beg/.csr and the walk is unchanged
+    deg = np.diff(beg.astype(np.int64))
+    src = np.repeat(np.arange(len(deg), dtype=np.int64), deg)
+    order = np.lexsort((-np.arange(len(csr)), src))
+    lines = ["# text copy"] + [f"{a} {b}" for a, b in zip(src[order].tolist(), csr[order].tolist())]
+    lines.insert(1, f"{int(src[0])} {int(src[0])}")
+    (tmp_path / "t.txt").write_text("\n".join(lines) + "\n")
+    ht = str(tmp_path / "ht.bin")
+    r3 = subprocess.run([os.path.join(build, "Node2vec"), str(tmp_path / "t.txt"), "length", "15", "walkpersource", "2",
+                         "memory", "1", "blocksize", "1", "p", "0.5", "q", "2", "gpu", "cpu", "GOWalker", "seed", "5",
+                         "hashes", ht], capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    print(r3.stdout[-2000:], r3.stderr[-2000:])
+    assert r3.returncode == 0
+    tb = np.fromfile(str(tmp_path / "t.beg"), dtype=np.uint64)
+    np.testing.assert_array_equal(np.fromfile(str(tmp_path / "t.csr"), dtype=np.uint32), csr)
+    np.testing.assert_array_equal(tb, beg[:len(tb)])
+    assert np.all(beg[len(tb):] == beg[-1])
+    if len(tb) == len(beg):
+        np.testing.assert_array_equal(np.fromfile(ht, dtype=np.uint64), o["hashes"])
     # the reference requires one scheduler flag
     r2 = subprocess.run([os.path.join(build, "Node2vec"), base + ".txt", "length", "5"], capture_output=True,
                         text=True, timeout=60, cwd=str(tmp_path))
