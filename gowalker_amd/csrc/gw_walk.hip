@@ -12,7 +12,9 @@
 //   * pos = (u32 * deg) >> 32.
 //   * Node2vec (gpuwalk.hpp:118-150): accept iff accept-word < T(class), classes
 //     x == prev (1/p), x in N(prev) (1), other (1/q), T = floor(w/max * 2^32); the
-//     smallest T is tested before the candidate is classified.
+//     smallest T is tested before the candidate is classified, and prev's list is
+//     searched only when the accept word lies between T(1) and T(1/q), the one band in
+//     which the two classes x != prev can fall into decide differently.
 //   * SOPR (gpuwalk.hpp:241-266): P(x) ~ (1-a)/dv + a/du [x in N(prev), x != prev] with
 //     a = A/2^16, sampled by envelope rejection or, when the envelope is loose, by an
 //     exact two-component mixture after counting |N(cur) & N(prev)| (see sopr_step).
@@ -145,6 +147,8 @@ __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64
                                          const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
                                          uint32_t& x, Cnt& c, uint32_t id_end)
 {
+    const uint64_t band_lo = tp.t_one < tp.t_q ? tp.t_one : tp.t_q;
+    const uint64_t band_hi = tp.t_one < tp.t_q ? tp.t_q : tp.t_one;
     uint4 d = make_uint4(0, 0, 0, 0);
     for (uint32_t k = 0; k < kMaxTrials; ++k) {
         if (!(k & 1)) d = philox(w, hop, k >> 1, kTagStep, k0, k1);
@@ -155,6 +159,8 @@ __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64
         if (ra < tp.t_min) return true;
         uint64_t t;
         if (x == prev) t = tp.t_p;
+        else if (ra < band_lo) return true;      // accepted whether or not x is in N(prev)
+        else if (ra >= band_hi) continue;        // rejected either way
         else t = member<INTERP>(nu, du, x, c.probes, id_end) ? tp.t_one : tp.t_q;
         if (ra < t) return true;
     }
@@ -652,6 +658,9 @@ __device__ __forceinline__ void flat_consume(FlatLane& L, uint64_t v0, uint64_t 
                 L.phase = PH_TRIAL;
                 break;
             }
+            // outside the band between T(1) and T(1/q) membership cannot change the answer
+            if (L.ra < (tp.t_one < tp.t_q ? tp.t_one : tp.t_q)) { L.phase = PH_MOVE; break; }
+            if (L.ra >= (tp.t_one < tp.t_q ? tp.t_q : tp.t_one)) { ++L.k; L.phase = PH_TRIAL; break; }
         } else {
             const uint64_t A = tp.alpha_a, B = tp.alpha_b;
             if ((u128)L.ra * (B * L.du + A * L.dv) < ((u128)(B * L.du) << 32)) { L.phase = PH_MOVE; break; }

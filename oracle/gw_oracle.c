@@ -27,7 +27,10 @@
  *     weights 1/p (x == prev), 1 (x in N(prev)), 1/q (otherwise); envelope
  *     M = max(1/p, 1, 1/q); a class with weight w accepts iff acc < T(w) with
  *     T(w) = floor(w/M * 2^32) (T = 2^32 when w == M).  acc < min T accepts before the
- *     candidate is classified (the reference's rv <= min_val fast path).
+ *     candidate is classified (the reference's rv <= min_val fast path).  For x != prev,
+ *     prev's list is searched only when min(T(1), T(1/q)) <= acc < max(T(1), T(1/q)):
+ *     outside that band both classes give the same answer (the walk is unchanged; the
+ *     probe counter counts the searches made).
  *
  *   SOPR / SOSR (gpu/gpuwalk.hpp:241-266, engine/engine.hpp:323-376):
  *     P(x) proportional to (1-a)/dv + a/du * [x in N(prev), x != prev], a = A/2^16.
@@ -308,6 +311,9 @@ static void walk_impl(const orc_graph* g, const orc_walkcfg* cfg, uint32_t wid, 
         uint32_t x = 0;
         int ok = 0;
         if (tc->algo == ORC_N2V) {
+            /* prev's list decides only between the thresholds of the in-N(prev) and other classes */
+            const uint64_t band_lo = tc->t_one < tc->t_q ? tc->t_one : tc->t_q;
+            const uint64_t band_hi = tc->t_one < tc->t_q ? tc->t_q : tc->t_one;
             for (uint32_t k = 0; k < ORC_MAX_TRIALS; k++) {
                 if ((k & 1) == 0) draw(cfg->seed, wid, hop, k >> 1, 0, d);
                 uint32_t rp = (k & 1) ? d[2] : d[0];
@@ -317,6 +323,8 @@ static void walk_impl(const orc_graph* g, const orc_walkcfg* cfg, uint32_t wid, 
                 if (ra < tc->t_min) { ok = 1; break; }
                 uint64_t t;
                 if (x == prev) t = tc->t_p;
+                else if (ra < band_lo) { ok = 1; break; }   /* both x != prev classes accept */
+                else if (ra >= band_hi) continue;           /* both reject */
                 else if (orc_member(csr + u_head, du, x, tc->search, (uint32_t)g->nverts, &probes)) t = tc->t_one;
                 else t = tc->t_q;
                 if (ra < t) { ok = 1; break; }
