@@ -125,6 +125,7 @@ def cpu_baseline(beg, csr, cfg, seconds, threads):
                      % r1["done"])
 
 
+LDS_BLOCKS = 128          # block tables up to this size are staged in LDS by the step kernel
 PCIE_PEAK_GBS = 64.0   # PCIe Gen5 x16, one direction
 
 
@@ -162,7 +163,8 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                 first = (cap_gb, name) not in hashes
                 rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
                                               memory_bytes=mem, block_bytes=block_bytes, zero_threshold=zt,
-                                              scheduler=sched, hashes=first, flags=flags)
+                                              scheduler=sched, hashes=first, flags=flags,
+                                              launch=None if args.variant is None else dict(variant=args.variant))
                 if first:
                     hashes[(cap_gb, name)] = h
                 print("[bench] cap %g GB %s run %d: %.3f s, %d steps, %.2f GB H2D (%d uploads + %d staged), "
@@ -222,7 +224,10 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "walk_kernel<false, %s>" % ("true" if (r0.get("tuned") or {}).get("interp") else "false"),
+                     "kernel": "%s<false, %s, %s>" % ("walk_lane_kernel" if (r0.get("tuned") or {}).get("variant") == 2
+                                                       else "walk_kernel",
+                                                       "true" if (r0.get("tuned") or {}).get("interp") else "false",
+                                                       "true" if r0["nblocks"] <= LDS_BLOCKS else "false"),
                      "kernel_ms": r0["kernel_ms"] / launches, "algorithmic_bytes": abytes,
                      "sector_bytes": sector_bytes(r0, nwalkers, False) / launches,
                      "launches": launches,
@@ -247,7 +252,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--variant", type=int, default=0, help="0: per-step lane loop, 1: flattened state machine")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="step kernel: 0 per-step lane loop, 1 flattened state machine, 2 lean per-load lanes "
+                         "(default: the autotuner's pick for resident workloads, 2 out of core)")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
     ap.add_argument("--ablation", action="store_true", help="out-of-core: also run scheduler variants")
@@ -289,6 +296,8 @@ def main():
     nwalkers = nverts * cfg["wpv"]
     visits = cfg["types"][0]["algo"] == 1
     outputs = E.OUT_VISITS if visits else E.OUT_HASH
+    if args.variant is None:
+        args.variant = 0
     launch = dict(block=args.block, waves_per_cu=args.waves_per_cu, variant=args.variant)
 
     def one_step():
@@ -352,7 +361,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "walk_flat_kernel<true>" if args.variant == 1 else
-                               "walk_kernel<true, %s>" % ("true" if (tuned or {}).get("interp") else "false"),
+                               "%s<true, %s, false>" % ("walk_lane_kernel" if args.variant == 2 else "walk_kernel",
+                                                        "true" if (tuned or {}).get("interp") else "false"),
                      "kernel_ms": avg_kms, "algorithmic_bytes": abytes,
                      "sector_bytes": sector_bytes(st0, nwalkers, visits)},
         "cpu_baseline": None,

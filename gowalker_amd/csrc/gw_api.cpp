@@ -657,12 +657,20 @@ gw_status gw_set_walk(gw_ctx* c, const gw_walk_params* wp)
             d.t_one = thr(1.0, m);
             d.t_q = thr(iq, m);
             d.t_min = std::min(std::min(d.t_p, d.t_one), d.t_q);
+            if (d.t_min == 0) return fail(c, GW_ERR_ARG, "a Node2vec acceptance threshold rounds to zero");
+            d.m_p = (uint32_t)(d.t_p - 1);
+            d.m_one = (uint32_t)(d.t_one - 1);
+            d.m_q = (uint32_t)(d.t_q - 1);
+            d.m_min = (uint32_t)(d.t_min - 1);
+            d.m_lo = (uint32_t)(std::min(d.t_one, d.t_q) - 1);
+            d.m_hi = (uint32_t)(std::max(d.t_one, d.t_q) - 1);
         } else {
             if (!(ty.alpha >= 0.0 && ty.alpha < 1.0)) return fail(c, GW_ERR_ARG, "alpha must be in [0, 1)");
             const long long a = std::llround(ty.alpha * 65536.0);
             if (a >= 65536) return fail(c, GW_ERR_ARG, "alpha rounds to 1 in 16-bit fixed point");
             d.alpha_a = (uint32_t)a;
             d.alpha_b = 65536u - (uint32_t)a;
+            d.m_prop = (uint32_t)(((uint64_t)d.alpha_b << 16) - 1);
             if (ty.sopr_sampler > 1) return fail(c, GW_ERR_ARG, "unknown SOPR sampler %u", ty.sopr_sampler);
             d.sampler = ty.sopr_sampler;
         }
@@ -830,7 +838,7 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
 static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t cap)
 {
     if (l.block % 64 || l.block > 1024) return fail(c, GW_ERR_ARG, "block must be a multiple of 64 up to 1024");
-    if (l.variant > 1) return fail(c, GW_ERR_ARG, "unknown kernel variant %u", l.variant);
+    if (l.variant > 2) return fail(c, GW_ERR_ARG, "unknown kernel variant %u", l.variant);
     if (l.variant == 1 && l.block > 256) return fail(c, GW_ERR_ARG, "the flattened kernel runs at most 256 threads per workgroup");
 
     gw_status st;
@@ -895,11 +903,18 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     if (l.variant == 1)
         for (uint32_t b = 0; b < c->nblocks; ++b)
             if (c->hblk[b].resident == kMapped) { l.variant = 0; break; }
+    // the lane kernel covers Node2vec and the SOPR proposal sampler on 32-bit degrees
+    if (l.variant == 2) {
+        for (uint32_t t = 0; t < 2; ++t)
+            if (c->dt[t].algo == 1 && c->dt[t].sampler != 0) l.variant = 0;
+        for (uint32_t b = 0; b < c->nblocks; ++b)
+            if (c->hblk[b].nedges >> 32) l.variant = 0;
+    }
     // persistent grid: as many workgroups as the CUs hold at once (occupancy), or the
     // caller's waves per CU
     const bool one_blk = c->nblocks == 1;
     uint64_t grid = l.waves_per_cu ? (uint64_t)c->ncu * l.waves_per_cu * 64 / l.block
-                                   : (uint64_t)c->ncu * walk_blocks_per_cu(one_blk, l.variant, l.block);
+                                   : (uint64_t)c->ncu * walk_blocks_per_cu(one_blk, c->nblocks, l.variant, l.block);
     const uint64_t need = (nwork + l.block - 1) / l.block;   // an upper bound with a device-side count
     if (grid > need) grid = need;
     if (grid == 0) grid = 1;
@@ -989,12 +1004,13 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, uint32_t what, gw_launch* best
     static const gw_launch cand[] = {{64, 0, 0, 0},   {128, 0, 0, 0},  {256, 0, 0, 0},  {512, 0, 0, 0},
                                      {1024, 0, 0, 0}, {64, 0, 1, 0},   {128, 0, 1, 0},  {256, 0, 1, 0},
                                      {64, 0, 0, I},   {128, 0, 0, I},  {256, 0, 0, I},  {512, 0, 0, I},
-                                     {1024, 0, 0, I}};
+                                     {1024, 0, 0, I}, {64, 0, 2, 0},   {256, 0, 2, 0},  {1024, 0, 2, 0},
+                                     {64, 0, 2, I},   {256, 0, 2, I},  {1024, 0, 2, I}};
     double bms = 1e300;
     gw_launch bl = cand[2];
     for (const gw_launch& L : cand) {
         // candidates outside `what` are skipped; 256 threads of variant 0 always run
-        if (L.variant == 1 && !(what & GW_TUNE_VARIANT)) continue;
+        if (L.variant != 0 && !(what & GW_TUNE_VARIANT)) continue;
         if (L.variant == 0 && L.block != 256 && !(what & GW_TUNE_BLOCK)) continue;
         if ((L.flags & GW_RUN_INTERP) && !(what & GW_TUNE_SEARCH)) continue;
         double t = 1e300;

@@ -38,6 +38,10 @@ struct DevBlock {
 // Per walker type, in the integer form the step logic uses.
 struct DevType {
     uint64_t t_p, t_one, t_q, t_min;   // node2vec acceptance thresholds, each <= 2^32
+    // the same thresholds as T - 1 (ra < T <=> ra <= T - 1; every T is at least 1), the
+    // bounds of the band in which membership decides (min and max of t_one, t_q) and the
+    // SOPR proposal's N(cur) threshold (alpha_b << 16) - 1, for 32-bit compares
+    uint32_t m_p, m_one, m_q, m_min, m_lo, m_hi, m_prop;
     uint64_t t_term;                   // floor(term * 2^32), 0 = no geometric termination
     uint32_t algo;                     // 0 node2vec, 1 SOPR/SOSR
     uint32_t length;
@@ -119,10 +123,11 @@ hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipSt
 // enqueued; d2h returns with the data on the host.
 hipError_t h2d_small(gw_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 hipError_t d2h_small(gw_ctx* ctx, void* dst, const void* src, uint64_t bytes);
-uint32_t walk_blocks_per_cu(bool one_block, uint32_t variant, uint32_t block);   // resident workgroups per CU
+uint32_t walk_blocks_per_cu(bool one_block, uint32_t nblocks, uint32_t variant, uint32_t block);   // resident workgroups per CU
 hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age, float ahead,
                            float* feats, hipStream_t s);
 constexpr uint32_t kTraceLdsBlocks = 2048;   // per-workgroup LDS histogram size of the trace
+constexpr uint32_t kLdsBlocks = 128;         // block tables up to this size are staged in LDS by the step kernel
 constexpr uint32_t kNumFeatures = 12;
 constexpr uint32_t kFilterLdsBuckets = 12288;  // bucket matrices up to 110 x 110 blocks count in LDS
 

@@ -147,22 +147,20 @@ __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64
                                          const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
                                          uint32_t& x, Cnt& c, uint32_t id_end)
 {
-    const uint64_t band_lo = tp.t_one < tp.t_q ? tp.t_one : tp.t_q;
-    const uint64_t band_hi = tp.t_one < tp.t_q ? tp.t_q : tp.t_one;
     uint4 d = make_uint4(0, 0, 0, 0);
     for (uint32_t k = 0; k < kMaxTrials; ++k) {
         if (!(k & 1)) d = philox(w, hop, k >> 1, kTagStep, k0, k1);
         const uint32_t rp = (k & 1) ? d.z : d.x;
-        const uint64_t ra = (k & 1) ? d.w : d.y;
+        const uint32_t ra = (k & 1) ? d.w : d.y;   // accept iff ra < T, i.e. ra <= T - 1
         ++c.trials;
         x = nv[((uint64_t)rp * dv) >> 32];
-        if (ra < tp.t_min) return true;
-        uint64_t t;
-        if (x == prev) t = tp.t_p;
-        else if (ra < band_lo) return true;      // accepted whether or not x is in N(prev)
-        else if (ra >= band_hi) continue;        // rejected either way
-        else t = member<INTERP>(nu, du, x, c.probes, id_end) ? tp.t_one : tp.t_q;
-        if (ra < t) return true;
+        if (ra <= tp.m_min) return true;
+        uint32_t t;
+        if (x == prev) t = tp.m_p;
+        else if (ra <= tp.m_lo) return true;      // accepted whether or not x is in N(prev)
+        else if (ra > tp.m_hi) continue;          // rejected either way
+        else t = member<INTERP>(nu, du, x, c.probes, id_end) ? tp.m_one : tp.m_q;
+        if (ra <= t) return true;
     }
     return false;
 }
@@ -303,17 +301,28 @@ __device__ __forceinline__ uint32_t wave_take(WaveQueue& q, uint64_t mask, uint3
 // Persistent step kernel.  Each lane owns one walker at a time and takes one whole step
 // per loop iteration; a lane whose walker ends or leaves the resident set takes the
 // next one from the work queue (one atomic per wavefront), so a wavefront never waits
-// for its longest walker -- only for its slowest step.
-template <bool ONE, bool INTERP>
+// for its longest walker -- only for its slowest step.  ONE: a single resident block,
+// held in registers.  LB: a block table of at most kLdsBlocks entries, copied with its
+// cut points into LDS, so that the descriptor reads and the vertex->block search a step
+// across blocks makes stay on chip (the table does not change during a launch).
+template <bool ONE, bool INTERP, bool LB>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_kernel(WalkArgs a)
 {
     // block-access trace: steps per current block, histogrammed in LDS
-    __shared__ uint32_t trace[kTraceLdsBlocks];
-    const bool lds_trace = a.block_steps && a.nblocks <= kTraceLdsBlocks;
-    if (a.block_steps) {
-        for (uint32_t i = threadIdx.x; i < kTraceLdsBlocks; i += blockDim.x) trace[i] = 0;
-        __syncthreads();
+    constexpr uint32_t kTrace = ONE ? 1u : kTraceLdsBlocks;
+    __shared__ uint32_t trace[kTrace];
+    __shared__ DevBlock sblk[LB ? kLdsBlocks : 1];
+    __shared__ uint32_t svcut[LB ? kLdsBlocks + 1 : 1];
+    const bool lds_trace = a.block_steps && a.nblocks <= kTrace;
+    if (LB) {
+        for (uint32_t i = threadIdx.x; i < a.nblocks; i += blockDim.x) sblk[i] = a.blocks[i];
+        for (uint32_t i = threadIdx.x; i <= a.nblocks; i += blockDim.x) svcut[i] = a.vcut[i];
     }
+    if (a.block_steps)
+        for (uint32_t i = threadIdx.x; i < kTrace; i += blockDim.x) trace[i] = 0;
+    if (LB || a.block_steps) __syncthreads();
+    const DevBlock* blocks = LB ? sblk : a.blocks;
+    const uint32_t* vcut = LB ? svcut : a.vcut;
     Cnt c = {0, 0, 0, 0, 0, 0, 0};
     const uint32_t lane = __lane_id();
     DevBlock one;
@@ -337,7 +346,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
                 if (idx >= nwork) {
                     exhausted = true;
                 } else if (((m >> kStatusShift) & 0xFu) == kActive &&
-                           (!a.scan || runnable(a.blocks, a.bpair[a.worklist ? a.worklist[idx] : idx]))) {
+                           (!a.scan || runnable(blocks, a.bpair[a.worklist ? a.worklist[idx] : idx]))) {
                     w = a.worklist ? a.worklist[idx] : idx;
                     cur = a.cur[w];
                     prev = a.prev[w];
@@ -351,8 +360,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
                         rv = load_row(one, cur);
                         ru = load_row(one, prev);
                     } else {
-                        rv = load_row(a.blocks[bc], cur);
-                        ru = load_row(a.blocks[bp], prev);
+                        rv = load_row(blocks[bc], cur);
+                        ru = load_row(blocks[bp], prev);
                     }
                     have = true;
                 }
@@ -379,8 +388,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
                                                     a.search_end);
             if (a.zc_count) {
                 // candidates come from cur's row, membership probes from prev's row
-                const bool mv = (ONE ? one.resident : a.blocks[bc].resident) == kMapped;
-                const bool mu = (ONE ? one.resident : a.blocks[bp].resident) == kMapped;
+                const bool mv = (ONE ? one.resident : blocks[bc].resident) == kMapped;
+                const bool mu = (ONE ? one.resident : blocks[bp].resident) == kMapped;
                 c.zc += (mv ? 1u + (c.trials - t0) : 0u) + (mu ? c.probes - p0 : 0u);
             }
             if (!ok) {
@@ -404,9 +413,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
                 } else if (ONE) {
                     rv = load_row(one, cur);
                 } else {
-                    const DevBlock& cb = a.blocks[bc];
-                    if (cur - cb.start_vert >= cb.nverts) bc = find_block(a.vcut, a.nblocks, cur);
-                    const DevBlock& nb = a.blocks[bc];
+                    const DevBlock& cb = blocks[bc];
+                    if (cur - cb.start_vert >= cb.nverts) bc = find_block(vcut, a.nblocks, cur);
+                    const DevBlock& nb = blocks[bc];
                     if (nb.resident) rv = load_row(nb, cur);
                     else release = true;
                 }
@@ -800,10 +809,11 @@ __device__ __forceinline__ void flat_consume(FlatLane& L, uint64_t v0, uint64_t 
 template <bool ONE>
 __global__ void __launch_bounds__(256) walk_flat_kernel(WalkArgs a)
 {
-    __shared__ uint32_t trace[kTraceLdsBlocks];
-    const bool lds_trace = a.block_steps && a.nblocks <= kTraceLdsBlocks;
+    constexpr uint32_t kTrace = ONE ? 1u : kTraceLdsBlocks;
+    __shared__ uint32_t trace[kTrace];
+    const bool lds_trace = a.block_steps && a.nblocks <= kTrace;
     if (a.block_steps) {
-        for (uint32_t i = threadIdx.x; i < kTraceLdsBlocks; i += blockDim.x) trace[i] = 0;
+        for (uint32_t i = threadIdx.x; i < kTrace; i += blockDim.x) trace[i] = 0;
         __syncthreads();
     }
     Cnt c = {0, 0, 0, 0, 0, 0, 0};
@@ -872,6 +882,378 @@ __global__ void __launch_bounds__(256) walk_flat_kernel(WalkArgs a)
         atomicAdd(a.counters + 3, (unsigned long long)s3);
         atomicAdd(a.counters + 4, (unsigned long long)s4);
         atomicAdd(a.counters + 5, (unsigned long long)s5);
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Lane kernel (launch variant 2).  Node2vec and the SOPR proposal sampler restated as a
+// per-lane state machine: every loop iteration issues one round of loads for every busy
+// lane -- picking up a walker (work item, walker state, both adjacency rows) included --
+// then each lane consumes its value and runs compute-only states up to its next load.
+// A lane deep in a membership search therefore no longer keeps the other lanes of its
+// wavefront waiting, and a wavefront load carries a request for nearly every lane (the
+// per-step loop of walk_kernel averages about 8 of 64 on RMAT graphs).  The state is
+// 32-bit (degrees, search bounds, thresholds stored as T - 1), so the kernel keeps 8
+// waves per SIMD.  Draws, trials, probes and moves are those of n2v_step and sopr_step
+// (proposal sampler), so walks and counters are bit-identical to variant 0.  The host
+// uses it only when every walker type is Node2vec or the SOPR proposal sampler and no
+// block holds 2^32 or more edges.
+// ------------------------------------------------------------------------------------
+enum : uint32_t {
+    LS_IDLE = 0,
+    // load states (one round trip each)
+    LS_FETCH,   // worklist[idx] -> walker id
+    LS_STATE,   // meta, cur, prev, block pair (and hash) of the walker
+    LS_ROWS,    // beg words of cur's and prev's rows
+    LS_ROW,     // beg words of the new cur's row
+    LS_CAND,    // Node2vec candidate N(cur)[pos]
+    LS_PICK,    // SOPR: N(cur)[pos] is the move
+    LS_PROPU,   // SOPR: proposal y = N(prev)[pos]
+    LS_PROBE,   // membership probe
+    // compute states
+    LS_MOVE, LS_STEP, LS_TRIAL, LS_END
+};
+
+constexpr uint32_t kLaneBisect = 1u, kLaneMv = 2u, kLaneMu = 4u;   // Lane::fl bits
+
+struct Lane {
+    uint32_t ph, w, cur, prev;
+    uint32_t meta;                  // the stored word: hop | status << 20 | type << 24
+    uint32_t bcp;                   // cur block | prev block << 16
+    uint32_t k, x, ra, dz, dw;      // trial, candidate or search target, accept word, odd trial's draws
+    uint32_t lo, hx, vlo, vhi, fl;  // search range [lo, hx), its value bounds, flags
+    uint32_t dv, du;
+    uint64_t hash;
+    const uint32_t* nv;
+    const uint32_t* nu;             // between LS_STATE and LS_ROWS: prev's beg words
+    const void* addr;               // the pending load
+};
+
+__device__ __forceinline__ void lane_row(const DevBlock& b, uint64_t v0, uint64_t v1, const uint32_t*& adj,
+                                         uint32_t& deg)
+{
+    uint64_t t = v1 - b.start_edge, h = v0 - b.start_edge;
+    if (t > b.nedges) t = b.nedges;
+    if (h > t) h = t;
+    adj = b.csr + h;
+    deg = (uint32_t)(t - h);
+}
+
+// next probe of the search for L.x in [L.lo, L.hx): member's positions
+template <bool INTERP>
+__device__ __forceinline__ uint32_t lane_mid(const Lane& L)
+{
+    const uint32_t len = L.hx - L.lo;
+    if (!INTERP || (L.fl & kLaneBisect) || len <= 16 || L.vhi - L.vlo < 2) return L.lo + ((len - 1) >> 1);
+    const uint32_t off = (uint32_t)((float)(L.x - L.vlo) * (float)len / (float)(L.vhi - L.vlo));
+    return L.lo + (off < len ? off : len - 1);
+}
+
+// starts the membership search of L.x in list[0, n); false = decided (absent) without a probe
+template <bool INTERP>
+__device__ __forceinline__ bool lane_search(Lane& L, const uint32_t* list, uint32_t n, uint32_t id_end)
+{
+    if (n == 0 || (INTERP && L.x >= id_end)) return false;
+    L.lo = 0;
+    L.hx = n;
+    L.vlo = 0;
+    L.vhi = id_end;
+    L.fl &= ~kLaneBisect;
+    L.addr = list + lane_mid<INTERP>(L);
+    return true;
+}
+
+template <bool ONE, bool INTERP, bool LB>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_lane_kernel(WalkArgs a)
+{
+    constexpr uint32_t kTrace = ONE ? 1u : kTraceLdsBlocks;
+    __shared__ uint32_t trace[kTrace];
+    __shared__ DevBlock sblk[LB ? kLdsBlocks : 1];
+    __shared__ uint32_t svcut[LB ? kLdsBlocks + 1 : 1];
+    const bool lds_trace = a.block_steps && a.nblocks <= kTrace;
+    if (LB) {
+        for (uint32_t i = threadIdx.x; i < a.nblocks; i += blockDim.x) sblk[i] = a.blocks[i];
+        for (uint32_t i = threadIdx.x; i <= a.nblocks; i += blockDim.x) svcut[i] = a.vcut[i];
+    }
+    if (a.block_steps)
+        for (uint32_t i = threadIdx.x; i < kTrace; i += blockDim.x) trace[i] = 0;
+    if (LB || a.block_steps) __syncthreads();
+    const DevBlock* blocks = LB ? sblk : a.blocks;
+    const uint32_t* vcut = LB ? svcut : a.vcut;
+    Cnt c = {0, 0, 0, 0, 0, 0, 0};
+    const uint32_t lane = __lane_id();
+    DevBlock one;
+    if (ONE) one = a.blocks[0];
+    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    Lane L{};
+    L.ph = LS_IDLE;
+    bool exhausted = false;
+    WaveQueue q{0, 0};
+    uint32_t nwork, chunk;
+    work_size(a, nwork, chunk);
+
+    for (;;) {
+        const bool need = L.ph == LS_IDLE && !exhausted;
+        const uint64_t mask = __ballot(need);
+        if (mask) {
+            const uint32_t idx = wave_take(q, mask, a.queue, lane, chunk);
+            if (need) {
+                if (idx >= nwork) {
+                    exhausted = true;
+                } else if (a.worklist) {
+                    L.addr = a.worklist + idx;
+                    L.ph = LS_FETCH;
+                } else {
+                    L.w = idx;
+                    L.ph = LS_STATE;
+                }
+            }
+        }
+        const bool busy = L.ph != LS_IDLE;
+        if (!__any(busy || !exhausted)) break;
+        if (!busy) continue;
+
+        // ---- one round of loads
+        uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        uint32_t v = 0;
+        if (L.ph == LS_STATE) {
+            L.meta = a.meta[L.w];
+            L.cur = a.cur[L.w];
+            L.prev = a.prev[L.w];
+            L.bcp = a.bpair[L.w];
+            L.hash = a.hash ? a.hash[L.w] : 0;
+        } else if (L.ph == LS_ROWS) {
+            const uint64_t* pc = (const uint64_t*)L.addr;
+            const uint64_t* pp = (const uint64_t*)L.nu;
+            r0 = pc[0];
+            r1 = pc[1];
+            r2 = pp[0];
+            r3 = pp[1];
+        } else if (L.ph == LS_ROW) {
+            const uint64_t* pc = (const uint64_t*)L.addr;
+            r0 = pc[0];
+            r1 = pc[1];
+        } else {
+            v = *(const uint32_t*)L.addr;   // worklist entry, candidate, proposal or probe
+        }
+
+        // ---- consume
+        const DevType& tp = ((L.meta >> kTypeShift) & 1u) ? a.type1 : a.type0;
+        switch (L.ph) {
+        case LS_FETCH:
+            L.w = v;
+            L.ph = LS_STATE;
+            break;
+        case LS_STATE: {
+            const DevBlock& B = ONE ? one : blocks[L.bcp & 0xFFFFu];
+            const DevBlock& P = ONE ? one : blocks[L.bcp >> 16];
+            if (((L.meta >> kStatusShift) & 0xFu) != kActive || (a.scan && !(B.resident && P.resident))) {
+                L.ph = LS_IDLE;   // not runnable: left untouched
+                break;
+            }
+            L.addr = B.beg + (L.cur - B.start_vert);
+            L.nu = (const uint32_t*)(P.beg + (L.prev - P.start_vert));
+            L.fl = (B.resident == kMapped ? kLaneMv : 0u) | (P.resident == kMapped ? kLaneMu : 0u);
+            L.ph = LS_ROWS;
+            break;
+        }
+        case LS_ROWS:
+            lane_row(ONE ? one : blocks[L.bcp & 0xFFFFu], r0, r1, L.nv, L.dv);
+            lane_row(ONE ? one : blocks[L.bcp >> 16], r2, r3, L.nu, L.du);
+            L.ph = LS_STEP;
+            break;
+        case LS_ROW:
+            lane_row(ONE ? one : blocks[L.bcp & 0xFFFFu], r0, r1, L.nv, L.dv);
+            L.ph = LS_STEP;
+            break;
+        case LS_CAND:   // Node2vec: accept iff ra < T(class); prev's list decides only inside the band
+            L.x = v;
+            if (L.ra <= tp.m_min) {
+                L.ph = LS_MOVE;
+            } else if (L.x == L.prev) {
+                L.ph = L.ra <= tp.m_p ? LS_MOVE : LS_TRIAL;
+            } else if (L.ra <= tp.m_lo) {
+                L.ph = LS_MOVE;
+            } else if (L.ra > tp.m_hi) {
+                L.ph = LS_TRIAL;
+            } else if (lane_search<INTERP>(L, L.nu, L.du, a.search_end)) {
+                ++c.probes;
+                if (a.zc_count && (L.fl & kLaneMu)) ++c.zc;
+                L.ph = LS_PROBE;
+            } else {
+                L.ph = L.ra <= tp.m_q ? LS_MOVE : LS_TRIAL;
+            }
+            if (L.ph == LS_TRIAL) ++L.k;
+            break;
+        case LS_PICK:
+            L.x = v;
+            L.ph = LS_MOVE;
+            break;
+        case LS_PROPU:   // SOPR: a prev-neighbour y != prev is accepted iff y is in N(cur)
+            L.x = v;
+            if (L.x != L.prev && lane_search<INTERP>(L, L.nv, L.dv, a.search_end)) {
+                ++c.probes;
+                if (a.zc_count && (L.fl & kLaneMv)) ++c.zc;
+                L.ph = LS_PROBE;
+            } else {
+                ++L.k;
+                L.ph = LS_TRIAL;
+            }
+            break;
+        case LS_PROBE: {
+            const bool sopr = tp.algo != 0;
+            const uint32_t* list = sopr ? L.nv : L.nu;
+            const uint32_t mid = (uint32_t)((const uint32_t*)L.addr - list);
+            bool done = true, found = v == L.x;
+            if (!found) {
+                const uint32_t len = L.hx - L.lo;
+                if (v < L.x) {
+                    L.lo = mid + 1;
+                    L.vlo = v + 1;
+                } else {
+                    L.hx = mid;
+                    L.vhi = v;
+                }
+                if (L.lo < L.hx) {
+                    done = false;
+                    if (INTERP) L.fl = ((uint64_t)(L.hx - L.lo) * 2 > len) ? (L.fl | kLaneBisect) : (L.fl & ~kLaneBisect);
+                    L.addr = list + lane_mid<INTERP>(L);
+                    ++c.probes;
+                    if (a.zc_count && (L.fl & (sopr ? kLaneMv : kLaneMu))) ++c.zc;
+                }
+            }
+            if (done) {
+                const bool acc = sopr ? found : L.ra <= (found ? tp.m_one : tp.m_q);
+                if (acc) {
+                    L.ph = LS_MOVE;
+                } else {
+                    ++L.k;
+                    L.ph = LS_TRIAL;
+                }
+            }
+            break;
+        }
+        default:
+            break;
+        }
+
+        // ---- compute states, in the order they can follow each other
+        if (L.ph == LS_MOVE) {
+            const uint32_t bc = L.bcp & 0xFFFFu;
+            if (a.block_steps) {
+                if (lds_trace) atomicAdd(trace + bc, 1u);
+                else atomicAdd(a.block_steps + bc, 1ull);
+            }
+            L.prev = L.cur;
+            L.cur = L.x;
+            ++L.meta;   // hop + 1 (hop < length <= 2^20 - 1)
+            ++c.steps;
+            L.hash = (L.hash ^ L.x) * kFnvPrime;
+            if (a.visits) atomicAdd(a.visits + L.x, 1u);
+            const uint32_t hop = L.meta & kHopMask;
+            if (a.paths) a.paths[(uint64_t)L.w * a.path_stride + hop] = L.x;
+            L.nu = L.nv;
+            L.du = L.dv;
+            L.fl = (L.fl & ~kLaneMu) | ((L.fl & kLaneMv) ? kLaneMu : 0u);
+            if (hop >= tp.length) {
+                L.meta |= kDoneLength << kStatusShift;
+                L.bcp = bc | (bc << 16);
+                L.ph = LS_END;
+            } else if (ONE) {
+                L.bcp = bc | (bc << 16);
+                L.addr = one.beg + (L.cur - one.start_vert);
+                if (a.zc_count && (L.fl & kLaneMv)) ++c.zc;
+                L.ph = LS_ROW;
+            } else {
+                uint32_t nbc = bc;
+                const DevBlock& cb = blocks[bc];
+                if (L.cur - cb.start_vert >= cb.nverts) nbc = find_block(vcut, a.nblocks, L.cur);
+                const DevBlock& nb = blocks[nbc];
+                L.bcp = nbc | (bc << 16);
+                if (nb.resident) {
+                    L.fl = (L.fl & ~kLaneMv) | (nb.resident == kMapped ? kLaneMv : 0u);
+                    L.addr = nb.beg + (L.cur - nb.start_vert);
+                    if (a.zc_count && (L.fl & kLaneMv)) ++c.zc;
+                    L.ph = LS_ROW;
+                } else {
+                    L.ph = LS_END;   // steps into a block that is not resident: released, still active
+                }
+            }
+        }
+        if (L.ph == LS_STEP) {
+            const uint32_t hop = L.meta & kHopMask;
+            if (tp.t_term && (uint64_t)philox(L.w, hop, 0, kTagTerm, k0, k1).x < tp.t_term) {
+                L.meta |= kDoneTerm << kStatusShift;
+                L.ph = LS_END;
+            } else if (L.dv == 0) {
+                L.meta |= kDoneDeadEnd << kStatusShift;
+                L.ph = LS_END;
+            } else {
+                L.k = 0;
+                L.ph = LS_TRIAL;
+            }
+        }
+        if (L.ph == LS_TRIAL) {
+            if (L.k >= kMaxTrials) {
+                L.meta |= kDoneTrialCap << kStatusShift;
+                L.ph = LS_END;
+            } else {
+                const uint32_t hop = L.meta & kHopMask;
+                ++c.trials;
+                if (tp.algo == 0) {
+                    uint32_t rp;
+                    if (!(L.k & 1)) {
+                        const uint4 d = philox(L.w, hop, L.k >> 1, kTagStep, k0, k1);
+                        rp = d.x;
+                        L.ra = d.y;
+                        L.dz = d.z;
+                        L.dw = d.w;
+                    } else {
+                        rp = L.dz;
+                        L.ra = L.dw;
+                    }
+                    L.addr = L.nv + (((uint64_t)rp * L.dv) >> 32);
+                    if (a.zc_count && (L.fl & kLaneMv)) ++c.zc;
+                    L.ph = LS_CAND;
+                } else {
+                    const uint4 d = philox(L.w, hop, L.k, kTagStep, k0, k1);
+                    if (d.x <= tp.m_prop) {
+                        L.addr = L.nv + (((uint64_t)d.y * L.dv) >> 32);
+                        if (a.zc_count && (L.fl & kLaneMv)) ++c.zc;
+                        L.ph = LS_PICK;
+                    } else {
+                        L.addr = L.nu + (((uint64_t)d.y * L.du) >> 32);
+                        if (a.zc_count && (L.fl & kLaneMu)) ++c.zc;
+                        L.ph = LS_PROPU;
+                    }
+                }
+            }
+        }
+        if (L.ph == LS_END) {
+            a.cur[L.w] = L.cur;
+            a.prev[L.w] = L.prev;
+            a.meta[L.w] = L.meta;
+            a.bpair[L.w] = L.bcp;
+            if (a.hash) a.hash[L.w] = L.hash;
+            if (((L.meta >> kStatusShift) & 0xFu) != kActive) ++c.ended; else ++c.left;
+            L.ph = LS_IDLE;
+        }
+    }
+    if (lds_trace) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < a.nblocks; i += blockDim.x)
+            if (trace[i]) atomicAdd(a.block_steps + i, (unsigned long long)trace[i]);
+    }
+    const uint64_t s0 = wave_sum(c.steps), s1 = wave_sum(c.trials), s2 = wave_sum(c.probes);
+    const uint64_t s4 = wave_sum(c.ended), s5 = wave_sum(c.left);
+    const uint64_t s7 = a.zc_count ? wave_sum(c.zc) : 0;
+    if (lane == 0) {
+        atomicAdd(a.counters + 0, (unsigned long long)s0);
+        atomicAdd(a.counters + 1, (unsigned long long)s1);
+        atomicAdd(a.counters + 2, (unsigned long long)s2);
+        atomicAdd(a.counters + 4, (unsigned long long)s4);
+        atomicAdd(a.counters + 5, (unsigned long long)s5);
+        if (s7) atomicAdd(a.counters + 7, (unsigned long long)s7);
     }
 }
 
@@ -1059,29 +1441,47 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
                        uint32_t block, hipStream_t s)
 {
     if (a.nwork == 0) return hipSuccess;
-    if (variant == 1) {
+    if (variant == 2) {
+        const bool in = a.search_end != 0, lb = a.nblocks <= kLdsBlocks;
+        if (one_block && !in) hipLaunchKernelGGL((walk_lane_kernel<true, false, false>), dim3(grid), dim3(block), 0, s, a);
+        else if (one_block) hipLaunchKernelGGL((walk_lane_kernel<true, true, false>), dim3(grid), dim3(block), 0, s, a);
+        else if (!in && lb) hipLaunchKernelGGL((walk_lane_kernel<false, false, true>), dim3(grid), dim3(block), 0, s, a);
+        else if (lb) hipLaunchKernelGGL((walk_lane_kernel<false, true, true>), dim3(grid), dim3(block), 0, s, a);
+        else if (!in) hipLaunchKernelGGL((walk_lane_kernel<false, false, false>), dim3(grid), dim3(block), 0, s, a);
+        else hipLaunchKernelGGL((walk_lane_kernel<false, true, false>), dim3(grid), dim3(block), 0, s, a);
+    } else if (variant == 1) {
         if (one_block) hipLaunchKernelGGL(walk_flat_kernel<true>, dim3(grid), dim3(block), 0, s, a);
         else hipLaunchKernelGGL(walk_flat_kernel<false>, dim3(grid), dim3(block), 0, s, a);
     } else {
-        const bool in = a.search_end != 0;
-        if (one_block && !in) hipLaunchKernelGGL((walk_kernel<true, false>), dim3(grid), dim3(block), 0, s, a);
-        else if (one_block) hipLaunchKernelGGL((walk_kernel<true, true>), dim3(grid), dim3(block), 0, s, a);
-        else if (!in) hipLaunchKernelGGL((walk_kernel<false, false>), dim3(grid), dim3(block), 0, s, a);
-        else hipLaunchKernelGGL((walk_kernel<false, true>), dim3(grid), dim3(block), 0, s, a);
+        const bool in = a.search_end != 0, lb = a.nblocks <= kLdsBlocks;
+        if (one_block && !in) hipLaunchKernelGGL((walk_kernel<true, false, false>), dim3(grid), dim3(block), 0, s, a);
+        else if (one_block) hipLaunchKernelGGL((walk_kernel<true, true, false>), dim3(grid), dim3(block), 0, s, a);
+        else if (!in && lb) hipLaunchKernelGGL((walk_kernel<false, false, true>), dim3(grid), dim3(block), 0, s, a);
+        else if (lb) hipLaunchKernelGGL((walk_kernel<false, true, true>), dim3(grid), dim3(block), 0, s, a);
+        else if (!in) hipLaunchKernelGGL((walk_kernel<false, false, false>), dim3(grid), dim3(block), 0, s, a);
+        else hipLaunchKernelGGL((walk_kernel<false, true, false>), dim3(grid), dim3(block), 0, s, a);
     }
     return hipGetLastError();
 }
 
-uint32_t walk_blocks_per_cu(bool one_block, uint32_t variant, uint32_t block)
+uint32_t walk_blocks_per_cu(bool one_block, uint32_t nblocks, uint32_t variant, uint32_t block)
 {
     int n = 0;
     hipError_t e;
     if (variant == 1)
         e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<true>, (int)block, 0)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<false>, (int)block, 0);
+    else if (variant == 2)
+        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<true, false, false>, (int)block, 0)
+            : nblocks <= kLdsBlocks
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, true>, (int)block, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, false>, (int)block, 0);
+    else if (one_block)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true, false, false>, (int)block, 0);
+    else if (nblocks <= kLdsBlocks)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, false, true>, (int)block, 0);
     else
-        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true, false>, (int)block, 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, false>, (int)block, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, false, false>, (int)block, 0);
     return (e == hipSuccess && n > 0) ? (uint32_t)n : 1u;
 }
 

@@ -176,7 +176,7 @@ class Context:
 
     def autotune(self, sample, what=7):
         """Pick the fastest launch shape / search order on the first `sample` walkers (state
-        restored); `what`: 1 workgroup sizes, 2 kernel variant 1, 4 search order."""
+        restored); `what`: 1 workgroup sizes, 2 kernel variants 1 and 2, 4 search order."""
         best = GwLaunch()
         ms = C.c_double()
         self._chk(lib().gw_autotune(self.h, sample, what, C.byref(best), C.byref(ms)))
@@ -310,9 +310,9 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
     o.scheduler = scheduler
     o.verbose = 1 if verbose else 0
     o.flags = flags
-    if launch:
-        o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 0),
-                            GW_RUN_INTERP if launch.get("interp") else 0)
+    launch = launch or {}
+    o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 2),
+                        GW_RUN_INTERP if launch.get("interp") else 0)
     wp = walk_params(types, seed)
     rep = GwhReport()
     h = np.zeros(nwalkers, dtype=np.uint64) if hashes else None

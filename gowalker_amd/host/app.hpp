@@ -81,7 +81,8 @@ struct HostGraph {
 inline void print_help()
 {
     std::printf(
-        "- dataset:       the dataset path (<base>.beg/.csr/.meta, see gw_generate)\n"
+        "- dataset:       a text edge list sorted by source, converted on first use to\n"
+        "                 <base>.beg/.csr/.meta (or those files, e.g. from gw_generate)\n"
         "- blocksize:     the size of each block in MB (default 1024)\n"
         "- memory:        the HBM budget in GB for graph slots and walker state (default 4)\n"
         "- walkpersource: the number of walks for each vertex (default 1)\n"
@@ -94,6 +95,7 @@ inline void print_help()
         "- GOWalker:      use the walker-count block scheduler (required)\n"
         "- sched:         plain | learned (default plain)\n"
         "- seed:          Philox seed (default 42)\n"
+        "- threadperblock: workgroup size of the step kernel (default: the engine's)\n"
         "- source:        SOPR/SOSR sources: reference (vertex 1, or 1 and 3) | pervertex\n"
         "- visits <file>, hashes <file>: write per-vertex visit counts / per-walker path hashes\n"
         "- h:             print this message\n");
@@ -126,7 +128,9 @@ inline int app_main(int argc, const char** argv, App app)
     const uint32_t length = (uint32_t)o.get_int("length", 30);
     const double p = o.get_float("p", 0.5f), q = o.get_float("q", 2.0f), alpha = o.get_float("alpha", 0.2f);
     const double term = o.get_float("term", 0.0f);
-    const uint32_t tpb = (uint32_t)o.get_int("threadperblock", 512);
+    // threadperblock (apps/Node2vec.cu:47, 512 there) fixes the workgroup size when given;
+    // otherwise the engine picks its default shape and the search order by its probe model
+    const uint32_t tpb = (uint32_t)o.get_int("threadperblock", 0);
     const bool zero = o.get_bool("zero");
     const bool verbose = o.get_bool("verbose");
     const uint64_t seed = o.get_long("seed", 42);
@@ -165,7 +169,7 @@ inline int app_main(int argc, const char** argv, App app)
         EngineConfig ec;
         ec.memory_bytes = memory;
         ec.zero_threshold = zero ? blocksize / 128 / 24 : 0;   // apps/Node2vec.cu:100-108
-        ec.launch.block = std::max<uint32_t>(64, std::min<uint32_t>(1024, tpb / 64 * 64));
+        if (tpb) ec.launch.block = std::max<uint32_t>(64, std::min<uint32_t>(1024, tpb / 64 * 64));
         ec.verbose = verbose;
         Engine engine(ec, g.beg, g.csr, blocks);
 

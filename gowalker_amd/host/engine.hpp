@@ -37,7 +37,9 @@ struct EngineConfig {
     uint64_t memory_bytes = 0;      // HBM budget for slots + walker state (0: every block fits)
     uint64_t zero_threshold = 0;
     bool zero_copy_tail = true;
-    gw_launch launch{0, 0, 0, 0};   // block 0: 256 threads, search order from the probe model
+    // block 0: 256 threads of `variant` (default: the lane kernel, which walks out-of-core
+    // Node2vec fastest; other walker types run variant 0), search order from the probe model
+    gw_launch launch{0, 0, 2, 0};
     bool verbose = false;
 };
 
@@ -186,7 +188,8 @@ public:
             // both orders need on sampled Node2vec-style queries of this graph (timing on the
             // device would need a resident cache that is not there yet)
             auto tt = clk::now();
-            base = gw_launch{256, 0, 0, prefer_interpolation(beg_, csr_, total_verts()) ? (uint32_t)GW_RUN_INTERP : 0u};
+            base = gw_launch{256, 0, cfg_.launch.variant,
+                             prefer_interpolation(beg_, csr_, total_verts()) ? (uint32_t)GW_RUN_INTERP : 0u};
             r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();
             r.tuned = base;
         }

@@ -91,7 +91,9 @@ enum {
 typedef struct {
     uint32_t block;          /* threads per workgroup (64..1024, multiple of 64); 0 = tuned */
     uint32_t waves_per_cu;   /* persistent grid = CUs * waves_per_cu * 64 / block; 0 = tuned */
-    uint32_t variant;        /* 0 = per-step lane loop, 1 = flattened per-probe loop */
+    uint32_t variant;        /* 0 = per-step lane loop, 1 = flattened per-probe loop,
+                                2 = lean per-load lane state machine (Node2vec and the SOPR
+                                proposal sampler; other walker types run variant 0) */
     uint32_t flags;          /* GW_RUN_* */
 } gw_launch;
 
@@ -101,7 +103,7 @@ enum {
     GW_RUN_TRACE = 1u << 1,   /* record steps per current block (gw_trace_read) */
     GW_RUN_INTERP = 1u << 2   /* membership tests by guarded interpolation search instead of
                                  bisection: same walks, fewer reads on evenly spread ids
-                                 (randomly permuted graphs); step kernel variant 0 only */
+                                 (randomly permuted graphs); step kernel variants 0 and 2 */
 };
 
 typedef struct {

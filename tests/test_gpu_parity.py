@@ -96,7 +96,7 @@ def test_n2v_multi_block_resident(gpu):
     assert o["totals"]["probes"] > 0
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_sopr_envelope_sampler(gpu, variant):
     # multigraph lists; the envelope sampler switches between rejection and the mixture
     beg, csr = G.power_law_directed(3000, 40000, seed=2)
@@ -105,7 +105,7 @@ def test_sopr_envelope_sampler(gpu, variant):
     assert 0 < o["totals"]["mix_steps"] < o["totals"]["steps"]
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_sopr_proposal_sampler(gpu, variant):
     beg, csr = G.power_law_directed(3000, 40000, seed=2, multi=False)
     g, o = check_case(beg, csr, [dict(algo=SOPR, length=30, alpha=0.2, term=0.15)], 42, 6000, wpv=2,
@@ -113,14 +113,14 @@ def test_sopr_proposal_sampler(gpu, variant):
     assert o["totals"]["mix_steps"] == 0 and o["totals"]["probes"] > 0
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_node2vec_both_variants(gpu, variant):
     beg, csr = G.rmat(12, 30000, seed=6, permute=True)
     check_case(beg, csr, [dict(algo=NODE2VEC, length=30, p=0.5, q=2.0)], 8, 4096, block_bytes=8192,
                launch=dict(variant=variant))
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_mixed_batch(gpu, variant):
     beg, csr = G.rmat(11, 20000, seed=9, undirected=False)
     types = [dict(algo=NODE2VEC, length=20, p=1.0, q=0.25), dict(algo=SOPR, length=20, alpha=0.2, term=0.2)]
@@ -131,7 +131,7 @@ def test_mixed_batch(gpu, variant):
 def test_tiny_graphs(gpu, name):
     beg, csr = getattr(G, name)()
     n = len(beg) - 1
-    for variant in (0, 1):
+    for variant in (0, 1, 2):
         L = dict(variant=variant)
         check_case(beg, csr, [dict(algo=NODE2VEC, length=12, p=0.5, q=2.0)], 1, n * 8, wpv=8, launch=L)
         check_case(beg, csr, [dict(algo=SOPR, length=12, alpha=0.3, term=0.1)], 2, n * 8, wpv=8, launch=L)
@@ -139,7 +139,7 @@ def test_tiny_graphs(gpu, name):
                    launch=L)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_out_of_core_window(gpu, variant):
     beg, csr = G.rmat(10, 8000, seed=4, permute=True)
     g, o = check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 5, 1024, block_bytes=4096,
@@ -153,7 +153,8 @@ def test_sources_and_launch_shapes(gpu):
     src = np.random.default_rng(0).integers(0, 500, 3000).astype(np.uint32)
     types = [dict(algo=NODE2VEC, length=15, p=2.0, q=0.5)]
     for launch in (dict(block=64, waves_per_cu=1), dict(block=1024, waves_per_cu=32),
-                   dict(block=64, waves_per_cu=3, variant=1), dict(block=256, variant=1)):
+                   dict(block=64, waves_per_cu=3, variant=1), dict(block=256, variant=1),
+                   dict(block=64, waves_per_cu=1, variant=2), dict(block=1024, variant=2)):
         check_case(beg, csr, types, 3, 3000, sources=src, launch=launch)
 
 
@@ -190,9 +191,10 @@ def test_interpolation_search_order(gpu, graph):
         beg, csr = G.rmat(12, 40000, seed=31, permute=True)
     else:
         beg, csr = G.rmat(12, 40000, seed=31, permute=False)
-    L = dict(interp=True)
-    check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 3, 4096, launch=L)
-    check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1)], 4, 4096, launch=L)
-    check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1, sampler=1)], 5, 4096, launch=L)
-    check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 6, 2048, block_bytes=8192, nslots=3,
-               launch=L)
+    for variant in (0, 2):
+        L = dict(interp=True, variant=variant)
+        check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 3, 4096, launch=L)
+        check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1)], 4, 4096, launch=L)
+        check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1, sampler=1)], 5, 4096, launch=L)
+        check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 6, 2048, block_bytes=8192, nslots=3,
+                   launch=L)
