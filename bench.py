@@ -255,6 +255,8 @@ def main():
     ap.add_argument("--variant", type=int, default=None,
                     help="step kernel: 0 per-step lane loop, 1 flattened state machine, 2 lean per-load lanes "
                          "(default: the autotuner's pick for resident workloads, 2 out of core)")
+    ap.add_argument("--interp", action="store_true",
+                    help="resident workloads with a fixed shape: membership by interpolation search")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
     ap.add_argument("--ablation", action="store_true", help="out-of-core: also run scheduler variants")
@@ -298,7 +300,7 @@ def main():
     outputs = E.OUT_VISITS if visits else E.OUT_HASH
     if args.variant is None:
         args.variant = 0
-    launch = dict(block=args.block, waves_per_cu=args.waves_per_cu, variant=args.variant)
+    launch = dict(block=args.block, waves_per_cu=args.waves_per_cu, variant=args.variant, interp=args.interp)
 
     def one_step():
         ctx.init_walkers(nwalkers, cfg["wpv"], outputs=outputs)

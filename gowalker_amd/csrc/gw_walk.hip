@@ -304,8 +304,9 @@ __device__ __forceinline__ uint32_t wave_take(WaveQueue& q, uint64_t mask, uint3
 // for its longest walker -- only for its slowest step.  ONE: a single resident block,
 // held in registers.  LB: a block table of at most kLdsBlocks entries, copied with its
 // cut points into LDS, so that the descriptor reads and the vertex->block search a step
-// across blocks makes stay on chip (the table does not change during a launch).
-template <bool ONE, bool INTERP, bool LB>
+// across blocks makes stay on chip (the table does not change during a launch).  MIX:
+// two walker types; with one, the type's parameters are wavefront-uniform (SGPRs).
+template <bool ONE, bool INTERP, bool LB, bool MIX>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_kernel(WalkArgs a)
 {
     // block-access trace: steps per current block, histogrammed in LDS
@@ -370,7 +371,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
         if (!__any(have || !exhausted)) break;
         if (!have) continue;
 
-        const DevType& tp = type ? a.type1 : a.type0;
+        const DevType& tp = (MIX && type) ? a.type1 : a.type0;
         uint32_t status = kActive;
         bool release = false;
         if (tp.t_term) {
@@ -963,7 +964,7 @@ __device__ __forceinline__ bool lane_search(Lane& L, const uint32_t* list, uint3
     return true;
 }
 
-template <bool ONE, bool INTERP, bool LB>
+template <bool ONE, bool INTERP, bool LB, bool MIX>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_lane_kernel(WalkArgs a)
 {
     constexpr uint32_t kTrace = ONE ? 1u : kTraceLdsBlocks;
@@ -1038,7 +1039,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
         }
 
         // ---- consume
-        const DevType& tp = ((L.meta >> kTypeShift) & 1u) ? a.type1 : a.type0;
+        const DevType& tp = (MIX && ((L.meta >> kTypeShift) & 1u)) ? a.type1 : a.type0;
         switch (L.ph) {
         case LS_FETCH:
             L.w = v;
@@ -1437,29 +1438,41 @@ hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s)
     return hipGetLastError();
 }
 
+// Step kernel instances: variant x single block x search order x LDS table x walker types.
+template <bool ONE, bool INTERP, bool LB, bool MIX>
+static void launch_shape(const WalkArgs& a, uint32_t variant, uint32_t grid, uint32_t block, hipStream_t s)
+{
+    if (variant == 2) hipLaunchKernelGGL((walk_lane_kernel<ONE, INTERP, LB, MIX>), dim3(grid), dim3(block), 0, s, a);
+    else hipLaunchKernelGGL((walk_kernel<ONE, INTERP, LB, MIX>), dim3(grid), dim3(block), 0, s, a);
+}
+
+template <bool ONE, bool INTERP, bool LB>
+static void launch_types(const WalkArgs& a, uint32_t variant, uint32_t grid, uint32_t block, hipStream_t s)
+{
+    if (a.ntypes == 2) launch_shape<ONE, INTERP, LB, true>(a, variant, grid, block, s);
+    else launch_shape<ONE, INTERP, LB, false>(a, variant, grid, block, s);
+}
+
+template <bool INTERP>
+static void launch_search(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid, uint32_t block,
+                          hipStream_t s)
+{
+    if (one_block) launch_types<true, INTERP, false>(a, variant, grid, block, s);
+    else if (a.nblocks <= kLdsBlocks) launch_types<false, INTERP, true>(a, variant, grid, block, s);
+    else launch_types<false, INTERP, false>(a, variant, grid, block, s);
+}
+
 hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid,
                        uint32_t block, hipStream_t s)
 {
     if (a.nwork == 0) return hipSuccess;
-    if (variant == 2) {
-        const bool in = a.search_end != 0, lb = a.nblocks <= kLdsBlocks;
-        if (one_block && !in) hipLaunchKernelGGL((walk_lane_kernel<true, false, false>), dim3(grid), dim3(block), 0, s, a);
-        else if (one_block) hipLaunchKernelGGL((walk_lane_kernel<true, true, false>), dim3(grid), dim3(block), 0, s, a);
-        else if (!in && lb) hipLaunchKernelGGL((walk_lane_kernel<false, false, true>), dim3(grid), dim3(block), 0, s, a);
-        else if (lb) hipLaunchKernelGGL((walk_lane_kernel<false, true, true>), dim3(grid), dim3(block), 0, s, a);
-        else if (!in) hipLaunchKernelGGL((walk_lane_kernel<false, false, false>), dim3(grid), dim3(block), 0, s, a);
-        else hipLaunchKernelGGL((walk_lane_kernel<false, true, false>), dim3(grid), dim3(block), 0, s, a);
-    } else if (variant == 1) {
+    if (variant == 1) {
         if (one_block) hipLaunchKernelGGL(walk_flat_kernel<true>, dim3(grid), dim3(block), 0, s, a);
         else hipLaunchKernelGGL(walk_flat_kernel<false>, dim3(grid), dim3(block), 0, s, a);
+    } else if (a.search_end != 0) {
+        launch_search<true>(a, one_block, variant, grid, block, s);
     } else {
-        const bool in = a.search_end != 0, lb = a.nblocks <= kLdsBlocks;
-        if (one_block && !in) hipLaunchKernelGGL((walk_kernel<true, false, false>), dim3(grid), dim3(block), 0, s, a);
-        else if (one_block) hipLaunchKernelGGL((walk_kernel<true, true, false>), dim3(grid), dim3(block), 0, s, a);
-        else if (!in && lb) hipLaunchKernelGGL((walk_kernel<false, false, true>), dim3(grid), dim3(block), 0, s, a);
-        else if (lb) hipLaunchKernelGGL((walk_kernel<false, true, true>), dim3(grid), dim3(block), 0, s, a);
-        else if (!in) hipLaunchKernelGGL((walk_kernel<false, false, false>), dim3(grid), dim3(block), 0, s, a);
-        else hipLaunchKernelGGL((walk_kernel<false, true, false>), dim3(grid), dim3(block), 0, s, a);
+        launch_search<false>(a, one_block, variant, grid, block, s);
     }
     return hipGetLastError();
 }
@@ -1472,16 +1485,16 @@ uint32_t walk_blocks_per_cu(bool one_block, uint32_t nblocks, uint32_t variant, 
         e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<true>, (int)block, 0)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<false>, (int)block, 0);
     else if (variant == 2)
-        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<true, false, false>, (int)block, 0)
+        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<true, false, false, true>, (int)block, 0)
             : nblocks <= kLdsBlocks
-                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, true>, (int)block, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, false>, (int)block, 0);
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, true, true>, (int)block, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, false, true>, (int)block, 0);
     else if (one_block)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true, false, false>, (int)block, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true, false, false, true>, (int)block, 0);
     else if (nblocks <= kLdsBlocks)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, false, true>, (int)block, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, false, true, true>, (int)block, 0);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, false, false>, (int)block, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<false, false, false, true>, (int)block, 0);
     return (e == hipSuccess && n > 0) ? (uint32_t)n : 1u;
 }
 
