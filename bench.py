@@ -224,10 +224,11 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "%s<false, %s, %s>" % ("walk_lane_kernel" if (r0.get("tuned") or {}).get("variant") == 2
-                                                       else "walk_kernel",
-                                                       "true" if (r0.get("tuned") or {}).get("interp") else "false",
-                                                       "true" if r0["nblocks"] <= LDS_BLOCKS else "false"),
+                     "kernel": "%s<false, %s, %s, %s>" % ("walk_lane_kernel" if (r0.get("tuned") or {}).get("variant") == 2
+                                                           else "walk_kernel",
+                                                           "true" if (r0.get("tuned") or {}).get("interp") else "false",
+                                                           "true" if r0["nblocks"] <= LDS_BLOCKS else "false",
+                                                           "true" if len(cfg["types"]) == 2 else "false"),
                      "kernel_ms": r0["kernel_ms"] / launches, "algorithmic_bytes": abytes,
                      "sector_bytes": sector_bytes(r0, nwalkers, False) / launches,
                      "launches": launches,
@@ -363,8 +364,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "walk_flat_kernel<true>" if args.variant == 1 else
-                               "%s<true, %s, false>" % ("walk_lane_kernel" if args.variant == 2 else "walk_kernel",
-                                                        "true" if (tuned or {}).get("interp") else "false"),
+                               "%s<true, %s, false, %s>" % ("walk_lane_kernel" if args.variant == 2 else "walk_kernel",
+                                                            "true" if (tuned or {}).get("interp") else "false",
+                                                            "true" if len(cfg["types"]) == 2 else "false"),
                      "kernel_ms": avg_kms, "algorithmic_bytes": abytes,
                      "sector_bytes": sector_bytes(st0, nwalkers, visits)},
         "cpu_baseline": None,
