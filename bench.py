@@ -164,7 +164,8 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                 rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
                                               memory_bytes=mem, block_bytes=block_bytes, zero_threshold=zt,
                                               scheduler=sched, hashes=first, flags=flags,
-                                              launch=None if args.variant is None else dict(variant=args.variant))
+                                              launch=None if args.variant is None else dict(variant=args.variant),
+                                              prefetch_depth=args.prefetch_depth, verbose=args.verbose)
                 if first:
                     hashes[(cap_gb, name)] = h
                 print("[bench] cap %g GB %s run %d: %.3f s, %d steps, %.2f GB H2D (%d uploads + %d staged), "
@@ -256,6 +257,9 @@ def main():
     ap.add_argument("--variant", type=int, default=None,
                     help="step kernel: 0 per-step lane loop, 1 flattened state machine, 2 lean per-load lanes "
                          "(default: the autotuner's pick for resident workloads, 2 out of core)")
+    ap.add_argument("--prefetch-depth", type=int, default=0,
+                    help="out-of-core learned scheduler: staged block copies in flight (0: the library's 4)")
+    ap.add_argument("--verbose", action="store_true", help="out-of-core: the engine logs every iteration to stderr")
     ap.add_argument("--interp", action="store_true",
                     help="resident workloads with a fixed shape: membership by interpolation search")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))

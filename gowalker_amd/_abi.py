@@ -151,7 +151,7 @@ HOST_LIB_PATH = os.path.join(_HERE, "_build", "libgwhost.so")
 class GwhEngineOpts(C.Structure):
     _fields_ = [("memory_bytes", C.c_uint64), ("block_bytes", C.c_uint64), ("zero_threshold", C.c_uint64),
                 ("scheduler", C.c_uint32), ("verbose", C.c_uint32), ("launch", GwLaunch), ("device", C.c_int32),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("prefetch_depth", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class GwhReport(C.Structure):

@@ -75,6 +75,9 @@ struct gw_ctx {
     unsigned long long* h_cnt = nullptr;   // pinned readback
     unsigned long long* m_cnt = nullptr;   // its device-side (mapped) address
     uint32_t* m_bucket = nullptr;          // mapped address of h_bucket
+    unsigned long long* h_trace = nullptr; // pinned copy of d_trace, written by traced runs
+    unsigned long long* m_trace = nullptr; // its mapped address
+    bool trace_valid = false;              // h_trace holds the last traced run's counts
     DevBlock* m_blocks = nullptr;          // mapped address of h_blocks_pinned
     uint8_t* h_ring = nullptr;             // pinned ring for small transfers
     uint8_t* m_ring = nullptr;
@@ -324,6 +327,7 @@ gw_status gw_destroy(gw_ctx* c)
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
     if (c->h_blocks_pinned) (void)hipHostFree(c->h_blocks_pinned);
     if (c->h_bucket) (void)hipHostFree(c->h_bucket);
+    if (c->h_trace) (void)hipHostFree(c->h_trace);
     if (c->h_ring) (void)hipHostFree(c->h_ring);
     hipEvent_t evs[] = {c->ev_copy, c->ev_f0, c->ev_f1, c->ev_k0, c->ev_k1, c->ev_c0, c->ev_c1};
     for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
@@ -382,7 +386,9 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     c->d_trace = nullptr; c->d_feat = nullptr; c->d_age = nullptr;
     if (c->h_blocks_pinned) { (void)hipHostFree(c->h_blocks_pinned); c->h_blocks_pinned = nullptr; }
     if (c->h_bucket) { (void)hipHostFree(c->h_bucket); c->h_bucket = nullptr; }
+    if (c->h_trace) { (void)hipHostFree(c->h_trace); c->h_trace = nullptr; }
     c->w_valid = false;
+    c->trace_valid = false;
     c->nblocks = nb;
     c->desc.assign(b, b + nb);
     c->hblk.assign(nb, DevBlock{});
@@ -418,6 +424,8 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     HIPCHK(c, hipHostMalloc((void**)&c->h_blocks_pinned, sizeof(DevBlock) * nb, hipHostMallocDefault));
     HIPCHK(c, hipHostMalloc((void**)&c->h_bucket, sizeof(uint32_t) * (uint64_t)nb * nb, hipHostMallocDefault));
     HIPCHK(c, hipHostGetDevicePointer((void**)&c->m_bucket, c->h_bucket, 0));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_trace, sizeof(unsigned long long) * nb, hipHostMallocDefault));
+    HIPCHK(c, hipHostGetDevicePointer((void**)&c->m_trace, c->h_trace, 0));
     HIPCHK(c, hipHostGetDevicePointer((void**)&c->m_blocks, c->h_blocks_pinned, 0));
     HIPCHK(c, hipMemcpy(c->d_vcut, c->vcut.data(), sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice));
     return GW_OK;
@@ -928,12 +936,17 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     HIPCHK(c, launch_walk(a, one_blk, l.variant, (uint32_t)grid, l.block, c->s_compute));
     HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));
     HIPCHK(c, launch_copy_u32((uint32_t*)c->m_cnt, (const uint32_t*)c->d_cnt, 16, c->s_compute));
+    // the trace comes back with the counters, so gw_trace_read needs no device round trip
+    const bool traced = a.block_steps != nullptr;
+    if (traced) HIPCHK(c, launch_copy_u32((uint32_t*)c->m_trace, (const uint32_t*)c->d_trace, 2ull * c->nblocks, c->s_compute));
+    c->trace_valid = false;
     const bool buckets = (l.flags & GW_RUN_BUCKETS) != 0;
     if (buckets) {
         // bucket matrix of the walkers left, read back with the counters (one host wait)
         if ((st = enqueue_bucket_pass(c)) != GW_OK) return st;
     }
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    c->trace_valid = traced;
     HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
     if (buckets) {
         HIPCHK(c, hipEventElapsedTime(&fms, c->ev_f0, c->ev_f1));
@@ -1055,6 +1068,10 @@ gw_status gw_trace_read(gw_ctx* c, uint64_t* block_steps, uint64_t n)
     if (!c || !block_steps) return fail(c, GW_ERR_ARG, "null argument");
     if (!c->nblocks) return fail(c, GW_ERR_STATE, "no block table");
     if (n < c->nblocks) return fail(c, GW_ERR_ARG, "trace buffer too small");
+    if (c->trace_valid) {   // brought back with the traced run's counters
+        std::memcpy(block_steps, c->h_trace, c->nblocks * 8ull);
+        return GW_OK;
+    }
     (void)hipSetDevice(c->device);
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
     HIPCHK(c, d2h_small(c, block_steps, c->d_trace, c->nblocks * 8ull));

@@ -298,7 +298,7 @@ SCHED_PLAIN, SCHED_LEARNED = 0, 1
 
 def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory_bytes=0, block_bytes=256 << 20,
                     zero_threshold=0, scheduler=SCHED_PLAIN, hashes=False, visits=False, verbose=False, launch=None,
-                    flags=0):
+                    flags=0, prefetch_depth=0):
     """Runs the C++ out-of-core engine (libgwhost.so) over a host CSR graph; beg/csr should
     be pinned (PinnedArray.a) for the zero-copy tail.  Returns (report dict, hashes, visits)."""
     from ._abi import GwhEngineOpts, GwhReport, host_lib
@@ -310,6 +310,7 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
     o.scheduler = scheduler
     o.verbose = 1 if verbose else 0
     o.flags = flags
+    o.prefetch_depth = prefetch_depth
     launch = launch or {}
     o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 2),
                         GW_RUN_INTERP if launch.get("interp") else 0)

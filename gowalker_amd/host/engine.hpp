@@ -283,11 +283,11 @@ public:
             r.kernel_ms += st.kernel_ms;
             r.filter_ms += st.filter_ms;
             if (cfg_.verbose)
-                std::fprintf(stderr, "[engine] it %llu: %zu resident, %zu loaded, %zu staged, steps %llu, walkers run %llu, "
-                             "left %llu, kernel %.2f ms\n",
-                             (unsigned long long)it, sched.buckets.size(), out.loaded.size(), pf.size(),
-                             (unsigned long long)st.steps, (unsigned long long)st.walkers_run,
-                             (unsigned long long)st.walkers_left, st.kernel_ms);
+                std::fprintf(stderr, "[engine] %.1f ms it %llu: %zu resident, %zu loaded, %zu staged, steps %llu, "
+                             "walkers run %llu, left %llu, kernel %.2f ms\n",
+                             std::chrono::duration<double, std::milli>(clk::now() - t0).count(), (unsigned long long)it,
+                             sched.buckets.size(), out.loaded.size(), pf.size(), (unsigned long long)st.steps,
+                             (unsigned long long)st.walkers_run, (unsigned long long)st.walkers_left, st.kernel_ms);
         }
         ck(gw_wait_uploads(ctx_), ctx_, "gw_wait_uploads");
         gw_counters c1;

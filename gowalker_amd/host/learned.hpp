@@ -165,7 +165,7 @@ public:
         for (uint32_t b : buckets)
             if (!was_resident_[b]) fresh.push_back(b);
         for (uint32_t b : fresh) {
-            if (feats_.size() < (size_t)(b + 1) * kF) continue;
+            if (!have_feats_ || feats_.size() < (size_t)(b + 1) * kF) continue;
             Open& o = open_[b];
             o.live = true;
             o.f.assign(feats_.begin() + (size_t)b * kF, feats_.begin() + (size_t)(b + 1) * kF);
@@ -242,13 +242,11 @@ private:
     {
         const uint32_t nb = in.nblocks;
         const uint64_t n = nb;
-        feats_.assign((size_t)nb * kF, 0.f);
-        have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
         std::vector<char> cached(nb, 0);
         for (uint32_t b : buckets) cached[b] = 1;
         was_resident_ = cached;
         uint64_t wl = 0;
-        if (gw_pending_work(ctx_, &wl, &pending_hops_) != GW_OK) pending_hops_ = ~0ULL;
+        if (gw_pending_work(ctx_, &wl, &pending_hops_) != GW_OK) pending_hops_ = ~0ULL;   // host copy, no device trip
         add_.assign(nb, 0);
         demand_.assign(nb, 0);
         for (uint32_t b = 0; b < nb; ++b) {
@@ -262,16 +260,23 @@ private:
             add_[b] = a;
             demand_[b] = d;
         }
-        // the model is consulted while free slots are being filled and near the zero-copy
-        // tail (best candidate within 64x the reference's threshold); elsewhere the plain
-        // rule decides and a forward pass would be wasted
-        uint64_t best_add = 0, max_bytes = 0;
+        // the model is consulted while free slots are being filled and once its tail rule
+        // can fire (all the work left is worth at most tail_guard uploads in zero-copy
+        // reads); elsewhere the plain rule decides and a forward pass would be wasted
+        uint64_t max_bytes = 0;
         for (uint32_t b = 0; b < nb; ++b) {
-            if (!cached[b]) best_add = std::max(best_add, add_[b]);
             const gw_block_desc& d = in.blocks[b];
             max_bytes = std::max<uint64_t>(max_bytes, ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4);
         }
-        const bool need = buckets.size() < capacity(in) || (double)best_add <= 64.0 * zero_floor((double)max_bytes);
+        const bool need = buckets.size() < capacity(in) ||
+                          (double)pending_hops_ * zc_bytes() < cfg_.tail_guard * (double)max_bytes;
+        // features (a kernel and a readback): whenever the model is consulted, and on every
+        // 4th iteration in between so that loads keep opening training samples
+        have_feats_ = false;
+        if (need || in.iteration % 4 == 0) {
+            feats_.assign((size_t)nb * kF, 0.f);
+            have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
+        }
         pred_.assign(nb, 0.f);
         use_model_ = need && trained() && have_feats_ &&
                      gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
