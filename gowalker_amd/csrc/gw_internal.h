@@ -128,6 +128,7 @@ hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBl
                            float* feats, hipStream_t s);
 constexpr uint32_t kTraceLdsBlocks = 2048;   // per-workgroup LDS histogram size of the trace
 constexpr uint32_t kLdsBlocks = 128;         // block tables up to this size are staged in LDS by the step kernel
+constexpr uint32_t kLutSize = 2048;          // entries of the LDS vertex -> block lookup table
 constexpr uint32_t kNumFeatures = 12;
 constexpr uint32_t kFilterLdsBuckets = 12288;  // bucket matrices up to 110 x 110 blocks count in LDS
 

@@ -231,6 +231,32 @@ __device__ __forceinline__ uint32_t find_block(const uint32_t* __restrict__ vcut
     return lo;
 }
 
+// The same answer from an LDS table: lut[i] = the largest b with vcut[b] <= i << shift,
+// then forward to the largest b with vcut[b] <= v (one or two reads when blocks span more
+// ids than a table entry).
+__device__ __forceinline__ uint32_t find_block_lut(const uint16_t* lut, uint32_t shift, const uint32_t* vcut,
+                                                   uint32_t nblocks, uint32_t v)
+{
+    uint32_t b = lut[v >> shift];
+    while (b + 1 < nblocks && vcut[b + 1] <= v) ++b;
+    return b;
+}
+
+// Copies a block table of at most kLdsBlocks entries and its cut points into LDS and
+// builds the lookup table; returns its shift.  Every thread of the workgroup calls it.
+__device__ __forceinline__ uint32_t stage_blocks(const WalkArgs& a, DevBlock* sblk, uint32_t* svcut, uint16_t* slut)
+{
+    for (uint32_t i = threadIdx.x; i < a.nblocks; i += blockDim.x) sblk[i] = a.blocks[i];
+    for (uint32_t i = threadIdx.x; i <= a.nblocks; i += blockDim.x) svcut[i] = a.vcut[i];
+    __syncthreads();
+    const uint32_t top = svcut[a.nblocks] ? svcut[a.nblocks] - 1 : 0;
+    uint32_t shift = 0;
+    while ((top >> shift) >= kLutSize) ++shift;
+    for (uint32_t i = threadIdx.x; i < kLutSize; i += blockDim.x)
+        slut[i] = (uint16_t)find_block(svcut, a.nblocks, i << shift);
+    return shift;
+}
+
 struct Row {
     const uint32_t* adj;
     uint64_t deg;
@@ -310,15 +336,13 @@ template <bool ONE, bool INTERP, bool LB, bool MIX>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_kernel(WalkArgs a)
 {
     // block-access trace: steps per current block, histogrammed in LDS
-    constexpr uint32_t kTrace = ONE ? 1u : kTraceLdsBlocks;
+    constexpr uint32_t kTrace = ONE ? 1u : (LB ? kLdsBlocks : kTraceLdsBlocks);
     __shared__ uint32_t trace[kTrace];
     __shared__ DevBlock sblk[LB ? kLdsBlocks : 1];
     __shared__ uint32_t svcut[LB ? kLdsBlocks + 1 : 1];
+    __shared__ uint16_t slut[LB ? kLutSize : 1];
     const bool lds_trace = a.block_steps && a.nblocks <= kTrace;
-    if (LB) {
-        for (uint32_t i = threadIdx.x; i < a.nblocks; i += blockDim.x) sblk[i] = a.blocks[i];
-        for (uint32_t i = threadIdx.x; i <= a.nblocks; i += blockDim.x) svcut[i] = a.vcut[i];
-    }
+    const uint32_t lut_shift = LB ? stage_blocks(a, sblk, svcut, slut) : 0u;
     if (a.block_steps)
         for (uint32_t i = threadIdx.x; i < kTrace; i += blockDim.x) trace[i] = 0;
     if (LB || a.block_steps) __syncthreads();
@@ -415,7 +439,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
                     rv = load_row(one, cur);
                 } else {
                     const DevBlock& cb = blocks[bc];
-                    if (cur - cb.start_vert >= cb.nverts) bc = find_block(vcut, a.nblocks, cur);
+                    if (cur - cb.start_vert >= cb.nverts)
+                        bc = LB ? find_block_lut(slut, lut_shift, vcut, a.nblocks, cur) : find_block(vcut, a.nblocks, cur);
                     const DevBlock& nb = blocks[bc];
                     if (nb.resident) rv = load_row(nb, cur);
                     else release = true;
@@ -967,15 +992,13 @@ __device__ __forceinline__ bool lane_search(Lane& L, const uint32_t* list, uint3
 template <bool ONE, bool INTERP, bool LB, bool MIX>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_lane_kernel(WalkArgs a)
 {
-    constexpr uint32_t kTrace = ONE ? 1u : kTraceLdsBlocks;
+    constexpr uint32_t kTrace = ONE ? 1u : (LB ? kLdsBlocks : kTraceLdsBlocks);
     __shared__ uint32_t trace[kTrace];
     __shared__ DevBlock sblk[LB ? kLdsBlocks : 1];
     __shared__ uint32_t svcut[LB ? kLdsBlocks + 1 : 1];
+    __shared__ uint16_t slut[LB ? kLutSize : 1];
     const bool lds_trace = a.block_steps && a.nblocks <= kTrace;
-    if (LB) {
-        for (uint32_t i = threadIdx.x; i < a.nblocks; i += blockDim.x) sblk[i] = a.blocks[i];
-        for (uint32_t i = threadIdx.x; i <= a.nblocks; i += blockDim.x) svcut[i] = a.vcut[i];
-    }
+    const uint32_t lut_shift = LB ? stage_blocks(a, sblk, svcut, slut) : 0u;
     if (a.block_steps)
         for (uint32_t i = threadIdx.x; i < kTrace; i += blockDim.x) trace[i] = 0;
     if (LB || a.block_steps) __syncthreads();
@@ -1168,7 +1191,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
             } else {
                 uint32_t nbc = bc;
                 const DevBlock& cb = blocks[bc];
-                if (L.cur - cb.start_vert >= cb.nverts) nbc = find_block(vcut, a.nblocks, L.cur);
+                if (L.cur - cb.start_vert >= cb.nverts)
+                    nbc = LB ? find_block_lut(slut, lut_shift, vcut, a.nblocks, L.cur) : find_block(vcut, a.nblocks, L.cur);
                 const DevBlock& nb = blocks[nbc];
                 L.bcp = nbc | (bc << 16);
                 if (nb.resident) {

@@ -198,3 +198,20 @@ def test_interpolation_search_order(gpu, graph):
         check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1, sampler=1)], 5, 4096, launch=L)
         check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 6, 2048, block_bytes=8192, nslots=3,
                    launch=L)
+
+
+@pytest.mark.parametrize("variant", [0, 2])
+def test_block_tables_in_lds_and_global(gpu, variant):
+    # up to 128 blocks the step kernels stage the table, its cut points and a vertex->block
+    # lookup table in LDS; beyond that they read the table from global memory.  Degrees
+    # stay below blocksize/4, so no single-vertex block is truncated (engine/cache.hpp:170-172)
+    beg, csr = G.random_undirected(8000, 24000, seed=17)
+    for block_bytes in (16384, 1024):
+        bl = Blocks(beg, csr, block_bytes)
+        print("block_bytes", block_bytes, "nblocks", bl.nblocks)
+        assert int(np.diff(beg.astype(np.int64)).max()) <= block_bytes // 4
+        assert (bl.nblocks <= 128) == (block_bytes == 16384)
+        check_case(beg, csr, [dict(algo=NODE2VEC, length=25, p=0.5, q=2.0)], 19, 4096, block_bytes=block_bytes,
+                   launch=dict(variant=variant, interp=True))
+        check_case(beg, csr, [dict(algo=SOPR, length=25, alpha=0.3, term=0.1)], 20, 4096, block_bytes=block_bytes,
+                   nslots=8, launch=dict(variant=variant))
