@@ -164,7 +164,9 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                 rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
                                               memory_bytes=mem, block_bytes=block_bytes, zero_threshold=zt,
                                               scheduler=sched, hashes=first, flags=flags,
-                                              launch=None if args.variant is None else dict(variant=args.variant),
+                                              launch=None if args.variant is None and not args.waves_per_cu else
+                                              dict(variant=2 if args.variant is None else args.variant,
+                                                   waves_per_cu=args.waves_per_cu),
                                               prefetch_depth=args.prefetch_depth, verbose=args.verbose)
                 if first:
                     hashes[(cap_gb, name)] = h

@@ -188,7 +188,7 @@ public:
             // both orders need on sampled Node2vec-style queries of this graph (timing on the
             // device would need a resident cache that is not there yet)
             auto tt = clk::now();
-            base = gw_launch{256, 0, cfg_.launch.variant,
+            base = gw_launch{256, cfg_.launch.waves_per_cu, cfg_.launch.variant,
                              prefer_interpolation(beg_, csr_, total_verts()) ? (uint32_t)GW_RUN_INTERP : 0u};
             r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();
             r.tuned = base;
