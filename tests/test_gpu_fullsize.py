@@ -1,0 +1,41 @@
+"""Parity at the full out-of-core size (Node2vec on RMAT scale 26, 3.5B arcs, 256 MB
+blocks), where the oracle cannot follow, through properties that do not depend on size:
+
+* on a symmetrized graph without termination every walker that starts on a vertex with
+  neighbours takes exactly `length` steps (a vertex a walk reaches has its predecessor as
+  a neighbour, so no later dead end), and the others none: steps == 40 x #non-isolated;
+* walks depend only on (seed, walker, hop, trial), so the per-walker path hashes are
+  identical across block schedulers, HBM caps (16 GB: every block resident once; 8 GB:
+  evictions and the zero-copy tail) and step kernel variants."""
+import numpy as np
+import pytest
+
+from gowalker_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+
+def test_node2vec_rmat26_out_of_core_invariants(gpu):
+    ctx = E.Context(0)
+    beg, csr = E.gen_rmat(ctx, 26, 1_800_000_000, seed=42, undirected=True, permute=True, pinned=True)
+    ctx.close()
+    b, c = beg.a, csr.a
+    n = len(b) - 1
+    active = int(np.count_nonzero(b[1:] != b[:-1]))
+    print("nverts", n, "narcs", len(c), "non-isolated", active, flush=True)
+    types = [dict(algo=0, length=40, p=0.5, q=2.0)]
+    ref = None
+    for name, sched, mem_gb, zt, launch in (("plain v0 16GB", E.SCHED_PLAIN, 16, 0, dict(variant=0)),
+                                           ("learned v2 16GB", E.SCHED_LEARNED, 16, 0, None),
+                                           ("learned v2 8GB", E.SCHED_LEARNED, 8, 0, None),
+                                           ("plain+zero v2 8GB", E.SCHED_PLAIN, 8, (256 << 20) // 3072, None)):
+        rep, h, _ = E.run_out_of_core(b, c, types, 42, n, memory_bytes=mem_gb << 30, block_bytes=256 << 20,
+                                      zero_threshold=zt, scheduler=sched, hashes=True, launch=launch)
+        print(name, "steps", rep["steps"], "uploads", rep["uploads"], "staged", rep["staged"], "zero-copy reads",
+              rep["zc_touches"], "wall %.2f s" % rep["wall_s"], flush=True)
+        assert rep["steps"] == 40 * active
+        if ref is None:
+            ref = h.copy()
+        else:
+            np.testing.assert_array_equal(h, ref)
+    assert rep["zc_touches"] > 0   # the 8 GB plain+zero walk ended with zero-copy reads
