@@ -168,8 +168,13 @@ class Context:
         self.nwalkers = n
         self.outputs = outputs
 
-    def run(self, block=0, waves_per_cu=0, variant=0, buckets=False, interp=False):
-        L = GwLaunch(block, waves_per_cu, variant, (RUN_BUCKETS if buckets else 0) | (GW_RUN_INTERP if interp else 0))
+    def run(self, block=0, waves_per_cu=0, variant=None, buckets=False, interp=False):
+        """One gw_run.  block 0 takes the whole launch from the autotuner (or 256 threads of
+        variant 0); naming a variant without a block runs it at 256 threads."""
+        if variant is not None and block == 0:
+            block = 256
+        L = GwLaunch(block, waves_per_cu, variant or 0,
+                     (RUN_BUCKETS if buckets else 0) | (GW_RUN_INTERP if interp else 0))
         s = GwRunStats()
         self._chk(lib().gw_run(self.h, C.byref(L), C.byref(s)))
         return {f: getattr(s, f) for f, _ in GwRunStats._fields_}

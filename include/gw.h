@@ -87,10 +87,13 @@ enum {
     GW_OUT_PATHS = 1u << 2   /* full paths (length+1 u32 per walker; small runs only) */
 };
 
-/* Launch shape of the step kernel (all shapes give bit-identical walks). */
+/* Launch shape of the step kernel (all shapes give bit-identical walks).  block == 0
+ * takes block, waves_per_cu, variant and the search order from gw_autotune (256 threads
+ * of variant 0 before any tuning); only the caller's other flags are kept. */
 typedef struct {
     uint32_t block;          /* threads per workgroup (64..1024, multiple of 64); 0 = tuned */
-    uint32_t waves_per_cu;   /* persistent grid = CUs * waves_per_cu * 64 / block; 0 = tuned */
+    uint32_t waves_per_cu;   /* persistent grid = CUs * waves_per_cu * 64 / block; 0 = as many
+                                workgroups as the CUs hold */
     uint32_t variant;        /* 0 = per-step lane loop, 1 = flattened per-probe loop,
                                 2 = lean per-load lane state machine (Node2vec and the SOPR
                                 proposal sampler; other walker types run variant 0) */

@@ -39,3 +39,38 @@ def test_node2vec_rmat26_out_of_core_invariants(gpu):
         else:
             np.testing.assert_array_equal(h, ref)
     assert rep["zc_touches"] > 0   # the 8 GB plain+zero walk ended with zero-copy reads
+
+
+def test_sopr_chung_lu_resident_invariants(gpu):
+    """SOPR (proposal sampler, termination 0.15) on the directed Chung-Lu graph of 41.6M
+    vertices and ~1.47B arcs, 4 walkers per vertex, fully resident: the visit counts sum to
+    the steps, and paths, visits and counters are identical whether the graph is one block
+    (descriptor in registers) or ~25 blocks of 256 MB (table in LDS), and for step kernel
+    variants 0 and 2."""
+    ctx = E.Context(0)
+    beg, csr = E.gen_chung_lu(ctx, 41_600_000, 1_470_000_000, 2.2, 3_000_000, 42, pinned=True)
+    ctx.close()
+    b, c = beg.a, csr.a
+    n = len(b) - 1
+    types = [dict(algo=E.SOPR, length=30, alpha=0.2, term=0.15)]
+    ref = None
+    for block_bytes, variant in ((1 << 34, 0), (1 << 34, 2), (256 << 20, 0), (256 << 20, 2)):
+        bl = E.Blocks(b, c, block_bytes)
+        with E.Context() as cx:
+            cx.set_blocks(bl)
+            cx.load_all()
+            cx.set_walk(types, 42)
+            cx.init_walkers(4 * n, 4, outputs=E.OUT_HASH | E.OUT_VISITS)
+            st = cx.run(variant=variant)
+            h, v = cx.hashes(), cx.visits()
+        print("blocks", bl.nblocks, "variant", variant, "steps", st["steps"], "trials", st["trials"], "probes",
+              st["probes"], "kernel %.1f ms" % st["kernel_ms"], flush=True)
+        assert st["walkers_left"] == 0
+        assert int(v.sum(dtype=np.uint64)) == st["steps"]
+        tot = (st["steps"], st["trials"], st["probes"])
+        if ref is None:
+            ref = (h.copy(), v.copy(), tot)
+        else:
+            np.testing.assert_array_equal(h, ref[0])
+            np.testing.assert_array_equal(v, ref[1])
+            assert tot == ref[2]
