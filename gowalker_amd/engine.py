@@ -298,7 +298,7 @@ def gen_chung_lu(ctx, nverts, nedges, gamma=2.2, max_degree=3000000, seed=42, de
         lib().gw_graph_free(ctx.h, g)
 
 
-SCHED_PLAIN, SCHED_LEARNED = 0, 1
+SCHED_PLAIN, SCHED_LEARNED, SCHED_SOWALKER = 0, 1, 2
 
 
 def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory_bytes=0, block_bytes=256 << 20,

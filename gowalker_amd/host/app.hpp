@@ -92,7 +92,7 @@ inline void print_help()
         "- alpha:         SOPR/SOSR second-order weight (default 0.2)\n"
         "- term:          geometric termination probability per step (default 0)\n"
         "- zero:          finish with zero-copy reads below blocksize/128/24 walkers\n"
-        "- GOWalker:      use the walker-count block scheduler (required)\n"
+        "- GOWalker | SOWalker: block scheduler, walker-count rule or simulated annealing (one required)\n"
         "- sched:         plain | learned (default plain)\n"
         "- seed:          Philox seed (default 42)\n"
         "- threadperblock: workgroup size of the step kernel (default: the engine's)\n"
@@ -140,10 +140,6 @@ inline int app_main(int argc, const char** argv, App app)
     if (go == so) {
         std::printf("Please select one walker from GOWalker and SOWalker!\n");
         return 0;
-    }
-    if (so) {
-        std::fprintf(stderr, "the SOWalker simulated-annealing scheduler is not part of this engine; use GOWalker\n");
-        return 1;
     }
     try {
         // preprocess/graph_converter.hpp:286-320: convert the text edge list unless its
@@ -214,14 +210,17 @@ inline int app_main(int argc, const char** argv, App app)
             wc.sources = sources.data();
         }
 
+        // GOWalker: the walker-count rule (or `sched learned`); SOWalker: simulated annealing
+        // over whole resident sets (apps/Node2vec.cu:157-163)
         std::unique_ptr<Scheduler> sched;
-        if (sched_name == "learned") sched.reset(new LearnedScheduler(engine.ctx(), LearnedConfig()));
+        if (so) sched.reset(new SOWalkerScheduler());
+        else if (sched_name == "learned") sched.reset(new LearnedScheduler(engine.ctx(), LearnedConfig()));
         else sched.reset(new PlainScheduler());
 
         RunReport r = engine.run(wc, *sched);
         std::string mname = (app == App::Node2vec ? std::string("node2vec") : std::string("SOPR")) + "_numwalks_" +
                             std::to_string(wc.nwalkers) + "_steps_" + std::to_string(length) + "_dataset_" + input;
-        if (app == App::Node2vec) mname += "_schedule_GOWalker";
+        if (app == App::Node2vec) mname += so ? "_schedule_SOWalker" : "_schedule_GOWalker";
         Metrics m(mname);
         m.set("Graph blocks", r.nblocks, true);
         m.set("Gpu memory blocks", r.nslots, true);

@@ -53,6 +53,8 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
             if (opts->prefetch_depth) lc.max_inflight = opts->prefetch_depth;
             ls = new LearnedScheduler(engine.ctx(), lc);
             sched.reset(ls);
+        } else if (opts->scheduler == GWH_SCHED_SOWALKER) {
+            sched.reset(new SOWalkerScheduler());
         } else {
             sched.reset(new PlainScheduler());
         }

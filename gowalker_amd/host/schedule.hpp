@@ -7,7 +7,10 @@
 // remaining walkers by zero-copy, engine/engine.hpp:593-634).
 #pragma once
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
+#include <limits>
+#include <random>
 #include <vector>
 
 #include "gw.h"
@@ -104,6 +107,144 @@ public:
         else buckets.push_back(best_c);
         return true;
     }
+};
+
+// The SOWalker scheduler (SOwalker_scheduler_t::choose_blocks, engine/schedule.hpp:460-580):
+// every iteration picks a whole resident set of `ncblock` blocks.  Start: the ncblock - 1
+// blocks holding the most walkers as their current block (column sums of W, :480-488),
+// plus the block that pairs the most walkers with them (:490-503).  Then simulated
+// annealing (:518-578): swap a random member for a random non-member; the score of a set
+// is the walkers whose (prev, cur) blocks both lie in it, per block of the set that is not
+// resident yet; better sets are kept, worse ones with probability exp(dy / T) (when
+// dy > 0 is not met but the new score is positive), T from 100 to 0.001 by x0.998 per
+// swap.  It never finishes by zero-copy (its schedule() always returns true, :626-634).
+// Differences, all deterministic: the annealing draws come from a seeded generator (the
+// reference seeds rand() with the time, :530), the swap budget C(nblocks, ncblock) is
+// computed without the int overflow of :444-456, ranking ties keep the lower block id,
+// and a set in which no walker can step is completed with the blocks of the fullest
+// bucket (the reference would choose it again forever); returning false (no set can make
+// progress) hands the walk to the engine's zero-copy tail.
+class SOWalkerScheduler : public Scheduler {
+public:
+    explicit SOWalkerScheduler(uint64_t seed = 42) : rng_(seed) {}
+    const char* name() const override { return "sowalker"; }
+
+    // Blocks ranked for the starting set: the first min(ncblock, nblocks) entries are it
+    // (the ncblock - 1 with the most walkers as their current block, then the block
+    // pairing the most walkers with those), the rest is the pool the annealing swaps from.
+    static std::vector<uint32_t> initial_order(const SchedInput& in)
+    {
+        const uint32_t nb = in.nblocks;
+        const uint64_t n = nb;
+        const uint32_t k = std::min(in.ncblock, nb);
+        std::vector<uint64_t> part(nb, 0);
+        for (uint32_t p = 0; p < nb; ++p)
+            for (uint32_t c = 0; c < nb; ++c) part[c] += in.W[p * n + c];
+        std::vector<uint32_t> idx(nb);
+        for (uint32_t b = 0; b < nb; ++b) idx[b] = b;
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return part[a] > part[b]; });
+        if (k == 0) return idx;
+        uint64_t most = 0;
+        uint32_t best = k - 1;
+        for (uint32_t pi = k - 1; pi < nb; ++pi) {
+            uint64_t w = 0;
+            for (uint32_t ci = 0; ci + 1 < k; ++ci) w += in.W[idx[pi] * n + idx[ci]] + in.W[(uint64_t)idx[ci] * n + idx[pi]];
+            if (w > most) { most = w; best = pi; }
+        }
+        std::swap(idx[k - 1], idx[best]);
+        return idx;
+    }
+
+    bool schedule(const SchedInput& in) override
+    {
+        const uint32_t nb = in.nblocks;
+        const uint64_t n = nb;
+        const uint32_t k = std::min(in.ncblock, nb);
+        if (k == 0) return false;
+        uint64_t total = 0;
+        for (uint64_t i = 0; i < n * n; ++i) total += in.W[i];
+        if (total == 0) return false;
+        std::vector<uint32_t> idx = initial_order(in);
+        std::vector<char> cached(nb, 0);
+        for (uint32_t b : buckets) cached[b] = 1;
+        std::vector<uint32_t> cand(idx.begin(), idx.begin() + k);
+        uint64_t S = 0;
+        uint32_t comm = 0;
+        for (uint32_t p : cand) {
+            comm += cached[p];
+            for (uint32_t c : cand) S += in.W[p * n + c];
+        }
+        const double inf = std::numeric_limits<double>::infinity();
+        double y = comm < k ? (double)S / (double)(k - comm) : inf;   // the reference divides by zero here
+        if (k < nb) {
+            // walkers paired with x inside the current set (both directions, diagonal once)
+            auto pair_sum = [&](uint32_t x) {
+                uint64_t t = 0;
+                for (uint32_t j : cand)
+                    if (j != x) t += in.W[(uint64_t)x * n + j] + in.W[(uint64_t)j * n + x];
+                return t;
+            };
+            double T = 100.0;
+            const uint64_t budget = swap_budget(nb, k);
+            for (uint64_t it = 0; it < budget && T > 0.001; ++it) {
+                const uint32_t pos = k + (uint32_t)(rng_() % (nb - k)), tp = (uint32_t)(rng_() % k);
+                const uint32_t a = cand[tp], b = idx[pos];
+                // S of the set with a swapped for b
+                const uint64_t without_a = S - pair_sum(a) - in.W[(uint64_t)a * n + a];
+                cand[tp] = b;
+                const uint64_t S2 = without_a + pair_sum(b) + in.W[(uint64_t)b * n + b];
+                const uint32_t comm2 = comm - cached[a] + cached[b];
+                const double y2 = comm2 < k ? (double)S2 / (double)(k - comm2) : 0.0;
+                bool keep = y2 > y;
+                if (!keep) {
+                    const double u = (double)(rng_() >> 11) * (1.0 / 9007199254740992.0);
+                    keep = y2 > 0 && u < std::exp((y2 - y) / T);
+                }
+                if (keep) {
+                    idx[pos] = a;   // a goes back to the pool in b's place
+                    S = S2;
+                    comm = comm2;
+                    y = y2;
+                } else {
+                    cand[tp] = a;
+                }
+                T *= 0.998;
+            }
+        }
+        if (S == 0) {
+            // no walker can step in this set: take the blocks of the fullest bucket
+            uint64_t bw = 0;
+            uint32_t bp = 0, bc = 0;
+            for (uint32_t p = 0; p < nb; ++p)
+                for (uint32_t c = 0; c < nb; ++c)
+                    if (in.W[p * n + c] > bw && (p == c || k >= 2)) { bw = in.W[p * n + c]; bp = p; bc = c; }
+            if (bw == 0) return false;
+            std::vector<uint32_t> s = {bp};
+            if (bc != bp) s.push_back(bc);
+            for (uint32_t b : idx) {
+                if (s.size() >= k) break;
+                if (b != bp && b != bc) s.push_back(b);
+            }
+            cand = s;
+        }
+        buckets = cand;
+        return true;
+    }
+
+    // min(C(n, k), 2^40): the reference's loop bound, without its int overflow
+    static uint64_t swap_budget(uint32_t n, uint32_t k)
+    {
+        const uint64_t cap = 1ull << 40;
+        uint64_t r = 1;
+        for (uint32_t i = 0; i < k; ++i) {
+            r = r * (n - i) / (i + 1);
+            if (r >= cap) return cap;
+        }
+        return r;
+    }
+
+private:
+    std::mt19937_64 rng_;
 };
 
 // Slot assignment of GOwalker_scheduler_t::copyblocks (engine/schedule.hpp:144-197): a

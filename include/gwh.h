@@ -12,7 +12,9 @@
 extern "C" {
 #endif
 
-enum { GWH_SCHED_PLAIN = 0, GWH_SCHED_LEARNED = 1 };
+/* plain: GOwalker_scheduler_t (engine/schedule.hpp:58-365); learned: the MLP scheduler;
+ * sowalker: SOwalker_scheduler_t's simulated annealing (engine/schedule.hpp:369-635) */
+enum { GWH_SCHED_PLAIN = 0, GWH_SCHED_LEARNED = 1, GWH_SCHED_SOWALKER = 2 };
 /* gwh_engine_opts.flags */
 enum {
     GWH_SWAP_PREFETCH = 1u << 0, /* learned: keep a staging slot once the cache is full */

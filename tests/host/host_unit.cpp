@@ -203,6 +203,45 @@ static void test_scheduler()
     CHECK(st.free_slots({0, 1, 4}) == 0 && st.place(5, {0, 1, 4, 5}) == -2);
 }
 
+// SOWalker (engine/schedule.hpp:460-580): the starting set by hand, then properties of the
+// annealed set: size, distinct valid blocks, a walker able to step, determinism per seed
+static void test_sowalker()
+{
+    // 5 blocks; walkers per (prev, cur): column sums 0:3 1:11 2:9 3:2 4:0
+    std::vector<uint64_t> W(25, 0);
+    auto w = [&](uint32_t p, uint32_t c, uint64_t x) { W[p * 5 + c] = x; };
+    w(1, 1, 4); w(2, 1, 7); w(1, 2, 9); w(0, 0, 3); w(3, 3, 2);
+    SchedInput in{W.data(), 5, 3, 0, nullptr, 0};
+    // top two by column sum: 1 (11), 2 (9); the third maximises walkers paired with {1, 2}:
+    // block 0 pairs none, 3 none, 4 none -> first candidate (0, ranked third) stays
+    auto idx = SOWalkerScheduler::initial_order(in);
+    CHECK(idx[0] == 1 && idx[1] == 2 && idx[2] == 0);
+    w(4, 2, 5);   // column 2 now holds 14 walkers, and block 4 pairs 5 of them with {2, 1}
+    idx = SOWalkerScheduler::initial_order(in);
+    CHECK(idx[0] == 2 && idx[1] == 1 && idx[2] == 4);
+    CHECK(SOWalkerScheduler::swap_budget(5, 3) == 10 && SOWalkerScheduler::swap_budget(200, 100) == (1ull << 40));
+    for (uint32_t k = 1; k <= 6; ++k) {
+        SchedInput ik{W.data(), 5, k, 0, nullptr, 0};
+        SOWalkerScheduler a(7), b(7);
+        CHECK(a.schedule(ik) && b.schedule(ik) && a.buckets == b.buckets);
+        std::vector<uint32_t> s = a.buckets;
+        std::sort(s.begin(), s.end());
+        CHECK(s.size() == std::min<uint32_t>(k, 5) && std::unique(s.begin(), s.end()) == s.end() && s.back() < 5);
+        uint64_t pairs = 0;
+        for (uint32_t p : s)
+            for (uint32_t c : s) pairs += W[p * 5 + c];
+        CHECK(pairs > 0);
+    }
+    // one slot and only cross-block walkers: no set can make progress
+    std::vector<uint64_t> X(9, 0);
+    X[0 * 3 + 2] = 6;
+    SchedInput one{X.data(), 3, 1, 0, nullptr, 0};
+    SOWalkerScheduler z;
+    CHECK(!z.schedule(one));
+    X[1 * 3 + 1] = 1;   // a diagonal bucket: that block alone
+    CHECK(z.schedule(one) && z.buckets == std::vector<uint32_t>{1});
+}
+
 // the engine's search-order model: evenly spread ids favour interpolation, lists of
 // clustered small ids (an unpermuted power-law graph) do not
 static void test_search_order_model()
@@ -235,6 +274,7 @@ int main(int argc, char** argv)
     test_format(dir);
     test_convert(dir);
     test_scheduler();
+    test_sowalker();
     test_search_order_model();
     std::printf("host_unit: all passed\n");
     return 0;

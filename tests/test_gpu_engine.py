@@ -33,7 +33,7 @@ def oracle_hashes(beg, csr, types, seed, n, wpv):
     return O.run(beg, csr, O.walk_cfg(seed, tc), n, wpv=wpv, visits=True)
 
 
-@pytest.mark.parametrize("sched", [E.SCHED_PLAIN, E.SCHED_LEARNED])
+@pytest.mark.parametrize("sched", [E.SCHED_PLAIN, E.SCHED_LEARNED, E.SCHED_SOWALKER])
 @pytest.mark.parametrize("memory_blocks", [3, 6, 100])
 def test_out_of_core_matches_oracle(gpu, sched, memory_blocks):
     beg, csr = G.rmat(13, 60000, seed=3, permute=True)
@@ -153,6 +153,14 @@ def test_apps_end_to_end(gpu, tmp_path):
     assert np.all(beg[len(tb):] == beg[-1])
     if len(tb) == len(beg):
         np.testing.assert_array_equal(np.fromfile(ht, dtype=np.uint64), o["hashes"])
+    # the SOWalker scheduler (simulated annealing over resident sets) walks the same paths
+    hs = str(tmp_path / "hs.bin")
+    r4 = subprocess.run([os.path.join(build, "Node2vec"), base + ".txt", "length", "15", "walkpersource", "2",
+                         "memory", "1", "blocksize", "1", "p", "0.5", "q", "2", "SOWalker", "seed", "5",
+                         "hashes", hs], capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    print(r4.stdout[-1000:], r4.stderr[-1000:])
+    assert r4.returncode == 0
+    np.testing.assert_array_equal(np.fromfile(hs, dtype=np.uint64), o["hashes"])
     # the reference requires one scheduler flag
     r2 = subprocess.run([os.path.join(build, "Node2vec"), base + ".txt", "length", "5"], capture_output=True,
                         text=True, timeout=60, cwd=str(tmp_path))
