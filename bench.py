@@ -132,10 +132,11 @@ PCIE_PEAK_GBS = 64.0   # PCIe Gen5 x16, one direction
 def ooc_schedulers(E, block_bytes, ablation=False, only=None):
     # plain: the reference's max_choose_blocks without the optional zero-copy tail;
     # plain_zero: the same with the reference's `zero` option (threshold blocksize/128/24
-    # walkers, apps/Node2vec.cu:100-104); learned: the MLP scheduler (with --ablation also
-    # with a staging slot once the cache is full, and with next-run labels)
+    # walkers, apps/Node2vec.cu:100-104); sowalker: the reference's simulated-annealing
+    # scheduler (engine/schedule.hpp:369-635); learned: the MLP scheduler (with --ablation
+    # also with a staging slot once the cache is full, and with next-run labels)
     s = [("plain", E.SCHED_PLAIN, 0, 0), ("plain_zero", E.SCHED_PLAIN, block_bytes // 128 // 24, 0),
-         ("learned", E.SCHED_LEARNED, 0, 0)]
+         ("sowalker", E.SCHED_SOWALKER, 0, 0), ("learned", E.SCHED_LEARNED, 0, 0)]
     if ablation:
         s += [("learned_swap_prefetch", E.SCHED_LEARNED, 0, 1), ("learned_myopic", E.SCHED_LEARNED, 0, 2)]
     if only:
@@ -197,7 +198,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
         ref = hashes[(cap_gb, names[0])]
         ident = all(bool((hashes[(cap_gb, n)] == ref).all()) for n in names)
         entry = {"memory_gb": cap_gb, "walks_identical": ident, "schedulers": res}
-        for other in ("plain", "plain_zero"):
+        for other in ("plain", "plain_zero", "sowalker"):
             if "learned" in res and other in res:
                 entry["learned_vs_%s_steps_per_s" % other] = res["learned"]["steps_per_s"] / res[other]["steps_per_s"]
                 entry["learned_vs_%s_pcie_bytes" % other] = res["learned"]["pcie_gb_total"] / res[other]["pcie_gb_total"]
