@@ -249,7 +249,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="node2vec_rmat26_ooc", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=42)
