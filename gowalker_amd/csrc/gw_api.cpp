@@ -65,6 +65,7 @@ struct gw_ctx {
     uint64_t* d_hash = nullptr;
     uint32_t* d_paths = nullptr;
     uint32_t* d_visits = nullptr;
+    uint32_t visit_parts = 1;              // visit-count arrays (summed by gw_read_visits)
     uint32_t* d_work = nullptr;
     uint32_t* d_bucket = nullptr;
     unsigned long long* d_trace = nullptr; // per-block steps of the last traced run
@@ -157,7 +158,7 @@ static void free_walkers(gw_ctx* c)
     dfree(c, c->d_work, c->nwalkers * 4, true);
     dfree(c, c->d_hash, c->nwalkers * 8, true);
     dfree(c, c->d_paths, c->nwalkers * (uint64_t)c->path_stride * 4, true);
-    dfree(c, c->d_visits, c->nverts_total * 4, true);
+    dfree(c, c->d_visits, c->nverts_total * 4 * c->visit_parts, true);
     c->d_cur = c->d_prev = c->d_meta = c->d_bpair = c->d_work = c->d_paths = c->d_visits = nullptr;
     c->d_hash = nullptr;
     c->nwalkers = 0;
@@ -730,11 +731,15 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
             c->d_paths = (uint32_t*)p;
         }
         if (outputs & GW_OUT_VISITS) {
-            if ((st = dalloc(c, &p, c->nverts_total * 4, true)) != GW_OK) { free_walkers(c); return st; }
+            // one count array per XCD when that costs at most 1/16 of the budget (or there is
+            // no budget): the step kernels' arrival counts then stay in one XCD's L2
+            const uint64_t one = c->nverts_total * 4;
+            c->visit_parts = (!c->cap || kVisitParts * one <= c->cap / 16) ? kVisitParts : 1u;
+            if ((st = dalloc(c, &p, one * c->visit_parts, true)) != GW_OK) { free_walkers(c); return st; }
             c->d_visits = (uint32_t*)p;
         }
     }
-    if (c->d_visits) HIPCHK(c, launch_fill_u32(c->d_visits, 0, c->nverts_total, c->s_compute));
+    if (c->d_visits) HIPCHK(c, launch_fill_u32(c->d_visits, 0, c->nverts_total * c->visit_parts, c->s_compute));
     c->outputs = outputs;
     uint32_t* d_src = nullptr;
     if (sources) {
@@ -893,6 +898,8 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     a.type1 = c->dt[1];
     a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta; a.bpair = c->d_bpair;
     a.hash = c->d_hash; a.paths = c->d_paths; a.path_stride = c->path_stride; a.visits = c->d_visits;
+    a.visits_stride = c->nverts_total;
+    a.visits_parts = c->d_visits ? c->visit_parts : 1u;
     a.worklist = all_res ? nullptr : c->d_work;
     a.nwork_dev = all_res ? nullptr : c->d_misc;
     a.scan = 0u;
@@ -996,7 +1003,7 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, uint32_t what, gw_launch* best
     std::vector<Snap> snaps = {{c->d_cur, nullptr, n * 4}, {c->d_prev, nullptr, n * 4}, {c->d_meta, nullptr, n * 4},
                                {c->d_bpair, nullptr, n * 4}};
     if (c->d_hash) snaps.push_back({c->d_hash, nullptr, n * 8});
-    if (c->d_visits) snaps.push_back({c->d_visits, nullptr, c->nverts_total * 4});
+    if (c->d_visits) snaps.push_back({c->d_visits, nullptr, c->nverts_total * 4 * c->visit_parts});
     gw_status st = GW_OK;
     auto cleanup = [&]() { for (auto& x : snaps) if (x.save) (void)hipFree(x.save); };
     for (auto& x : snaps) {
@@ -1132,6 +1139,10 @@ gw_status gw_read_visits(gw_ctx* c, uint32_t* out, uint64_t n)
 {
     if (!c || !out) return fail(c, GW_ERR_ARG, "null argument");
     if (n < c->nverts_total) return fail(c, GW_ERR_ARG, "visit buffer too small");
+    if (c->d_visits && c->visit_parts > 1) {
+        (void)hipSetDevice(c->device);
+        HIPCHK(c, launch_fold_visits(c->d_visits, c->nverts_total, c->visit_parts, c->s_compute));
+    }
     return readback(c, out, c->d_visits, c->nverts_total * 4, "visits");
 }
 

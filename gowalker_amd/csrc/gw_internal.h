@@ -65,7 +65,9 @@ struct WalkArgs {
     uint64_t* hash;           // optional
     uint32_t* paths;          // optional, path_stride entries per walker
     uint32_t path_stride;
-    uint32_t* visits;         // optional
+    uint32_t* visits;         // optional: visits_parts arrays of visits_stride counters
+    uint64_t visits_stride;
+    uint32_t visits_parts;    // a workgroup counts into array blockIdx.x % visits_parts
     // work
     const uint32_t* worklist; // walker ids; nullptr = identity (finished walkers are skipped)
     uint32_t nwork;
@@ -117,6 +119,8 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
                        uint32_t block, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s);
+// v[i] = sum over p < parts of v[p * n + i]; the other arrays are cleared
+hipError_t launch_fold_visits(uint32_t* v, uint64_t n, uint32_t parts, hipStream_t s);
 
 // Small transfers (multiples of 4 bytes, up to a few MB) on the context's compute stream
 // through its pinned ring and a copy kernel (no DMA engine).  h2d returns once the copy is
@@ -129,6 +133,7 @@ hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBl
 constexpr uint32_t kTraceLdsBlocks = 2048;   // per-workgroup LDS histogram size of the trace
 constexpr uint32_t kLdsBlocks = 128;         // block tables up to this size are staged in LDS by the step kernel
 constexpr uint32_t kLutSize = 2048;          // entries of the LDS vertex -> block lookup table
+constexpr uint32_t kVisitParts = 8;          // visit-count arrays, one per XCD when HBM allows
 constexpr uint32_t kNumFeatures = 12;
 constexpr uint32_t kFilterLdsBuckets = 12288;  // bucket matrices up to 110 x 110 blocks count in LDS
 
