@@ -1025,8 +1025,7 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, uint32_t what, gw_launch* best
                                      {1024, 0, 0, 0}, {64, 0, 1, 0},   {128, 0, 1, 0},  {256, 0, 1, 0},
                                      {64, 0, 0, I},   {128, 0, 0, I},  {256, 0, 0, I},  {512, 0, 0, I},
                                      {1024, 0, 0, I}, {64, 0, 2, 0},   {256, 0, 2, 0},  {1024, 0, 2, 0},
-                                     {64, 0, 2, I},   {256, 0, 2, I},  {1024, 0, 2, I}, {256, 16, 2, 0},
-                                     {1024, 16, 2, 0}, {256, 16, 2, I}, {1024, 16, 2, I}};
+                                     {64, 0, 2, I},   {256, 0, 2, I},  {1024, 0, 2, I}};
     double bms = 1e300;
     gw_launch bl = cand[2];
     for (const gw_launch& L : cand) {

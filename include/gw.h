@@ -204,8 +204,7 @@ gw_status gw_run(gw_ctx* ctx, const gw_launch* launch, gw_run_stats* stats);
 
 /* Launch-shape autotuner (no reference counterpart; the reference hard-codes 512 x 512,
  * apps/Node2vec.cu:47-48): times the step kernel on the first `sample` work items for each
- * candidate kernel variant, workgroup size (for variant 2 also at 16 waves per CU) and
- * membership search order (GW_RUN_INTERP) that
+ * candidate kernel variant, workgroup size and membership search order (GW_RUN_INTERP) that
  * `what` selects (GW_TUNE_*; 256 threads, variant 0, bisection is always a candidate),
  * restoring walker state after every trial, and makes the fastest the shape of later
  * gw_run calls that pass block == 0 (best->flags carries the search order).  Time it on
