@@ -74,3 +74,43 @@ def test_sopr_chung_lu_resident_invariants(gpu):
             np.testing.assert_array_equal(h, ref[0])
             np.testing.assert_array_equal(v, ref[1])
             assert tot == ref[2]
+
+
+def test_node2vec_rmat20_full_size_matches_oracle(gpu):
+    """The resident Node2vec workload at its full size (RMAT scale 20, ~31M arcs, 2^20
+    walkers x 80 hops, p 2, q 0.5), bit for bit against the oracle's replay: path hashes,
+    final states, and step / trial / probe counts, for the autotuned launch and for both
+    search orders of both resident kernel variants."""
+    import _oracle as O
+
+    ctx = E.Context(0)
+    beg, csr = E.gen_rmat(ctx, 20, 16 * 1024 * 1024, seed=42, undirected=True, permute=False)
+    ctx.close()
+    n = len(beg) - 1
+    types = [dict(algo=E.NODE2VEC, length=80, p=2.0, q=0.5)]
+    want = {}
+    for search in (0, 1):
+        cfg = O.walk_cfg(42, O.type_cfg(O.N2V, 80, p=2.0, q=0.5, search=search))
+        want[search] = O.run(beg, csr, cfg, n, nthreads=16)
+    print("oracle totals", want[0]["totals"], want[1]["totals"], flush=True)
+    bl = E.Blocks(beg, csr, 1 << 34)
+    with E.Context() as cx:
+        cx.set_blocks(bl)
+        cx.load_all()
+        cx.set_walk(types, 42)
+        cx.init_walkers(n, 1, outputs=E.OUT_HASH)
+        cx.autotune(1 << 18)
+        for launch in (dict(block=0), dict(variant=0), dict(variant=2), dict(variant=0, interp=True),
+                       dict(variant=2, interp=True)):
+            cx.init_walkers(n, 1, outputs=E.OUT_HASH)
+            st = cx.run(**launch)
+            o = want[1 if launch.get("interp") else 0]
+            print(launch, {k: st[k] for k in ("steps", "trials", "probes")}, "kernel %.2f ms" % st["kernel_ms"],
+                  flush=True)
+            if launch != dict(block=0):
+                assert (st["steps"], st["trials"], st["probes"]) == (o["totals"]["steps"], o["totals"]["trials"],
+                                                                     o["totals"]["probes"])
+            else:
+                assert st["steps"] == o["totals"]["steps"] and st["trials"] == o["totals"]["trials"]
+            np.testing.assert_array_equal(cx.hashes(), o["hashes"])
+            np.testing.assert_array_equal(cx.state(), o["state"])
