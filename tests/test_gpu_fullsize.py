@@ -46,7 +46,9 @@ def test_sopr_chung_lu_resident_invariants(gpu):
     vertices and ~1.47B arcs, 4 walkers per vertex, fully resident: the visit counts sum to
     the steps, and paths, visits and counters are identical whether the graph is one block
     (descriptor in registers) or ~25 blocks of 256 MB (table in LDS), and for step kernel
-    variants 0 and 2."""
+    variants 0 and 2 -- and they are the oracle's, replayed on the host cores."""
+    import _oracle as O
+
     ctx = E.Context(0)
     beg, csr = E.gen_chung_lu(ctx, 41_600_000, 1_470_000_000, 2.2, 3_000_000, 42, pinned=True)
     ctx.close()
@@ -69,6 +71,13 @@ def test_sopr_chung_lu_resident_invariants(gpu):
         assert int(v.sum(dtype=np.uint64)) == st["steps"]
         tot = (st["steps"], st["trials"], st["probes"])
         if ref is None:
+            o = O.run(b, c, O.walk_cfg(42, O.type_cfg(O.SOPR, 30, alpha=0.2, term=0.15)), 4 * n, wpv=4, visits=True,
+                      state=False, nthreads=16)
+            print("oracle totals", o["totals"], flush=True)
+            np.testing.assert_array_equal(h, o["hashes"])
+            np.testing.assert_array_equal(v, o["visits"])
+            assert tot == (o["totals"]["steps"], o["totals"]["trials"], o["totals"]["probes"])
+            del o
             ref = (h.copy(), v.copy(), tot)
         else:
             np.testing.assert_array_equal(h, ref[0])
