@@ -21,7 +21,7 @@
 // The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: the model is
 // ignored while it has too few samples and, for a cool-down period, whenever its recent
 // predictions of the steps a load buys are off by more than a factor e on average (the
-// judge).  Walks stay bit-identical either way (counter-based
+// error monitor).  Walks stay bit-identical either way (counter-based
 // RNG).
 #pragma once
 #include <algorithm>
@@ -51,7 +51,7 @@ struct LearnedConfig {
     bool swap_prefetch = false;     // full cache: keep one slot for staging the next load
                                     // (overlaps every upload, costs a slot and missed copies)
     bool myopic_label = false;      // label a load with the next run's steps only
-    double judge_error = 1.0;       // mean |log error| of recent predictions above this => fall back
+    double error_limit = 1.0;       // mean |log error| of recent predictions above this => fall back
     uint32_t cooldown = 8;
 };
 
@@ -224,12 +224,12 @@ private:
         o.live = false;
         const float y = std::log1p((float)o.steps);
         add_sample(o.f.data(), y);
-        // judge: how far the model's prediction for this load was from what it bought
+        // error monitor: how far the model's prediction for this load was from what it bought
         if (o.pred >= 0.f) {
             const double e = std::fabs((double)o.pred - (double)y);
             err_ = err_n_ ? 0.8 * err_ + 0.2 * e : e;
             ++err_n_;
-            if (err_n_ >= 4 && err_ > cfg_.judge_error) {
+            if (err_n_ >= 4 && err_ > cfg_.error_limit) {
                 cooldown_ = cfg_.cooldown;
                 err_n_ = 0;
                 ++fallbacks_;
