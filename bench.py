@@ -114,15 +114,31 @@ def cpu_baseline(beg, csr, cfg, seconds, threads):
     r, dt = timed(False, threads, seconds, sample)
     r1, dt1 = timed(False, 1, seconds / 4, sample // 8)
     rl, dtl = timed(True, threads, seconds / 4, sample)
+    # this build's own algorithm on the host cores: the oracle's Philox replay (same walks as
+    # the GPU), on a walker count sized from a short trial run to about seconds / 4
+    if t["algo"] == 0:
+        tc = O.type_cfg(O.N2V, t["length"], p=t.get("p", 1.0), q=t.get("q", 1.0), term=t.get("term", 0.0))
+    else:
+        tc = O.type_cfg(O.SOPR, t["length"], alpha=t.get("alpha", 0.2), term=t.get("term", 0.0))
+    wc = O.walk_cfg(42, tc)
+    nw = 65536
+    t0 = time.perf_counter()
+    O.run(beg, csr, wc, nw, sources=src[:nw], state=False, nthreads=threads)
+    nw = int(min(sample, max(nw, nw * (seconds / 4) / max(time.perf_counter() - t0, 1e-3))))
+    t0 = time.perf_counter()
+    rp = O.run(beg, csr, wc, nw, sources=src[:nw], state=False, nthreads=threads)
+    dtp = time.perf_counter() - t0
     return dict(value=r["moves"] / dt, unit="steps/s", cores=threads, kind="port",
                 sample="%d of %d random-source walkers run to completion in %.1f s (%d moves); C transcription of "
                        "the reference CPU step (engine/engine.hpp:158-404, fixed-mode RandNum), OpenMP"
                        % (r["done"], sample, dt, r["moves"]),
                 value_1core=r1["moves"] / dt1,
                 value_literal_randnum=rl["moves"] / dtl,
+                value_philox_replay=rp["totals"]["steps"] / dtp,
                 note="value_1core: same code on one thread (%d walkers); value_literal_randnum: the reference's "
-                     "type-punned inlineRandPair (util/util.hpp:291-301), nearly first-order sampling"
-                     % r1["done"])
+                     "type-punned inlineRandPair (util/util.hpp:291-301), nearly first-order sampling; "
+                     "value_philox_replay: this build's walk (Philox draws, integer thresholds, the GPU's samplers) "
+                     "replayed on the same cores (%d walkers)" % (r1["done"], nw))
 
 
 LDS_BLOCKS = 128          # block tables up to this size are staged in LDS by the step kernel
