@@ -137,13 +137,22 @@ public:
             }
             return;
         }
-        // swap: stage the best remaining block into the spare slot, unless the walk is
-        // near its zero-copy tail (a wasted copy would cost more than it saves)
-        if (!cfg_.swap_prefetch || !use_model_ || cooldown_ > 0 || cand.empty()) return;
-        const uint32_t p = cand[0];
-        const gw_block_desc& d = in.blocks[p];
-        const double bytes = ((double)d.nverts + 1) * 8 + (double)d.nedges * 4;
-        if (std::expm1((double)std::max(0.f, pred_[p])) * zc_bytes() > 2.0 * bytes) out.push_back(p);
+        // swap: stage into the spare slot the block the plain rule would load next on the
+        // current bucket matrix, so that when it does, the copy has overlapped this run
+        if (!cfg_.swap_prefetch) return;
+        PlainScheduler next;
+        next.buckets = buckets;
+        SchedInput pin = in;
+        pin.ncblock = cap;
+        uint64_t mx = 0;
+        for (uint32_t b = 0; b < nb; ++b) {
+            const gw_block_desc& d = in.blocks[b];
+            mx = std::max<uint64_t>(mx, ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4);
+        }
+        pin.zero_threshold = std::max<uint64_t>(in.zero_threshold, (uint64_t)zero_floor((double)mx));
+        if (!next.schedule(pin)) return;
+        for (uint32_t b : next.buckets)
+            if (!skip[b]) { out.push_back(b); return; }
     }
 
     void feedback(const SchedInput& in, const IterationOutcome& out) override
