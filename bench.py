@@ -277,7 +277,7 @@ def main():
                     help="step kernel: 0 per-step lane loop, 1 flattened state machine, 2 lean per-load lanes "
                          "(default: the autotuner's pick for resident workloads, 2 out of core)")
     ap.add_argument("--prefetch-depth", type=int, default=0,
-                    help="out-of-core learned scheduler: staged block copies in flight (0: the library's 4)")
+                    help="out-of-core learned scheduler: staged block copies in flight (0: the library's 8)")
     ap.add_argument("--verbose", action="store_true", help="out-of-core: the engine logs every iteration to stderr")
     ap.add_argument("--interp", action="store_true",
                     help="resident workloads with a fixed shape: membership by interpolation search")

@@ -51,6 +51,7 @@ struct gw_ctx {
     std::vector<hipEvent_t> slot_ev0, slot_ev1;   // start / end of the slot's last copy
     std::vector<uint8_t> slot_wait;       // a run must wait for the slot's copy
     std::vector<uint8_t> slot_timed;      // copy time not yet added to tot.h2d_ms
+    std::vector<uint8_t> slot_read;       // a block in the slot has been readable by kernels
 
     // walk configuration
     bool walk_set = false;
@@ -198,6 +199,7 @@ static void free_slots(gw_ctx* c)
     c->slot_ev1.clear();
     c->slot_wait.clear();
     c->slot_timed.clear();
+    c->slot_read.clear();
     c->nslots = 0;
     for (auto& b : c->blk_slot) b = -1;
 }
@@ -468,6 +470,7 @@ gw_status gw_reserve_slots(gw_ctx* c, uint32_t nslots)
     c->slot_ev1.assign(nslots, nullptr);
     c->slot_wait.assign(nslots, 0);
     c->slot_timed.assign(nslots, 0);
+    c->slot_read.assign(nslots, 0);
     c->nslots = nslots;
     for (uint32_t s = 0; s < nslots; ++s) {
         void* p;
@@ -496,8 +499,12 @@ static void block_nonresident(gw_ctx* c, uint32_t blk)
 }
 
 // Starts the copy of block `blk` into `slot` on the copy stream (no host wait).  The
-// slot's previous block stops being readable at once; the copy waits for kernels already
-// enqueued on the compute stream, which may still read that block.
+// slot's previous block stops being readable at once; if the slot ever held a readable
+// block and the compute stream still has work queued, the copy waits for that work, which
+// may still read it.  A slot that only ever held staged, uncommitted copies has no
+// readers.  The wait is left out whenever it is not needed: a copy queued behind a
+// cross-stream wait has been measured to run at half the link rate for its whole length
+// (15 of 53 block copies of a 16 GB fill, 8.8 ms instead of 4.7 ms).
 static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const uint32_t* csr)
 {
     const gw_block_desc& d = c->desc[blk];
@@ -509,8 +516,10 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
     c->slot_blk[slot] = -1;
     c->slot_staged[slot] = -1;
     collect_copy_time(c, slot, true);   // the slot's events are about to be re-recorded
-    HIPCHK(c, hipEventRecord(c->ev_c0, c->s_compute));
-    HIPCHK(c, hipStreamWaitEvent(c->s_copy, c->ev_c0, 0));
+    if (c->slot_read[slot] && hipStreamQuery(c->s_compute) != hipSuccess) {
+        HIPCHK(c, hipEventRecord(c->ev_c0, c->s_compute));
+        HIPCHK(c, hipStreamWaitEvent(c->s_copy, c->ev_c0, 0));
+    }
     const uint64_t bb = ((uint64_t)d.nverts + 1) * 8, cb = d.nedges * 4;
     HIPCHK(c, hipEventRecord(c->slot_ev0[slot], c->s_copy));
     HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, c->s_copy));
@@ -528,6 +537,7 @@ static void make_resident(gw_ctx* c, uint32_t slot, uint32_t blk)
     c->slot_blk[slot] = (int32_t)blk;
     c->blk_slot[blk] = (int32_t)slot;
     c->slot_wait[slot] = 1;   // the next run's kernels wait for the copy (device-side)
+    c->slot_read[slot] = 1;
     DevBlock& h = c->hblk[blk];
     h.beg = c->slot_beg[slot];
     h.csr = c->slot_csr[slot];

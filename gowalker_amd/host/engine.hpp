@@ -183,21 +183,33 @@ public:
         std::vector<uint8_t> staged(nb, 0);
         std::vector<uint32_t> pf;
         gw_launch base = cfg_.launch;
-        if (base.block == 0) {
-            // the membership search order is chosen before the first run from the probes
-            // both orders need on sampled Node2vec-style queries of this graph (timing on the
-            // device would need a resident cache that is not there yet)
-            auto tt = clk::now();
-            // the lane kernel runs 16 waves per CU: each lane keeps a few adjacency lines in
-            // use, and half the resident waves keep that working set nearer the L2 (out of
-            // core: -2% walk time at a 16 GB cap, -13% step-kernel time at 8 GB)
-            const uint32_t wpc = cfg_.launch.waves_per_cu ? cfg_.launch.waves_per_cu
-                                                           : (cfg_.launch.variant == 2 ? 16u : 0u);
-            base = gw_launch{256, wpc, cfg_.launch.variant,
-                             prefer_interpolation(beg_, csr_, total_verts()) ? (uint32_t)GW_RUN_INTERP : 0u};
-            r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();
-            r.tuned = base;
-        }
+        bool pick_shape = base.block == 0;
+        // launch of the next run; on the first call with no shape given, the membership
+        // search order is chosen from the probes both orders need on sampled Node2vec-style
+        // queries of this graph (timing on the device would need a resident cache that is
+        // not there yet).  That host work happens after the first copies are queued, so it
+        // overlaps them.
+        auto next_launch = [&]() {
+            if (pick_shape) {
+                auto tt = clk::now();
+                // the lane kernel runs 16 waves per CU: each lane keeps a few adjacency lines
+                // in use, and half the resident waves keep that working set nearer the L2 (out
+                // of core: -2% walk time at a 16 GB cap, -13% step-kernel time at 8 GB)
+                const uint32_t wpc = cfg_.launch.waves_per_cu ? cfg_.launch.waves_per_cu
+                                                               : (cfg_.launch.variant == 2 ? 16u : 0u);
+                base = gw_launch{256, wpc, cfg_.launch.variant,
+                                 prefer_interpolation(beg_, csr_, total_verts()) ? (uint32_t)GW_RUN_INTERP : 0u};
+                r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();
+                r.tuned = base;
+                pick_shape = false;
+            }
+            gw_launch l = base;
+            l.flags |= sched.run_flags() | GW_RUN_BUCKETS;
+            return l;
+        };
+        if (cfg_.verbose)
+            std::fprintf(stderr, "[engine] %.1f ms walkers and slots ready\n",
+                         std::chrono::duration<double, std::milli>(clk::now() - t0).count());
         for (uint64_t it = 0;; ++it) {
             ck(gw_bucket_counts(ctx_, W.data(), W.size()), ctx_, "gw_bucket_counts");
             uint64_t left = 0;
@@ -216,8 +228,6 @@ public:
             IterationOutcome out;
             out.iteration = it;
             gw_run_stats st{};
-            gw_launch launch = base;
-            launch.flags |= sched.run_flags() | GW_RUN_BUCKETS;
             if (sched.schedule(in)) {
                 uint64_t bytes = 0;
                 // staged copies the scheduler wants become resident (the run waits for them)
@@ -256,6 +266,7 @@ public:
                 }
                 for (uint32_t b = 0; b < nb; ++b) staged[b] = slots_.blk_stg[b] >= 0 ? 1 : 0;
                 r.sched_ms += std::chrono::duration<double, std::milli>(clk::now() - ts).count();
+                const gw_launch launch = next_launch();
                 ck(gw_run(ctx_, &launch, &st), ctx_, "gw_run");
                 out.h2d_bytes = bytes;
             } else {
@@ -267,6 +278,7 @@ public:
                     ck(gw_map_host_block(ctx_, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,
                        "gw_map_host_block");
                 }
+                const gw_launch launch = next_launch();
                 ck(gw_run(ctx_, &launch, &st), ctx_, "gw_run (zero-copy)");
                 ck(gw_unmap_host_blocks(ctx_), ctx_, "gw_unmap_host_blocks");
                 r.zero_copy_runs++;

@@ -46,8 +46,10 @@ struct LearnedConfig {
     double zc_read_bytes = 64;      // PCIe bytes of one zero-copy read (row, candidate or probe)
     double tail_margin = 1.0;       // zero-copy once steps x zc bytes x margin < block bytes
     double tail_guard = 64;         // ... and the work left x zc bytes < guard x block bytes
-    uint32_t max_inflight = 4;      // staged copies in flight at once (a deeper copy queue
-                                    // makes the host block when it enqueues)
+    uint32_t max_inflight = 8;      // staged copies in flight at once: more blocks land per
+                                    // run than with 4 (C3 16 GB: 0.471 s vs 0.478 s); deeper
+                                    // queues make a run wait for a wanted block queued behind
+                                    // others (0.49 s at 16 and 53)
     bool swap_prefetch = false;     // full cache: keep one slot for staging the next load
                                     // (overlaps every upload, costs a slot and missed copies)
     bool myopic_label = false;      // label a load with the next run's steps only

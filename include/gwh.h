@@ -32,7 +32,7 @@ typedef struct {
     gw_launch launch;
     int32_t device;
     uint32_t flags;            /* GWH_* flags */
-    uint32_t prefetch_depth;   /* learned scheduler: staged block copies in flight (0 = 4) */
+    uint32_t prefetch_depth;   /* learned scheduler: staged block copies in flight (0 = 8) */
     uint32_t reserved;
 } gwh_engine_opts;
 
