@@ -67,6 +67,13 @@ CONFIGS = {
 }
 
 
+def search_mode(launch):
+    """The step kernels' search-mode template argument as rocprof prints it: bit 0
+    interpolated guesses, bit 1 window reads."""
+    launch = launch or {}
+    return "%du" % ((1 if launch.get("interp") else 0) | (2 if launch.get("window") else 0))
+
+
 def algorithmic_bytes(st, nwalkers, visits):
     """Bytes the step kernel must move by the algorithm's definition:
     per walker: worklist id 4 + state read 16 + state write 16 + the two adjacency rows
@@ -181,9 +188,10 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                 rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
                                               memory_bytes=mem, block_bytes=block_bytes, zero_threshold=zt,
                                               scheduler=sched, hashes=first, flags=flags,
-                                              launch=None if args.variant is None and not args.waves_per_cu else
+                                              launch=None if args.variant is None and not args.waves_per_cu
+                                              and not args.window else
                                               dict(variant=2 if args.variant is None else args.variant,
-                                                   waves_per_cu=args.waves_per_cu),
+                                                   waves_per_cu=args.waves_per_cu, window=args.window),
                                               prefetch_depth=args.prefetch_depth, verbose=args.verbose)
                 if first:
                     hashes[(cap_gb, name)] = h
@@ -207,7 +215,8 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                              zero_copy_steps=r0["zero_copy_steps"], kernel_ms=r0["kernel_ms"],
                              copy_ms=r0["h2d_ms"], sched_ms=r0["sched_ms"], learned_decisions=r0["learned_decisions"],
                              tail_decisions=r0["tail_decisions"], fallbacks=r0["fallbacks"], nslots=r0["nslots"],
-                             nblocks=r0["nblocks"], steps=r0["steps"], zero_threshold=zt)
+                             nblocks=r0["nblocks"], steps=r0["steps"], trials=r0["trials"], probes=r0["probes"],
+                             zero_threshold=zt)
             if learned_reps is None and (name == "learned" or (args.schedulers and name == args.schedulers[0])):
                 learned_reps = reps
         names = [n for n, _, _, _ in ooc_schedulers(E, block_bytes, args.ablation, args.schedulers)]
@@ -246,7 +255,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "%s<false, %s, %s, %s>" % ("walk_lane_kernel" if (r0.get("tuned") or {}).get("variant") == 2
                                                            else "walk_kernel",
-                                                           "true" if (r0.get("tuned") or {}).get("interp") else "false",
+                                                           search_mode(r0.get("tuned")),
                                                            "true" if r0["nblocks"] <= LDS_BLOCKS else "false",
                                                            "true" if len(cfg["types"]) == 2 else "false"),
                      "kernel_ms": r0["kernel_ms"] / launches, "algorithmic_bytes": abytes,
@@ -281,6 +290,8 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="out-of-core: the engine logs every iteration to stderr")
     ap.add_argument("--interp", action="store_true",
                     help="resident workloads with a fixed shape: membership by interpolation search")
+    ap.add_argument("--window", action="store_true",
+                    help="membership probes read aligned 64-byte windows (resident: with a fixed shape)")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
     ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
     ap.add_argument("--ablation", action="store_true", help="out-of-core: also run scheduler variants")
@@ -324,7 +335,8 @@ def main():
     outputs = E.OUT_VISITS if visits else E.OUT_HASH
     if args.variant is None:
         args.variant = 0
-    launch = dict(block=args.block, waves_per_cu=args.waves_per_cu, variant=args.variant, interp=args.interp)
+    launch = dict(block=args.block, waves_per_cu=args.waves_per_cu, variant=args.variant, interp=args.interp,
+                  window=args.window)
 
     def one_step():
         ctx.init_walkers(nwalkers, cfg["wpv"], outputs=outputs)
@@ -388,7 +400,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "walk_flat_kernel<true>" if args.variant == 1 else
                                "%s<true, %s, false, %s>" % ("walk_lane_kernel" if args.variant == 2 else "walk_kernel",
-                                                            "true" if (tuned or {}).get("interp") else "false",
+                                                            search_mode(tuned or launch),
                                                             "true" if len(cfg["types"]) == 2 else "false"),
                      "kernel_ms": avg_kms, "algorithmic_bytes": abytes,
                      "sector_bytes": sector_bytes(st0, nwalkers, visits)},

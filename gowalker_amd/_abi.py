@@ -13,7 +13,7 @@ GW_OK = 0
 GW_ERR = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "STATE", -5: "BUDGET", -6: "IO", -7: "FAULT"}
 GW_NODE2VEC, GW_SOPR = 0, 1
 GW_OUT_VISITS, GW_OUT_HASH, GW_OUT_PATHS = 1, 2, 4
-GW_RUN_BUCKETS, GW_RUN_TRACE, GW_RUN_INTERP = 1, 2, 4
+GW_RUN_BUCKETS, GW_RUN_TRACE, GW_RUN_INTERP, GW_RUN_WINDOW = 1, 2, 4, 8
 GW_GEN_UNDIRECTED, GW_GEN_PERMUTE, GW_GEN_DEDUP = 1, 2, 4
 
 

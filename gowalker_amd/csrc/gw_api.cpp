@@ -456,7 +456,10 @@ gw_status gw_reserve_slots(gw_ctx* c, uint32_t nslots)
     uint64_t mb = 0, me = 0;
     for (auto& d : c->desc) { mb = std::max<uint64_t>(mb, (uint64_t)d.nverts + 1); me = std::max<uint64_t>(me, d.nedges); }
     c->slot_beg_bytes = (mb * 8 + 255) & ~255ULL;
-    c->slot_csr_bytes = (me * 4 + 16 + 255) & ~255ULL;   // 16 B tail pad
+    // a block's edge words start 4 x (start_edge mod 16) bytes into the slot, so every
+    // word keeps its offset modulo 64 bytes from the whole edge array (the window search
+    // reads aligned 64-byte pieces); 64 B of head room and 64 B of tail pad
+    c->slot_csr_bytes = (me * 4 + 128 + 255) & ~255ULL;
     uint64_t need = (uint64_t)nslots * (c->slot_beg_bytes + c->slot_csr_bytes);
     if (c->cap && c->held + need > c->cap)
         return fail(c, GW_ERR_BUDGET, "%u slots of %llu B plus %llu B held exceed the %llu B cache budget", nslots,
@@ -498,6 +501,12 @@ static void block_nonresident(gw_ctx* c, uint32_t blk)
     else { d.beg = nullptr; d.csr = nullptr; d.resident = 0; }
 }
 
+// where block blk's edge words go in a slot (see gw_reserve_slots)
+static uint32_t* slot_edges(gw_ctx* c, uint32_t slot, uint32_t blk)
+{
+    return c->slot_csr[slot] + (c->desc[blk].start_edge & 15);
+}
+
 // Starts the copy of block `blk` into `slot` on the copy stream (no host wait).  The
 // slot's previous block stops being readable at once; if the slot ever held a readable
 // block and the compute stream still has work queued, the copy waits for that work, which
@@ -523,7 +532,7 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
     const uint64_t bb = ((uint64_t)d.nverts + 1) * 8, cb = d.nedges * 4;
     HIPCHK(c, hipEventRecord(c->slot_ev0[slot], c->s_copy));
     HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, c->s_copy));
-    if (cb) HIPCHK(c, hipMemcpyAsync(c->slot_csr[slot], csr, cb, hipMemcpyHostToDevice, c->s_copy));
+    if (cb) HIPCHK(c, hipMemcpyAsync(slot_edges(c, slot, blk), csr, cb, hipMemcpyHostToDevice, c->s_copy));
     HIPCHK(c, hipEventRecord(c->slot_ev1[slot], c->s_copy));
     HIPCHK(c, hipEventRecord(c->ev_copy, c->s_copy));
     c->slot_timed[slot] = 1;
@@ -540,7 +549,7 @@ static void make_resident(gw_ctx* c, uint32_t slot, uint32_t blk)
     c->slot_read[slot] = 1;
     DevBlock& h = c->hblk[blk];
     h.beg = c->slot_beg[slot];
-    h.csr = c->slot_csr[slot];
+    h.csr = slot_edges(c, slot, blk);
     h.resident = kInSlot;
 }
 
@@ -852,7 +861,7 @@ gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
     if (l.block == 0) {   // shape and search order chosen by gw_autotune, else the default
         const uint32_t flags = l.flags;
         l = c->have_tuned ? c->tuned : gw_launch{256, 0, 0, 0};
-        l.flags = flags | (l.flags & GW_RUN_INTERP);
+        l.flags = flags | (l.flags & (GW_RUN_INTERP | GW_RUN_WINDOW));
     }
     return run_impl(c, l, stats, 0);
 }
@@ -915,6 +924,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     a.scan = 0u;
     a.zc_count = mapped ? 1u : 0u;
     a.search_end = (l.flags & GW_RUN_INTERP) ? (uint32_t)c->nverts_total : 0u;
+    a.search_win = (l.flags & GW_RUN_WINDOW) ? 1u : 0u;
     a.nwork = (uint32_t)nwork;
     a.queue = c->d_misc + 1;
     a.counters = c->d_cnt;
@@ -924,7 +934,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     }
     // the flattened variant reads u32s through aligned 8-byte words, which is only
     // safe inside padded HBM slots: zero-copy (host-mapped) blocks use variant 0
-    if (l.variant == 1 && (l.flags & GW_RUN_INTERP)) l.variant = 0;   // bisection only in variant 1
+    if (l.variant == 1 && (l.flags & (GW_RUN_INTERP | GW_RUN_WINDOW))) l.variant = 0;   // bisection only in variant 1
     if (l.variant == 1)
         for (uint32_t b = 0; b < c->nblocks; ++b)
             if (c->hblk[b].resident == kMapped) { l.variant = 0; break; }
@@ -1030,19 +1040,21 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, uint32_t what, gw_launch* best
     };
     const gw_counters tot0 = c->tot;
     const uint64_t unfinished0 = c->unfinished;
-    const uint32_t I = GW_RUN_INTERP;
+    const uint32_t I = GW_RUN_INTERP, W = GW_RUN_WINDOW;
     static const gw_launch cand[] = {{64, 0, 0, 0},   {128, 0, 0, 0},  {256, 0, 0, 0},  {512, 0, 0, 0},
                                      {1024, 0, 0, 0}, {64, 0, 1, 0},   {128, 0, 1, 0},  {256, 0, 1, 0},
                                      {64, 0, 0, I},   {128, 0, 0, I},  {256, 0, 0, I},  {512, 0, 0, I},
                                      {1024, 0, 0, I}, {64, 0, 2, 0},   {256, 0, 2, 0},  {1024, 0, 2, 0},
-                                     {64, 0, 2, I},   {256, 0, 2, I},  {1024, 0, 2, I}};
+                                     {64, 0, 2, I},   {256, 0, 2, I},  {1024, 0, 2, I}, {256, 0, 0, W},
+                                     {1024, 0, 0, W}, {256, 0, 2, W},  {1024, 0, 2, W}, {256, 0, 0, I | W},
+                                     {1024, 0, 0, I | W}, {64, 0, 2, I | W}, {256, 0, 2, I | W}, {1024, 0, 2, I | W}};
     double bms = 1e300;
     gw_launch bl = cand[2];
     for (const gw_launch& L : cand) {
         // candidates outside `what` are skipped; 256 threads of variant 0 always run
         if (L.variant != 0 && !(what & GW_TUNE_VARIANT)) continue;
         if (L.variant == 0 && L.block != 256 && !(what & GW_TUNE_BLOCK)) continue;
-        if ((L.flags & GW_RUN_INTERP) && !(what & GW_TUNE_SEARCH)) continue;
+        if ((L.flags & (GW_RUN_INTERP | GW_RUN_WINDOW)) && !(what & GW_TUNE_SEARCH)) continue;
         double t = 1e300;
         for (int rep = 0; rep < 2 && st == GW_OK; ++rep) {
             gw_run_stats rs{};

@@ -75,6 +75,7 @@ struct WalkArgs {
     uint32_t scan;            // identity list: also skip walkers whose blocks are not both resident
     uint32_t zc_count;        // some block is host-mapped: count zero-copy reads (counters[7])
     uint32_t search_end;      // 0: membership by bisection; else interpolation, ids below it
+    uint32_t search_win;      // probes read aligned 16-entry windows
     uint32_t* queue;          // work-fetch counter, zero at launch
     uint32_t chunk;           // walkers a wavefront takes per queue atomic (multiple of 64)
     unsigned long long* counters;  // steps, trials, probes, mix, ended, left

@@ -34,21 +34,100 @@ struct Cnt {
 
 __device__ __forceinline__ uint32_t bitlen64(uint64_t n) { return n ? 64u - (uint32_t)__clzll(n) : 0u; }
 
-// engine/engine.hpp:224-243
-// Membership of x in the sorted list a[0..n), probes counting the list reads.  INTERP =
-// false bisects (the reference's search, engine/engine.hpp:224-243).  INTERP = true
-// (every id below id_end) interpolates the next probe from the value bounds [vlo, vhi) of
-// the remaining range -- a few probes when ids are spread evenly, as in randomly
-// permuted graphs -- in exact fp32 arithmetic (the oracle computes the same positions),
-// and bisects whenever a probe failed to halve the range and inside ranges of at most 16
-// entries (one 64-byte line), so no list takes more than about twice the bisection's
-// probes.  Both orders give the same answer.
+// One window probe of the window order: the 16 list entries in the 64-byte piece (aligned
+// to 64 bytes in memory, half a 128-byte line) that holds the guessed position.  Slots
+// place each block so that an entry's address is 4 x its index in the whole graph's edge
+// array modulo 64, so the pieces are those the oracle's window order reads.  scan_window
+// compares x with the entries [s, e) of the piece that belong to the remaining range:
+// found, how many are below x, and the first and last of them.
+struct WinScan {
+    bool found;
+    uint32_t below, first, last;
+};
+
+__device__ __forceinline__ WinScan scan_window(const uint4& q0, const uint4& q1, const uint4& q2, const uint4& q3,
+                                               uint32_t s, uint32_t e, uint32_t x)
+{
+    const uint32_t v[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                            q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+    WinScan r{false, 0u, 0u, 0u};
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) {
+        const bool in = i >= s && i < e;
+        r.found |= in && v[i] == x;
+        r.below += (in && v[i] < x) ? 1u : 0u;
+        r.first = i == s ? v[i] : r.first;
+        r.last = i + 1 == e ? v[i] : r.last;
+    }
+    return r;
+}
+
+__device__ __forceinline__ const uint32_t* window_of(const uint32_t* p)
+{
+    return (const uint32_t*)((uintptr_t)p & ~(uintptr_t)63);
+}
+
+// Window orders (the oracle's searches 2 and 3): each probe reads the window holding the
+// guess, which bisects the remaining range (INTERP false) or is interpolated from its
+// value bounds (INTERP true, bisecting after a probe that failed to halve the range); a
+// miss either shows x absent (it falls between two entries read) or keeps the range on
+// one side of the window, bounded by the value next to it.
 template <bool INTERP>
+__device__ __forceinline__ bool member_window(const uint32_t* __restrict__ a, uint64_t n, uint32_t x,
+                                              uint32_t& probes, uint32_t id_end)
+{
+    uint64_t lo = 0, hi = n;   // remaining range [lo, hi), values in [vlo, vhi)
+    uint32_t vlo = 0, vhi = id_end;
+    bool bisect = false;
+    if (INTERP && x >= vhi) return false;
+    while (lo < hi) {
+        const uint32_t len = (uint32_t)(hi - lo);
+        uint64_t g;
+        if (!INTERP || bisect || vhi - vlo < 2) {
+            g = lo + ((len - 1) >> 1);
+        } else {
+            const uint32_t off = (uint32_t)((float)(x - vlo) * (float)len / (float)(vhi - vlo));
+            g = lo + (off < len ? off : len - 1);
+        }
+        const uint32_t* wp = window_of(a + g);
+        const uint4* q = (const uint4*)wp;
+        const uint4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        ++probes;
+        const int64_t w0 = wp - a;   // list index of wp[0], in (g - 16, g]
+        const int64_t s = w0 > (int64_t)lo ? w0 : (int64_t)lo;
+        const int64_t e = w0 + 16 < (int64_t)hi ? w0 + 16 : (int64_t)hi;
+        const WinScan r = scan_window(q0, q1, q2, q3, (uint32_t)(s - w0), (uint32_t)(e - w0), x);
+        if (r.found) return true;
+        if (r.below == 0) {
+            hi = (uint64_t)s;
+            vhi = r.first;
+        } else if (r.below == (uint32_t)(e - s)) {
+            lo = (uint64_t)e;
+            vlo = r.last + 1;
+        } else {
+            return false;
+        }
+        bisect = (hi - lo) * 2 > len;
+    }
+    return false;
+}
+
+// engine/engine.hpp:224-243
+// Membership of x in the sorted list a[0..n), probes counting the list reads.  Search
+// mode SM: 0 bisects (the reference's search).  1 (every id below id_end) interpolates
+// the next probe from the value bounds [vlo, vhi) of the remaining range -- a few probes
+// when ids are spread evenly, as in randomly permuted graphs -- in exact fp32 arithmetic
+// (the oracle computes the same positions), and bisects whenever a probe failed to halve
+// the range and inside ranges of at most 16 entries (one 64-byte line), so no list takes
+// more than about twice the bisection's probes.  2 and 3 read windows (member_window).
+// All orders give the same answer.
+template <uint32_t SM>
 __device__ __forceinline__ bool member(const uint32_t* __restrict__ a, uint64_t n, uint32_t x, uint32_t& probes,
                                        uint32_t id_end)
 {
+    if (SM & 2) return member_window<(SM & 1) != 0>(a, n, x, probes, id_end);
     int64_t lo = 0, hi = (int64_t)n - 1;
-    if (!INTERP) {
+    if (!(SM & 1)) {
         while (lo <= hi) {
             const int64_t mid = lo + ((hi - lo) >> 1);
             const uint32_t m = a[mid];
@@ -141,7 +220,7 @@ __device__ __forceinline__ bool sopr_use_mixture(uint64_t du, uint64_t dv, uint3
     return du * (4ull * bitlen64(dv) + 2) < e * (1 + bitlen64(du));
 }
 
-template <bool INTERP>
+template <uint32_t SM>
 __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64_t dv,
                                          const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
                                          const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
@@ -159,13 +238,13 @@ __device__ __forceinline__ bool n2v_step(const uint32_t* __restrict__ nv, uint64
         if (x == prev) t = tp.m_p;
         else if (ra <= tp.m_lo) return true;      // accepted whether or not x is in N(prev)
         else if (ra > tp.m_hi) continue;          // rejected either way
-        else t = member<INTERP>(nu, du, x, c.probes, id_end) ? tp.m_one : tp.m_q;
+        else t = member<SM>(nu, du, x, c.probes, id_end) ? tp.m_one : tp.m_q;
         if (ra <= t) return true;
     }
     return false;
 }
 
-template <bool INTERP>
+template <uint32_t SM>
 __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint64_t dv,
                                           const uint32_t* __restrict__ nu, uint64_t du, uint32_t prev,
                                           const DevType& tp, uint32_t w, uint32_t hop, uint32_t k0, uint32_t k1,
@@ -183,7 +262,7 @@ __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint6
                 return true;
             }
             const uint32_t y = nu[((uint64_t)d.y * du) >> 32];
-            if (y != prev && member<INTERP>(nv, dv, y, c.probes, id_end)) {
+            if (y != prev && member<SM>(nv, dv, y, c.probes, id_end)) {
                 x = y;
                 return true;
             }
@@ -214,7 +293,7 @@ __device__ __forceinline__ bool sopr_step(const uint32_t* __restrict__ nv, uint6
         ++c.trials;
         x = nv[((uint64_t)rp * dv) >> 32];
         if ((u128)ra * den < num32) return true;
-        if (x != prev && member<INTERP>(nu, du, x, c.probes, id_end)) return true;
+        if (x != prev && member<SM>(nu, du, x, c.probes, id_end)) return true;
     }
     return false;
 }
@@ -341,8 +420,8 @@ __device__ __forceinline__ uint32_t wave_take(WaveQueue& q, uint64_t mask, uint3
 // cut points into LDS, so that the descriptor reads and the vertex->block search a step
 // across blocks makes stay on chip (the table does not change during a launch).  MIX:
 // two walker types; with one, the type's parameters are wavefront-uniform (SGPRs).
-template <bool ONE, bool INTERP, bool LB, bool MIX>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_kernel(WalkArgs a)
+template <bool ONE, uint32_t SM, bool LB, bool MIX>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM & 2) ? 4 : 8))) walk_kernel(WalkArgs a)
 {
     // block-access trace: steps per current block, histogrammed in LDS
     constexpr uint32_t kTrace = ONE ? 1u : (LB ? kLdsBlocks : kTraceLdsBlocks);
@@ -417,9 +496,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
             uint32_t x = 0;
             const uint32_t t0 = c.trials, p0 = c.probes;
             const bool ok = tp.algo == 0
-                                ? n2v_step<INTERP>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
+                                ? n2v_step<SM>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
                                                    a.search_end)
-                                : sopr_step<INTERP>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
+                                : sopr_step<SM>(rv.adj, rv.deg, ru.adj, ru.deg, prev, tp, w, hop, k0, k1, x, c,
                                                     a.search_end);
             if (a.zc_count) {
                 // candidates come from cur's row, membership probes from prev's row
@@ -976,31 +1055,40 @@ __device__ __forceinline__ void lane_row(const DevBlock& b, uint64_t v0, uint64_
 }
 
 // next probe of the search for L.x in [L.lo, L.hx): member's positions
-template <bool INTERP>
+template <uint32_t SM>
 __device__ __forceinline__ uint32_t lane_mid(const Lane& L)
 {
     const uint32_t len = L.hx - L.lo;
-    if (!INTERP || (L.fl & kLaneBisect) || len <= 16 || L.vhi - L.vlo < 2) return L.lo + ((len - 1) >> 1);
+    if (!(SM & 1) || (L.fl & kLaneBisect) || (!(SM & 2) && len <= 16) || L.vhi - L.vlo < 2)
+        return L.lo + ((len - 1) >> 1);
     const uint32_t off = (uint32_t)((float)(L.x - L.vlo) * (float)len / (float)(L.vhi - L.vlo));
     return L.lo + (off < len ? off : len - 1);
 }
 
+// address of the next probe: the entry, or the window holding it
+template <uint32_t SM>
+__device__ __forceinline__ const void* lane_probe_addr(const Lane& L, const uint32_t* list)
+{
+    const uint32_t* p = list + lane_mid<SM>(L);
+    return (SM & 2) ? (const void*)window_of(p) : (const void*)p;
+}
+
 // starts the membership search of L.x in list[0, n); false = decided (absent) without a probe
-template <bool INTERP>
+template <uint32_t SM>
 __device__ __forceinline__ bool lane_search(Lane& L, const uint32_t* list, uint32_t n, uint32_t id_end)
 {
-    if (n == 0 || (INTERP && L.x >= id_end)) return false;
+    if (n == 0 || ((SM & 1) && L.x >= id_end)) return false;
     L.lo = 0;
     L.hx = n;
     L.vlo = 0;
     L.vhi = id_end;
     L.fl &= ~kLaneBisect;
-    L.addr = list + lane_mid<INTERP>(L);
+    L.addr = lane_probe_addr<SM>(L, list);
     return true;
 }
 
-template <bool ONE, bool INTERP, bool LB, bool MIX>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) walk_lane_kernel(WalkArgs a)
+template <bool ONE, uint32_t SM, bool LB, bool MIX>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM & 2) ? 4 : 8))) walk_lane_kernel(WalkArgs a)
 {
     constexpr uint32_t kTrace = ONE ? 1u : (LB ? kLdsBlocks : kTraceLdsBlocks);
     __shared__ uint32_t trace[kTrace];
@@ -1051,7 +1139,14 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
         // ---- one round of loads
         uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
         uint32_t v = 0;
-        if (L.ph == LS_STATE) {
+        uint4 q0 = make_uint4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
+        if ((SM & 2) && L.ph == LS_PROBE) {
+            const uint4* q = (const uint4*)L.addr;   // one window: 64 aligned bytes
+            q0 = q[0];
+            q1 = q[1];
+            q2 = q[2];
+            q3 = q[3];
+        } else if (L.ph == LS_STATE) {
             L.meta = a.meta[L.w];
             L.cur = a.cur[L.w];
             L.prev = a.prev[L.w];
@@ -1111,7 +1206,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
                 L.ph = LS_MOVE;
             } else if (L.ra > tp.m_hi) {
                 L.ph = LS_TRIAL;
-            } else if (lane_search<INTERP>(L, L.nu, L.du, a.search_end)) {
+            } else if (lane_search<SM>(L, L.nu, L.du, a.search_end)) {
                 ++c.probes;
                 if (a.zc_count && (L.fl & kLaneMu)) ++c.zc;
                 L.ph = LS_PROBE;
@@ -1126,7 +1221,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
             break;
         case LS_PROPU:   // SOPR: a prev-neighbour y != prev is accepted iff y is in N(cur)
             L.x = v;
-            if (L.x != L.prev && lane_search<INTERP>(L, L.nv, L.dv, a.search_end)) {
+            if (L.x != L.prev && lane_search<SM>(L, L.nv, L.dv, a.search_end)) {
                 ++c.probes;
                 if (a.zc_count && (L.fl & kLaneMv)) ++c.zc;
                 L.ph = LS_PROBE;
@@ -1138,23 +1233,48 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
         case LS_PROBE: {
             const bool sopr = tp.algo != 0;
             const uint32_t* list = sopr ? L.nv : L.nu;
-            const uint32_t mid = (uint32_t)((const uint32_t*)L.addr - list);
-            bool done = true, found = v == L.x;
-            if (!found) {
-                const uint32_t len = L.hx - L.lo;
-                if (v < L.x) {
-                    L.lo = mid + 1;
-                    L.vlo = v + 1;
-                } else {
-                    L.hx = mid;
-                    L.vhi = v;
+            bool done = true, found;
+            const uint32_t len = L.hx - L.lo;
+            if (SM & 2) {
+                // window [w0, w0 + 16) of the list; the entries [s, e) of it are in range
+                const int64_t w0 = (const uint32_t*)L.addr - list;
+                const int64_t s0 = w0 > (int64_t)L.lo ? w0 : (int64_t)L.lo;
+                const int64_t e0 = w0 + 16 < (int64_t)L.hx ? w0 + 16 : (int64_t)L.hx;
+                const WinScan r = scan_window(q0, q1, q2, q3, (uint32_t)(s0 - w0), (uint32_t)(e0 - w0), L.x);
+                found = r.found;
+                if (!found) {
+                    if (r.below == 0) {
+                        L.hx = (uint32_t)s0;
+                        L.vhi = r.first;
+                        done = false;
+                    } else if (r.below == (uint32_t)(e0 - s0)) {
+                        L.lo = (uint32_t)e0;
+                        L.vlo = r.last + 1;
+                        done = false;
+                    }
                 }
-                if (L.lo < L.hx) {
+            } else {
+                const uint32_t mid = (uint32_t)((const uint32_t*)L.addr - list);
+                found = v == L.x;
+                if (!found) {
+                    if (v < L.x) {
+                        L.lo = mid + 1;
+                        L.vlo = v + 1;
+                    } else {
+                        L.hx = mid;
+                        L.vhi = v;
+                    }
                     done = false;
-                    if (INTERP) L.fl = ((uint64_t)(L.hx - L.lo) * 2 > len) ? (L.fl | kLaneBisect) : (L.fl & ~kLaneBisect);
-                    L.addr = list + lane_mid<INTERP>(L);
+                }
+            }
+            if (!done) {
+                if (L.lo < L.hx) {
+                    if (SM & 1) L.fl = ((uint64_t)(L.hx - L.lo) * 2 > len) ? (L.fl | kLaneBisect) : (L.fl & ~kLaneBisect);
+                    L.addr = lane_probe_addr<SM>(L, list);
                     ++c.probes;
                     if (a.zc_count && (L.fl & (sopr ? kLaneMv : kLaneMu))) ++c.zc;
+                } else {
+                    done = true;
                 }
             }
             if (done) {
@@ -1474,27 +1594,27 @@ hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s)
 }
 
 // Step kernel instances: variant x single block x search order x LDS table x walker types.
-template <bool ONE, bool INTERP, bool LB, bool MIX>
+template <bool ONE, uint32_t SM, bool LB, bool MIX>
 static void launch_shape(const WalkArgs& a, uint32_t variant, uint32_t grid, uint32_t block, hipStream_t s)
 {
-    if (variant == 2) hipLaunchKernelGGL((walk_lane_kernel<ONE, INTERP, LB, MIX>), dim3(grid), dim3(block), 0, s, a);
-    else hipLaunchKernelGGL((walk_kernel<ONE, INTERP, LB, MIX>), dim3(grid), dim3(block), 0, s, a);
+    if (variant == 2) hipLaunchKernelGGL((walk_lane_kernel<ONE, SM, LB, MIX>), dim3(grid), dim3(block), 0, s, a);
+    else hipLaunchKernelGGL((walk_kernel<ONE, SM, LB, MIX>), dim3(grid), dim3(block), 0, s, a);
 }
 
-template <bool ONE, bool INTERP, bool LB>
+template <bool ONE, uint32_t SM, bool LB>
 static void launch_types(const WalkArgs& a, uint32_t variant, uint32_t grid, uint32_t block, hipStream_t s)
 {
-    if (a.ntypes == 2) launch_shape<ONE, INTERP, LB, true>(a, variant, grid, block, s);
-    else launch_shape<ONE, INTERP, LB, false>(a, variant, grid, block, s);
+    if (a.ntypes == 2) launch_shape<ONE, SM, LB, true>(a, variant, grid, block, s);
+    else launch_shape<ONE, SM, LB, false>(a, variant, grid, block, s);
 }
 
-template <bool INTERP>
+template <uint32_t SM>
 static void launch_search(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid, uint32_t block,
                           hipStream_t s)
 {
-    if (one_block) launch_types<true, INTERP, false>(a, variant, grid, block, s);
-    else if (a.nblocks <= kLdsBlocks) launch_types<false, INTERP, true>(a, variant, grid, block, s);
-    else launch_types<false, INTERP, false>(a, variant, grid, block, s);
+    if (one_block) launch_types<true, SM, false>(a, variant, grid, block, s);
+    else if (a.nblocks <= kLdsBlocks) launch_types<false, SM, true>(a, variant, grid, block, s);
+    else launch_types<false, SM, false>(a, variant, grid, block, s);
 }
 
 hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint32_t grid,
@@ -1504,10 +1624,14 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
     if (variant == 1) {
         if (one_block) hipLaunchKernelGGL(walk_flat_kernel<true>, dim3(grid), dim3(block), 0, s, a);
         else hipLaunchKernelGGL(walk_flat_kernel<false>, dim3(grid), dim3(block), 0, s, a);
+    } else if (a.search_end != 0 && a.search_win) {
+        launch_search<3>(a, one_block, variant, grid, block, s);
+    } else if (a.search_win) {
+        launch_search<2>(a, one_block, variant, grid, block, s);
     } else if (a.search_end != 0) {
-        launch_search<true>(a, one_block, variant, grid, block, s);
+        launch_search<1>(a, one_block, variant, grid, block, s);
     } else {
-        launch_search<false>(a, one_block, variant, grid, block, s);
+        launch_search<0>(a, one_block, variant, grid, block, s);
     }
     return hipGetLastError();
 }

@@ -9,7 +9,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import (GW_NODE2VEC, GW_OUT_HASH, GW_OUT_PATHS, GW_OUT_VISITS, GW_RUN_BUCKETS, GW_RUN_INTERP, GW_SOPR, GwBlockDesc,
+from ._abi import (GW_NODE2VEC, GW_OUT_HASH, GW_OUT_PATHS, GW_OUT_VISITS, GW_RUN_BUCKETS, GW_RUN_INTERP, GW_RUN_WINDOW, GW_SOPR, GwBlockDesc,
                    GwConfig, GwCounters, GwDeviceInfo, GwLaunch, GwRunStats, GwWalkParams, check, lib)
 
 NODE2VEC, SOPR = GW_NODE2VEC, GW_SOPR
@@ -168,13 +168,15 @@ class Context:
         self.nwalkers = n
         self.outputs = outputs
 
-    def run(self, block=0, waves_per_cu=0, variant=None, buckets=False, interp=False):
+    def run(self, block=0, waves_per_cu=0, variant=None, buckets=False, interp=False, window=False):
         """One gw_run.  block 0 takes the whole launch from the autotuner (or 256 threads of
-        variant 0); naming a variant without a block runs it at 256 threads."""
+        variant 0); naming a variant without a block runs it at 256 threads.  interp /
+        window: membership search order (interpolated guesses / 64-byte window reads)."""
         if variant is not None and block == 0:
             block = 256
         L = GwLaunch(block, waves_per_cu, variant or 0,
-                     (RUN_BUCKETS if buckets else 0) | (GW_RUN_INTERP if interp else 0))
+                     (RUN_BUCKETS if buckets else 0) | (GW_RUN_INTERP if interp else 0) |
+                     (GW_RUN_WINDOW if window else 0))
         s = GwRunStats()
         self._chk(lib().gw_run(self.h, C.byref(L), C.byref(s)))
         return {f: getattr(s, f) for f, _ in GwRunStats._fields_}
@@ -186,7 +188,7 @@ class Context:
         ms = C.c_double()
         self._chk(lib().gw_autotune(self.h, sample, what, C.byref(best), C.byref(ms)))
         return dict(block=best.block, waves_per_cu=best.waves_per_cu, variant=best.variant,
-                    interp=bool(best.flags & GW_RUN_INTERP)), ms.value
+                    interp=bool(best.flags & GW_RUN_INTERP), window=bool(best.flags & GW_RUN_WINDOW)), ms.value
 
     def bucket_counts(self):
         nb = self.blocks.nblocks
@@ -318,7 +320,7 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
     o.prefetch_depth = prefetch_depth
     launch = launch or {}
     o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 2),
-                        GW_RUN_INTERP if launch.get("interp") else 0)
+                        (GW_RUN_INTERP if launch.get("interp") else 0) | (GW_RUN_WINDOW if launch.get("window") else 0))
     wp = walk_params(types, seed)
     rep = GwhReport()
     h = np.zeros(nwalkers, dtype=np.uint64) if hashes else None
@@ -331,5 +333,6 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
         raise GwError(st, L.gwh_last_error().decode())
     out = {f: getattr(rep, f) for f, _ in GwhReport._fields_ if f not in ("pad", "tuned")}
     t = rep.tuned
-    out["tuned"] = dict(block=t.block, variant=t.variant, interp=bool(t.flags & GW_RUN_INTERP)) if t.block else None
+    out["tuned"] = dict(block=t.block, variant=t.variant, interp=bool(t.flags & GW_RUN_INTERP),
+                        window=bool(t.flags & GW_RUN_WINDOW)) if t.block else None
     return out, h, v

@@ -104,9 +104,17 @@ enum {
     GW_RUN_BUCKETS = 1u << 0, /* after the run, recount the (prev_blk, cur_blk) bucket matrix of
                                  the walkers left, so gw_bucket_counts needs no extra pass */
     GW_RUN_TRACE = 1u << 1,   /* record steps per current block (gw_trace_read) */
-    GW_RUN_INTERP = 1u << 2   /* membership tests by guarded interpolation search instead of
+    GW_RUN_INTERP = 1u << 2,  /* membership tests by guarded interpolation search instead of
                                  bisection: same walks, fewer reads on evenly spread ids
                                  (randomly permuted graphs); step kernel variants 0 and 2 */
+    GW_RUN_WINDOW = 1u << 3   /* membership tests read aligned 64-byte windows of 16 entries
+                                 around each guess (bisected, or interpolated with
+                                 GW_RUN_INTERP): same walks, one read where the last levels of
+                                 a search took one each.  Probe counts equal the oracle's
+                                 window orders when each block's edge words keep their offset
+                                 modulo 64 bytes (HBM slots do; host-mapped blocks do when the
+                                 host edge array starts on a 64-byte boundary); variants 0
+                                 and 2 */
 };
 
 typedef struct {

@@ -231,8 +231,50 @@ static int member_interp(const uint32_t* a, uint64_t n, uint32_t x, uint32_t id_
     return 0;
 }
 
-int orc_member(const uint32_t* a, uint64_t n, uint32_t x, uint32_t search, uint32_t id_end, uint64_t* probes)
+/* Window orders (not in the reference; same answer as member() for every sorted list):
+ * each probe reads the 16 entries of the list that lie in one 16-entry piece of the whole
+ * graph's edge array, aligned to 16 there (head = index of a[0] in that array), around
+ * the guessed position -- one 64-byte read of a 128-byte line, so the last few levels of
+ * a search cost one probe instead of one each.  The guess bisects the remaining range
+ * (search 2) or is member_interp's interpolated position (search 3, bisecting after a
+ * probe that failed to halve the range, with no small-range rule).  A probe that does
+ * not find x either shows it absent (x falls between two entries read) or cuts the range
+ * to one side of the entries read, whose values become the new bound. */
+static int member_window(const uint32_t* a, uint64_t n, uint32_t x, int interp, uint32_t id_end, uint64_t head,
+                         uint64_t* probes)
 {
+    uint64_t lo = 0, hi = n;   /* remaining range [lo, hi) */
+    uint32_t vlo = 0, vhi = id_end;
+    int bisect = 0;
+    if (interp && x >= vhi) return 0;
+    while (lo < hi) {
+        const uint32_t len = (uint32_t)(hi - lo);
+        uint64_t g;
+        if (!interp || bisect || vhi - vlo < 2) {
+            g = lo + ((len - 1) >> 1);
+        } else {
+            volatile float num = (float)(x - vlo) * (float)len;   /* no excess precision */
+            const uint32_t off = (uint32_t)(num / (float)(vhi - vlo));
+            g = lo + (off < len ? off : len - 1);
+        }
+        const int64_t w0 = (int64_t)g - (int64_t)((head + g) & 15);
+        const uint64_t s = w0 > (int64_t)lo ? (uint64_t)w0 : lo;
+        const uint64_t e = (uint64_t)(w0 + 16) < hi ? (uint64_t)(w0 + 16) : hi;
+        (*probes)++;
+        for (uint64_t i = s; i < e; ++i)
+            if (a[i] == x) return 1;
+        if (x < a[s]) { hi = s; vhi = a[s]; }
+        else if (x > a[e - 1]) { lo = e; vlo = a[e - 1] + 1; }
+        else return 0;
+        bisect = (uint64_t)(hi - lo) * 2 > len;
+    }
+    return 0;
+}
+
+int orc_member(const uint32_t* a, uint64_t n, uint32_t x, uint32_t search, uint32_t id_end, uint64_t head,
+               uint64_t* probes)
+{
+    if (search & 2) return member_window(a, n, x, (int)(search & 1), id_end, head, probes);
     return search ? member_interp(a, n, x, id_end, probes) : member(a, n, x, probes);
 }
 
@@ -325,7 +367,7 @@ static void walk_impl(const orc_graph* g, const orc_walkcfg* cfg, uint32_t wid, 
                 if (x == prev) t = tc->t_p;
                 else if (ra < band_lo) { ok = 1; break; }   /* both x != prev classes accept */
                 else if (ra >= band_hi) continue;           /* both reject */
-                else if (orc_member(csr + u_head, du, x, tc->search, (uint32_t)g->nverts, &probes)) t = tc->t_one;
+                else if (orc_member(csr + u_head, du, x, tc->search, (uint32_t)g->nverts, u_head, &probes)) t = tc->t_one;
                 else t = tc->t_q;
                 if (ra < t) { ok = 1; break; }
             }
@@ -340,7 +382,7 @@ static void walk_impl(const orc_graph* g, const orc_walkcfg* cfg, uint32_t wid, 
                     break;
                 }
                 uint32_t y = csr[u_head + (((uint64_t)d[1] * du) >> 32)];
-                if (y != prev && orc_member(csr + v_head, dv, y, tc->search, (uint32_t)g->nverts, &probes)) {
+                if (y != prev && orc_member(csr + v_head, dv, y, tc->search, (uint32_t)g->nverts, v_head, &probes)) {
                     x = y;
                     ok = 1;
                     break;
@@ -371,7 +413,7 @@ static void walk_impl(const orc_graph* g, const orc_walkcfg* cfg, uint32_t wid, 
                     trials++;
                     x = csr[v_head + (((uint64_t)rp * dv) >> 32)];
                     if ((u128)ra * den < num32) { ok = 1; break; }
-                    if (x != prev && orc_member(csr + u_head, du, x, tc->search, (uint32_t)g->nverts, &probes)) { ok = 1; break; }
+                    if (x != prev && orc_member(csr + u_head, du, x, tc->search, (uint32_t)g->nverts, u_head, &probes)) { ok = 1; break; }
                 }
             }
         }

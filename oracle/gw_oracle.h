@@ -64,7 +64,8 @@ typedef struct {
     uint64_t t_term;        /* geometric termination threshold floor(term*2^32); 0 = none */
     uint32_t sopr_sampler;  /* 0: proposal (duplicate-free lists), 1: envelope/mixture (any lists) */
     uint32_t search;        /* membership search order: 0 bisection (the reference's), 1 guarded
-                               interpolation; same answers, different probe counts */
+                               interpolation, 2 / 3 16-entry windows with bisected / interpolated
+                               guesses; same answers, different probe counts */
 } orc_typecfg;
 
 typedef struct {
@@ -80,8 +81,10 @@ typedef struct {
 } orc_walk_result;
 
 /* membership of x in the sorted list a[0..n) with the given search order (values are
- * below id_end); probes counts the list reads */
-int orc_member(const uint32_t* a, uint64_t n, uint32_t x, uint32_t search, uint32_t id_end, uint64_t* probes);
+ * below id_end; a[0] is entry `head` of the edge array, which places the window orders'
+ * aligned reads); probes counts the list reads */
+int orc_member(const uint32_t* a, uint64_t n, uint32_t x, uint32_t search, uint32_t id_end, uint64_t head,
+               uint64_t* probes);
 
 /* host-side parameter conversion (the product computes the same values independently) */
 void orc_n2v_thresholds(double p, double q, orc_typecfg* t);

@@ -81,7 +81,7 @@ def lib():
         L.orc_split_blocks.restype = C.c_size_t
         L.orc_max_choose.argtypes = [P, C.c_uint32, P, P, C.c_uint32, C.c_uint64]
         L.orc_max_choose.restype = C.c_int
-        L.orc_member.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
+        L.orc_member.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]
         L.orc_member.restype = C.c_int
         _lib = L
     return _lib
@@ -200,10 +200,11 @@ def max_choose(W, buckets, ncblock, zero_threshold=0):
     return r, list(b[:n.value])
 
 
-def member(a, x, search=0, id_end=None):
-    """(found, probes) of the oracle's membership test on the sorted u32 list a."""
+def member(a, x, search=0, id_end=None, head=0):
+    """(found, probes) of the oracle's membership test on the sorted u32 list a, whose
+    first entry is entry `head` of an edge array (places the window orders' reads)."""
     a = np.ascontiguousarray(a, dtype=np.uint32)
     pr = C.c_uint64(0)
     end = int(id_end if id_end is not None else (int(a.max()) + 1 if len(a) else 1))
-    r = lib().orc_member(a.ctypes.data, len(a), int(x), search, end, C.byref(pr))
+    r = lib().orc_member(a.ctypes.data, len(a), int(x), search, end, int(head), C.byref(pr))
     return bool(r), pr.value

@@ -88,8 +88,8 @@ def test_sopr_chung_lu_resident_invariants(gpu):
 def test_node2vec_rmat20_full_size_matches_oracle(gpu):
     """The resident Node2vec workload at its full size (RMAT scale 20, ~31M arcs, 2^20
     walkers x 80 hops, p 2, q 0.5), bit for bit against the oracle's replay: path hashes,
-    final states, and step / trial / probe counts, for the autotuned launch and for both
-    search orders of both resident kernel variants."""
+    final states, and step / trial / probe counts, for the autotuned launch and for every
+    search order of both resident kernel variants."""
     import _oracle as O
 
     ctx = E.Context(0)
@@ -98,10 +98,10 @@ def test_node2vec_rmat20_full_size_matches_oracle(gpu):
     n = len(beg) - 1
     types = [dict(algo=E.NODE2VEC, length=80, p=2.0, q=0.5)]
     want = {}
-    for search in (0, 1):
+    for search in (0, 1, 2, 3):
         cfg = O.walk_cfg(42, O.type_cfg(O.N2V, 80, p=2.0, q=0.5, search=search))
         want[search] = O.run(beg, csr, cfg, n, nthreads=16)
-    print("oracle totals", want[0]["totals"], want[1]["totals"], flush=True)
+    print("oracle totals", [want[s]["totals"] for s in want], flush=True)
     bl = E.Blocks(beg, csr, 1 << 34)
     with E.Context() as cx:
         cx.set_blocks(bl)
@@ -110,10 +110,11 @@ def test_node2vec_rmat20_full_size_matches_oracle(gpu):
         cx.init_walkers(n, 1, outputs=E.OUT_HASH)
         cx.autotune(1 << 18)
         for launch in (dict(block=0), dict(variant=0), dict(variant=2), dict(variant=0, interp=True),
-                       dict(variant=2, interp=True)):
+                       dict(variant=2, interp=True), dict(variant=0, window=True), dict(variant=2, window=True),
+                       dict(variant=2, interp=True, window=True)):
             cx.init_walkers(n, 1, outputs=E.OUT_HASH)
             st = cx.run(**launch)
-            o = want[1 if launch.get("interp") else 0]
+            o = want[(1 if launch.get("interp") else 0) | (2 if launch.get("window") else 0)]
             print(launch, {k: st[k] for k in ("steps", "trials", "probes")}, "kernel %.2f ms" % st["kernel_ms"],
                   flush=True)
             if launch != dict(block=0):

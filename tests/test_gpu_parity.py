@@ -69,7 +69,8 @@ def gpu_run(beg, csr, types, seed, n, wpv=1, block_bytes=1 << 30, nslots=None, s
 
 def check_case(beg, csr, types, seed, n, wpv=1, **kw):
     g = gpu_run(beg, csr, types, seed, n, wpv, **kw)
-    ts = oracle_types(types, search=1 if (kw.get("launch") or {}).get("interp") else 0)
+    L = kw.get("launch") or {}
+    ts = oracle_types(types, search=(1 if L.get("interp") else 0) | (2 if L.get("window") else 0))
     cfg = O.walk_cfg(seed, *ts)
     o = O.run(beg, csr, cfg, n, wpv=wpv, sources=kw.get("sources"), ntypes=len(types), paths=True, visits=True)
     print("case", types, "n", n, "gpu totals", g["totals"], "oracle totals", o["totals"], "runs", g["runs"])
@@ -197,6 +198,26 @@ def test_interpolation_search_order(gpu, graph):
         check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1)], 4, 4096, launch=L)
         check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1, sampler=1)], 5, 4096, launch=L)
         check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 6, 2048, block_bytes=8192, nslots=3,
+                   launch=L)
+
+
+@pytest.mark.parametrize("graph,interp", [("permuted", True), ("skewed", True), ("skewed", False)])
+def test_window_search_order(gpu, graph, interp):
+    # membership by aligned 64-byte windows (bisected or interpolated guesses): walks
+    # identical, probe counts equal the oracle's window order, which places the windows
+    # by the edge array's alignment -- slots keep each block's offset modulo 64 bytes, in
+    # one resident block and in out-of-core windows of 8 KB blocks (odd start edges)
+    beg, csr = G.rmat(12, 40000, seed=31, permute=graph == "permuted")
+    bl = Blocks(beg, csr, 8192)
+    assert any(int(bl.ecut[b]) % 16 for b in range(bl.nblocks))
+    for variant in (0, 2):
+        L = dict(interp=interp, window=True, variant=variant)
+        check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 3, 4096, launch=L)
+        check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1)], 4, 4096, launch=L)
+        check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.2, term=0.1, sampler=1)], 5, 4096, launch=L)
+        check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)], 6, 2048, block_bytes=8192, nslots=3,
+                   launch=L)
+        check_case(beg, csr, [dict(algo=SOPR, length=20, alpha=0.3, term=0.1)], 7, 2048, block_bytes=8192, nslots=3,
                    launch=L)
 
 
