@@ -1107,7 +1107,25 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
     const uint32_t lane = __lane_id();
     DevBlock one;
     if (ONE) one = a.blocks[0];
-    const uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    uint32_t k0 = a.seed_lo, k1 = a.seed_hi;
+    // The walker arrays, the worklist and the Philox key are used by some lane in almost
+    // every iteration.  In the interpolated-window instance (the out-of-core default, run
+    // at 4 waves per SIMD) they are held in VGPRs (the empty asm statement makes them
+    // opaque), which leaves the SGPRs to the type parameters and exec masks: otherwise
+    // those spill into VGPR lanes and come back through v_readlane in the loop (141 of the
+    // loop's 959 VALU instructions; C3 step kernel -2%).  The copies cost ~35 VGPRs, so
+    // the other instances keep the arguments in SGPRs: built for 64 VGPRs they would
+    // spill, and the bisected-window instance keeps 7 waves per SIMD for resident graphs
+    // (C2: 85 ms at 69 VGPRs, 93 ms at 101).
+    uint32_t* w_cur = a.cur;
+    uint32_t* w_prev = a.prev;
+    uint32_t* w_meta = a.meta;
+    uint32_t* w_bpair = a.bpair;
+    uint64_t* w_hash = a.hash;
+    const uint32_t* w_list = a.worklist;
+    if constexpr (SM == 3)
+        asm volatile("" : "+v"(w_cur), "+v"(w_prev), "+v"(w_meta), "+v"(w_bpair), "+v"(w_hash), "+v"(w_list),
+                     "+v"(k0), "+v"(k1));
     Lane L{};
     L.ph = LS_IDLE;
     bool exhausted = false;
@@ -1123,8 +1141,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             if (need) {
                 if (idx >= nwork) {
                     exhausted = true;
-                } else if (a.worklist) {
-                    L.addr = a.worklist + idx;
+                } else if (w_list) {
+                    L.addr = w_list + idx;
                     L.ph = LS_FETCH;
                 } else {
                     L.w = idx;
@@ -1147,11 +1165,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             q2 = q[2];
             q3 = q[3];
         } else if (L.ph == LS_STATE) {
-            L.meta = a.meta[L.w];
-            L.cur = a.cur[L.w];
-            L.prev = a.prev[L.w];
-            L.bcp = a.bpair[L.w];
-            L.hash = a.hash ? a.hash[L.w] : 0;
+            L.meta = w_meta[L.w];
+            L.cur = w_cur[L.w];
+            L.prev = w_prev[L.w];
+            L.bcp = w_bpair[L.w];
+            L.hash = w_hash ? w_hash[L.w] : 0;
         } else if (L.ph == LS_ROWS) {
             const uint64_t* pc = (const uint64_t*)L.addr;
             const uint64_t* pp = (const uint64_t*)L.nu;
@@ -1386,11 +1404,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             }
         }
         if (L.ph == LS_END) {
-            a.cur[L.w] = L.cur;
-            a.prev[L.w] = L.prev;
-            a.meta[L.w] = L.meta;
-            a.bpair[L.w] = L.bcp;
-            if (a.hash) a.hash[L.w] = L.hash;
+            w_cur[L.w] = L.cur;
+            w_prev[L.w] = L.prev;
+            w_meta[L.w] = L.meta;
+            w_bpair[L.w] = L.bcp;
+            if (w_hash) w_hash[L.w] = L.hash;
             if (((L.meta >> kStatusShift) & 0xFu) != kActive) ++c.ended; else ++c.left;
             L.ph = LS_IDLE;
         }
