@@ -38,27 +38,41 @@ __device__ __forceinline__ uint32_t bitlen64(uint64_t n) { return n ? 64u - (uin
 // to 64 bytes in memory, half a 128-byte line) that holds the guessed position.  Slots
 // place each block so that an entry's address is 4 x its index in the whole graph's edge
 // array modulo 64, so the pieces are those the oracle's window order reads.  scan_window
-// compares x with the entries [s, e) of the piece that belong to the remaining range:
-// found, how many are below x, and the first and last of them.
+// looks at the entries [s, e) of the piece that belong to the remaining range: whether x
+// is among them, and the first and last of them (x below the first or above the last
+// keeps the search on that side; x between them is absent).
 struct WinScan {
     bool found;
-    uint32_t below, first, last;
+    uint32_t first, last;
 };
+
+// entry i in [0, 16) of the window q0..q3, by a tree of selects on the bits of i (no
+// indexed registers, no private arrays)
+__device__ __forceinline__ uint32_t pick16(const uint4& q0, const uint4& q1, const uint4& q2, const uint4& q3,
+                                           uint32_t i)
+{
+    const bool b0 = i & 1u, b1 = i & 2u, b2 = i & 4u, b3 = i & 8u;
+    const uint32_t a0 = b0 ? q0.y : q0.x, a1 = b0 ? q0.w : q0.z, a2 = b0 ? q1.y : q1.x, a3 = b0 ? q1.w : q1.z;
+    const uint32_t a4 = b0 ? q2.y : q2.x, a5 = b0 ? q2.w : q2.z, a6 = b0 ? q3.y : q3.x, a7 = b0 ? q3.w : q3.z;
+    const uint32_t c0 = b1 ? a1 : a0, c1 = b1 ? a3 : a2, c2 = b1 ? a5 : a4, c3 = b1 ? a7 : a6;
+    const uint32_t d0 = b2 ? c1 : c0, d1 = b2 ? c3 : c2;
+    return b3 ? d1 : d0;
+}
+
+// bits 4j..4j+3: which of q's four entries equal x
+__device__ __forceinline__ uint32_t eq_bits(const uint4& q, uint32_t x, uint32_t j)
+{
+    return ((q.x == x ? 1u : 0u) | (q.y == x ? 2u : 0u) | (q.z == x ? 4u : 0u) | (q.w == x ? 8u : 0u)) << (4 * j);
+}
 
 __device__ __forceinline__ WinScan scan_window(const uint4& q0, const uint4& q1, const uint4& q2, const uint4& q3,
                                                uint32_t s, uint32_t e, uint32_t x)
 {
-    const uint32_t v[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                            q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-    WinScan r{false, 0u, 0u, 0u};
-#pragma unroll
-    for (uint32_t i = 0; i < 16; ++i) {
-        const bool in = i >= s && i < e;
-        r.found |= in && v[i] == x;
-        r.below += (in && v[i] < x) ? 1u : 0u;
-        r.first = i == s ? v[i] : r.first;
-        r.last = i + 1 == e ? v[i] : r.last;
-    }
+    const uint32_t eq = eq_bits(q0, x, 0) | eq_bits(q1, x, 1) | eq_bits(q2, x, 2) | eq_bits(q3, x, 3);
+    WinScan r;
+    r.found = ((eq >> s) & ((1u << (e - s)) - 1u)) != 0;
+    r.first = pick16(q0, q1, q2, q3, s);
+    r.last = pick16(q0, q1, q2, q3, e - 1);
     return r;
 }
 
@@ -98,10 +112,10 @@ __device__ __forceinline__ bool member_window(const uint32_t* __restrict__ a, ui
         const int64_t e = w0 + 16 < (int64_t)hi ? w0 + 16 : (int64_t)hi;
         const WinScan r = scan_window(q0, q1, q2, q3, (uint32_t)(s - w0), (uint32_t)(e - w0), x);
         if (r.found) return true;
-        if (r.below == 0) {
+        if (x < r.first) {
             hi = (uint64_t)s;
             vhi = r.first;
-        } else if (r.below == (uint32_t)(e - s)) {
+        } else if (x > r.last) {
             lo = (uint64_t)e;
             vlo = r.last + 1;
         } else {
@@ -1261,11 +1275,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
                 const WinScan r = scan_window(q0, q1, q2, q3, (uint32_t)(s0 - w0), (uint32_t)(e0 - w0), L.x);
                 found = r.found;
                 if (!found) {
-                    if (r.below == 0) {
+                    if (L.x < r.first) {
                         L.hx = (uint32_t)s0;
                         L.vhi = r.first;
                         done = false;
-                    } else if (r.below == (uint32_t)(e0 - s0)) {
+                    } else if (L.x > r.last) {
                         L.lo = (uint32_t)e0;
                         L.vlo = r.last + 1;
                         done = false;
