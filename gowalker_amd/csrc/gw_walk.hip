@@ -1040,7 +1040,7 @@ enum : uint32_t {
     LS_PROPU,   // SOPR: proposal y = N(prev)[pos]
     LS_PROBE,   // membership probe
     // compute states
-    LS_MOVE, LS_STEP, LS_TRIAL, LS_END
+    LS_GUESS, LS_MOVE, LS_STEP, LS_TRIAL, LS_END
 };
 
 constexpr uint32_t kLaneBisect = 1u, kLaneMv = 2u, kLaneMu = 4u;   // Lane::fl bits
@@ -1087,9 +1087,10 @@ __device__ __forceinline__ const void* lane_probe_addr(const Lane& L, const uint
     return (SM & 2) ? (const void*)window_of(p) : (const void*)p;
 }
 
-// starts the membership search of L.x in list[0, n); false = decided (absent) without a probe
+// starts the membership search of L.x in a list of n entries (the first probe's address
+// is computed in LS_GUESS); false = decided (absent) without a probe
 template <uint32_t SM>
-__device__ __forceinline__ bool lane_search(Lane& L, const uint32_t* list, uint32_t n, uint32_t id_end)
+__device__ __forceinline__ bool lane_search(Lane& L, uint32_t n, uint32_t id_end)
 {
     if (n == 0 || ((SM & 1) && L.x >= id_end)) return false;
     L.lo = 0;
@@ -1097,7 +1098,6 @@ __device__ __forceinline__ bool lane_search(Lane& L, const uint32_t* list, uint3
     L.vlo = 0;
     L.vhi = id_end;
     L.fl &= ~kLaneBisect;
-    L.addr = lane_probe_addr<SM>(L, list);
     return true;
 }
 
@@ -1238,10 +1238,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
                 L.ph = LS_MOVE;
             } else if (L.ra > tp.m_hi) {
                 L.ph = LS_TRIAL;
-            } else if (lane_search<SM>(L, L.nu, L.du, a.search_end)) {
+            } else if (lane_search<SM>(L, L.du, a.search_end)) {
                 ++c.probes;
                 if (a.zc_count && (L.fl & kLaneMu)) ++c.zc;
-                L.ph = LS_PROBE;
+                L.ph = LS_GUESS;
             } else {
                 L.ph = L.ra <= tp.m_q ? LS_MOVE : LS_TRIAL;
             }
@@ -1253,10 +1253,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             break;
         case LS_PROPU:   // SOPR: a prev-neighbour y != prev is accepted iff y is in N(cur)
             L.x = v;
-            if (L.x != L.prev && lane_search<SM>(L, L.nv, L.dv, a.search_end)) {
+            if (L.x != L.prev && lane_search<SM>(L, L.dv, a.search_end)) {
                 ++c.probes;
                 if (a.zc_count && (L.fl & kLaneMv)) ++c.zc;
-                L.ph = LS_PROBE;
+                L.ph = LS_GUESS;
             } else {
                 ++L.k;
                 L.ph = LS_TRIAL;
@@ -1302,7 +1302,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             if (!done) {
                 if (L.lo < L.hx) {
                     if (SM & 1) L.fl = ((uint64_t)(L.hx - L.lo) * 2 > len) ? (L.fl | kLaneBisect) : (L.fl & ~kLaneBisect);
-                    L.addr = lane_probe_addr<SM>(L, list);
+                    L.ph = LS_GUESS;
                     ++c.probes;
                     if (a.zc_count && (L.fl & (sopr ? kLaneMv : kLaneMu))) ++c.zc;
                 } else {
@@ -1325,6 +1325,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
         }
 
         // ---- compute states, in the order they can follow each other
+        if (L.ph == LS_GUESS) {   // next probe of a membership search (one site for every phase)
+            L.addr = lane_probe_addr<SM>(L, tp.algo != 0 ? L.nv : L.nu);
+            L.ph = LS_PROBE;
+        }
         if (L.ph == LS_MOVE) {
             const uint32_t bc = L.bcp & 0xFFFFu;
             if (a.block_steps) {
