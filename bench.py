@@ -161,7 +161,7 @@ def ooc_schedulers(E, block_bytes, ablation=False, only=None):
     s = [("plain", E.SCHED_PLAIN, 0, 0), ("plain_zero", E.SCHED_PLAIN, block_bytes // 128 // 24, 0),
          ("sowalker", E.SCHED_SOWALKER, 0, 0), ("learned", E.SCHED_LEARNED, 0, 0)]
     if ablation:
-        s += [("learned_swap_prefetch", E.SCHED_LEARNED, 0, 1), ("learned_myopic", E.SCHED_LEARNED, 0, 2)]
+        s += [("learned_no_swap_prefetch", E.SCHED_LEARNED, 0, 1), ("learned_myopic", E.SCHED_LEARNED, 0, 2)]
     if only:
         s = [x for x in s if x[0] in only]
     return s

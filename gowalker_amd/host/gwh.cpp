@@ -48,7 +48,7 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         LearnedScheduler* ls = nullptr;
         if (opts->scheduler == GWH_SCHED_LEARNED) {
             LearnedConfig lc;
-            lc.swap_prefetch = (opts->flags & GWH_SWAP_PREFETCH) != 0;
+            lc.swap_prefetch = (opts->flags & GWH_NO_SWAP_PREFETCH) == 0;
             lc.myopic_label = (opts->flags & GWH_MYOPIC_LABEL) != 0;
             if (opts->prefetch_depth) lc.max_inflight = opts->prefetch_depth;
             ls = new LearnedScheduler(engine.ctx(), lc);

@@ -17,9 +17,10 @@ extern "C" {
 enum { GWH_SCHED_PLAIN = 0, GWH_SCHED_LEARNED = 1, GWH_SCHED_SOWALKER = 2 };
 /* gwh_engine_opts.flags */
 enum {
-    GWH_SWAP_PREFETCH = 1u << 0, /* learned: once the cache is full, keep a slot for staging the
-                                    block the plain rule would load next (overlaps each upload
-                                    with the previous run; costs a slot and mispredicted copies) */
+    GWH_NO_SWAP_PREFETCH = 1u << 0, /* learned: once the cache is full, do not keep a slot for
+                                       staging the block the plain rule would load next (the
+                                       default keeps one: each upload then overlaps the previous
+                                       run, for a slot and some mispredicted copies) */
     GWH_MYOPIC_LABEL = 1u << 1   /* learned: a load's label is the next run's steps only */
 };
 
