@@ -221,6 +221,27 @@ def test_window_search_order(gpu, graph, interp):
                    launch=L)
 
 
+@pytest.mark.parametrize("interp", [False, True])
+def test_window_trial_skips(gpu, interp):
+    # the window lane kernel rejects a Node2vec trial without reading its candidate when
+    # only x == prev could accept it and the step's earlier reads place prev's positions
+    # elsewhere: walks and trial / probe counts equal the oracle's on directed graphs (prev
+    # often not in N(cur)), multigraphs (prev repeated in N(cur)), tiny graphs, both
+    # threshold orders and mixed walker types
+    L = dict(variant=2, window=True, interp=interp)
+    for multi in (False, True):
+        beg, csr = G.power_law_directed(3000, 40000, seed=2, multi=multi)
+        for p, q in ((0.5, 2.0), (2.0, 0.25), (0.25, 4.0)):
+            check_case(beg, csr, [dict(algo=NODE2VEC, length=20, p=p, q=q)], 12, 6000, wpv=2, launch=L)
+    for name in ("star", "bridge", "triangle"):
+        beg, csr = getattr(G, name)()
+        n = len(beg) - 1
+        check_case(beg, csr, [dict(algo=NODE2VEC, length=12, p=0.5, q=2.0)], 13, n * 8, wpv=8, launch=L)
+    beg, csr = G.rmat(11, 20000, seed=9, undirected=False)
+    types = [dict(algo=NODE2VEC, length=20, p=0.5, q=2.0), dict(algo=SOPR, length=20, alpha=0.2, term=0.2)]
+    check_case(beg, csr, types, 14, 4096, wpv=2, block_bytes=8192, launch=L)
+
+
 @pytest.mark.parametrize("variant", [0, 2])
 def test_block_tables_in_lds_and_global(gpu, variant):
     # up to 128 blocks the step kernels stage the table, its cut points and a vertex->block
