@@ -1397,9 +1397,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
         // x == prev alone; with sorted lists, a candidate position outside [blo, bhi) is not
         // prev (the candidates read in this step bound prev's positions), so such a trial is
         // rejected without a read and, in the window instances, the next trial is drawn in
-        // the same iteration (up to kSkipPasses times; about 0.8 of C3's 3.7 candidate reads
-        // per step).  Trials are counted as before.
-        constexpr uint32_t kSkipPasses = (SM & 2) ? 2u : 0u;
+        // the same iteration (once per iteration: C3's step kernel took 268 ms with one such
+        // pass, 269 ms with two, 274 ms with four; the rule applies to about 0.8 of its 3.7
+        // candidate reads per step).  Trials are counted as before.
+        constexpr uint32_t kSkipPasses = (SM & 2) ? 1u : 0u;
         for (uint32_t pass = 0; L.ph == LS_TRIAL; ++pass) {
             if (L.k >= kMaxTrials) {
                 L.meta |= kDoneTrialCap << kStatusShift;
