@@ -18,8 +18,10 @@
 //   * SOPR (gpuwalk.hpp:241-266): P(x) ~ (1-a)/dv + a/du [x in N(prev), x != prev] with
 //     a = A/2^16, sampled by envelope rejection or, when the envelope is loose, by an
 //     exact two-component mixture after counting |N(cur) & N(prev)| (see sopr_step).
-//   * Membership: textbook binary search over prev's sorted adjacency
-//     (engine/engine.hpp:224-243, not the defective device search of gpuwalk.hpp:26-48).
+//   * Membership: the answer of the textbook binary search over prev's sorted adjacency
+//     (engine/engine.hpp:224-243, not the defective device search of gpuwalk.hpp:26-48),
+//     found by bisection, guarded interpolation or aligned 64-byte window reads (search
+//     mode SM); the oracle restates every order, probe counts included.
 #include "gw_device.h"
 
 namespace gw {
