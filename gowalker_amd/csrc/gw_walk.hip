@@ -1390,8 +1390,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
                 L.ph = LS_END;
             } else {
                 L.k = 0;
-                L.blo = 0;
-                L.bhi = L.dv;
+                if (SM & 2) {
+                    L.blo = 0;
+                    L.bhi = L.dv;
+                }
                 L.ph = LS_TRIAL;
             }
         }
@@ -1403,7 +1405,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
         // pass, 269 ms with two, 274 ms with four; the rule applies to about 0.8 of its 3.7
         // candidate reads per step).  Trials are counted as before.
         constexpr uint32_t kSkipPasses = (SM & 2) ? 1u : 0u;
-        for (uint32_t pass = 0; L.ph == LS_TRIAL; ++pass) {
+        for (uint32_t pass = 0; pass <= kSkipPasses && L.ph == LS_TRIAL; ++pass) {
             if (L.k >= kMaxTrials) {
                 L.meta |= kDoneTrialCap << kStatusShift;
                 L.ph = LS_END;
