@@ -54,3 +54,12 @@ def test_split_blocks_product_matches_oracle():
         vb, eb = O.split_blocks(beg, bs)
         np.testing.assert_array_equal(b.vcut, vb)
         np.testing.assert_array_equal(b.ecut, eb)
+
+
+def test_python_flag_constants_match_header():
+    from gowalker_amd import _abi
+    src = open(os.path.join(REPO, "include", "gw.h")).read()
+    flags = {m.group(1): 1 << int(m.group(2)) for m in re.finditer(r"\b(GW_RUN_[A-Z]+)\s*=\s*1u\s*<<\s*(\d+)", src)}
+    assert {"GW_RUN_BUCKETS", "GW_RUN_TRACE", "GW_RUN_INTERP", "GW_RUN_WINDOW"} <= set(flags)
+    for name, value in flags.items():
+        assert getattr(_abi, name) == value, name
