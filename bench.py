@@ -2,20 +2,23 @@
 """Walk-throughput benchmark of the MI355X walk engine (one GPU).
 
 Default workload (Node2vec out of core, learned vs plain block scheduler):
-  undirected RMAT graph, 2^26 ids with 1.8B generated edges (a,b,c,d = .57,.19,.19,.05)
-  symmetrized, deduplicated, ids randomly permuted (3.5B arcs, generated on the GPU, seed
-  42), cut into 256 MB blocks; HBM block cache capped at 16 GB (the stated size, where
-  the whole CSR fits) and at 8 GB (where blocks must be evicted); p 0.5, q 2, one walker
-  per vertex, length 40.  One timed step = one full walk of every walker through the
-  out-of-core engine (graph already in pinned host memory), for each scheduler.
-  value = steps/s of the learned scheduler at the first cap; the plain scheduler (the
-  reference's max_choose_blocks), the same with the reference's zero-copy tail, H2D and
-  zero-copy bytes are in config.caps.
+  undirected RMAT graph, 2^26 ids (a,b,c,d = .57,.19,.19,.05), symmetrized, deduplicated,
+  self-loops dropped, ids randomly permuted; 1.856B edges are drawn so that at least the
+  stated 3.6B arcs remain after deduplication (generated on the GPU, seed 42); cut into
+  256 MB blocks; HBM block cache capped at 16 GB (the stated size, where the whole CSR
+  fits) and at 8 GB (where blocks must be evicted); p 0.5, q 2, one walker per vertex,
+  length 40.  One timed step = one full walk of every walker through the out-of-core
+  engine (graph already in pinned host memory).  At 16 GB the learned scheduler, the same
+  code without its network (staging), the plain scheduler (the reference's
+  max_choose_blocks) and the plain scheduler with the reference's zero-copy tail each run
+  --warmup untimed and --steps timed walks, interleaved; SOWalker and the 8 GB cap run one
+  timed walk per scheduler.  value = steps/s of the learned scheduler at 16 GB; the others,
+  H2D and zero-copy bytes are in config.caps.
 
 Other workloads (--config): sopr_chung_lu (SOPR on a directed Chung-Lu graph, 41.6M V /
 1.47B arcs, 4 walkers/vertex, term 0.15, fully HBM-resident), node2vec_rmat20 (the
-reference's CPU-runnable case, single resident block) and node2vec_rmat24_ooc (reduced
-out-of-core case).
+reference's CPU-runnable case, single resident block), mixed_rmat27_ooc (Node2vec + SOSR
+out of core, RMAT scale 27, >= 5.5B arcs) and node2vec_rmat24_ooc (reduced out-of-core case).
 
 Prints one JSON line: walk steps/s plus the roofline of the step kernel (algorithmic bytes
 per launch over its HIP-event time vs the 8 TB/s HBM peak) and a CPU baseline (the
@@ -40,10 +43,11 @@ CONFIGS = {
         graph=dict(kind="chung_lu", nverts=41_600_000, nedges=1_470_000_000, gamma=2.2, max_degree=3_000_000),
         types=[dict(algo=1, length=30, alpha=0.2, term=0.15)], wpv=4),
     "node2vec_rmat26_ooc": dict(
-        workload="Node2vec out of core, RMAT scale 26 (67.1M ids) with 1.8B generated edges symmetrized, dedup, "
-                 "random id permutation, 256 MB blocks, HBM cap 16 GB (and 8 GB), p 0.5, q 2, 1 walker/vertex, "
-                 "length 40; plain vs learned block scheduler",
-        graph=dict(kind="rmat", scale=26, nedges=1_800_000_000, undirected=True, permute=True),
+        workload="Node2vec out of core, RMAT scale 26 (67.1M ids), edges drawn until >= 3.6B arcs remain after "
+                 "symmetrizing and dedup, random id permutation, 256 MB blocks, HBM cap 16 GB (and 8 GB), p 0.5, q 2, "
+                 "1 walker/vertex, length 40; learned vs plain block scheduler",
+        graph=dict(kind="rmat", scale=26, nedges=1_856_000_000, min_arcs=3_600_000_000, undirected=True,
+                   permute=True),
         types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=256, memory_gb=[16, 8]),
     "node2vec_rmat24_ooc": dict(
         workload="Node2vec out of core (reduced), RMAT scale 24 with 400M generated edges symmetrized, dedup, "
@@ -52,11 +56,12 @@ CONFIGS = {
         graph=dict(kind="rmat", scale=24, nedges=400_000_000, undirected=True, permute=True),
         types=[dict(algo=0, length=40, p=0.5, q=2.0)], wpv=1, out_of_core=True, block_mb=64, memory_gb=[3]),
     "mixed_rmat27_ooc": dict(
-        workload="Mixed batch out of core: RMAT scale 27 (134.2M ids) with 5.5B generated directed edges, dedup, "
-                 "ids in generation order (locality kept), 256 MB blocks, HBM cap 16 GB; 2 walkers/vertex: "
-                 "Node2vec p 1, q 0.25, length 80 and SOSR alpha 0.2, termination 0.2 (length cap 80) "
-                 "co-scheduled in one kernel",
-        graph=dict(kind="rmat", scale=27, nedges=5_500_000_000, undirected=False, permute=False),
+        workload="Mixed batch out of core: RMAT scale 27 (134.2M ids), directed edges drawn until >= 5.5B arcs "
+                 "remain after dedup, ids in generation order (locality kept), 256 MB blocks, HBM cap 16 GB; "
+                 "2 walkers/vertex: Node2vec p 1, q 0.25, length 80 and SOSR alpha 0.2, termination 0.2 (length "
+                 "cap 80) co-scheduled in one kernel",
+        graph=dict(kind="rmat", scale=27, nedges=5_630_000_000, min_arcs=5_500_000_000, undirected=False,
+                   permute=False),
         types=[dict(algo=0, length=80, p=1.0, q=0.25), dict(algo=1, length=80, alpha=0.2, term=0.2)], wpv=2,
         out_of_core=True, block_mb=256, memory_gb=[16]),
     "node2vec_rmat20": dict(
@@ -93,11 +98,25 @@ def sector_bytes(st, nwalkers, visits):
 
 
 def build_graph(ctx, g, seed):
+    """The configuration's graph in pinned host memory, and the edge count drawn.  RMAT
+    draws `nedges` edges; when fewer than `min_arcs` arcs survive deduplication it draws
+    again, scaled by the shortfall (the generator is deterministic in (seed, edge index),
+    so a larger draw extends the smaller one)."""
     from gowalker_amd import engine as E
     if g["kind"] == "chung_lu":
-        return E.gen_chung_lu(ctx, g["nverts"], g["nedges"], g["gamma"], g["max_degree"], seed, pinned=True)
-    return E.gen_rmat(ctx, g["scale"], g["nedges"], seed=seed, undirected=g["undirected"], permute=g["permute"],
-                      pinned=True)
+        return E.gen_chung_lu(ctx, g["nverts"], g["nedges"], g["gamma"], g["max_degree"], seed, pinned=True) + \
+            (g["nedges"],)
+    drawn = g["nedges"]
+    while True:
+        beg, csr = E.gen_rmat(ctx, g["scale"], drawn, seed=seed, undirected=g["undirected"], permute=g["permute"],
+                              pinned=True)
+        if csr.n >= g.get("min_arcs", 0):
+            return beg, csr, drawn
+        print("[bench] %d edges drawn left %d arcs (< %d): drawing more" % (drawn, csr.n, g["min_arcs"]),
+              file=sys.stderr, flush=True)
+        drawn = int(drawn * g["min_arcs"] / csr.n * 1.002) + 1
+        beg.close()
+        csr.close()
 
 
 def cpu_baseline(beg, csr, cfg, seconds, threads):
@@ -151,123 +170,205 @@ def cpu_baseline(beg, csr, cfg, seconds, threads):
 LDS_BLOCKS = 128          # block tables up to this size are staged in LDS by the step kernel
 PCIE_PEAK_GBS = 64.0   # PCIe Gen5 x16, one direction
 
+# name -> (scheduler, zero-copy threshold rule, gwh flags); the zero-copy threshold of
+# plain_zero is the reference's `zero` option (blocksize/128/24 walkers, apps/Node2vec.cu:100-104)
+SCHEDULERS = {
+    "learned": ("SCHED_LEARNED", False, 0),         # the MLP scheduler (network carried across walks)
+    "staging": ("SCHED_STAGING", False, 0),         # the same staging/prefetch/tail code, no network
+    "plain": ("SCHED_PLAIN", False, 0),             # the reference's max_choose_blocks, no zero-copy tail
+    "plain_zero": ("SCHED_PLAIN", True, 0),         # ... with the reference's zero-copy tail
+    "sowalker": ("SCHED_SOWALKER", False, 0),       # the reference's simulated-annealing scheduler
+    "learned_model_swap": ("SCHED_LEARNED", False, 4),        # ablation: the network picks swap loads
+    "learned_model_swap_myopic": ("SCHED_LEARNED", False, 6),  # ... trained on next-run labels
+    "learned_value_victim": ("SCHED_LEARNED", False, 12),     # ... and victims by value
+    "learned_no_swap_prefetch": ("SCHED_LEARNED", False, 1),  # ablation: no staging slot
+    "staging_no_sticky": ("SCHED_STAGING", False, 16),        # ablation: no preference for the staged block
+}
+HEADLINE = ["learned", "staging", "plain", "plain_zero"]   # timed --warmup + --steps walks each
+ONCE = ["sowalker"]                                        # one timed walk at the headline cap
+EXTRA = ["learned", "staging", "learned_model_swap", "plain_zero", "plain", "sowalker"]   # one timed walk
+                                                                            # each at other caps
 
-def ooc_schedulers(E, block_bytes, ablation=False, only=None):
-    # plain: the reference's max_choose_blocks without the optional zero-copy tail;
-    # plain_zero: the same with the reference's `zero` option (threshold blocksize/128/24
-    # walkers, apps/Node2vec.cu:100-104); sowalker: the reference's simulated-annealing
-    # scheduler (engine/schedule.hpp:369-635); learned: the MLP scheduler (with --ablation
-    # also with a staging slot once the cache is full, and with next-run labels)
-    s = [("plain", E.SCHED_PLAIN, 0, 0), ("plain_zero", E.SCHED_PLAIN, block_bytes // 128 // 24, 0),
-         ("sowalker", E.SCHED_SOWALKER, 0, 0), ("learned", E.SCHED_LEARNED, 0, 0)]
-    if ablation:
-        s += [("learned_no_swap_prefetch", E.SCHED_LEARNED, 0, 1), ("learned_myopic", E.SCHED_LEARNED, 0, 2)]
-    if only:
-        s = [x for x in s if x[0] in only]
-    return s
+
+def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models):
+    sched, zero, flags = SCHEDULERS[name]
+    block_bytes = cfg["block_mb"] << 20
+    launch = None
+    if args.variant is not None or args.waves_per_cu or args.window:
+        launch = dict(variant=2 if args.variant is None else args.variant, waves_per_cu=args.waves_per_cu,
+                      window=args.window)
+    # every walk keeps the same outputs (the per-walker path hash, 8 B per walker counted
+    # against the cap), so warm-up and timed walks run with the same slot count
+    rep, h, _ = E.run_out_of_core(beg, csr, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
+                                  memory_bytes=int(cap_gb * (1 << 30)), block_bytes=block_bytes,
+                                  zero_threshold=block_bytes // 128 // 24 if zero else 0,
+                                  scheduler=getattr(E, sched), hashes=True, flags=flags, launch=launch,
+                                  prefetch_depth=args.prefetch_depth, verbose=args.verbose,
+                                  model=models.setdefault(name, E.Model()) if getattr(E, sched) == E.SCHED_LEARNED
+                                  else None)
+    return rep, h
 
 
-def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s):
-    """Out-of-core configuration: the C++ engine under each scheduler, at the stated HBM cap
-    and at the tighter caps listed with it.  One timed step = one full walk of every walker."""
+def summarize(reps, zt):
+    r0 = reps[0]
+    wall = sum(r["wall_s"] for r in reps)
+    zc_gb = r0["zc_touches"] * 64 / 1e9
+    return dict(reps=len(reps), steps_per_s=sum(r["steps"] for r in reps) / wall, ms_per_run=wall * 1e3 / len(reps),
+                ms_min=min(r["wall_s"] for r in reps) * 1e3, ms_max=max(r["wall_s"] for r in reps) * 1e3,
+                h2d_gb=r0["h2d_bytes"] / 1e9, zero_copy_gb_est=zc_gb, pcie_gb_total=r0["h2d_bytes"] / 1e9 + zc_gb,
+                h2d_gbs=(r0["h2d_bytes"] / 1e9) / (r0["h2d_ms"] / 1e3) if r0["h2d_ms"] else None,
+                iterations=r0["iterations"], uploads=r0["uploads"], staged=r0["staged"],
+                zero_copy_steps=r0["zero_copy_steps"], kernel_ms=r0["kernel_ms"], copy_ms=r0["h2d_ms"],
+                sched_ms=r0["sched_ms"], learned_decisions=r0["learned_decisions"], fill_model=r0["fill_model"],
+                swap_model=r0["swap_model"], swap_differs=r0["swap_differs"], tail_model=r0["tail_model"],
+                tail_threshold=r0["tail_threshold"], model_samples=r0["model_samples"], fallbacks=r0["fallbacks"],
+                nslots=r0["nslots"], nblocks=r0["nblocks"], steps=r0["steps"], trials=r0["trials"],
+                probes=r0["probes"], zero_threshold=zt)
+
+
+def log_walk(cap_gb, name, tag, rep):
+    print("[bench] cap %g GB %s %s: %.3f s, %d steps, %.2f GB H2D (%d uploads + %d staged), %.3f GB zero-copy, "
+          "%d iterations, kernel %.0f ms, copy %.0f ms, sched %.0f ms, %d slots / %d blocks, model: fill %d swap %d "
+          "(%d differ) tail %d/%d samples %d, sticky %d"
+          % (cap_gb, name, tag, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["uploads"], rep["staged"],
+             rep["zc_touches"] * 64 / 1e9, rep["iterations"], rep["kernel_ms"], rep["h2d_ms"], rep["sched_ms"],
+             rep["nslots"], rep["nblocks"], rep["fill_model"], rep["swap_model"], rep["swap_differs"],
+             rep["tail_model"], rep["tail_threshold"], rep["model_samples"], rep["sticky_swaps"]),
+          file=sys.stderr, flush=True)
+
+
+def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
+    """Out-of-core configuration: the C++ engine under each scheduler.  One timed step = one
+    full walk of every walker.  At the headline cap the learned, staging, plain and
+    plain_zero schedulers each run --warmup untimed and --steps timed walks, interleaved
+    walk by walk; SOWalker and the other caps run one timed walk each (the learned one
+    after --extra-warmup untimed walks).  A wall-clock guard (--time-budget) stops adding
+    headline walks when the budget would be overrun, and the line reports the walks done."""
+    import numpy as np
+
     from gowalker_amd import engine as E
     ctx.close()
-    caps = [args.memory_gb] if args.memory_gb else cfg["memory_gb"]
+    caps = args.caps or cfg["memory_gb"]
     block_bytes = cfg["block_mb"] << 20
+    zt = block_bytes // 128 // 24
     nwalkers = nverts * cfg["wpv"]
+    models = {}      # one network per learned variant, carried across its walks
+    ref_hash = None
+    identical = True
+
+    def check_hash(h):
+        nonlocal ref_hash, identical
+        if ref_hash is None:
+            ref_hash = h
+        elif not np.array_equal(h, ref_hash):
+            identical = False
+
+    head = [n for n in (args.schedulers or HEADLINE)]
+    cap0 = caps[0]
+    reps = {n: [] for n in head}
+    done_rounds = 0
+    round_s = []
+    reserve = (0 if args.no_cpu_baseline else args.cpu_seconds * 2.2 + 5) + (40 * (len(caps) - 1))
+    for i in range(args.warmup + args.steps):
+        elapsed = time.perf_counter() - t_start
+        est = max(round_s) if round_s else 0.0
+        if i >= args.warmup and elapsed + est + reserve > args.time_budget:
+            print("[bench] time budget: %.0f s used, stopping after %d timed walks per scheduler"
+                  % (elapsed, done_rounds), file=sys.stderr, flush=True)
+            break
+        t0 = time.perf_counter()
+        for name in head:
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models)
+            check_hash(h)
+            log_walk(cap0, name, "warmup %d" % i if i < args.warmup else "run %d" % (i - args.warmup), rep)
+            if i >= args.warmup:
+                reps[name].append(rep)
+        round_s.append(time.perf_counter() - t0)
+        if i >= args.warmup:
+            done_rounds += 1
     per_cap = []
-    hashes = {}
-    learned_reps = None
-    for cap_gb in caps:
-        mem = int(cap_gb * (1 << 30))
+    res0 = {n: summarize(reps[n], zt if SCHEDULERS[n][1] else 0) for n in head if reps[n]}
+    if not args.schedulers:
+        for name in ONCE:
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models)
+            check_hash(h)
+            log_walk(cap0, name, "run 0", rep)
+            res0[name] = summarize([rep], 0)
+    per_cap.append({"memory_gb": cap0, "schedulers": res0})
+    for cap_gb in caps[1:]:
         res = {}
-        for name, sched, zt, flags in ooc_schedulers(E, block_bytes, args.ablation, args.schedulers):
-            reps = []
-            for i in range(args.warmup + args.steps):
-                first = (cap_gb, name) not in hashes
-                rep, h, _ = E.run_out_of_core(beg.a, csr.a, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
-                                              memory_bytes=mem, block_bytes=block_bytes, zero_threshold=zt,
-                                              scheduler=sched, hashes=first, flags=flags,
-                                              launch=None if args.variant is None and not args.waves_per_cu
-                                              and not args.window else
-                                              dict(variant=2 if args.variant is None else args.variant,
-                                                   waves_per_cu=args.waves_per_cu, window=args.window),
-                                              prefetch_depth=args.prefetch_depth, verbose=args.verbose)
-                if first:
-                    hashes[(cap_gb, name)] = h
-                print("[bench] cap %g GB %s run %d: %.3f s, %d steps, %.2f GB H2D (%d uploads + %d staged), "
-                      "%.3f GB zero-copy, %d iterations, kernel %.0f ms, copy %.0f ms, sched %.0f ms, tune %.0f ms %s, "
-                      "%d slots / %d blocks"
-                      % (cap_gb, name, i, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["uploads"],
-                         rep["staged"], rep["zc_touches"] * 64 / 1e9, rep["iterations"], rep["kernel_ms"],
-                         rep["h2d_ms"], rep["sched_ms"], rep["tune_ms"], rep["tuned"], rep["nslots"], rep["nblocks"]),
-                      file=sys.stderr, flush=True)
-                if i >= args.warmup:
-                    reps.append(rep)
-            r0 = reps[0]
-            wall = sum(r["wall_s"] for r in reps)
-            zc_gb = r0["zc_touches"] * 64 / 1e9
-            res[name] = dict(steps_per_s=sum(r["steps"] for r in reps) / wall, ms_per_run=wall * 1e3 / len(reps),
-                             h2d_gb=r0["h2d_bytes"] / 1e9, zero_copy_gb_est=zc_gb,
-                             pcie_gb_total=r0["h2d_bytes"] / 1e9 + zc_gb,
-                             h2d_gbs=(r0["h2d_bytes"] / 1e9) / (r0["h2d_ms"] / 1e3) if r0["h2d_ms"] else None,
-                             iterations=r0["iterations"], uploads=r0["uploads"], staged=r0["staged"],
-                             zero_copy_steps=r0["zero_copy_steps"], kernel_ms=r0["kernel_ms"],
-                             copy_ms=r0["h2d_ms"], sched_ms=r0["sched_ms"], learned_decisions=r0["learned_decisions"],
-                             tail_decisions=r0["tail_decisions"], fallbacks=r0["fallbacks"], nslots=r0["nslots"],
-                             nblocks=r0["nblocks"], steps=r0["steps"], trials=r0["trials"], probes=r0["probes"],
-                             zero_threshold=zt)
-            if learned_reps is None and (name == "learned" or (args.schedulers and name == args.schedulers[0])):
-                learned_reps = reps
-        names = [n for n, _, _, _ in ooc_schedulers(E, block_bytes, args.ablation, args.schedulers)]
-        ref = hashes[(cap_gb, names[0])]
-        ident = all(bool((hashes[(cap_gb, n)] == ref).all()) for n in names)
-        entry = {"memory_gb": cap_gb, "walks_identical": ident, "schedulers": res}
-        for other in ("plain", "plain_zero", "sowalker"):
+        for name in (args.schedulers or EXTRA):
+            if name.startswith("learned"):
+                for w in range(args.extra_warmup):
+                    rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models)
+                    check_hash(h)
+                    log_walk(cap_gb, name, "warmup %d" % w, rep)
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models)
+            check_hash(h)
+            log_walk(cap_gb, name, "run 0", rep)
+            res[name] = summarize([rep], zt if SCHEDULERS[name][1] else 0)
+        per_cap.append({"memory_gb": cap_gb, "schedulers": res})
+    for entry in per_cap:
+        res = entry["schedulers"]
+        for other in ("staging", "plain", "plain_zero", "sowalker"):
             if "learned" in res and other in res:
                 entry["learned_vs_%s_steps_per_s" % other] = res["learned"]["steps_per_s"] / res[other]["steps_per_s"]
                 entry["learned_vs_%s_pcie_bytes" % other] = res["learned"]["pcie_gb_total"] / res[other]["pcie_gb_total"]
-        per_cap.append(entry)
-    lead = "learned" if "learned" in per_cap[0]["schedulers"] else names[0]
-    head = per_cap[0]["schedulers"][lead]
-    r0 = learned_reps[0]
+        if "staging" in res and "plain" in res:
+            entry["staging_vs_plain_steps_per_s"] = res["staging"]["steps_per_s"] / res["plain"]["steps_per_s"]
+    lead = "learned" if "learned" in res0 else head[0]
+    headrec = res0[lead]
+    r0 = reps[lead][0]
     launches = max(1, r0["iterations"])
     abytes = algorithmic_bytes(r0, nwalkers, False) / launches      # per launch, averaged
     achieved = abytes / (r0["kernel_ms"] / launches * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_note = None, None
     if os.path.exists(args.traffic_json):
         try:
-            traffic = json.load(open(args.traffic_json)).get(args.config)
+            tj = json.load(open(args.traffic_json)).get(args.config)
         except Exception:
-            traffic = None
+            tj = None
+        if tj and tj.get("bytes_per_step"):
+            traffic = tj["bytes_per_step"] * r0["steps"] / launches
+            traffic_note = tj.get("source")
+    mi = {n: m.info() for n, m in models.items()}
+    for m in models.values():
+        m.close()
     line = {
         "metric": "walk steps/s on 1 MI355X + H2D GB, learned vs plain block scheduler",
-        "value": head["steps_per_s"],
+        "value": headrec["steps_per_s"],
         "unit": "steps/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": head["ms_per_run"],
+        "n_gpus": 1, "steps": done_rounds, "warmup": args.warmup,
+        "ms_per_step": headrec["ms_per_run"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic (generated on the GPU, seed %d)" % args.seed,
-        "config": {"workload": cfg["workload"], "nverts": nverts, "narcs": narcs, "walkers": nwalkers,
-                   "caps": per_cap, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS,
-                   "parallelism": "1 GPU"},
+        "config": {"workload": cfg["workload"], "nverts": nverts, "narcs": narcs,
+                   "non_isolated": int(np.count_nonzero(beg[1:] != beg[:-1])), "walkers": nwalkers,
+                   "edges_drawn": cfg.get("edges_drawn"), "caps": per_cap, "walks_identical": identical,
+                   "model": mi, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS, "parallelism": "1 GPU",
+                   "steps_requested": args.steps, "launch": r0.get("tuned")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_over_algorithmic": traffic / abytes if traffic else None,
                      "kernel": "%s<false, %s, %s, %s>" % ("walk_lane_kernel" if (r0.get("tuned") or {}).get("variant") == 2
                                                            else "walk_kernel",
                                                            search_mode(r0.get("tuned")),
                                                            "true" if r0["nblocks"] <= LDS_BLOCKS else "false",
                                                            "true" if len(cfg["types"]) == 2 else "false"),
                      "kernel_ms": r0["kernel_ms"] / launches, "algorithmic_bytes": abytes,
-                     "sector_bytes": sector_bytes(r0, nwalkers, False) / launches,
-                     "launches": launches,
-                     "note": "per launch, averaged over the step-kernel launches of one %s walk at %g GB" %
-                             (lead, per_cap[0]["memory_gb"])},
+                     "sector_bytes": sector_bytes(r0, nwalkers, False) / launches, "launches": launches,
+                     "per_step": {"steps_per_launch": r0["steps"] / launches,
+                                  "algorithmic_bytes": algorithmic_bytes(r0, nwalkers, False) / r0["steps"],
+                                  "traffic_bytes": (traffic * launches / r0["steps"]) if traffic else None},
+                     "traffic_source": traffic_note,
+                     "note": "per launch, averaged over the step-kernel launches of the first timed %s walk at %g GB; "
+                             "traffic = HBM bytes per step from rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE (profiles/) "
+                             "times this walk's steps per launch" % (lead, cap0)},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
-        line["cpu_baseline"] = cpu_baseline(beg.a, csr.a, cfg, args.cpu_seconds, threads)
+        line["cpu_baseline"] = cpu_baseline(beg, csr, cfg, args.cpu_seconds, threads)
     print(json.dumps(line), flush=True)
 
 
@@ -278,7 +379,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="node2vec_rmat26_ooc", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline: seconds of the main all-core sample (the three side samples take a "
+                         "quarter each)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
@@ -293,10 +396,16 @@ def main():
     ap.add_argument("--window", action="store_true",
                     help="membership probes read aligned 64-byte windows (resident: with a fixed shape)")
     ap.add_argument("--traffic-json", default=os.path.join(REPO, "profiles", "traffic.json"))
-    ap.add_argument("--memory-gb", type=float, default=0.0, help="HBM cap of the out-of-core configuration")
-    ap.add_argument("--ablation", action="store_true", help="out-of-core: also run scheduler variants")
+    ap.add_argument("--caps", type=lambda s: [float(x) for x in s.split(",")], default=None,
+                    help="out-of-core: comma list of HBM caps in GB (the first is the headline cap)")
     ap.add_argument("--schedulers", type=lambda s: s.split(","), default=None,
-                    help="out-of-core: comma list of schedulers to run (default plain,plain_zero,learned)")
+                    help="out-of-core: comma list of schedulers (%s), run at every cap; default: %s timed "
+                         "at the headline cap, SOWalker once, %s once at other caps"
+                         % (",".join(SCHEDULERS), ",".join(HEADLINE), ",".join(EXTRA)))
+    ap.add_argument("--extra-warmup", type=int, default=2,
+                    help="out-of-core: untimed learned walks before the timed one at caps after the first")
+    ap.add_argument("--time-budget", type=float, default=400.0,
+                    help="out-of-core: seconds after which no further headline walk starts (the line still prints)")
     ap.add_argument("--autotune", type=int, default=4_000_000,
                     help="resident workloads: time every launch shape and search order on this many walkers "
                          "(untimed, state restored) and run with the fastest; 0 = fixed shape")
@@ -308,17 +417,18 @@ def main():
         sys.exit(2)
 
     from gowalker_amd import engine as E
-    cfg = CONFIGS[args.config]
+    t_start = time.perf_counter()
+    cfg = dict(CONFIGS[args.config])
     ctx = E.Context(0)
     t0 = time.perf_counter()
-    beg, csr = build_graph(ctx, cfg["graph"], args.seed)
+    beg, csr, cfg["edges_drawn"] = build_graph(ctx, cfg["graph"], args.seed)
     gen_s = time.perf_counter() - t0
     nverts = beg.n - 1
     narcs = csr.n
-    print("[bench] graph %s: %d vertices, %d arcs, generated in %.2f s" % (args.config, nverts, narcs, gen_s),
-          file=sys.stderr, flush=True)
+    print("[bench] graph %s: %d vertices, %d arcs (%d edges drawn), generated in %.2f s"
+          % (args.config, nverts, narcs, cfg["edges_drawn"], gen_s), file=sys.stderr, flush=True)
     if cfg.get("out_of_core"):
-        run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s)
+        run_ooc(args, cfg, ctx, beg.a, csr.a, nverts, narcs, gen_s, t_start)
         beg.close()
         csr.close()
         return
@@ -371,13 +481,15 @@ def main():
     avg_kms = sum(kms) / len(kms)
     abytes = algorithmic_bytes(st0, nwalkers, visits)
     achieved = abytes / (avg_kms * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_note = None, None
     if os.path.exists(args.traffic_json):
         try:
-            tj = json.load(open(args.traffic_json))
-            traffic = tj.get(args.config)
+            tj = json.load(open(args.traffic_json)).get(args.config)
         except Exception:
-            traffic = None
+            tj = None
+        if tj and tj.get("bytes_per_step"):
+            traffic = tj["bytes_per_step"] * st0["steps"]       # one launch per pass
+            traffic_note = tj.get("source")
     line = {
         "metric": "walk steps/s on 1 MI355X",
         "value": steps / dt,
@@ -398,6 +510,10 @@ def main():
                    "parallelism": "1 GPU", "launch": tuned or launch},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_over_algorithmic": traffic / abytes if traffic else None,
+                     "per_step": {"algorithmic_bytes": abytes / st0["steps"],
+                                  "traffic_bytes": traffic / st0["steps"] if traffic else None},
+                     "traffic_source": traffic_note,
                      "kernel": "walk_flat_kernel<true>" if args.variant == 1 else
                                "%s<true, %s, false, %s>" % ("walk_lane_kernel" if args.variant == 2 else "walk_kernel",
                                                             search_mode(tuned or launch),

@@ -151,7 +151,8 @@ HOST_LIB_PATH = os.path.join(_HERE, "_build", "libgwhost.so")
 class GwhEngineOpts(C.Structure):
     _fields_ = [("memory_bytes", C.c_uint64), ("block_bytes", C.c_uint64), ("zero_threshold", C.c_uint64),
                 ("scheduler", C.c_uint32), ("verbose", C.c_uint32), ("launch", GwLaunch), ("device", C.c_int32),
-                ("flags", C.c_uint32), ("prefetch_depth", C.c_uint32), ("reserved", C.c_uint32)]
+                ("flags", C.c_uint32), ("prefetch_depth", C.c_uint32), ("reserved", C.c_uint32),
+                ("model", C.c_void_p)]
 
 
 class GwhReport(C.Structure):
@@ -162,13 +163,25 @@ class GwhReport(C.Structure):
                 ("nslots", C.c_uint32), ("nblocks", C.c_uint32), ("learned_decisions", C.c_uint64),
                 ("fallbacks", C.c_uint64), ("final_loss", C.c_float), ("pad", C.c_uint32),
                 ("staged", C.c_uint64), ("zc_touches", C.c_uint64), ("tail_decisions", C.c_uint64),
-                ("sched_ms", C.c_double), ("tune_ms", C.c_double), ("tuned", GwLaunch)]
+                ("sched_ms", C.c_double), ("tune_ms", C.c_double), ("tuned", GwLaunch),
+                ("tail_model", C.c_uint64), ("tail_threshold", C.c_uint64), ("swap_model", C.c_uint64),
+                ("swap_differs", C.c_uint64), ("fill_model", C.c_uint64), ("model_samples", C.c_uint64),
+                ("sticky_swaps", C.c_uint64)]
+
+
+class GwhModelInfo(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("walks", C.c_uint64), ("adam_steps", C.c_uint64), ("loss", C.c_float),
+                ("has_params", C.c_uint32), ("scored", C.c_uint64), ("net_abs_err", C.c_double),
+                ("mean_abs_err", C.c_double)]
 
 
 HOST_SIGNATURES = {
     "gwh_run": [P, P, C.c_uint64, C.POINTER(GwhEngineOpts), C.POINTER(GwWalkParams), C.c_uint64, C.c_uint32, P,
                 C.POINTER(GwhReport), P, P],
     "gwh_last_error": [],
+    "gwh_model_create": [C.POINTER(P)],
+    "gwh_model_destroy": [P],
+    "gwh_model_info_get": [P, C.POINTER(GwhModelInfo)],
 }
 
 _hlib = None

@@ -434,13 +434,27 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     return GW_OK;
 }
 
+// The two buffers of one slot, sized for the largest block of the table: offsets
+// (nverts + 1 u64) and edges.  A block's edge words start 4 x (start_edge mod 16) bytes
+// into the slot, so every word keeps its offset modulo 64 bytes from the whole edge array
+// (the window search reads aligned 64-byte pieces); 64 B of head room and 64 B of tail
+// pad.  gw_slot_bytes and gw_reserve_slots both take their sizes from here, so the bytes a
+// caller budgets per slot are the bytes a reservation charges.
+static void slot_sizes(const gw_ctx* c, uint64_t* beg_bytes, uint64_t* csr_bytes)
+{
+    uint64_t mb = 0, me = 0;
+    for (auto& d : c->desc) { mb = std::max<uint64_t>(mb, (uint64_t)d.nverts + 1); me = std::max<uint64_t>(me, d.nedges); }
+    *beg_bytes = (mb * 8 + 255) & ~255ULL;
+    *csr_bytes = (me * 4 + 128 + 255) & ~255ULL;
+}
+
 gw_status gw_slot_bytes(gw_ctx* c, uint64_t* bytes)
 {
     if (!c || !bytes) return fail(c, GW_ERR_ARG, "null argument");
     if (!c->nblocks) return fail(c, GW_ERR_STATE, "no block table");
-    uint64_t mb = 0, me = 0;
-    for (auto& d : c->desc) { mb = std::max<uint64_t>(mb, (uint64_t)d.nverts + 1); me = std::max<uint64_t>(me, d.nedges); }
-    *bytes = ((mb * 8 + 255) & ~255ULL) + ((me * 4 + 16 + 255) & ~255ULL);
+    uint64_t b, e;
+    slot_sizes(c, &b, &e);
+    *bytes = b + e;
     return GW_OK;
 }
 
@@ -453,13 +467,7 @@ gw_status gw_reserve_slots(gw_ctx* c, uint32_t nslots)
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
     free_slots(c);
     for (auto& d : c->hblk) { d.resident = 0; d.beg = nullptr; d.csr = nullptr; }
-    uint64_t mb = 0, me = 0;
-    for (auto& d : c->desc) { mb = std::max<uint64_t>(mb, (uint64_t)d.nverts + 1); me = std::max<uint64_t>(me, d.nedges); }
-    c->slot_beg_bytes = (mb * 8 + 255) & ~255ULL;
-    // a block's edge words start 4 x (start_edge mod 16) bytes into the slot, so every
-    // word keeps its offset modulo 64 bytes from the whole edge array (the window search
-    // reads aligned 64-byte pieces); 64 B of head room and 64 B of tail pad
-    c->slot_csr_bytes = (me * 4 + 128 + 255) & ~255ULL;
+    slot_sizes(c, &c->slot_beg_bytes, &c->slot_csr_bytes);
     uint64_t need = (uint64_t)nslots * (c->slot_beg_bytes + c->slot_csr_bytes);
     if (c->cap && c->held + need > c->cap)
         return fail(c, GW_ERR_BUDGET, "%u slots of %llu B plus %llu B held exceed the %llu B cache budget", nslots,

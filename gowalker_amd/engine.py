@@ -300,14 +300,47 @@ def gen_chung_lu(ctx, nverts, nedges, gamma=2.2, max_degree=3000000, seed=42, de
         lib().gw_graph_free(ctx.h, g)
 
 
-SCHED_PLAIN, SCHED_LEARNED, SCHED_SOWALKER = 0, 1, 2
+SCHED_PLAIN, SCHED_LEARNED, SCHED_SOWALKER, SCHED_STAGING = 0, 1, 2, 3
+# gwh_engine_opts.flags
+NO_SWAP_PREFETCH, MYOPIC_LABEL, MODEL_SWAP, VALUE_VICTIM, NO_STICKY = 1, 2, 4, 8, 16
+
+
+class Model:
+    """The learned scheduler's state across walks (gwh_model): hand the same Model to
+    successive run_out_of_core calls and each walk starts from the network the earlier ones
+    trained."""
+
+    def __init__(self):
+        from ._abi import host_lib
+        h = C.c_void_p()
+        check(host_lib().gwh_model_create(C.byref(h)))
+        self.h = h
+
+    def info(self):
+        from ._abi import GwhModelInfo, host_lib
+        i = GwhModelInfo()
+        check(host_lib().gwh_model_info_get(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in GwhModelInfo._fields_}
+
+    def close(self):
+        if self.h:
+            from ._abi import host_lib
+            host_lib().gwh_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory_bytes=0, block_bytes=256 << 20,
                     zero_threshold=0, scheduler=SCHED_PLAIN, hashes=False, visits=False, verbose=False, launch=None,
-                    flags=0, prefetch_depth=0):
+                    flags=0, prefetch_depth=0, model=None):
     """Runs the C++ out-of-core engine (libgwhost.so) over a host CSR graph; beg/csr should
-    be pinned (PinnedArray.a) for the zero-copy tail.  Returns (report dict, hashes, visits)."""
+    be pinned (PinnedArray.a) for the zero-copy tail.  model: a Model carried across walks
+    (learned scheduler).  Returns (report dict, hashes, visits)."""
     from ._abi import GwhEngineOpts, GwhReport, host_lib
     L = host_lib()
     o = GwhEngineOpts()
@@ -318,6 +351,7 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
     o.verbose = 1 if verbose else 0
     o.flags = flags
     o.prefetch_depth = prefetch_depth
+    o.model = model.h if model is not None else None
     launch = launch or {}
     o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 2),
                         (GW_RUN_INTERP if launch.get("interp") else 0) | (GW_RUN_WINDOW if launch.get("window") else 0))

@@ -143,7 +143,8 @@ public:
     uint64_t total_verts() const { return blocks_.empty() ? 0 : blocks_.back().start_vert + blocks_.back().nverts; }
     uint32_t nblocks() const { return (uint32_t)blocks_.size(); }
 
-    // Number of HBM slots that fit next to the walker state under the budget.
+    // Number of HBM slots that fit next to the walker state under the budget.  The slots
+    // this engine already holds count as free: a new reservation releases them first.
     uint32_t slots_for_budget()
     {
         const uint32_t nb = nblocks();
@@ -152,7 +153,9 @@ public:
         ck(gw_counters_get(ctx_, &c), ctx_, "gw_counters_get");
         uint64_t per = 0;
         ck(gw_slot_bytes(ctx_, &per), ctx_, "gw_slot_bytes");
-        uint64_t free_b = cfg_.memory_bytes > c.hbm_bytes_held ? cfg_.memory_bytes - c.hbm_bytes_held : 0;
+        const uint64_t mine = (uint64_t)nslots_ * per;
+        const uint64_t held = c.hbm_bytes_held > mine ? c.hbm_bytes_held - mine : 0;
+        uint64_t free_b = cfg_.memory_bytes > held ? cfg_.memory_bytes - held : 0;
         uint64_t n = free_b / per;
         return (uint32_t)(n < nb ? n : nb);
     }
@@ -292,6 +295,9 @@ public:
             out.trials = st.trials;
             out.probes = st.probes;
             out.kernel_ms = st.kernel_ms;
+            out.walkers_run = st.walkers_run;
+            out.zc_touches = st.zc_touches;
+            out.zero_copy = st.zc_touches > 0;
             auto tf = clk::now();
             sched.feedback(in, out);
             r.sched_ms += std::chrono::duration<double, std::milli>(clk::now() - tf).count();
@@ -309,6 +315,7 @@ public:
                              sched.buckets.size(), out.loaded.size(), pf.size(), (unsigned long long)st.steps,
                              (unsigned long long)st.walkers_run, (unsigned long long)st.walkers_left, st.kernel_ms);
         }
+        sched.finish();
         ck(gw_wait_uploads(ctx_), ctx_, "gw_wait_uploads");
         gw_counters c1;
         ck(gw_counters_get(ctx_, &c1), ctx_, "gw_counters_get");

@@ -12,7 +12,45 @@
 
 static std::string g_err;
 
+struct gwh_model {
+    gwh::ModelState state;
+};
+
 extern "C" const char* gwh_last_error(void) { return g_err.c_str(); }
+
+extern "C" gw_status gwh_model_create(gwh_model** out)
+{
+    if (!out) {
+        g_err = "gwh_model_create: null argument";
+        return GW_ERR_ARG;
+    }
+    *out = new gwh_model();
+    return GW_OK;
+}
+
+extern "C" gw_status gwh_model_destroy(gwh_model* m)
+{
+    delete m;
+    return GW_OK;
+}
+
+extern "C" gw_status gwh_model_info_get(const gwh_model* m, gwh_model_info* out)
+{
+    if (!m || !out) {
+        g_err = "gwh_model_info_get: null argument";
+        return GW_ERR_ARG;
+    }
+    *out = gwh_model_info{};
+    out->samples = m->state.samples;
+    out->walks = m->state.walks;
+    out->adam_steps = m->state.adam_steps;
+    out->loss = m->state.loss;
+    out->has_params = m->state.params.empty() ? 0u : 1u;
+    out->scored = m->state.scored;
+    out->net_abs_err = m->state.scored ? m->state.net_abs_err / (double)m->state.scored : 0.0;
+    out->mean_abs_err = m->state.scored ? m->state.mean_abs_err / (double)m->state.scored : 0.0;
+    return GW_OK;
+}
 
 extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t entries, const gwh_engine_opts* opts,
                              const gw_walk_params* params, uint64_t nwalkers, uint32_t wpv, const uint32_t* sources,
@@ -46,12 +84,16 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         wc.outputs = (hashes ? (uint32_t)GW_OUT_HASH : 0u) | (visits ? (uint32_t)GW_OUT_VISITS : 0u);
         std::unique_ptr<Scheduler> sched;
         LearnedScheduler* ls = nullptr;
-        if (opts->scheduler == GWH_SCHED_LEARNED) {
+        if (opts->scheduler == GWH_SCHED_LEARNED || opts->scheduler == GWH_SCHED_STAGING) {
             LearnedConfig lc;
+            lc.use_model = opts->scheduler == GWH_SCHED_LEARNED;
             lc.swap_prefetch = (opts->flags & GWH_NO_SWAP_PREFETCH) == 0;
             lc.myopic_label = (opts->flags & GWH_MYOPIC_LABEL) != 0;
+            lc.model_swap = (opts->flags & GWH_MODEL_SWAP) != 0;
+            lc.value_victim = (opts->flags & GWH_VALUE_VICTIM) != 0;
+            lc.sticky = (opts->flags & GWH_NO_STICKY) == 0;
             if (opts->prefetch_depth) lc.max_inflight = opts->prefetch_depth;
-            ls = new LearnedScheduler(engine.ctx(), lc);
+            ls = new LearnedScheduler(engine.ctx(), lc, opts->model ? &opts->model->state : nullptr);
             sched.reset(ls);
         } else if (opts->scheduler == GWH_SCHED_SOWALKER) {
             sched.reset(new SOWalkerScheduler());
@@ -59,6 +101,7 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
             sched.reset(new PlainScheduler());
         }
         RunReport r = engine.run(wc, *sched);
+        if (ls) ls->save();
         *report = gwh_report{};
         report->steps = r.steps;
         report->trials = r.trials;
@@ -85,6 +128,13 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
             report->fallbacks = ls->fallbacks();
             report->final_loss = ls->last_loss();
             report->tail_decisions = ls->tail_decisions();
+            report->tail_model = ls->tail_model();
+            report->tail_threshold = ls->tail_threshold();
+            report->swap_model = ls->swap_model();
+            report->swap_differs = ls->swap_differs();
+            report->fill_model = ls->fill_model();
+            report->model_samples = ls->samples();
+            report->sticky_swaps = ls->sticky_swaps();
         }
         if (hashes) ck(gw_read_hashes(engine.ctx(), hashes, nwalkers), engine.ctx(), "gw_read_hashes");
         if (visits) ck(gw_read_visits(engine.ctx(), visits, entries - 1), engine.ctx(), "gw_read_visits");

@@ -1,28 +1,45 @@
 // learned.hpp -- learned block scheduler: a 12-64-64-1 MLP (BF16 weights, FP32
-// accumulation, trained on the GPU by libgw.so's kernels) predicts the walk steps that
-// loading a block buys over its stay in the cache (log1p scale), from the block's 12
-// features at decision time (gw_block_features).  It is trained online on what the
-// engine records: a load opens a sample (the features, the steps of the run that
-// followed it) and every later run adds the steps the block served as the walkers'
-// current block (the block-access trace, gw_trace_read) until its eviction closes it.
+// accumulation, trained on the GPU by libgw.so's kernels) predicts V(b), the walk steps
+// block b serves from a decision until its eviction (log1p scale), from the block's 12
+// features at that decision (gw_block_features).  It is trained online on what the
+// engine records (the block-access trace, gw_trace_read):
+//   * a load opens a sample: the features when the load was decided, the steps of the run
+//     that followed, then every later run's steps with the block as the walkers' current
+//     block, until its eviction closes it;
+//   * every keep_every-th iteration each resident block without an open keep sample opens
+//     one the same way (features with residency 1), so V is also learned for blocks that
+//     are already resident -- the value an eviction gives up.
+// The samples and the parameters outlive a walk (ModelState): walks of the same workload
+// train one network, as the plain scheduler's runs would record traces for it.
 //
-// Decisions, given the predictions y(b):
+// Decisions, given V:
 //   fill   (free slots left): commit staged blocks whose copies are done, else the best
-//          staged one (the run waits for it), else upload the best candidate; then stage
-//          the next-best blocks into the free slots so their copies overlap the run;
-//   swap   (cache full): the plain rule's load and victim;
-//   tail   when y(c) of the block the plain rule would load says the upload buys fewer
-//          steps than its bytes buy in zero-copy reads (steps x reads per step x 64 B <
-//          block bytes, reads per step measured on the walk so far) and all the work left
-//          is worth at most tail_guard uploads in zero-copy bytes, finish the walk by
-//          zero-copy instead; the reference's fixed walker-count threshold
-//          (apps/Node2vec.cu:100-104, the plain rule's version of this decision) is the
-//          latest point it switches at.
-// The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: the model is
+//          staged one (the run waits for it), else upload the best-ranked candidate; then
+//          stage the next-best blocks into the free slots so their copies overlap the run;
+//          rank = V(b);
+//   swap   (cache full): the plain rule's scores (walkers a load makes runnable, minus the
+//          pairing its victim takes away) form a shortlist -- candidates scoring at least
+//          shortlist_frac of the best, at most `shortlist` of them; the network picks the
+//          load c with the largest V(c) - V(m(c)), m(c) the plain rule's victim for c (its
+//          least-paired resident block; among victims paired as little, the one with the
+//          smallest V); with value_victim, m(c) minimises the steps its eviction gives up,
+//          s x pairing(c, m) + V(m), s the measured steps per walker and run;
+//   tail   when V(c) of the block that would be loaded says the upload takes longer than
+//          zero-copy reads would take for the steps it buys (V x reads per step / the
+//          zero-copy read rate > block bytes / the copy rate; reads per step from the walk
+//          so far, both rates measured: the zero-copy rate on earlier walks' tails) and
+//          all the work left would take at most tail_guard uploads' time in zero-copy
+//          reads, finish the walk by zero-copy instead; the reference's fixed walker-count
+//          threshold (apps/Node2vec.cu:100-104, the plain rule's version of this decision)
+//          is the latest point it switches at.
+// The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: the network is
 // ignored while it has too few samples and, for a cool-down period, whenever its recent
-// predictions of the steps a load buys are off by more than a factor e on average (the
-// error monitor).  Walks stay bit-identical either way (counter-based
-// RNG).
+// predictions are off by more than a factor e on average (the error monitor).
+//
+// With use_model false the same code runs with the network left out (the staging
+// ablation): fill rank = log1p(walkers a load would release), swap = the plain rule, tail
+// = the reference's threshold only.  Walks stay bit-identical either way (counter-based
+// RNG): only bytes moved and time change.
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -34,18 +51,38 @@
 
 namespace gwh {
 
+// What carries over from one walk to the next: the network and its samples.
+struct ModelState {
+    std::vector<float> params;          // fp32 image of the bf16 weights (empty: untrained)
+    std::vector<float> xs, ys;          // sample ring: features (kF each) and log1p labels
+    uint64_t samples = 0, ring = 0, trained_at = 0, adam_steps = 0, walks = 0;
+    uint64_t rng = 0x853c49e6748fea9bULL;
+    float loss = 0.f;
+    double zc_reads_per_ms = 0;         // zero-copy reads per ms measured on tail runs (0: none yet)
+    // prediction quality on closed samples the network had predicted: its mean |log
+    // error| against that of the running mean of all labels seen before (a predictor
+    // with no features)
+    uint64_t scored = 0;
+    double net_abs_err = 0, mean_abs_err = 0, label_sum = 0;
+    uint64_t labels = 0;
+};
+
 struct LearnedConfig {
+    bool use_model = true;          // false: the staging ablation (no network)
     uint32_t h1 = 64, h2 = 64;
     uint64_t seed = 7;
-    uint32_t min_samples = 24;      // plain swap decisions until this many upload samples
+    uint32_t min_samples = 24;      // plain decisions until this many samples
     uint32_t train_steps = 4;       // Adam steps per training round
     uint32_t train_every = 8;       // new samples between training rounds
     uint32_t batch = 256;
     uint32_t max_samples = 16384;   // ring buffer of trace samples
+    uint32_t keep_every = 4;        // iterations between keep samples of resident blocks
     float lr = 2e-3f;
-    double zc_read_bytes = 64;      // PCIe bytes of one zero-copy read (row, candidate or probe)
-    double tail_margin = 1.0;       // zero-copy once steps x zc bytes x margin < block bytes
-    double tail_guard = 64;         // ... and the work left x zc bytes < guard x block bytes
+    double zc_prior_reads_per_ms = 375e3;   // zero-copy read rate before one is measured
+                                            // (64-byte reads at ~24 GB/s)
+    double copy_prior_bytes_per_ms = 55e6;  // block copy rate before one is measured
+    double tail_margin = 1.0;       // zero-copy once its time for V(c) steps x margin < upload time
+    double tail_guard = 8;          // ... and the whole tail's zero-copy time < guard uploads' time
     uint32_t max_inflight = 8;      // staged copies in flight at once: more blocks land per
                                     // run than with 4 (C3 16 GB: 0.471 s vs 0.478 s); deeper
                                     // queues make a run wait for a wanted block queued behind
@@ -54,6 +91,15 @@ struct LearnedConfig {
                                     // (overlaps every upload, costs a slot and missed copies;
                                     // C5 29.4 -> 22.7 s, C3 8 GB 4.31 -> 3.82 s, 11-13% more
                                     // PCIe bytes)
+    bool model_swap = false;        // the network chooses swap loads among near-best candidates
+                                    // (else the plain rule; the plain rule moved 1-5% fewer
+                                    // bytes in every measurement)
+    bool value_victim = false;      // ... and victims by the steps an eviction gives up
+    bool sticky = true;             // a swap takes the block already staged when its plain
+                                    // score is within the shortlist (its copy has overlapped
+                                    // the last run; any other load waits for a fresh copy)
+    uint32_t shortlist = 4;         // swap candidates the network chooses among
+    double shortlist_frac = 0.8;    // ... each scoring at least this share of the best
     bool myopic_label = false;      // label a load with the next run's steps only
     double error_limit = 1.0;       // mean |log error| of recent predictions above this => fall back
     uint32_t cooldown = 8;
@@ -61,28 +107,59 @@ struct LearnedConfig {
 
 class LearnedScheduler : public Scheduler {
 public:
-    LearnedScheduler(gw_ctx* ctx, const LearnedConfig& cfg) : ctx_(ctx), cfg_(cfg)
+    LearnedScheduler(gw_ctx* ctx, const LearnedConfig& cfg, ModelState* state = nullptr)
+        : ctx_(ctx), cfg_(cfg), state_(state)
     {
+        if (!cfg_.use_model) return;
         if (gw_mlp_create(ctx_, kF, cfg_.h1, cfg_.h2, cfg_.seed, &mlp_) != GW_OK) mlp_ = nullptr;
+        if (state_) {
+            m_ = *state_;
+            uint32_t np = 0;
+            if (mlp_ && !m_.params.empty() && gw_mlp_num_params(mlp_, &np) == GW_OK && np == m_.params.size())
+                (void)gw_mlp_set_params(ctx_, mlp_, m_.params.data());
+        }
     }
     ~LearnedScheduler() override
     {
         if (mlp_) gw_mlp_destroy(ctx_, mlp_);
     }
-    const char* name() const override { return "learned"; }
-    uint32_t run_flags() const override { return GW_RUN_TRACE; }
+    const char* name() const override { return cfg_.use_model ? "learned" : "staging"; }
+    uint32_t run_flags() const override { return cfg_.use_model ? (uint32_t)GW_RUN_TRACE : 0u; }
+
+    // Stores the network and its samples for the next walk (call once the walk is done).
+    void save()
+    {
+        if (!state_ || !mlp_) return;
+        uint32_t np = 0;
+        if (gw_mlp_num_params(mlp_, &np) == GW_OK) {
+            m_.params.resize(np);
+            if (gw_mlp_get_params(ctx_, mlp_, m_.params.data()) != GW_OK) m_.params.clear();
+        }
+        m_.walks++;
+        *state_ = m_;
+    }
+
+    // The walk is over: samples still open are closed with the steps their blocks served
+    // until the end (at a cap where nothing is evicted, every sample ends this way).
+    void finish() override
+    {
+        for (Open& o : open_)
+            if (o.live) close_sample(o);
+        for (Open& o : keep_)
+            if (o.live) close_sample(o);
+    }
 
     bool schedule(const SchedInput& in) override
     {
         const uint32_t nb = in.nblocks;
         if (age_.size() != nb) age_.assign(nb, 64.f);
         prepare(in);
-        last_kind_ = kPlain;
         std::vector<char> cached(nb, 0);
         for (uint32_t b : buckets) cached[b] = 1;
         const uint32_t cap = capacity(in);
 
         if (buckets.size() < cap) {   // ---- fill: free slots remain
+            if (use_model_) ++fill_model_;
             bool took = false;
             for (uint32_t b = 0; b < nb && buckets.size() < cap; ++b)
                 if (!cached[b] && in.staged && in.staged[b] == 2) { buckets.push_back(b); cached[b] = 1; took = true; }
@@ -92,21 +169,39 @@ public:
                     if (!cached[b] && in.staged[b] == 1 && (best < 0 || rank_[b] > rank_[best])) best = (int)b;
                 if (best >= 0) { buckets.push_back((uint32_t)best); cached[best] = 1; took = true; }
             }
-            if (took) { last_kind_ = kFill; return true; }
+            if (took) return true;
             const int c = best_candidate(cached);
-            if (c < 0) return plain_decide(in);
+            if (c < 0) {
+                if (plain_decide(in)) return true;
+                ++tail_threshold_;
+                return false;
+            }
             if (tail_now(in, (uint32_t)c)) return false;
             buckets.push_back((uint32_t)c);
-            last_kind_ = kFill;
             return true;
         }
 
-        // ---- swap: the cache is full.  The plain rule picks the load and the victim (its
-        // pairing-aware eviction moved fewer bytes than evicting by predicted value in our
-        // measurements); the model decides when uploads stop paying for themselves.
+        // ---- swap: the cache is full
         if (cooldown_ > 0) --cooldown_;
         const std::vector<uint32_t> before = buckets;
-        if (!plain_decide(in)) return false;
+        if (!plain_decide(in)) {                // the plain rule finds nothing worth loading
+            ++tail_threshold_;
+            return false;
+        }
+        const std::vector<uint32_t> plain_choice = buckets;
+        {
+            uint32_t c, pos;
+            int kind = 0;
+            if (swap_choice(in, before, in.staged, &c, &pos, &kind)) {
+                buckets = before;
+                buckets[pos] = c;
+                if (kind == kSticky) ++sticky_;
+                if (kind == kModel) {
+                    ++swap_model_;
+                    if (buckets != plain_choice) ++swap_differs_;
+                }
+            }
+        }
         int c = -1;
         for (uint32_t b : buckets)
             if (std::find(before.begin(), before.end(), b) == before.end()) c = (int)b;
@@ -141,39 +236,43 @@ public:
             }
             return;
         }
-        // swap: stage into the spare slot the block the plain rule would load next on the
+        // swap: stage into the spare slot the block this scheduler would load next on the
         // current bucket matrix, so that when it does, the copy has overlapped this run
         if (!cfg_.swap_prefetch) return;
         PlainScheduler next;
         next.buckets = buckets;
         SchedInput pin = in;
         pin.ncblock = cap;
-        uint64_t mx = 0;
-        for (uint32_t b = 0; b < nb; ++b) {
-            const gw_block_desc& d = in.blocks[b];
-            mx = std::max<uint64_t>(mx, ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4);
-        }
-        pin.zero_threshold = std::max<uint64_t>(in.zero_threshold, (uint64_t)zero_floor((double)mx));
+        pin.zero_threshold = std::max<uint64_t>(in.zero_threshold, (uint64_t)zero_floor((double)max_block_bytes(in)));
         if (!next.schedule(pin)) return;
+        {
+            uint32_t c, pos;
+            int kind = 0;
+            if (swap_choice(in, buckets, nullptr, &c, &pos, &kind) && !skip[c]) { out.push_back(c); return; }
+        }
         for (uint32_t b : next.buckets)
             if (!skip[b]) { out.push_back(b); return; }
     }
 
     void feedback(const SchedInput& in, const IterationOutcome& out) override
     {
+        if (!cfg_.use_model) return;
         const uint32_t nb = in.nblocks;
         std::vector<uint64_t> trace(nb, 0);
         if (gw_trace_read(ctx_, trace.data(), nb) != GW_OK) return;
-        if (open_.size() != nb) open_.assign(nb, Open{});
+        if (open_.size() != nb) { open_.assign(nb, Open{}); keep_.assign(nb, Open{}); }
         std::vector<char> now(nb, 0);
         for (uint32_t b : buckets) now[b] = 1;
-        // a block that left the resident set closes its sample: the steps it served from
-        // its upload to its eviction
+        // a block that left the resident set closes its samples: the steps it served from
+        // the decision to its eviction
         for (uint32_t b = 0; b < nb; ++b)
-            if (open_[b].live && !now[b]) close_sample(b);
-        // blocks that became resident open one: (features when the load was decided, the
-        // run's steps if it was the only new block, else the steps it served as the
-        // walkers' current block)
+            if (!now[b]) {
+                if (open_[b].live) close_sample(open_[b]);
+                if (keep_[b].live) close_sample(keep_[b]);
+            }
+        // blocks that became resident open a load sample: (features when the load was
+        // decided, the run's steps if it was the only new block, else the steps it served
+        // as the walkers' current block)
         std::vector<uint32_t> fresh;
         for (uint32_t b : buckets)
             if (!was_resident_[b]) fresh.push_back(b);
@@ -185,35 +284,66 @@ public:
             o.steps = fresh.size() == 1 ? (double)out.steps : (double)trace[b];
             o.pred = use_model_ ? pred_[b] : -1.f;
         }
+        // resident blocks that stayed open a keep sample now and then (features with
+        // residency 1; label = the steps they serve from here to their eviction)
+        if (have_feats_ && cfg_.keep_every && in.iteration % cfg_.keep_every == 0)
+            for (uint32_t b : buckets)
+                if (was_resident_[b] && !keep_[b].live && feats_.size() >= (size_t)(b + 1) * kF) {
+                    Open& o = keep_[b];
+                    o.live = true;
+                    o.f.assign(feats_.begin() + (size_t)b * kF, feats_.begin() + (size_t)(b + 1) * kF);
+                    o.steps = (double)trace[b];
+                    o.pred = use_model_ ? pred_[b] : -1.f;
+                }
         // reads per step (row + candidates + probes) of the walks so far: a zero-copy step
-        // costs about that many PCIe reads
-        if (out.steps) {
+        // costs about that many PCIe reads; steps per walker and run
+        if (out.steps && !out.zero_copy) {
             const double r = 1.0 + (double)(out.trials + out.probes) / (double)out.steps;
             reads_per_step_ = reads_n_ ? 0.9 * reads_per_step_ + 0.1 * r : r;
             ++reads_n_;
+            if (out.walkers_run) {
+                const double sw = (double)out.steps / (double)out.walkers_run;
+                steps_per_walker_ = sw_n_ ? 0.9 * steps_per_walker_ + 0.1 * sw : sw;
+                ++sw_n_;
+            }
+        }
+        // the zero-copy read rate of a tail run (kept for later walks)
+        if (out.zero_copy && out.kernel_ms > 0 && out.zc_touches >= 1000000) {
+            const double rate = (double)out.zc_touches / out.kernel_ms;
+            m_.zc_reads_per_ms = m_.zc_reads_per_ms > 0 ? 0.5 * m_.zc_reads_per_ms + 0.5 * rate : rate;
         }
         for (uint32_t b = 0; b < nb; ++b) {
-            if (now[b] && open_[b].live && !cfg_.myopic_label &&
-                std::find(fresh.begin(), fresh.end(), b) == fresh.end())
-                open_[b].steps += (double)trace[b];
+            if (now[b] && !cfg_.myopic_label && std::find(fresh.begin(), fresh.end(), b) == fresh.end()) {
+                if (open_[b].live) open_[b].steps += (double)trace[b];
+                if (keep_[b].live) keep_[b].steps += (double)trace[b];
+            }
             age_[b] = trace[b] ? 0.f : std::min(age_[b] + 1.f, 1024.f);
         }
-        if (mlp_ && samples_ >= 8 && samples_ >= trained_at_ + cfg_.train_every) {
-            trained_at_ = samples_;
+        if (mlp_ && m_.samples >= 8 && m_.samples >= m_.trained_at + cfg_.train_every) {
+            m_.trained_at = m_.samples;
             train();
         }
     }
 
-    uint64_t learned_decisions() const { return learned_decisions_; }
+    // iterations in which the network's predictions ranked the fill, chose a swap or the
+    // tail
+    uint64_t learned_decisions() const { return fill_model_ + swap_model_ + tail_model_; }
     uint64_t fallbacks() const { return fallbacks_; }
-    uint64_t tail_decisions() const { return tail_decisions_; }
-    float last_loss() const { return last_loss_; }
+    uint64_t tail_decisions() const { return tail_model_ + tail_threshold_; }
+    uint64_t tail_model() const { return tail_model_; }
+    uint64_t tail_threshold() const { return tail_threshold_; }
+    uint64_t swap_model() const { return swap_model_; }
+    uint64_t swap_differs() const { return swap_differs_; }
+    uint64_t fill_model() const { return fill_model_; }
+    uint64_t sticky_swaps() const { return sticky_; }
+    uint64_t samples() const { return m_.samples; }
+    float last_loss() const { return m_.loss; }
 
 private:
     static constexpr uint32_t kF = 12;
-    enum Kind { kPlain, kFill, kLearnedSwap };
 
-    bool trained() const { return mlp_ && samples_ >= cfg_.min_samples; }
+    bool trained() const { return mlp_ && m_.samples >= cfg_.min_samples; }
+    bool model_decides() const { return use_model_ && cooldown_ == 0 && cfg_.model_swap; }
 
     // resident blocks allowed: every slot, or all but the staging slot when the graph does
     // not fit and the next load is prefetched
@@ -222,7 +352,23 @@ private:
         return cfg_.swap_prefetch && in.nblocks > in.ncblock && in.ncblock > 1 ? in.ncblock - 1 : in.ncblock;
     }
 
-    double zc_bytes() const { return cfg_.zc_read_bytes * (reads_n_ ? reads_per_step_ : 6.0); }
+    static uint64_t block_bytes(const gw_block_desc& d) { return ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4; }
+    static uint64_t max_block_bytes(const SchedInput& in)
+    {
+        uint64_t mx = 0;
+        for (uint32_t b = 0; b < in.nblocks; ++b) mx = std::max<uint64_t>(mx, block_bytes(in.blocks[b]));
+        return mx;
+    }
+
+    double reads_per_step() const { return reads_n_ ? reads_per_step_ : 6.0; }
+    double zc_reads_per_ms() const { return m_.zc_reads_per_ms > 0 ? m_.zc_reads_per_ms : cfg_.zc_prior_reads_per_ms; }
+    double copy_bytes_per_ms() const
+    {
+        gw_counters c;
+        if (gw_counters_get(ctx_, &c) == GW_OK && c.h2d_ms > 10 && c.h2d_bytes)
+            return (double)c.h2d_bytes / c.h2d_ms;
+        return cfg_.copy_prior_bytes_per_ms;
+    }
 
     struct Open {
         bool live = false;
@@ -231,13 +377,19 @@ private:
         float pred = -1.f;
     };
 
-    void close_sample(uint32_t b)
+    void close_sample(Open& o)
     {
-        Open& o = open_[b];
         o.live = false;
         const float y = std::log1p((float)o.steps);
+        if (o.pred >= 0.f && m_.labels) {
+            ++m_.scored;
+            m_.net_abs_err += std::fabs((double)o.pred - (double)y);
+            m_.mean_abs_err += std::fabs(m_.label_sum / (double)m_.labels - (double)y);
+        }
+        m_.label_sum += y;
+        ++m_.labels;
         add_sample(o.f.data(), y);
-        // error monitor: how far the model's prediction for this load was from what it bought
+        // error monitor: how far the network's prediction was from what the block served
         if (o.pred >= 0.f) {
             const double e = std::fabs((double)o.pred - (double)y);
             err_ = err_n_ ? 0.8 * err_ + 0.2 * e : e;
@@ -273,30 +425,22 @@ private:
             add_[b] = a;
             demand_[b] = d;
         }
-        // the model is consulted while free slots are being filled and once its tail rule
-        // can fire (all the work left is worth at most tail_guard uploads in zero-copy
-        // reads); elsewhere the plain rule decides and a forward pass would be wasted
-        uint64_t max_bytes = 0;
-        for (uint32_t b = 0; b < nb; ++b) {
-            const gw_block_desc& d = in.blocks[b];
-            max_bytes = std::max<uint64_t>(max_bytes, ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4);
-        }
-        const bool need = buckets.size() < capacity(in) ||
-                          (double)pending_hops_ * zc_bytes() < cfg_.tail_guard * (double)max_bytes;
-        // features (a kernel and a readback): whenever the model is consulted, and on every
-        // 4th iteration in between so that loads keep opening training samples
+        // features (a kernel and a readback) and a forward pass whenever the network may
+        // decide, and on every keep_every-th iteration for samples
         have_feats_ = false;
-        if (need || in.iteration % 4 == 0) {
-            feats_.assign((size_t)nb * kF, 0.f);
-            have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
-        }
+        use_model_ = false;
         pred_.assign(nb, 0.f);
-        use_model_ = need && trained() && have_feats_ &&
-                     gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
-        if (use_model_) ++learned_decisions_;
+        if (cfg_.use_model) {
+            const bool want = trained() || in.iteration % std::max(1u, cfg_.keep_every) == 0;
+            if (want || buckets.size() < capacity(in)) {
+                feats_.assign((size_t)nb * kF, 0.f);
+                have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
+            }
+            use_model_ = trained() && have_feats_ && gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
+        }
         rank_.resize(nb);
-        // before the model is trained: the walkers a load would release (log scale), with
-        // the walkers that need the block at all as a tie-break
+        // without the network: the walkers a load would release (log scale), with the
+        // walkers that need the block at all as a tie-break
         for (uint32_t b = 0; b < nb; ++b)
             rank_[b] = use_model_ ? pred_[b] : std::log1p((float)add_[b]) + 1e-3f * std::log1p((float)demand_[b]);
     }
@@ -312,27 +456,94 @@ private:
         return best;
     }
 
+    // The network's swap on resident set C: among the plain rule's shortlist (scores at
+    // least shortlist_frac of the best, at most `shortlist` candidates), the load c with
+    // the largest V(c) - V(m(c)); m(c) is c's least-paired resident block, ties broken by
+    // the smaller V.  Returns the load and the position in C it replaces.
+    enum { kPlain = 0, kSticky = 1, kModel = 2 };
+
+    bool swap_choice(const SchedInput& in, const std::vector<uint32_t>& C, const uint8_t* staged, uint32_t* load,
+                     uint32_t* pos, int* kind) const
+    {
+        const uint32_t nb = in.nblocks;
+        const uint64_t n = nb;
+        if (C.empty()) return false;
+        std::vector<char> cached(nb, 0);
+        for (uint32_t b : C) cached[b] = 1;
+        struct Cand { uint64_t score; uint32_t c, pos; };
+        std::vector<Cand> cs;
+        uint64_t best = 0;
+        for (uint32_t c = 0; c < nb; ++c) {
+            if (cached[c]) continue;
+            uint64_t mn = ~0ull, all = 0;
+            uint32_t mpos = 0;
+            double mcost = 0;
+            const double sw = sw_n_ ? steps_per_walker_ : 2.0;
+            for (uint32_t j = 0; j < C.size(); ++j) {
+                const uint64_t x = in.W[c * n + C[j]] + in.W[(uint64_t)C[j] * n + c];
+                all += x;
+                if (cfg_.value_victim) {
+                    // steps the eviction of C[j] gives up: its walkers paired with c, and
+                    // what it would serve if it stayed
+                    const double cost = sw * (double)x + std::expm1((double)pred_[C[j]]);
+                    if (j == 0 || cost < mcost) { mcost = cost; mpos = j; mn = x; }
+                } else if (x < mn || (x == mn && pred_[C[j]] < pred_[C[mpos]])) {
+                    mn = x;
+                    mpos = j;
+                }
+            }
+            const uint64_t s = all - mn;
+            if (s == 0) continue;
+            cs.push_back({s, c, mpos});
+            best = std::max(best, s);
+        }
+        if (cs.empty()) return false;
+        // best first; equal scores keep the lower block id first, as the plain rule does
+        std::stable_sort(cs.begin(), cs.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });
+        uint32_t len = 0;
+        while (len < cs.size() && len < cfg_.shortlist && (double)cs[len].score >= cfg_.shortlist_frac * (double)best)
+            ++len;
+        int pick = 0;
+        *kind = kPlain;
+        bool sticky = false;
+        if (cfg_.sticky && staged)
+            for (uint32_t i = 0; i < len && !sticky; ++i)
+                if (staged[cs[i].c]) { pick = (int)i; sticky = true; *kind = kSticky; }
+        if (!sticky && model_decides()) {
+            double pv = 0;
+            for (uint32_t i = 0; i < len; ++i) {
+                const double v = std::expm1((double)pred_[cs[i].c]) - std::expm1((double)pred_[C[cs[i].pos]]);
+                if (i == 0 || v > pv) { pick = (int)i; pv = v; }
+            }
+            *kind = kModel;
+        }
+        *load = cs[pick].c;
+        *pos = cs[pick].pos;
+        return true;
+    }
+
     // the reference's zero-copy threshold for a block of this size, in waiting walkers
     // (blocksize / 128 / 24, apps/Node2vec.cu:100-104): the learned tail never comes later
     static double zero_floor(double bytes) { return bytes / 3072.0; }
 
     bool tail_now(const SchedInput& in, uint32_t c)
     {
-        const gw_block_desc& d = in.blocks[c];
-        const double bytes = ((double)d.nverts + 1) * 8 + (double)d.nedges * 4;
+        const double bytes = (double)block_bytes(in.blocks[c]);
         // the reference's rule looks at the best walker count of all candidates
         uint64_t best_add = 0;
         for (uint32_t b = 0; b < in.nblocks; ++b)
             if (!was_resident_[b]) best_add = std::max(best_add, add_[b]);
         if ((double)best_add <= zero_floor(bytes)) {
-            ++tail_decisions_;
+            ++tail_threshold_;
             return true;
         }
         if (!use_model_ || cooldown_ > 0) return false;
+        // time of the upload against the time zero-copy reads need for the steps it buys
         const double steps = std::expm1((double)std::max(0.f, pred_[c]));
-        const double zc = zc_bytes();
-        if (steps * zc * cfg_.tail_margin < bytes && (double)pending_hops_ * zc < cfg_.tail_guard * bytes) {
-            ++tail_decisions_;
+        const double t_up = bytes / copy_bytes_per_ms();
+        const double per_step = reads_per_step() / zc_reads_per_ms();
+        if (steps * per_step * cfg_.tail_margin < t_up && (double)pending_hops_ * per_step < cfg_.tail_guard * t_up) {
+            ++tail_model_;
             return true;
         }
         return false;
@@ -343,69 +554,63 @@ private:
         plain_.buckets = buckets;
         SchedInput pin = in;
         pin.ncblock = capacity(in);
-        uint64_t mx = 0;
-        for (uint32_t b = 0; b < in.nblocks; ++b) {
-            const gw_block_desc& d = in.blocks[b];
-            mx = std::max<uint64_t>(mx, ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4);
-        }
-        pin.zero_threshold = std::max<uint64_t>(in.zero_threshold, (uint64_t)zero_floor((double)mx));
+        pin.zero_threshold = std::max<uint64_t>(in.zero_threshold, (uint64_t)zero_floor((double)max_block_bytes(in)));
         bool r = plain_.schedule(pin);
         buckets = plain_.buckets;
-        last_kind_ = kPlain;
         return r;
     }
 
     void add_sample(const float* f, float y)
     {
-        if (xs_.size() < (size_t)cfg_.max_samples * kF) {
-            xs_.insert(xs_.end(), f, f + kF);
-            ys_.push_back(y);
+        if (m_.xs.size() < (size_t)cfg_.max_samples * kF) {
+            m_.xs.insert(m_.xs.end(), f, f + kF);
+            m_.ys.push_back(y);
         } else {
-            const size_t i = ring_++ % cfg_.max_samples;
-            std::copy(f, f + kF, xs_.begin() + i * kF);
-            ys_[i] = y;
+            const size_t i = m_.ring++ % cfg_.max_samples;
+            std::copy(f, f + kF, m_.xs.begin() + i * kF);
+            m_.ys[i] = y;
         }
-        ++samples_;
+        ++m_.samples;
     }
 
     void train()
     {
-        const uint32_t n = (uint32_t)ys_.size();
+        const uint32_t n = (uint32_t)m_.ys.size();
         if (!n) return;
         const uint32_t bs = std::min(cfg_.batch, n);
         std::vector<float> bx((size_t)bs * kF), by(bs);
         for (uint32_t e = 0; e < cfg_.train_steps; ++e) {
             for (uint32_t i = 0; i < bs; ++i) {
-                rng_ = rng_ * 6364136223846793005ULL + 1442695040888963407ULL;
-                const uint32_t k = (uint32_t)((rng_ >> 33) % n);
-                std::copy(xs_.begin() + (size_t)k * kF, xs_.begin() + (size_t)(k + 1) * kF, bx.begin() + (size_t)i * kF);
-                by[i] = ys_[k];
+                m_.rng = m_.rng * 6364136223846793005ULL + 1442695040888963407ULL;
+                const uint32_t k = (uint32_t)((m_.rng >> 33) % n);
+                std::copy(m_.xs.begin() + (size_t)k * kF, m_.xs.begin() + (size_t)(k + 1) * kF, bx.begin() + (size_t)i * kF);
+                by[i] = m_.ys[k];
             }
             float loss = 0.f;
             if (gw_mlp_step(ctx_, mlp_, bx.data(), by.data(), bs, cfg_.lr, &loss) != GW_OK) return;
-            last_loss_ = loss;
+            m_.loss = loss;
+            ++m_.adam_steps;
         }
     }
 
     gw_ctx* ctx_;
     LearnedConfig cfg_;
+    ModelState* state_;
+    ModelState m_;
     gw_mlp* mlp_ = nullptr;
     PlainScheduler plain_;
-    std::vector<float> age_, feats_, pred_, rank_, xs_, ys_;
+    std::vector<float> age_, feats_, pred_, rank_;
     std::vector<uint64_t> add_, demand_;
     std::vector<char> was_resident_;
-    std::vector<Open> open_;
-    double reads_per_step_ = 0;
-    uint64_t reads_n_ = 0;
-    uint64_t trained_at_ = 0;
+    std::vector<Open> open_, keep_;
+    double reads_per_step_ = 0, steps_per_walker_ = 0;
+    uint64_t reads_n_ = 0, sw_n_ = 0;
     uint64_t pending_hops_ = 0;
     bool have_feats_ = false, use_model_ = false;
-    uint64_t samples_ = 0, ring_ = 0, rng_ = 0x853c49e6748fea9bULL;
     uint32_t cooldown_ = 0;
-    Kind last_kind_ = kPlain;
-    uint64_t learned_decisions_ = 0, fallbacks_ = 0, tail_decisions_ = 0, err_n_ = 0;
+    uint64_t fill_model_ = 0, swap_model_ = 0, sticky_ = 0, swap_differs_ = 0, tail_model_ = 0, tail_threshold_ = 0;
+    uint64_t fallbacks_ = 0, err_n_ = 0;
     double err_ = 0;
-    float last_loss_ = 0.f;
 };
 
 }  // namespace gwh

@@ -35,6 +35,9 @@ struct IterationOutcome {
     uint64_t h2d_bytes;         // block bytes uploaded for it
     double kernel_ms;
     std::vector<uint32_t> loaded;   // blocks uploaded for it
+    uint64_t walkers_run = 0;   // walkers whose (prev, cur) pair was resident
+    uint64_t zc_touches = 0;    // zero-copy reads (the tail run)
+    bool zero_copy = false;     // the run read non-resident blocks from host memory
 };
 
 class Scheduler {
@@ -52,6 +55,8 @@ public:
         (void)in; (void)spare; (void)out;
     }
     virtual void feedback(const SchedInput& in, const IterationOutcome& out) { (void)in; (void)out; }
+    // The walk is over (every walker finished).
+    virtual void finish() {}
     virtual const char* name() const = 0;
     virtual uint32_t run_flags() const { return 0; }   // extra GW_RUN_* flags the scheduler needs
     std::vector<uint32_t> buckets;

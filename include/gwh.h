@@ -13,15 +13,50 @@ extern "C" {
 #endif
 
 /* plain: GOwalker_scheduler_t (engine/schedule.hpp:58-365); learned: the MLP scheduler;
- * sowalker: SOwalker_scheduler_t's simulated annealing (engine/schedule.hpp:369-635) */
-enum { GWH_SCHED_PLAIN = 0, GWH_SCHED_LEARNED = 1, GWH_SCHED_SOWALKER = 2 };
+ * sowalker: SOwalker_scheduler_t's simulated annealing (engine/schedule.hpp:369-635);
+ * staging: the learned scheduler's staging, prefetch and zero-copy code with the network
+ * left out (an ablation with no reference counterpart): fill order by the walkers a load
+ * would release, swap loads and victims by the plain rule, the zero-copy tail at the
+ * reference's walker threshold (apps/Node2vec.cu:100-104) */
+enum { GWH_SCHED_PLAIN = 0, GWH_SCHED_LEARNED = 1, GWH_SCHED_SOWALKER = 2, GWH_SCHED_STAGING = 3 };
+
+/* What the learned scheduler keeps from one walk to the next (no reference counterpart):
+ * the network's parameters and its ring of block-access samples (features at a decision,
+ * steps the block served afterwards).  Hand the same model to successive gwh_run calls
+ * and each walk starts from the network the earlier walks trained; NULL in
+ * gwh_engine_opts.model starts every walk from an untrained network. */
+typedef struct gwh_model gwh_model;
+typedef struct {
+    uint64_t samples;          /* samples recorded so far (all walks) */
+    uint64_t walks;            /* walks that trained it */
+    uint64_t adam_steps;       /* optimizer steps taken */
+    float loss;                /* last training batch loss (MSE on log1p steps) */
+    uint32_t has_params;       /* 0 until a walk has stored parameters */
+    uint64_t scored;           /* closed samples the network had predicted */
+    double net_abs_err;        /* their mean |log1p error| ... */
+    double mean_abs_err;       /* ... and that of the running mean of earlier labels */
+} gwh_model_info;
+gw_status gwh_model_create(gwh_model** out);
+gw_status gwh_model_destroy(gwh_model* m);
+gw_status gwh_model_info_get(const gwh_model* m, gwh_model_info* out);
 /* gwh_engine_opts.flags */
 enum {
     GWH_NO_SWAP_PREFETCH = 1u << 0, /* learned: once the cache is full, do not keep a slot for
                                        staging the block the plain rule would load next (the
                                        default keeps one: each upload then overlaps the previous
                                        run, for a slot and some mispredicted copies) */
-    GWH_MYOPIC_LABEL = 1u << 1   /* learned: a load's label is the next run's steps only */
+    GWH_MYOPIC_LABEL = 1u << 1,  /* learned: a load's label is the next run's steps only */
+    GWH_MODEL_SWAP = 1u << 2,    /* learned: once the cache is full, the network picks the load
+                                    among the plain rule's near-best candidates (default: the
+                                    plain rule's load and victim; the network ranks the fill
+                                    and decides the zero-copy tail either way) */
+    GWH_VALUE_VICTIM = 1u << 3,  /* learned, with GWH_MODEL_SWAP: a swap evicts the resident
+                                    block whose eviction
+                                    gives up the fewest steps (its walkers paired with the
+                                    load plus the network's value of keeping it) instead of
+                                    the least-paired one */
+    GWH_NO_STICKY = 1u << 4      /* learned / staging: a swap does not prefer the block already
+                                    staged among near-best candidates */
 };
 
 typedef struct {
@@ -35,6 +70,7 @@ typedef struct {
     uint32_t flags;            /* GWH_* flags */
     uint32_t prefetch_depth;   /* learned scheduler: staged block copies in flight (0 = 8) */
     uint32_t reserved;
+    gwh_model* model;          /* learned scheduler: state carried across walks (may be NULL) */
 } gwh_engine_opts;
 
 typedef struct {
@@ -47,10 +83,17 @@ typedef struct {
     uint32_t pad;
     uint64_t staged;           /* blocks prefetched into free slots (their bytes are in h2d_bytes) */
     uint64_t zc_touches;       /* zero-copy reads of host-mapped blocks (about 64 B of PCIe each) */
-    uint64_t tail_decisions;   /* learned: zero-copy chosen over an upload */
+    uint64_t tail_decisions;   /* learned / staging: zero-copy chosen over an upload (both kinds) */
     double sched_ms;           /* host time spent scheduling and placing blocks */
     double tune_ms;            /* autotuner time (launch shape and search order) */
     gw_launch tuned;           /* what it picked (block 0: no tuning) */
+    uint64_t tail_model;       /* ... of which the network's rule chose */
+    uint64_t tail_threshold;   /* ... of which the reference's walker threshold chose */
+    uint64_t swap_model;       /* cache-full iterations whose load and victim the network chose */
+    uint64_t swap_differs;     /* ... of which differ from the plain rule's choice */
+    uint64_t fill_model;       /* free-slot iterations whose load order the network ranked */
+    uint64_t model_samples;    /* samples the network had after the walk (all walks) */
+    uint64_t sticky_swaps;     /* swaps that took the already staged near-best block */
 } gwh_report;
 
 /* Runs walkers to completion out of core: beg has `entries` u64 offsets (ids

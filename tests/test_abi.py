@@ -8,10 +8,10 @@ import pytest
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared():
-    src = open(os.path.join(REPO, "include", "gw.h")).read()
+def declared(header="gw.h", prefix="gw_"):
+    src = open(os.path.join(REPO, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(gw_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(%s[a-z0-9_]+)\s*\(" % prefix, src)))
 
 
 def test_library_exports_every_declaration():
@@ -28,6 +28,31 @@ def test_library_exports_every_declaration():
 def test_python_signatures_cover_header():
     from gowalker_amd._abi import SIGNATURES
     assert set(declared()) == set(SIGNATURES)
+
+
+def test_host_library_exports_every_declaration():
+    """libgwhost.so (include/gwh.h, the C++ engine behind one C call) exports every
+    function its header declares, and the Python mirror names each of them."""
+    from gowalker_amd._abi import HOST_LIB_PATH, HOST_SIGNATURES, host_lib
+    host_lib()
+    out = subprocess.check_output(["nm", "-D", "--defined-only", HOST_LIB_PATH], text=True)
+    exported = set(l.split()[-1] for l in out.splitlines() if l.strip())
+    names = declared("gwh.h", "gwh_")
+    assert len(names) >= 5
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    assert set(names) == set(HOST_SIGNATURES)
+
+
+def test_host_model_handle_without_gpu():
+    """gwh_model holds the learned scheduler's state between walks; creating and querying
+    one touches no GPU."""
+    from gowalker_amd.engine import Model
+    m = Model()
+    info = m.info()
+    assert info == dict(samples=0, walks=0, adam_steps=0, loss=0.0, has_params=0, scored=0, net_abs_err=0.0,
+                        mean_abs_err=0.0)
+    m.close()
 
 
 def test_errors_are_returned_not_raised_by_library():

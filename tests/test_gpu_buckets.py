@@ -1,0 +1,87 @@
+"""The bucket matrix W[prev_blk, cur_blk] of unfinished walkers (gw_bucket_counts; the
+reference counts it with atomicAdd(&block_offset[prev_blk * ncblock + cur_blk]) at the
+exit of gpu_update, gpu/gpuwalk.hpp:177-184, and scans it in `prefix`, :321-338) against an
+independent count: the walker state read back with gw_read_state (cur, prev, hop,
+status), each unfinished walker put in the bucket of its (prev, cur) blocks by a binary
+search of the block table's cut points in numpy.  Checked after initialisation, after
+runs with a part of the blocks resident (the count left by the run's fused bucket pass,
+GW_RUN_BUCKETS, and the separate pass of a later call), for tables staged in LDS and read
+from global memory, Node2vec and mixed Node2vec + SOSR batches with termination, and
+once the walk is over (all zero)."""
+import numpy as np
+import pytest
+
+import graphs as G
+from gowalker_amd.engine import NODE2VEC, OUT_HASH, SOPR, Blocks, Context
+
+pytestmark = pytest.mark.gpu
+
+
+def histogram(ctx, bl):
+    s = ctx.state()
+    live = s[:, 3] == 0
+    cut = bl.vcut[:-1].astype(np.int64)
+    cb = np.searchsorted(cut, s[live, 0].astype(np.int64), side="right") - 1
+    pb = np.searchsorted(cut, s[live, 1].astype(np.int64), side="right") - 1
+    H = np.zeros((bl.nblocks, bl.nblocks), dtype=np.uint64)
+    np.add.at(H, (pb, cb), 1)
+    return H, int(live.sum())
+
+
+def check(ctx, bl, what):
+    W = ctx.bucket_counts()
+    H, live = histogram(ctx, bl)
+    print(what, "walkers left", live, "buckets used", int((H > 0).sum()), flush=True)
+    np.testing.assert_array_equal(W, H, err_msg=what)
+    assert int(W.sum()) == live == ctx.walkers_left()
+    return live
+
+
+# (graph, block bytes, mixed batch): degrees stay below blocksize/4, so no single-vertex
+# block is truncated (engine/cache.hpp:170-172); 1024-byte blocks give > 128 blocks (table
+# read from global memory), the others <= 128 (table staged in LDS)
+CASES = [("undirected", 16384, False), ("undirected", 1024, False), ("directed", 4096, True),
+         ("undirected", 1024, True)]
+
+
+@pytest.mark.parametrize("graph,block_bytes,mixed", CASES)
+def test_bucket_matrix_matches_walker_histogram(gpu, graph, block_bytes, mixed):
+    if graph == "undirected":
+        beg, csr = G.random_undirected(8000, 24000, seed=5)
+    else:
+        beg, csr = G.rmat(13, 30000, seed=8, undirected=False)   # dead ends too
+    n = len(beg) - 1
+    assert int(np.diff(beg.astype(np.int64)).max()) <= block_bytes // 4
+    bl = Blocks(beg, csr, block_bytes)
+    nb = bl.nblocks
+    assert nb >= 8 and (nb > 128) == (block_bytes == 1024)
+    types = [dict(algo=NODE2VEC, length=25, p=0.5, q=2.0)]
+    if mixed:
+        types.append(dict(algo=SOPR, length=25, alpha=0.2, term=0.15))
+    wpv = 2 if mixed else 1
+    rng = np.random.default_rng(block_bytes + int(mixed))
+    with Context() as ctx:
+        ctx.set_blocks(bl)
+        ctx.set_walk(types, 31)
+        ctx.init_walkers(n * wpv, wpv, outputs=OUT_HASH)
+        assert check(ctx, bl, "after init") == n * wpv
+        k = max(2, nb // 3)
+        ctx.reserve_slots(k)
+        for it in range(6):
+            # a different part of the table resident each time; the fused pass of the run
+            # leaves W, then the separate pass of the next call counts it again
+            resident = sorted(rng.choice(nb, k, replace=False).tolist())
+            for s in range(k):
+                ctx.evict(s)
+            for s, b in enumerate(resident):
+                ctx.upload(s, b)
+            ctx.wait_uploads()
+            st = ctx.run(buckets=(it % 2 == 0))
+            print("run", it, "resident", resident, "walkers run", st["walkers_run"], flush=True)
+            check(ctx, bl, "after run %d (%s pass)" % (it, "fused" if it % 2 == 0 else "separate"))
+        ctx.reserve_slots(nb)
+        for b in range(nb):
+            ctx.upload(b, b)
+        ctx.wait_uploads()
+        assert ctx.run(buckets=True)["walkers_left"] == 0
+        assert check(ctx, bl, "after the walk") == 0

@@ -83,6 +83,31 @@ def test_zero_copy_tail_and_sopr(gpu):
     c.close()
 
 
+def test_tight_budget_over_block_sizes(gpu):
+    """The engine sizes its slot count with gw_slot_bytes and gw_reserve_slots charges the
+    same bytes per slot: with a budget of the walker state plus exactly three slots (64 B
+    of slack), the walk must finish for every block size, whatever the largest block's
+    edge count modulo 64 (the padding the slot's window alignment adds)."""
+    beg, csr = G.rmat(12, 30000, seed=4, permute=True)
+    n = len(beg) - 1
+    b, c = pinned_graph(beg, csr)
+    types = [dict(algo=E.NODE2VEC, length=10, p=0.5, q=2.0)]
+    ref = None
+    for block_bytes in range(16384, 16384 + 33 * 8, 8):
+        with E.Context() as ctx:
+            ctx.set_blocks(E.Blocks(beg, csr, block_bytes))
+            slot = ctx.slot_bytes()
+        mem = n * (5 * 4 + 8) + 3 * slot + 64
+        rep, h, _ = E.run_out_of_core(b.a, c.a, types, 6, n, memory_bytes=mem, block_bytes=block_bytes,
+                                      scheduler=E.SCHED_PLAIN, hashes=True)
+        assert rep["nslots"] == 3, (block_bytes, rep["nslots"])
+        if ref is None:
+            ref = h
+        np.testing.assert_array_equal(h, ref)
+    b.close()
+    c.close()
+
+
 def test_budget_too_small_is_an_error(gpu):
     beg, csr = G.rmat(10, 5000, seed=1)
     b, c = pinned_graph(beg, csr)
