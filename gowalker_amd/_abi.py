@@ -109,7 +109,11 @@ SIGNATURES = {
     "gw_mlp_forward": [P, P, P, C.c_uint32, P],
     "gw_mlp_backward": [P, P, P, P, C.c_uint32, P, P],
     "gw_mlp_step": [P, P, P, P, C.c_uint32, C.c_float, P],
+    "gw_walk_resident_noalloc": [P, P, C.c_uint32, C.c_uint64, C.POINTER(GwWalkParams), C.c_uint64, C.c_uint32, P, P,
+                                 P, C.POINTER(GwRunStats)],
 }
+GW_NOALLOC_MAX_WALKERS = 1 << 20
+GW_NOALLOC_SCRATCH_BYTES = 1024 + 16 * GW_NOALLOC_MAX_WALKERS
 _RESTYPE = {"gw_last_error": C.c_char_p}
 
 _lib = None

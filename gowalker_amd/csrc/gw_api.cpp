@@ -670,47 +670,60 @@ static uint64_t thr(double w, double m)
     return (uint64_t)std::floor(w / m * 4294967296.0);
 }
 
+}  // extern "C"
+
+namespace gw {
+// The integer form of one walker type (acceptance thresholds, fixed-point alpha,
+// termination threshold); nullptr on success, else what is wrong with it.
+const char* dev_type_from(const gw_walk_type& ty, DevType* out)
+{
+    static thread_local char msg[160];
+    DevType d{};
+    if (ty.algo > 1) { snprintf(msg, sizeof msg, "unknown algorithm %u", ty.algo); return msg; }
+    if (ty.length > kMaxLength) { snprintf(msg, sizeof msg, "length %u exceeds %u", ty.length, kMaxLength); return msg; }
+    if (!(ty.term >= 0.0 && ty.term < 1.0)) return "term must be in [0, 1)";
+    d.algo = ty.algo;
+    d.length = ty.length;
+    d.t_term = ty.term > 0.0 ? (uint64_t)std::floor(ty.term * 4294967296.0) : 0;
+    if (ty.algo == GW_NODE2VEC) {
+        if (!(ty.p >= 1.0 / 65536 && ty.p <= 65536 && ty.q >= 1.0 / 65536 && ty.q <= 65536))
+            return "p and q must lie in [2^-16, 2^16]";
+        const double ip = 1.0 / ty.p, iq = 1.0 / ty.q;
+        const double m = std::max(std::max(ip, 1.0), iq);
+        d.t_p = thr(ip, m);
+        d.t_one = thr(1.0, m);
+        d.t_q = thr(iq, m);
+        d.t_min = std::min(std::min(d.t_p, d.t_one), d.t_q);
+        if (d.t_min == 0) return "a Node2vec acceptance threshold rounds to zero";
+        d.m_p = (uint32_t)(d.t_p - 1);
+        d.m_one = (uint32_t)(d.t_one - 1);
+        d.m_q = (uint32_t)(d.t_q - 1);
+        d.m_min = (uint32_t)(d.t_min - 1);
+        d.m_lo = (uint32_t)(std::min(d.t_one, d.t_q) - 1);
+        d.m_hi = (uint32_t)(std::max(d.t_one, d.t_q) - 1);
+    } else {
+        if (!(ty.alpha >= 0.0 && ty.alpha < 1.0)) return "alpha must be in [0, 1)";
+        const long long a = std::llround(ty.alpha * 65536.0);
+        if (a >= 65536) return "alpha rounds to 1 in 16-bit fixed point";
+        d.alpha_a = (uint32_t)a;
+        d.alpha_b = 65536u - (uint32_t)a;
+        d.m_prop = (uint32_t)(((uint64_t)d.alpha_b << 16) - 1);
+        if (ty.sopr_sampler > 1) { snprintf(msg, sizeof msg, "unknown SOPR sampler %u", ty.sopr_sampler); return msg; }
+        d.sampler = ty.sopr_sampler;
+    }
+    *out = d;
+    return nullptr;
+}
+}  // namespace gw
+
+extern "C" {
+
 gw_status gw_set_walk(gw_ctx* c, const gw_walk_params* wp)
 {
     if (!c || !wp) return fail(c, GW_ERR_ARG, "null argument");
     if (wp->ntypes < 1 || wp->ntypes > 2) return fail(c, GW_ERR_ARG, "ntypes must be 1 or 2");
-    for (uint32_t t = 0; t < wp->ntypes; ++t) {
-        const gw_walk_type& ty = wp->type[t];
-        DevType& d = c->dt[t];
-        d = DevType{};
-        if (ty.algo > 1) return fail(c, GW_ERR_ARG, "unknown algorithm %u", ty.algo);
-        if (ty.length > kMaxLength) return fail(c, GW_ERR_ARG, "length %u exceeds %u", ty.length, kMaxLength);
-        if (!(ty.term >= 0.0 && ty.term < 1.0)) return fail(c, GW_ERR_ARG, "term must be in [0, 1)");
-        d.algo = ty.algo;
-        d.length = ty.length;
-        d.t_term = ty.term > 0.0 ? (uint64_t)std::floor(ty.term * 4294967296.0) : 0;
-        if (ty.algo == GW_NODE2VEC) {
-            if (!(ty.p >= 1.0 / 65536 && ty.p <= 65536 && ty.q >= 1.0 / 65536 && ty.q <= 65536))
-                return fail(c, GW_ERR_ARG, "p and q must lie in [2^-16, 2^16]");
-            const double ip = 1.0 / ty.p, iq = 1.0 / ty.q;
-            const double m = std::max(std::max(ip, 1.0), iq);
-            d.t_p = thr(ip, m);
-            d.t_one = thr(1.0, m);
-            d.t_q = thr(iq, m);
-            d.t_min = std::min(std::min(d.t_p, d.t_one), d.t_q);
-            if (d.t_min == 0) return fail(c, GW_ERR_ARG, "a Node2vec acceptance threshold rounds to zero");
-            d.m_p = (uint32_t)(d.t_p - 1);
-            d.m_one = (uint32_t)(d.t_one - 1);
-            d.m_q = (uint32_t)(d.t_q - 1);
-            d.m_min = (uint32_t)(d.t_min - 1);
-            d.m_lo = (uint32_t)(std::min(d.t_one, d.t_q) - 1);
-            d.m_hi = (uint32_t)(std::max(d.t_one, d.t_q) - 1);
-        } else {
-            if (!(ty.alpha >= 0.0 && ty.alpha < 1.0)) return fail(c, GW_ERR_ARG, "alpha must be in [0, 1)");
-            const long long a = std::llround(ty.alpha * 65536.0);
-            if (a >= 65536) return fail(c, GW_ERR_ARG, "alpha rounds to 1 in 16-bit fixed point");
-            d.alpha_a = (uint32_t)a;
-            d.alpha_b = 65536u - (uint32_t)a;
-            d.m_prop = (uint32_t)(((uint64_t)d.alpha_b << 16) - 1);
-            if (ty.sopr_sampler > 1) return fail(c, GW_ERR_ARG, "unknown SOPR sampler %u", ty.sopr_sampler);
-            d.sampler = ty.sopr_sampler;
-        }
-    }
+    for (uint32_t t = 0; t < wp->ntypes; ++t)
+        if (const char* e = dev_type_from(wp->type[t], &c->dt[t])) return fail(c, GW_ERR_ARG, "%s", e);
     if (wp->ntypes == 1) c->dt[1] = c->dt[0];
     c->wp = *wp;
     c->walk_set = true;

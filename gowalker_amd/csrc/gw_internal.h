@@ -113,6 +113,12 @@ struct FilterArgs {
     uint32_t len0, len1;      // hop caps of walker types 0 / 1
 };
 
+// records a message for gw_last_error (gw_api.cpp)
+gw_status gen_fail(gw_ctx* ctx, gw_status st, const char* msg);
+
+// the integer form of one walker type (gw_api.cpp); nullptr on success, else the reason
+const char* dev_type_from(const gw_walk_type& ty, DevType* out);
+
 // launchers (gw_walk.hip)
 hipError_t launch_init(const InitArgs& a, hipStream_t s);
 hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s);

@@ -266,6 +266,28 @@ gw_status gw_mlp_backward(gw_ctx* ctx, gw_mlp* m, const float* x, const float* t
 gw_status gw_mlp_step(gw_ctx* ctx, gw_mlp* m, const float* x, const float* t, uint32_t n,
                       float lr, float* loss);
 
+/* ---- allocation-free walk (no counterpart in the reference).  For callers that cannot
+ *      allocate device memory: code under HIP graph capture, tight serving loops, harnesses
+ *      that forbid hipMalloc.  It is gw_init_walkers + gw_run over a graph the caller holds
+ *      in device memory as one resident block: beg = nverts + 1 offsets into csr (beg[0] =
+ *      0), csr = nedges sorted u32 targets.  Walker w = 0 .. nwalkers-1 starts at
+ *      w / walkers_per_vertex; the walks equal gw_run's bit for bit.  hashes (nwalkers u64)
+ *      and visits (nverts u32, cleared first) are optional device outputs.  The walker
+ *      state, work queue and counters live in a fixed device array of the library,
+ *      GW_NOALLOC_SCRATCH_BYTES per GPU, held from the first call on a GPU until the
+ *      process ends; nwalkers above GW_NOALLOC_MAX_WALKERS is rejected (GW_ERR_ARG).  Work
+ *      is queued on `stream` (a hipStream_t; NULL = the null stream).  stats == NULL: the
+ *      call returns without waiting (make a first call outside a capture: it looks up the
+ *      array's address on that GPU); otherwise it waits for the stream and fills steps,
+ *      trials, probes, mix_steps and the walker counts (kernel_ms 0).  Nothing is
+ *      allocated; calls that overlap on one GPU share its array and race.  Errors are read
+ *      with gw_last_error(NULL). ---- */
+#define GW_NOALLOC_MAX_WALKERS (1u << 20)
+#define GW_NOALLOC_SCRATCH_BYTES (1024ull + 16ull * GW_NOALLOC_MAX_WALKERS)
+gw_status gw_walk_resident_noalloc(const uint64_t* beg, const uint32_t* csr, uint32_t nverts, uint64_t nedges,
+                                   const gw_walk_params* params, uint64_t nwalkers, uint32_t walkers_per_vertex,
+                                   uint64_t* hashes, uint32_t* visits, void* stream, gw_run_stats* stats);
+
 /* ---- outputs ---- */
 gw_status gw_read_visits(gw_ctx* ctx, uint32_t* out, uint64_t n);
 gw_status gw_read_hashes(gw_ctx* ctx, uint64_t* out, uint64_t n);

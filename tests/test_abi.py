@@ -88,3 +88,27 @@ def test_python_flag_constants_match_header():
     assert {"GW_RUN_BUCKETS", "GW_RUN_TRACE", "GW_RUN_INTERP", "GW_RUN_WINDOW"} <= set(flags)
     for name, value in flags.items():
         assert getattr(_abi, name) == value, name
+
+
+def test_noalloc_walk_rejects_sizes_beyond_its_array():
+    """gw_walk_resident_noalloc checks its sizes on the host before it touches a GPU: more
+    walkers than its fixed device array holds come back as GW_ERR_ARG with the limit in
+    the message; the header and the Python mirror state the same limit."""
+    import ctypes as C
+    from gowalker_amd._abi import GW_NOALLOC_MAX_WALKERS, GW_NOALLOC_SCRATCH_BYTES, lib
+    from gowalker_amd.engine import walk_params
+    hdr = open(os.path.join(REPO, "include", "gw.h")).read()
+    assert "#define GW_NOALLOC_MAX_WALKERS (1u << 20)" in hdr
+    assert "#define GW_NOALLOC_SCRATCH_BYTES (1024ull + 16ull * GW_NOALLOC_MAX_WALKERS)" in hdr
+    assert GW_NOALLOC_SCRATCH_BYTES == 1024 + 16 * (1 << 20) and GW_NOALLOC_MAX_WALKERS == 1 << 20
+    L = lib()
+    wp = walk_params([dict(algo=0, length=10, p=1.0, q=1.0)])
+    fake = C.c_void_p(0x1000)   # never dereferenced: the checks come first
+    st = L.gw_walk_resident_noalloc(fake, fake, 1 << 22, 100, C.byref(wp), GW_NOALLOC_MAX_WALKERS + 1, 1, None,
+                                    None, None, None)
+    assert st == -1
+    msg = L.gw_last_error(None).decode()
+    assert str(GW_NOALLOC_MAX_WALKERS) in msg and "walkers" in msg, msg
+    assert L.gw_walk_resident_noalloc(fake, fake, 1 << 22, 100, C.byref(wp), 0, 1, None, None, None, None) == -1
+    assert L.gw_walk_resident_noalloc(fake, fake, 10, 100, C.byref(wp), 100, 1, None, None, None, None) == -1
+    assert b"exceed" in L.gw_last_error(None)
