@@ -194,9 +194,11 @@ def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models):
     sched, zero, flags = SCHEDULERS[name]
     block_bytes = cfg["block_mb"] << 20
     launch = None
-    if args.variant is not None or args.waves_per_cu or args.window:
-        launch = dict(variant=2 if args.variant is None else args.variant, waves_per_cu=args.waves_per_cu,
-                      window=args.window)
+    if args.variant is not None or args.waves_per_cu or args.window or args.block:
+        # a fixed launch: with --block the search order is the flags given, else the engine's
+        # probe model picks it
+        launch = dict(block=args.block, variant=2 if args.variant is None else args.variant,
+                      waves_per_cu=args.waves_per_cu, window=args.window, interp=args.interp)
     # every walk keeps the same outputs (the per-walker path hash, 8 B per walker counted
     # against the cap), so warm-up and timed walks run with the same slot count
     rep, h, _ = E.run_out_of_core(beg, csr, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
