@@ -190,8 +190,13 @@ EXTRA = ["learned", "staging", "learned_model_swap", "plain_zero", "plain", "sow
                                                                             # each at other caps
 
 
-def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models):
+def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, timed):
+    """One full walk.  Every scheduler carries a model across its walks: the learned one
+    its network, all of them the launch tuner, which tries a candidate launch per untimed
+    walk and runs the fastest tried in timed walks."""
     sched, zero, flags = SCHEDULERS[name]
+    if timed:
+        flags |= E.TUNE_FREEZE
     block_bytes = cfg["block_mb"] << 20
     launch = None
     if args.variant is not None or args.waves_per_cu or args.window or args.block:
@@ -206,8 +211,7 @@ def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models):
                                   zero_threshold=block_bytes // 128 // 24 if zero else 0,
                                   scheduler=getattr(E, sched), hashes=True, flags=flags, launch=launch,
                                   prefetch_depth=args.prefetch_depth, verbose=args.verbose,
-                                  model=models.setdefault(name, E.Model()) if getattr(E, sched) == E.SCHED_LEARNED
-                                  else None)
+                                  model=models.setdefault(name, E.Model()))
     return rep, h
 
 
@@ -225,17 +229,18 @@ def summarize(reps, zt):
                 swap_model=r0["swap_model"], swap_differs=r0["swap_differs"], tail_model=r0["tail_model"],
                 tail_threshold=r0["tail_threshold"], model_samples=r0["model_samples"], fallbacks=r0["fallbacks"],
                 nslots=r0["nslots"], nblocks=r0["nblocks"], steps=r0["steps"], trials=r0["trials"],
-                probes=r0["probes"], zero_threshold=zt)
+                probes=r0["probes"], zero_threshold=zt, launch=r0["tuned"])
 
 
 def log_walk(cap_gb, name, tag, rep):
     print("[bench] cap %g GB %s %s: %.3f s, %d steps, %.2f GB H2D (%d uploads + %d staged), %.3f GB zero-copy, "
           "%d iterations, kernel %.0f ms, copy %.0f ms, sched %.0f ms, %d slots / %d blocks, model: fill %d swap %d "
-          "(%d differ) tail %d/%d samples %d, sticky %d"
+          "(%d differ) tail %d/%d samples %d, sticky %d, launch %s"
           % (cap_gb, name, tag, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["uploads"], rep["staged"],
              rep["zc_touches"] * 64 / 1e9, rep["iterations"], rep["kernel_ms"], rep["h2d_ms"], rep["sched_ms"],
              rep["nslots"], rep["nblocks"], rep["fill_model"], rep["swap_model"], rep["swap_differs"],
-             rep["tail_model"], rep["tail_threshold"], rep["model_samples"], rep["sticky_swaps"]),
+             rep["tail_model"], rep["tail_threshold"], rep["model_samples"], rep["sticky_swaps"],
+             "%(block)d/%(waves_per_cu)d/v%(variant)d" % rep["tuned"] if rep["tuned"] else "-"),
           file=sys.stderr, flush=True)
 
 
@@ -280,7 +285,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
             break
         t0 = time.perf_counter()
         for name in head:
-            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models)
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models, i >= args.warmup)
             check_hash(h)
             log_walk(cap0, name, "warmup %d" % i if i < args.warmup else "run %d" % (i - args.warmup), rep)
             if i >= args.warmup:
@@ -290,9 +295,14 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
             done_rounds += 1
     per_cap = []
     res0 = {n: summarize(reps[n], zt if SCHEDULERS[n][1] else 0) for n in head if reps[n]}
+    tuning = {}
+    for n in head:
+        rows, ch = models[n].tuning()
+        tuning[n] = dict(chosen=ch, candidates=[dict(r, kernel_ms_per_walk=r["ms_per_gstep"] * reps[n][0]["steps"] / 1e9
+                                                     if reps[n] else None) for r in rows])
     if not args.schedulers:
         for name in ONCE:
-            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models)
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models, True)
             check_hash(h)
             log_walk(cap0, name, "run 0", rep)
             res0[name] = summarize([rep], 0)
@@ -302,10 +312,10 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         for name in (args.schedulers or EXTRA):
             if name.startswith("learned"):
                 for w in range(args.extra_warmup):
-                    rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models)
+                    rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, False)
                     check_hash(h)
                     log_walk(cap_gb, name, "warmup %d" % w, rep)
-            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models)
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, True)
             check_hash(h)
             log_walk(cap_gb, name, "run 0", rep)
             res[name] = summarize([rep], zt if SCHEDULERS[name][1] else 0)
@@ -333,7 +343,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         if tj and tj.get("bytes_per_step"):
             traffic = tj["bytes_per_step"] * r0["steps"] / launches
             traffic_note = tj.get("source")
-    mi = {n: m.info() for n, m in models.items()}
+    mi = {n: m.info() for n, m in models.items() if n.startswith("learned")}
     for m in models.values():
         m.close()
     line = {
@@ -347,7 +357,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         "config": {"workload": cfg["workload"], "nverts": nverts, "narcs": narcs,
                    "non_isolated": int(np.count_nonzero(beg[1:] != beg[:-1])), "walkers": nwalkers,
                    "edges_drawn": cfg.get("edges_drawn"), "caps": per_cap, "walks_identical": identical,
-                   "model": mi, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS, "parallelism": "1 GPU",
+                   "model": mi, "launch_tuning": tuning, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS, "parallelism": "1 GPU",
                    "steps_requested": args.steps, "launch": r0.get("tuned")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -404,6 +414,8 @@ def main():
                     help="out-of-core: comma list of schedulers (%s), run at every cap; default: %s timed "
                          "at the headline cap, SOWalker once, %s once at other caps"
                          % (",".join(SCHEDULERS), ",".join(HEADLINE), ",".join(EXTRA)))
+    ap.add_argument("--block-mb", type=int, default=0,
+                    help="out-of-core: graph block size in MB (default: the configuration's, 256)")
     ap.add_argument("--extra-warmup", type=int, default=2,
                     help="out-of-core: untimed learned walks before the timed one at caps after the first")
     ap.add_argument("--time-budget", type=float, default=400.0,
@@ -421,6 +433,8 @@ def main():
     from gowalker_amd import engine as E
     t_start = time.perf_counter()
     cfg = dict(CONFIGS[args.config])
+    if args.block_mb:
+        cfg["block_mb"] = args.block_mb
     ctx = E.Context(0)
     t0 = time.perf_counter()
     beg, csr, cfg["edges_drawn"] = build_graph(ctx, cfg["graph"], args.seed)

@@ -170,7 +170,11 @@ class GwhReport(C.Structure):
                 ("sched_ms", C.c_double), ("tune_ms", C.c_double), ("tuned", GwLaunch),
                 ("tail_model", C.c_uint64), ("tail_threshold", C.c_uint64), ("swap_model", C.c_uint64),
                 ("swap_differs", C.c_uint64), ("fill_model", C.c_uint64), ("model_samples", C.c_uint64),
-                ("sticky_swaps", C.c_uint64)]
+                ("sticky_swaps", C.c_uint64), ("tune_candidate", C.c_uint32), ("pad2", C.c_uint32)]
+
+
+class GwhTuneEntry(C.Structure):
+    _fields_ = [("launch", GwLaunch), ("ms_per_gstep", C.c_double), ("walks", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class GwhModelInfo(C.Structure):
@@ -186,6 +190,7 @@ HOST_SIGNATURES = {
     "gwh_model_create": [C.POINTER(P)],
     "gwh_model_destroy": [P],
     "gwh_model_info_get": [P, C.POINTER(GwhModelInfo)],
+    "gwh_model_tuning": [P, P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
 }
 
 _hlib = None

@@ -302,7 +302,7 @@ def gen_chung_lu(ctx, nverts, nedges, gamma=2.2, max_degree=3000000, seed=42, de
 
 SCHED_PLAIN, SCHED_LEARNED, SCHED_SOWALKER, SCHED_STAGING = 0, 1, 2, 3
 # gwh_engine_opts.flags
-NO_SWAP_PREFETCH, MYOPIC_LABEL, MODEL_SWAP, VALUE_VICTIM, NO_STICKY = 1, 2, 4, 8, 16
+NO_SWAP_PREFETCH, MYOPIC_LABEL, MODEL_SWAP, VALUE_VICTIM, NO_STICKY, NO_TUNE, TUNE_FREEZE = 1, 2, 4, 8, 16, 32, 64
 
 
 class Model:
@@ -321,6 +321,17 @@ class Model:
         i = GwhModelInfo()
         check(host_lib().gwh_model_info_get(self.h, C.byref(i)))
         return {f: getattr(i, f) for f, _ in GwhModelInfo._fields_}
+
+    def tuning(self):
+        """The launch tuner's table for the workload of the last walk: a list of dicts
+        (block, waves_per_cu, variant, ms_per_gstep, walks) and the index of the fastest."""
+        from ._abi import GwhTuneEntry, host_lib
+        out = (GwhTuneEntry * 16)()
+        n, ch = C.c_uint32(0), C.c_uint32(0)
+        check(host_lib().gwh_model_tuning(self.h, out, 16, C.byref(n), C.byref(ch)))
+        rows = [dict(block=e.launch.block, waves_per_cu=e.launch.waves_per_cu, variant=e.launch.variant,
+                     ms_per_gstep=e.ms_per_gstep, walks=e.walks) for e in out[:n.value]]
+        return rows, ch.value
 
     def close(self):
         if self.h:
@@ -365,8 +376,8 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
     if st != 0:
         from ._abi import GwError
         raise GwError(st, L.gwh_last_error().decode())
-    out = {f: getattr(rep, f) for f, _ in GwhReport._fields_ if f not in ("pad", "tuned")}
+    out = {f: getattr(rep, f) for f, _ in GwhReport._fields_ if f not in ("pad", "pad2", "tuned")}
     t = rep.tuned
-    out["tuned"] = dict(block=t.block, variant=t.variant, interp=bool(t.flags & GW_RUN_INTERP),
-                        window=bool(t.flags & GW_RUN_WINDOW)) if t.block else None
+    out["tuned"] = dict(block=t.block, waves_per_cu=t.waves_per_cu, variant=t.variant,
+                        interp=bool(t.flags & GW_RUN_INTERP), window=bool(t.flags & GW_RUN_WINDOW)) if t.block else None
     return out, h, v

@@ -40,6 +40,9 @@ struct EngineConfig {
     // block 0: 256 threads of `variant` (default: the lane kernel, which walks out-of-core
     // Node2vec fastest; other walker types run variant 0), search order from the probe model
     gw_launch launch{0, 0, 2, 0};
+    // with a block given: its shape, but the search order from the probe model (the launch
+    // tuner's candidates fix the shape only)
+    bool auto_search = false;
     bool verbose = false;
 };
 
@@ -186,7 +189,7 @@ public:
         std::vector<uint8_t> staged(nb, 0);
         std::vector<uint32_t> pf;
         gw_launch base = cfg_.launch;
-        bool pick_shape = base.block == 0;
+        bool pick_shape = base.block == 0 || cfg_.auto_search;
         // launch of the next run; on the first call with no shape given, the membership
         // search order is chosen from the probes both orders need on sampled Node2vec-style
         // queries of this graph (timing on the device would need a resident cache that is
@@ -202,7 +205,7 @@ public:
                                                                : (cfg_.launch.variant == 2 ? 16u : 0u);
                 // membership probes read 64-byte windows (a third of the reads of single
                 // probes on C3: 1.9 vs 5.3 per step, -7% step-kernel time)
-                base = gw_launch{256, wpc, cfg_.launch.variant,
+                base = gw_launch{cfg_.launch.block ? cfg_.launch.block : 256u, wpc, cfg_.launch.variant,
                                  (prefer_interpolation(beg_, csr_, total_verts()) ? (uint32_t)GW_RUN_INTERP : 0u) |
                                      (uint32_t)GW_RUN_WINDOW};
                 r.tune_ms = std::chrono::duration<double, std::milli>(clk::now() - tt).count();

@@ -9,11 +9,13 @@
 #include "format.hpp"
 #include "learned.hpp"
 #include "schedule.hpp"
+#include "tuner.hpp"
 
 static std::string g_err;
 
 struct gwh_model {
-    gwh::ModelState state;
+    gwh::ModelState state;     // the learned scheduler's network and samples
+    gwh::LaunchTuner tuner;    // step-kernel launch timings per workload
 };
 
 extern "C" const char* gwh_last_error(void) { return g_err.c_str(); }
@@ -52,6 +54,29 @@ extern "C" gw_status gwh_model_info_get(const gwh_model* m, gwh_model_info* out)
     return GW_OK;
 }
 
+extern "C" gw_status gwh_model_tuning(const gwh_model* m, gwh_tune_entry* out, uint32_t cap, uint32_t* n,
+                                      uint32_t* chosen)
+{
+    if (!m || !n) {
+        g_err = "gwh_model_tuning: null argument";
+        return GW_ERR_ARG;
+    }
+    *n = 0;
+    const gwh::LaunchTuner::Key* k = m->tuner.last_key();
+    const auto* t = k ? m->tuner.find(*k) : nullptr;
+    if (!t) return GW_OK;
+    uint32_t best = 0;
+    bool have = false;
+    for (uint32_t i = 0; i < t->size(); ++i) {
+        const auto& e = (*t)[i];
+        if (e.walks && (!have || e.ms_per_gstep < (*t)[best].ms_per_gstep)) { best = i; have = true; }
+        if (out && i < cap) out[i] = gwh_tune_entry{e.launch, e.ms_per_gstep, e.walks, 0};
+    }
+    *n = (uint32_t)t->size();
+    if (chosen) *chosen = best;
+    return GW_OK;
+}
+
 extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t entries, const gwh_engine_opts* opts,
                              const gw_walk_params* params, uint64_t nwalkers, uint32_t wpv, const uint32_t* sources,
                              gwh_report* report, uint64_t* hashes, uint32_t* visits)
@@ -75,6 +100,17 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         ec.zero_threshold = opts->zero_threshold;
         ec.launch = opts->launch;
         ec.verbose = opts->verbose != 0;
+        // launch tuning across walks: with a model and no launch given, this walk runs the
+        // tuner's next candidate shape
+        const bool tune = opts->model && opts->launch.block == 0 && (opts->flags & GWH_NO_TUNE) == 0;
+        const LaunchTuner::Key tkey{opts->memory_bytes, opts->block_bytes, nwalkers, entries - 1};
+        uint32_t tcand = 0;
+        if (tune) {
+            tcand = (opts->flags & GWH_TUNE_FREEZE) ? opts->model->tuner.best(tkey) : opts->model->tuner.pick(tkey);
+            const gw_launch l = LaunchTuner::candidates()[tcand].launch;
+            ec.launch = gw_launch{l.block, l.waves_per_cu, l.variant, 0};
+            ec.auto_search = true;
+        }
         Engine engine(ec, beg, csr, make_block_table(vb, eb, opts->block_bytes));
         WalkConfig wc;
         wc.params = *params;
@@ -101,7 +137,8 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
             sched.reset(new PlainScheduler());
         }
         RunReport r = engine.run(wc, *sched);
-        if (ls) ls->save();
+        if (ls && opts->model) ls->save();
+        if (tune && !(opts->flags & GWH_TUNE_FREEZE)) opts->model->tuner.record(tkey, tcand, r.kernel_ms, r.steps);
         *report = gwh_report{};
         report->steps = r.steps;
         report->trials = r.trials;
@@ -123,6 +160,7 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         report->sched_ms = r.sched_ms;
         report->tune_ms = r.tune_ms;
         report->tuned = r.tuned;
+        report->tune_candidate = tune ? tcand : 0xFFFFFFFFu;
         if (ls) {
             report->learned_decisions = ls->learned_decisions();
             report->fallbacks = ls->fallbacks();

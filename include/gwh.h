@@ -20,11 +20,16 @@ extern "C" {
  * reference's walker threshold (apps/Node2vec.cu:100-104) */
 enum { GWH_SCHED_PLAIN = 0, GWH_SCHED_LEARNED = 1, GWH_SCHED_SOWALKER = 2, GWH_SCHED_STAGING = 3 };
 
-/* What the learned scheduler keeps from one walk to the next (no reference counterpart):
- * the network's parameters and its ring of block-access samples (features at a decision,
- * steps the block served afterwards).  Hand the same model to successive gwh_run calls
- * and each walk starts from the network the earlier walks trained; NULL in
- * gwh_engine_opts.model starts every walk from an untrained network. */
+/* What the engine keeps from one walk of a workload to the next (no reference
+ * counterpart): the learned scheduler's network parameters and its ring of block-access
+ * samples (features at a decision, steps the block served afterwards), and the launch
+ * tuner's step-kernel timings.  Hand the same model to successive gwh_run calls and each
+ * walk starts from the network the earlier walks trained; with gwh_engine_opts.launch.block
+ * == 0 each walk also runs the launch tuner's next untried candidate (workgroup size, waves
+ * per CU, kernel variant; the membership search order stays the probe model's), and once
+ * every candidate has run, the fastest (least step-kernel time per step).  Timings are kept
+ * per workload (cap, block size, walkers, vertices).  NULL in gwh_engine_opts.model starts
+ * every walk from an untrained network with the engine's default launch. */
 typedef struct gwh_model gwh_model;
 typedef struct {
     uint64_t samples;          /* samples recorded so far (all walks) */
@@ -39,6 +44,15 @@ typedef struct {
 gw_status gwh_model_create(gwh_model** out);
 gw_status gwh_model_destroy(gwh_model* m);
 gw_status gwh_model_info_get(const gwh_model* m, gwh_model_info* out);
+typedef struct {
+    gw_launch launch;          /* a candidate launch (flags: the probe model's search order) */
+    double ms_per_gstep;       /* its step-kernel time per 10^9 steps (mean over its walks) */
+    uint32_t walks;            /* walks timed with it (0: not tried yet) */
+    uint32_t pad;
+} gwh_tune_entry;
+/* The launch tuner's table for the workload of the model's last walk: up to cap entries
+ * into out (may be NULL), their number in *n, the fastest timed one in *chosen. */
+gw_status gwh_model_tuning(const gwh_model* m, gwh_tune_entry* out, uint32_t cap, uint32_t* n, uint32_t* chosen);
 /* gwh_engine_opts.flags */
 enum {
     GWH_NO_SWAP_PREFETCH = 1u << 0, /* learned: once the cache is full, do not keep a slot for
@@ -55,8 +69,11 @@ enum {
                                     gives up the fewest steps (its walkers paired with the
                                     load plus the network's value of keeping it) instead of
                                     the least-paired one */
-    GWH_NO_STICKY = 1u << 4      /* learned / staging: a swap does not prefer the block already
+    GWH_NO_STICKY = 1u << 4,     /* learned / staging: a swap does not prefer the block already
                                     staged among near-best candidates */
+    GWH_NO_TUNE = 1u << 5,       /* with a model: keep the engine's default launch (no tuning) */
+    GWH_TUNE_FREEZE = 1u << 6    /* with a model: run the fastest candidate timed so far, trying
+                                    no new one (for walks that are measured) */
 };
 
 typedef struct {
@@ -94,6 +111,8 @@ typedef struct {
     uint64_t fill_model;       /* free-slot iterations whose load order the network ranked */
     uint64_t model_samples;    /* samples the network had after the walk (all walks) */
     uint64_t sticky_swaps;     /* swaps that took the already staged near-best block */
+    uint32_t tune_candidate;   /* the launch tuner's candidate this walk ran (~0u: no tuning) */
+    uint32_t pad2;
 } gwh_report;
 
 /* Runs walkers to completion out of core: beg has `entries` u64 offsets (ids

@@ -1,7 +1,7 @@
 // Unit tests of the host layer that need no GPU: option parsing (api/cmdopts.hpp
 // semantics), the on-disk format (names, meta, block files, truncation), the text edge
-// list converter, the plain scheduler against the oracle's known answers, and slot
-// placement.
+// list converter, the plain scheduler against the oracle's known answers, slot
+// placement and the launch tuner's walk-by-walk choice.
 #include <algorithm>
 #include <cassert>
 #include <cstdio>
@@ -13,6 +13,7 @@
 #include "format.hpp"
 #include "engine.hpp"
 #include "schedule.hpp"
+#include "tuner.hpp"
 
 using namespace gwh;
 
@@ -266,10 +267,39 @@ static void test_search_order_model()
     CHECK(!prefer_interpolation(beg.data(), skew.data(), n));
 }
 
+static void test_launch_tuner()
+{
+    LaunchTuner t;
+    const LaunchTuner::Key a{16ull << 30, 256ull << 20, 1000, 1000}, b{8ull << 30, 256ull << 20, 1000, 1000};
+    const uint32_t nc = (uint32_t)LaunchTuner::candidates().size();
+    CHECK(nc >= 4);
+    CHECK(LaunchTuner::candidates()[0].launch.block == 256 && LaunchTuner::candidates()[0].launch.waves_per_cu == 16);
+    CHECK(t.best(a) == 0);   // nothing timed: the engine's default
+    // the first walk of a workload runs the default and is not timed
+    CHECK(t.pick(a) == 0);
+    t.record(a, 0, 1.0, 1000000000ull);
+    CHECK(t.find(a) && (*t.find(a))[0].walks == 0);
+    // then each walk runs the next untried candidate; candidate 2 is the fastest per step
+    for (uint32_t i = 0; i < nc; ++i) {
+        const uint32_t c = t.pick(a);
+        CHECK(c == i);
+        t.record(a, c, c == 2 ? 100.0 : 150.0 + c, 1000000000ull);
+    }
+    CHECK(t.pick(a) == 2 && t.best(a) == 2);
+    t.record(a, 2, 300.0, 1000000000ull);   // a slower repeat moves the mean: (100 + 300) / 2
+    CHECK(t.best(a) == 0);                  // 150 per 10^9 steps now beats 200
+    const auto* tab = t.find(a);
+    CHECK(tab && (*tab)[2].walks == 2 && (*tab)[2].ms_per_gstep == 200.0);
+    // another workload keeps its own table
+    CHECK(t.pick(b) == 0 && t.last_key() && *t.last_key() == b);
+    CHECK(t.find(b) && (*t.find(b))[0].walks == 0);
+}
+
 int main(int argc, char** argv)
 {
     std::string dir = argc > 1 ? argv[1] : "/tmp";
     test_options(dir);
+    test_launch_tuner();
     test_names();
     test_format(dir);
     test_convert(dir);
