@@ -190,7 +190,7 @@ EXTRA = ["learned", "staging", "learned_model_swap", "plain_zero", "plain", "sow
                                                                             # each at other caps
 
 
-def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, timed):
+def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, timed, packed=None):
     """One full walk.  Every scheduler carries a model across its walks: the learned one
     its network, all of them the launch tuner, which tries a candidate launch per untimed
     walk and runs the fastest tried in timed walks."""
@@ -211,7 +211,7 @@ def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, timed):
                                   zero_threshold=block_bytes // 128 // 24 if zero else 0,
                                   scheduler=getattr(E, sched), hashes=True, flags=flags, launch=launch,
                                   prefetch_depth=args.prefetch_depth, verbose=args.verbose,
-                                  model=models.setdefault(name, E.Model()))
+                                  model=models.setdefault(name, E.Model()), packed=packed)
     return rep, h
 
 
@@ -221,7 +221,8 @@ def summarize(reps, zt):
     zc_gb = r0["zc_touches"] * 64 / 1e9
     return dict(reps=len(reps), steps_per_s=sum(r["steps"] for r in reps) / wall, ms_per_run=wall * 1e3 / len(reps),
                 ms_min=min(r["wall_s"] for r in reps) * 1e3, ms_max=max(r["wall_s"] for r in reps) * 1e3,
-                h2d_gb=r0["h2d_bytes"] / 1e9, zero_copy_gb_est=zc_gb, pcie_gb_total=r0["h2d_bytes"] / 1e9 + zc_gb,
+                h2d_gb=r0["h2d_bytes"] / 1e9, h2d_raw_gb=r0["h2d_raw_bytes"] / 1e9, zero_copy_gb_est=zc_gb,
+                pcie_gb_total=r0["h2d_bytes"] / 1e9 + zc_gb,
                 h2d_gbs=(r0["h2d_bytes"] / 1e9) / (r0["h2d_ms"] / 1e3) if r0["h2d_ms"] else None,
                 iterations=r0["iterations"], uploads=r0["uploads"], staged=r0["staged"],
                 zero_copy_steps=r0["zero_copy_steps"], kernel_ms=r0["kernel_ms"], copy_ms=r0["h2d_ms"],
@@ -233,10 +234,11 @@ def summarize(reps, zt):
 
 
 def log_walk(cap_gb, name, tag, rep):
-    print("[bench] cap %g GB %s %s: %.3f s, %d steps, %.2f GB H2D (%d uploads + %d staged), %.3f GB zero-copy, "
+    print("[bench] cap %g GB %s %s: %.3f s, %d steps, %.2f GB H2D (%.2f GB raw; %d uploads + %d staged), %.3f GB zero-copy, "
           "%d iterations, kernel %.0f ms, copy %.0f ms, sched %.0f ms, %d slots / %d blocks, model: fill %d swap %d "
           "(%d differ) tail %d/%d samples %d, sticky %d, launch %s"
-          % (cap_gb, name, tag, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["uploads"], rep["staged"],
+          % (cap_gb, name, tag, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["h2d_raw_bytes"] / 1e9,
+             rep["uploads"], rep["staged"],
              rep["zc_touches"] * 64 / 1e9, rep["iterations"], rep["kernel_ms"], rep["h2d_ms"], rep["sched_ms"],
              rep["nslots"], rep["nblocks"], rep["fill_model"], rep["swap_model"], rep["swap_differs"],
              rep["tail_model"], rep["tail_threshold"], rep["model_samples"], rep["sticky_swaps"],
@@ -261,6 +263,17 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
     nwalkers = nverts * cfg["wpv"]
     models = {}      # one network per learned variant, carried across its walks
     ref_hash = None
+    packed, pack = None, None
+    if not args.no_pack:
+        # the blocks in the transfer encoding, built once on the host cores (not timed, like
+        # the graph's preprocessing into blocks)
+        t0 = time.perf_counter()
+        packed = E.PackedGraph(beg, csr, block_bytes)
+        pack = dict(packed.info(), pack_s=time.perf_counter() - t0)
+        pack["ratio"] = pack["packed_bytes"] / pack["raw_bytes"]
+        print("[bench] packed %d of %d blocks: %.2f GB of csr sent as %.2f GB (%.3f) in %.1f s"
+              % (pack["blocks_packed"], pack["nblocks"], pack["raw_bytes"] / 1e9, pack["packed_bytes"] / 1e9,
+                 pack["ratio"], pack["pack_s"]), file=sys.stderr, flush=True)
     identical = True
 
     def check_hash(h):
@@ -279,13 +292,13 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
     for i in range(args.warmup + args.steps):
         elapsed = time.perf_counter() - t_start
         est = max(round_s) if round_s else 0.0
-        if i >= args.warmup and elapsed + est + reserve > args.time_budget:
+        if i >= args.warmup and done_rounds >= 1 and elapsed + est + reserve > args.time_budget:
             print("[bench] time budget: %.0f s used, stopping after %d timed walks per scheduler"
                   % (elapsed, done_rounds), file=sys.stderr, flush=True)
             break
         t0 = time.perf_counter()
         for name in head:
-            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models, i >= args.warmup)
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models, i >= args.warmup, packed)
             check_hash(h)
             log_walk(cap0, name, "warmup %d" % i if i < args.warmup else "run %d" % (i - args.warmup), rep)
             if i >= args.warmup:
@@ -302,7 +315,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
                                                      if reps[n] else None) for r in rows])
     if not args.schedulers:
         for name in ONCE:
-            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models, True)
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models, True, packed)
             check_hash(h)
             log_walk(cap0, name, "run 0", rep)
             res0[name] = summarize([rep], 0)
@@ -312,10 +325,10 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         for name in (args.schedulers or EXTRA):
             if name.startswith("learned"):
                 for w in range(args.extra_warmup):
-                    rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, False)
+                    rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, False, packed)
                     check_hash(h)
                     log_walk(cap_gb, name, "warmup %d" % w, rep)
-            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, True)
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, True, packed)
             check_hash(h)
             log_walk(cap_gb, name, "run 0", rep)
             res[name] = summarize([rep], zt if SCHEDULERS[name][1] else 0)
@@ -344,6 +357,8 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
             traffic = tj["bytes_per_step"] * r0["steps"] / launches
             traffic_note = tj.get("source")
     mi = {n: m.info() for n, m in models.items() if n.startswith("learned")}
+    if packed is not None:
+        packed.close()
     for m in models.values():
         m.close()
     line = {
@@ -357,7 +372,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         "config": {"workload": cfg["workload"], "nverts": nverts, "narcs": narcs,
                    "non_isolated": int(np.count_nonzero(beg[1:] != beg[:-1])), "walkers": nwalkers,
                    "edges_drawn": cfg.get("edges_drawn"), "caps": per_cap, "walks_identical": identical,
-                   "model": mi, "launch_tuning": tuning, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS, "parallelism": "1 GPU",
+                   "model": mi, "launch_tuning": tuning, "packed_uploads": pack, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS, "parallelism": "1 GPU",
                    "steps_requested": args.steps, "launch": r0.get("tuned")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -414,12 +429,15 @@ def main():
                     help="out-of-core: comma list of schedulers (%s), run at every cap; default: %s timed "
                          "at the headline cap, SOWalker once, %s once at other caps"
                          % (",".join(SCHEDULERS), ",".join(HEADLINE), ",".join(EXTRA)))
+    ap.add_argument("--no-pack", action="store_true",
+                    help="out-of-core: upload blocks as stored (4 bytes per arc) instead of in the transfer encoding")
     ap.add_argument("--block-mb", type=int, default=0,
                     help="out-of-core: graph block size in MB (default: the configuration's, 256)")
     ap.add_argument("--extra-warmup", type=int, default=2,
                     help="out-of-core: untimed learned walks before the timed one at caps after the first")
     ap.add_argument("--time-budget", type=float, default=400.0,
-                    help="out-of-core: seconds after which no further headline walk starts (the line still prints)")
+                    help="out-of-core: seconds after which no further headline walk starts once one timed round is "
+                         "done (the line still prints)")
     ap.add_argument("--autotune", type=int, default=4_000_000,
                     help="resident workloads: time every launch shape and search order on this many walkers "
                          "(untimed, state restored) and run with the fastest; 0 = fixed shape")

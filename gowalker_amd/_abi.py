@@ -49,7 +49,8 @@ class GwRunStats(C.Structure):
 class GwCounters(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("trials", C.c_uint64), ("probes", C.c_uint64), ("mix_steps", C.c_uint64),
                 ("h2d_bytes", C.c_uint64), ("h2d_uploads", C.c_uint64), ("h2d_ms", C.c_double),
-                ("kernel_ms", C.c_double), ("hbm_bytes_held", C.c_uint64), ("zc_touches", C.c_uint64)]
+                ("kernel_ms", C.c_double), ("hbm_bytes_held", C.c_uint64), ("zc_touches", C.c_uint64),
+                ("h2d_raw_bytes", C.c_uint64)]
 
 
 class GwDeviceInfo(C.Structure):
@@ -109,6 +110,10 @@ SIGNATURES = {
     "gw_mlp_forward": [P, P, P, C.c_uint32, P],
     "gw_mlp_backward": [P, P, P, P, C.c_uint32, P, P],
     "gw_mlp_step": [P, P, P, P, C.c_uint32, C.c_float, P],
+    "gw_set_pack_staging": [P, C.c_uint64],
+    "gw_upload_block_packed": [P, C.c_uint32, C.c_uint32, P, P, C.c_uint64],
+    "gw_stage_block_packed": [P, C.c_uint32, C.c_uint32, P, P, C.c_uint64],
+    "gw_pack_block": [P, P, C.c_uint32, C.c_uint64, P, C.c_uint64, u64p],
     "gw_walk_resident_noalloc": [P, P, C.c_uint32, C.c_uint64, C.POINTER(GwWalkParams), C.c_uint64, C.c_uint32, P, P,
                                  P, C.POINTER(GwRunStats)],
 }
@@ -156,7 +161,7 @@ class GwhEngineOpts(C.Structure):
     _fields_ = [("memory_bytes", C.c_uint64), ("block_bytes", C.c_uint64), ("zero_threshold", C.c_uint64),
                 ("scheduler", C.c_uint32), ("verbose", C.c_uint32), ("launch", GwLaunch), ("device", C.c_int32),
                 ("flags", C.c_uint32), ("prefetch_depth", C.c_uint32), ("reserved", C.c_uint32),
-                ("model", C.c_void_p)]
+                ("model", C.c_void_p), ("packed", C.c_void_p)]
 
 
 class GwhReport(C.Structure):
@@ -170,7 +175,8 @@ class GwhReport(C.Structure):
                 ("sched_ms", C.c_double), ("tune_ms", C.c_double), ("tuned", GwLaunch),
                 ("tail_model", C.c_uint64), ("tail_threshold", C.c_uint64), ("swap_model", C.c_uint64),
                 ("swap_differs", C.c_uint64), ("fill_model", C.c_uint64), ("model_samples", C.c_uint64),
-                ("sticky_swaps", C.c_uint64), ("tune_candidate", C.c_uint32), ("pad2", C.c_uint32)]
+                ("sticky_swaps", C.c_uint64), ("tune_candidate", C.c_uint32), ("pad2", C.c_uint32),
+                ("h2d_raw_bytes", C.c_uint64)]
 
 
 class GwhTuneEntry(C.Structure):
@@ -191,6 +197,9 @@ HOST_SIGNATURES = {
     "gwh_model_destroy": [P],
     "gwh_model_info_get": [P, C.POINTER(GwhModelInfo)],
     "gwh_model_tuning": [P, P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
+    "gwh_pack_graph": [P, P, C.c_uint64, C.c_uint64, C.POINTER(P)],
+    "gwh_packed_info": [P, u64p, u64p, u32p, u32p],
+    "gwh_packed_destroy": [P],
 }
 
 _hlib = None

@@ -41,6 +41,10 @@ struct gw_ctx {
     uint32_t* d_vcut = nullptr;
     std::vector<uint32_t> vcut;
 
+    // staging buffer of packed block copies (decoded into the slots on the copy stream)
+    uint32_t* d_pack = nullptr;
+    uint64_t pack_cap = 0;
+
     // slots
     uint32_t nslots = 0;
     uint64_t slot_beg_bytes = 0, slot_csr_bytes = 0;
@@ -319,6 +323,7 @@ gw_status gw_destroy(gw_ctx* c)
     if (c->s_copy) (void)hipStreamSynchronize(c->s_copy);
     free_walkers(c);
     free_slots(c);
+    dfree(c, c->d_pack, c->pack_cap, true);
     dfree(c, c->d_bucket, 0, false);
     dfree(c, c->d_trace, 0, false);
     dfree(c, c->d_feat, 0, false);
@@ -522,10 +527,21 @@ static uint32_t* slot_edges(gw_ctx* c, uint32_t slot, uint32_t blk)
 // readers.  The wait is left out whenever it is not needed: a copy queued behind a
 // cross-stream wait has been measured to run at half the link rate for its whole length
 // (15 of 53 block copies of a 16 GB fill, 8.8 ms instead of 4.7 ms).
-static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const uint32_t* csr)
+static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const uint32_t* csr,
+                            const void* packed = nullptr, uint64_t packed_bytes = 0)
 {
     const gw_block_desc& d = c->desc[blk];
-    if (!beg || (d.nedges && !csr)) return fail(c, GW_ERR_ARG, "null block buffer");
+    if (!beg || (d.nedges && !csr && !packed)) return fail(c, GW_ERR_ARG, "null block buffer");
+    // a packed block: its chunk headers and list starts go to the staging buffer whole, its
+    // payload in pieces of what is left of the buffer (at least the largest chunk, 1 KiB)
+    const uint64_t nch = packed ? (d.nedges + kPackChunkArcs - 1) / kPackChunkArcs : 0;
+    const uint64_t head_w = nch * 4 + d.nverts;
+    if (packed && (packed_bytes & 3)) return fail(c, GW_ERR_ARG, "packed block size must be a multiple of 4");
+    if (packed && packed_bytes < head_w * 4) return fail(c, GW_ERR_ARG, "packed block shorter than its headers");
+    if (packed && head_w * 4 + 32 * 32 > c->pack_cap)
+        return fail(c, GW_ERR_ARG, "the %llu B staging buffer cannot hold the block's %llu B of headers and list starts "
+                    "plus one chunk (gw_set_pack_staging)", (unsigned long long)c->pack_cap,
+                    (unsigned long long)(head_w * 4));
     if (c->fault_kind == 1 && ++c->n_upload == c->fault_at)
         return fail(c, GW_ERR_FAULT, "injected upload failure (GW_FAULT=upload:%ld)", c->fault_at);
     (void)hipSetDevice(c->device);
@@ -540,12 +556,54 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
     const uint64_t bb = ((uint64_t)d.nverts + 1) * 8, cb = d.nedges * 4;
     HIPCHK(c, hipEventRecord(c->slot_ev0[slot], c->s_copy));
     HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, c->s_copy));
-    if (cb) HIPCHK(c, hipMemcpyAsync(slot_edges(c, slot, blk), csr, cb, hipMemcpyHostToDevice, c->s_copy));
+    if (cb && packed) {
+        // the copies and decodes share the copy stream, so each copy into the staging buffer
+        // queues behind the decode that read the previous piece
+        const uint32_t* pw = (const uint32_t*)packed;
+        const uint64_t piece_w = (c->pack_cap / 4) - head_w;
+        HIPCHK(c, hipMemcpyAsync(c->d_pack, pw, head_w * 4, hipMemcpyHostToDevice, c->s_copy));
+        const uint64_t total_w = packed_bytes / 4 - head_w;
+        uint64_t c0 = 0;
+        while (c0 < nch) {
+            const uint32_t w0 = pw[c0 * 4];
+            uint64_t c1 = c0;
+            while (c1 < nch && (uint64_t)pw[c1 * 4] + 8ull * pw[c1 * 4 + 1] - w0 <= piece_w) ++c1;
+            if (c1 == c0) return fail(c, GW_ERR_ARG, "a chunk of block %u does not fit the staging buffer", blk);
+            const uint64_t w1 = c1 < nch ? pw[c1 * 4] : total_w;
+            if (w1 > w0)
+                HIPCHK(c, hipMemcpyAsync(c->d_pack + head_w, pw + head_w + w0, (w1 - w0) * 4, hipMemcpyHostToDevice,
+                                         c->s_copy));
+            HIPCHK(c, launch_unpack(c->d_pack, c->slot_beg[slot], d.start_edge, d.nverts, d.nedges, c0, c1, w0,
+                                    slot_edges(c, slot, blk), c->s_copy));
+            c0 = c1;
+        }
+    } else if (cb) {
+        HIPCHK(c, hipMemcpyAsync(slot_edges(c, slot, blk), csr, cb, hipMemcpyHostToDevice, c->s_copy));
+    }
     HIPCHK(c, hipEventRecord(c->slot_ev1[slot], c->s_copy));
     HIPCHK(c, hipEventRecord(c->ev_copy, c->s_copy));
     c->slot_timed[slot] = 1;
-    c->tot.h2d_bytes += bb + cb;
+    c->tot.h2d_bytes += bb + (cb && packed ? packed_bytes : cb);
+    c->tot.h2d_raw_bytes += bb + cb;
     c->tot.h2d_uploads += 1;
+    return GW_OK;
+}
+
+gw_status gw_set_pack_staging(gw_ctx* c, uint64_t bytes)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->s_copy));
+    dfree(c, c->d_pack, c->pack_cap, true);
+    c->d_pack = nullptr;
+    c->pack_cap = 0;
+    if (!bytes) return GW_OK;
+    bytes = (bytes + 255) & ~255ULL;
+    void* p = nullptr;
+    gw_status st;
+    if ((st = dalloc(c, &p, bytes, true)) != GW_OK) return st;
+    c->d_pack = (uint32_t*)p;
+    c->pack_cap = bytes;
     return GW_OK;
 }
 
@@ -579,6 +637,37 @@ gw_status gw_upload_block(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t
     }
     if ((st = start_copy(c, slot, blk, beg, csr)) != GW_OK) return st;
     make_resident(c, slot, blk);
+    return GW_OK;
+}
+
+gw_status gw_upload_block_packed(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const void* packed,
+                                 uint64_t packed_bytes)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (!packed) return fail(c, GW_ERR_ARG, "null packed block");
+    gw_status st;
+    if ((st = check_slot_blk(c, slot, blk)) != GW_OK) return st;
+    if (c->blk_slot[blk] >= 0) {
+        c->slot_blk[c->blk_slot[blk]] = -1;
+        block_nonresident(c, blk);
+    }
+    if ((st = start_copy(c, slot, blk, beg, nullptr, packed, packed_bytes)) != GW_OK) return st;
+    make_resident(c, slot, blk);
+    return GW_OK;
+}
+
+gw_status gw_stage_block_packed(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const void* packed,
+                                uint64_t packed_bytes)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (!packed) return fail(c, GW_ERR_ARG, "null packed block");
+    gw_status st;
+    if ((st = check_slot_blk(c, slot, blk)) != GW_OK) return st;
+    if (c->blk_slot[blk] >= 0) return fail(c, GW_ERR_ARG, "block %u is already resident (slot %d)", blk, c->blk_slot[blk]);
+    for (uint32_t s = 0; s < c->nslots; ++s)
+        if (s != slot && c->slot_staged[s] == (int32_t)blk) c->slot_staged[s] = -1;
+    if ((st = start_copy(c, slot, blk, beg, nullptr, packed, packed_bytes)) != GW_OK) return st;
+    c->slot_staged[slot] = (int32_t)blk;
     return GW_OK;
 }
 

@@ -126,6 +126,13 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
                        uint32_t block, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s);
+// decodes chunks [c_begin, c_end) of a packed block (gw_pack.hip: chunk headers, list
+// starts, bit-packed gaps) into its csr words at out; staging holds the block's headers and
+// starts followed by its payload words from word_base on; beg: the block's nv + 1 global
+// offsets on the device
+constexpr uint32_t kPackChunkArcs = 256;
+hipError_t launch_unpack(const uint32_t* staging, const uint64_t* beg, uint64_t start_edge, uint32_t nv, uint64_t ne,
+                         uint64_t c_begin, uint64_t c_end, uint32_t word_base, uint32_t* out, hipStream_t s);
 // v[i] = sum over p < parts of v[p * n + i]; the other arrays are cleared
 hipError_t launch_fold_visits(uint32_t* v, uint64_t n, uint32_t parts, hipStream_t s);
 

@@ -59,6 +59,19 @@ class Blocks:
         return int(np.searchsorted(self.vcut[:-1], v, side="right") - 1)
 
 
+def pack_block(beg, csr):
+    """gw_pack_block: the transfer encoding of one block (beg: nverts + 1 global offsets,
+    csr: the block's arcs), as a u32 array."""
+    beg = np.ascontiguousarray(beg, dtype=np.uint64)
+    csr = np.ascontiguousarray(csr, dtype=np.uint32)
+    nb = C.c_uint64(0)
+    check(lib().gw_pack_block(_p(beg), _p(csr) if len(csr) else None, len(beg) - 1, len(csr), None, 0, C.byref(nb)))
+    out = np.zeros(max(1, nb.value // 4), dtype=np.uint32)
+    check(lib().gw_pack_block(_p(beg), _p(csr) if len(csr) else None, len(beg) - 1, len(csr), _p(out), nb.value,
+                              C.byref(nb)))
+    return out[:nb.value // 4]
+
+
 def walk_params(types, seed=42):
     """types: list of dicts {algo, length, p, q, alpha, term} (1 or 2 entries)."""
     wp = GwWalkParams()
@@ -133,6 +146,25 @@ class Context:
         beg, csr = self.blocks.arrays(blk)
         self._keep.append((beg, csr))
         self._chk(lib().gw_stage_block(self.h, slot, blk, _p(beg), _p(csr)))
+
+    def set_pack_staging(self, nbytes):
+        self._chk(lib().gw_set_pack_staging(self.h, nbytes))
+
+    def upload_packed(self, slot, blk):
+        """Upload blk into slot in the transfer encoding (packed on the host here); the
+        staging buffer must hold it (set_pack_staging)."""
+        beg, csr = self.blocks.arrays(blk)
+        pk = pack_block(beg, csr)
+        self._keep.append((beg, pk))
+        self._chk(lib().gw_upload_block_packed(self.h, slot, blk, _p(beg), _p(pk), 4 * len(pk)))
+        return 4 * len(pk)
+
+    def stage_packed(self, slot, blk):
+        beg, csr = self.blocks.arrays(blk)
+        pk = pack_block(beg, csr)
+        self._keep.append((beg, pk))
+        self._chk(lib().gw_stage_block_packed(self.h, slot, blk, _p(beg), _p(pk), 4 * len(pk)))
+        return 4 * len(pk)
 
     def commit(self, slot):
         self._chk(lib().gw_commit_block(self.h, slot))
@@ -346,9 +378,42 @@ class Model:
             pass
 
 
+class PackedGraph:
+    """gwh_pack_graph: the blocks of (beg, csr) at block_bytes in the transfer encoding, in
+    pinned memory; pass it to run_out_of_core over the same arrays and block size."""
+
+    def __init__(self, beg, csr, block_bytes):
+        from ._abi import host_lib
+        h = C.c_void_p()
+        st = host_lib().gwh_pack_graph(_p(beg), _p(csr), len(beg), block_bytes, C.byref(h))
+        if st != 0:
+            from ._abi import GwError
+            raise GwError(st, host_lib().gwh_last_error().decode())
+        self.h = h
+        self._keep = (beg, csr)
+
+    def info(self):
+        from ._abi import host_lib
+        r, p, n, k = C.c_uint64(), C.c_uint64(), C.c_uint32(), C.c_uint32()
+        check(host_lib().gwh_packed_info(self.h, C.byref(r), C.byref(p), C.byref(n), C.byref(k)))
+        return dict(raw_bytes=r.value, packed_bytes=p.value, nblocks=n.value, blocks_packed=k.value)
+
+    def close(self):
+        if self.h:
+            from ._abi import host_lib
+            host_lib().gwh_packed_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory_bytes=0, block_bytes=256 << 20,
                     zero_threshold=0, scheduler=SCHED_PLAIN, hashes=False, visits=False, verbose=False, launch=None,
-                    flags=0, prefetch_depth=0, model=None):
+                    flags=0, prefetch_depth=0, model=None, packed=None):
     """Runs the C++ out-of-core engine (libgwhost.so) over a host CSR graph; beg/csr should
     be pinned (PinnedArray.a) for the zero-copy tail.  model: a Model carried across walks
     (learned scheduler).  Returns (report dict, hashes, visits)."""
@@ -363,6 +428,7 @@ def run_out_of_core(beg, csr, types, seed, nwalkers, wpv=1, sources=None, memory
     o.flags = flags
     o.prefetch_depth = prefetch_depth
     o.model = model.h if model is not None else None
+    o.packed = packed.h if packed is not None else None
     launch = launch or {}
     o.launch = GwLaunch(launch.get("block", 0), launch.get("waves_per_cu", 0), launch.get("variant", 2),
                         (GW_RUN_INTERP if launch.get("interp") else 0) | (GW_RUN_WINDOW if launch.get("window") else 0))

@@ -32,6 +32,12 @@ inline void ck(gw_status s, gw_ctx* c, const char* what)
     if (s != GW_OK) throw GwError(s, std::string(what) + ": " + gw_last_error(c));
 }
 
+// One block in the transfer encoding (pinned host memory), or data == nullptr: sent raw.
+struct PackedBlock {
+    const void* data = nullptr;
+    uint64_t bytes = 0;
+};
+
 struct EngineConfig {
     int device = 0;
     uint64_t memory_bytes = 0;      // HBM budget for slots + walker state (0: every block fits)
@@ -43,6 +49,8 @@ struct EngineConfig {
     // with a block given: its shape, but the search order from the probe model (the launch
     // tuner's candidates fix the shape only)
     bool auto_search = false;
+    // block uploads in the transfer encoding (one entry per block), or nullptr: raw
+    const std::vector<PackedBlock>* packed = nullptr;
     bool verbose = false;
 };
 
@@ -56,7 +64,7 @@ struct WalkConfig {
 
 struct RunReport {
     uint64_t steps = 0, trials = 0, probes = 0, mix_steps = 0;
-    uint64_t h2d_bytes = 0, uploads = 0, staged = 0, iterations = 0;
+    uint64_t h2d_bytes = 0, h2d_raw_bytes = 0, uploads = 0, staged = 0, iterations = 0;
     uint64_t zero_copy_runs = 0, zero_copy_steps = 0, zc_touches = 0;
     double wall_s = 0, kernel_ms = 0, filter_ms = 0, h2d_ms = 0, sched_ms = 0, tune_ms = 0;
     uint32_t nslots = 0, nblocks = 0;
@@ -173,6 +181,18 @@ public:
         gw_counters c0;
         ck(gw_set_walk(ctx_, &wc.params), ctx_, "gw_set_walk");
         ck(gw_init_walkers(ctx_, wc.nwalkers, wc.wpv, wc.sources, wc.outputs), ctx_, "gw_init_walkers");
+        // the staging buffer of packed uploads (the largest packed block) is counted against
+        // the budget before slots.  Smaller buffers work too (payload in pieces), but each
+        // piece adds a decode between copies on the copy stream: with 16 MB pieces the C3
+        // 8 GB walk took 3.81 s instead of 3.43 s, and at that cap not even 25 MB of staging
+        // left room for the slot a whole block's staging costs.
+        if (cfg_.packed && !pack_staging_) {
+            uint64_t whole = 0;
+            for (uint32_t b = 0; b < nb; ++b)
+                if (const PackedBlock* p = packed(b)) whole = std::max<uint64_t>(whole, p->bytes);
+            if (whole) ck(gw_set_pack_staging(ctx_, whole), ctx_, "gw_set_pack_staging");
+            pack_staging_ = true;
+        }
         const uint32_t want = slots_for_budget();
         if (want == 0) throw GwError(GW_ERR_BUDGET, "the memory budget leaves no room for a single block slot");
         if (want != nslots_) {
@@ -250,8 +270,12 @@ public:
                     if (s == -2) throw GwError(GW_ERR_STATE, "scheduler wants more blocks than slots");
                     if (s < 0) continue;
                     const gw_block_desc& d = blocks_[b];
-                    ck(gw_upload_block(ctx_, (uint32_t)s, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,
-                       "gw_upload_block");
+                    if (const PackedBlock* pk = packed(b))
+                        ck(gw_upload_block_packed(ctx_, (uint32_t)s, b, beg_ + d.start_vert, pk->data, pk->bytes), ctx_,
+                           "gw_upload_block_packed");
+                    else
+                        ck(gw_upload_block(ctx_, (uint32_t)s, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,
+                           "gw_upload_block");
                     bytes += ((uint64_t)d.nverts + 1) * 8 + d.nedges * 4;
                     out.loaded.push_back(b);
                     r.uploads++;
@@ -269,8 +293,12 @@ public:
                     if (s < 0) continue;
                     keep.push_back(b);
                     const gw_block_desc& d = blocks_[b];
-                    ck(gw_stage_block(ctx_, (uint32_t)s, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,
-                       "gw_stage_block");
+                    if (const PackedBlock* pk = packed(b))
+                        ck(gw_stage_block_packed(ctx_, (uint32_t)s, b, beg_ + d.start_vert, pk->data, pk->bytes), ctx_,
+                           "gw_stage_block_packed");
+                    else
+                        ck(gw_stage_block(ctx_, (uint32_t)s, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,
+                           "gw_stage_block");
                     r.staged++;
                 }
                 for (uint32_t b = 0; b < nb; ++b) staged[b] = slots_.blk_stg[b] >= 0 ? 1 : 0;
@@ -323,13 +351,22 @@ public:
         gw_counters c1;
         ck(gw_counters_get(ctx_, &c1), ctx_, "gw_counters_get");
         r.h2d_bytes = c1.h2d_bytes - c0.h2d_bytes;
+        r.h2d_raw_bytes = c1.h2d_raw_bytes - c0.h2d_raw_bytes;
         r.h2d_ms = c1.h2d_ms - c0.h2d_ms;
         r.wall_s = std::chrono::duration<double>(clk::now() - t0).count();
         return r;
     }
 
 private:
+    const PackedBlock* packed(uint32_t b) const
+    {
+        if (!cfg_.packed || b >= cfg_.packed->size()) return nullptr;
+        const PackedBlock& p = (*cfg_.packed)[b];
+        return p.data ? &p : nullptr;
+    }
+
     EngineConfig cfg_;
+    bool pack_staging_ = false;
     const uint64_t* beg_;
     const uint32_t* csr_;
     std::vector<gw_block_desc> blocks_;

@@ -72,8 +72,9 @@ struct LearnedConfig {
     uint32_t h1 = 64, h2 = 64;
     uint64_t seed = 7;
     uint32_t min_samples = 24;      // plain decisions until this many samples
-    uint32_t train_steps = 4;       // Adam steps per training round
-    uint32_t train_every = 8;       // new samples between training rounds
+    uint32_t train_steps = 2;       // Adam steps per training round
+    uint32_t train_every = 32;      // new samples between training rounds (every 8 with 4 steps
+                                    // cost 11 ms of host time per 0.19 s RMAT-24 walk)
     uint32_t batch = 256;
     uint32_t max_samples = 16384;   // ring buffer of trace samples
     uint32_t keep_every = 4;        // iterations between keep samples of resident blocks
@@ -431,12 +432,22 @@ private:
         use_model_ = false;
         pred_.assign(nb, 0.f);
         if (cfg_.use_model) {
-            const bool want = trained() || in.iteration % std::max(1u, cfg_.keep_every) == 0;
-            if (want || buckets.size() < capacity(in)) {
+            // predictions are used by the fill, by network swaps, and by the tail rule once
+            // the work left could be done in zero-copy reads within tail_guard uploads' time;
+            // features are also taken every keep_every-th iteration for samples
+            uint64_t max_bytes = 0;
+            for (uint32_t b = 0; b < nb; ++b) max_bytes = std::max<uint64_t>(max_bytes, block_bytes(in.blocks[b]));
+            const bool tail_near = (double)pending_hops_ * reads_per_step() / zc_reads_per_ms() <
+                                   cfg_.tail_guard * (double)max_bytes / copy_bytes_per_ms();
+            const bool fill = buckets.size() < capacity(in);
+            const bool want = fill || in.iteration % std::max(1u, cfg_.keep_every) == 0 ||
+                              (trained() && (cfg_.model_swap || tail_near));
+            if (want) {
                 feats_.assign((size_t)nb * kF, 0.f);
                 have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
             }
-            use_model_ = trained() && have_feats_ && gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
+            use_model_ = trained() && have_feats_ && (fill || cfg_.model_swap || tail_near) &&
+                         gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
         }
         rank_.resize(nb);
         // without the network: the walkers a load would release (log scale), with the

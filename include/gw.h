@@ -139,6 +139,8 @@ typedef struct {
     double kernel_ms;
     uint64_t hbm_bytes_held; /* bytes counted against cache_bytes */
     uint64_t zc_touches;     /* zero-copy reads of host-mapped blocks */
+    uint64_t h2d_raw_bytes;  /* block uploads in the reference's format, (nverts+1)*8 + nedges*4
+                                each, whether sent raw or packed (h2d_bytes counts what crossed) */
 } gw_counters;
 
 typedef struct {
@@ -187,6 +189,22 @@ gw_status gw_upload_block(gw_ctx* ctx, uint32_t slot, uint32_t blk,
 gw_status gw_stage_block(gw_ctx* ctx, uint32_t slot, uint32_t blk,
                          const uint64_t* beg, const uint32_t* csr);
 gw_status gw_commit_block(gw_ctx* ctx, uint32_t slot);
+/* Packed uploads (no reference counterpart: the reference copies each block's csr as stored,
+ * 4 bytes per arc, gpu/memory.hpp:124-139).  `packed` is the block's adjacency in the
+ * transfer encoding written by gw_pack_block (pinned memory, valid until the copy is done);
+ * beg is sent as is.  The copy stream moves the packed bytes into a staging buffer and a
+ * decode kernel writes the block's csr into the slot, so the slot holds exactly what
+ * gw_upload_block / gw_stage_block would put there; h2d_bytes counts the bytes that crossed,
+ * h2d_raw_bytes the block's size in the reference's format.  gw_set_pack_staging sizes
+ * the staging buffer (counted against cache_bytes; 0 frees it): it holds a block's chunk
+ * headers and list starts (16 B per 256 arcs + 4 B per vertex) whole and its payload in
+ * pieces of the rest, so any size from those plus 1 KiB up works; larger buffers mean fewer,
+ * longer copies. */
+gw_status gw_set_pack_staging(gw_ctx* ctx, uint64_t bytes);
+gw_status gw_upload_block_packed(gw_ctx* ctx, uint32_t slot, uint32_t blk, const uint64_t* beg,
+                                 const void* packed, uint64_t packed_bytes);
+gw_status gw_stage_block_packed(gw_ctx* ctx, uint32_t slot, uint32_t blk, const uint64_t* beg,
+                                const void* packed, uint64_t packed_bytes);
 gw_status gw_staged_block(gw_ctx* ctx, uint32_t slot, int32_t* blk, uint32_t* ready);
 /* Marks a slot free and its block non-resident: replaces the INACTIVE update of
  * GOwalker_scheduler_t::copyblocks (engine/schedule.hpp:181-183). */
@@ -317,6 +335,17 @@ gw_status gw_graph_read(gw_ctx* ctx, const gw_graph* g, uint64_t* beg, uint32_t*
 gw_status gw_graph_free(gw_ctx* ctx, gw_graph* g);
 
 /* ---- host-side block format helpers (no GPU needed) ---- */
+/* The transfer encoding of one block's adjacency (host only, no GPU): beg = the block's
+ * nverts + 1 global offsets, csr = its nedges arcs (a truncated giant vertex's list clipped at
+ * nedges).  Layout, in u32 words: ceil(nedges / 256) chunk headers {payload word offset, bit
+ * width w (0..32), carry (the arc before the chunk when the chunk starts inside a list, else
+ * 0), block-local vertex holding the chunk's first arc}; nverts list starts (each vertex's
+ * first neighbour, 0 when empty); then per chunk 8w payload words, arc j of the chunk in bits
+ * [jw, jw + w): its difference (mod 2^32) to the arc before it in the same list, 0 for a
+ * list's first arc.  With out == NULL only *bytes is set. */
+gw_status gw_pack_block(const uint64_t* beg, const uint32_t* csr, uint32_t nverts, uint64_t nedges,
+                        void* out, uint64_t cap_bytes, uint64_t* bytes);
+
 /* split_blocks (preprocess/split.hpp:15-87) over an in-memory .beg array. */
 gw_status gw_split_blocks(const uint64_t* beg, uint64_t entries, uint64_t block_size,
                           uint32_t* vblocks, uint64_t* eblocks, uint64_t cap, uint64_t* ncuts);

@@ -53,6 +53,19 @@ typedef struct {
 /* The launch tuner's table for the workload of the model's last walk: up to cap entries
  * into out (may be NULL), their number in *n, the fastest timed one in *chosen. */
 gw_status gwh_model_tuning(const gwh_model* m, gwh_tune_entry* out, uint32_t cap, uint32_t* n, uint32_t* chosen);
+/* A graph's blocks in the transfer encoding (gw_pack_block), built once on the host
+ * cores into pinned memory and reused by every gwh_run over the same beg / csr arrays and
+ * block size (no reference counterpart).  Blocks whose encoding is not smaller than their
+ * csr stay raw.  With gwh_engine_opts.packed set, block uploads send the packed bytes and
+ * the GPU decodes them into the slots: walks are unchanged, fewer bytes cross PCIe. */
+typedef struct gwh_packed gwh_packed;
+gw_status gwh_pack_graph(const uint64_t* beg, const uint32_t* csr, uint64_t entries, uint64_t block_bytes,
+                         gwh_packed** out);
+/* raw_bytes: the blocks' csr bytes; packed_bytes: what their uploads send instead */
+gw_status gwh_packed_info(const gwh_packed* p, uint64_t* raw_bytes, uint64_t* packed_bytes, uint32_t* nblocks,
+                          uint32_t* blocks_packed);
+gw_status gwh_packed_destroy(gwh_packed* p);
+
 /* gwh_engine_opts.flags */
 enum {
     GWH_NO_SWAP_PREFETCH = 1u << 0, /* learned: once the cache is full, do not keep a slot for
@@ -88,6 +101,7 @@ typedef struct {
     uint32_t prefetch_depth;   /* learned scheduler: staged block copies in flight (0 = 8) */
     uint32_t reserved;
     gwh_model* model;          /* learned scheduler: state carried across walks (may be NULL) */
+    const gwh_packed* packed;  /* block uploads in the transfer encoding (may be NULL: raw) */
 } gwh_engine_opts;
 
 typedef struct {
@@ -113,6 +127,8 @@ typedef struct {
     uint64_t sticky_swaps;     /* swaps that took the already staged near-best block */
     uint32_t tune_candidate;   /* the launch tuner's candidate this walk ran (~0u: no tuning) */
     uint32_t pad2;
+    uint64_t h2d_raw_bytes;    /* the uploads' size in the reference's block format (h2d_bytes:
+                                  what crossed PCIe, smaller with packed uploads) */
 } gwh_report;
 
 /* Runs walkers to completion out of core: beg has `entries` u64 offsets (ids
