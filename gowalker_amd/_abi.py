@@ -111,8 +111,9 @@ SIGNATURES = {
     "gw_mlp_backward": [P, P, P, P, C.c_uint32, P, P],
     "gw_mlp_step": [P, P, P, P, C.c_uint32, C.c_float, P],
     "gw_set_pack_staging": [P, C.c_uint64],
-    "gw_upload_block_packed": [P, C.c_uint32, C.c_uint32, P, P, C.c_uint64],
-    "gw_stage_block_packed": [P, C.c_uint32, C.c_uint32, P, P, C.c_uint64],
+    "gw_read_slot": [P, C.c_uint32, P, P],
+    "gw_upload_block_packed": [P, C.c_uint32, C.c_uint32, P, C.c_uint64],
+    "gw_stage_block_packed": [P, C.c_uint32, C.c_uint32, P, C.c_uint64],
     "gw_pack_block": [P, P, C.c_uint32, C.c_uint64, P, C.c_uint64, u64p],
     "gw_walk_resident_noalloc": [P, P, C.c_uint32, C.c_uint64, C.POINTER(GwWalkParams), C.c_uint64, C.c_uint32, P, P,
                                  P, C.POINTER(GwRunStats)],

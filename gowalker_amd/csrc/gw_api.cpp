@@ -44,6 +44,14 @@ struct gw_ctx {
     // staging buffer of packed block copies (decoded into the slots on the copy stream)
     uint32_t* d_pack = nullptr;
     uint64_t pack_cap = 0;
+    // per block: the packed buffer (address, size, fingerprint) whose headers were last
+    // checked in full
+    struct PackSeen {
+        const void* p;
+        uint64_t bytes, fp;
+        bool operator==(const PackSeen& o) const { return p == o.p && bytes == o.bytes && fp == o.fp; }
+    };
+    std::vector<PackSeen> pack_checked;
 
     // slots
     uint32_t nslots = 0;
@@ -399,6 +407,7 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     c->trace_valid = false;
     c->nblocks = nb;
     c->desc.assign(b, b + nb);
+    c->pack_checked.assign(nb, gw_ctx::PackSeen{nullptr, 0, 0});
     c->hblk.assign(nb, DevBlock{});
     c->blk_slot.assign(nb, -1);
     c->map_beg.assign(nb, nullptr);
@@ -531,17 +540,26 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
                             const void* packed = nullptr, uint64_t packed_bytes = 0)
 {
     const gw_block_desc& d = c->desc[blk];
-    if (!beg || (d.nedges && !csr && !packed)) return fail(c, GW_ERR_ARG, "null block buffer");
-    // a packed block: its chunk headers and list starts go to the staging buffer whole, its
-    // payload in pieces of what is left of the buffer (at least the largest chunk, 1 KiB)
-    const uint64_t nch = packed ? (d.nedges + kPackChunkArcs - 1) / kPackChunkArcs : 0;
-    const uint64_t head_w = nch * 4 + d.nverts;
-    if (packed && (packed_bytes & 3)) return fail(c, GW_ERR_ARG, "packed block size must be a multiple of 4");
-    if (packed && packed_bytes < head_w * 4) return fail(c, GW_ERR_ARG, "packed block shorter than its headers");
-    if (packed && head_w * 4 + 32 * 32 > c->pack_cap)
-        return fail(c, GW_ERR_ARG, "the %llu B staging buffer cannot hold the block's %llu B of headers and list starts "
-                    "plus one chunk (gw_set_pack_staging)", (unsigned long long)c->pack_cap,
-                    (unsigned long long)(head_w * 4));
+    if (packed ? !d.nedges : (!beg || (d.nedges && !csr)))
+        return fail(c, GW_ERR_ARG, packed ? "a block without arcs is uploaded raw" : "null block buffer");
+    // a packed block: its headers and beg payload go to the staging buffer whole, its arc
+    // payload in pieces of what is left of the buffer (at least the largest chunk, 1088 B)
+    PackLayout lay{};
+    if (packed) {
+        if (packed_bytes & 3) return fail(c, GW_ERR_ARG, "packed block size must be a multiple of 4");
+        // every header is checked the first time a block is sent from a buffer; later uploads
+        // of the same buffer read only its first words (the decoders keep every read inside
+        // the staging buffer whatever the headers say)
+        const gw_ctx::PackSeen seen{packed, packed_bytes, pack_fingerprint((const uint32_t*)packed, packed_bytes / 4)};
+        const bool checked = c->pack_checked[blk] == seen;
+        if (const char* e = pack_layout((const uint32_t*)packed, packed_bytes / 4, d.nverts, d.nedges, !checked, &lay))
+            return fail(c, GW_ERR_ARG, "block %u: %s", blk, e);
+        c->pack_checked[blk] = seen;
+        if (lay.head_w * 4 + 4 * (8 * 31 + 24) > c->pack_cap)
+            return fail(c, GW_ERR_ARG, "the %llu B staging buffer cannot hold the block's %llu B of headers and offsets "
+                        "plus one chunk (gw_set_pack_staging)", (unsigned long long)c->pack_cap,
+                        (unsigned long long)(lay.head_w * 4));
+    }
     if (c->fault_kind == 1 && ++c->n_upload == c->fault_at)
         return fail(c, GW_ERR_FAULT, "injected upload failure (GW_FAULT=upload:%ld)", c->fault_at);
     (void)hipSetDevice(c->device);
@@ -555,35 +573,45 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
     }
     const uint64_t bb = ((uint64_t)d.nverts + 1) * 8, cb = d.nedges * 4;
     HIPCHK(c, hipEventRecord(c->slot_ev0[slot], c->s_copy));
-    HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, c->s_copy));
-    if (cb && packed) {
+    if (packed) {
         // the copies and decodes share the copy stream, so each copy into the staging buffer
         // queues behind the decode that read the previous piece
         const uint32_t* pw = (const uint32_t*)packed;
-        const uint64_t piece_w = (c->pack_cap / 4) - head_w;
-        HIPCHK(c, hipMemcpyAsync(c->d_pack, pw, head_w * 4, hipMemcpyHostToDevice, c->s_copy));
-        const uint64_t total_w = packed_bytes / 4 - head_w;
+        const uint32_t* ah = pw + lay.ah_off;
+        const uint64_t piece_w = (c->pack_cap / 4) - lay.head_w;
+        const uint64_t total_w = packed_bytes / 4 - lay.head_w;
+        HIPCHK(c, hipMemcpyAsync(c->d_pack, pw, lay.head_w * 4, hipMemcpyHostToDevice, c->s_copy));
+        HIPCHK(c, launch_unpack_beg(c->d_pack, lay, d.nverts, c->slot_beg[slot], c->s_copy));
+        // pieces: the most chunks whose payload fits (chunk offsets are contiguous, so chunks
+        // [c0, c1) take words [off(c0), off(c1)), off(nch) = total_w; a binary search)
+        auto off = [&](uint64_t k) { return k < lay.nch ? (uint64_t)ah[k * 4] : total_w; };
         uint64_t c0 = 0;
-        while (c0 < nch) {
-            const uint32_t w0 = pw[c0 * 4];
-            uint64_t c1 = c0;
-            while (c1 < nch && (uint64_t)pw[c1 * 4] + 8ull * pw[c1 * 4 + 1] - w0 <= piece_w) ++c1;
+        while (c0 < lay.nch) {
+            const uint64_t w0 = off(c0);
+            uint64_t lo = c0, hi = lay.nch;   // off(lo) - w0 <= piece_w; find the largest such
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi + 1) / 2;
+                if (off(mid) - w0 <= piece_w) lo = mid; else hi = mid - 1;
+            }
+            const uint64_t c1 = lo;
             if (c1 == c0) return fail(c, GW_ERR_ARG, "a chunk of block %u does not fit the staging buffer", blk);
-            const uint64_t w1 = c1 < nch ? pw[c1 * 4] : total_w;
+            const uint64_t w1 = off(c1);
+            if (w1 > total_w) return fail(c, GW_ERR_ARG, "block %u: packed chunk offsets past its end", blk);
             if (w1 > w0)
-                HIPCHK(c, hipMemcpyAsync(c->d_pack + head_w, pw + head_w + w0, (w1 - w0) * 4, hipMemcpyHostToDevice,
-                                         c->s_copy));
-            HIPCHK(c, launch_unpack(c->d_pack, c->slot_beg[slot], d.start_edge, d.nverts, d.nedges, c0, c1, w0,
-                                    slot_edges(c, slot, blk), c->s_copy));
+                HIPCHK(c, hipMemcpyAsync(c->d_pack + lay.head_w, pw + lay.head_w + w0, (w1 - w0) * 4,
+                                         hipMemcpyHostToDevice, c->s_copy));
+            HIPCHK(c, launch_unpack_arcs(c->d_pack, lay, c->slot_beg[slot], d.start_edge, d.nverts, d.nedges, c0, c1,
+                                         w0, w1 - w0, slot_edges(c, slot, blk), c->s_copy));
             c0 = c1;
         }
-    } else if (cb) {
-        HIPCHK(c, hipMemcpyAsync(slot_edges(c, slot, blk), csr, cb, hipMemcpyHostToDevice, c->s_copy));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, c->s_copy));
+        if (cb) HIPCHK(c, hipMemcpyAsync(slot_edges(c, slot, blk), csr, cb, hipMemcpyHostToDevice, c->s_copy));
     }
     HIPCHK(c, hipEventRecord(c->slot_ev1[slot], c->s_copy));
     HIPCHK(c, hipEventRecord(c->ev_copy, c->s_copy));
     c->slot_timed[slot] = 1;
-    c->tot.h2d_bytes += bb + (cb && packed ? packed_bytes : cb);
+    c->tot.h2d_bytes += packed ? packed_bytes : bb + cb;
     c->tot.h2d_raw_bytes += bb + cb;
     c->tot.h2d_uploads += 1;
     return GW_OK;
@@ -640,8 +668,7 @@ gw_status gw_upload_block(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t
     return GW_OK;
 }
 
-gw_status gw_upload_block_packed(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const void* packed,
-                                 uint64_t packed_bytes)
+gw_status gw_upload_block_packed(gw_ctx* c, uint32_t slot, uint32_t blk, const void* packed, uint64_t packed_bytes)
 {
     if (!c) return fail(c, GW_ERR_ARG, "null context");
     if (!packed) return fail(c, GW_ERR_ARG, "null packed block");
@@ -651,13 +678,12 @@ gw_status gw_upload_block_packed(gw_ctx* c, uint32_t slot, uint32_t blk, const u
         c->slot_blk[c->blk_slot[blk]] = -1;
         block_nonresident(c, blk);
     }
-    if ((st = start_copy(c, slot, blk, beg, nullptr, packed, packed_bytes)) != GW_OK) return st;
+    if ((st = start_copy(c, slot, blk, nullptr, nullptr, packed, packed_bytes)) != GW_OK) return st;
     make_resident(c, slot, blk);
     return GW_OK;
 }
 
-gw_status gw_stage_block_packed(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const void* packed,
-                                uint64_t packed_bytes)
+gw_status gw_stage_block_packed(gw_ctx* c, uint32_t slot, uint32_t blk, const void* packed, uint64_t packed_bytes)
 {
     if (!c) return fail(c, GW_ERR_ARG, "null context");
     if (!packed) return fail(c, GW_ERR_ARG, "null packed block");
@@ -666,7 +692,7 @@ gw_status gw_stage_block_packed(gw_ctx* c, uint32_t slot, uint32_t blk, const ui
     if (c->blk_slot[blk] >= 0) return fail(c, GW_ERR_ARG, "block %u is already resident (slot %d)", blk, c->blk_slot[blk]);
     for (uint32_t s = 0; s < c->nslots; ++s)
         if (s != slot && c->slot_staged[s] == (int32_t)blk) c->slot_staged[s] = -1;
-    if ((st = start_copy(c, slot, blk, beg, nullptr, packed, packed_bytes)) != GW_OK) return st;
+    if ((st = start_copy(c, slot, blk, nullptr, nullptr, packed, packed_bytes)) != GW_OK) return st;
     c->slot_staged[slot] = (int32_t)blk;
     return GW_OK;
 }
@@ -706,6 +732,22 @@ gw_status gw_staged_block(gw_ctx* c, uint32_t slot, int32_t* blk, uint32_t* read
         if (e != hipSuccess && e != hipErrorNotReady) return fail(c, GW_ERR_HIP, "copy event: %s", hipGetErrorString(e));
         *ready = e == hipSuccess ? 1u : 0u;
     }
+    return GW_OK;
+}
+
+gw_status gw_read_slot(gw_ctx* c, uint32_t slot, uint64_t* beg, uint32_t* csr)
+{
+    if (!c || !beg) return fail(c, GW_ERR_ARG, "null argument");
+    if (slot >= c->nslots) return fail(c, GW_ERR_ARG, "slot %u out of range (%u slots)", slot, c->nslots);
+    const int32_t blk = c->slot_blk[slot] >= 0 ? c->slot_blk[slot] : c->slot_staged[slot];
+    if (blk < 0) return fail(c, GW_ERR_ARG, "slot %u holds no block", slot);
+    const gw_block_desc& d = c->desc[blk];
+    if (d.nedges && !csr) return fail(c, GW_ERR_ARG, "null argument");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->s_copy));
+    HIPCHK(c, hipMemcpy(beg, c->slot_beg[slot], ((uint64_t)d.nverts + 1) * 8, hipMemcpyDeviceToHost));
+    if (d.nedges)
+        HIPCHK(c, hipMemcpy(csr, slot_edges(c, slot, (uint32_t)blk), d.nedges * 4, hipMemcpyDeviceToHost));
     return GW_OK;
 }
 

@@ -126,13 +126,45 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
                        uint32_t block, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s);
-// decodes chunks [c_begin, c_end) of a packed block (gw_pack.hip: chunk headers, list
-// starts, bit-packed gaps) into its csr words at out; staging holds the block's headers and
-// starts followed by its payload words from word_base on; beg: the block's nv + 1 global
-// offsets on the device
+// Packed blocks (gw_pack.hip: a header, beg chunks of bit-packed differences, arc chunks of
+// Elias-Fano-coded gap sums).  Word offsets of the parts, from the block's first word.
 constexpr uint32_t kPackChunkArcs = 256;
-hipError_t launch_unpack(const uint32_t* staging, const uint64_t* beg, uint64_t start_edge, uint32_t nv, uint64_t ne,
-                         uint64_t c_begin, uint64_t c_end, uint32_t word_base, uint32_t* out, hipStream_t s);
+constexpr uint32_t kPackMagic = 0x32505747u;
+struct PackLayout {
+    uint64_t nvc, vpw, nch;   // beg chunks, beg payload words, arc chunks
+    uint64_t vh_off, vp_off, ah_off, head_w;   // beg headers, beg payload, arc headers; words before the arc payload
+};
+// payload words of an arc chunk of n arcs with header h
+inline uint64_t pack_chunk_words(const uint32_t* h, uint64_t n)
+{
+    return 8ull * (h[1] & 0xFFu) + ((n + (h[1] >> 8) + 31) >> 5);
+}
+// Fills lay from a packed block of `words` words of the block of nv vertices and ne arcs and
+// checks its header; with `full`, every beg and arc chunk header too (offsets contiguous,
+// widths and sizes in range, first vertices in the block and non-decreasing).  Returns an
+// error message or nullptr.  Host code over host memory.
+const char* pack_layout(const uint32_t* pk, uint64_t words, uint32_t nv, uint64_t ne, bool full, PackLayout* lay);
+// FNV-1a over a packed block's size and a few of its words (the first 16, 16 at the middle,
+// the last 16): tells blocks apart that reuse an address without reading them whole
+inline uint64_t pack_fingerprint(const uint32_t* pk, uint64_t words)
+{
+    uint64_t h = 1469598103934665603ULL ^ words;
+    auto mix = [&](uint64_t i0) {
+        for (uint64_t i = i0; i < i0 + 16 && i < words; ++i) h = (h ^ pk[i]) * 1099511628211ULL;
+    };
+    mix(0);
+    mix(words / 2);
+    mix(words > 16 ? words - 16 : 0);
+    return h;
+}
+// beg (nv + 1 global offsets) from a packed block whose first lay.head_w words are at staging
+hipError_t launch_unpack_beg(const uint32_t* staging, const PackLayout& lay, uint32_t nv, uint64_t* beg, hipStream_t s);
+// csr words of arc chunks [c_begin, c_end) into out; staging holds the block's first
+// lay.head_w words followed by `avail` of its arc payload words from word_base on; beg: the
+// decoded offsets on the device
+hipError_t launch_unpack_arcs(const uint32_t* staging, const PackLayout& lay, const uint64_t* beg, uint64_t start_edge,
+                              uint32_t nv, uint64_t ne, uint64_t c_begin, uint64_t c_end, uint64_t word_base,
+                              uint64_t avail, uint32_t* out, hipStream_t s);
 // v[i] = sum over p < parts of v[p * n + i]; the other arrays are cleared
 hipError_t launch_fold_visits(uint32_t* v, uint64_t n, uint32_t parts, hipStream_t s);
 

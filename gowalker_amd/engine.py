@@ -61,7 +61,7 @@ class Blocks:
 
 def pack_block(beg, csr):
     """gw_pack_block: the transfer encoding of one block (beg: nverts + 1 global offsets,
-    csr: the block's arcs), as a u32 array."""
+    csr: the block's arcs), as a u32 array (empty for a block without arcs)."""
     beg = np.ascontiguousarray(beg, dtype=np.uint64)
     csr = np.ascontiguousarray(csr, dtype=np.uint32)
     nb = C.c_uint64(0)
@@ -155,16 +155,24 @@ class Context:
         staging buffer must hold it (set_pack_staging)."""
         beg, csr = self.blocks.arrays(blk)
         pk = pack_block(beg, csr)
-        self._keep.append((beg, pk))
-        self._chk(lib().gw_upload_block_packed(self.h, slot, blk, _p(beg), _p(pk), 4 * len(pk)))
+        self._keep.append(pk)
+        self._chk(lib().gw_upload_block_packed(self.h, slot, blk, _p(pk), 4 * len(pk)))
         return 4 * len(pk)
 
     def stage_packed(self, slot, blk):
         beg, csr = self.blocks.arrays(blk)
         pk = pack_block(beg, csr)
-        self._keep.append((beg, pk))
-        self._chk(lib().gw_stage_block_packed(self.h, slot, blk, _p(beg), _p(pk), 4 * len(pk)))
+        self._keep.append(pk)
+        self._chk(lib().gw_stage_block_packed(self.h, slot, blk, _p(pk), 4 * len(pk)))
         return 4 * len(pk)
+
+    def read_slot(self, slot, blk):
+        """(beg, csr) as slot holds them for blk (resident or staged), after its copy."""
+        d = self.blocks.desc[blk]
+        beg = np.zeros(d.nverts + 1, dtype=np.uint64)
+        csr = np.zeros(max(1, d.nedges), dtype=np.uint32)
+        self._chk(lib().gw_read_slot(self.h, slot, _p(beg), _p(csr)))
+        return beg, csr[:d.nedges]
 
     def commit(self, slot):
         self._chk(lib().gw_commit_block(self.h, slot))

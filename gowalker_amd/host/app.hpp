@@ -1,7 +1,7 @@
 // app.hpp -- the Node2vec / SOPR / SOSR entry points (apps/Node2vec.cu, apps/SOPR.cu,
 // apps/SOSR.cu) over libgw.so.  Same option names and defaults as the reference
-// (apps/Node2vec.cu:35-59); additions: seed, term, sched (plain|learned), source
-// (reference|pervertex, SOPR/SOSR), visits <file>, hashes <file>, verbose.
+// (apps/Node2vec.cu:35-59); additions: seed, term, sched (plain|learned|staging), source
+// (reference|pervertex, SOPR/SOSR), pack, visits <file>, hashes <file>, verbose.
 #pragma once
 #include <omp.h>
 
@@ -17,6 +17,7 @@
 #include "format.hpp"
 #include "learned.hpp"
 #include "metrics.hpp"
+#include "pack.hpp"
 #include "schedule.hpp"
 
 namespace gwh {
@@ -93,7 +94,8 @@ inline void print_help()
         "- term:          geometric termination probability per step (default 0)\n"
         "- zero:          finish with zero-copy reads below blocksize/128/24 walkers\n"
         "- GOWalker | SOWalker: block scheduler, walker-count rule or simulated annealing (one required)\n"
-        "- sched:         plain | learned (default plain)\n"
+        "- sched:         plain | learned | staging (the learned scheduler without its network; default plain)\n"
+        "- pack:          upload blocks as bit-packed gaps, decoded on the GPU (fewer PCIe bytes, same walks)\n"
         "- seed:          Philox seed (default 42)\n"
         "- threadperblock: workgroup size of the step kernel (default: the engine's)\n"
         "- source:        SOPR/SOSR sources: reference (vertex 1, or 1 and 3) | pervertex\n"
@@ -167,6 +169,17 @@ inline int app_main(int argc, const char** argv, App app)
         ec.zero_threshold = zero ? blocksize / 128 / 24 : 0;   // apps/Node2vec.cu:100-108
         if (tpb) ec.launch.block = std::max<uint32_t>(64, std::min<uint32_t>(1024, tpb / 64 * 64));
         ec.verbose = verbose;
+        PackedGraph packed;
+        double pack_s = 0;
+        if (o.get_bool("pack")) {
+            const double t0 = omp_get_wtime();
+            packed.build(g.beg, g.csr, blocks);
+            pack_s = omp_get_wtime() - t0;
+            ec.packed = &packed.blocks;
+            if (verbose)
+                std::printf("packed %u of %zu blocks: %llu of %llu csr bytes in %.2f s\n", packed.npacked, blocks.size(),
+                            (unsigned long long)packed.bytes, (unsigned long long)packed.raw, pack_s);
+        }
         Engine engine(ec, g.beg, g.csr, blocks);
 
         WalkConfig wc;
@@ -215,6 +228,11 @@ inline int app_main(int argc, const char** argv, App app)
         std::unique_ptr<Scheduler> sched;
         if (so) sched.reset(new SOWalkerScheduler());
         else if (sched_name == "learned") sched.reset(new LearnedScheduler(engine.ctx(), LearnedConfig()));
+        else if (sched_name == "staging") {
+            LearnedConfig lc;
+            lc.use_model = false;
+            sched.reset(new LearnedScheduler(engine.ctx(), lc));
+        }
         else sched.reset(new PlainScheduler());
 
         RunReport r = engine.run(wc, *sched);
@@ -232,6 +250,8 @@ inline int app_main(int argc, const char** argv, App app)
         m.set("walk_steps_per_s", r.wall_s > 0 ? r.steps / r.wall_s : 0.0);
         m.set("h2d_bytes", (double)r.h2d_bytes, true);
         m.set("h2d_gb", r.h2d_bytes / 1e9);
+        m.set("h2d_raw_bytes", (double)r.h2d_raw_bytes, true);
+        if (o.get_bool("pack")) m.set("pack_time", pack_s);
         m.set("iterations", (double)r.iterations, true);
         m.set("zero_copy_steps", (double)r.zero_copy_steps, true);
         m.set_string("scheduler", sched->name());

@@ -271,7 +271,7 @@ public:
                     if (s < 0) continue;
                     const gw_block_desc& d = blocks_[b];
                     if (const PackedBlock* pk = packed(b))
-                        ck(gw_upload_block_packed(ctx_, (uint32_t)s, b, beg_ + d.start_vert, pk->data, pk->bytes), ctx_,
+                        ck(gw_upload_block_packed(ctx_, (uint32_t)s, b, pk->data, pk->bytes), ctx_,
                            "gw_upload_block_packed");
                     else
                         ck(gw_upload_block(ctx_, (uint32_t)s, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,
@@ -294,7 +294,7 @@ public:
                     keep.push_back(b);
                     const gw_block_desc& d = blocks_[b];
                     if (const PackedBlock* pk = packed(b))
-                        ck(gw_stage_block_packed(ctx_, (uint32_t)s, b, beg_ + d.start_vert, pk->data, pk->bytes), ctx_,
+                        ck(gw_stage_block_packed(ctx_, (uint32_t)s, b, pk->data, pk->bytes), ctx_,
                            "gw_stage_block_packed");
                     else
                         ck(gw_stage_block(ctx_, (uint32_t)s, b, beg_ + d.start_vert, csr_ + d.start_edge), ctx_,

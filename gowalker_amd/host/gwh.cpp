@@ -8,6 +8,7 @@
 #include "engine.hpp"
 #include "format.hpp"
 #include "learned.hpp"
+#include "pack.hpp"
 #include "schedule.hpp"
 #include "tuner.hpp"
 
@@ -21,20 +22,14 @@ struct gwh_model {
 struct gwh_packed {
     const uint64_t* beg = nullptr;
     const uint32_t* csr = nullptr;
-    uint64_t entries = 0, block_bytes = 0, raw = 0, bytes = 0;
-    uint32_t npacked = 0;
-    std::vector<gwh::PackedBlock> blocks;
-    std::vector<void*> pinned;
-    ~gwh_packed()
-    {
-        for (void* p : pinned) gw_free_pinned(p);
-    }
+    uint64_t entries = 0, block_bytes = 0;
+    gwh::PackedGraph g;
 };
 
 extern "C" const char* gwh_last_error(void) { return g_err.c_str(); }
 
 // Splits the graph as gwh_run does and packs every block whose encoding is smaller than its
-// csr: sizes in parallel over blocks, pinned buffers one by one, then packing in parallel.
+// csr.
 extern "C" gw_status gwh_pack_graph(const uint64_t* beg, const uint32_t* csr, uint64_t entries, uint64_t block_bytes,
                                     gwh_packed** out)
 {
@@ -52,42 +47,7 @@ extern "C" gw_status gwh_pack_graph(const uint64_t* beg, const uint32_t* csr, ui
         ck(gw_split_blocks(beg, entries, block_bytes, vb.data(), eb.data(), vb.size(), &n), nullptr, "gw_split_blocks");
         vb.resize(n);
         eb.resize(n);
-        const std::vector<gw_block_desc> blocks = make_block_table(vb, eb, block_bytes);
-        const int64_t nb = (int64_t)blocks.size();
-        std::vector<uint64_t> sz(nb, 0);
-        std::vector<gw_status> st(nb, GW_OK);
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int64_t b = 0; b < nb; ++b) {
-            const gw_block_desc& d = blocks[b];
-            if (d.nedges) st[b] = gw_pack_block(beg + d.start_vert, csr + d.start_edge, d.nverts, d.nedges, nullptr, 0,
-                                                &sz[b]);
-        }
-        for (int64_t b = 0; b < nb; ++b) ck(st[b], nullptr, "gw_pack_block (size)");
-        pk->blocks.assign(nb, PackedBlock{});
-        for (int64_t b = 0; b < nb; ++b) {
-            const uint64_t raw = blocks[b].nedges * 4;
-            pk->raw += raw;
-            if (!blocks[b].nedges || sz[b] >= raw) {
-                pk->bytes += raw;
-                continue;
-            }
-            void* p = nullptr;
-            ck(gw_alloc_pinned(sz[b], &p), nullptr, "gw_alloc_pinned");
-            pk->pinned.push_back(p);
-            pk->blocks[b] = PackedBlock{p, sz[b]};
-            pk->bytes += sz[b];
-            pk->npacked++;
-        }
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int64_t b = 0; b < nb; ++b) {
-            const gw_block_desc& d = blocks[b];
-            PackedBlock& q = pk->blocks[b];
-            if (!q.data) continue;
-            uint64_t got = 0;
-            st[b] = gw_pack_block(beg + d.start_vert, csr + d.start_edge, d.nverts, d.nedges, const_cast<void*>(q.data),
-                                  q.bytes, &got);
-        }
-        for (int64_t b = 0; b < nb; ++b) ck(st[b], nullptr, "gw_pack_block");
+        pk->g.build(beg, csr, make_block_table(vb, eb, block_bytes));
     } catch (const GwError& e) {
         g_err = e.what();
         return e.status;
@@ -110,10 +70,10 @@ extern "C" gw_status gwh_packed_info(const gwh_packed* p, uint64_t* raw_bytes, u
         g_err = "gwh_packed_info: null argument";
         return GW_ERR_ARG;
     }
-    if (raw_bytes) *raw_bytes = p->raw;
-    if (packed_bytes) *packed_bytes = p->bytes;
-    if (nblocks) *nblocks = (uint32_t)p->blocks.size();
-    if (blocks_packed) *blocks_packed = p->npacked;
+    if (raw_bytes) *raw_bytes = p->g.raw;
+    if (packed_bytes) *packed_bytes = p->g.bytes;
+    if (nblocks) *nblocks = (uint32_t)p->g.blocks.size();
+    if (blocks_packed) *blocks_packed = p->g.npacked;
     return GW_OK;
 }
 
@@ -209,7 +169,7 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
                 g_err = "gwh_run: the packed graph was built from other arrays or another block size";
                 return GW_ERR_ARG;
             }
-            ec.packed = &pk->blocks;
+            ec.packed = &pk->g.blocks;
         }
         // launch tuning across walks: with a model and no launch given, this walk runs the
         // tuner's next candidate shape

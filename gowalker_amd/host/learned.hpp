@@ -65,6 +65,14 @@ struct ModelState {
     uint64_t scored = 0;
     double net_abs_err = 0, mean_abs_err = 0, label_sum = 0;
     uint64_t labels = 0;
+    // the network is trained on standardized labels (y - y_mu) / y_sigma, the mean and
+    // deviation of the sample ring at the last training round (y_sigma 0: none yet), and
+    // predicts y_mu + y_sigma x its output: an untrained network predicts the mean label,
+    // and Adam need not first walk the output bias to the label scale (log1p of millions
+    // of steps is ~15; at a 2e-3 rate that alone takes thousands of steps); recomputing
+    // both per round follows the labels as the ring turns over (the first samples of a
+    // workload can be unlike the later ones)
+    float y_mu = 0.f, y_sigma = 0.f;
 };
 
 struct LearnedConfig {
@@ -75,6 +83,7 @@ struct LearnedConfig {
     uint32_t train_steps = 2;       // Adam steps per training round
     uint32_t train_every = 32;      // new samples between training rounds (every 8 with 4 steps
                                     // cost 11 ms of host time per 0.19 s RMAT-24 walk)
+    uint32_t finish_steps = 16;     // Adam steps when a walk ends (~70 us each)
     uint32_t batch = 256;
     uint32_t max_samples = 16384;   // ring buffer of trace samples
     uint32_t keep_every = 4;        // iterations between keep samples of resident blocks
@@ -113,11 +122,20 @@ public:
     {
         if (!cfg_.use_model) return;
         if (gw_mlp_create(ctx_, kF, cfg_.h1, cfg_.h2, cfg_.seed, &mlp_) != GW_OK) mlp_ = nullptr;
-        if (state_) {
-            m_ = *state_;
-            uint32_t np = 0;
-            if (mlp_ && !m_.params.empty() && gw_mlp_num_params(mlp_, &np) == GW_OK && np == m_.params.size())
-                (void)gw_mlp_set_params(ctx_, mlp_, m_.params.data());
+        if (state_) m_ = *state_;
+        uint32_t np = 0;
+        if (!mlp_ || gw_mlp_num_params(mlp_, &np) != GW_OK) return;
+        if (!m_.params.empty() && np == m_.params.size()) {
+            (void)gw_mlp_set_params(ctx_, mlp_, m_.params.data());
+        } else {
+            // a new network: its output layer (w3, b3: the last h2 + 1 parameters) starts at
+            // zero, so before training it predicts the mean label exactly instead of the mean
+            // plus the noise of a random output layer; the gradients of w3 start the rest
+            std::vector<float> p(np);
+            if (gw_mlp_get_params(ctx_, mlp_, p.data()) == GW_OK && np > cfg_.h2) {
+                std::fill(p.end() - (cfg_.h2 + 1), p.end(), 0.f);
+                (void)gw_mlp_set_params(ctx_, mlp_, p.data());
+            }
         }
     }
     ~LearnedScheduler() override
@@ -148,6 +166,12 @@ public:
             if (o.live) close_sample(o);
         for (Open& o : keep_)
             if (o.live) close_sample(o);
+        // the walk is over: a longer training round on every sample so far, off the
+        // per-iteration path (the next walk decides with it)
+        if (mlp_ && m_.samples >= cfg_.min_samples) {
+            m_.trained_at = m_.samples;
+            train(cfg_.finish_steps);
+        }
     }
 
     bool schedule(const SchedInput& in) override
@@ -320,7 +344,8 @@ public:
             }
             age_[b] = trace[b] ? 0.f : std::min(age_[b] + 1.f, 1024.f);
         }
-        if (mlp_ && m_.samples >= 8 && m_.samples >= m_.trained_at + cfg_.train_every) {
+        const bool first = m_.y_sigma <= 0.f && m_.samples >= cfg_.min_samples;
+        if (mlp_ && (first || (m_.y_sigma > 0.f && m_.samples >= m_.trained_at + cfg_.train_every))) {
             m_.trained_at = m_.samples;
             train();
         }
@@ -343,7 +368,7 @@ public:
 private:
     static constexpr uint32_t kF = 12;
 
-    bool trained() const { return mlp_ && m_.samples >= cfg_.min_samples; }
+    bool trained() const { return mlp_ && m_.y_sigma > 0.f; }   // trained at least once
     bool model_decides() const { return use_model_ && cooldown_ == 0 && cfg_.model_swap; }
 
     // resident blocks allowed: every slot, or all but the staging slot when the graph does
@@ -448,6 +473,8 @@ private:
             }
             use_model_ = trained() && have_feats_ && (fill || cfg_.model_swap || tail_near) &&
                          gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
+            if (use_model_)
+                for (float& p : pred_) p = std::max(0.f, m_.y_mu + m_.y_sigma * p);
         }
         rank_.resize(nb);
         // without the network: the walkers a load would release (log scale), with the
@@ -584,18 +611,30 @@ private:
         ++m_.samples;
     }
 
-    void train()
+    void train() { train(cfg_.train_steps); }
+    void train(uint32_t steps)
     {
         const uint32_t n = (uint32_t)m_.ys.size();
         if (!n) return;
+        {
+            double s = 0, s2 = 0;
+            for (float y : m_.ys) {
+                s += y;
+                s2 += (double)y * y;
+            }
+            const double mu = s / n;
+            m_.y_mu = (float)mu;
+            // floor: labels that barely vary would otherwise blow small errors up
+            m_.y_sigma = (float)std::max(0.05, std::sqrt(std::max(0.0, s2 / n - mu * mu)));
+        }
         const uint32_t bs = std::min(cfg_.batch, n);
         std::vector<float> bx((size_t)bs * kF), by(bs);
-        for (uint32_t e = 0; e < cfg_.train_steps; ++e) {
+        for (uint32_t e = 0; e < steps; ++e) {
             for (uint32_t i = 0; i < bs; ++i) {
                 m_.rng = m_.rng * 6364136223846793005ULL + 1442695040888963407ULL;
                 const uint32_t k = (uint32_t)((m_.rng >> 33) % n);
                 std::copy(m_.xs.begin() + (size_t)k * kF, m_.xs.begin() + (size_t)(k + 1) * kF, bx.begin() + (size_t)i * kF);
-                by[i] = m_.ys[k];
+                by[i] = (m_.ys[k] - m_.y_mu) / m_.y_sigma;
             }
             float loss = 0.f;
             if (gw_mlp_step(ctx_, mlp_, bx.data(), by.data(), bs, cfg_.lr, &loss) != GW_OK) return;

@@ -133,7 +133,8 @@ typedef struct {
 
 typedef struct {
     uint64_t steps, trials, probes, mix_steps;
-    uint64_t h2d_bytes;      /* block upload bytes: (nverts+1)*8 + nedges*4 per upload */
+    uint64_t h2d_bytes;      /* block upload bytes that crossed: (nverts+1)*8 + nedges*4 per raw
+                                upload, the packed size per packed one */
     uint64_t h2d_uploads;
     double h2d_ms;           /* copy-stream busy time of block uploads */
     double kernel_ms;
@@ -189,23 +190,31 @@ gw_status gw_upload_block(gw_ctx* ctx, uint32_t slot, uint32_t blk,
 gw_status gw_stage_block(gw_ctx* ctx, uint32_t slot, uint32_t blk,
                          const uint64_t* beg, const uint32_t* csr);
 gw_status gw_commit_block(gw_ctx* ctx, uint32_t slot);
-/* Packed uploads (no reference counterpart: the reference copies each block's csr as stored,
- * 4 bytes per arc, gpu/memory.hpp:124-139).  `packed` is the block's adjacency in the
- * transfer encoding written by gw_pack_block (pinned memory, valid until the copy is done);
- * beg is sent as is.  The copy stream moves the packed bytes into a staging buffer and a
- * decode kernel writes the block's csr into the slot, so the slot holds exactly what
- * gw_upload_block / gw_stage_block would put there; h2d_bytes counts the bytes that crossed,
- * h2d_raw_bytes the block's size in the reference's format.  gw_set_pack_staging sizes
- * the staging buffer (counted against cache_bytes; 0 frees it): it holds a block's chunk
- * headers and list starts (16 B per 256 arcs + 4 B per vertex) whole and its payload in
- * pieces of the rest, so any size from those plus 1 KiB up works; larger buffers mean fewer,
- * longer copies. */
+/* Packed uploads (no reference counterpart: the reference copies each block's beg and csr
+ * as stored, 8 bytes per vertex and 4 per arc, gpu/memory.hpp:124-139).  `packed` is the
+ * block's offsets and adjacency in the transfer encoding written by gw_pack_block (pinned
+ * memory, valid until the copy is done; the block must have arcs).  The copy stream moves
+ * the packed bytes into a staging buffer and two decode kernels write the block's beg and
+ * csr into the slot, so the slot holds exactly what gw_upload_block / gw_stage_block would
+ * put there; h2d_bytes counts the bytes that crossed, h2d_raw_bytes the block's size in the
+ * reference's format.  The first time a block is sent from a buffer (its address, size and
+ * a fingerprint of a few words), every header is checked against the block's vertex and arc
+ * counts before anything is copied; later sends check the block header only, and the decode
+ * kernels keep every read inside the staging buffer whatever the headers hold.  gw_set_pack_staging sizes the staging
+ * buffer (counted against cache_bytes; 0 frees it): it holds a block's headers and packed
+ * offsets (16 B per 256 arcs, 16 B per 256 vertices and the offsets' bits) whole and its arc
+ * payload in pieces of the rest, so any size from those plus 1088 B up works; larger buffers
+ * mean fewer, longer copies. */
 gw_status gw_set_pack_staging(gw_ctx* ctx, uint64_t bytes);
-gw_status gw_upload_block_packed(gw_ctx* ctx, uint32_t slot, uint32_t blk, const uint64_t* beg,
-                                 const void* packed, uint64_t packed_bytes);
-gw_status gw_stage_block_packed(gw_ctx* ctx, uint32_t slot, uint32_t blk, const uint64_t* beg,
-                                const void* packed, uint64_t packed_bytes);
+gw_status gw_upload_block_packed(gw_ctx* ctx, uint32_t slot, uint32_t blk, const void* packed,
+                                 uint64_t packed_bytes);
+gw_status gw_stage_block_packed(gw_ctx* ctx, uint32_t slot, uint32_t blk, const void* packed,
+                                uint64_t packed_bytes);
 gw_status gw_staged_block(gw_ctx* ctx, uint32_t slot, int32_t* blk, uint32_t* ready);
+/* Copies the beg (nverts + 1 entries) and csr (nedges) that slot holds for its resident or
+ * staged block to host memory, after the slot's copy (and decode) is done; no reference
+ * counterpart (an inspection call for tests of the upload paths). */
+gw_status gw_read_slot(gw_ctx* ctx, uint32_t slot, uint64_t* beg, uint32_t* csr);
 /* Marks a slot free and its block non-resident: replaces the INACTIVE update of
  * GOwalker_scheduler_t::copyblocks (engine/schedule.hpp:181-183). */
 gw_status gw_evict(gw_ctx* ctx, uint32_t slot);
@@ -335,14 +344,24 @@ gw_status gw_graph_read(gw_ctx* ctx, const gw_graph* g, uint64_t* beg, uint32_t*
 gw_status gw_graph_free(gw_ctx* ctx, gw_graph* g);
 
 /* ---- host-side block format helpers (no GPU needed) ---- */
-/* The transfer encoding of one block's adjacency (host only, no GPU): beg = the block's
- * nverts + 1 global offsets, csr = its nedges arcs (a truncated giant vertex's list clipped at
- * nedges).  Layout, in u32 words: ceil(nedges / 256) chunk headers {payload word offset, bit
- * width w (0..32), carry (the arc before the chunk when the chunk starts inside a list, else
- * 0), block-local vertex holding the chunk's first arc}; nverts list starts (each vertex's
- * first neighbour, 0 when empty); then per chunk 8w payload words, arc j of the chunk in bits
- * [jw, jw + w): its difference (mod 2^32) to the arc before it in the same list, 0 for a
- * list's first arc.  With out == NULL only *bytes is set. */
+/* The transfer encoding of one block (host only, no GPU): beg = the block's nverts + 1
+ * global offsets, csr = its nedges arcs (a truncated giant vertex's list clipped at nedges).
+ * Layout, in u32 words (arc and entry indices local to the block):
+ *   header       {0x32505747, nvc = ceil((nverts + 1) / 256), vpw, 0};
+ *   beg chunks   nvc x {beg[256k] low, high, payload word offset, widths w0..w3 (a byte
+ *                each)}; then vpw payload words: per chunk, per group g of 64 entries, 2 w_g
+ *                words, entry e of the group in bits [e w_g, (e + 1) w_g): beg[i] - beg[i-1]
+ *                (0 for a chunk's first entry);
+ *   arc chunks   ceil(nedges / 256) x {payload word offset, L | nz << 8, carry (the arc
+ *                before the chunk when it starts inside a list, else 0), block-local vertex
+ *                holding the chunk's first arc}; then per chunk of n arcs, with g_j the arc's
+ *                difference (mod 2^32) to the arc before it in its list (to carry for a
+ *                continued list's first arc in the chunk; its value at a list start) and S_j =
+ *                g_0 + ... + g_j (64-bit): 8L words holding S_j mod 2^L in bits [jL, (j+1)L),
+ *                then ceil((n + nz) / 32) words with bit (S_j >> L) + j set for each j (an
+ *                Elias-Fano code; L = floor(log2(S_{n-1} / n)) or 0, nz = S_{n-1} >> L).
+ * A block without arcs packs to 0 bytes (upload it raw).  With out == NULL only *bytes is
+ * set. */
 gw_status gw_pack_block(const uint64_t* beg, const uint32_t* csr, uint32_t nverts, uint64_t nedges,
                         void* out, uint64_t cap_bytes, uint64_t* bytes);
 

@@ -35,7 +35,7 @@ typedef struct {
     uint64_t samples;          /* samples recorded so far (all walks) */
     uint64_t walks;            /* walks that trained it */
     uint64_t adam_steps;       /* optimizer steps taken */
-    float loss;                /* last training batch loss (MSE on log1p steps) */
+    float loss;                /* last training batch loss (MSE on standardized log1p steps) */
     uint32_t has_params;       /* 0 until a walk has stored parameters */
     uint64_t scored;           /* closed samples the network had predicted */
     double net_abs_err;        /* their mean |log1p error| ... */
@@ -61,7 +61,7 @@ gw_status gwh_model_tuning(const gwh_model* m, gwh_tune_entry* out, uint32_t cap
 typedef struct gwh_packed gwh_packed;
 gw_status gwh_pack_graph(const uint64_t* beg, const uint32_t* csr, uint64_t entries, uint64_t block_bytes,
                          gwh_packed** out);
-/* raw_bytes: the blocks' csr bytes; packed_bytes: what their uploads send instead */
+/* raw_bytes: the blocks' beg and csr bytes; packed_bytes: what their uploads send instead */
 gw_status gwh_packed_info(const gwh_packed* p, uint64_t* raw_bytes, uint64_t* packed_bytes, uint32_t* nblocks,
                           uint32_t* blocks_packed);
 gw_status gwh_packed_destroy(gwh_packed* p);

@@ -186,6 +186,24 @@ def test_apps_end_to_end(gpu, tmp_path):
     print(r4.stdout[-1000:], r4.stderr[-1000:])
     assert r4.returncode == 0
     np.testing.assert_array_equal(np.fromfile(hs, dtype=np.uint64), o["hashes"])
+    # packed block uploads and the staging scheduler walk the same paths; the metrics file
+    # records the bytes that crossed PCIe below the blocks' size in the reference's format
+    hp = str(tmp_path / "hp.bin")
+    mp = str(tmp_path / "mp.txt")
+    r5 = subprocess.run([os.path.join(build, "Node2vec"), base + ".txt", "length", "15", "walkpersource", "2",
+                         "memory", "1", "blocksize", "1", "p", "0.5", "q", "2", "GOWalker", "sched", "staging", "pack",
+                         "seed", "5", "hashes", hp, "metrics.reporter.filename", mp], capture_output=True, text=True,
+                        timeout=300, cwd=str(tmp_path))
+    print(r5.stdout[-1000:], r5.stderr[-1000:])
+    assert r5.returncode == 0
+    np.testing.assert_array_equal(np.fromfile(hp, dtype=np.uint64), o["hashes"])
+    kv = dict(l.split("=", 1) for l in open(mp).read().splitlines() if "=" in l)
+
+    def metric(key):   # keys read <run name>.<key>
+        return float([v for k, v in kv.items() if k.endswith("." + key)][0])
+
+    assert metric("pack_time") >= 0 and metric("h2d_bytes") < metric("h2d_raw_bytes"), kv
+    assert "staging" in open(mp).read()
     # the reference requires one scheduler flag
     r2 = subprocess.run([os.path.join(build, "Node2vec"), base + ".txt", "length", "5"], capture_output=True,
                         text=True, timeout=60, cwd=str(tmp_path))

@@ -48,6 +48,24 @@ def giant_graph():
     return G.from_edges(n, src, dst, undirected=True, dedup=True, drop_self=True)
 
 
+def head_bytes(pk, ne):
+    """bytes of a packed block before its arc payload: header, beg chunks and payload, arc
+    chunk headers"""
+    if ne == 0:
+        return 0
+    return 4 * (4 + 4 * int(pk[1]) + int(pk[2]) + 4 * ((ne + 255) // 256))
+
+
+def unsorted_graph():
+    """lists in random order: gaps wrap modulo 2^32, a chunk's gap sums pass 2^32"""
+    rng = np.random.default_rng(4)
+    n = 600
+    deg = rng.integers(0, 90, n)
+    beg = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    csr = rng.integers(0, n, int(beg[-1])).astype(np.uint32)
+    return beg, csr
+
+
 CASES = [
     ("rmat-permuted", lambda: G.rmat(13, 60000, seed=3, permute=True), 16384, N2V, 1),
     ("directed-mixed", lambda: G.rmat(12, 30000, seed=8, undirected=False), 8192, MIXED, 2),
@@ -63,14 +81,14 @@ def test_packed_uploads_match_oracle(gpu, name, make, block_bytes, types, wpv, h
     bl = Blocks(beg, csr, block_bytes)
     nb = bl.nblocks
     biggest = max(len(pack_block(*bl.arrays(b))) * 4 for b in range(nb))
-    # "pieces": a staging buffer of the largest block's headers and list starts plus 1 KiB,
-    # so every payload crosses in pieces of a few chunks
-    head = max(16 * ((bl.desc[b].nedges + 255) // 256) + 4 * bl.desc[b].nverts for b in range(nb))
+    # "pieces": a staging buffer of the largest block's headers and packed offsets plus the
+    # largest possible chunk (1088 B), so every payload crosses in pieces of a few chunks
+    head = max(head_bytes(pack_block(*bl.arrays(b)), bl.desc[b].nedges) for b in range(nb))
     with Context() as ctx:
         ctx.set_blocks(bl)
         ctx.set_walk(types, 13)
         ctx.init_walkers(n, wpv, outputs=OUT_HASH | OUT_VISITS)
-        ctx.set_pack_staging(head + 1024 if how == "pieces" else biggest)
+        ctx.set_pack_staging(head + 1088 if how == "pieces" else biggest)
         ctx.reserve_slots(nb)
         sent = 0
         for b in range(nb):
@@ -143,3 +161,84 @@ def test_engine_with_packed_graph(gpu):
     pk.close()
     b.close()
     c.close()
+
+
+SLOT_CASES = [
+    ("rmat-permuted", lambda: G.rmat(14, 120000, seed=6, permute=True), 65536),
+    ("unsorted", unsorted_graph, 8192),
+    ("giant-vertex", giant_graph, 4096),
+    ("power-law-multi", lambda: G.power_law_directed(3000, 40000, seed=2, multi=True), 8192),
+]
+
+
+@pytest.mark.parametrize("name,make,block_bytes", SLOT_CASES)
+def test_packed_slots_bit_exact(gpu, name, make, block_bytes):
+    """Every block decoded into its slot, uploaded or staged, in one piece or in pieces,
+    holds exactly the block's beg (global offsets) and csr."""
+    beg, csr = make()
+    bl = Blocks(beg, csr, block_bytes)
+    nb = bl.nblocks
+    pks = [pack_block(*bl.arrays(b)) for b in range(nb)]
+    biggest = max(4 * len(p) for p in pks)
+    head = max(head_bytes(pks[b], bl.desc[b].nedges) for b in range(nb))
+    checked = 0
+    for staging in (biggest, head + 1088):
+        with Context() as ctx:
+            ctx.set_blocks(bl)
+            ctx.set_walk(N2V, 3)
+            ctx.init_walkers(len(beg) - 1, 1, outputs=OUT_HASH)
+            ctx.set_pack_staging(staging)
+            ctx.reserve_slots(nb)
+            for b in range(nb):
+                if bl.desc[b].nedges == 0:
+                    continue
+                if b % 2:
+                    ctx.stage_packed(b, b)
+                else:
+                    ctx.upload_packed(b, b)
+            ctx.wait_uploads()
+            for b in range(nb):
+                if bl.desc[b].nedges == 0:
+                    continue
+                eb, ec = bl.arrays(b)
+                gb, gc = ctx.read_slot(b, b)
+                np.testing.assert_array_equal(gb, eb.astype(np.uint64), err_msg="%s block %d beg" % (name, b))
+                np.testing.assert_array_equal(gc, ec, err_msg="%s block %d csr" % (name, b))
+                checked += 1
+            cnt = ctx.counters()
+        print(name, "staging", staging, "blocks", nb, "h2d", cnt["h2d_bytes"], "raw", cnt["h2d_raw_bytes"], flush=True)
+    assert checked >= 2
+
+
+def test_malformed_packed_block_rejected(gpu):
+    """A packed block whose headers disagree with the block (wrong magic, payload offsets,
+    an Elias-Fano width or upper-bit count out of range, a first vertex past the block) is
+    refused on the host before anything is copied; the context stays usable."""
+    beg, csr = G.rmat(12, 30000, seed=9, permute=True)
+    bl = Blocks(beg, csr, 1 << 20)
+    blk = next(b for b in range(bl.nblocks) if bl.desc[b].nedges > 512)
+    pk = pack_block(*bl.arrays(blk))
+    nv = bl.desc[blk].nverts
+    ah = 4 + 4 * int(pk[1]) + int(pk[2])
+    bad = []
+    x = pk.copy(); x[0] ^= 1; bad.append(("magic", x))
+    x = pk.copy(); x[ah + 4] += 1; bad.append(("chunk offset", x))
+    x = pk.copy(); x[ah + 1] = (x[ah + 1] & ~np.uint32(0xFF)) | np.uint32(40); bad.append(("L above 31", x))
+    x = pk.copy(); x[ah + 1] = (x[ah + 1] & np.uint32(0xFF)) | np.uint32(600 << 8); bad.append(("nz", x))
+    x = pk.copy(); x[ah + 3] = nv; bad.append(("first vertex", x))
+    x = pk[:-1].copy(); bad.append(("truncated", x))
+    with Context() as ctx:
+        ctx.set_blocks(bl)
+        ctx.set_walk(N2V, 3)
+        ctx.init_walkers(len(beg) - 1, 1, outputs=OUT_HASH)
+        ctx.set_pack_staging(4 * len(pk))
+        ctx.reserve_slots(1)
+        for what, x in bad:
+            st = E.lib().gw_upload_block_packed(ctx.h, 0, blk, E._p(x), 4 * len(x))
+            assert st != 0, what
+        ctx.upload_packed(0, blk)
+        ctx.wait_uploads()
+        gb, gc = ctx.read_slot(0, blk)
+        eb, ec = bl.arrays(blk)
+        np.testing.assert_array_equal(gb, eb.astype(np.uint64))
+        np.testing.assert_array_equal(gc, ec)

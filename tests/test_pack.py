@@ -1,11 +1,11 @@
-"""The transfer encoding of a block's adjacency (gw_pack_block, host code, no GPU) decoded
-by a numpy restatement of its documented layout (include/gw.h): chunk headers {payload word
-offset, bit width, carry, first vertex}, per-vertex list starts, bit-packed differences.
-The decoded arcs must equal the block's csr exactly, for chunks that start inside lists,
-empty lists, a truncated giant vertex, duplicate targets (zero differences, width 0),
-unsorted lists (differences that wrap modulo 2^32, width 32) and vertex-limited blocks; the
-packed size is what the layout predicts.  The GPU decoder is checked against the same
-blocks through walks (tests/test_gpu_pack.py)."""
+"""The transfer encoding of a block (gw_pack_block, host code, no GPU) decoded by a numpy
+restatement of its documented layout (include/gw.h): a header, beg chunks of bit-packed
+offset differences, arc chunks of Elias-Fano-coded gap sums.  The decoded beg and csr must
+equal the block's exactly, for chunks that start inside lists, empty lists, a truncated
+giant vertex, duplicate targets (zero gaps), unsorted lists (gaps that wrap modulo 2^32,
+64-bit gap sums) and vertex-limited blocks; the packed size is what the layout predicts.
+The GPU decoders are checked against the same blocks through walks and slot contents
+(tests/test_gpu_pack.py)."""
 import numpy as np
 import pytest
 
@@ -13,43 +13,75 @@ import graphs as G
 from gowalker_amd.engine import Blocks, pack_block
 
 CHUNK = 256
+MAGIC = 0x32505747
 
 
-def decode(pk, beg, ne):
-    """numpy restatement of the decoder: arcs from (headers, starts, payload)"""
-    beg = beg.astype(np.int64)
-    nv = len(beg) - 1
-    se = int(beg[0])
+def bits_of(words):
+    """little-endian bit array of u32 words"""
+    return np.unpackbits(np.ascontiguousarray(words, dtype=np.uint32).view(np.uint8), bitorder="little")
+
+
+def fields(words, n, w):
+    """n w-bit little-endian fields packed from bit 0 of words"""
+    if w == 0:
+        return np.zeros(n, dtype=np.uint64)
+    b = bits_of(words)[:n * w].reshape(n, w).astype(np.uint64)
+    return (b << np.arange(w, dtype=np.uint64)).sum(axis=1, dtype=np.uint64)
+
+
+def decode(pk, nv, ne):
+    """numpy restatement of the decoders: (beg, csr, per-chunk L) from a packed block"""
+    assert pk[0] == MAGIC
+    nvc = (nv + 1 + CHUNK - 1) // CHUNK
+    assert pk[1] == nvc
+    vpw = int(pk[2])
+    vh = pk[4:4 + 4 * nvc].reshape(nvc, 4).astype(np.int64)
+    vp = pk[4 + 4 * nvc:4 + 4 * nvc + vpw]
+    beg = np.zeros(nv + 1, dtype=np.uint64)
+    for k in range(nvc):
+        base = np.uint64(vh[k, 0]) | (np.uint64(vh[k, 1]) << np.uint64(32))
+        off = int(vh[k, 2])
+        d = []
+        for g in range(4):
+            w = (int(vh[k, 3]) >> (8 * g)) & 0xFF
+            d.append(fields(vp[off:off + 2 * w], 64, w))
+            off += 2 * w
+        d = np.concatenate(d)
+        assert d[0] == 0
+        cnt = min(CHUNK, nv + 1 - k * CHUNK)
+        beg[k * CHUNK:k * CHUNK + cnt] = base + np.cumsum(d, dtype=np.uint64)[:cnt]
     nch = (ne + CHUNK - 1) // CHUNK
-    hdr = pk[:nch * 4].reshape(nch, 4).astype(np.int64) if nch else np.zeros((0, 4), np.int64)
-    starts = pk[nch * 4:nch * 4 + nv]
-    payload = pk[nch * 4 + nv:]
+    ah_off = 4 + 4 * nvc + vpw
+    ah = pk[ah_off:ah_off + 4 * nch].reshape(nch, 4).astype(np.int64)
+    ap = pk[ah_off + 4 * nch:]
+    se = int(beg[0])
+    starts = set(int(x) - se for x, y in zip(beg[:-1], beg[1:]) if y > x)
     out = np.zeros(ne, dtype=np.uint32)
-    first = beg[:-1] - se
-    nonempty = (beg[1:] - beg[:-1]) > 0
-    start_of = {}
-    for v in np.nonzero(nonempty)[0]:
-        if first[v] < ne:
-            start_of[int(first[v])] = int(v)
+    Ls = []
+    words = 0
     for c in range(nch):
-        word, w, carry, vfirst = (int(x) for x in hdr[c])
-        a0, a1 = c * CHUNK, min(c * CHUNK + CHUNK, ne)
-        bits = np.unpackbits(payload[word:word + 8 * w].view(np.uint8), bitorder="little") if w else None
-        prev = carry
-        for a in range(a0, a1):
-            if a in start_of:
-                x = int(starts[start_of[a]])
-            else:
-                g = 0
-                if w:
-                    j = a - a0
-                    g = int(np.dot(bits[j * w:(j + 1) * w].astype(np.uint64), (1 << np.arange(w, dtype=np.uint64))))
-                x = (prev + g) & 0xFFFFFFFF
-            out[a] = x
-            prev = x
+        off, lz, carry, vfirst = (int(x) for x in ah[c])
+        L, nz = lz & 0xFF, lz >> 8
+        Ls.append(L)
+        a0 = c * CHUNK
+        n = min(CHUNK, ne - a0)
+        assert off == words
+        hw = (n + nz + 31) // 32
+        words += 8 * L + hw
+        low = fields(ap[off:off + 8 * L], n, L)
+        ones = np.nonzero(bits_of(ap[off + 8 * L:off + 8 * L + hw]))[0]
+        assert len(ones) == n
+        S = ((ones - np.arange(n)).astype(np.uint64) << np.uint64(L)) | low
+        assert int(S[-1]) >> L == nz
+        prev_S, base = 0, carry          # a continued list: carry + S_j
+        for j in range(n):
+            if a0 + j in starts:
+                prev_S, base = (int(S[j - 1]) if j else 0), 0
+            out[a0 + j] = (base + int(S[j]) - prev_S) & 0xFFFFFFFF
         # the header's vertex holds the chunk's first arc
-        assert first[vfirst] <= a0 < beg[vfirst + 1] - se
-    return out, nch
+        assert int(beg[vfirst]) - se <= a0 < int(beg[vfirst + 1]) - se
+    assert len(pk) == ah_off + 4 * nch + words
+    return beg, out, np.array(Ls)
 
 
 def check_graph(beg, csr, block_bytes, nblocks_min=2):
@@ -64,11 +96,10 @@ def check_graph(beg, csr, block_bytes, nblocks_min=2):
         if d.nedges == 0:
             assert len(pk) == 0
             continue
-        got, nch = decode(pk, bb, d.nedges)
+        gb, got, _ = decode(pk, d.nverts, d.nedges)
+        np.testing.assert_array_equal(gb, bb.astype(np.uint64), err_msg="block %d beg" % b)
         np.testing.assert_array_equal(got, cc, err_msg="block %d" % b)
-        widths = pk[:nch * 4].reshape(nch, 4)[:, 1].astype(np.int64)
-        assert len(pk) == nch * 4 + d.nverts + 8 * int(widths.sum())
-        total_raw += 4 * d.nedges
+        total_raw += 8 * (d.nverts + 1) + 4 * d.nedges
         total_packed += 4 * len(pk)
     return total_packed / total_raw
 
@@ -97,18 +128,37 @@ def test_pack_giant_vertex_and_vertex_limited_blocks():
 
 
 @pytest.mark.parametrize("kind", ["unsorted", "dense", "duplicates"])
-def test_pack_extreme_widths(kind):
+def test_pack_extreme_gaps(kind):
     rng = np.random.default_rng(5)
     n, deg = 40, 700
     beg = np.arange(0, (n + 1) * deg, deg, dtype=np.uint64)
-    if kind == "unsorted":       # differences wrap modulo 2^32: width 32
+    if kind == "unsorted":       # gaps wrap modulo 2^32; chunk gap sums pass 2^32
         csr = rng.integers(0, 2**32, n * deg, dtype=np.uint64).astype(np.uint32)
-    elif kind == "dense":        # consecutive ids: width 1
+    elif kind == "dense":        # consecutive ids: gap sums below the chunk's arc count
         csr = np.tile(np.arange(deg, dtype=np.uint32), n)
-    else:                        # one repeated id per list: width 0
+    else:                        # one repeated id per list: zero gaps
         csr = np.repeat(np.arange(n, dtype=np.uint32) * 3, deg)
     pk = pack_block(beg, csr)
-    got, nch = decode(pk, beg, len(csr))
+    gb, got, Ls = decode(pk, n, len(csr))
+    np.testing.assert_array_equal(gb, beg)
     np.testing.assert_array_equal(got, csr)
-    widths = set(pk[:nch * 4].reshape(nch, 4)[:, 1].tolist())
-    assert widths == {"unsorted": {32}, "dense": {1}, "duplicates": {0}}[kind]
+    if kind == "unsorted":
+        assert Ls.min() >= 29 and Ls.max() <= 31
+    else:
+        assert Ls.max() == 0
+
+
+def test_pack_size_follows_mean_gap():
+    """An arc costs about log2(its chunk's mean gap) + 2 bits: lists of a fixed gap 2^k
+    pack to L = k per chunk, with 8k words of lower bits and at most 24 of upper bits."""
+    for k in (0, 3, 11, 20):
+        nv, deg = 8, 512
+        beg = np.arange(0, (nv + 1) * deg, deg, dtype=np.uint64)
+        csr = np.tile((np.arange(deg, dtype=np.uint64) << np.uint64(k)) + np.uint64(1 << k), nv).astype(np.uint32)
+        pk = pack_block(beg, csr)
+        gb, got, Ls = decode(pk, nv, len(csr))
+        np.testing.assert_array_equal(got, csr)
+        # lists start at 2^k and step by 2^k: every chunk's gap sums are 256 x 2^k (two
+        # lists per chunk at most, each restarting at 2^k)
+        assert set(Ls.tolist()) <= {max(k - 1, 0), k}, (k, Ls)
+        assert len(pk) * 32 <= len(csr) * (k + 2) + 64 * 48 + 32 * 80, (k, len(pk))
