@@ -111,6 +111,7 @@ SIGNATURES = {
     "gw_mlp_backward": [P, P, P, P, C.c_uint32, P, P],
     "gw_mlp_step": [P, P, P, P, C.c_uint32, C.c_float, P],
     "gw_set_pack_staging": [P, C.c_uint64],
+    "gw_set_copy_lanes": [P, C.c_uint32],
     "gw_read_slot": [P, C.c_uint32, P, P],
     "gw_upload_block_packed": [P, C.c_uint32, C.c_uint32, P, C.c_uint64],
     "gw_stage_block_packed": [P, C.c_uint32, C.c_uint32, P, C.c_uint64],
@@ -176,7 +177,7 @@ class GwhReport(C.Structure):
                 ("sched_ms", C.c_double), ("tune_ms", C.c_double), ("tuned", GwLaunch),
                 ("tail_model", C.c_uint64), ("tail_threshold", C.c_uint64), ("swap_model", C.c_uint64),
                 ("swap_differs", C.c_uint64), ("fill_model", C.c_uint64), ("model_samples", C.c_uint64),
-                ("sticky_swaps", C.c_uint64), ("tune_candidate", C.c_uint32), ("pad2", C.c_uint32),
+                ("sticky_swaps", C.c_uint64), ("tune_candidate", C.c_uint32), ("copy_lanes", C.c_uint32),
                 ("h2d_raw_bytes", C.c_uint64)]
 
 

@@ -21,6 +21,11 @@ using namespace gw;
 struct gw_ctx {
     int device = 0;
     hipStream_t s_compute = nullptr, s_copy = nullptr, s_read = nullptr;
+    // copy lanes: block copies go on s_copy, or alternate between s_copy and s_copy2, each
+    // lane with its own part of the packed staging buffer, so a decode on one lane does not
+    // hold up the link while the other copies
+    hipStream_t s_copy2 = nullptr;
+    uint32_t lanes = 1, next_lane = 0;
     hipEvent_t ev_copy = nullptr, ev_f0 = nullptr, ev_f1 = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
     hipEvent_t ev_c0 = nullptr, ev_c1 = nullptr;
     uint32_t ncu = 0;
@@ -63,6 +68,7 @@ struct gw_ctx {
     std::vector<hipEvent_t> slot_ev0, slot_ev1;   // start / end of the slot's last copy
     std::vector<uint8_t> slot_wait;       // a run must wait for the slot's copy
     std::vector<uint8_t> slot_timed;      // copy time not yet added to tot.h2d_ms
+    std::vector<uint8_t> slot_lane;       // lane of the slot's last copy
     std::vector<uint8_t> slot_read;       // a block in the slot has been readable by kernels
 
     // walk configuration
@@ -190,12 +196,20 @@ static void collect_copy_time(gw_ctx* c, uint32_t s, bool wait)
     c->slot_timed[s] = 0;
 }
 
+// Waits for every block copy (and decode) of every lane.
+static hipError_t sync_copies(gw_ctx* c)
+{
+    hipError_t e = c->s_copy ? hipStreamSynchronize(c->s_copy) : hipSuccess;
+    const hipError_t e2 = c->s_copy2 ? hipStreamSynchronize(c->s_copy2) : hipSuccess;
+    return e != hipSuccess ? e : e2;
+}
+
 static gw_status ensure_buckets(gw_ctx* c);
 static gw_status push_block_table(gw_ctx* c);
 
 static void free_slots(gw_ctx* c)
 {
-    if (c->s_copy) (void)hipStreamSynchronize(c->s_copy);
+    (void)sync_copies(c);
     for (uint32_t s = 0; s < c->nslots; ++s) {
         collect_copy_time(c, s, true);
         dfree(c, c->slot_beg[s], c->slot_beg_bytes, true);
@@ -211,6 +225,7 @@ static void free_slots(gw_ctx* c)
     c->slot_ev1.clear();
     c->slot_wait.clear();
     c->slot_timed.clear();
+    c->slot_lane.clear();
     c->slot_read.clear();
     c->nslots = 0;
     for (auto& b : c->blk_slot) b = -1;
@@ -328,7 +343,7 @@ gw_status gw_destroy(gw_ctx* c)
     if (!c) return GW_OK;
     (void)hipSetDevice(c->device);
     if (c->s_compute) (void)hipStreamSynchronize(c->s_compute);
-    if (c->s_copy) (void)hipStreamSynchronize(c->s_copy);
+    (void)sync_copies(c);
     free_walkers(c);
     free_slots(c);
     dfree(c, c->d_pack, c->pack_cap, true);
@@ -349,6 +364,7 @@ gw_status gw_destroy(gw_ctx* c)
     for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
     if (c->s_compute) (void)hipStreamDestroy(c->s_compute);
     if (c->s_copy) (void)hipStreamDestroy(c->s_copy);
+    if (c->s_copy2) (void)hipStreamDestroy(c->s_copy2);
     if (c->s_read) (void)hipStreamDestroy(c->s_read);
     delete c;
     return GW_OK;
@@ -477,7 +493,7 @@ gw_status gw_reserve_slots(gw_ctx* c, uint32_t nslots)
     if (!c) return fail(c, GW_ERR_ARG, "null context");
     if (!c->nblocks) return fail(c, GW_ERR_STATE, "gw_reserve_slots before gw_set_block_table");
     (void)hipSetDevice(c->device);
-    HIPCHK(c, hipStreamSynchronize(c->s_copy));
+    HIPCHK(c, sync_copies(c));
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
     free_slots(c);
     for (auto& d : c->hblk) { d.resident = 0; d.beg = nullptr; d.csr = nullptr; }
@@ -495,6 +511,7 @@ gw_status gw_reserve_slots(gw_ctx* c, uint32_t nslots)
     c->slot_ev1.assign(nslots, nullptr);
     c->slot_wait.assign(nslots, 0);
     c->slot_timed.assign(nslots, 0);
+    c->slot_lane.assign(nslots, 0);
     c->slot_read.assign(nslots, 0);
     c->nslots = nslots;
     for (uint32_t s = 0; s < nslots; ++s) {
@@ -542,6 +559,23 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
     const gw_block_desc& d = c->desc[blk];
     if (packed ? !d.nedges : (!beg || (d.nedges && !csr)))
         return fail(c, GW_ERR_ARG, packed ? "a block without arcs is uploaded raw" : "null block buffer");
+    // the lane: the one of the slot's last copy while that copy may still run (a stream keeps
+    // its copies in order, so two copies never write one slot at once, and no copy waits on
+    // another stream), else the next in turn
+    uint32_t lane = 0;
+    if (c->lanes > 1) {
+        const hipError_t q = hipEventQuery(c->slot_ev1[slot]);
+        if (q != hipSuccess && q != hipErrorNotReady) return fail(c, GW_ERR_HIP, "copy event: %s", hipGetErrorString(q));
+        if (q == hipErrorNotReady) {
+            lane = c->slot_lane[slot];
+        } else {
+            lane = c->next_lane;
+            c->next_lane = (c->next_lane + 1) % c->lanes;
+        }
+    }
+    hipStream_t cs = lane ? c->s_copy2 : c->s_copy;
+    const uint64_t lane_cap = (c->pack_cap / c->lanes) & ~255ULL;   // staging bytes of each lane
+    uint32_t* stage = c->d_pack + lane * (lane_cap / 4);
     // a packed block: its headers and beg payload go to the staging buffer whole, its arc
     // payload in pieces of what is left of the buffer (at least the largest chunk, 1088 B)
     PackLayout lay{};
@@ -555,9 +589,9 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
         if (const char* e = pack_layout((const uint32_t*)packed, packed_bytes / 4, d.nverts, d.nedges, !checked, &lay))
             return fail(c, GW_ERR_ARG, "block %u: %s", blk, e);
         c->pack_checked[blk] = seen;
-        if (lay.head_w * 4 + 4 * (8 * 31 + 24) > c->pack_cap)
-            return fail(c, GW_ERR_ARG, "the %llu B staging buffer cannot hold the block's %llu B of headers and offsets "
-                        "plus one chunk (gw_set_pack_staging)", (unsigned long long)c->pack_cap,
+        if (lay.head_w * 4 + 4 * (8 * 31 + 24) > lane_cap)
+            return fail(c, GW_ERR_ARG, "the %llu B staging buffer of a copy lane cannot hold the block's %llu B of "
+                        "headers and offsets plus one chunk (gw_set_pack_staging)", (unsigned long long)lane_cap,
                         (unsigned long long)(lay.head_w * 4));
     }
     if (c->fault_kind == 1 && ++c->n_upload == c->fault_at)
@@ -569,19 +603,19 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
     collect_copy_time(c, slot, true);   // the slot's events are about to be re-recorded
     if (c->slot_read[slot] && hipStreamQuery(c->s_compute) != hipSuccess) {
         HIPCHK(c, hipEventRecord(c->ev_c0, c->s_compute));
-        HIPCHK(c, hipStreamWaitEvent(c->s_copy, c->ev_c0, 0));
+        HIPCHK(c, hipStreamWaitEvent(cs, c->ev_c0, 0));
     }
     const uint64_t bb = ((uint64_t)d.nverts + 1) * 8, cb = d.nedges * 4;
-    HIPCHK(c, hipEventRecord(c->slot_ev0[slot], c->s_copy));
+    HIPCHK(c, hipEventRecord(c->slot_ev0[slot], cs));
     if (packed) {
         // the copies and decodes share the copy stream, so each copy into the staging buffer
         // queues behind the decode that read the previous piece
         const uint32_t* pw = (const uint32_t*)packed;
         const uint32_t* ah = pw + lay.ah_off;
-        const uint64_t piece_w = (c->pack_cap / 4) - lay.head_w;
+        const uint64_t piece_w = (lane_cap / 4) - lay.head_w;
         const uint64_t total_w = packed_bytes / 4 - lay.head_w;
-        HIPCHK(c, hipMemcpyAsync(c->d_pack, pw, lay.head_w * 4, hipMemcpyHostToDevice, c->s_copy));
-        HIPCHK(c, launch_unpack_beg(c->d_pack, lay, d.nverts, c->slot_beg[slot], c->s_copy));
+        HIPCHK(c, hipMemcpyAsync(stage, pw, lay.head_w * 4, hipMemcpyHostToDevice, cs));
+        HIPCHK(c, launch_unpack_beg(stage, lay, d.nverts, c->slot_beg[slot], cs));
         // pieces: the most chunks whose payload fits (chunk offsets are contiguous, so chunks
         // [c0, c1) take words [off(c0), off(c1)), off(nch) = total_w; a binary search)
         auto off = [&](uint64_t k) { return k < lay.nch ? (uint64_t)ah[k * 4] : total_w; };
@@ -598,19 +632,20 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
             const uint64_t w1 = off(c1);
             if (w1 > total_w) return fail(c, GW_ERR_ARG, "block %u: packed chunk offsets past its end", blk);
             if (w1 > w0)
-                HIPCHK(c, hipMemcpyAsync(c->d_pack + lay.head_w, pw + lay.head_w + w0, (w1 - w0) * 4,
-                                         hipMemcpyHostToDevice, c->s_copy));
-            HIPCHK(c, launch_unpack_arcs(c->d_pack, lay, c->slot_beg[slot], d.start_edge, d.nverts, d.nedges, c0, c1,
-                                         w0, w1 - w0, slot_edges(c, slot, blk), c->s_copy));
+                HIPCHK(c, hipMemcpyAsync(stage + lay.head_w, pw + lay.head_w + w0, (w1 - w0) * 4,
+                                         hipMemcpyHostToDevice, cs));
+            HIPCHK(c, launch_unpack_arcs(stage, lay, c->slot_beg[slot], d.start_edge, d.nverts, d.nedges, c0, c1,
+                                         w0, w1 - w0, slot_edges(c, slot, blk), cs));
             c0 = c1;
         }
     } else {
-        HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, c->s_copy));
-        if (cb) HIPCHK(c, hipMemcpyAsync(slot_edges(c, slot, blk), csr, cb, hipMemcpyHostToDevice, c->s_copy));
+        HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, cs));
+        if (cb) HIPCHK(c, hipMemcpyAsync(slot_edges(c, slot, blk), csr, cb, hipMemcpyHostToDevice, cs));
     }
-    HIPCHK(c, hipEventRecord(c->slot_ev1[slot], c->s_copy));
-    HIPCHK(c, hipEventRecord(c->ev_copy, c->s_copy));
+    HIPCHK(c, hipEventRecord(c->slot_ev1[slot], cs));
+    HIPCHK(c, hipEventRecord(c->ev_copy, cs));
     c->slot_timed[slot] = 1;
+    c->slot_lane[slot] = (uint8_t)lane;
     c->tot.h2d_bytes += packed ? packed_bytes : bb + cb;
     c->tot.h2d_raw_bytes += bb + cb;
     c->tot.h2d_uploads += 1;
@@ -621,17 +656,34 @@ gw_status gw_set_pack_staging(gw_ctx* c, uint64_t bytes)
 {
     if (!c) return fail(c, GW_ERR_ARG, "null context");
     (void)hipSetDevice(c->device);
-    HIPCHK(c, hipStreamSynchronize(c->s_copy));
+    HIPCHK(c, sync_copies(c));
     dfree(c, c->d_pack, c->pack_cap, true);
     c->d_pack = nullptr;
     c->pack_cap = 0;
     if (!bytes) return GW_OK;
-    bytes = (bytes + 255) & ~255ULL;
+    // each copy lane gets an equal part, rounded up to 256 B
+    const uint64_t per = ((bytes + c->lanes - 1) / c->lanes + 255) & ~255ULL;
+    bytes = per * c->lanes;
     void* p = nullptr;
     gw_status st;
     if ((st = dalloc(c, &p, bytes, true)) != GW_OK) return st;
     c->d_pack = (uint32_t*)p;
     c->pack_cap = bytes;
+    return GW_OK;
+}
+
+gw_status gw_set_copy_lanes(gw_ctx* c, uint32_t lanes)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (lanes < 1 || lanes > 2) return fail(c, GW_ERR_ARG, "copy lanes must be 1 or 2 (got %u)", lanes);
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, sync_copies(c));
+    if (lanes == 2 && !c->s_copy2 && hipStreamCreateWithFlags(&c->s_copy2, hipStreamNonBlocking) != hipSuccess) {
+        c->s_copy2 = nullptr;
+        return fail(c, GW_ERR_HIP, "stream creation failed");
+    }
+    c->lanes = lanes;
+    c->next_lane = 0;
     return GW_OK;
 }
 
@@ -744,7 +796,7 @@ gw_status gw_read_slot(gw_ctx* c, uint32_t slot, uint64_t* beg, uint32_t* csr)
     const gw_block_desc& d = c->desc[blk];
     if (d.nedges && !csr) return fail(c, GW_ERR_ARG, "null argument");
     (void)hipSetDevice(c->device);
-    HIPCHK(c, hipStreamSynchronize(c->s_copy));
+    HIPCHK(c, sync_copies(c));
     HIPCHK(c, hipMemcpy(beg, c->slot_beg[slot], ((uint64_t)d.nverts + 1) * 8, hipMemcpyDeviceToHost));
     if (d.nedges)
         HIPCHK(c, hipMemcpy(csr, slot_edges(c, slot, (uint32_t)blk), d.nedges * 4, hipMemcpyDeviceToHost));
@@ -764,7 +816,7 @@ gw_status gw_evict(gw_ctx* c, uint32_t slot)
 gw_status gw_wait_uploads(gw_ctx* c)
 {
     if (!c) return fail(c, GW_ERR_ARG, "null context");
-    HIPCHK(c, hipStreamSynchronize(c->s_copy));
+    HIPCHK(c, sync_copies(c));
     for (uint32_t s = 0; s < c->nslots; ++s) collect_copy_time(c, s, true);
     return GW_OK;
 }

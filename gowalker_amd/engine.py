@@ -150,6 +150,9 @@ class Context:
     def set_pack_staging(self, nbytes):
         self._chk(lib().gw_set_pack_staging(self.h, nbytes))
 
+    def set_copy_lanes(self, lanes):
+        self._chk(lib().gw_set_copy_lanes(self.h, lanes))
+
     def upload_packed(self, slot, blk):
         """Upload blk into slot in the transfer encoding (packed on the host here); the
         staging buffer must hold it (set_pack_staging)."""
@@ -342,7 +345,8 @@ def gen_chung_lu(ctx, nverts, nedges, gamma=2.2, max_degree=3000000, seed=42, de
 
 SCHED_PLAIN, SCHED_LEARNED, SCHED_SOWALKER, SCHED_STAGING = 0, 1, 2, 3
 # gwh_engine_opts.flags
-NO_SWAP_PREFETCH, MYOPIC_LABEL, MODEL_SWAP, VALUE_VICTIM, NO_STICKY, NO_TUNE, TUNE_FREEZE = 1, 2, 4, 8, 16, 32, 64
+NO_SWAP_PREFETCH, MYOPIC_LABEL, MODEL_SWAP, VALUE_VICTIM, NO_STICKY, NO_TUNE, TUNE_FREEZE, TWO_LANES = \
+    1, 2, 4, 8, 16, 32, 64, 128
 
 
 class Model:

@@ -51,6 +51,12 @@ struct EngineConfig {
     bool auto_search = false;
     // block uploads in the transfer encoding (one entry per block), or nullptr: raw
     const std::vector<PackedBlock>* packed = nullptr;
+    // copy lanes of packed uploads (gw_set_copy_lanes): 1 = one copy stream; 2 = two, with a
+    // staging buffer of the largest block each when that costs no slot, else half of one
+    // each.  One is the default: on C3 two lanes were no faster at 8 GB (3.10 vs 3.14 s) and
+    // slower at 16 GB (380 vs 366 ms), where two copies sharing the link land the fill's
+    // first blocks later
+    uint32_t copy_lanes = 1;
     bool verbose = false;
 };
 
@@ -67,7 +73,7 @@ struct RunReport {
     uint64_t h2d_bytes = 0, h2d_raw_bytes = 0, uploads = 0, staged = 0, iterations = 0;
     uint64_t zero_copy_runs = 0, zero_copy_steps = 0, zc_touches = 0;
     double wall_s = 0, kernel_ms = 0, filter_ms = 0, h2d_ms = 0, sched_ms = 0, tune_ms = 0;
-    uint32_t nslots = 0, nblocks = 0;
+    uint32_t nslots = 0, nblocks = 0, copy_lanes = 1;
     gw_launch tuned{0, 0, 0, 0};    // launch the autotuner picked (block 0: none)
 };
 
@@ -186,13 +192,42 @@ public:
         // piece adds a decode between copies on the copy stream: with 16 MB pieces the C3
         // 8 GB walk took 3.81 s instead of 3.43 s, and at that cap not even 25 MB of staging
         // left room for the slot a whole block's staging costs.
+        // Two copy lanes (on request) let one lane's decode overlap the other's copy: with a
+        // whole-block staging buffer each when the second buffer leaves the slot count as it
+        // is, else sharing the one buffer (each lane sends its blocks in pieces of half of it).
         if (cfg_.packed && !pack_staging_) {
-            uint64_t whole = 0;
+            uint64_t whole = 0, head = 0;
             for (uint32_t b = 0; b < nb; ++b)
-                if (const PackedBlock* p = packed(b)) whole = std::max<uint64_t>(whole, p->bytes);
-            if (whole) ck(gw_set_pack_staging(ctx_, whole), ctx_, "gw_set_pack_staging");
+                if (const PackedBlock* p = packed(b)) {
+                    whole = std::max<uint64_t>(whole, p->bytes);
+                    const uint32_t* w = (const uint32_t*)p->data;   // block header: magic, nvc, vpw
+                    const uint64_t nch = (blocks_[b].nedges + 255) / 256;
+                    head = std::max<uint64_t>(head, 4 * (4 + 4 * (uint64_t)w[1] + w[2] + 4 * nch));
+                }
+            whole = (whole + 255) & ~255ULL;
+            if (whole) {
+                ck(gw_set_copy_lanes(ctx_, 1), ctx_, "gw_set_copy_lanes");
+                ck(gw_set_pack_staging(ctx_, whole), ctx_, "gw_set_pack_staging");
+                if (cfg_.copy_lanes == 2) {
+                    const uint32_t one = slots_for_budget();
+                    ck(gw_set_copy_lanes(ctx_, 2), ctx_, "gw_set_copy_lanes");
+                    ck(gw_set_pack_staging(ctx_, 2 * whole), ctx_, "gw_set_pack_staging");
+                    lanes_ = 2;
+                    if (slots_for_budget() < one) {
+                        // the halves must hold a block's headers plus its largest chunk
+                        if (whole / 2 >= head + 1088 + 512) {
+                            ck(gw_set_pack_staging(ctx_, whole), ctx_, "gw_set_pack_staging");
+                        } else {
+                            lanes_ = 1;
+                            ck(gw_set_copy_lanes(ctx_, 1), ctx_, "gw_set_copy_lanes");
+                            ck(gw_set_pack_staging(ctx_, whole), ctx_, "gw_set_pack_staging");
+                        }
+                    }
+                }
+            }
             pack_staging_ = true;
         }
+        r.copy_lanes = lanes_;
         const uint32_t want = slots_for_budget();
         if (want == 0) throw GwError(GW_ERR_BUDGET, "the memory budget leaves no room for a single block slot");
         if (want != nslots_) {
@@ -367,6 +402,7 @@ private:
 
     EngineConfig cfg_;
     bool pack_staging_ = false;
+    uint32_t lanes_ = 1;
     const uint64_t* beg_;
     const uint32_t* csr_;
     std::vector<gw_block_desc> blocks_;

@@ -171,6 +171,7 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
             }
             ec.packed = &pk->g.blocks;
         }
+        ec.copy_lanes = (opts->flags & GWH_TWO_LANES) ? 2u : 1u;
         // launch tuning across walks: with a model and no launch given, this walk runs the
         // tuner's next candidate shape
         const bool tune = opts->model && opts->launch.block == 0 && (opts->flags & GWH_NO_TUNE) == 0;
@@ -233,6 +234,7 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         report->tune_ms = r.tune_ms;
         report->tuned = r.tuned;
         report->tune_candidate = tune ? tcand : 0xFFFFFFFFu;
+        report->copy_lanes = r.copy_lanes;
         if (ls) {
             report->learned_decisions = ls->learned_decisions();
             report->fallbacks = ls->fallbacks();

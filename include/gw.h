@@ -206,6 +206,13 @@ gw_status gw_commit_block(gw_ctx* ctx, uint32_t slot);
  * payload in pieces of the rest, so any size from those plus 1088 B up works; larger buffers
  * mean fewer, longer copies. */
 gw_status gw_set_pack_staging(gw_ctx* ctx, uint64_t bytes);
+/* Copy lanes (no reference counterpart): with 2, block copies alternate between two copy
+ * streams, each decoding its packed blocks from its own half of the staging buffer, so one
+ * lane's decode does not hold up the PCIe link while the other lane copies; a copy into a
+ * slot whose last copy may still run stays on that copy's lane.  1 (the default) keeps one
+ * copy stream.  Waits for every copy in flight.  gw_set_pack_staging gives each lane an
+ * equal part of its size, rounded up to 256 B: set the lanes first. */
+gw_status gw_set_copy_lanes(gw_ctx* ctx, uint32_t lanes);
 gw_status gw_upload_block_packed(gw_ctx* ctx, uint32_t slot, uint32_t blk, const void* packed,
                                  uint64_t packed_bytes);
 gw_status gw_stage_block_packed(gw_ctx* ctx, uint32_t slot, uint32_t blk, const void* packed,

@@ -85,8 +85,10 @@ enum {
     GWH_NO_STICKY = 1u << 4,     /* learned / staging: a swap does not prefer the block already
                                     staged among near-best candidates */
     GWH_NO_TUNE = 1u << 5,       /* with a model: keep the engine's default launch (no tuning) */
-    GWH_TUNE_FREEZE = 1u << 6    /* with a model: run the fastest candidate timed so far, trying
+    GWH_TUNE_FREEZE = 1u << 6,   /* with a model: run the fastest candidate timed so far, trying
                                     no new one (for walks that are measured) */
+    GWH_TWO_LANES = 1u << 7      /* packed uploads on two copy lanes (gw_set_copy_lanes; default
+                                    one) */
 };
 
 typedef struct {
@@ -126,7 +128,7 @@ typedef struct {
     uint64_t model_samples;    /* samples the network had after the walk (all walks) */
     uint64_t sticky_swaps;     /* swaps that took the already staged near-best block */
     uint32_t tune_candidate;   /* the launch tuner's candidate this walk ran (~0u: no tuning) */
-    uint32_t pad2;
+    uint32_t copy_lanes;       /* copy lanes of packed uploads (gw_set_copy_lanes) */
     uint64_t h2d_raw_bytes;    /* the uploads' size in the reference's block format (h2d_bytes:
                                   what crossed PCIe, smaller with packed uploads) */
 } gwh_report;

@@ -74,7 +74,7 @@ CASES = [
 
 
 @pytest.mark.parametrize("name,make,block_bytes,types,wpv", CASES)
-@pytest.mark.parametrize("how", ["all", "mixed", "staged", "pieces"])
+@pytest.mark.parametrize("how", ["all", "mixed", "staged", "pieces", "lanes"])
 def test_packed_uploads_match_oracle(gpu, name, make, block_bytes, types, wpv, how):
     beg, csr = make()
     n = (len(beg) - 1) * wpv
@@ -88,21 +88,23 @@ def test_packed_uploads_match_oracle(gpu, name, make, block_bytes, types, wpv, h
         ctx.set_blocks(bl)
         ctx.set_walk(types, 13)
         ctx.init_walkers(n, wpv, outputs=OUT_HASH | OUT_VISITS)
-        ctx.set_pack_staging(head + 1088 if how == "pieces" else biggest)
+        if how == "lanes":   # two copy lanes, uploads and staged copies alternating
+            ctx.set_copy_lanes(2)
+        ctx.set_pack_staging(head + 1088 if how == "pieces" else (2 if how == "lanes" else 1) * biggest)
         ctx.reserve_slots(nb)
         sent = 0
         for b in range(nb):
             if bl.desc[b].nedges == 0 or (how == "mixed" and b % 2):
                 ctx.upload(b, b)
                 sent += 4 * bl.desc[b].nedges
-            elif how == "staged":
+            elif how == "staged" or (how == "lanes" and b % 3 == 1):
                 sent += ctx.stage_packed(b, b)
             else:
                 sent += ctx.upload_packed(b, b)
         ctx.wait_uploads()
-        if how == "staged":
+        if how in ("staged", "lanes"):
             for b in range(nb):
-                if bl.desc[b].nedges:
+                if bl.desc[b].nedges and (how == "staged" or b % 3 == 1):
                     ctx.commit(b)
         st = ctx.run()
         assert st["walkers_left"] == 0
@@ -144,9 +146,15 @@ def test_engine_with_packed_graph(gpu):
             rep, h, _ = E.run_out_of_core(b.a, c.a, N2V, 21, n, memory_bytes=mem, block_bytes=block_bytes,
                                           scheduler=sched, hashes=True, packed=packed)
             out[(name, packed is not None)] = (rep, h)
-            print(name, "packed" if packed else "raw", {k: rep[k] for k in ("h2d_bytes", "h2d_raw_bytes", "uploads",
-                                                                             "staged", "nslots", "nblocks")},
-                  flush=True)
+    # the same packed walk on two copy lanes (half of the staging buffer each at this cap)
+    rep, h, _ = E.run_out_of_core(b.a, c.a, N2V, 21, n, memory_bytes=mem, block_bytes=block_bytes,
+                                  scheduler=E.SCHED_STAGING, hashes=True, packed=pk, flags=E.TWO_LANES)
+    assert rep["copy_lanes"] == 2
+    out[("staging two lanes", True)] = (rep, h)
+    for (name, packed), (rep, _) in out.items():
+        print(name, "packed" if packed else "raw", {k: rep[k] for k in ("h2d_bytes", "h2d_raw_bytes", "uploads", "staged",
+                                                                         "nslots", "nblocks", "copy_lanes")},
+              flush=True)
     o = oracle(beg, csr, N2V, 21, n, 1)
     for (name, packed), (rep, h) in out.items():
         np.testing.assert_array_equal(h, o["hashes"], err_msg="%s packed=%s" % (name, packed))
@@ -182,12 +190,13 @@ def test_packed_slots_bit_exact(gpu, name, make, block_bytes):
     biggest = max(4 * len(p) for p in pks)
     head = max(head_bytes(pks[b], bl.desc[b].nedges) for b in range(nb))
     checked = 0
-    for staging in (biggest, head + 1088):
+    for lanes, staging in ((1, biggest), (1, head + 1088), (2, biggest), (2, head + 1088)):
         with Context() as ctx:
             ctx.set_blocks(bl)
             ctx.set_walk(N2V, 3)
             ctx.init_walkers(len(beg) - 1, 1, outputs=OUT_HASH)
-            ctx.set_pack_staging(staging)
+            ctx.set_copy_lanes(lanes)
+            ctx.set_pack_staging(lanes * staging)
             ctx.reserve_slots(nb)
             for b in range(nb):
                 if bl.desc[b].nedges == 0:
@@ -206,8 +215,9 @@ def test_packed_slots_bit_exact(gpu, name, make, block_bytes):
                 np.testing.assert_array_equal(gc, ec, err_msg="%s block %d csr" % (name, b))
                 checked += 1
             cnt = ctx.counters()
-        print(name, "staging", staging, "blocks", nb, "h2d", cnt["h2d_bytes"], "raw", cnt["h2d_raw_bytes"], flush=True)
-    assert checked >= 2
+        print(name, "lanes", lanes, "staging", staging, "blocks", nb, "h2d", cnt["h2d_bytes"], "raw",
+              cnt["h2d_raw_bytes"], flush=True)
+    assert checked >= 8
 
 
 def test_malformed_packed_block_rejected(gpu):
@@ -242,3 +252,36 @@ def test_malformed_packed_block_rejected(gpu):
         eb, ec = bl.arrays(blk)
         np.testing.assert_array_equal(gb, eb.astype(np.uint64))
         np.testing.assert_array_equal(gc, ec)
+
+
+def test_lanes_same_slot_rewrites(gpu):
+    """With two copy lanes, blocks written into one slot back to back (uploads and staged
+    copies, no wait between them) leave the slot holding the last one exactly: a copy into a
+    slot whose last copy may still run stays on that copy's lane."""
+    beg, csr = G.rmat(14, 120000, seed=11, permute=True)
+    bl = Blocks(beg, csr, 65536)
+    nb = bl.nblocks
+    blocks = [b for b in range(nb) if bl.desc[b].nedges > 0]
+    assert len(blocks) >= 6
+    biggest = max(len(pack_block(*bl.arrays(b))) * 4 for b in blocks)
+    with Context() as ctx:
+        ctx.set_blocks(bl)
+        ctx.set_walk(N2V, 3)
+        ctx.init_walkers(len(beg) - 1, 1, outputs=OUT_HASH)
+        ctx.set_copy_lanes(2)
+        ctx.set_pack_staging(2 * biggest)
+        ctx.reserve_slots(2)
+        # slot 0: six blocks in a row; slot 1: one block, sent between them
+        for i, b in enumerate(blocks[:6]):
+            if i % 2:
+                ctx.stage_packed(0, b)
+            else:
+                ctx.upload_packed(0, b)
+            if i == 2:
+                ctx.upload_packed(1, blocks[-1])
+        ctx.wait_uploads()
+        for slot, b in ((0, blocks[5]), (1, blocks[-1])):
+            gb, gc = ctx.read_slot(slot, b)
+            eb, ec = bl.arrays(b)
+            np.testing.assert_array_equal(gb, eb.astype(np.uint64))
+            np.testing.assert_array_equal(gc, ec)
