@@ -9,14 +9,31 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace gw;
+
+// One block copy (and decode) to put on the copy streams.
+struct CopyJob {
+    uint32_t slot = 0, blk = 0, lane = 0;
+    const uint64_t* beg = nullptr;       // raw upload
+    const uint32_t* csr = nullptr;
+    const uint32_t* packed = nullptr;    // packed upload: its words
+    uint64_t words = 0;
+    PackLayout lay{};
+    bool ring = false;                   // through the staging ring and s_dec
+    hipEvent_t after_compute = nullptr;  // compute-stream work that may still read the slot
+    uint64_t seq = 0;
+};
 
 struct gw_ctx {
     int device = 0;
@@ -26,8 +43,8 @@ struct gw_ctx {
     // hold up the link while the other copies
     hipStream_t s_copy2 = nullptr;
     uint32_t lanes = 1, next_lane = 0;
-    hipEvent_t ev_copy = nullptr, ev_f0 = nullptr, ev_f1 = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
-    hipEvent_t ev_c0 = nullptr, ev_c1 = nullptr;
+    hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    hipEvent_t ev_c1 = nullptr;
     uint32_t ncu = 0;
     gw_device_info info{};
     uint64_t cap = 0, held = 0;
@@ -46,9 +63,47 @@ struct gw_ctx {
     uint32_t* d_vcut = nullptr;
     std::vector<uint32_t> vcut;
 
-    // staging buffer of packed block copies (decoded into the slots on the copy stream)
+    // staging buffer of packed block copies
     uint32_t* d_pack = nullptr;
     uint64_t pack_cap = 0;
+    // With one copy lane, packed blocks are decoded on their own stream: s_copy only copies,
+    // a block's headers into one of two head areas at the front of the staging buffer (the
+    // blocks alternate) and its arc payload in pieces into a ring over the rest, and s_dec
+    // decodes each piece into the slot once its copy has landed.  The link then carries the
+    // next piece while a piece decodes, and the copy stream never waits for another stream:
+    // the decoder, which writes the slot, waits for the compute stream, and the host enqueues
+    // a copy once the decodes that still read the head area or ring region it overwrites are
+    // done.  GW_PACK_INLINE_DECODE=1 decodes on the copy stream instead (the two-lane path,
+    // and blocks whose headers do not fit).
+    hipStream_t s_dec = nullptr;
+    bool inline_decode = false;
+    struct PkRegion {
+        uint64_t a, b;        // words [a, b) of the ring
+        hipEvent_t freed;     // recorded on s_dec after the decode that reads them
+    };
+    std::vector<PkRegion> pk_live;
+    uint64_t pk_wp = 0;                        // next ring position (words)
+    uint32_t pk_head_next = 0;                 // head area of the next block
+    hipEvent_t pk_head_ev[2] = {nullptr, nullptr};   // last decode that read each head area
+    std::vector<hipEvent_t> pk_pool, pk_retired;     // free events; events a stream may still wait on
+
+    // Upload worker (one copy lane): a thread of the context puts block copies and decodes
+    // on the streams, in the order they were asked for, so that a host wait inside the HIP
+    // runtime behind a long copy queue holds up neither scheduling nor kernel launches.  The
+    // caller's thread validates and books each upload; anything that needs a slot's copy
+    // events first waits until the worker has enqueued that slot's last job.
+    // GW_UPLOAD_THREAD=0 enqueues on the caller's thread instead.
+    bool sync_uploads = false;
+    std::thread worker;
+    std::mutex wmu;
+    std::condition_variable wcv;
+    std::deque<CopyJob> jobs;
+    uint64_t job_seq = 0, job_done = 0;   // jobs submitted; jobs enqueued (in order)
+    bool wstop = false;
+    gw_status werr = GW_OK;               // the worker's first failure
+    std::string werr_msg;
+    std::vector<uint64_t> slot_seq;       // last job of each slot
+    std::vector<hipEvent_t> cev_pool;     // events for jobs' compute-stream waits
     // per block: the packed buffer (address, size, fingerprint) whose headers were last
     // checked in full
     struct PackSeen {
@@ -123,6 +178,7 @@ struct gw_ctx {
 };
 
 static std::string g_err;
+static thread_local std::string* t_err = nullptr;   // the upload worker's messages go here
 
 static gw_status fail(gw_ctx* c, gw_status st, const char* fmt, ...)
 {
@@ -131,7 +187,7 @@ static gw_status fail(gw_ctx* c, gw_status st, const char* fmt, ...)
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    if (c) c->err = buf; else g_err = buf;
+    if (t_err) *t_err = buf; else if (c) c->err = buf; else g_err = buf;
     return st;
 }
 
@@ -184,11 +240,75 @@ static void free_walkers(gw_ctx* c)
     c->walker_bytes = 0;
 }
 
+static gw_status enqueue_copy(gw_ctx* c, const CopyJob& j);
+
+static void upload_worker(gw_ctx* c)
+{
+    (void)hipSetDevice(c->device);
+    std::string msg;
+    t_err = &msg;
+    std::unique_lock<std::mutex> lk(c->wmu);
+    for (;;) {
+        c->wcv.wait(lk, [&] { return c->wstop || !c->jobs.empty(); });
+        if (c->jobs.empty()) break;   // asked to stop, nothing left
+        const CopyJob j = c->jobs.front();
+        c->jobs.pop_front();
+        const bool skip = c->werr != GW_OK;   // after a failure the rest is dropped
+        lk.unlock();
+        const gw_status st = skip ? GW_OK : enqueue_copy(c, j);
+        lk.lock();
+        if (j.after_compute) c->cev_pool.push_back(j.after_compute);
+        if (st != GW_OK && c->werr == GW_OK) {
+            c->werr = st;
+            c->werr_msg = msg;
+        }
+        c->job_done = j.seq;
+        c->wcv.notify_all();
+    }
+}
+
+static gw_status worker_failed(gw_ctx* c)
+{
+    std::lock_guard<std::mutex> lk(c->wmu);
+    if (c->werr == GW_OK) return GW_OK;
+    return fail(c, c->werr, "block upload: %s", c->werr_msg.c_str());
+}
+
+// Waits until the worker has enqueued the last job of `slot` (every job when slot is -1).
+static void wait_enqueued(gw_ctx* c, int64_t slot)
+{
+    if (!c->worker.joinable()) return;
+    std::unique_lock<std::mutex> lk(c->wmu);
+    const uint64_t want = slot < 0 ? c->job_seq : c->slot_seq[(size_t)slot];
+    c->wcv.wait(lk, [&] { return c->job_done >= want; });
+}
+
+static bool enqueued(gw_ctx* c, uint32_t slot)
+{
+    if (!c->worker.joinable()) return true;
+    std::lock_guard<std::mutex> lk(c->wmu);
+    return c->job_done >= c->slot_seq[slot];
+}
+
+static void stop_worker(gw_ctx* c)
+{
+    if (!c->worker.joinable()) return;
+    {
+        std::lock_guard<std::mutex> lk(c->wmu);
+        c->wstop = true;
+    }
+    c->wcv.notify_all();
+    c->worker.join();
+    c->wstop = false;
+}
+
 // Adds the copy-engine time of a slot's last copy to tot.h2d_ms (waiting for the copy
 // when `wait`; otherwise only if it has completed).
 static void collect_copy_time(gw_ctx* c, uint32_t s, bool wait)
 {
     if (!c->slot_timed[s]) return;
+    if (wait) wait_enqueued(c, s);
+    else if (!enqueued(c, s)) return;
     if (wait) (void)hipEventSynchronize(c->slot_ev1[s]);
     else if (hipEventQuery(c->slot_ev1[s]) != hipSuccess) return;
     float ms = 0.f;
@@ -196,12 +316,31 @@ static void collect_copy_time(gw_ctx* c, uint32_t s, bool wait)
     c->slot_timed[s] = 0;
 }
 
+// After every copy and decode has completed: the staging ring holds nothing in use.
+static void pk_reset(gw_ctx* c)
+{
+    for (const auto& r : c->pk_live) c->pk_pool.push_back(r.freed);
+    for (hipEvent_t& e : c->pk_head_ev)
+        if (e) {
+            c->pk_pool.push_back(e);
+            e = nullptr;
+        }
+    for (hipEvent_t e : c->pk_retired) c->pk_pool.push_back(e);
+    c->pk_live.clear();
+    c->pk_retired.clear();
+    c->pk_wp = 0;
+    c->pk_head_next = 0;
+}
+
 // Waits for every block copy (and decode) of every lane.
 static hipError_t sync_copies(gw_ctx* c)
 {
+    wait_enqueued(c, -1);
     hipError_t e = c->s_copy ? hipStreamSynchronize(c->s_copy) : hipSuccess;
     const hipError_t e2 = c->s_copy2 ? hipStreamSynchronize(c->s_copy2) : hipSuccess;
-    return e != hipSuccess ? e : e2;
+    const hipError_t e3 = c->s_dec ? hipStreamSynchronize(c->s_dec) : hipSuccess;
+    if (e == hipSuccess && e2 == hipSuccess && e3 == hipSuccess) pk_reset(c);
+    return e != hipSuccess ? e : (e2 != hipSuccess ? e2 : e3);
 }
 
 static gw_status ensure_buckets(gw_ctx* c);
@@ -227,6 +366,7 @@ static void free_slots(gw_ctx* c)
     c->slot_timed.clear();
     c->slot_lane.clear();
     c->slot_read.clear();
+    c->slot_seq.clear();
     c->nslots = 0;
     for (auto& b : c->blk_slot) b = -1;
 }
@@ -316,9 +456,12 @@ gw_status gw_create(const gw_config* cfg, gw_ctx** out)
     c->info.lds_per_cu = (uint32_t)prop.maxSharedMemoryPerMultiProcessor;
     if (hipStreamCreateWithFlags(&c->s_compute, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s_copy, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->s_read, hipStreamNonBlocking) != hipSuccess)
+        hipStreamCreateWithFlags(&c->s_read, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->s_dec, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(nullptr, GW_ERR_HIP, "stream creation failed"));
-    hipEvent_t* evs[] = {&c->ev_copy, &c->ev_f0, &c->ev_f1, &c->ev_k0, &c->ev_k1, &c->ev_c0, &c->ev_c1};
+    if (const char* f = getenv("GW_PACK_INLINE_DECODE")) c->inline_decode = atoi(f) != 0;
+    if (const char* f = getenv("GW_UPLOAD_THREAD")) c->sync_uploads = atoi(f) == 0;
+    hipEvent_t* evs[] = {&c->ev_f0, &c->ev_f1, &c->ev_k0, &c->ev_k1, &c->ev_c1};
     for (hipEvent_t* ev : evs)
         if (hipEventCreate(ev) != hipSuccess) return bail(fail(nullptr, GW_ERR_HIP, "event creation failed"));
     void* p = nullptr;
@@ -344,6 +487,7 @@ gw_status gw_destroy(gw_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->s_compute) (void)hipStreamSynchronize(c->s_compute);
     (void)sync_copies(c);
+    stop_worker(c);
     free_walkers(c);
     free_slots(c);
     dfree(c, c->d_pack, c->pack_cap, true);
@@ -360,12 +504,16 @@ gw_status gw_destroy(gw_ctx* c)
     if (c->h_bucket) (void)hipHostFree(c->h_bucket);
     if (c->h_trace) (void)hipHostFree(c->h_trace);
     if (c->h_ring) (void)hipHostFree(c->h_ring);
-    hipEvent_t evs[] = {c->ev_copy, c->ev_f0, c->ev_f1, c->ev_k0, c->ev_k1, c->ev_c0, c->ev_c1};
+    hipEvent_t evs[] = {c->ev_f0, c->ev_f1, c->ev_k0, c->ev_k1, c->ev_c1};
     for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
     if (c->s_compute) (void)hipStreamDestroy(c->s_compute);
     if (c->s_copy) (void)hipStreamDestroy(c->s_copy);
     if (c->s_copy2) (void)hipStreamDestroy(c->s_copy2);
     if (c->s_read) (void)hipStreamDestroy(c->s_read);
+    pk_reset(c);
+    for (hipEvent_t e : c->pk_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->cev_pool) (void)hipEventDestroy(e);
+    if (c->s_dec) (void)hipStreamDestroy(c->s_dec);
     delete c;
     return GW_OK;
 }
@@ -513,6 +661,7 @@ gw_status gw_reserve_slots(gw_ctx* c, uint32_t nslots)
     c->slot_timed.assign(nslots, 0);
     c->slot_lane.assign(nslots, 0);
     c->slot_read.assign(nslots, 0);
+    c->slot_seq.assign(nslots, 0);
     c->nslots = nslots;
     for (uint32_t s = 0; s < nslots; ++s) {
         void* p;
@@ -546,13 +695,156 @@ static uint32_t* slot_edges(gw_ctx* c, uint32_t slot, uint32_t blk)
     return c->slot_csr[slot] + (c->desc[blk].start_edge & 15);
 }
 
-// Starts the copy of block `blk` into `slot` on the copy stream (no host wait).  The
-// slot's previous block stops being readable at once; if the slot ever held a readable
-// block and the compute stream still has work queued, the copy waits for that work, which
-// may still read it.  A slot that only ever held staged, uncommitted copies has no
+// Staging ring of packed uploads (one copy lane): two head areas of this size at the front
+// of the staging buffer, the arc-payload ring behind them, which must hold at least
+// kPkMinRing words (pieces of half the ring hold at least the largest chunk's 272 words).
+constexpr uint64_t kPkMinRing = 2048;
+static uint64_t pk_head_bytes(const gw_ctx* c) { return (c->pack_cap / 8) & ~255ULL; }
+
+// An event for the ring: from the pool, or the oldest retired one once its recorded work
+// has completed (a stream wait enqueued on it has then been satisfied), or a new one.
+// Retired events complete roughly in the order they were retired, so only the oldest few
+// are queried.
+static gw_status pk_event(gw_ctx* c, hipEvent_t* ev)
+{
+    if (c->pk_pool.empty()) {
+        size_t done = 0;
+        for (; done < c->pk_retired.size() && done < 4; ++done) {
+            const hipError_t q = hipEventQuery(c->pk_retired[done]);
+            if (q == hipErrorNotReady) break;
+            if (q != hipSuccess) return fail(c, GW_ERR_HIP, "staging event: %s", hipGetErrorString(q));
+        }
+        c->pk_pool.insert(c->pk_pool.end(), c->pk_retired.begin(), c->pk_retired.begin() + done);
+        c->pk_retired.erase(c->pk_retired.begin(), c->pk_retired.begin() + done);
+    }
+    if (!c->pk_pool.empty()) {
+        *ev = c->pk_pool.back();
+        c->pk_pool.pop_back();
+        return GW_OK;
+    }
+    HIPCHK(c, hipEventCreateWithFlags(ev, hipEventDisableTiming));
+    return GW_OK;
+}
+
+// Takes n words (a multiple of 64) of the ring at the write position, or at its start when
+// they do not fit before the end, once the decodes that still read any of them are done.
+// The wait is the host's (the upload worker's thread), not the copy stream's: a copy queued
+// behind a cross-stream wait runs at a fraction of the link rate (ring pieces of 60 MB took
+// 1.9-4.6 ms instead of 1.03 ms), while the decode of the region taken next has normally
+// finished before the copy in flight does.
+static gw_status pk_take(gw_ctx* c, uint64_t n, uint64_t ring_w, uint64_t* at)
+{
+    if (n > ring_w) return fail(c, GW_ERR_STATE, "staging ring piece of %llu words > ring of %llu",
+                                (unsigned long long)n, (unsigned long long)ring_w);
+    uint64_t a = c->pk_wp;
+    if (a + n > ring_w) a = 0;
+    const uint64_t b = a + n;
+    for (size_t i = 0; i < c->pk_live.size();) {
+        const gw_ctx::PkRegion r = c->pk_live[i];
+        if (r.a < b && a < r.b) {
+            HIPCHK(c, hipEventSynchronize(r.freed));
+            c->pk_pool.push_back(r.freed);
+            c->pk_live[i] = c->pk_live.back();
+            c->pk_live.pop_back();
+        } else {
+            ++i;
+        }
+    }
+    c->pk_wp = b;
+    *at = a;
+    return GW_OK;
+}
+
+// Records an event on `from` that `to` waits for (the event is retired at once).
+static gw_status pk_signal(gw_ctx* c, hipStream_t from, hipStream_t to)
+{
+    hipEvent_t ev;
+    gw_status st;
+    if ((st = pk_event(c, &ev)) != GW_OK) return st;
+    HIPCHK(c, hipEventRecord(ev, from));
+    HIPCHK(c, hipStreamWaitEvent(to, ev, 0));
+    c->pk_retired.push_back(ev);
+    return GW_OK;
+}
+
+// A packed block through the staging ring: its headers into the next head area and its
+// arc payload in pieces of half the ring on s_copy; beg and each piece decoded into
+// the slot on s_dec as their copies land.  The slot's copy events bracket the block: the
+// start on s_copy, the end after its last decode.
+static gw_status ring_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint32_t* pw, uint64_t words,
+                           const PackLayout& lay, hipEvent_t after_compute)
+{
+    const gw_block_desc& d = c->desc[blk];
+    const uint64_t hw = pk_head_bytes(c) / 4;
+    const uint64_t ring_w = c->pack_cap / 4 - 2 * hw;
+    const uint64_t piece_w = (ring_w / 2) & ~63ULL;
+    const uint32_t h = c->pk_head_next;
+    c->pk_head_next ^= 1u;
+    uint32_t* head = c->d_pack + h * hw;
+    uint32_t* ring = c->d_pack + 2 * hw;
+    gw_status st;
+    if (c->pk_head_ev[h]) {   // the head area after the decodes of the block that used it last
+        HIPCHK(c, hipEventSynchronize(c->pk_head_ev[h]));
+        c->pk_pool.push_back(c->pk_head_ev[h]);
+        c->pk_head_ev[h] = nullptr;
+    }
+    HIPCHK(c, hipEventRecord(c->slot_ev0[slot], c->s_copy));
+    HIPCHK(c, hipMemcpyAsync(head, pw, lay.head_w * 4, hipMemcpyHostToDevice, c->s_copy));
+    if ((st = pk_signal(c, c->s_copy, c->s_dec)) != GW_OK) return st;
+    // the decoder writes the slot: after the compute stream's work that may still read it
+    if (after_compute) HIPCHK(c, hipStreamWaitEvent(c->s_dec, after_compute, 0));
+    HIPCHK(c, launch_unpack_beg(head, lay, d.nverts, c->slot_beg[slot], c->s_dec));
+    // pieces: the most chunks whose payload fits a piece (chunk offsets are contiguous, so
+    // chunks [c0, c1) take words [off(c0), off(c1)), off(nch) = the payload size)
+    const uint32_t* ah = pw + lay.ah_off;
+    const uint64_t total_w = words - lay.head_w;
+    auto off = [&](uint64_t k) { return k < lay.nch ? (uint64_t)ah[k * 4] : total_w; };
+    uint64_t c0 = 0;
+    while (c0 < lay.nch) {
+        const uint64_t w0 = off(c0);
+        uint64_t lo = c0, hi = lay.nch;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi + 1) / 2;
+            if (off(mid) - w0 <= piece_w) lo = mid; else hi = mid - 1;
+        }
+        const uint64_t c1 = lo;
+        if (c1 == c0) return fail(c, GW_ERR_ARG, "a chunk of block %u does not fit a staging piece", blk);
+        const uint64_t w1 = off(c1);
+        if (w1 > total_w) return fail(c, GW_ERR_ARG, "block %u: packed chunk offsets past its end", blk);
+        const uint64_t n = w1 - w0, nr = (n + 63) & ~63ULL;
+        uint64_t a = 0;
+        if (n) {
+            if ((st = pk_take(c, nr, ring_w, &a)) != GW_OK) return st;
+            HIPCHK(c, hipMemcpyAsync(ring + a, pw + lay.head_w + w0, n * 4, hipMemcpyHostToDevice, c->s_copy));
+            if ((st = pk_signal(c, c->s_copy, c->s_dec)) != GW_OK) return st;
+        }
+        HIPCHK(c, launch_unpack_arcs(head, ring + a, lay, c->slot_beg[slot], d.start_edge, d.nverts, d.nedges, c0,
+                                     c1, w0, n, slot_edges(c, slot, blk), c->s_dec));
+        if (n) {
+            hipEvent_t ev;
+            if ((st = pk_event(c, &ev)) != GW_OK) return st;
+            HIPCHK(c, hipEventRecord(ev, c->s_dec));
+            c->pk_live.push_back({a, a + nr, ev});
+        }
+        c0 = c1;
+    }
+    hipEvent_t ev;
+    if ((st = pk_event(c, &ev)) != GW_OK) return st;
+    HIPCHK(c, hipEventRecord(ev, c->s_dec));
+    c->pk_head_ev[h] = ev;
+    HIPCHK(c, hipEventRecord(c->slot_ev1[slot], c->s_dec));
+    return GW_OK;
+}
+
+// Starts the copy of block `blk` into `slot` (no host wait): checks and books it here and
+// hands it to the upload worker (one copy lane) or enqueues it (two lanes).  The slot's
+// previous block stops being readable at once; if the slot ever held a readable block and
+// the compute stream still has work queued, the write into the slot waits for that work,
+// which may still read it.  A slot that only ever held staged, uncommitted copies has no
 // readers.  The wait is left out whenever it is not needed: a copy queued behind a
 // cross-stream wait has been measured to run at half the link rate for its whole length
-// (15 of 53 block copies of a 16 GB fill, 8.8 ms instead of 4.7 ms).
+// (15 of 53 block copies of a 16 GB fill, 8.8 ms instead of 4.7 ms); through the staging
+// ring the wait is the decoder's, and copies never wait for the compute stream.
 static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64_t* beg, const uint32_t* csr,
                             const void* packed = nullptr, uint64_t packed_bytes = 0)
 {
@@ -573,9 +865,7 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
             c->next_lane = (c->next_lane + 1) % c->lanes;
         }
     }
-    hipStream_t cs = lane ? c->s_copy2 : c->s_copy;
     const uint64_t lane_cap = (c->pack_cap / c->lanes) & ~255ULL;   // staging bytes of each lane
-    uint32_t* stage = c->d_pack + lane * (lane_cap / 4);
     // a packed block: its headers and beg payload go to the staging buffer whole, its arc
     // payload in pieces of what is left of the buffer (at least the largest chunk, 1088 B)
     PackLayout lay{};
@@ -596,24 +886,97 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
     }
     if (c->fault_kind == 1 && ++c->n_upload == c->fault_at)
         return fail(c, GW_ERR_FAULT, "injected upload failure (GW_FAULT=upload:%ld)", c->fault_at);
+    gw_status st;
+    if ((st = worker_failed(c)) != GW_OK) return st;
     (void)hipSetDevice(c->device);
     if (c->slot_blk[slot] >= 0) block_nonresident(c, (uint32_t)c->slot_blk[slot]);
     c->slot_blk[slot] = -1;
     c->slot_staged[slot] = -1;
     collect_copy_time(c, slot, true);   // the slot's events are about to be re-recorded
+    CopyJob j;
+    j.slot = slot;
+    j.blk = blk;
+    j.lane = lane;
+    j.beg = beg;
+    j.csr = csr;
+    j.packed = (const uint32_t*)packed;
+    j.words = packed_bytes / 4;
+    j.lay = lay;
+    j.ring = packed && c->lanes == 1 && !c->inline_decode && lay.head_w * 4 <= pk_head_bytes(c) &&
+             c->pack_cap / 4 >= 2 * (pk_head_bytes(c) / 4) + kPkMinRing;
+    // the compute stream's work queued now may still read the slot's previous block: the
+    // copy (or the decode) that writes the slot waits for it, and only for it
     if (c->slot_read[slot] && hipStreamQuery(c->s_compute) != hipSuccess) {
-        HIPCHK(c, hipEventRecord(c->ev_c0, c->s_compute));
-        HIPCHK(c, hipStreamWaitEvent(cs, c->ev_c0, 0));
+        {
+            std::lock_guard<std::mutex> lk(c->wmu);
+            if (!c->cev_pool.empty()) {
+                j.after_compute = c->cev_pool.back();
+                c->cev_pool.pop_back();
+            }
+        }
+        if (!j.after_compute) HIPCHK(c, hipEventCreateWithFlags(&j.after_compute, hipEventDisableTiming));
+        HIPCHK(c, hipEventRecord(j.after_compute, c->s_compute));
     }
     const uint64_t bb = ((uint64_t)d.nverts + 1) * 8, cb = d.nedges * 4;
+    c->slot_timed[slot] = 1;
+    c->slot_lane[slot] = (uint8_t)lane;
+    c->tot.h2d_bytes += packed ? packed_bytes : bb + cb;
+    c->tot.h2d_raw_bytes += bb + cb;
+    c->tot.h2d_uploads += 1;
+    if (c->lanes == 1 && !c->sync_uploads) {
+        if (!c->worker.joinable()) c->worker = std::thread(upload_worker, c);
+        std::lock_guard<std::mutex> lk(c->wmu);
+        j.seq = ++c->job_seq;
+        c->slot_seq[slot] = j.seq;
+        c->jobs.push_back(j);
+        c->wcv.notify_all();
+        return GW_OK;
+    }
+    // two copy lanes choose a lane from the slot's copy events: enqueued here, in order
+    wait_enqueued(c, -1);
+    st = enqueue_copy(c, j);
+    if (j.after_compute) {
+        std::lock_guard<std::mutex> lk(c->wmu);
+        c->cev_pool.push_back(j.after_compute);
+    }
+    return st;
+}
+
+// Puts one block copy on the streams (the upload worker's thread, or the caller's): a
+// packed block through the staging ring when the job says so, else copied and decoded on
+// its lane's copy stream.
+static gw_status enqueue_copy(gw_ctx* c, const CopyJob& j)
+{
+    const uint32_t slot = j.slot, blk = j.blk;
+    const gw_block_desc& d = c->desc[blk];
+    if (j.ring) return ring_copy(c, slot, blk, j.packed, j.words, j.lay, j.after_compute);
+    hipStream_t cs = j.lane ? c->s_copy2 : c->s_copy;
+    const uint64_t lane_cap = (c->pack_cap / c->lanes) & ~255ULL;   // staging bytes of each lane
+    uint32_t* stage = c->d_pack + j.lane * (lane_cap / 4);
+    const PackLayout& lay = j.lay;
+    if (j.packed && (!c->pk_live.empty() || c->pk_head_ev[0] || c->pk_head_ev[1])) {
+        // decoding on the copy stream reuses the staging buffer from its start: after every
+        // decode of the ring
+        HIPCHK(c, hipEventRecord(c->ev_c1, c->s_dec));
+        HIPCHK(c, hipStreamWaitEvent(cs, c->ev_c1, 0));
+        for (const auto& r : c->pk_live) c->pk_retired.push_back(r.freed);
+        for (hipEvent_t& e : c->pk_head_ev)
+            if (e) {
+                c->pk_retired.push_back(e);
+                e = nullptr;
+            }
+        c->pk_live.clear();
+        c->pk_wp = 0;
+    }
+    if (j.after_compute) HIPCHK(c, hipStreamWaitEvent(cs, j.after_compute, 0));
     HIPCHK(c, hipEventRecord(c->slot_ev0[slot], cs));
-    if (packed) {
+    if (j.packed) {
         // the copies and decodes share the copy stream, so each copy into the staging buffer
         // queues behind the decode that read the previous piece
-        const uint32_t* pw = (const uint32_t*)packed;
+        const uint32_t* pw = j.packed;
         const uint32_t* ah = pw + lay.ah_off;
         const uint64_t piece_w = (lane_cap / 4) - lay.head_w;
-        const uint64_t total_w = packed_bytes / 4 - lay.head_w;
+        const uint64_t total_w = j.words - lay.head_w;
         HIPCHK(c, hipMemcpyAsync(stage, pw, lay.head_w * 4, hipMemcpyHostToDevice, cs));
         HIPCHK(c, launch_unpack_beg(stage, lay, d.nverts, c->slot_beg[slot], cs));
         // pieces: the most chunks whose payload fits (chunk offsets are contiguous, so chunks
@@ -634,21 +997,16 @@ static gw_status start_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint64
             if (w1 > w0)
                 HIPCHK(c, hipMemcpyAsync(stage + lay.head_w, pw + lay.head_w + w0, (w1 - w0) * 4,
                                          hipMemcpyHostToDevice, cs));
-            HIPCHK(c, launch_unpack_arcs(stage, lay, c->slot_beg[slot], d.start_edge, d.nverts, d.nedges, c0, c1,
-                                         w0, w1 - w0, slot_edges(c, slot, blk), cs));
+            HIPCHK(c, launch_unpack_arcs(stage, stage + lay.head_w, lay, c->slot_beg[slot], d.start_edge, d.nverts,
+                                         d.nedges, c0, c1, w0, w1 - w0, slot_edges(c, slot, blk), cs));
             c0 = c1;
         }
     } else {
-        HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], beg, bb, hipMemcpyHostToDevice, cs));
-        if (cb) HIPCHK(c, hipMemcpyAsync(slot_edges(c, slot, blk), csr, cb, hipMemcpyHostToDevice, cs));
+        const uint64_t bb = ((uint64_t)d.nverts + 1) * 8, cb = d.nedges * 4;
+        HIPCHK(c, hipMemcpyAsync(c->slot_beg[slot], j.beg, bb, hipMemcpyHostToDevice, cs));
+        if (cb) HIPCHK(c, hipMemcpyAsync(slot_edges(c, slot, blk), j.csr, cb, hipMemcpyHostToDevice, cs));
     }
     HIPCHK(c, hipEventRecord(c->slot_ev1[slot], cs));
-    HIPCHK(c, hipEventRecord(c->ev_copy, cs));
-    c->slot_timed[slot] = 1;
-    c->slot_lane[slot] = (uint8_t)lane;
-    c->tot.h2d_bytes += packed ? packed_bytes : bb + cb;
-    c->tot.h2d_raw_bytes += bb + cb;
-    c->tot.h2d_uploads += 1;
     return GW_OK;
 }
 
@@ -780,6 +1138,10 @@ gw_status gw_staged_block(gw_ctx* c, uint32_t slot, int32_t* blk, uint32_t* read
     if (slot >= c->nslots) return fail(c, GW_ERR_ARG, "slot %u out of range (%u slots)", slot, c->nslots);
     *blk = c->slot_staged[slot];
     if (ready) {
+        if (*blk >= 0 && !enqueued(c, slot)) {   // the worker has not even enqueued it
+            *ready = 0;
+            return worker_failed(c);
+        }
         const hipError_t e = *blk >= 0 ? hipEventQuery(c->slot_ev1[slot]) : hipSuccess;
         if (e != hipSuccess && e != hipErrorNotReady) return fail(c, GW_ERR_HIP, "copy event: %s", hipGetErrorString(e));
         *ready = e == hipSuccess ? 1u : 0u;
@@ -818,7 +1180,7 @@ gw_status gw_wait_uploads(gw_ctx* c)
     if (!c) return fail(c, GW_ERR_ARG, "null context");
     HIPCHK(c, sync_copies(c));
     for (uint32_t s = 0; s < c->nslots; ++s) collect_copy_time(c, s, true);
-    return GW_OK;
+    return worker_failed(c);
 }
 
 gw_status gw_map_host_block(gw_ctx* c, uint32_t blk, const uint64_t* beg, const uint32_t* csr)
@@ -1040,8 +1402,11 @@ static gw_status ensure_buckets(gw_ctx* c)
 static gw_status push_block_table(gw_ctx* c)
 {
     // kernels read resident slots only after their copies (staged copies are not waited)
+    gw_status st;
     for (uint32_t s = 0; s < c->nslots; ++s)
         if (c->slot_wait[s]) {
+            wait_enqueued(c, s);
+            if ((st = worker_failed(c)) != GW_OK) return st;
             HIPCHK(c, hipStreamWaitEvent(c->s_compute, c->slot_ev1[s], 0));
             c->slot_wait[s] = 0;
         }

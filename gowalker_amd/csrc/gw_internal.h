@@ -159,12 +159,12 @@ inline uint64_t pack_fingerprint(const uint32_t* pk, uint64_t words)
 }
 // beg (nv + 1 global offsets) from a packed block whose first lay.head_w words are at staging
 hipError_t launch_unpack_beg(const uint32_t* staging, const PackLayout& lay, uint32_t nv, uint64_t* beg, hipStream_t s);
-// csr words of arc chunks [c_begin, c_end) into out; staging holds the block's first
-// lay.head_w words followed by `avail` of its arc payload words from word_base on; beg: the
-// decoded offsets on the device
-hipError_t launch_unpack_arcs(const uint32_t* staging, const PackLayout& lay, const uint64_t* beg, uint64_t start_edge,
-                              uint32_t nv, uint64_t ne, uint64_t c_begin, uint64_t c_end, uint64_t word_base,
-                              uint64_t avail, uint32_t* out, hipStream_t s);
+// csr words of arc chunks [c_begin, c_end) into out; head holds the block's first lay.head_w
+// words, payload `avail` of its arc payload words from word_base on; beg: the decoded
+// offsets on the device
+hipError_t launch_unpack_arcs(const uint32_t* head, const uint32_t* payload, const PackLayout& lay, const uint64_t* beg,
+                              uint64_t start_edge, uint32_t nv, uint64_t ne, uint64_t c_begin, uint64_t c_end,
+                              uint64_t word_base, uint64_t avail, uint32_t* out, hipStream_t s);
 // v[i] = sum over p < parts of v[p * n + i]; the other arrays are cleared
 hipError_t launch_fold_visits(uint32_t* v, uint64_t n, uint32_t parts, hipStream_t s);
 

@@ -238,14 +238,14 @@ hipError_t launch_unpack_beg(const uint32_t* staging, const PackLayout& lay, uin
     return hipGetLastError();
 }
 
-hipError_t launch_unpack_arcs(const uint32_t* staging, const PackLayout& lay, const uint64_t* beg, uint64_t start_edge,
-                              uint32_t nv, uint64_t ne, uint64_t c_begin, uint64_t c_end, uint64_t word_base,
-                              uint64_t avail, uint32_t* out, hipStream_t s)
+hipError_t launch_unpack_arcs(const uint32_t* head, const uint32_t* payload, const PackLayout& lay, const uint64_t* beg,
+                              uint64_t start_edge, uint32_t nv, uint64_t ne, uint64_t c_begin, uint64_t c_end,
+                              uint64_t word_base, uint64_t avail, uint32_t* out, hipStream_t s)
 {
     if (c_end > lay.nch) c_end = lay.nch;
     if (c_begin >= c_end) return hipSuccess;
-    hipLaunchKernelGGL(unpack_arcs_kernel, dim3((uint32_t)(c_end - c_begin)), dim3(64), 0, s, staging + lay.ah_off,
-                       staging + lay.head_w, word_base, avail, beg, start_edge, nv, ne, lay.nch, c_begin, out);
+    hipLaunchKernelGGL(unpack_arcs_kernel, dim3((uint32_t)(c_end - c_begin)), dim3(64), 0, s, head + lay.ah_off,
+                       payload, word_base, avail, beg, start_edge, nv, ne, lay.nch, c_begin, out);
     return hipGetLastError();
 }
 

@@ -59,9 +59,11 @@ struct ModelState {
     uint64_t rng = 0x853c49e6748fea9bULL;
     float loss = 0.f;
     double zc_reads_per_ms = 0;         // zero-copy reads per ms measured on tail runs (0: none yet)
-    // prediction quality on closed samples the network had predicted: its mean |log
-    // error| against that of the running mean of all labels seen before (a predictor
-    // with no features)
+    // prediction quality on samples the network has not been trained on: when a sample
+    // closes (its label is known, it is not yet in the ring), the current network predicts
+    // it from its features; its mean |log error| against that of the running mean of all
+    // labels seen before (a predictor with no features).  Scored once the network has
+    // trained on score_after samples.
     uint64_t scored = 0;
     double net_abs_err = 0, mean_abs_err = 0, label_sum = 0;
     uint64_t labels = 0;
@@ -73,6 +75,9 @@ struct ModelState {
     // both per round follows the labels as the ring turns over (the first samples of a
     // workload can be unlike the later ones)
     float y_mu = 0.f, y_sigma = 0.f;
+    // inputs are standardized the same way: per feature, the mean and deviation of the ring
+    // at the last training round (the features mix log counts of up to ~17 with fractions)
+    std::vector<float> x_mu, x_sigma;
 };
 
 struct LearnedConfig {
@@ -80,10 +85,11 @@ struct LearnedConfig {
     uint32_t h1 = 64, h2 = 64;
     uint64_t seed = 7;
     uint32_t min_samples = 24;      // plain decisions until this many samples
-    uint32_t train_steps = 2;       // Adam steps per training round
+    uint32_t train_steps = 4;       // Adam steps per training round
     uint32_t train_every = 32;      // new samples between training rounds (every 8 with 4 steps
                                     // cost 11 ms of host time per 0.19 s RMAT-24 walk)
-    uint32_t finish_steps = 16;     // Adam steps when a walk ends (~70 us each)
+    uint32_t finish_steps = 48;     // Adam steps when a walk ends (~70 us each)
+    uint32_t score_after = 256;     // samples trained on before predictions are scored
     uint32_t batch = 256;
     uint32_t max_samples = 16384;   // ring buffer of trace samples
     uint32_t keep_every = 4;        // iterations between keep samples of resident blocks
@@ -166,6 +172,7 @@ public:
             if (o.live) close_sample(o);
         for (Open& o : keep_)
             if (o.live) close_sample(o);
+        flush_closed();
         // the walk is over: a longer training round on every sample so far, off the
         // per-iteration path (the next walk decides with it)
         if (mlp_ && m_.samples >= cfg_.min_samples) {
@@ -403,18 +410,56 @@ private:
         float pred = -1.f;
     };
 
+    // The network's predictions (log1p steps) for n feature rows: inputs and outputs in the
+    // units the network was trained on (standardized), the result back in label units.
+    bool predict(const std::vector<float>& f, uint32_t n, std::vector<float>& out)
+    {
+        std::vector<float> x(f.begin(), f.begin() + (size_t)n * kF);
+        standardize(x.data(), n);
+        out.resize(n);
+        if (gw_mlp_forward(ctx_, mlp_, x.data(), n, out.data()) != GW_OK) return false;
+        for (float& p : out) p = std::max(0.f, m_.y_mu + m_.y_sigma * p);
+        return true;
+    }
+
+    void standardize(float* x, uint32_t n) const
+    {
+        if (m_.x_mu.size() != kF) return;
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t j = 0; j < kF; ++j) x[(size_t)i * kF + j] = (x[(size_t)i * kF + j] - m_.x_mu[j]) / m_.x_sigma[j];
+    }
+
+    // Closed samples wait here (a few dozen at a time) to be scored by the current network
+    // -- which has never trained on them -- and then join the ring.
+    void flush_closed()
+    {
+        if (closed_y_.empty()) return;
+        const uint32_t n = (uint32_t)closed_y_.size();
+        std::vector<float> p;
+        if (trained() && m_.trained_at >= cfg_.score_after && m_.labels && predict(closed_f_, n, p)) {
+            const double mean = m_.label_sum / (double)m_.labels;
+            for (uint32_t i = 0; i < n; ++i) {
+                ++m_.scored;
+                m_.net_abs_err += std::fabs((double)p[i] - (double)closed_y_[i]);
+                m_.mean_abs_err += std::fabs(mean - (double)closed_y_[i]);
+            }
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            m_.label_sum += closed_y_[i];
+            ++m_.labels;
+            add_sample(closed_f_.data() + (size_t)i * kF, closed_y_[i]);
+        }
+        closed_f_.clear();
+        closed_y_.clear();
+    }
+
     void close_sample(Open& o)
     {
         o.live = false;
         const float y = std::log1p((float)o.steps);
-        if (o.pred >= 0.f && m_.labels) {
-            ++m_.scored;
-            m_.net_abs_err += std::fabs((double)o.pred - (double)y);
-            m_.mean_abs_err += std::fabs(m_.label_sum / (double)m_.labels - (double)y);
-        }
-        m_.label_sum += y;
-        ++m_.labels;
-        add_sample(o.f.data(), y);
+        closed_f_.insert(closed_f_.end(), o.f.begin(), o.f.end());
+        closed_y_.push_back(y);
+        if (closed_y_.size() >= cfg_.train_every) flush_closed();
         // error monitor: how far the network's prediction was from what the block served
         if (o.pred >= 0.f) {
             const double e = std::fabs((double)o.pred - (double)y);
@@ -471,10 +516,7 @@ private:
                 feats_.assign((size_t)nb * kF, 0.f);
                 have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
             }
-            use_model_ = trained() && have_feats_ && (fill || cfg_.model_swap || tail_near) &&
-                         gw_mlp_forward(ctx_, mlp_, feats_.data(), nb, pred_.data()) == GW_OK;
-            if (use_model_)
-                for (float& p : pred_) p = std::max(0.f, m_.y_mu + m_.y_sigma * p);
+            use_model_ = trained() && have_feats_ && (fill || cfg_.model_swap || tail_near) && predict(feats_, nb, pred_);
         }
         rank_.resize(nb);
         // without the network: the walkers a load would release (log scale), with the
@@ -626,6 +668,20 @@ private:
             m_.y_mu = (float)mu;
             // floor: labels that barely vary would otherwise blow small errors up
             m_.y_sigma = (float)std::max(0.05, std::sqrt(std::max(0.0, s2 / n - mu * mu)));
+            std::vector<double> xs(kF, 0.0), xs2(kF, 0.0);
+            for (uint32_t i = 0; i < n; ++i)
+                for (uint32_t j = 0; j < kF; ++j) {
+                    const double v = m_.xs[(size_t)i * kF + j];
+                    xs[j] += v;
+                    xs2[j] += v * v;
+                }
+            m_.x_mu.assign(kF, 0.f);
+            m_.x_sigma.assign(kF, 1.f);
+            for (uint32_t j = 0; j < kF; ++j) {
+                const double m = xs[j] / n;
+                m_.x_mu[j] = (float)m;
+                m_.x_sigma[j] = (float)std::max(0.05, std::sqrt(std::max(0.0, xs2[j] / n - m * m)));
+            }
         }
         const uint32_t bs = std::min(cfg_.batch, n);
         std::vector<float> bx((size_t)bs * kF), by(bs);
@@ -636,6 +692,7 @@ private:
                 std::copy(m_.xs.begin() + (size_t)k * kF, m_.xs.begin() + (size_t)(k + 1) * kF, bx.begin() + (size_t)i * kF);
                 by[i] = (m_.ys[k] - m_.y_mu) / m_.y_sigma;
             }
+            standardize(bx.data(), bs);
             float loss = 0.f;
             if (gw_mlp_step(ctx_, mlp_, bx.data(), by.data(), bs, cfg_.lr, &loss) != GW_OK) return;
             m_.loss = loss;
@@ -653,6 +710,7 @@ private:
     std::vector<uint64_t> add_, demand_;
     std::vector<char> was_resident_;
     std::vector<Open> open_, keep_;
+    std::vector<float> closed_f_, closed_y_;   // closed samples not yet scored and in the ring
     double reads_per_step_ = 0, steps_per_walker_ = 0;
     uint64_t reads_n_ = 0, sw_n_ = 0;
     uint64_t pending_hops_ = 0;

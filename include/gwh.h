@@ -37,8 +37,9 @@ typedef struct {
     uint64_t adam_steps;       /* optimizer steps taken */
     float loss;                /* last training batch loss (MSE on standardized log1p steps) */
     uint32_t has_params;       /* 0 until a walk has stored parameters */
-    uint64_t scored;           /* closed samples the network had predicted */
-    double net_abs_err;        /* their mean |log1p error| ... */
+    uint64_t scored;           /* closed samples predicted before the network trained on them
+                                  (once it has trained on 256 samples) */
+    double net_abs_err;        /* its mean |log1p error| on them ... */
     double mean_abs_err;       /* ... and that of the running mean of earlier labels */
 } gwh_model_info;
 gw_status gwh_model_create(gwh_model** out);

@@ -74,7 +74,7 @@ CASES = [
 
 
 @pytest.mark.parametrize("name,make,block_bytes,types,wpv", CASES)
-@pytest.mark.parametrize("how", ["all", "mixed", "staged", "pieces", "lanes"])
+@pytest.mark.parametrize("how", ["all", "mixed", "staged", "pieces", "lanes", "ring", "ring-pieces"])
 def test_packed_uploads_match_oracle(gpu, name, make, block_bytes, types, wpv, how):
     beg, csr = make()
     n = (len(beg) - 1) * wpv
@@ -90,19 +90,24 @@ def test_packed_uploads_match_oracle(gpu, name, make, block_bytes, types, wpv, h
         ctx.init_walkers(n, wpv, outputs=OUT_HASH | OUT_VISITS)
         if how == "lanes":   # two copy lanes, uploads and staged copies alternating
             ctx.set_copy_lanes(2)
-        ctx.set_pack_staging(head + 1088 if how == "pieces" else (2 if how == "lanes" else 1) * biggest)
+        if how == "ring":            # the staging ring with room for whole blocks
+            ctx.set_pack_staging(8 * biggest + 65536)
+        elif how == "ring-pieces":   # the staging ring, blocks in pieces
+            ctx.set_pack_staging(max(8 * head, 16384))
+        else:
+            ctx.set_pack_staging(head + 1088 if how == "pieces" else (2 if how == "lanes" else 1) * biggest)
         ctx.reserve_slots(nb)
         sent = 0
         for b in range(nb):
             if bl.desc[b].nedges == 0 or (how == "mixed" and b % 2):
                 ctx.upload(b, b)
                 sent += 4 * bl.desc[b].nedges
-            elif how == "staged" or (how == "lanes" and b % 3 == 1):
+            elif how == "staged" or (how in ("lanes", "ring", "ring-pieces") and b % 3 == 1):
                 sent += ctx.stage_packed(b, b)
             else:
                 sent += ctx.upload_packed(b, b)
         ctx.wait_uploads()
-        if how in ("staged", "lanes"):
+        if how in ("staged", "lanes", "ring", "ring-pieces"):
             for b in range(nb):
                 if bl.desc[b].nedges and (how == "staged" or b % 3 == 1):
                     ctx.commit(b)
@@ -119,6 +124,46 @@ def test_packed_uploads_match_oracle(gpu, name, make, block_bytes, types, wpv, h
     assert st["steps"] == o["totals"]["steps"]
     assert cnt["h2d_raw_bytes"] == sum(8 * (bl.desc[b].nverts + 1) + 4 * bl.desc[b].nedges for b in range(nb))
     assert cnt["h2d_bytes"] < cnt["h2d_raw_bytes"]
+
+
+UPLOAD_MODES = {
+    "ring+worker": {},                                   # the default
+    "ring, caller's thread": {"GW_UPLOAD_THREAD": "0"},
+    "copy-stream decode+worker": {"GW_PACK_INLINE_DECODE": "1"},
+}
+
+
+@pytest.mark.parametrize("mode", sorted(UPLOAD_MODES))
+def test_engine_upload_modes(gpu, mode, monkeypatch):
+    """Block uploads enqueued by the context's upload worker or by the caller's thread,
+    decoded on the decode stream through the staging ring or on the copy stream: the
+    engine's walks equal the oracle's under a cap with evictions, staged copies, slots
+    rewritten while runs may still read them, and the zero-copy tail."""
+    for k, v in UPLOAD_MODES[mode].items():
+        monkeypatch.setenv(k, v)
+    beg, csr = G.rmat(14, 150000, seed=12, permute=True)
+    n = len(beg) - 1
+    b = E.PinnedArray(len(beg), np.uint64)
+    c = E.PinnedArray(len(csr), np.uint32)
+    b.a[:] = beg
+    c.a[:] = csr
+    block_bytes = 40000
+    with Context() as ctx:
+        ctx.set_blocks(Blocks(beg, csr, block_bytes))
+        slot = ctx.slot_bytes()
+    pk = E.PackedGraph(b.a, c.a, block_bytes)
+    mem = n * (5 * 4 + 8) + 8 * slot + 65536 + 4 * (pk.info()["raw_bytes"] // max(1, pk.info()["nblocks"]))
+    o = oracle(beg, csr, N2V, 5, n, 1)
+    for sched, zt in ((E.SCHED_LEARNED, 0), (E.SCHED_PLAIN, block_bytes // 3072)):
+        rep, h, _ = E.run_out_of_core(b.a, c.a, N2V, 5, n, memory_bytes=mem, block_bytes=block_bytes,
+                                      scheduler=sched, zero_threshold=zt, hashes=True, packed=pk)
+        print(mode, sched, {k: rep[k] for k in ("uploads", "staged", "nslots", "nblocks", "zc_touches", "h2d_bytes")},
+              flush=True)
+        assert rep["nslots"] < rep["nblocks"] and rep["uploads"] + rep["staged"] > rep["nblocks"]
+        np.testing.assert_array_equal(h, o["hashes"])
+    pk.close()
+    b.close()
+    c.close()
 
 
 def test_engine_with_packed_graph(gpu):
@@ -189,8 +234,17 @@ def test_packed_slots_bit_exact(gpu, name, make, block_bytes):
     pks = [pack_block(*bl.arrays(b)) for b in range(nb)]
     biggest = max(4 * len(p) for p in pks)
     head = max(head_bytes(pks[b], bl.desc[b].nedges) for b in range(nb))
+    # one lane decodes through the staging ring when a block's headers fit an eighth of the
+    # buffer: with room for whole blocks, with pieces of a fraction of a block (the ring
+    # wraps within blocks), and with a buffer that fits some blocks' headers and not others
+    # (ring and copy-stream decodes alternate)
+    heads = sorted(head_bytes(pks[b], bl.desc[b].nedges) for b in range(nb) if bl.desc[b].nedges)
+    ring_roomy = 8 * biggest + 65536
+    ring_tight = max(8 * head, 16384)
+    ring_mixed = max(8 * heads[len(heads) // 2], 16384)
     checked = 0
-    for lanes, staging in ((1, biggest), (1, head + 1088), (2, biggest), (2, head + 1088)):
+    for lanes, staging in ((1, biggest), (1, head + 1088), (2, biggest), (2, head + 1088), (1, ring_roomy),
+                           (1, ring_tight), (1, ring_mixed)):
         with Context() as ctx:
             ctx.set_blocks(bl)
             ctx.set_walk(N2V, 3)
@@ -254,10 +308,12 @@ def test_malformed_packed_block_rejected(gpu):
         np.testing.assert_array_equal(gc, ec)
 
 
-def test_lanes_same_slot_rewrites(gpu):
-    """With two copy lanes, blocks written into one slot back to back (uploads and staged
-    copies, no wait between them) leave the slot holding the last one exactly: a copy into a
-    slot whose last copy may still run stays on that copy's lane."""
+@pytest.mark.parametrize("lanes", [2, 1])
+def test_lanes_same_slot_rewrites(gpu, lanes):
+    """Blocks written into one slot back to back (uploads and staged copies, no wait between
+    them) leave the slot holding the last one exactly: with two copy lanes, a copy into a slot
+    whose last copy may still run stays on that copy's lane; with one lane and the staging
+    ring, the decodes of one stream write the slot in order."""
     beg, csr = G.rmat(14, 120000, seed=11, permute=True)
     bl = Blocks(beg, csr, 65536)
     nb = bl.nblocks
@@ -268,8 +324,8 @@ def test_lanes_same_slot_rewrites(gpu):
         ctx.set_blocks(bl)
         ctx.set_walk(N2V, 3)
         ctx.init_walkers(len(beg) - 1, 1, outputs=OUT_HASH)
-        ctx.set_copy_lanes(2)
-        ctx.set_pack_staging(2 * biggest)
+        ctx.set_copy_lanes(lanes)
+        ctx.set_pack_staging(2 * biggest if lanes == 2 else 8 * biggest + 65536)
         ctx.reserve_slots(2)
         # slot 0: six blocks in a row; slot 1: one block, sent between them
         for i, b in enumerate(blocks[:6]):
