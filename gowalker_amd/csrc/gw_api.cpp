@@ -22,6 +22,16 @@
 
 using namespace gw;
 
+// Everything a run's captured graph depends on (arguments zeroed bytewise before they are
+// filled in): equal signatures launch the same kernels with the same arguments.
+struct RunSig {
+    WalkArgs a;
+    FilterArgs f;
+    uint64_t grid;
+    uint32_t block, variant, flags, ncu;
+    const void *d_bucket, *m_cnt, *m_trace, *m_bucket, *m_blocks;
+};
+
 // One block copy (and decode) to put on the copy streams.
 struct CopyJob {
     uint32_t slot = 0, blk = 0, lane = 0;
@@ -168,6 +178,23 @@ struct gw_ctx {
     // cumulative counters
     gw_counters tot{};
 
+    // graphs of run sequences (GW_RUN_GRAPH=1; by default every kernel is launched one by
+    // one): the few sequences of a walk (fill runs with a filter pass, runs with every
+    // block resident, zero-copy runs) are captured once and replayed.  Events recorded in a
+    // graph cannot be timed, so graph runs time the step kernel and the bucket pass's filter
+    // with device-clock stamps written into pinned memory (h_ts / m_ts: [0, 1] step kernel,
+    // [2, 3] filter), ts_khz ticks per ms.
+    bool run_graphs = false;
+    unsigned long long *h_ts = nullptr, *m_ts = nullptr;
+    double ts_khz = 0;
+    struct GraphEntry {
+        RunSig sig;
+        hipGraphExec_t ex;
+        uint64_t used;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_clock = 0, graph_runs = 0, graph_captures = 0;
+
     // launch shape chosen by gw_autotune
     bool have_tuned = false;
     gw_launch tuned{256, 0, 0, 0};
@@ -224,8 +251,13 @@ static void dfree(gw_ctx* c, void* p, uint64_t bytes, bool counted)
     if (counted) c->held -= std::min(c->held, bytes);
 }
 
+static void graphs_clear(gw_ctx* c);
+
 static void free_walkers(gw_ctx* c)
 {
+    // captured runs name the walker arrays
+    if (c->s_compute) (void)hipStreamSynchronize(c->s_compute);
+    graphs_clear(c);
     dfree(c, c->d_cur, c->nwalkers * 4, true);
     dfree(c, c->d_prev, c->nwalkers * 4, true);
     dfree(c, c->d_meta, c->nwalkers * 4, true);
@@ -344,7 +376,7 @@ static hipError_t sync_copies(gw_ctx* c)
 }
 
 static gw_status ensure_buckets(gw_ctx* c);
-static gw_status push_block_table(gw_ctx* c);
+static gw_status push_block_table(gw_ctx* c, bool enqueue_copy = true);
 
 static void free_slots(gw_ctx* c)
 {
@@ -461,6 +493,13 @@ gw_status gw_create(const gw_config* cfg, gw_ctx** out)
         return bail(fail(nullptr, GW_ERR_HIP, "stream creation failed"));
     if (const char* f = getenv("GW_PACK_INLINE_DECODE")) c->inline_decode = atoi(f) != 0;
     if (const char* f = getenv("GW_UPLOAD_THREAD")) c->sync_uploads = atoi(f) == 0;
+    if (const char* f = getenv("GW_RUN_GRAPH")) c->run_graphs = atoi(f) != 0;
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0)
+            c->run_graphs = false;   // no clock to time graph runs with
+        c->ts_khz = khz;
+    }
     hipEvent_t* evs[] = {&c->ev_f0, &c->ev_f1, &c->ev_k0, &c->ev_k1, &c->ev_c1};
     for (hipEvent_t* ev : evs)
         if (hipEventCreate(ev) != hipSuccess) return bail(fail(nullptr, GW_ERR_HIP, "event creation failed"));
@@ -473,6 +512,9 @@ gw_status gw_create(const gw_config* cfg, gw_ctx** out)
     if (hipHostMalloc((void**)&c->h_cnt, 16 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
         hipHostGetDevicePointer((void**)&c->m_cnt, c->h_cnt, 0) != hipSuccess)
         return bail(fail(nullptr, GW_ERR_NOMEM, "pinned counter buffer"));
+    if (hipHostMalloc((void**)&c->h_ts, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->m_ts, c->h_ts, 0) != hipSuccess)
+        return bail(fail(nullptr, GW_ERR_NOMEM, "pinned timestamp buffer"));
     c->ring_cap = 4ull << 20;
     if (hipHostMalloc((void**)&c->h_ring, c->ring_cap, hipHostMallocDefault) != hipSuccess ||
         hipHostGetDevicePointer((void**)&c->m_ring, c->h_ring, 0) != hipSuccess)
@@ -488,6 +530,7 @@ gw_status gw_destroy(gw_ctx* c)
     if (c->s_compute) (void)hipStreamSynchronize(c->s_compute);
     (void)sync_copies(c);
     stop_worker(c);
+    graphs_clear(c);
     free_walkers(c);
     free_slots(c);
     dfree(c, c->d_pack, c->pack_cap, true);
@@ -504,6 +547,7 @@ gw_status gw_destroy(gw_ctx* c)
     if (c->h_bucket) (void)hipHostFree(c->h_bucket);
     if (c->h_trace) (void)hipHostFree(c->h_trace);
     if (c->h_ring) (void)hipHostFree(c->h_ring);
+    if (c->h_ts) (void)hipHostFree(c->h_ts);
     hipEvent_t evs[] = {c->ev_f0, c->ev_f1, c->ev_k0, c->ev_k1, c->ev_c1};
     for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
     if (c->s_compute) (void)hipStreamDestroy(c->s_compute);
@@ -1359,7 +1403,7 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
 // Enqueues the bucket pass over all walkers on the compute stream: the matrix of
 // (prev_blk, cur_blk) counts of unfinished walkers into d_bucket and, on the host, into
 // the pinned h_bucket (valid after the stream is synchronized).
-static gw_status enqueue_bucket_pass(gw_ctx* c)
+static gw_status enqueue_bucket_pass(gw_ctx* c, bool stamps = false)
 {
     const uint64_t nn = (uint64_t)c->nblocks * c->nblocks;
     HIPCHK(c, launch_fill_u32((uint32_t*)(c->d_cnt + 6), 0, 2, c->s_compute));
@@ -1376,9 +1420,9 @@ static gw_status enqueue_bucket_pass(gw_ctx* c)
     f.hops = c->d_cnt + 8;
     f.len0 = c->dt[0].length;
     f.len1 = c->dt[1].length;
-    HIPCHK(c, hipEventRecord(c->ev_f0, c->s_compute));
+    HIPCHK(c, stamps ? launch_timestamp(c->m_ts + 2, c->s_compute) : hipEventRecord(c->ev_f0, c->s_compute));
     HIPCHK(c, launch_filter(f, c->ncu * 2, c->s_compute));
-    HIPCHK(c, hipEventRecord(c->ev_f1, c->s_compute));
+    HIPCHK(c, stamps ? launch_timestamp(c->m_ts + 3, c->s_compute) : hipEventRecord(c->ev_f1, c->s_compute));
     HIPCHK(c, launch_copy_u32(c->m_bucket, c->d_bucket, nn, c->s_compute));
     HIPCHK(c, launch_copy_u32((uint32_t*)(c->m_cnt + 8), (const uint32_t*)(c->d_cnt + 6), 2, c->s_compute));
     HIPCHK(c, launch_copy_u32((uint32_t*)(c->m_cnt + 9), (const uint32_t*)(c->d_cnt + 8), 2, c->s_compute));
@@ -1399,7 +1443,10 @@ static gw_status ensure_buckets(gw_ctx* c)
     return GW_OK;
 }
 
-static gw_status push_block_table(gw_ctx* c)
+// The compute stream's waits for the copies of newly resident slots, and the block table
+// written into pinned memory; the copy kernel that pushes it to the device is enqueued
+// unless `enqueue_copy` is false (a run captures it into its graph).
+static gw_status push_block_table(gw_ctx* c, bool enqueue_copy)
 {
     // kernels read resident slots only after their copies (staged copies are not waited)
     gw_status st;
@@ -1414,12 +1461,52 @@ static gw_status push_block_table(gw_ctx* c)
     // before it returns
     memcpy(c->h_blocks_pinned, c->hblk.data(), sizeof(DevBlock) * c->nblocks);
     static_assert(sizeof(DevBlock) % 4 == 0, "block descriptors are copied as words");
-    HIPCHK(c, launch_copy_u32((uint32_t*)c->d_blocks, (const uint32_t*)c->m_blocks, sizeof(DevBlock) / 4 * c->nblocks,
-                              c->s_compute));
+    if (enqueue_copy)
+        HIPCHK(c, launch_copy_u32((uint32_t*)c->d_blocks, (const uint32_t*)c->m_blocks,
+                                  sizeof(DevBlock) / 4 * c->nblocks, c->s_compute));
     return GW_OK;
 }
 
 static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t cap);
+
+constexpr size_t kMaxGraphs = 8;
+
+gw_status gw_graph_stats(gw_ctx* c, uint64_t* replays, uint64_t* captures)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (replays) *replays = c->graph_runs;
+    if (captures) *captures = c->graph_captures;
+    return GW_OK;
+}
+
+static hipGraphExec_t graph_lookup(gw_ctx* c, const RunSig& sig)
+{
+    for (auto& g : c->graphs)
+        if (!memcmp(&g.sig, &sig, sizeof sig)) {
+            g.used = ++c->graph_clock;
+            return g.ex;
+        }
+    return nullptr;
+}
+
+// keeps the kMaxGraphs most recently used graphs
+static void graph_store(gw_ctx* c, const RunSig& sig, hipGraphExec_t ex)
+{
+    c->graph_captures++;
+    if (c->graphs.size() >= kMaxGraphs) {
+        auto old = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                    [](const gw_ctx::GraphEntry& x, const gw_ctx::GraphEntry& y) { return x.used < y.used; });
+        (void)hipGraphExecDestroy(old->ex);
+        c->graphs.erase(old);
+    }
+    c->graphs.push_back({sig, ex, ++c->graph_clock});
+}
+
+static void graphs_clear(gw_ctx* c)
+{
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.ex);
+    c->graphs.clear();
+}
 
 gw_status gw_run(gw_ctx* c, const gw_launch* L, gw_run_stats* stats)
 {
@@ -1443,7 +1530,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     if (l.variant == 1 && l.block > 256) return fail(c, GW_ERR_ARG, "the flattened kernel runs at most 256 threads per workgroup");
 
     gw_status st;
-    if ((st = push_block_table(c)) != GW_OK) return st;
+    if ((st = push_block_table(c, false)) != GW_OK) return st;
     c->w_valid = false;
     // The whole walker array is the work list: the kernel skips finished walkers and,
     // unless every block is readable, walkers whose (prev, cur) blocks are not both
@@ -1457,25 +1544,22 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     uint64_t nwork = c->nwalkers;
     if (cap && nwork > cap) nwork = cap;
     float fms = 0.f, kms = 0.f;
-    HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
-    HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
-    if (!all_res) {
-        // compact the runnable walkers on the device; the step kernel reads the count
-        // there, so no host round trip separates the two kernels
-        FilterArgs f{};
-        f.blocks = c->d_blocks;
-        f.nblocks = c->nblocks;
-        f.nwalkers = cap && cap < c->nwalkers ? cap : c->nwalkers;
-        f.meta = c->d_meta;
-        f.bpair = c->d_bpair;
-        f.worklist = c->d_work;
-        f.nwork = c->d_misc;
-        f.left = c->d_cnt + 6;
-        f.len0 = c->dt[0].length;
-        f.len1 = c->dt[1].length;
-        HIPCHK(c, launch_filter(f, c->ncu * 2, c->s_compute));
-    }
-    WalkArgs a{};
+    // compact the runnable walkers on the device; the step kernel reads the count there,
+    // so no host round trip separates the two kernels
+    FilterArgs f;
+    memset(&f, 0, sizeof f);   // the arguments are compared bytewise (graph cache)
+    f.blocks = c->d_blocks;
+    f.nblocks = c->nblocks;
+    f.nwalkers = cap && cap < c->nwalkers ? cap : c->nwalkers;
+    f.meta = c->d_meta;
+    f.bpair = c->d_bpair;
+    f.worklist = c->d_work;
+    f.nwork = c->d_misc;
+    f.left = c->d_cnt + 6;
+    f.len0 = c->dt[0].length;
+    f.len1 = c->dt[1].length;
+    WalkArgs a;
+    memset(&a, 0, sizeof a);
     a.blocks = c->d_blocks;
     a.vcut = c->d_vcut;
     a.nblocks = c->nblocks;
@@ -1497,10 +1581,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     a.nwork = (uint32_t)nwork;
     a.queue = c->d_misc + 1;
     a.counters = c->d_cnt;
-    if (l.flags & GW_RUN_TRACE) {
-        HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
-        a.block_steps = c->d_trace;
-    }
+    if (l.flags & GW_RUN_TRACE) a.block_steps = c->d_trace;
     // the flattened variant reads u32s through aligned 8-byte words, which is only
     // safe inside padded HBM slots: zero-copy (host-mapped) blocks use variant 0
     if (l.variant == 1 && (l.flags & (GW_RUN_INTERP | GW_RUN_WINDOW))) l.variant = 0;   // bisection only in variant 1
@@ -1528,26 +1609,77 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
         ch = ch < 64 ? 64 : (ch > 1024 ? 1024 : ch);
         a.chunk = (uint32_t)(ch / 64 * 64);
     }
-    HIPCHK(c, hipEventRecord(c->ev_k0, c->s_compute));
-    HIPCHK(c, launch_walk(a, one_blk, l.variant, (uint32_t)grid, l.block, c->s_compute));
-    HIPCHK(c, hipEventRecord(c->ev_k1, c->s_compute));
-    HIPCHK(c, launch_copy_u32((uint32_t*)c->m_cnt, (const uint32_t*)c->d_cnt, 16, c->s_compute));
     // the trace comes back with the counters, so gw_trace_read needs no device round trip
     const bool traced = a.block_steps != nullptr;
-    if (traced) HIPCHK(c, launch_copy_u32((uint32_t*)c->m_trace, (const uint32_t*)c->d_trace, 2ull * c->nblocks, c->s_compute));
-    c->trace_valid = false;
     const bool buckets = (l.flags & GW_RUN_BUCKETS) != 0;
-    if (buckets) {
+    c->trace_valid = false;
+    // the run's device work: the block table push, counter clears, filter, step kernel,
+    // readbacks and the bucket pass -- launched one by one, or as a graph captured the
+    // first time this exact sequence (every kernel argument equal) runs and replayed after
+    const bool graph = c->run_graphs && !cap;
+    auto enqueue = [&]() -> gw_status {
+        HIPCHK(c, launch_copy_u32((uint32_t*)c->d_blocks, (const uint32_t*)c->m_blocks,
+                                  sizeof(DevBlock) / 4 * c->nblocks, c->s_compute));
+        HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
+        HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
+        if (!all_res) HIPCHK(c, launch_filter(f, c->ncu * 2, c->s_compute));
+        if (traced) HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
+        HIPCHK(c, graph ? launch_timestamp(c->m_ts, c->s_compute) : hipEventRecord(c->ev_k0, c->s_compute));
+        HIPCHK(c, launch_walk(a, one_blk, l.variant, (uint32_t)grid, l.block, c->s_compute));
+        HIPCHK(c, graph ? launch_timestamp(c->m_ts + 1, c->s_compute) : hipEventRecord(c->ev_k1, c->s_compute));
+        HIPCHK(c, launch_copy_u32((uint32_t*)c->m_cnt, (const uint32_t*)c->d_cnt, 16, c->s_compute));
+        if (traced)
+            HIPCHK(c, launch_copy_u32((uint32_t*)c->m_trace, (const uint32_t*)c->d_trace, 2ull * c->nblocks, c->s_compute));
         // bucket matrix of the walkers left, read back with the counters (one host wait)
-        if ((st = enqueue_bucket_pass(c)) != GW_OK) return st;
+        if (buckets) return enqueue_bucket_pass(c, graph);
+        return GW_OK;
+    };
+    if (graph) {
+        RunSig sig;
+        memset(&sig, 0, sizeof sig);
+        sig.a = a;
+        sig.f = all_res ? FilterArgs{} : f;
+        sig.grid = grid;
+        sig.block = l.block;
+        sig.variant = l.variant;
+        sig.flags = (all_res ? 1u : 0u) | (traced ? 2u : 0u) | (buckets ? 4u : 0u) | (one_blk ? 8u : 0u);
+        sig.ncu = c->ncu;
+        sig.d_bucket = c->d_bucket;
+        sig.m_cnt = c->m_cnt;
+        sig.m_trace = c->m_trace;
+        sig.m_bucket = c->m_bucket;
+        sig.m_blocks = c->m_blocks;
+        hipGraphExec_t ex = graph_lookup(c, sig);
+        if (!ex) {
+            HIPCHK(c, hipStreamBeginCapture(c->s_compute, hipStreamCaptureModeThreadLocal));
+            const gw_status est = enqueue();
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(c->s_compute, &g);
+            if (est != GW_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return est;
+            }
+            if (ce != hipSuccess) return fail(c, GW_ERR_HIP, "run graph capture: %s", hipGetErrorString(ce));
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return fail(c, GW_ERR_HIP, "run graph instantiation: %s", hipGetErrorString(ie));
+            graph_store(c, sig, ex);
+        }
+        HIPCHK(c, hipGraphLaunch(ex, c->s_compute));
+        c->graph_runs++;
+    } else {
+        if ((st = enqueue()) != GW_OK) return st;
     }
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
     c->trace_valid = traced;
-    HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
-    if (buckets) {
-        HIPCHK(c, hipEventElapsedTime(&fms, c->ev_f0, c->ev_f1));
-        c->w_valid = true;
+    if (graph) {
+        kms = (float)((double)(c->h_ts[1] - c->h_ts[0]) / c->ts_khz);
+        if (buckets) fms = (float)((double)(c->h_ts[3] - c->h_ts[2]) / c->ts_khz);
+    } else {
+        HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
+        if (buckets) HIPCHK(c, hipEventElapsedTime(&fms, c->ev_f0, c->ev_f1));
     }
+    if (buckets) c->w_valid = true;
     for (uint32_t sl = 0; sl < c->nslots; ++sl) collect_copy_time(c, sl, false);
     gw_run_stats s{};
     s.steps = c->h_cnt[0];

@@ -42,6 +42,7 @@ struct DevType {
     // bounds of the band in which membership decides (min and max of t_one, t_q) and the
     // SOPR proposal's N(cur) threshold (alpha_b << 16) - 1, for 32-bit compares
     uint32_t m_p, m_one, m_q, m_min, m_lo, m_hi, m_prop;
+    uint32_t pad0;                     // no implicit padding: run arguments are compared bytewise
     uint64_t t_term;                   // floor(term * 2^32), 0 = no geometric termination
     uint32_t algo;                     // 0 node2vec, 1 SOPR/SOSR
     uint32_t length;
@@ -126,6 +127,7 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
                        uint32_t block, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s);
+hipError_t launch_timestamp(unsigned long long* dst, hipStream_t s);   // device clock into dst
 // Packed blocks (gw_pack.hip: a header, beg chunks of bit-packed differences, arc chunks of
 // Elias-Fano-coded gap sums).  Word offsets of the parts, from the block's first word.
 constexpr uint32_t kPackChunkArcs = 256;

@@ -1758,6 +1758,19 @@ hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipSt
     return hipGetLastError();
 }
 
+// The device's constant-rate clock into dst: kernel timing inside captured graphs, whose
+// event records cannot be timed.
+__global__ void timestamp_kernel(unsigned long long* dst)
+{
+    if (threadIdx.x == 0) *dst = wall_clock64();
+}
+
+hipError_t launch_timestamp(unsigned long long* dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(timestamp_kernel, dim3(1), dim3(64), 0, s, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s)
 {
     if (n == 0) return hipSuccess;

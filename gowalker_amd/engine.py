@@ -279,6 +279,12 @@ class Context:
         self._chk(lib().gw_read_paths(self.h, _p(out), out.size))
         return out.reshape(self.nwalkers, L + 1)
 
+    def graph_stats(self):
+        """(runs replayed from captured graphs, graphs captured)"""
+        r, c = C.c_uint64(), C.c_uint64()
+        self._chk(lib().gw_graph_stats(self.h, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
     def counters(self):
         c = GwCounters()
         self._chk(lib().gw_counters_get(self.h, C.byref(c)))

@@ -329,6 +329,15 @@ gw_status gw_read_hashes(gw_ctx* ctx, uint64_t* out, uint64_t n);
 gw_status gw_read_state(gw_ctx* ctx, uint32_t* out4, uint64_t n);
 gw_status gw_read_paths(gw_ctx* ctx, uint32_t* out, uint64_t n);
 gw_status gw_counters_get(gw_ctx* ctx, gw_counters* out);
+/* Runs replayed from captured graphs and graphs captured since gw_create (no reference
+ * counterpart: the reference launches every kernel of an iteration from the host,
+ * engine/engine.hpp:406-448).  With GW_RUN_GRAPH=1 in the environment at gw_create, a
+ * run's device work -- block table push, counter clears, filter pass, step kernel,
+ * readbacks, bucket pass -- is captured as a HIP graph the first time its exact launch
+ * sequence occurs and replayed when it recurs, and kernel_ms / filter_ms come from the
+ * device clock (events in a graph cannot be timed); by default every kernel is launched
+ * one by one and timed with HIP events. */
+gw_status gw_graph_stats(gw_ctx* ctx, uint64_t* replays, uint64_t* captures);
 
 /* ---- synthetic graphs generated in HBM (the reference downloads real datasets,
  *      script/download_*.sh; its text->CSR converter cannot run as shipped,
