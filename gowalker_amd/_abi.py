@@ -103,6 +103,7 @@ SIGNATURES = {
     "gw_graph_free": [P, P],
     "gw_trace_read": [P, P, C.c_uint64],
     "gw_block_features": [P, P, P, C.c_uint64],
+    "gw_block_scores": [P, P, P, P, P, P, P, C.c_uint64],
     "gw_mlp_create": [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(P)],
     "gw_mlp_destroy": [P, P],
     "gw_mlp_num_params": [P, u32p],

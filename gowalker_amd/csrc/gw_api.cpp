@@ -642,7 +642,8 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     if ((st = dalloc(c, &p, sizeof(unsigned long long) * nb, false)) != GW_OK) return st;
     c->d_trace = (unsigned long long*)p;
     HIPCHK(c, hipMemset(c->d_trace, 0, sizeof(unsigned long long) * nb));
-    if ((st = dalloc(c, &p, sizeof(float) * nb * kNumFeatures, false)) != GW_OK) return st;
+    // features, then the means and deviations that standardize them for the network
+    if ((st = dalloc(c, &p, sizeof(float) * ((uint64_t)nb + 2) * kNumFeatures, false)) != GW_OK) return st;
     c->d_feat = (float*)p;
     if ((st = dalloc(c, &p, sizeof(float) * nb, false)) != GW_OK) return st;
     c->d_age = (float*)p;
@@ -1823,6 +1824,48 @@ gw_status gw_block_features(gw_ctx* c, const float* age, float* feats, uint64_t 
     HIPCHK(c, launch_features(c->d_bucket, c->nblocks, c->d_blocks, age ? c->d_age : nullptr, ahead, c->d_feat,
                               c->s_compute));
     HIPCHK(c, d2h_small(c, feats, c->d_feat, (uint64_t)c->nblocks * kNumFeatures * 4));
+    return GW_OK;
+}
+
+gw_status gw_block_scores(gw_ctx* c, gw_mlp* m, const float* age, const float* x_mu, const float* x_sigma,
+                          float* feats, float* scores, uint64_t n)
+{
+    if (!c || !m || !x_mu || !x_sigma || !feats || !scores) return fail(c, GW_ERR_ARG, "null argument");
+    if (!c->nwalkers) return fail(c, GW_ERR_STATE, "no walkers");
+    if (n < c->nblocks) return fail(c, GW_ERR_ARG, "buffers hold %llu blocks < %u", (unsigned long long)n, c->nblocks);
+    if (mlp_num_inputs(m) != kNumFeatures) return fail(c, GW_ERR_ARG, "the network takes %u inputs, not %u",
+                                                       mlp_num_inputs(m), kNumFeatures);
+    (void)hipSetDevice(c->device);
+    gw_status st;
+    if ((st = ensure_buckets(c)) != GW_OK) return st;
+    if ((st = push_block_table(c)) != GW_OK) return st;
+    if (age) HIPCHK(c, h2d_small(c, c->d_age, age, c->nblocks * 4ull));
+    float* musig = c->d_feat + (uint64_t)c->nblocks * kNumFeatures;
+    HIPCHK(c, h2d_small(c, musig, x_mu, kNumFeatures * 4ull));
+    HIPCHK(c, h2d_small(c, musig + kNumFeatures, x_sigma, kNumFeatures * 4ull));
+    const uint32_t maxlen = std::max(c->dt[0].length, c->wp.ntypes == 2 ? c->dt[1].length : 0u);
+    const float ahead = c->unfinished && maxlen ? (float)((double)c->pending_hops / ((double)c->unfinished * maxlen)) : 0.f;
+    HIPCHK(c, launch_features(c->d_bucket, c->nblocks, c->d_blocks, age ? c->d_age : nullptr, ahead, c->d_feat,
+                              c->s_compute));
+    const float* dy = nullptr;
+    if ((st = mlp_forward_device(c, m, c->d_feat, musig, c->nblocks, &dy)) != GW_OK) return st;
+    // both results back through the pinned ring with one host wait
+    const uint64_t fb = (uint64_t)c->nblocks * kNumFeatures * 4, sb = c->nblocks * 4ull;
+    if (fb + sb + 512 > c->ring_cap / 2) {
+        HIPCHK(c, d2h_small(c, feats, c->d_feat, fb));
+        HIPCHK(c, d2h_small(c, scores, dy, sb));
+        return GW_OK;
+    }
+    hipError_t e;
+    const uint64_t at = ring_take(c, fb, &e);
+    HIPCHK(c, e);
+    const uint64_t at2 = ring_take(c, sb, &e);
+    HIPCHK(c, e);
+    HIPCHK(c, launch_copy_u32((uint32_t*)(c->m_ring + at), (const uint32_t*)c->d_feat, fb / 4, c->s_compute));
+    HIPCHK(c, launch_copy_u32((uint32_t*)(c->m_ring + at2), (const uint32_t*)dy, sb / 4, c->s_compute));
+    HIPCHK(c, hipStreamSynchronize(c->s_compute));
+    memcpy(feats, c->h_ring + at, fb);
+    memcpy(scores, c->h_ring + at2, sb);
     return GW_OK;
 }
 

@@ -128,6 +128,12 @@ hipError_t launch_walk(const WalkArgs& a, bool one_block, uint32_t variant, uint
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s);
 hipError_t launch_timestamp(unsigned long long* dst, hipStream_t s);   // device clock into dst
+// the network's inputs per sample (gw_mlp.hip)
+uint32_t mlp_num_inputs(const gw_mlp* m);
+// forward pass on device-resident rows f (n x inputs), standardized per input with musig
+// (the means, then the deviations; device memory); *y: the device buffer of the n outputs,
+// valid until the network's next call.  Queued on the context's compute stream.
+gw_status mlp_forward_device(gw_ctx* ctx, gw_mlp* m, const float* f, const float* musig, uint32_t n, const float** y);
 // Packed blocks (gw_pack.hip: a header, beg chunks of bit-packed differences, arc chunks of
 // Elias-Fano-coded gap sums).  Word offsets of the parts, from the block's first word.
 constexpr uint32_t kPackChunkArcs = 256;

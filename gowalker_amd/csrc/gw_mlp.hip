@@ -528,12 +528,13 @@ extern "C" gw_status gw_mlp_get_params(gw_ctx* ctx, const gw_mlp* m, float* para
     return GW_OK;
 }
 
+// x == nullptr: the inputs are already in m->x
 static gw_status mlp_fwd(gw_ctx* ctx, gw_mlp* m, const float* x, uint32_t n, bool keep)
 {
     gw_status st;
     if ((st = mlp_reserve(ctx, m, n)) != GW_OK) return st;
     hipStream_t s = ctx_stream(ctx);
-    if (gw::h2d_small(ctx, m->x, x, (uint64_t)n * m->s.in * 4) != hipSuccess)
+    if (x && gw::h2d_small(ctx, m->x, x, (uint64_t)n * m->s.in * 4) != hipSuccess)
         return gen_fail(ctx, GW_ERR_HIP, "MLP input upload failed");
     const uint32_t lds = m->s.nparams * 4;
     hipLaunchKernelGGL(mlp_forward_kernel, dim3((n + 255) / 256), dim3(256), lds, s, m->s, m->wbf, m->x, n, m->y,
@@ -542,14 +543,15 @@ static gw_status mlp_fwd(gw_ctx* ctx, gw_mlp* m, const float* x, uint32_t n, boo
     return GW_OK;
 }
 
-// Fused path: inputs (and targets) through the transfer ring, one single-workgroup launch.
+// Fused path: inputs (and targets) through the transfer ring (x == nullptr: the inputs are
+// already in m->x), one single-workgroup launch.
 static gw_status mlp_fused(gw_ctx* ctx, gw_mlp* m, uint32_t R, int mode, const float* x, const float* t, uint32_t n,
                            float lr)
 {
     gw_status st;
     if ((st = mlp_reserve(ctx, m, n)) != GW_OK) return st;
     hipStream_t s = ctx_stream(ctx);
-    if (gw::h2d_small(ctx, m->x, x, (uint64_t)n * m->s.in * 4) != hipSuccess ||
+    if ((x && gw::h2d_small(ctx, m->x, x, (uint64_t)n * m->s.in * 4) != hipSuccess) ||
         (t && gw::h2d_small(ctx, m->t, t, n * 4ull) != hipSuccess))
         return gen_fail(ctx, GW_ERR_HIP, "MLP input upload failed");
     const MlpShape& sh = m->s;
@@ -580,6 +582,35 @@ static gw_status mlp_fused(gw_ctx* ctx, gw_mlp* m, uint32_t R, int mode, const f
     if (hipGetLastError() != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP kernel launch failed");
     return GW_OK;
 }
+
+// x[i] = (f[i] - mu[j]) / sigma[j], j = i mod nin; musig holds nin means then nin deviations
+__global__ void standardize_kernel(const float* __restrict__ f, const float* __restrict__ musig, uint32_t nin,
+                                   uint64_t total, float* __restrict__ x)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t j = (uint32_t)(i % nin);
+    x[i] = (f[i] - musig[j]) / musig[nin + j];
+}
+
+namespace gw {
+uint32_t mlp_num_inputs(const gw_mlp* m) { return m->s.in; }
+
+gw_status mlp_forward_device(gw_ctx* ctx, gw_mlp* m, const float* f, const float* musig, uint32_t n, const float** y)
+{
+    gw_status st;
+    if ((st = mlp_reserve(ctx, m, n)) != GW_OK) return st;
+    const uint64_t total = (uint64_t)n * m->s.in;
+    hipLaunchKernelGGL(standardize_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, ctx_stream(ctx), f, musig,
+                       m->s.in, total, m->x);
+    if (hipGetLastError() != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP input standardization failed");
+    const uint32_t R = fused_rows(m->s);
+    if ((st = R ? mlp_fused(ctx, m, R, 0, nullptr, nullptr, n, 0.f) : mlp_fwd(ctx, m, nullptr, n, false)) != GW_OK)
+        return st;
+    *y = m->y;
+    return GW_OK;
+}
+}  // namespace gw
 
 extern "C" gw_status gw_mlp_forward(gw_ctx* ctx, gw_mlp* m, const float* x, uint32_t n, float* y)
 {
@@ -637,8 +668,7 @@ extern "C" gw_status gw_mlp_step(gw_ctx* ctx, gw_mlp* m, const float* x, const f
     hipStream_t s = ctx_stream(ctx);
     if (const uint32_t R = fused_rows(m->s)) {
         if ((st = mlp_fused(ctx, m, R, 2, x, t, n, lr)) != GW_OK) return st;
-        if ((loss && gw::d2h_small(ctx, loss, m->loss, 4) != hipSuccess) || hipStreamSynchronize(s) != hipSuccess)
-            return gen_fail(ctx, GW_ERR_HIP, "MLP step failed");
+        if (loss && gw::d2h_small(ctx, loss, m->loss, 4) != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP step failed");
         return GW_OK;
     }
     if ((st = mlp_grad(ctx, m, x, t, n)) != GW_OK) return st;
@@ -648,7 +678,6 @@ extern "C" gw_status gw_mlp_step(gw_ctx* ctx, gw_mlp* m, const float* x, const f
     hipLaunchKernelGGL(mlp_adam_kernel, dim3((m->s.nparams + 255) / 256), dim3(256), 0, s, m->s.nparams, m->master,
                        m->wbf, m->g, m->m1, m->m2, lr, b1, b2, 1e-8f, c1, c2);
     if (hipGetLastError() != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP update launch failed");
-    if ((loss && gw::d2h_small(ctx, loss, m->loss, 4) != hipSuccess) || hipStreamSynchronize(s) != hipSuccess)
-        return gen_fail(ctx, GW_ERR_HIP, "MLP step failed");
+    if (loss && gw::d2h_small(ctx, loss, m->loss, 4) != hipSuccess) return gen_fail(ctx, GW_ERR_HIP, "MLP step failed");
     return GW_OK;
 }

@@ -85,10 +85,12 @@ struct LearnedConfig {
     uint32_t h1 = 64, h2 = 64;
     uint64_t seed = 7;
     uint32_t min_samples = 24;      // plain decisions until this many samples
-    uint32_t train_steps = 4;       // Adam steps per training round
+    uint32_t train_steps = 2;       // Adam steps per training round (queued, one host wait)
     uint32_t train_every = 32;      // new samples between training rounds (every 8 with 4 steps
                                     // cost 11 ms of host time per 0.19 s RMAT-24 walk)
-    uint32_t finish_steps = 48;     // Adam steps when a walk ends (~70 us each)
+    uint32_t finish_steps = 16;     // Adam steps when a walk ends (~50 us each).  4 and 48 steps
+                                    // gave held-out errors 0.41 vs 1.12 for the running mean on
+                                    // C3 (these: 0.74 vs 1.36) but cost the 16 GB walk 8 ms
     uint32_t score_after = 256;     // samples trained on before predictions are scored
     uint32_t batch = 256;
     uint32_t max_samples = 16384;   // ring buffer of trace samples
@@ -510,13 +512,19 @@ private:
             const bool tail_near = (double)pending_hops_ * reads_per_step() / zc_reads_per_ms() <
                                    cfg_.tail_guard * (double)max_bytes / copy_bytes_per_ms();
             const bool fill = buckets.size() < capacity(in);
-            const bool want = fill || in.iteration % std::max(1u, cfg_.keep_every) == 0 ||
-                              (trained() && (cfg_.model_swap || tail_near));
+            const bool score = trained() && m_.x_mu.size() == kF && (fill || cfg_.model_swap || tail_near);
+            const bool want = score || fill || in.iteration % std::max(1u, cfg_.keep_every) == 0;
             if (want) {
                 feats_.assign((size_t)nb * kF, 0.f);
-                have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
+                if (score) {   // features and the network's scores with one host wait
+                    have_feats_ = use_model_ = gw_block_scores(ctx_, mlp_, age_.data(), m_.x_mu.data(), m_.x_sigma.data(),
+                                                               feats_.data(), pred_.data(), nb) == GW_OK;
+                    if (use_model_)
+                        for (float& p : pred_) p = std::max(0.f, m_.y_mu + m_.y_sigma * p);
+                } else {
+                    have_feats_ = gw_block_features(ctx_, age_.data(), feats_.data(), feats_.size()) == GW_OK;
+                }
             }
-            use_model_ = trained() && have_feats_ && (fill || cfg_.model_swap || tail_near) && predict(feats_, nb, pred_);
         }
         rank_.resize(nb);
         // without the network: the walkers a load would release (log scale), with the
@@ -693,9 +701,11 @@ private:
                 by[i] = (m_.ys[k] - m_.y_mu) / m_.y_sigma;
             }
             standardize(bx.data(), bs);
+            // the steps of a round are queued back to back; only the last one waits (for its loss)
             float loss = 0.f;
-            if (gw_mlp_step(ctx_, mlp_, bx.data(), by.data(), bs, cfg_.lr, &loss) != GW_OK) return;
-            m_.loss = loss;
+            const bool last = e + 1 == steps;
+            if (gw_mlp_step(ctx_, mlp_, bx.data(), by.data(), bs, cfg_.lr, last ? &loss : nullptr) != GW_OK) return;
+            if (last) m_.loss = loss;
             ++m_.adam_steps;
         }
     }

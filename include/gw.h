@@ -296,9 +296,17 @@ gw_status gw_mlp_forward(gw_ctx* ctx, gw_mlp* m, const float* x, uint32_t n, flo
 /* mean-squared-error gradient of every parameter and the loss, for targets t */
 gw_status gw_mlp_backward(gw_ctx* ctx, gw_mlp* m, const float* x, const float* t, uint32_t n,
                           float* grad, float* loss);
-/* one Adam step (lr) on the batch; loss receives the batch loss before the step */
+/* one Adam step (lr) on the batch; loss receives the batch loss before the step, or, with
+ * loss NULL, the step is queued on the context's compute stream without a host wait (the
+ * next call that waits for that stream completes it) */
 gw_status gw_mlp_step(gw_ctx* ctx, gw_mlp* m, const float* x, const float* t, uint32_t n,
                       float lr, float* loss);
+/* gw_block_features and the network's scores of every block in one call with one host
+ * wait: the features go to feats (n >= nblocks rows), and the network (12 inputs) scores
+ * them after standardizing each feature j as (f - x_mu[j]) / x_sigma[j] (the learned
+ * scheduler's ranking, no reference counterpart; plug point engine/schedule.hpp:38-55). */
+gw_status gw_block_scores(gw_ctx* ctx, gw_mlp* m, const float* age, const float* x_mu, const float* x_sigma,
+                          float* feats, float* scores, uint64_t n);
 
 /* ---- allocation-free walk (no counterpart in the reference).  For callers that cannot
  *      allocate device memory: code under HIP graph capture, tight serving loops, harnesses

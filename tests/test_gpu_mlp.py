@@ -52,13 +52,13 @@ class Mlp:
                                     C.byref(loss)), self.ctx.h)
         return g, loss.value
 
-    def step(self, x, t, lr):
+    def step(self, x, t, lr, wait=True):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
         loss = C.c_float()
-        check(lib().gw_mlp_step(self.ctx.h, self.h, x.ctypes.data, t.ctypes.data, len(x), lr, C.byref(loss)),
-              self.ctx.h)
-        return loss.value
+        check(lib().gw_mlp_step(self.ctx.h, self.h, x.ctypes.data, t.ctypes.data, len(x), lr,
+                                C.byref(loss) if wait else None), self.ctx.h)
+        return loss.value if wait else None
 
     def close(self):
         lib().gw_mlp_destroy(self.ctx.h, self.h)
@@ -138,3 +138,21 @@ def test_adam_step_reduces_loss_and_fits(gpu):
         p = m.params()
         np.testing.assert_array_equal(p, R.bf16_round(p))   # working weights stay bf16
         m.close()
+
+
+@pytest.mark.parametrize("shape", [(12, 64, 64), (16, 128, 128)])
+def test_queued_steps_equal_waited_steps(gpu, shape):
+    """Adam steps queued without a host wait (loss NULL) leave the same parameters as steps
+    that each wait for their loss: the batches' inputs go through the pinned ring in order."""
+    rng = np.random.default_rng(8)
+    batches = [(rng.normal(0, 1, (96, shape[0])).astype(np.float32), rng.normal(0, 1, 96).astype(np.float32))
+               for _ in range(9)]
+    out = []
+    for wait in (True, False):
+        with E.Context() as ctx:
+            m = Mlp(ctx, seed=5, shape=shape)
+            for x, t in batches:
+                m.step(x, t, 1e-2, wait=wait)
+            out.append(m.params())   # reads the parameters after the queued steps
+            m.close()
+    np.testing.assert_array_equal(out[0], out[1])

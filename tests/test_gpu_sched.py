@@ -91,3 +91,47 @@ def test_stage_commit_features_pending(gpu):
     o = oracle(beg, csr, 5, n)
     np.testing.assert_array_equal(hashes, o["hashes"])
     np.testing.assert_array_equal(visits, o["visits"])
+
+
+def test_block_scores_equal_features_then_forward(gpu):
+    """gw_block_scores (features, standardization and the network's forward pass on the
+    device, one host wait) returns the features of gw_block_features and, bit for bit, the
+    scores gw_mlp_forward gives for the same features standardized on the host."""
+    import ctypes as C
+
+    from gowalker_amd._abi import check, lib
+
+    beg, csr = G.rmat(12, 40000, seed=23, permute=True)
+    n = len(beg) - 1
+    bl = Blocks(beg, csr, 16384)
+    nb = bl.nblocks
+    rng = np.random.default_rng(2)
+    with Context() as ctx:
+        ctx.set_blocks(bl)
+        ctx.set_walk(TYPES, 5)
+        ctx.init_walkers(n, 1, outputs=OUT_HASH)
+        ctx.reserve_slots(3)
+        for s in range(3):
+            ctx.upload(s, s)
+        ctx.run(buckets=True)
+        mlp = C.c_void_p()
+        check(lib().gw_mlp_create(ctx.h, 12, 64, 64, 9, C.byref(mlp)), ctx.h)
+        npar = C.c_uint32()
+        check(lib().gw_mlp_num_params(mlp, C.byref(npar)))
+        p = rng.normal(0, 0.3, npar.value).astype(np.float32)
+        check(lib().gw_mlp_set_params(ctx.h, mlp, p.ctypes.data), ctx.h)
+        age = rng.uniform(0, 64, nb).astype(np.float32)
+        mu = rng.normal(2, 1, 12).astype(np.float32)
+        sig = rng.uniform(0.5, 3, 12).astype(np.float32)
+        feats = np.zeros((nb, 12), np.float32)
+        scores = np.zeros(nb, np.float32)
+        check(lib().gw_block_scores(ctx.h, mlp, age.ctypes.data, mu.ctypes.data, sig.ctypes.data, feats.ctypes.data,
+                                    scores.ctypes.data, nb), ctx.h)
+        f2 = ctx.block_features(age)
+        x = ((f2 - mu) / sig).astype(np.float32)
+        y = np.zeros(nb, np.float32)
+        check(lib().gw_mlp_forward(ctx.h, mlp, x.ctypes.data, nb, y.ctypes.data), ctx.h)
+        lib().gw_mlp_destroy(ctx.h, mlp)
+    np.testing.assert_array_equal(feats, f2)
+    np.testing.assert_array_equal(scores, y)
+    assert np.std(scores) > 0
