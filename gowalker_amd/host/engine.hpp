@@ -237,6 +237,15 @@ public:
             for (uint32_t s = 0; s < nslots_; ++s) ck(gw_evict(ctx_, s), ctx_, "gw_evict");
         }
         r.nslots = nslots_;
+        if (cfg_.verbose) {
+            gw_counters hc;
+            uint64_t per = 0;
+            ck(gw_counters_get(ctx_, &hc), ctx_, "gw_counters_get");
+            ck(gw_slot_bytes(ctx_, &per), ctx_, "gw_slot_bytes");
+            std::fprintf(stderr, "[engine] budget %.3f GB: %u slots of %.1f MB, %.3f GB held, %.1f MB free\n",
+                         cfg_.memory_bytes / 1e9, nslots_, per / 1e6, hc.hbm_bytes_held / 1e9,
+                         cfg_.memory_bytes > hc.hbm_bytes_held ? (cfg_.memory_bytes - hc.hbm_bytes_held) / 1e6 : 0.0);
+        }
         slots_.reset(nslots_, nb);
         sched.buckets.clear();
         ck(gw_counters_get(ctx_, &c0), ctx_, "gw_counters_get");
