@@ -231,19 +231,23 @@ def summarize(reps, zt):
                 swap_model=r0["swap_model"], swap_differs=r0["swap_differs"], tail_model=r0["tail_model"],
                 tail_threshold=r0["tail_threshold"], model_samples=r0["model_samples"], fallbacks=r0["fallbacks"],
                 nslots=r0["nslots"], nblocks=r0["nblocks"], steps=r0["steps"], trials=r0["trials"],
-                probes=r0["probes"], zero_threshold=zt, launch=r0["tuned"], copy_lanes=r0["copy_lanes"])
+                probes=r0["probes"], zero_threshold=zt, launch=r0["tuned"], copy_lanes=r0["copy_lanes"],
+                decoded_in_place=r0["inplace_blocks"], decoded_ring=r0["ring_blocks"],
+                decoded_copy_stream=r0["inline_blocks"])
 
 
 def log_walk(cap_gb, name, tag, rep):
     print("[bench] cap %g GB %s %s: %.3f s, %d steps, %.2f GB H2D (%.2f GB raw; %d uploads + %d staged), %.3f GB zero-copy, "
           "%d iterations, kernel %.0f ms, copy %.0f ms, sched %.0f ms, %d slots / %d blocks, model: fill %d swap %d "
-          "(%d differ) tail %d/%d samples %d, sticky %d, launch %s, copy lanes %d"
+          "(%d differ) tail %d/%d samples %d, sticky %d, launch %s, copy lanes %d, decoded in place/ring/copy stream "
+          "%d/%d/%d"
           % (cap_gb, name, tag, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["h2d_raw_bytes"] / 1e9,
              rep["uploads"], rep["staged"],
              rep["zc_touches"] * 64 / 1e9, rep["iterations"], rep["kernel_ms"], rep["h2d_ms"], rep["sched_ms"],
              rep["nslots"], rep["nblocks"], rep["fill_model"], rep["swap_model"], rep["swap_differs"],
              rep["tail_model"], rep["tail_threshold"], rep["model_samples"], rep["sticky_swaps"],
-             "%(block)d/%(waves_per_cu)d/v%(variant)d" % rep["tuned"] if rep["tuned"] else "-", rep["copy_lanes"]),
+             "%(block)d/%(waves_per_cu)d/v%(variant)d" % rep["tuned"] if rep["tuned"] else "-", rep["copy_lanes"],
+             rep["inplace_blocks"], rep["ring_blocks"], rep["inline_blocks"]),
           file=sys.stderr, flush=True)
 
 

@@ -94,6 +94,7 @@ SIGNATURES = {
     "gw_read_paths": [P, P, C.c_uint64],
     "gw_counters_get": [P, C.POINTER(GwCounters)],
     "gw_graph_stats": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "gw_upload_stats": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "gw_split_blocks": [P, C.c_uint64, C.c_uint64, P, P, C.c_uint64, u64p],
     "gw_gen_rmat": [P, C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
                     C.POINTER(P)],
@@ -180,7 +181,8 @@ class GwhReport(C.Structure):
                 ("tail_model", C.c_uint64), ("tail_threshold", C.c_uint64), ("swap_model", C.c_uint64),
                 ("swap_differs", C.c_uint64), ("fill_model", C.c_uint64), ("model_samples", C.c_uint64),
                 ("sticky_swaps", C.c_uint64), ("tune_candidate", C.c_uint32), ("copy_lanes", C.c_uint32),
-                ("h2d_raw_bytes", C.c_uint64)]
+                ("h2d_raw_bytes", C.c_uint64), ("inplace_blocks", C.c_uint64), ("ring_blocks", C.c_uint64),
+                ("inline_blocks", C.c_uint64)]
 
 
 class GwhTuneEntry(C.Structure):

@@ -87,6 +87,8 @@ struct gw_ctx {
     // and blocks whose headers do not fit).
     hipStream_t s_dec = nullptr;
     bool inline_decode = false;
+    bool no_inplace = false;        // GW_PACK_INPLACE=0: every payload through the ring
+    uint64_t inplace_blocks = 0, ring_blocks = 0, inline_blocks = 0;   // packed blocks by decode path
     struct PkRegion {
         uint64_t a, b;        // words [a, b) of the ring
         hipEvent_t freed;     // recorded on s_dec after the decode that reads them
@@ -492,6 +494,7 @@ gw_status gw_create(const gw_config* cfg, gw_ctx** out)
         hipStreamCreateWithFlags(&c->s_dec, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(nullptr, GW_ERR_HIP, "stream creation failed"));
     if (const char* f = getenv("GW_PACK_INLINE_DECODE")) c->inline_decode = atoi(f) != 0;
+    if (const char* f = getenv("GW_PACK_INPLACE")) c->no_inplace = atoi(f) == 0;
     if (const char* f = getenv("GW_UPLOAD_THREAD")) c->sync_uploads = atoi(f) == 0;
     if (const char* f = getenv("GW_RUN_GRAPH")) c->run_graphs = atoi(f) != 0;
     {
@@ -663,12 +666,18 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
 // (the window search reads aligned 64-byte pieces); 64 B of head room and 64 B of tail
 // pad.  gw_slot_bytes and gw_reserve_slots both take their sizes from here, so the bytes a
 // caller budgets per slot are the bytes a reservation charges.
+// A slot: beg of the largest vertex count, and the edge words of the largest block with 64 B
+// of head room and tail pad each, plus slack for decoding a packed block in place (its
+// payload sits at the buffer's end while the decoder writes from the start; with no slack
+// the last chunk's output would cover its own input): 1/64 of the edge words, between 4 KB
+// and 1 MB (C3: 54 MB over 54 slots; a largest block then decodes in ~8 passes).
 static void slot_sizes(const gw_ctx* c, uint64_t* beg_bytes, uint64_t* csr_bytes)
 {
     uint64_t mb = 0, me = 0;
     for (auto& d : c->desc) { mb = std::max<uint64_t>(mb, (uint64_t)d.nverts + 1); me = std::max<uint64_t>(me, d.nedges); }
+    const uint64_t slack = std::min<uint64_t>(1ull << 20, std::max<uint64_t>(4096, me * 4 / 64));
     *beg_bytes = (mb * 8 + 255) & ~255ULL;
-    *csr_bytes = (me * 4 + 128 + 255) & ~255ULL;
+    *csr_bytes = (me * 4 + 128 + slack + 255) & ~255ULL;
 }
 
 gw_status gw_slot_bytes(gw_ctx* c, uint64_t* bytes)
@@ -740,11 +749,40 @@ static uint32_t* slot_edges(gw_ctx* c, uint32_t slot, uint32_t blk)
     return c->slot_csr[slot] + (c->desc[blk].start_edge & 15);
 }
 
-// Staging ring of packed uploads (one copy lane): two head areas of this size at the front
-// of the staging buffer, the arc-payload ring behind them, which must hold at least
-// kPkMinRing words (pieces of half the ring hold at least the largest chunk's 272 words).
-constexpr uint64_t kPkMinRing = 2048;
-static uint64_t pk_head_bytes(const gw_ctx* c) { return (c->pack_cap / 8) & ~255ULL; }
+// Staging buffer of packed uploads (one copy lane): two head areas of a quarter each at its
+// front, the arc-payload ring over the other half, which must hold at least kPkMinRing
+// words (pieces of half the ring hold at least the largest chunk's 272 words).
+constexpr uint64_t kPkMinRing = 1024;
+static uint64_t pk_head_bytes(const gw_ctx* c) { return (c->pack_cap / 4) & ~255ULL; }
+
+// In-place decode of a block's arc payload, copied whole to the end of the slot's edge
+// buffer at byte pstart: the decoder writes the block's edge words from byte E of that buffer
+// up, 1024 bytes per 256-arc chunk, while the payload of chunk c starts at byte
+// pstart + 4 off(c).  A pass decodes chunks [c0, c1) whose outputs all end at or below the
+// first payload byte not yet read (chunk c0's), so no pass overwrites input that it or a
+// later pass still reads; passes run in order on one stream.  Outputs grow by 1024 bytes per
+// chunk and inputs by ~560 (C3), so a block whose payload fits below its slot's slack
+// decodes in a handful of passes.  Fills cuts with c0 = 0, ..., nch; false when a pass
+// would make no progress or more than kMaxPasses are needed.
+constexpr size_t kMaxPasses = 48;
+static bool inplace_plan(const uint32_t* ah, uint64_t nch, uint64_t ne, uint64_t total_w, uint64_t pstart, uint64_t E,
+                         std::vector<uint64_t>& cuts)
+{
+    auto off = [&](uint64_t k) { return k < nch ? (uint64_t)ah[k * 4] : total_w; };
+    cuts.assign(1, 0);
+    uint64_t c0 = 0;
+    while (c0 < nch) {
+        const uint64_t lim = pstart + 4 * off(c0);   // first payload byte not yet read
+        if (lim < E) return false;
+        // the chunks whose outputs end by lim: whole chunks of 1024 bytes, or all of them
+        // once the last (possibly partial) chunk's end, E + 4 ne, fits
+        const uint64_t c1 = E + 4 * ne <= lim ? nch : std::min<uint64_t>(nch, (lim - E) / 1024);
+        if (c1 <= c0 || cuts.size() > kMaxPasses) return false;
+        cuts.push_back(c1);
+        c0 = c1;
+    }
+    return true;
+}
 
 // An event for the ring: from the pool, or the oldest retired one once its recorded work
 // has completed (a stream wait enqueued on it has then been satisfied), or a new one.
@@ -812,10 +850,11 @@ static gw_status pk_signal(gw_ctx* c, hipStream_t from, hipStream_t to)
     return GW_OK;
 }
 
-// A packed block through the staging ring: its headers into the next head area and its
-// arc payload in pieces of half the ring on s_copy; beg and each piece decoded into
-// the slot on s_dec as their copies land.  The slot's copy events bracket the block: the
-// start on s_copy, the end after its last decode.
+// A packed block with one copy lane: its headers into the next head area on s_copy and beg
+// decoded into the slot on s_dec; its arc payload copied whole to the end of the slot's edge
+// buffer and decoded there in passes (inplace_plan), or, when that plan fails, through the
+// staging ring in pieces of half the ring, each decoded as its copy lands.  The slot's copy
+// events bracket the block: the start on s_copy, the end after its last decode.
 static gw_status ring_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint32_t* pw, uint64_t words,
                            const PackLayout& lay, hipEvent_t after_compute)
 {
@@ -839,10 +878,33 @@ static gw_status ring_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint32_
     // the decoder writes the slot: after the compute stream's work that may still read it
     if (after_compute) HIPCHK(c, hipStreamWaitEvent(c->s_dec, after_compute, 0));
     HIPCHK(c, launch_unpack_beg(head, lay, d.nverts, c->slot_beg[slot], c->s_dec));
-    // pieces: the most chunks whose payload fits a piece (chunk offsets are contiguous, so
-    // chunks [c0, c1) take words [off(c0), off(c1)), off(nch) = the payload size)
     const uint32_t* ah = pw + lay.ah_off;
     const uint64_t total_w = words - lay.head_w;
+    // in place: the payload lands in the slot itself (one copy), decoded in a few passes
+    std::vector<uint64_t> cuts;
+    const uint64_t E = 4 * (d.start_edge & 15);
+    if (!c->no_inplace && c->slot_csr_bytes >= 4 * total_w + 256) {
+        const uint64_t pstart = (c->slot_csr_bytes - 4 * total_w) & ~255ULL;
+        if (inplace_plan(ah, lay.nch, d.nedges, total_w, pstart, E, cuts)) {
+            // the copy writes the slot: after the compute stream's work that may still read it
+            if (after_compute) HIPCHK(c, hipEventSynchronize(after_compute));
+            uint32_t* payload = (uint32_t*)((uint8_t*)c->slot_csr[slot] + pstart);
+            HIPCHK(c, hipMemcpyAsync(payload, pw + lay.head_w, total_w * 4, hipMemcpyHostToDevice, c->s_copy));
+            if ((st = pk_signal(c, c->s_copy, c->s_dec)) != GW_OK) return st;
+            for (size_t i = 0; i + 1 < cuts.size(); ++i)
+                HIPCHK(c, launch_unpack_arcs(head, payload, lay, c->slot_beg[slot], d.start_edge, d.nverts, d.nedges,
+                                             cuts[i], cuts[i + 1], 0, total_w, slot_edges(c, slot, blk), c->s_dec));
+            c->inplace_blocks++;
+            hipEvent_t ev;
+            if ((st = pk_event(c, &ev)) != GW_OK) return st;
+            HIPCHK(c, hipEventRecord(ev, c->s_dec));
+            c->pk_head_ev[h] = ev;
+            HIPCHK(c, hipEventRecord(c->slot_ev1[slot], c->s_dec));
+            return GW_OK;
+        }
+    }
+    // pieces: the most chunks whose payload fits a piece (chunk offsets are contiguous, so
+    // chunks [c0, c1) take words [off(c0), off(c1)), off(nch) = the payload size)
     auto off = [&](uint64_t k) { return k < lay.nch ? (uint64_t)ah[k * 4] : total_w; };
     uint64_t c0 = 0;
     while (c0 < lay.nch) {
@@ -873,6 +935,7 @@ static gw_status ring_copy(gw_ctx* c, uint32_t slot, uint32_t blk, const uint32_
         }
         c0 = c1;
     }
+    c->ring_blocks++;
     hipEvent_t ev;
     if ((st = pk_event(c, &ev)) != GW_OK) return st;
     HIPCHK(c, hipEventRecord(ev, c->s_dec));
@@ -1016,6 +1079,7 @@ static gw_status enqueue_copy(gw_ctx* c, const CopyJob& j)
     if (j.after_compute) HIPCHK(c, hipStreamWaitEvent(cs, j.after_compute, 0));
     HIPCHK(c, hipEventRecord(c->slot_ev0[slot], cs));
     if (j.packed) {
+        c->inline_blocks++;
         // the copies and decodes share the copy stream, so each copy into the staging buffer
         // queues behind the decode that read the previous piece
         const uint32_t* pw = j.packed;
@@ -1471,6 +1535,17 @@ static gw_status push_block_table(gw_ctx* c, bool enqueue_copy)
 static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t cap);
 
 constexpr size_t kMaxGraphs = 8;
+
+gw_status gw_upload_stats(gw_ctx* c, uint64_t* inplace, uint64_t* ring, uint64_t* copy_stream)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    wait_enqueued(c, -1);
+    std::lock_guard<std::mutex> lk(c->wmu);
+    if (inplace) *inplace = c->inplace_blocks;
+    if (ring) *ring = c->ring_blocks;
+    if (copy_stream) *copy_stream = c->inline_blocks;
+    return GW_OK;
+}
 
 gw_status gw_graph_stats(gw_ctx* c, uint64_t* replays, uint64_t* captures)
 {

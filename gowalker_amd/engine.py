@@ -279,6 +279,12 @@ class Context:
         self._chk(lib().gw_read_paths(self.h, _p(out), out.size))
         return out.reshape(self.nwalkers, L + 1)
 
+    def upload_stats(self):
+        """packed blocks decoded (in place, through the ring, on the copy stream)"""
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._chk(lib().gw_upload_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
     def graph_stats(self):
         """(runs replayed from captured graphs, graphs captured)"""
         r, c = C.c_uint64(), C.c_uint64()

@@ -13,6 +13,7 @@
 #pragma once
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -71,6 +72,7 @@ struct WalkConfig {
 struct RunReport {
     uint64_t steps = 0, trials = 0, probes = 0, mix_steps = 0;
     uint64_t h2d_bytes = 0, h2d_raw_bytes = 0, uploads = 0, staged = 0, iterations = 0;
+    uint64_t inplace_blocks = 0, ring_blocks = 0, inline_blocks = 0;   // packed blocks by decode path
     uint64_t zero_copy_runs = 0, zero_copy_steps = 0, zc_touches = 0;
     double wall_s = 0, kernel_ms = 0, filter_ms = 0, h2d_ms = 0, sched_ms = 0, tune_ms = 0;
     uint32_t nslots = 0, nblocks = 0, copy_lanes = 1;
@@ -207,7 +209,16 @@ public:
             whole = (whole + 255) & ~255ULL;
             if (whole) {
                 ck(gw_set_copy_lanes(ctx_, 1), ctx_, "gw_set_copy_lanes");
-                ck(gw_set_pack_staging(ctx_, whole), ctx_, "gw_set_pack_staging");
+                // one lane: the payloads land in their slots and decode in place, so the
+                // buffer holds two head areas and a small ring for blocks that cannot
+                // (C3: ~25 MB instead of the largest packed block, ~155 MB: one more slot at
+                // 8 GB); GW_PACK_INPLACE=0 sends every payload through a ring of that size
+                const char* ip = std::getenv("GW_PACK_INPLACE");
+                // (head areas are a quarter each, rounded down to 256 bytes)
+                const uint64_t one_lane = ip && std::atoi(ip) == 0
+                                              ? whole
+                                              : std::max<uint64_t>(4 * ((head + 511) & ~255ULL), 65536);
+                ck(gw_set_pack_staging(ctx_, one_lane), ctx_, "gw_set_pack_staging");
                 if (cfg_.copy_lanes == 2) {
                     const uint32_t one = slots_for_budget();
                     ck(gw_set_copy_lanes(ctx_, 2), ctx_, "gw_set_copy_lanes");
@@ -220,7 +231,7 @@ public:
                         } else {
                             lanes_ = 1;
                             ck(gw_set_copy_lanes(ctx_, 1), ctx_, "gw_set_copy_lanes");
-                            ck(gw_set_pack_staging(ctx_, whole), ctx_, "gw_set_pack_staging");
+                            ck(gw_set_pack_staging(ctx_, one_lane), ctx_, "gw_set_pack_staging");
                         }
                     }
                 }
@@ -249,6 +260,7 @@ public:
         slots_.reset(nslots_, nb);
         sched.buckets.clear();
         ck(gw_counters_get(ctx_, &c0), ctx_, "gw_counters_get");
+        ck(gw_upload_stats(ctx_, &up0_[0], &up0_[1], &up0_[2]), ctx_, "gw_upload_stats");
         std::vector<uint64_t> W((uint64_t)nb * nb);
         std::vector<uint8_t> staged(nb, 0);
         std::vector<uint32_t> pf;
@@ -396,6 +408,13 @@ public:
         ck(gw_counters_get(ctx_, &c1), ctx_, "gw_counters_get");
         r.h2d_bytes = c1.h2d_bytes - c0.h2d_bytes;
         r.h2d_raw_bytes = c1.h2d_raw_bytes - c0.h2d_raw_bytes;
+        {
+            uint64_t a = 0, b = 0, d = 0;
+            ck(gw_upload_stats(ctx_, &a, &b, &d), ctx_, "gw_upload_stats");
+            r.inplace_blocks = a - up0_[0];
+            r.ring_blocks = b - up0_[1];
+            r.inline_blocks = d - up0_[2];
+        }
         r.h2d_ms = c1.h2d_ms - c0.h2d_ms;
         r.wall_s = std::chrono::duration<double>(clk::now() - t0).count();
         return r;
@@ -417,6 +436,7 @@ private:
     std::vector<gw_block_desc> blocks_;
     gw_ctx* ctx_ = nullptr;
     uint32_t nslots_ = 0;
+    uint64_t up0_[3] = {0, 0, 0};   // gw_upload_stats at the walk's start
     SlotMap slots_;
 };
 

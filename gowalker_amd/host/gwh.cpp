@@ -218,6 +218,9 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         report->mix_steps = r.mix_steps;
         report->h2d_bytes = r.h2d_bytes;
         report->h2d_raw_bytes = r.h2d_raw_bytes;
+        report->inplace_blocks = r.inplace_blocks;
+        report->ring_blocks = r.ring_blocks;
+        report->inline_blocks = r.inline_blocks;
         report->uploads = r.uploads;
         report->iterations = r.iterations;
         report->zero_copy_runs = r.zero_copy_runs;

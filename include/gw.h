@@ -346,6 +346,12 @@ gw_status gw_counters_get(gw_ctx* ctx, gw_counters* out);
  * device clock (events in a graph cannot be timed); by default every kernel is launched
  * one by one and timed with HIP events. */
 gw_status gw_graph_stats(gw_ctx* ctx, uint64_t* replays, uint64_t* captures);
+/* Packed blocks by decode path since gw_create (no reference counterpart): decoded in their
+ * own slot (payload copied to the slot's end, decoded in ordered passes), through the
+ * staging ring in pieces, and on the copy stream (two copy lanes, GW_PACK_INLINE_DECODE=1,
+ * or headers larger than a head area).  GW_PACK_INPLACE=0 in the environment at gw_create
+ * sends every payload through the ring.  Waits until every upload asked for is enqueued. */
+gw_status gw_upload_stats(gw_ctx* ctx, uint64_t* inplace, uint64_t* ring, uint64_t* copy_stream);
 
 /* ---- synthetic graphs generated in HBM (the reference downloads real datasets,
  *      script/download_*.sh; its text->CSR converter cannot run as shipped,

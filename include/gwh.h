@@ -132,6 +132,9 @@ typedef struct {
     uint32_t copy_lanes;       /* copy lanes of packed uploads (gw_set_copy_lanes) */
     uint64_t h2d_raw_bytes;    /* the uploads' size in the reference's block format (h2d_bytes:
                                   what crossed PCIe, smaller with packed uploads) */
+    uint64_t inplace_blocks;   /* packed blocks decoded in their own slot (gw_upload_stats) ... */
+    uint64_t ring_blocks;      /* ... through the staging ring ... */
+    uint64_t inline_blocks;    /* ... and on the copy stream */
 } gwh_report;
 
 /* Runs walkers to completion out of core: beg has `entries` u64 offsets (ids

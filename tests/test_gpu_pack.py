@@ -127,8 +127,10 @@ def test_packed_uploads_match_oracle(gpu, name, make, block_bytes, types, wpv, h
 
 
 UPLOAD_MODES = {
-    "ring+worker": {},                                   # the default
-    "ring, caller's thread": {"GW_UPLOAD_THREAD": "0"},
+    "in place+worker": {},                                        # the default
+    "ring+worker": {"GW_PACK_INPLACE": "0"},
+    "in place, caller's thread": {"GW_UPLOAD_THREAD": "0"},
+    "ring, caller's thread": {"GW_UPLOAD_THREAD": "0", "GW_PACK_INPLACE": "0"},
     "copy-stream decode+worker": {"GW_PACK_INLINE_DECODE": "1"},
 }
 
@@ -157,10 +159,19 @@ def test_engine_upload_modes(gpu, mode, monkeypatch):
     for sched, zt in ((E.SCHED_LEARNED, 0), (E.SCHED_PLAIN, block_bytes // 3072)):
         rep, h, _ = E.run_out_of_core(b.a, c.a, N2V, 5, n, memory_bytes=mem, block_bytes=block_bytes,
                                       scheduler=sched, zero_threshold=zt, hashes=True, packed=pk)
-        print(mode, sched, {k: rep[k] for k in ("uploads", "staged", "nslots", "nblocks", "zc_touches", "h2d_bytes")},
-              flush=True)
+        print(mode, sched, {k: rep[k] for k in ("uploads", "staged", "nslots", "nblocks", "zc_touches", "h2d_bytes",
+                                                "inplace_blocks", "ring_blocks", "inline_blocks")}, flush=True)
         assert rep["nslots"] < rep["nblocks"] and rep["uploads"] + rep["staged"] > rep["nblocks"]
         np.testing.assert_array_equal(h, o["hashes"])
+        # the mode's decode path: every block in place or on the copy stream; through the
+        # ring, every block whose headers fit a head area (the rest on the copy stream)
+        blocks = rep["uploads"] + rep["staged"]
+        if "copy-stream" in mode:
+            assert rep["inline_blocks"] == blocks, rep
+        elif "ring" in mode:
+            assert rep["inplace_blocks"] == 0 and rep["ring_blocks"] > blocks // 2, rep
+        else:
+            assert rep["inplace_blocks"] == blocks, rep
     pk.close()
     b.close()
     c.close()
@@ -225,9 +236,10 @@ SLOT_CASES = [
 
 
 @pytest.mark.parametrize("name,make,block_bytes", SLOT_CASES)
-def test_packed_slots_bit_exact(gpu, name, make, block_bytes):
-    """Every block decoded into its slot, uploaded or staged, in one piece or in pieces,
-    holds exactly the block's beg (global offsets) and csr."""
+def test_packed_slots_bit_exact(gpu, name, make, block_bytes, monkeypatch):
+    """Every block decoded into its slot, uploaded or staged, in place, through the staging
+    ring in one piece or in pieces, or on the copy stream, holds exactly the block's beg
+    (global offsets) and csr."""
     beg, csr = make()
     bl = Blocks(beg, csr, block_bytes)
     nb = bl.nblocks
@@ -243,8 +255,11 @@ def test_packed_slots_bit_exact(gpu, name, make, block_bytes):
     ring_tight = max(8 * head, 16384)
     ring_mixed = max(8 * heads[len(heads) // 2], 16384)
     checked = 0
-    for lanes, staging in ((1, biggest), (1, head + 1088), (2, biggest), (2, head + 1088), (1, ring_roomy),
-                           (1, ring_tight), (1, ring_mixed)):
+    paths = [0, 0, 0]
+    for lanes, staging, inplace in ((1, biggest, True), (1, head + 1088, True), (2, biggest, True),
+                                    (2, head + 1088, True), (1, ring_roomy, False), (1, ring_tight, False),
+                                    (1, ring_mixed, False), (1, ring_mixed, True)):
+        monkeypatch.setenv("GW_PACK_INPLACE", "1" if inplace else "0")
         with Context() as ctx:
             ctx.set_blocks(bl)
             ctx.set_walk(N2V, 3)
@@ -269,9 +284,12 @@ def test_packed_slots_bit_exact(gpu, name, make, block_bytes):
                 np.testing.assert_array_equal(gc, ec, err_msg="%s block %d csr" % (name, b))
                 checked += 1
             cnt = ctx.counters()
-        print(name, "lanes", lanes, "staging", staging, "blocks", nb, "h2d", cnt["h2d_bytes"], "raw",
-              cnt["h2d_raw_bytes"], flush=True)
+            up = ctx.upload_stats()
+            paths = [x + y for x, y in zip(paths, up)]
+        print(name, "lanes", lanes, "staging", staging, "in place", inplace, "blocks", nb, "h2d", cnt["h2d_bytes"],
+              "raw", cnt["h2d_raw_bytes"], "decoded (in place, ring, copy stream)", up, flush=True)
     assert checked >= 8
+    assert all(p > 0 for p in paths), paths   # every decode path was taken
 
 
 def test_malformed_packed_block_rejected(gpu):
