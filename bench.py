@@ -180,6 +180,7 @@ SCHEDULERS = {
     "sowalker": ("SCHED_SOWALKER", False, 0),       # the reference's simulated-annealing scheduler
     "learned_model_swap": ("SCHED_LEARNED", False, 4),        # ablation: the network picks swap loads
     "learned_model_swap_myopic": ("SCHED_LEARNED", False, 6),  # ... trained on next-run labels
+    "learned_myopic": ("SCHED_LEARNED", False, 2),            # ablation: next-run labels, plain swaps
     "learned_value_victim": ("SCHED_LEARNED", False, 12),     # ... and victims by value
     "learned_no_swap_prefetch": ("SCHED_LEARNED", False, 1),  # ablation: no staging slot
     "staging_no_sticky": ("SCHED_STAGING", False, 16),        # ablation: no preference for the staged block
