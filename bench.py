@@ -192,12 +192,15 @@ EXTRA = ["learned", "staging", "learned_model_swap", "plain_zero", "plain", "sow
                                                                             # each at other caps
 
 
-def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, timed, packed=None):
+def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, timed, packed=None, hashes=False):
     """One full walk.  Every scheduler carries a model across its walks: the learned one
     its network, all of them the launch tuner, which tries a candidate launch per untimed
-    walk and runs the fastest tried in timed walks."""
+    walk and runs the fastest tried in timed walks.  Warm-up and timed walks keep no
+    per-walker output (the reference's walks have none either); a check walk (hashes=True,
+    untimed, after the timed ones) keeps the per-walker path hash, 8 B per walker counted
+    against the cap."""
     sched, zero, flags = SCHEDULERS[name]
-    if timed:
+    if timed or hashes:
         flags |= E.TUNE_FREEZE
     block_bytes = cfg["block_mb"] << 20
     launch = None
@@ -206,12 +209,10 @@ def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, timed, pack
         # probe model picks it
         launch = dict(block=args.block, variant=2 if args.variant is None else args.variant,
                       waves_per_cu=args.waves_per_cu, window=args.window, interp=args.interp)
-    # every walk keeps the same outputs (the per-walker path hash, 8 B per walker counted
-    # against the cap), so warm-up and timed walks run with the same slot count
     rep, h, _ = E.run_out_of_core(beg, csr, cfg["types"], args.seed, nwalkers, wpv=cfg["wpv"],
                                   memory_bytes=int(cap_gb * (1 << 30)), block_bytes=block_bytes,
                                   zero_threshold=block_bytes // 128 // 24 if zero else 0,
-                                  scheduler=getattr(E, sched), hashes=True, flags=flags, launch=launch,
+                                  scheduler=getattr(E, sched), hashes=hashes, flags=flags, launch=launch,
                                   prefetch_depth=args.prefetch_depth, verbose=args.verbose,
                                   model=models.setdefault(name, E.Model()), packed=packed)
     return rep, h
@@ -289,12 +290,22 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         elif not np.array_equal(h, ref_hash):
             identical = False
 
+    def check_walks(cap_gb, names):
+        """one untimed walk with per-walker path hashes per scheduler: walks must be
+        bit-identical across schedulers and caps"""
+        for name in names:
+            rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, False, packed, hashes=True)
+            check_hash(h)
+            log_walk(cap_gb, name, "check", rep)
+
     head = [n for n in (args.schedulers or HEADLINE)]
     cap0 = caps[0]
     reps = {n: [] for n in head}
     done_rounds = 0
     round_s = []
-    reserve = (0 if args.no_cpu_baseline else args.cpu_seconds * 2.2 + 5) + (40 * (len(caps) - 1))
+    # later work: the CPU baseline, check walks at the headline cap, and each further cap's
+    # walks and check walks
+    reserve = (0 if args.no_cpu_baseline else args.cpu_seconds * 2.2 + 5) + 10 + (80 * (len(caps) - 1))
     for i in range(args.warmup + args.steps):
         elapsed = time.perf_counter() - t_start
         est = max(round_s) if round_s else 0.0
@@ -305,7 +316,6 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         t0 = time.perf_counter()
         for name in head:
             rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models, i >= args.warmup, packed)
-            check_hash(h)
             log_walk(cap0, name, "warmup %d" % i if i < args.warmup else "run %d" % (i - args.warmup), rep)
             if i >= args.warmup:
                 reps[name].append(rep)
@@ -322,9 +332,9 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
     if not args.schedulers:
         for name in ONCE:
             rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap0, name, models, True, packed)
-            check_hash(h)
             log_walk(cap0, name, "run 0", rep)
             res0[name] = summarize([rep], 0)
+    check_walks(cap0, list(res0))
     per_cap.append({"memory_gb": cap0, "schedulers": res0})
     for cap_gb in caps[1:]:
         res = {}
@@ -332,12 +342,11 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
             if name.startswith("learned"):
                 for w in range(args.extra_warmup):
                     rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, False, packed)
-                    check_hash(h)
                     log_walk(cap_gb, name, "warmup %d" % w, rep)
             rep, h = ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, True, packed)
-            check_hash(h)
             log_walk(cap_gb, name, "run 0", rep)
             res[name] = summarize([rep], zt if SCHEDULERS[name][1] else 0)
+        check_walks(cap_gb, list(res))
         per_cap.append({"memory_gb": cap_gb, "schedulers": res})
     for entry in per_cap:
         res = entry["schedulers"]
@@ -378,6 +387,9 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         "config": {"workload": cfg["workload"], "nverts": nverts, "narcs": narcs,
                    "non_isolated": int(np.count_nonzero(beg[1:] != beg[:-1])), "walkers": nwalkers,
                    "edges_drawn": cfg.get("edges_drawn"), "caps": per_cap, "walks_identical": identical,
+                   "walk_check": "timed and warm-up walks keep no per-walker output; after them one untimed walk per "
+                                 "scheduler and cap keeps the per-walker path hash (8 B per walker, counted against the "
+                                 "cap), and walks_identical compares those hashes",
                    "model": mi, "launch_tuning": tuning, "packed_uploads": pack, "graph_gen_s": gen_s, "pcie_peak_gbs": PCIE_PEAK_GBS, "parallelism": "1 GPU",
                    "steps_requested": args.steps, "launch": r0.get("tuned")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
