@@ -154,6 +154,7 @@ struct gw_ctx {
     uint32_t visit_parts = 1;              // visit-count arrays (summed by gw_read_visits)
     uint32_t* d_work = nullptr;
     uint32_t* d_bucket = nullptr;
+    uint32_t* d_bpart = nullptr;           // per-workgroup rows of the bucket pass (one per CU)
     unsigned long long* d_trace = nullptr; // per-block steps of the last traced run
     float* d_feat = nullptr;               // nblocks x kNumFeatures
     float* d_age = nullptr;
@@ -538,6 +539,7 @@ gw_status gw_destroy(gw_ctx* c)
     free_slots(c);
     dfree(c, c->d_pack, c->pack_cap, true);
     dfree(c, c->d_bucket, 0, false);
+    dfree(c, c->d_bpart, 0, false);
     dfree(c, c->d_trace, 0, false);
     dfree(c, c->d_feat, 0, false);
     dfree(c, c->d_age, 0, false);
@@ -606,10 +608,12 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     dfree(c, c->d_blocks, 0, false);
     dfree(c, c->d_vcut, 0, false);
     dfree(c, c->d_bucket, 0, false);
+    dfree(c, c->d_bpart, 0, false);
     dfree(c, c->d_trace, 0, false);
     dfree(c, c->d_feat, 0, false);
     dfree(c, c->d_age, 0, false);
     c->d_blocks = nullptr; c->d_vcut = nullptr; c->d_bucket = nullptr;
+    c->d_bpart = nullptr;
     c->d_trace = nullptr; c->d_feat = nullptr; c->d_age = nullptr;
     if (c->h_blocks_pinned) { (void)hipHostFree(c->h_blocks_pinned); c->h_blocks_pinned = nullptr; }
     if (c->h_bucket) { (void)hipHostFree(c->h_bucket); c->h_bucket = nullptr; }
@@ -642,6 +646,12 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     c->d_vcut = (uint32_t*)p;
     if ((st = dalloc(c, &p, sizeof(uint32_t) * (uint64_t)nb * nb, false)) != GW_OK) return st;
     c->d_bucket = (uint32_t*)p;
+    // the bucket pass's per-workgroup rows: table metadata like the matrix itself, outside
+    // the budget (at most 48 KB per CU: 12 MB on 256 CUs)
+    if ((uint64_t)nb * nb <= kFilterLdsBuckets) {
+        if ((st = dalloc(c, &p, sizeof(uint32_t) * (uint64_t)nb * nb * c->ncu, false)) != GW_OK) return st;
+        c->d_bpart = (uint32_t*)p;
+    }
     if ((st = dalloc(c, &p, sizeof(unsigned long long) * nb, false)) != GW_OK) return st;
     c->d_trace = (unsigned long long*)p;
     HIPCHK(c, hipMemset(c->d_trace, 0, sizeof(unsigned long long) * nb));
@@ -1485,8 +1495,10 @@ static gw_status enqueue_bucket_pass(gw_ctx* c, bool stamps = false)
     f.hops = c->d_cnt + 8;
     f.len0 = c->dt[0].length;
     f.len1 = c->dt[1].length;
+    f.bucket_part = c->d_bpart;
+    f.part_rows = c->d_bpart ? c->ncu : 0;
     HIPCHK(c, stamps ? launch_timestamp(c->m_ts + 2, c->s_compute) : hipEventRecord(c->ev_f0, c->s_compute));
-    HIPCHK(c, launch_filter(f, c->ncu * 2, c->s_compute));
+    HIPCHK(c, launch_filter(f, c->ncu, c->s_compute));
     HIPCHK(c, stamps ? launch_timestamp(c->m_ts + 3, c->s_compute) : hipEventRecord(c->ev_f1, c->s_compute));
     HIPCHK(c, launch_copy_u32(c->m_bucket, c->d_bucket, nn, c->s_compute));
     HIPCHK(c, launch_copy_u32((uint32_t*)(c->m_cnt + 8), (const uint32_t*)(c->d_cnt + 6), 2, c->s_compute));

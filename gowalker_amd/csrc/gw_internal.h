@@ -112,6 +112,8 @@ struct FilterArgs {
     unsigned long long* left; // unfinished walkers
     unsigned long long* hops; // optional: sum of the remaining hop caps of unfinished walkers
     uint32_t len0, len1;      // hop caps of walker types 0 / 1
+    uint32_t* bucket_part;    // optional: part_rows x nblocks^2 per-workgroup rows of the bucket matrix
+    uint32_t part_rows;
 };
 
 // records a message for gw_last_error (gw_api.cpp)
@@ -190,5 +192,6 @@ constexpr uint32_t kLutSize = 2048;          // entries of the LDS vertex -> blo
 constexpr uint32_t kVisitParts = 8;          // visit-count arrays, one per XCD when HBM allows
 constexpr uint32_t kNumFeatures = 12;
 constexpr uint32_t kFilterLdsBuckets = 12288;  // bucket matrices up to 110 x 110 blocks count in LDS
+constexpr uint32_t kFilterLdsResident = 32768; // resident sets of up to 32768 blocks are an LDS bit table
 
 }  // namespace gw

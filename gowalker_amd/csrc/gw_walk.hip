@@ -1499,47 +1499,66 @@ __global__ void __launch_bounds__(256) init_kernel(InitArgs a)
 
 // The bucket matrix is the count matrix that prefix() scans in gpuwalk.hpp:321-338; the
 // worklist replaces insertglobal's scatter (gpuwalk.hpp:340-372).
-constexpr uint32_t kFilterThreads = 1024, kFilterItems = 4;
+constexpr uint32_t kFilterThreads = 1024, kFilterItems = 16;
 
 // Bucket / worklist pass over all walkers.  A workgroup takes a tile of
 // kFilterThreads * kFilterItems walkers (coalesced: item k of thread t is walker
-// tile + k * kFilterThreads + t), counts the unfinished ones into the (prev_blk, cur_blk)
-// histogram (LDS when the matrix fits) and, when a worklist is requested, appends the
-// runnable ones with one atomic per tile.
+// tile + k * kFilterThreads + t), so each lane has 16 status words and 16 block pairs in
+// flight before it waits; the resident set is a bit table in LDS.  Unfinished walkers are
+// counted into the (prev_blk, cur_blk) histogram (in LDS when the matrix fits) and, when a
+// worklist is requested, the runnable ones are appended with one atomic per tile.  An LDS
+// histogram leaves the workgroup as one row of plain stores into bucket_part, summed by
+// bucket_reduce_kernel: a device-scope atomic per bin and workgroup (1.5M of them on a
+// 54-block table) made the pass 2x slower and held up host-to-device copies meanwhile.
 __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
 {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t wave_cnt[kFilterThreads / 64];
     __shared__ uint32_t tile_base;
+    __shared__ uint32_t res_bits[kFilterLdsResident / 32];
     const uint32_t nbb = a.nblocks * a.nblocks;
     const bool lds_hist = a.bucket && nbb <= kFilterLdsBuckets;
-    if (lds_hist) {
+    const bool lds_res = a.worklist && a.nblocks <= kFilterLdsResident;
+    if (lds_hist)
         for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x) hist[i] = 0;
+    if (lds_res) {
+        for (uint32_t i = threadIdx.x; i < (a.nblocks + 31) / 32; i += blockDim.x) res_bits[i] = 0;
         __syncthreads();
+        for (uint32_t b = threadIdx.x; b < a.nblocks; b += blockDim.x)
+            if (a.blocks[b].resident) atomicOr(res_bits + (b >> 5), 1u << (b & 31));
     }
+    __syncthreads();
     const uint32_t lane = __lane_id(), wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
     const uint64_t tile = (uint64_t)blockDim.x * kFilterItems;
     uint64_t left = 0, hops = 0;
     for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < a.nwalkers; t0 += (uint64_t)gridDim.x * tile) {
+        uint32_t m[kFilterItems], pr[kFilterItems];
+#pragma unroll
+        for (uint32_t k = 0; k < kFilterItems; ++k) {
+            const uint64_t w = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
+            m[k] = w < a.nwalkers ? a.meta[w] : (kDoneLength << kStatusShift);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kFilterItems; ++k) {
+            const uint64_t w = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
+            pr[k] = ((m[k] >> kStatusShift) & 0xFu) == kActive ? a.bpair[w] : 0u;
+        }
         uint64_t masks[kFilterItems];
         uint32_t cnt = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kFilterItems; ++k) {
-            const uint64_t w = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
             bool take = false;
-            if (w < a.nwalkers) {
-                const uint32_t m = a.meta[w];
-                if (((m >> kStatusShift) & 0xFu) == kActive) {
-                    ++left;
-                    const uint32_t len = ((m >> kTypeShift) & 1u) ? a.len1 : a.len0;
-                    const uint32_t hop = m & kHopMask;
-                    hops += len > hop ? len - hop : 0u;
-                    const uint32_t pr = a.bpair[w];
-                    const uint32_t bc = pr & 0xFFFFu, bp = pr >> 16;
-                    take = a.worklist && a.blocks[bc].resident && a.blocks[bp].resident;
-                    if (lds_hist) atomicAdd(hist + bp * a.nblocks + bc, 1u);
-                    else if (a.bucket) atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
-                }
+            if (((m[k] >> kStatusShift) & 0xFu) == kActive) {
+                ++left;
+                const uint32_t len = ((m[k] >> kTypeShift) & 1u) ? a.len1 : a.len0;
+                const uint32_t hop = m[k] & kHopMask;
+                hops += len > hop ? len - hop : 0u;
+                const uint32_t bc = pr[k] & 0xFFFFu, bp = pr[k] >> 16;
+                if (a.worklist)
+                    take = lds_res ? ((res_bits[bc >> 5] >> (bc & 31)) & (res_bits[bp >> 5] >> (bp & 31)) & 1u) != 0
+                                   : a.blocks[bc].resident && a.blocks[bp].resident;
+                if (lds_hist) atomicAdd(hist + bp * a.nblocks + bc, 1u);
+                else if (a.bucket) atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
             }
             masks[k] = __ballot(take);
             cnt += (uint32_t)__popcll(masks[k]);
@@ -1571,9 +1590,33 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
     }
     if (lds_hist) {
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x)
-            if (hist[i]) atomicAdd(a.bucket + i, hist[i]);
+        if (a.bucket_part) {
+            uint32_t* row = a.bucket_part + (uint64_t)blockIdx.x * nbb;
+            for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x) row[i] = hist[i];
+        } else {
+            for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x)
+                if (hist[i]) atomicAdd(a.bucket + i, hist[i]);
+        }
     }
+}
+
+// bucket[i] = the sum of the filter workgroups' rows of bucket_part (integer sums: the
+// same matrix in any order).
+__global__ void __launch_bounds__(256) bucket_reduce_kernel(const uint32_t* __restrict__ part, uint32_t nrows,
+                                                            uint32_t nbb, uint32_t* __restrict__ bucket)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbb) return;
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    uint32_t r = 0;
+    for (; r + 4 <= nrows; r += 4) {
+        s0 += part[(uint64_t)r * nbb + i];
+        s1 += part[(uint64_t)(r + 1) * nbb + i];
+        s2 += part[(uint64_t)(r + 2) * nbb + i];
+        s3 += part[(uint64_t)(r + 3) * nbb + i];
+    }
+    for (; r < nrows; ++r) s0 += part[(uint64_t)r * nbb + i];
+    bucket[i] = s0 + s1 + s2 + s3;
 }
 
 // Per-block features of the learned scheduler, one thread per block b, from the bucket
@@ -1589,42 +1632,53 @@ __global__ void __launch_bounds__(256) features_kernel(const uint32_t* __restric
                                                         const float* __restrict__ age, float ahead,
                                                         float* __restrict__ f)
 {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    double row = 0, col = 0, res = 0, total = 0;
-    double mn = -1;
-    for (uint32_t j = 0; j < nb; ++j) {
-        const double r = W[(uint64_t)b * nb + j], c = W[(uint64_t)j * nb + b];
-        row += r;
-        col += c;
-        if (j != b && blocks[j].resident == kInSlot) {
-            res += r + c;
-            if (mn < 0 || r + c < mn) mn = r + c;
-        }
+    // one workgroup: the total of W is the sum of the row sums, reduced in LDS (integer
+    // counts, exact in double in any order)
+    __shared__ double part[256];
+    double t = 0;
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+        for (uint32_t j = 0; j < nb; ++j) t += W[(uint64_t)b * nb + j];
+    part[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
     }
-    for (uint64_t i = 0; i < (uint64_t)nb * nb; ++i) total += W[i];
-    const double diag = W[(uint64_t)b * nb + b];
-    const DevBlock& d = blocks[b];
-    float* o = f + (uint64_t)b * kNumFeatures;
-    o[0] = (float)log1p(row);
-    o[1] = (float)log1p(col);
-    o[2] = (float)log1p(diag);
-    o[3] = (float)log1p(res);
-    o[4] = d.resident == kInSlot ? 1.f : 0.f;
-    o[5] = (float)log1p(mn < 0 ? 0.0 : mn);
-    o[6] = (float)log1p(res + diag);
-    o[7] = age ? age[b] / 64.f : 0.f;
-    o[8] = (float)(log1p((double)d.nedges) / 24.0);
-    o[9] = ahead;   // run phase: mean fraction of the hop cap still ahead of unfinished walkers
-    o[10] = (float)(log1p(d.nverts ? (double)d.nedges / d.nverts : 0.0) / 8.0);
-    o[11] = total > 0 ? (float)((row + col - diag) / total) : 0.f;
+    const double total = part[0];
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+        double row = 0, col = 0, res = 0;
+        double mn = -1;
+        for (uint32_t j = 0; j < nb; ++j) {
+            const double r = W[(uint64_t)b * nb + j], c = W[(uint64_t)j * nb + b];
+            row += r;
+            col += c;
+            if (j != b && blocks[j].resident == kInSlot) {
+                res += r + c;
+                if (mn < 0 || r + c < mn) mn = r + c;
+            }
+        }
+        const double diag = W[(uint64_t)b * nb + b];
+        const DevBlock& d = blocks[b];
+        float* o = f + (uint64_t)b * kNumFeatures;
+        o[0] = (float)log1p(row);
+        o[1] = (float)log1p(col);
+        o[2] = (float)log1p(diag);
+        o[3] = (float)log1p(res);
+        o[4] = d.resident == kInSlot ? 1.f : 0.f;
+        o[5] = (float)log1p(mn < 0 ? 0.0 : mn);
+        o[6] = (float)log1p(res + diag);
+        o[7] = age ? age[b] / 64.f : 0.f;
+        o[8] = (float)(log1p((double)d.nedges) / 24.0);
+        o[9] = ahead;   // run phase: mean fraction of the hop cap still ahead of unfinished walkers
+        o[10] = (float)(log1p(d.nverts ? (double)d.nedges / d.nverts : 0.0) / 8.0);
+        o[11] = total > 0 ? (float)((row + col - diag) / total) : 0.f;
+    }
 }
 
 hipError_t launch_features(const uint32_t* bucket, uint32_t nblocks, const DevBlock* blocks, const float* age, float ahead,
                            float* feats, hipStream_t s)
 {
-    hipLaunchKernelGGL(features_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, s, bucket, nblocks, blocks, age,
-                       ahead, feats);
+    hipLaunchKernelGGL(features_kernel, dim3(1), dim3(256), 0, s, bucket, nblocks, blocks, age, ahead, feats);
     return hipGetLastError();
 }
 
@@ -1649,8 +1703,15 @@ hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s)
     const uint64_t need = (a.nwalkers + kFilterThreads * kFilterItems - 1) / (kFilterThreads * kFilterItems);
     if (grid == 0 || grid > need) grid = (uint32_t)(need < 65536 ? need : 65536);
     const uint32_t nbb = a.nblocks * a.nblocks;
-    const size_t lds = (a.bucket && nbb <= kFilterLdsBuckets) ? nbb * 4 : 0;
-    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(kFilterThreads), lds, s, a);
+    const bool lds_hist = a.bucket && nbb <= kFilterLdsBuckets;
+    FilterArgs f = a;
+    if (!lds_hist || a.part_rows == 0) f.bucket_part = nullptr;
+    if (f.bucket_part && grid > a.part_rows) grid = a.part_rows;
+    const size_t lds = lds_hist ? nbb * 4 : 0;
+    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(kFilterThreads), lds, s, f);
+    if (f.bucket_part)
+        hipLaunchKernelGGL(bucket_reduce_kernel, dim3((nbb + 255) / 256), dim3(256), 0, s, f.bucket_part, grid, nbb,
+                           f.bucket);
     return hipGetLastError();
 }
 
