@@ -95,6 +95,8 @@ SIGNATURES = {
     "gw_counters_get": [P, C.POINTER(GwCounters)],
     "gw_graph_stats": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "gw_upload_stats": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "gw_set_worklist_cap": [P, C.c_uint64],
+    "gw_worklist_stats": [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "gw_split_blocks": [P, C.c_uint64, C.c_uint64, P, P, C.c_uint64, u64p],
     "gw_gen_rmat": [P, C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
                     C.POINTER(P)],

@@ -152,7 +152,8 @@ struct gw_ctx {
     uint32_t* d_paths = nullptr;
     uint32_t* d_visits = nullptr;
     uint32_t visit_parts = 1;              // visit-count arrays (summed by gw_read_visits)
-    uint32_t* d_work = nullptr;
+    uint32_t* d_work = nullptr;            // worklist of runnable walkers: work_cap entries
+    uint64_t work_cap = 0, work_cap_req = 0, work_rounds = 0;
     uint32_t* d_bucket = nullptr;
     uint32_t* d_bpart = nullptr;           // per-workgroup rows of the bucket pass (one per CU)
     unsigned long long* d_trace = nullptr; // per-block steps of the last traced run
@@ -265,13 +266,14 @@ static void free_walkers(gw_ctx* c)
     dfree(c, c->d_prev, c->nwalkers * 4, true);
     dfree(c, c->d_meta, c->nwalkers * 4, true);
     dfree(c, c->d_bpair, c->nwalkers * 4, true);
-    dfree(c, c->d_work, c->nwalkers * 4, true);
+    dfree(c, c->d_work, c->work_cap * 4, true);
     dfree(c, c->d_hash, c->nwalkers * 8, true);
     dfree(c, c->d_paths, c->nwalkers * (uint64_t)c->path_stride * 4, true);
     dfree(c, c->d_visits, c->nverts_total * 4 * c->visit_parts, true);
     c->d_cur = c->d_prev = c->d_meta = c->d_bpair = c->d_work = c->d_paths = c->d_visits = nullptr;
     c->d_hash = nullptr;
     c->nwalkers = 0;
+    c->work_cap = 0;
     c->walker_bytes = 0;
 }
 
@@ -1394,6 +1396,33 @@ gw_status gw_set_walk(gw_ctx* c, const gw_walk_params* wp)
     return GW_OK;
 }
 
+// Worklist entries for n walkers: the caller's gw_set_worklist_cap, else n / 8 (at least
+// min(n, 2^20)).  A run never needs more than its runnable walkers, and one whose runnable
+// walkers exceed the worklist takes several filter + step rounds, so the 4 bytes per walker
+// a full-size list would hold go to block slots instead (C5: 0.94 GB, 2-3 slots).
+static uint64_t worklist_entries(const gw_ctx* c, uint64_t n)
+{
+    uint64_t w = c->work_cap_req ? c->work_cap_req : std::max<uint64_t>(n / 8, std::min<uint64_t>(n, 1u << 20));
+    return std::min<uint64_t>(std::max<uint64_t>(w, 64), std::max<uint64_t>(n, 1));
+}
+
+gw_status gw_set_worklist_cap(gw_ctx* c, uint64_t entries)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (entries > 0xFFFFFFFFull) return fail(c, GW_ERR_ARG, "worklist of %llu entries exceeds 2^32 - 1",
+                                             (unsigned long long)entries);
+    c->work_cap_req = entries;
+    return GW_OK;
+}
+
+gw_status gw_worklist_stats(gw_ctx* c, uint64_t* capacity, uint64_t* extra_rounds)
+{
+    if (!c) return fail(c, GW_ERR_ARG, "null context");
+    if (capacity) *capacity = c->work_cap;
+    if (extra_rounds) *extra_rounds = c->work_rounds;
+    return GW_OK;
+}
+
 gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* sources, uint32_t outputs)
 {
     if (!c) return fail(c, GW_ERR_ARG, "null context");
@@ -1414,18 +1443,22 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
     const uint32_t maxlen = std::max(c->dt[0].length, c->wp.ntypes == 2 ? c->dt[1].length : 0u);
     const uint32_t stride = (outputs & GW_OUT_PATHS) ? maxlen + 1 : 0;
+    const uint64_t wcap = worklist_entries(c, n);
     // a re-initialisation with the same shape reuses the device buffers
-    if (!(c->nwalkers == n && c->outputs == outputs && c->path_stride == stride)) {
+    if (!(c->nwalkers == n && c->outputs == outputs && c->path_stride == stride && c->work_cap == wcap)) {
         free_walkers(c);
         c->path_stride = stride;
         void* p;
         gw_status st;
         c->nwalkers = n;
-        uint32_t** u32s[] = {&c->d_cur, &c->d_prev, &c->d_meta, &c->d_bpair, &c->d_work};
+        uint32_t** u32s[] = {&c->d_cur, &c->d_prev, &c->d_meta, &c->d_bpair};
         for (uint32_t** q : u32s) {
             if ((st = dalloc(c, &p, n * 4, true)) != GW_OK) { free_walkers(c); return st; }
             *q = (uint32_t*)p;
         }
+        if ((st = dalloc(c, &p, wcap * 4, true)) != GW_OK) { free_walkers(c); return st; }
+        c->d_work = (uint32_t*)p;
+        c->work_cap = wcap;
         if (outputs & GW_OUT_HASH) {
             if ((st = dalloc(c, &p, n * 8, true)) != GW_OK) { free_walkers(c); return st; }
             c->d_hash = (uint64_t*)p;
@@ -1631,6 +1664,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     }
     uint64_t nwork = c->nwalkers;
     if (cap && nwork > cap) nwork = cap;
+    if (!all_res && nwork > c->work_cap) nwork = c->work_cap;   // the step kernel reads at most this many
     float fms = 0.f, kms = 0.f;
     // compact the runnable walkers on the device; the step kernel reads the count there,
     // so no host round trip separates the two kernels
@@ -1642,6 +1676,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     f.meta = c->d_meta;
     f.bpair = c->d_bpair;
     f.worklist = c->d_work;
+    f.work_cap = (uint32_t)c->work_cap;
     f.nwork = c->d_misc;
     f.left = c->d_cnt + 6;
     f.len0 = c->dt[0].length;
@@ -1705,93 +1740,102 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     // readbacks and the bucket pass -- launched one by one, or as a graph captured the
     // first time this exact sequence (every kernel argument equal) runs and replayed after
     const bool graph = c->run_graphs && !cap;
+    uint32_t round = 0;   // > 0: a further round for walkers the worklist could not hold
     auto enqueue = [&]() -> gw_status {
         HIPCHK(c, launch_copy_u32((uint32_t*)c->d_blocks, (const uint32_t*)c->m_blocks,
                                   sizeof(DevBlock) / 4 * c->nblocks, c->s_compute));
         HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
         HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
         if (!all_res) HIPCHK(c, launch_filter(f, c->ncu * 2, c->s_compute));
-        if (traced) HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
+        if (traced && round == 0) HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
         HIPCHK(c, graph ? launch_timestamp(c->m_ts, c->s_compute) : hipEventRecord(c->ev_k0, c->s_compute));
         HIPCHK(c, launch_walk(a, one_blk, l.variant, (uint32_t)grid, l.block, c->s_compute));
         HIPCHK(c, graph ? launch_timestamp(c->m_ts + 1, c->s_compute) : hipEventRecord(c->ev_k1, c->s_compute));
         HIPCHK(c, launch_copy_u32((uint32_t*)c->m_cnt, (const uint32_t*)c->d_cnt, 16, c->s_compute));
+        // the runnable count (beyond the worklist when it overflowed)
+        if (!all_res) HIPCHK(c, launch_copy_u32((uint32_t*)(c->m_cnt + 10), c->d_misc, 1, c->s_compute));
         if (traced)
             HIPCHK(c, launch_copy_u32((uint32_t*)c->m_trace, (const uint32_t*)c->d_trace, 2ull * c->nblocks, c->s_compute));
         // bucket matrix of the walkers left, read back with the counters (one host wait)
         if (buckets) return enqueue_bucket_pass(c, graph);
         return GW_OK;
     };
-    if (graph) {
-        RunSig sig;
-        memset(&sig, 0, sizeof sig);
-        sig.a = a;
-        sig.f = all_res ? FilterArgs{} : f;
-        sig.grid = grid;
-        sig.block = l.block;
-        sig.variant = l.variant;
-        sig.flags = (all_res ? 1u : 0u) | (traced ? 2u : 0u) | (buckets ? 4u : 0u) | (one_blk ? 8u : 0u);
-        sig.ncu = c->ncu;
-        sig.d_bucket = c->d_bucket;
-        sig.m_cnt = c->m_cnt;
-        sig.m_trace = c->m_trace;
-        sig.m_bucket = c->m_bucket;
-        sig.m_blocks = c->m_blocks;
-        hipGraphExec_t ex = graph_lookup(c, sig);
-        if (!ex) {
-            HIPCHK(c, hipStreamBeginCapture(c->s_compute, hipStreamCaptureModeThreadLocal));
-            const gw_status est = enqueue();
-            hipGraph_t g = nullptr;
-            const hipError_t ce = hipStreamEndCapture(c->s_compute, &g);
-            if (est != GW_OK) {
-                if (g) (void)hipGraphDestroy(g);
-                return est;
+    gw_run_stats s{};
+    for (;; ++round) {
+        if (graph) {
+            RunSig sig;
+            memset(&sig, 0, sizeof sig);
+            sig.a = a;
+            sig.f = all_res ? FilterArgs{} : f;
+            sig.grid = grid;
+            sig.block = l.block;
+            sig.variant = l.variant;
+            sig.flags = (all_res ? 1u : 0u) | (traced ? 2u : 0u) | (buckets ? 4u : 0u) | (one_blk ? 8u : 0u) |
+                        (round ? 16u : 0u);
+            sig.ncu = c->ncu;
+            sig.d_bucket = c->d_bucket;
+            sig.m_cnt = c->m_cnt;
+            sig.m_trace = c->m_trace;
+            sig.m_bucket = c->m_bucket;
+            sig.m_blocks = c->m_blocks;
+            hipGraphExec_t ex = graph_lookup(c, sig);
+            if (!ex) {
+                HIPCHK(c, hipStreamBeginCapture(c->s_compute, hipStreamCaptureModeThreadLocal));
+                const gw_status est = enqueue();
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(c->s_compute, &g);
+                if (est != GW_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return est;
+                }
+                if (ce != hipSuccess) return fail(c, GW_ERR_HIP, "run graph capture: %s", hipGetErrorString(ce));
+                const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ie != hipSuccess) return fail(c, GW_ERR_HIP, "run graph instantiation: %s", hipGetErrorString(ie));
+                graph_store(c, sig, ex);
             }
-            if (ce != hipSuccess) return fail(c, GW_ERR_HIP, "run graph capture: %s", hipGetErrorString(ce));
-            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ie != hipSuccess) return fail(c, GW_ERR_HIP, "run graph instantiation: %s", hipGetErrorString(ie));
-            graph_store(c, sig, ex);
+            HIPCHK(c, hipGraphLaunch(ex, c->s_compute));
+            c->graph_runs++;
+        } else {
+            if ((st = enqueue()) != GW_OK) return st;
         }
-        HIPCHK(c, hipGraphLaunch(ex, c->s_compute));
-        c->graph_runs++;
-    } else {
-        if ((st = enqueue()) != GW_OK) return st;
+        HIPCHK(c, hipStreamSynchronize(c->s_compute));
+        if (graph) {
+            kms = (float)((double)(c->h_ts[1] - c->h_ts[0]) / c->ts_khz);
+            if (buckets) fms = (float)((double)(c->h_ts[3] - c->h_ts[2]) / c->ts_khz);
+        } else {
+            HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
+            if (buckets) HIPCHK(c, hipEventElapsedTime(&fms, c->ev_f0, c->ev_f1));
+        }
+        s.steps += c->h_cnt[0];
+        s.trials += c->h_cnt[1];
+        s.probes += c->h_cnt[2];
+        s.mix_steps += c->h_cnt[3];
+        s.walkers_ended += c->h_cnt[4];
+        s.walkers_run += c->h_cnt[4] + c->h_cnt[5];
+        s.kernel_ms += kms;
+        s.filter_ms += fms;
+        s.zc_touches += c->h_cnt[7];
+        if (!buckets) c->unfinished -= std::min<uint64_t>(c->unfinished, c->h_cnt[4]);
+        // every walker of the list has run until it finished or left the resident blocks,
+        // so the runnable count falls by the list's length each round
+        if (all_res || (uint32_t)c->h_cnt[10] <= c->work_cap) break;
+        c->work_rounds++;
     }
-    HIPCHK(c, hipStreamSynchronize(c->s_compute));
     c->trace_valid = traced;
-    if (graph) {
-        kms = (float)((double)(c->h_ts[1] - c->h_ts[0]) / c->ts_khz);
-        if (buckets) fms = (float)((double)(c->h_ts[3] - c->h_ts[2]) / c->ts_khz);
-    } else {
-        HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
-        if (buckets) HIPCHK(c, hipEventElapsedTime(&fms, c->ev_f0, c->ev_f1));
-    }
     if (buckets) c->w_valid = true;
     for (uint32_t sl = 0; sl < c->nslots; ++sl) collect_copy_time(c, sl, false);
-    gw_run_stats s{};
-    s.steps = c->h_cnt[0];
-    s.trials = c->h_cnt[1];
-    s.probes = c->h_cnt[2];
-    s.mix_steps = c->h_cnt[3];
-    s.walkers_ended = c->h_cnt[4];
-    s.walkers_run = c->h_cnt[4] + c->h_cnt[5];
     if (buckets) {
         c->unfinished = c->h_cnt[8];
         c->pending_hops = c->h_cnt[9];
-    } else {
-        c->unfinished -= std::min<uint64_t>(c->unfinished, s.walkers_ended);
     }
     s.walkers_left = c->unfinished;
-    s.kernel_ms = kms;
-    s.filter_ms = fms;
-    s.zc_touches = c->h_cnt[7];
     c->tot.zc_touches += s.zc_touches;
     c->tot.steps += s.steps;
     c->tot.trials += s.trials;
     c->tot.probes += s.probes;
     c->tot.mix_steps += s.mix_steps;
-    c->tot.kernel_ms += kms;
+    c->tot.kernel_ms += s.kernel_ms;
     if (stats) *stats = s;
     return GW_OK;
 }

@@ -107,7 +107,8 @@ struct FilterArgs {
     const uint32_t* meta;
     const uint32_t* bpair;
     uint32_t* worklist;       // optional: ids of walkers whose (prev, cur) blocks are resident
-    uint32_t* nwork;          // append counter of the worklist
+    uint32_t* nwork;          // append counter of the worklist: every runnable walker, also past work_cap
+    uint32_t work_cap;        // worklist entries (ids past it are counted, not written)
     uint32_t* bucket;         // optional nblocks*nblocks counters
     unsigned long long* left; // unfinished walkers
     unsigned long long* hops; // optional: sum of the remaining hop caps of unfinished walkers

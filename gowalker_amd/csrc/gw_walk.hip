@@ -393,6 +393,7 @@ __device__ __forceinline__ void work_size(const WalkArgs& a, uint32_t& nwork, ui
         return;
     }
     nwork = *a.nwork_dev;
+    if (nwork > a.nwork) nwork = a.nwork;   // the filter counts runnable walkers past the worklist's end
     const uint64_t waves = (uint64_t)gridDim.x * blockDim.x / 64;
     uint64_t ch = nwork / (waves * 8 + 1);
     ch = ch < 64 ? 64 : (ch > 1024 ? 1024 : ch);
@@ -1506,7 +1507,8 @@ constexpr uint32_t kFilterThreads = 1024, kFilterItems = 16;
 // tile + k * kFilterThreads + t), so each lane has 16 status words and 16 block pairs in
 // flight before it waits; the resident set is a bit table in LDS.  Unfinished walkers are
 // counted into the (prev_blk, cur_blk) histogram (in LDS when the matrix fits) and, when a
-// worklist is requested, the runnable ones are appended with one atomic per tile.  An LDS
+// worklist is requested, the runnable ones are appended with one atomic per tile (ids past
+// work_cap are counted but not written: the run takes another round for them).  An LDS
 // histogram leaves the workgroup as one row of plain stores into bucket_part, summed by
 // bucket_reduce_kernel: a device-scope atomic per bin and workgroup (1.5M of them on a
 // 54-block table) made the pass 2x slower and held up host-to-device copies meanwhile.
@@ -1575,8 +1577,10 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
             uint32_t off = tile_base + wave_cnt[wave];
 #pragma unroll
             for (uint32_t k = 0; k < kFilterItems; ++k) {
-                if ((masks[k] >> lane) & 1ull)
-                    a.worklist[off + lane_rank(masks[k])] = (uint32_t)(t0 + (uint64_t)k * blockDim.x + threadIdx.x);
+                if ((masks[k] >> lane) & 1ull) {
+                    const uint32_t at = off + lane_rank(masks[k]);
+                    if (at < a.work_cap) a.worklist[at] = (uint32_t)(t0 + (uint64_t)k * blockDim.x + threadIdx.x);
+                }
                 off += (uint32_t)__popcll(masks[k]);
             }
             __syncthreads();   // wave_cnt / tile_base are reused by the next tile

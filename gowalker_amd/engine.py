@@ -285,6 +285,16 @@ class Context:
         self._chk(lib().gw_upload_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
+    def set_worklist_cap(self, entries):
+        """worklist entries from the next init_walkers (0: nwalkers / 8, at least min(nwalkers, 2^20))"""
+        self._chk(lib().gw_set_worklist_cap(self.h, entries))
+
+    def worklist_stats(self):
+        """(worklist capacity, further filter + step rounds taken since the context was made)"""
+        cap, rounds = C.c_uint64(), C.c_uint64()
+        self._chk(lib().gw_worklist_stats(self.h, C.byref(cap), C.byref(rounds)))
+        return cap.value, rounds.value
+
     def graph_stats(self):
         """(runs replayed from captured graphs, graphs captured)"""
         r, c = C.c_uint64(), C.c_uint64()

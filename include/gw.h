@@ -352,6 +352,15 @@ gw_status gw_graph_stats(gw_ctx* ctx, uint64_t* replays, uint64_t* captures);
  * or headers larger than a head area).  GW_PACK_INPLACE=0 in the environment at gw_create
  * sends every payload through the ring.  Waits until every upload asked for is enqueued. */
 gw_status gw_upload_stats(gw_ctx* ctx, uint64_t* inplace, uint64_t* ring, uint64_t* copy_stream);
+/* Worklist capacity in walker ids, taking effect at the next gw_init_walkers (no reference
+ * counterpart: the reference's walker and bucket arrays hold every walker,
+ * gpu/memory.hpp:15-20).
+ * 0 = the default, nwalkers / 8 (at least min(nwalkers, 2^20)).  A gw_run whose runnable
+ * walkers exceed it runs further filter + step rounds until none is left, with the same
+ * walks; the bytes saved (4 per walker) are left to block slots. */
+gw_status gw_set_worklist_cap(gw_ctx* ctx, uint64_t entries);
+/* The worklist's capacity and the further rounds gw_run has taken since gw_create. */
+gw_status gw_worklist_stats(gw_ctx* ctx, uint64_t* capacity, uint64_t* extra_rounds);
 
 /* ---- synthetic graphs generated in HBM (the reference downloads real datasets,
  *      script/download_*.sh; its text->CSR converter cannot run as shipped,

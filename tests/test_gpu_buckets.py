@@ -85,3 +85,60 @@ def test_bucket_matrix_matches_walker_histogram(gpu, graph, block_bytes, mixed):
         ctx.wait_uploads()
         assert ctx.run(buckets=True)["walkers_left"] == 0
         assert check(ctx, bl, "after the walk") == 0
+
+
+@pytest.mark.parametrize("variant,window", [(0, False), (2, True)])
+def test_short_worklist_takes_rounds_with_the_same_walks(gpu, variant, window):
+    """A worklist shorter than a run's runnable walkers (gw_set_worklist_cap) makes gw_run
+    take further filter + step rounds for the walkers it could not hold.  After every run,
+    with a part of the blocks resident each time, the walker state, the run's step / walker
+    counts and the bucket matrix equal those of the full-size list, and the final hashes
+    too; the short lists took extra rounds and the full one none."""
+    beg, csr = G.rmat(13, 30000, seed=8, undirected=False)
+    n = len(beg) - 1
+    bl = Blocks(beg, csr, 4096)
+    nb = bl.nblocks
+    types = [dict(algo=NODE2VEC, length=25, p=0.5, q=2.0), dict(algo=SOPR, length=25, alpha=0.2, term=0.15)]
+    logs, finals = [], []
+    for cap in (0, 64, 1000):
+        rng = np.random.default_rng(77)
+        log = []
+        with Context() as ctx:
+            ctx.set_worklist_cap(cap)
+            ctx.set_blocks(bl)
+            ctx.set_walk(types, 31)
+            ctx.init_walkers(n * 2, 2, outputs=OUT_HASH)
+            wcap, rounds0 = ctx.worklist_stats()
+            assert wcap == (cap or n * 2)
+            k = max(2, nb // 3)
+            ctx.reserve_slots(k)
+            for it in range(6):
+                resident = sorted(rng.choice(nb, k, replace=False).tolist())
+                for s in range(k):
+                    ctx.evict(s)
+                for s, b in enumerate(resident):
+                    ctx.upload(s, b)
+                ctx.wait_uploads()
+                st = ctx.run(buckets=True, variant=variant, window=window)
+                log.append((st["steps"], st["trials"], st["walkers_run"], st["walkers_left"]))
+                check(ctx, bl, "cap %d run %d" % (cap, it))
+                log.append(ctx.state().copy())
+            ctx.reserve_slots(nb)
+            for b in range(nb):
+                ctx.upload(b, b)
+            ctx.wait_uploads()
+            assert ctx.run(buckets=True, variant=variant, window=window)["walkers_left"] == 0
+            _, rounds = ctx.worklist_stats()
+            print("cap", cap, "capacity", wcap, "extra rounds", rounds - rounds0, flush=True)
+            assert (rounds - rounds0 > 0) == (cap != 0)
+            finals.append(ctx.hashes())
+        logs.append(log)
+    for log in logs[1:]:
+        assert len(log) == len(logs[0])
+        for x, y in zip(logs[0], log):
+            if isinstance(x, tuple):
+                assert x == y
+            else:
+                np.testing.assert_array_equal(x, y)
+    for h in finals[1:]:
+        np.testing.assert_array_equal(finals[0], h)
