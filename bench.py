@@ -81,10 +81,10 @@ def search_mode(launch):
 
 def algorithmic_bytes(st, nwalkers, visits):
     """Bytes the step kernel must move by the algorithm's definition:
-    per walker: worklist id 4 + state read 16 + state write 16 + the two adjacency rows
-    (beg[v], beg[v+1], beg[u], beg[u+1]) 32; per step: next row 16 (+ visit RMW 8);
-    per trial: candidate 4; per probe: 4."""
-    b = nwalkers * (4 + 16 + 16 + 32) + st["steps"] * (16 + (8 if visits else 0))
+    per walker: worklist id 4 + state read 12 (cur, prev, meta) + state write 12 + the two
+    adjacency rows (beg[v], beg[v+1], beg[u], beg[u+1]) 32; per step: next row 16 (+ visit
+    RMW 8); per trial: candidate 4; per probe: 4."""
+    b = nwalkers * (4 + 12 + 12 + 32) + st["steps"] * (16 + (8 if visits else 0))
     b += st["trials"] * 4 + st["probes"] * 4
     return b
 
@@ -92,9 +92,9 @@ def algorithmic_bytes(st, nwalkers, visits):
 def sector_bytes(st, nwalkers, visits):
     """The same accesses priced at the 64-byte granularity of a random HBM touch: every
     row lookup, candidate and probe is one sector, a visit RMW two (read + write-back);
-    walker state streams (40 B per walker)."""
+    walker state streams (worklist id, state read and written: 28 B per walker)."""
     touches = st["steps"] * (1 + (2 if visits else 0)) + st["trials"] + st["probes"] + nwalkers * 2
-    return touches * 64 + nwalkers * 40
+    return touches * 64 + nwalkers * 28
 
 
 def build_graph(ctx, g, seed):

@@ -125,7 +125,7 @@ SIGNATURES = {
                                  P, C.POINTER(GwRunStats)],
 }
 GW_NOALLOC_MAX_WALKERS = 1 << 20
-GW_NOALLOC_SCRATCH_BYTES = 1024 + 16 * GW_NOALLOC_MAX_WALKERS
+GW_NOALLOC_SCRATCH_BYTES = 1024 + 12 * GW_NOALLOC_MAX_WALKERS
 _RESTYPE = {"gw_last_error": C.c_char_p}
 
 _lib = None

@@ -147,7 +147,7 @@ struct gw_ctx {
     uint64_t nwalkers = 0;
     uint32_t outputs = 0;
     uint32_t path_stride = 0;
-    uint32_t *d_cur = nullptr, *d_prev = nullptr, *d_meta = nullptr, *d_bpair = nullptr;
+    uint32_t *d_cur = nullptr, *d_prev = nullptr, *d_meta = nullptr;
     uint64_t* d_hash = nullptr;
     uint32_t* d_paths = nullptr;
     uint32_t* d_visits = nullptr;
@@ -265,12 +265,11 @@ static void free_walkers(gw_ctx* c)
     dfree(c, c->d_cur, c->nwalkers * 4, true);
     dfree(c, c->d_prev, c->nwalkers * 4, true);
     dfree(c, c->d_meta, c->nwalkers * 4, true);
-    dfree(c, c->d_bpair, c->nwalkers * 4, true);
     dfree(c, c->d_work, c->work_cap * 4, true);
     dfree(c, c->d_hash, c->nwalkers * 8, true);
     dfree(c, c->d_paths, c->nwalkers * (uint64_t)c->path_stride * 4, true);
     dfree(c, c->d_visits, c->nverts_total * 4 * c->visit_parts, true);
-    c->d_cur = c->d_prev = c->d_meta = c->d_bpair = c->d_work = c->d_paths = c->d_visits = nullptr;
+    c->d_cur = c->d_prev = c->d_meta = c->d_work = c->d_paths = c->d_visits = nullptr;
     c->d_hash = nullptr;
     c->nwalkers = 0;
     c->work_cap = 0;
@@ -1451,7 +1450,7 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
         void* p;
         gw_status st;
         c->nwalkers = n;
-        uint32_t** u32s[] = {&c->d_cur, &c->d_prev, &c->d_meta, &c->d_bpair};
+        uint32_t** u32s[] = {&c->d_cur, &c->d_prev, &c->d_meta};
         for (uint32_t** q : u32s) {
             if ((st = dalloc(c, &p, n * 4, true)) != GW_OK) { free_walkers(c); return st; }
             *q = (uint32_t*)p;
@@ -1484,15 +1483,13 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
         HIPCHK(c, hipMemcpy(d_src, sources, n * 4, hipMemcpyHostToDevice));
     }
     InitArgs a{};
-    a.vcut = c->d_vcut;
-    a.nblocks = c->nblocks;
     a.ntypes = c->wp.ntypes;
     a.len0 = c->dt[0].length;
     a.len1 = c->dt[1].length;
     a.nwalkers = n;
     a.wpv = wpv ? wpv : 1;
     a.sources = d_src;
-    a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta; a.bpair = c->d_bpair;
+    a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta;
     a.hash = c->d_hash; a.paths = c->d_paths; a.path_stride = c->path_stride;
     hipError_t e = launch_init(a, c->s_compute);
     if (e == hipSuccess) e = hipStreamSynchronize(c->s_compute);
@@ -1522,7 +1519,9 @@ static gw_status enqueue_bucket_pass(gw_ctx* c, bool stamps = false)
     f.nblocks = c->nblocks;
     f.nwalkers = c->nwalkers;
     f.meta = c->d_meta;
-    f.bpair = c->d_bpair;
+    f.cur = c->d_cur;
+    f.prev = c->d_prev;
+    f.vcut = c->d_vcut;
     f.bucket = c->d_bucket;
     f.left = c->d_cnt + 6;
     f.hops = c->d_cnt + 8;
@@ -1674,7 +1673,9 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     f.nblocks = c->nblocks;
     f.nwalkers = cap && cap < c->nwalkers ? cap : c->nwalkers;
     f.meta = c->d_meta;
-    f.bpair = c->d_bpair;
+    f.cur = c->d_cur;
+    f.prev = c->d_prev;
+    f.vcut = c->d_vcut;
     f.worklist = c->d_work;
     f.work_cap = (uint32_t)c->work_cap;
     f.nwork = c->d_misc;
@@ -1691,13 +1692,12 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     a.seed_hi = (uint32_t)(c->wp.seed >> 32);
     a.type0 = c->dt[0];
     a.type1 = c->dt[1];
-    a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta; a.bpair = c->d_bpair;
+    a.cur = c->d_cur; a.prev = c->d_prev; a.meta = c->d_meta;
     a.hash = c->d_hash; a.paths = c->d_paths; a.path_stride = c->path_stride; a.visits = c->d_visits;
     a.visits_stride = c->nverts_total;
     a.visits_parts = c->d_visits ? c->visit_parts : 1u;
     a.worklist = all_res ? nullptr : c->d_work;
     a.nwork_dev = all_res ? nullptr : c->d_misc;
-    a.scan = 0u;
     a.zc_count = mapped ? 1u : 0u;
     a.search_end = (l.flags & GW_RUN_INTERP) ? (uint32_t)c->nverts_total : 0u;
     a.search_win = (l.flags & GW_RUN_WINDOW) ? 1u : 0u;
@@ -1853,8 +1853,7 @@ gw_status gw_autotune(gw_ctx* c, uint64_t sample, uint32_t what, gw_launch* best
     HIPCHK(c, hipStreamSynchronize(c->s_compute));
     const uint64_t n = c->nwalkers;
     struct Snap { void* dev; void* save; uint64_t bytes; };
-    std::vector<Snap> snaps = {{c->d_cur, nullptr, n * 4}, {c->d_prev, nullptr, n * 4}, {c->d_meta, nullptr, n * 4},
-                               {c->d_bpair, nullptr, n * 4}};
+    std::vector<Snap> snaps = {{c->d_cur, nullptr, n * 4}, {c->d_prev, nullptr, n * 4}, {c->d_meta, nullptr, n * 4}};
     if (c->d_hash) snaps.push_back({c->d_hash, nullptr, n * 8});
     if (c->d_visits) snaps.push_back({c->d_visits, nullptr, c->nverts_total * 4 * c->visit_parts});
     gw_status st = GW_OK;

@@ -62,7 +62,6 @@ struct WalkArgs {
     uint32_t* cur;
     uint32_t* prev;
     uint32_t* meta;
-    uint32_t* bpair;          // cur block in bits 0..15, prev block in 16..31
     uint64_t* hash;           // optional
     uint32_t* paths;          // optional, path_stride entries per walker
     uint32_t path_stride;
@@ -73,7 +72,6 @@ struct WalkArgs {
     const uint32_t* worklist; // walker ids; nullptr = identity (finished walkers are skipped)
     uint32_t nwork;
     const uint32_t* nwork_dev;  // if set, the worklist length is read here (written by the filter pass)
-    uint32_t scan;            // identity list: also skip walkers whose blocks are not both resident
     uint32_t zc_count;        // some block is host-mapped: count zero-copy reads (counters[7])
     uint32_t search_end;      // 0: membership by bisection; else interpolation, ids below it
     uint32_t search_win;      // probes read aligned 16-entry windows
@@ -84,8 +82,6 @@ struct WalkArgs {
 };
 
 struct InitArgs {
-    const uint32_t* vcut;
-    uint32_t nblocks;
     uint32_t ntypes;
     uint32_t len0, len1;
     uint64_t nwalkers;
@@ -94,7 +90,6 @@ struct InitArgs {
     uint32_t* cur;
     uint32_t* prev;
     uint32_t* meta;
-    uint32_t* bpair;
     uint64_t* hash;
     uint32_t* paths;
     uint32_t path_stride;
@@ -105,7 +100,9 @@ struct FilterArgs {
     uint32_t nblocks;
     uint64_t nwalkers;
     const uint32_t* meta;
-    const uint32_t* bpair;
+    const uint32_t* cur;      // a walker's (prev, cur) blocks come from its vertices and the cut points
+    const uint32_t* prev;
+    const uint32_t* vcut;
     uint32_t* worklist;       // optional: ids of walkers whose (prev, cur) blocks are resident
     uint32_t* nwork;          // append counter of the worklist: every runnable walker, also past work_cap
     uint32_t work_cap;        // worklist entries (ids past it are counted, not written)

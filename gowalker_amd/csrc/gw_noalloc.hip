@@ -3,7 +3,7 @@
 // capture, tight serving loops, harnesses that forbid hipMalloc).  The graph, the hashes
 // and the visit counts are the caller's device buffers; everything else the kernels
 // touch -- the block descriptor, the cut points, the counters, the work queue and the
-// walker state (cur, prev, meta, block pair: 16 B per walker) -- lives in `scratch`, a
+// walker state (cur, prev, meta: 12 B per walker) -- lives in `scratch`, a
 // fixed file-scope device array.  Each GPU holds its own copy of the array from the first
 // call on it until the process ends; its device address is looked up once per device
 // (hipGetSymbolAddress) and passed to the kernels like any buffer.  Calls on one GPU share
@@ -17,7 +17,7 @@
 namespace gw {
 
 constexpr uint64_t kNoallocHead = 1024;   // descriptor, cut points, counters, queue
-constexpr uint64_t kNoallocScratch = kNoallocHead + 16ull * GW_NOALLOC_MAX_WALKERS;
+constexpr uint64_t kNoallocScratch = kNoallocHead + 12ull * GW_NOALLOC_MAX_WALKERS;
 static_assert(kNoallocScratch == GW_NOALLOC_SCRATCH_BYTES, "gw.h states the scratch size");
 // offsets in the head
 constexpr uint64_t kOffBlock = 0, kOffVcut = 64, kOffCnt = 128, kOffMisc = 256;
@@ -153,8 +153,6 @@ extern "C" gw_status gw_walk_resident_noalloc(const uint64_t* beg, const uint32_
     if (visits) NCHK(launch_fill_u32(visits, 0, nverts, s));
 
     InitArgs ia{};
-    ia.vcut = vcut;
-    ia.nblocks = 1;
     ia.ntypes = params->ntypes;
     ia.len0 = dt[0].length;
     ia.len1 = dt[1].length;
@@ -163,7 +161,6 @@ extern "C" gw_status gw_walk_resident_noalloc(const uint64_t* beg, const uint32_
     ia.cur = soa;
     ia.prev = soa + cap;
     ia.meta = soa + 2 * cap;
-    ia.bpair = soa + 3 * cap;
     ia.hash = hashes;
     NCHK(launch_init(ia, s));
 
@@ -179,7 +176,6 @@ extern "C" gw_status gw_walk_resident_noalloc(const uint64_t* beg, const uint32_
     a.cur = ia.cur;
     a.prev = ia.prev;
     a.meta = ia.meta;
-    a.bpair = ia.bpair;
     a.hash = hashes;
     a.visits = visits;
     a.visits_stride = nverts;

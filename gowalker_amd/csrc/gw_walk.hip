@@ -376,11 +376,6 @@ __device__ __forceinline__ Row load_row(const DevBlock& b, uint32_t v)
     return Row{ b.csr + h, t - h };
 }
 
-// Both blocks of a walker's (prev, cur) pair are readable (HBM slot or host-mapped).
-__device__ __forceinline__ bool runnable(const DevBlock* __restrict__ blocks, uint32_t pair)
-{
-    return blocks[pair & 0xFFFFu].resident && blocks[pair >> 16].resident;
-}
 
 // Worklist length and work-queue chunk: from the host, or, after a filter pass on the
 // same stream, from device memory (no host round trip); every wavefront derives the same
@@ -476,16 +471,17 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
                 const uint32_t m = idx < nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
                 if (idx >= nwork) {
                     exhausted = true;
-                } else if (((m >> kStatusShift) & 0xFu) == kActive &&
-                           (!a.scan || runnable(blocks, a.bpair[a.worklist ? a.worklist[idx] : idx]))) {
+                } else if (((m >> kStatusShift) & 0xFu) == kActive) {
                     w = a.worklist ? a.worklist[idx] : idx;
                     cur = a.cur[w];
                     prev = a.prev[w];
                     hop = m & kHopMask;
                     type = (m >> kTypeShift) & 1u;
-                    const uint32_t pr = a.bpair[w];
-                    bc = pr & 0xFFFFu;
-                    bp = pr >> 16;
+                    if (!ONE) {   // the pair's blocks, from the cut points (no stored copy)
+                        bc = LB ? find_block_lut(slut, lut_shift, vcut, a.nblocks, cur) : find_block(vcut, a.nblocks, cur);
+                        bp = prev == cur ? bc
+                             : (LB ? find_block_lut(slut, lut_shift, vcut, a.nblocks, prev) : find_block(vcut, a.nblocks, prev));
+                    }
                     if (a.hash) hash = a.hash[w];
                     if (ONE) {
                         rv = load_row(one, cur);
@@ -557,7 +553,6 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             a.cur[w] = cur;
             a.prev[w] = prev;
             a.meta[w] = hop | (status << kStatusShift) | (type << kTypeShift);
-            a.bpair[w] = bc | (bp << 16);
             if (a.hash) a.hash[w] = hash;
             if (status != kActive) ++c.ended; else ++c.left;
             have = false;
@@ -761,7 +756,6 @@ __device__ __forceinline__ void flat_advance(FlatLane& L, const WalkArgs& a, con
             a.cur[L.w] = L.cur;
             a.prev[L.w] = L.prev;
             a.meta[L.w] = L.hop | (L.status << kStatusShift) | (L.type << kTypeShift);
-            a.bpair[L.w] = L.bc | (L.bp << 16);
             if (a.hash) a.hash[L.w] = L.hash;
             if (L.status != kActive) ++c.ended; else ++c.left;
             L.phase = PH_IDLE;
@@ -968,16 +962,14 @@ __global__ void __launch_bounds__(256) walk_flat_kernel(WalkArgs a)
                 const uint32_t m = idx < nwork ? a.meta[a.worklist ? a.worklist[idx] : idx] : 0u;
                 if (idx >= nwork) {
                     exhausted = true;
-                } else if (((m >> kStatusShift) & 0xFu) == kActive &&
-                           (!a.scan || runnable(a.blocks, a.bpair[a.worklist ? a.worklist[idx] : idx]))) {
+                } else if (((m >> kStatusShift) & 0xFu) == kActive) {
                     L.w = a.worklist ? a.worklist[idx] : idx;
                     L.cur = a.cur[L.w];
                     L.prev = a.prev[L.w];
                     L.hop = m & kHopMask;
                     L.type = (m >> kTypeShift) & 1u;
-                    const uint32_t pr = a.bpair[L.w];
-                    L.bc = pr & 0xFFFFu;
-                    L.bp = pr >> 16;
+                    L.bc = ONE ? 0u : find_block(a.vcut, a.nblocks, L.cur);
+                    L.bp = ONE || L.prev == L.cur ? L.bc : find_block(a.vcut, a.nblocks, L.prev);
                     L.hash = a.hash ? a.hash[L.w] : 0;
                     const Row rv = ONE ? load_row(one, L.cur) : load_row(a.blocks[L.bc], L.cur);
                     const Row ru = ONE ? load_row(one, L.prev) : load_row(a.blocks[L.bp], L.prev);
@@ -1138,11 +1130,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
     uint32_t* w_cur = a.cur;
     uint32_t* w_prev = a.prev;
     uint32_t* w_meta = a.meta;
-    uint32_t* w_bpair = a.bpair;
     uint64_t* w_hash = a.hash;
     const uint32_t* w_list = a.worklist;
     if constexpr (SM == 3)
-        asm volatile("" : "+v"(w_cur), "+v"(w_prev), "+v"(w_meta), "+v"(w_bpair), "+v"(w_hash), "+v"(w_list),
+        asm volatile("" : "+v"(w_cur), "+v"(w_prev), "+v"(w_meta), "+v"(w_hash), "+v"(w_list),
                      "+v"(k0), "+v"(k1));
     Lane L{};
     L.ph = LS_IDLE;
@@ -1186,7 +1177,6 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             L.meta = w_meta[L.w];
             L.cur = w_cur[L.w];
             L.prev = w_prev[L.w];
-            L.bcp = w_bpair[L.w];
             L.hash = w_hash ? w_hash[L.w] : 0;
         } else if (L.ph == LS_ROWS) {
             const uint64_t* pc = (const uint64_t*)L.addr;
@@ -1211,12 +1201,22 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             L.ph = LS_STATE;
             break;
         case LS_STATE: {
-            const DevBlock& B = ONE ? one : blocks[L.bcp & 0xFFFFu];
-            const DevBlock& P = ONE ? one : blocks[L.bcp >> 16];
-            if (((L.meta >> kStatusShift) & 0xFu) != kActive || (a.scan && !(B.resident && P.resident))) {
-                L.ph = LS_IDLE;   // not runnable: left untouched
+            if (((L.meta >> kStatusShift) & 0xFu) != kActive) {
+                L.ph = LS_IDLE;   // finished: left untouched
                 break;
             }
+            if (!ONE) {   // the pair's blocks, from the cut points (no stored copy)
+                const uint32_t bc = LB ? find_block_lut(slut, lut_shift, vcut, a.nblocks, L.cur)
+                                       : find_block(vcut, a.nblocks, L.cur);
+                const uint32_t bp = L.prev == L.cur ? bc
+                                    : (LB ? find_block_lut(slut, lut_shift, vcut, a.nblocks, L.prev)
+                                          : find_block(vcut, a.nblocks, L.prev));
+                L.bcp = bc | (bp << 16);
+            } else {
+                L.bcp = 0;
+            }
+            const DevBlock& B = ONE ? one : blocks[L.bcp & 0xFFFFu];
+            const DevBlock& P = ONE ? one : blocks[L.bcp >> 16];
             L.addr = B.beg + (L.cur - B.start_vert);
             L.nu = (const uint32_t*)(P.beg + (L.prev - P.start_vert));
             L.fl = (B.resident == kMapped ? kLaneMv : 0u) | (P.resident == kMapped ? kLaneMu : 0u);
@@ -1451,7 +1451,6 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             w_cur[L.w] = L.cur;
             w_prev[L.w] = L.prev;
             w_meta[L.w] = L.meta;
-            w_bpair[L.w] = L.bcp;
             if (w_hash) w_hash[L.w] = L.hash;
             if (((L.meta >> kStatusShift) & 0xFu) != kActive) ++c.ended; else ++c.left;
             L.ph = LS_IDLE;
@@ -1484,11 +1483,9 @@ __global__ void __launch_bounds__(256) init_kernel(InitArgs a)
         const uint32_t src = a.sources ? a.sources[w] : (uint32_t)(w / a.wpv);
         const uint32_t type = a.ntypes == 2 ? (uint32_t)(w & 1) : 0u;
         const uint32_t len = type ? a.len1 : a.len0;
-        const uint32_t b = find_block(a.vcut, a.nblocks, src);
         a.cur[w] = src;
         a.prev[w] = src;
         a.meta[w] = (len == 0 ? (kDoneLength << kStatusShift) : 0u) | (type << kTypeShift);
-        a.bpair[w] = b | (b << 16);
         if (a.hash) a.hash[w] = (kFnvOffset ^ src) * kFnvPrime;
         if (a.paths) {
             uint32_t* p = a.paths + w * a.path_stride;
@@ -1501,11 +1498,14 @@ __global__ void __launch_bounds__(256) init_kernel(InitArgs a)
 // The bucket matrix is the count matrix that prefix() scans in gpuwalk.hpp:321-338; the
 // worklist replaces insertglobal's scatter (gpuwalk.hpp:340-372).
 constexpr uint32_t kFilterThreads = 1024, kFilterItems = 16;
+constexpr uint32_t kFilterLdsCuts = 2048;   // cut points staged in LDS (with a lookup table) up to this many blocks
 
 // Bucket / worklist pass over all walkers.  A workgroup takes a tile of
 // kFilterThreads * kFilterItems walkers (coalesced: item k of thread t is walker
-// tile + k * kFilterThreads + t), so each lane has 16 status words and 16 block pairs in
-// flight before it waits; the resident set is a bit table in LDS.  Unfinished walkers are
+// tile + k * kFilterThreads + t), so each lane has 16 status words, then the vertex pairs
+// of the unfinished ones, in flight before it waits.  A walker's (prev, cur) blocks come
+// from its vertices through the cut points' lookup table in LDS, and the resident set is
+// a bit table in LDS.  Unfinished walkers are
 // counted into the (prev_blk, cur_blk) histogram (in LDS when the matrix fits) and, when a
 // worklist is requested, the runnable ones are appended with one atomic per tile (ids past
 // work_cap are counted but not written: the run takes another round for them).  An LDS
@@ -1518,9 +1518,21 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
     __shared__ uint32_t wave_cnt[kFilterThreads / 64];
     __shared__ uint32_t tile_base;
     __shared__ uint32_t res_bits[kFilterLdsResident / 32];
+    __shared__ uint32_t svcut[kFilterLdsCuts + 1];
+    __shared__ uint16_t slut[kLutSize];
     const uint32_t nbb = a.nblocks * a.nblocks;
     const bool lds_hist = a.bucket && nbb <= kFilterLdsBuckets;
     const bool lds_res = a.worklist && a.nblocks <= kFilterLdsResident;
+    const bool lds_cuts = a.nblocks <= kFilterLdsCuts;
+    uint32_t lut_shift = 0;
+    if (lds_cuts) {
+        for (uint32_t i = threadIdx.x; i <= a.nblocks; i += blockDim.x) svcut[i] = a.vcut[i];
+        __syncthreads();
+        const uint32_t top = svcut[a.nblocks] ? svcut[a.nblocks] - 1 : 0;
+        while ((top >> lut_shift) >= kLutSize) ++lut_shift;
+        for (uint32_t i = threadIdx.x; i < kLutSize; i += blockDim.x)
+            slut[i] = (uint16_t)find_block(svcut, a.nblocks, i << lut_shift);
+    }
     if (lds_hist)
         for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x) hist[i] = 0;
     if (lds_res) {
@@ -1534,7 +1546,7 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
     const uint64_t tile = (uint64_t)blockDim.x * kFilterItems;
     uint64_t left = 0, hops = 0;
     for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < a.nwalkers; t0 += (uint64_t)gridDim.x * tile) {
-        uint32_t m[kFilterItems], pr[kFilterItems];
+        uint32_t m[kFilterItems], vc[kFilterItems], vp[kFilterItems];
 #pragma unroll
         for (uint32_t k = 0; k < kFilterItems; ++k) {
             const uint64_t w = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
@@ -1543,7 +1555,9 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
 #pragma unroll
         for (uint32_t k = 0; k < kFilterItems; ++k) {
             const uint64_t w = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
-            pr[k] = ((m[k] >> kStatusShift) & 0xFu) == kActive ? a.bpair[w] : 0u;
+            const bool act = ((m[k] >> kStatusShift) & 0xFu) == kActive;
+            vc[k] = act ? a.cur[w] : 0u;
+            vp[k] = act ? a.prev[w] : 0u;
         }
         uint64_t masks[kFilterItems];
         uint32_t cnt = 0;
@@ -1555,7 +1569,11 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
                 const uint32_t len = ((m[k] >> kTypeShift) & 1u) ? a.len1 : a.len0;
                 const uint32_t hop = m[k] & kHopMask;
                 hops += len > hop ? len - hop : 0u;
-                const uint32_t bc = pr[k] & 0xFFFFu, bp = pr[k] >> 16;
+                const uint32_t bc = lds_cuts ? find_block_lut(slut, lut_shift, svcut, a.nblocks, vc[k])
+                                             : find_block(a.vcut, a.nblocks, vc[k]);
+                const uint32_t bp = vp[k] == vc[k] ? bc
+                                    : (lds_cuts ? find_block_lut(slut, lut_shift, svcut, a.nblocks, vp[k])
+                                                : find_block(a.vcut, a.nblocks, vp[k]));
                 if (a.worklist)
                     take = lds_res ? ((res_bits[bc >> 5] >> (bc & 31)) & (res_bits[bp >> 5] >> (bp & 31)) & 1u) != 0
                                    : a.blocks[bc].resident && a.blocks[bp].resident;

@@ -325,7 +325,7 @@ gw_status gw_block_scores(gw_ctx* ctx, gw_mlp* m, const float* age, const float*
  *      allocated; calls that overlap on one GPU share its array and race.  Errors are read
  *      with gw_last_error(NULL). ---- */
 #define GW_NOALLOC_MAX_WALKERS (1u << 20)
-#define GW_NOALLOC_SCRATCH_BYTES (1024ull + 16ull * GW_NOALLOC_MAX_WALKERS)
+#define GW_NOALLOC_SCRATCH_BYTES (1024ull + 12ull * GW_NOALLOC_MAX_WALKERS)
 gw_status gw_walk_resident_noalloc(const uint64_t* beg, const uint32_t* csr, uint32_t nverts, uint64_t nedges,
                                    const gw_walk_params* params, uint64_t nwalkers, uint32_t walkers_per_vertex,
                                    uint64_t* hashes, uint32_t* visits, void* stream, gw_run_stats* stats);

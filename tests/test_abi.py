@@ -99,8 +99,8 @@ def test_noalloc_walk_rejects_sizes_beyond_its_array():
     from gowalker_amd.engine import walk_params
     hdr = open(os.path.join(REPO, "include", "gw.h")).read()
     assert "#define GW_NOALLOC_MAX_WALKERS (1u << 20)" in hdr
-    assert "#define GW_NOALLOC_SCRATCH_BYTES (1024ull + 16ull * GW_NOALLOC_MAX_WALKERS)" in hdr
-    assert GW_NOALLOC_SCRATCH_BYTES == 1024 + 16 * (1 << 20) and GW_NOALLOC_MAX_WALKERS == 1 << 20
+    assert "#define GW_NOALLOC_SCRATCH_BYTES (1024ull + 12ull * GW_NOALLOC_MAX_WALKERS)" in hdr
+    assert GW_NOALLOC_SCRATCH_BYTES == 1024 + 12 * (1 << 20) and GW_NOALLOC_MAX_WALKERS == 1 << 20
     L = lib()
     wp = walk_params([dict(algo=0, length=10, p=1.0, q=1.0)])
     fake = C.c_void_p(0x1000)   # never dereferenced: the checks come first

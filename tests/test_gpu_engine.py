@@ -41,12 +41,13 @@ def test_out_of_core_matches_oracle(gpu, sched, memory_blocks):
     block_bytes = 16384
     b, c = pinned_graph(beg, csr)
     types = [dict(algo=E.NODE2VEC, length=20, p=0.5, q=2.0)]
-    # budget = walker state (5 u32 + a u64 hash per walker, a u32 visit count per vertex)
+    # budget = walker state (cur, prev, meta and a worklist entry: 4 u32, a u64 hash per walker,
+    # a u32 visit count per vertex)
     # + `memory_blocks` slots
     with E.Context() as ctx:
         ctx.set_blocks(E.Blocks(beg, csr, block_bytes))
         slot = ctx.slot_bytes()
-    walker_bytes = n * (5 * 4 + 8) + n * 4
+    walker_bytes = n * (4 * 4 + 8) + n * 4
     mem = walker_bytes + memory_blocks * slot + 4096
     rep, h, v = E.run_out_of_core(b.a, c.a, types, 9, n, memory_bytes=mem, block_bytes=block_bytes,
                                   scheduler=sched, hashes=True, visits=True)
@@ -72,7 +73,7 @@ def test_zero_copy_tail_and_sopr(gpu):
     with E.Context() as ctx:
         ctx.set_blocks(E.Blocks(beg, csr, 65536))
         slot = ctx.slot_bytes()
-    mem = (n * 2) * (5 * 4 + 8) + n * 4 + 3 * slot + 4096
+    mem = (n * 2) * (4 * 4 + 8) + n * 4 + 3 * slot + 4096
     rep, h, v = E.run_out_of_core(b.a, c.a, types, 4, n * 2, wpv=2, memory_bytes=mem, block_bytes=65536,
                                   zero_threshold=2000, hashes=True, visits=True)
     o = oracle_hashes(beg, csr, types, 4, n * 2, 2)
@@ -97,7 +98,7 @@ def test_tight_budget_over_block_sizes(gpu):
         with E.Context() as ctx:
             ctx.set_blocks(E.Blocks(beg, csr, block_bytes))
             slot = ctx.slot_bytes()
-        mem = n * (5 * 4 + 8) + 3 * slot + 64
+        mem = n * (4 * 4 + 8) + 3 * slot + 64
         rep, h, _ = E.run_out_of_core(b.a, c.a, types, 6, n, memory_bytes=mem, block_bytes=block_bytes,
                                       scheduler=E.SCHED_PLAIN, hashes=True)
         assert rep["nslots"] == 3, (block_bytes, rep["nslots"])

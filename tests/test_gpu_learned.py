@@ -47,7 +47,7 @@ def test_learned_scheduler(gpu):
         ctx.set_blocks(bl)
         slot = ctx.slot_bytes()
         nb = bl.nblocks
-    mem = n * (5 * 4 + 8) + int(0.6 * nb) * slot + 4096
+    mem = n * (4 * 4 + 8) + int(0.6 * nb) * slot + 4096
 
     def walk(sched, model=None, flags=0):
         rep, h, _ = E.run_out_of_core(b.a, c.a, types, 23, n, memory_bytes=mem, block_bytes=block_bytes,

@@ -154,7 +154,7 @@ def test_engine_upload_modes(gpu, mode, monkeypatch):
         ctx.set_blocks(Blocks(beg, csr, block_bytes))
         slot = ctx.slot_bytes()
     pk = E.PackedGraph(b.a, c.a, block_bytes)
-    mem = n * (5 * 4 + 8) + 8 * slot + 65536 + 4 * (pk.info()["raw_bytes"] // max(1, pk.info()["nblocks"]))
+    mem = n * (4 * 4 + 8) + 8 * slot + 65536 + 4 * (pk.info()["raw_bytes"] // max(1, pk.info()["nblocks"]))
     o = oracle(beg, csr, N2V, 5, n, 1)
     for sched, zt in ((E.SCHED_LEARNED, 0), (E.SCHED_PLAIN, block_bytes // 3072)):
         rep, h, _ = E.run_out_of_core(b.a, c.a, N2V, 5, n, memory_bytes=mem, block_bytes=block_bytes,
@@ -195,7 +195,7 @@ def test_engine_with_packed_graph(gpu):
     info = pk.info()
     print("packed", info, flush=True)
     assert info["blocks_packed"] > 0 and info["packed_bytes"] < info["raw_bytes"]
-    mem = n * (5 * 4 + 8) + 8 * slot + 65536 + 4 * (info["raw_bytes"] // max(1, info["nblocks"]))
+    mem = n * (4 * 4 + 8) + 8 * slot + 65536 + 4 * (info["raw_bytes"] // max(1, info["nblocks"]))
     out = {}
     for name, sched in (("plain", E.SCHED_PLAIN), ("staging", E.SCHED_STAGING)):
         for packed in (None, pk):
