@@ -11,9 +11,11 @@ Default workload (Node2vec out of core, learned vs plain block scheduler):
   engine (graph already in pinned host memory).  At 16 GB the learned scheduler, the same
   code without its network (staging), the plain scheduler (the reference's
   max_choose_blocks) and the plain scheduler with the reference's zero-copy tail each run
-  --warmup untimed and --steps timed walks, interleaved; SOWalker and the 8 GB cap run one
-  timed walk per scheduler.  value = steps/s of the learned scheduler at 16 GB; the others,
-  H2D and zero-copy bytes are in config.caps.
+  --warmup untimed and --steps timed walks, interleaved; SOWalker, the plain scheduler with
+  the reference's upload format (beg and csr as stored, plain_raw; every other scheduler
+  sends blocks in the transfer encoding) and the 8 GB cap run one timed walk per scheduler.
+  value = steps/s of the learned scheduler at 16 GB; the others, H2D and zero-copy bytes
+  are in config.caps.
 
 Other workloads (--config): sopr_chung_lu (SOPR on a directed Chung-Lu graph, 41.6M V /
 1.47B arcs, 4 walkers/vertex, term 0.15, fully HBM-resident), node2vec_rmat20 (the
@@ -177,6 +179,8 @@ SCHEDULERS = {
     "staging": ("SCHED_STAGING", False, 0),         # the same staging/prefetch/tail code, no network
     "plain": ("SCHED_PLAIN", False, 0),             # the reference's max_choose_blocks, no zero-copy tail
     "plain_zero": ("SCHED_PLAIN", True, 0),         # ... with the reference's zero-copy tail
+    "plain_raw": ("SCHED_PLAIN", False, 0),         # the reference's scheduler and upload format
+                                                    # (beg and csr as stored, gpu/memory.hpp:124-139)
     "sowalker": ("SCHED_SOWALKER", False, 0),       # the reference's simulated-annealing scheduler
     "learned_model_swap": ("SCHED_LEARNED", False, 4),        # ablation: the network picks swap loads
     "learned_model_swap_myopic": ("SCHED_LEARNED", False, 6),  # ... trained on next-run labels
@@ -187,8 +191,9 @@ SCHEDULERS = {
     "learned_two_lanes": ("SCHED_LEARNED", False, 128),       # ablation: packed uploads on two copy lanes
 }
 HEADLINE = ["learned", "staging", "plain", "plain_zero"]   # timed --warmup + --steps walks each
-ONCE = ["sowalker"]                                        # one timed walk at the headline cap
-EXTRA = ["learned", "staging", "learned_model_swap", "plain_zero", "plain", "sowalker"]   # one timed walk
+RAW = {"plain_raw"}                                        # uploads in the reference's format
+ONCE = ["sowalker", "plain_raw"]                           # one timed walk at the headline cap
+EXTRA = ["learned", "staging", "learned_model_swap", "plain_zero", "plain", "plain_raw", "sowalker"]   # one timed walk
                                                                             # each at other caps
 
 
@@ -214,7 +219,8 @@ def ooc_walk(E, args, cfg, beg, csr, nwalkers, cap_gb, name, models, timed, pack
                                   zero_threshold=block_bytes // 128 // 24 if zero else 0,
                                   scheduler=getattr(E, sched), hashes=hashes, flags=flags, launch=launch,
                                   prefetch_depth=args.prefetch_depth, verbose=args.verbose,
-                                  model=models.setdefault(name, E.Model()), packed=packed)
+                                  model=models.setdefault(name, E.Model()),
+                                  packed=None if name in RAW else packed)
     return rep, h
 
 
@@ -227,7 +233,8 @@ def summarize(reps, zt):
                 h2d_gb=r0["h2d_bytes"] / 1e9, h2d_raw_gb=r0["h2d_raw_bytes"] / 1e9, zero_copy_gb_est=zc_gb,
                 pcie_gb_total=r0["h2d_bytes"] / 1e9 + zc_gb,
                 h2d_gbs=(r0["h2d_bytes"] / 1e9) / (r0["h2d_ms"] / 1e3) if r0["h2d_ms"] else None,
-                iterations=r0["iterations"], uploads=r0["uploads"], staged=r0["staged"],
+                iterations=r0["iterations"], kernel_launches=r0["kernel_launches"], uploads=r0["uploads"],
+                staged=r0["staged"],
                 zero_copy_steps=r0["zero_copy_steps"], kernel_ms=r0["kernel_ms"], copy_ms=r0["h2d_ms"],
                 sched_ms=r0["sched_ms"], learned_decisions=r0["learned_decisions"], fill_model=r0["fill_model"],
                 swap_model=r0["swap_model"], swap_differs=r0["swap_differs"], tail_model=r0["tail_model"],
@@ -240,12 +247,13 @@ def summarize(reps, zt):
 
 def log_walk(cap_gb, name, tag, rep):
     print("[bench] cap %g GB %s %s: %.3f s, %d steps, %.2f GB H2D (%.2f GB raw; %d uploads + %d staged), %.3f GB zero-copy, "
-          "%d iterations, kernel %.0f ms, copy %.0f ms, sched %.0f ms, %d slots / %d blocks, model: fill %d swap %d "
+          "%d iterations (%d launches), kernel %.0f ms, copy %.0f ms, sched %.0f ms, %d slots / %d blocks, model: fill %d swap %d "
           "(%d differ) tail %d/%d samples %d, sticky %d, launch %s, copy lanes %d, decoded in place/ring/copy stream "
           "%d/%d/%d"
           % (cap_gb, name, tag, rep["wall_s"], rep["steps"], rep["h2d_bytes"] / 1e9, rep["h2d_raw_bytes"] / 1e9,
              rep["uploads"], rep["staged"],
-             rep["zc_touches"] * 64 / 1e9, rep["iterations"], rep["kernel_ms"], rep["h2d_ms"], rep["sched_ms"],
+             rep["zc_touches"] * 64 / 1e9, rep["iterations"], rep["kernel_launches"], rep["kernel_ms"], rep["h2d_ms"],
+             rep["sched_ms"],
              rep["nslots"], rep["nblocks"], rep["fill_model"], rep["swap_model"], rep["swap_differs"],
              rep["tail_model"], rep["tail_threshold"], rep["model_samples"], rep["sticky_swaps"],
              "%(block)d/%(waves_per_cu)d/v%(variant)d" % rep["tuned"] if rep["tuned"] else "-", rep["copy_lanes"],
@@ -350,7 +358,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         per_cap.append({"memory_gb": cap_gb, "schedulers": res})
     for entry in per_cap:
         res = entry["schedulers"]
-        for other in ("staging", "plain", "plain_zero", "sowalker"):
+        for other in ("staging", "plain", "plain_zero", "plain_raw", "sowalker"):
             if "learned" in res and other in res:
                 entry["learned_vs_%s_steps_per_s" % other] = res["learned"]["steps_per_s"] / res[other]["steps_per_s"]
                 entry["learned_vs_%s_pcie_bytes" % other] = res["learned"]["pcie_gb_total"] / res[other]["pcie_gb_total"]
@@ -359,7 +367,7 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
     lead = "learned" if "learned" in res0 else head[0]
     headrec = res0[lead]
     r0 = reps[lead][0]
-    launches = max(1, r0["iterations"])
+    launches = max(1, r0["kernel_launches"])   # iterations plus worklist-overflow rounds
     abytes = algorithmic_bytes(r0, nwalkers, False) / launches      # per launch, averaged
     achieved = abytes / (r0["kernel_ms"] / launches * 1e-3) / 1e9
     traffic, traffic_note = None, None

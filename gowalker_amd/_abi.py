@@ -184,7 +184,7 @@ class GwhReport(C.Structure):
                 ("swap_differs", C.c_uint64), ("fill_model", C.c_uint64), ("model_samples", C.c_uint64),
                 ("sticky_swaps", C.c_uint64), ("tune_candidate", C.c_uint32), ("copy_lanes", C.c_uint32),
                 ("h2d_raw_bytes", C.c_uint64), ("inplace_blocks", C.c_uint64), ("ring_blocks", C.c_uint64),
-                ("inline_blocks", C.c_uint64)]
+                ("inline_blocks", C.c_uint64), ("kernel_launches", C.c_uint64)]
 
 
 class GwhTuneEntry(C.Structure):

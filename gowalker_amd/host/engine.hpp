@@ -73,6 +73,7 @@ struct RunReport {
     uint64_t steps = 0, trials = 0, probes = 0, mix_steps = 0;
     uint64_t h2d_bytes = 0, h2d_raw_bytes = 0, uploads = 0, staged = 0, iterations = 0;
     uint64_t inplace_blocks = 0, ring_blocks = 0, inline_blocks = 0;   // packed blocks by decode path
+    uint64_t kernel_launches = 0;   // step-kernel launches: iterations plus worklist-overflow rounds
     uint64_t zero_copy_runs = 0, zero_copy_steps = 0, zc_touches = 0;
     double wall_s = 0, kernel_ms = 0, filter_ms = 0, h2d_ms = 0, sched_ms = 0, tune_ms = 0;
     uint32_t nslots = 0, nblocks = 0, copy_lanes = 1;
@@ -261,6 +262,8 @@ public:
         sched.buckets.clear();
         ck(gw_counters_get(ctx_, &c0), ctx_, "gw_counters_get");
         ck(gw_upload_stats(ctx_, &up0_[0], &up0_[1], &up0_[2]), ctx_, "gw_upload_stats");
+        uint64_t wcap = 0, rounds0 = 0;
+        ck(gw_worklist_stats(ctx_, &wcap, &rounds0), ctx_, "gw_worklist_stats");
         std::vector<uint64_t> W((uint64_t)nb * nb);
         std::vector<uint8_t> staged(nb, 0);
         std::vector<uint32_t> pf;
@@ -414,6 +417,11 @@ public:
             r.inplace_blocks = a - up0_[0];
             r.ring_blocks = b - up0_[1];
             r.inline_blocks = d - up0_[2];
+        }
+        {
+            uint64_t rounds1 = 0;
+            ck(gw_worklist_stats(ctx_, &wcap, &rounds1), ctx_, "gw_worklist_stats");
+            r.kernel_launches = r.iterations + (rounds1 - rounds0);
         }
         r.h2d_ms = c1.h2d_ms - c0.h2d_ms;
         r.wall_s = std::chrono::duration<double>(clk::now() - t0).count();

@@ -221,6 +221,7 @@ extern "C" gw_status gwh_run(const uint64_t* beg, const uint32_t* csr, uint64_t 
         report->inplace_blocks = r.inplace_blocks;
         report->ring_blocks = r.ring_blocks;
         report->inline_blocks = r.inline_blocks;
+        report->kernel_launches = r.kernel_launches;
         report->uploads = r.uploads;
         report->iterations = r.iterations;
         report->zero_copy_runs = r.zero_copy_runs;

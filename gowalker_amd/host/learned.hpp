@@ -16,7 +16,7 @@
 //   fill   (free slots left): commit staged blocks whose copies are done, else the best
 //          staged one (the run waits for it), else upload the best-ranked candidate; then
 //          stage the next-best blocks into the free slots so their copies overlap the run;
-//          rank = V(b);
+//          rank = V(b) (the fill picks the blocks the swap phase starts from);
 //   swap   (cache full): the plain rule's scores (walkers a load makes runnable, minus the
 //          pairing its victim takes away) form a shortlist -- candidates scoring at least
 //          shortlist_frac of the best, at most `shortlist` of them; the network picks the
@@ -34,7 +34,10 @@
 //          is the latest point it switches at.
 // The plain walker-count rule (engine/schedule.hpp:70-138) is its fallback: the network is
 // ignored while it has too few samples and, for a cool-down period, whenever its recent
-// predictions are off by more than a factor e on average (the error monitor).
+// predictions are off by more than a factor e on average (the error monitor).  When every
+// block fits the cache, each is loaded once and never evicted and the fill only wants steps
+// while copies are in flight, which the walker count measures: the network, its samples
+// and its training are left out of that walk (the staging ablation's decisions).
 //
 // With use_model false the same code runs with the network left out (the staging
 // ablation): fill rank = log1p(walkers a load would release), swap = the plain rule, tail
@@ -151,7 +154,8 @@ public:
         if (mlp_) gw_mlp_destroy(ctx_, mlp_);
     }
     const char* name() const override { return cfg_.use_model ? "learned" : "staging"; }
-    uint32_t run_flags() const override { return cfg_.use_model ? (uint32_t)GW_RUN_TRACE : 0u; }
+    // the block-access trace feeds the samples; none are taken when every block fits
+    uint32_t run_flags() const override { return cfg_.use_model && !fits_ ? (uint32_t)GW_RUN_TRACE : 0u; }
 
     // Stores the network and its samples for the next walk (call once the walk is done).
     void save()
@@ -177,7 +181,7 @@ public:
         flush_closed();
         // the walk is over: a longer training round on every sample so far, off the
         // per-iteration path (the next walk decides with it)
-        if (mlp_ && m_.samples >= cfg_.min_samples) {
+        if (mlp_ && !fits_ && m_.samples >= cfg_.min_samples) {
             m_.trained_at = m_.samples;
             train(cfg_.finish_steps);
         }
@@ -193,7 +197,12 @@ public:
         const uint32_t cap = capacity(in);
 
         if (buckets.size() < cap) {   // ---- fill: free slots remain
-            if (use_model_) ++fill_model_;
+            // the network decided this fill step when its tie-break picks another block
+            // than the walker-demand tie-break would
+            if (use_model_) {
+                const int a = best_candidate(cached, rank_), b = best_candidate(cached, rank0_);
+                if (a != b) ++fill_model_;
+            }
             bool took = false;
             for (uint32_t b = 0; b < nb && buckets.size() < cap; ++b)
                 if (!cached[b] && in.staged && in.staged[b] == 2) { buckets.push_back(b); cached[b] = 1; took = true; }
@@ -204,7 +213,7 @@ public:
                 if (best >= 0) { buckets.push_back((uint32_t)best); cached[best] = 1; took = true; }
             }
             if (took) return true;
-            const int c = best_candidate(cached);
+            const int c = best_candidate(cached, rank_);
             if (c < 0) {
                 if (plain_decide(in)) return true;
                 ++tail_threshold_;
@@ -290,7 +299,7 @@ public:
 
     void feedback(const SchedInput& in, const IterationOutcome& out) override
     {
-        if (!cfg_.use_model) return;
+        if (!cfg_.use_model || fits_) return;
         const uint32_t nb = in.nblocks;
         std::vector<uint64_t> trace(nb, 0);
         if (gw_trace_read(ctx_, trace.data(), nb) != GW_OK) return;
@@ -499,11 +508,16 @@ private:
             demand_[b] = d;
         }
         // features (a kernel and a readback) and a forward pass whenever the network may
-        // decide, and on every keep_every-th iteration for samples
+        // decide, and on every keep_every-th iteration for samples.  When every block fits
+        // the cache, each is loaded once and never evicted: no decision is left that V could
+        // make better than the walker count (ranking that fill by V loaded blocks whose
+        // walkers could not run yet: C3 16 GB 236-245 ms of step kernel per walk against
+        // 234 ms), so the network, its samples and its host waits are left out
         have_feats_ = false;
         use_model_ = false;
         pred_.assign(nb, 0.f);
-        if (cfg_.use_model) {
+        fits_ = nb <= capacity(in);
+        if (cfg_.use_model && !fits_) {
             // predictions are used by the fill, by network swaps, and by the tail rule once
             // the work left could be done in zero-copy reads within tail_guard uploads' time;
             // features are also taken every keep_every-th iteration for samples
@@ -526,20 +540,27 @@ private:
                 }
             }
         }
+        // fill rank: with the network (the cache cannot hold every block), V -- the fill
+        // chooses the blocks the swap phase starts from, and V is what a block serves until
+        // it is evicted; without it, the walkers a load would release now (an integer) and,
+        // as a tie-break in [0, 0.5), the walkers that need the block at all
         rank_.resize(nb);
-        // without the network: the walkers a load would release (log scale), with the
-        // walkers that need the block at all as a tie-break
-        for (uint32_t b = 0; b < nb; ++b)
-            rank_[b] = use_model_ ? pred_[b] : std::log1p((float)add_[b]) + 1e-3f * std::log1p((float)demand_[b]);
+        rank0_.resize(nb);
+        uint64_t dmax = 0;
+        for (uint32_t b = 0; b < nb; ++b) dmax = std::max(dmax, demand_[b]);
+        for (uint32_t b = 0; b < nb; ++b) {
+            rank0_[b] = (double)add_[b] + 0.5 * (double)demand_[b] / ((double)dmax + 1.0);
+            rank_[b] = use_model_ ? (double)pred_[b] : rank0_[b];
+        }
     }
 
     // the non-resident block with the best rank among those whose walkers could run now
-    int best_candidate(const std::vector<char>& cached) const
+    int best_candidate(const std::vector<char>& cached, const std::vector<double>& rank) const
     {
         int best = -1;
         for (uint32_t c = 0; c < add_.size(); ++c) {
             if (cached[c] || add_[c] == 0) continue;
-            if (best < 0 || rank_[c] > rank_[best]) best = (int)c;
+            if (best < 0 || rank[c] > rank[best]) best = (int)c;
         }
         return best;
     }
@@ -716,7 +737,8 @@ private:
     ModelState m_;
     gw_mlp* mlp_ = nullptr;
     PlainScheduler plain_;
-    std::vector<float> age_, feats_, pred_, rank_;
+    std::vector<float> age_, feats_, pred_;
+    std::vector<double> rank_, rank0_;   // fill ranks: with the network's tie-break, without
     std::vector<uint64_t> add_, demand_;
     std::vector<char> was_resident_;
     std::vector<Open> open_, keep_;
@@ -725,6 +747,7 @@ private:
     uint64_t reads_n_ = 0, sw_n_ = 0;
     uint64_t pending_hops_ = 0;
     bool have_feats_ = false, use_model_ = false;
+    bool fits_ = false;   // every block fits the cache (this walk): the network is left out
     uint32_t cooldown_ = 0;
     uint64_t fill_model_ = 0, swap_model_ = 0, sticky_ = 0, swap_differs_ = 0, tail_model_ = 0, tail_threshold_ = 0;
     uint64_t fallbacks_ = 0, err_n_ = 0;

