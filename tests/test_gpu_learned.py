@@ -97,3 +97,35 @@ def test_learned_scheduler(gpu):
     model.close()
     b.close()
     c.close()
+
+
+def test_learned_scheduler_leaves_the_network_out_when_every_block_fits(gpu):
+    """With a cache that holds every block, each block is loaded once and never evicted: the
+    learned scheduler takes no features, scores, samples or training (its decisions are the
+    staging ablation's), and walks are the plain scheduler's."""
+    beg, csr = G.rmat(14, 200_000, seed=5, permute=True)
+    n = len(beg) - 1
+    block_bytes = (len(csr) * 4) // 12 + 64
+    b, c = pinned(beg, csr)
+    types = [dict(algo=E.NODE2VEC, length=20, p=0.5, q=2.0)]
+    mem = 1 << 30
+    model = E.Model()
+    reps = []
+    for sched, m in ((E.SCHED_PLAIN, None), (E.SCHED_STAGING, None), (E.SCHED_LEARNED, model),
+                     (E.SCHED_LEARNED, model)):
+        rep, h, _ = E.run_out_of_core(b.a, c.a, types, 9, n, memory_bytes=mem, block_bytes=block_bytes, scheduler=sched,
+                                      hashes=True, model=m)
+        reps.append((rep, h))
+    (rp, hp), (rs, hs), (r1, h1), (r2, h2) = reps
+    assert rp["nslots"] == rp["nblocks"] >= 8
+    for h in (hs, h1, h2):
+        np.testing.assert_array_equal(h, hp)
+    info = model.info()
+    print("learned", {k: r2[k] for k in KEYS}, "model", info, flush=True)
+    for r in (r1, r2):
+        assert r["learned_decisions"] == 0 and r["model_samples"] == 0
+        assert r["h2d_bytes"] == rs["h2d_bytes"] and r["uploads"] + r["staged"] == rs["uploads"] + rs["staged"]
+    assert info["samples"] == 0 and info["adam_steps"] == 0
+    model.close()
+    b.close()
+    c.close()
