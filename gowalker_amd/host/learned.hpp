@@ -40,7 +40,8 @@
 // and its training are left out of that walk (the staging ablation's decisions).
 //
 // With use_model false the same code runs with the network left out (the staging
-// ablation): fill rank = log1p(walkers a load would release), swap = the plain rule, tail
+// ablation): fill rank = the walkers a load would release (ties: the walkers that need the
+// block at all), swap = the plain rule, tail
 // = the reference's threshold only.  Walks stay bit-identical either way (counter-based
 // RNG): only bytes moved and time change.
 #pragma once
@@ -197,8 +198,8 @@ public:
         const uint32_t cap = capacity(in);
 
         if (buckets.size() < cap) {   // ---- fill: free slots remain
-            // the network decided this fill step when its tie-break picks another block
-            // than the walker-demand tie-break would
+            // the network decided this fill step when its rank (V) picks another block than
+            // the walker rank would
             if (use_model_) {
                 const int a = best_candidate(cached, rank_), b = best_candidate(cached, rank0_);
                 if (a != b) ++fill_model_;
@@ -738,7 +739,7 @@ private:
     gw_mlp* mlp_ = nullptr;
     PlainScheduler plain_;
     std::vector<float> age_, feats_, pred_;
-    std::vector<double> rank_, rank0_;   // fill ranks: with the network's tie-break, without
+    std::vector<double> rank_, rank0_;   // fill ranks: the one in use (V with the network), the walker rank
     std::vector<uint64_t> add_, demand_;
     std::vector<char> was_resident_;
     std::vector<Open> open_, keep_;
