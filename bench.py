@@ -403,11 +403,13 @@ def run_ooc(args, cfg, ctx, beg, csr, nverts, narcs, gen_s, t_start):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_over_algorithmic": traffic / abytes if traffic else None,
-                     "kernel": "%s<false, %s, %s, %s>" % ("walk_lane_kernel" if (r0.get("tuned") or {}).get("variant") == 2
-                                                           else "walk_kernel",
-                                                           search_mode(r0.get("tuned")),
-                                                           "true" if r0["nblocks"] <= LDS_BLOCKS else "false",
-                                                           "true" if len(cfg["types"]) == 2 else "false"),
+                     # the lane kernel's last argument: walkers whose previous block was evicted
+                     # (runs after the first eviction of a walk; none where every block fits)
+                     "kernel": ("walk_lane_kernel<false, %s, %s, %s, " + ("true>" if r0["nslots"] < r0["nblocks"]
+                                                                            else "false>")
+                                if (r0.get("tuned") or {}).get("variant") == 2 else "walk_kernel<false, %s, %s, %s>")
+                               % (search_mode(r0.get("tuned")), "true" if r0["nblocks"] <= LDS_BLOCKS else "false",
+                                  "true" if len(cfg["types"]) == 2 else "false"),
                      "kernel_ms": r0["kernel_ms"] / launches, "algorithmic_bytes": abytes,
                      "sector_bytes": sector_bytes(r0, nwalkers, False) / launches, "launches": launches,
                      "per_step": {"steps_per_launch": r0["steps"] / launches,
@@ -575,9 +577,9 @@ def main():
                                   "traffic_bytes": traffic / st0["steps"] if traffic else None},
                      "traffic_source": traffic_note,
                      "kernel": "walk_flat_kernel<true>" if args.variant == 1 else
-                               "%s<true, %s, false, %s>" % ("walk_lane_kernel" if args.variant == 2 else "walk_kernel",
-                                                            search_mode(tuned or launch),
-                                                            "true" if len(cfg["types"]) == 2 else "false"),
+                               ("walk_lane_kernel<true, %s, false, %s, false>" if args.variant == 2
+                                else "walk_kernel<true, %s, false, %s>")
+                               % (search_mode(tuned or launch), "true" if len(cfg["types"]) == 2 else "false"),
                      "kernel_ms": avg_kms, "algorithmic_bytes": abytes,
                      "sector_bytes": sector_bytes(st0, nwalkers, visits)},
         "cpu_baseline": None,

@@ -154,6 +154,9 @@ struct gw_ctx {
     uint32_t visit_parts = 1;              // visit-count arrays (summed by gw_read_visits)
     uint32_t* d_work = nullptr;            // worklist of runnable walkers: work_cap entries
     uint64_t work_cap = 0, work_cap_req = 0, work_rounds = 0;
+    // blocks resident in some run since gw_init_walkers: a walker's previous block was resident
+    // when it stepped from it, so only a block of this set that is gone can be one that is not
+    std::vector<uint8_t> ever_res;
     uint32_t* d_bucket = nullptr;
     uint32_t* d_bpart = nullptr;           // per-workgroup rows of the bucket pass (one per CU)
     unsigned long long* d_trace = nullptr; // per-block steps of the last traced run
@@ -1499,6 +1502,7 @@ gw_status gw_init_walkers(gw_ctx* c, uint64_t n, uint32_t wpv, const uint32_t* s
     const uint64_t n1 = c->wp.ntypes == 2 ? n / 2 : 0, n0 = n - n1;
     const uint64_t l0 = c->dt[0].length, l1 = c->wp.ntypes == 2 ? c->dt[1].length : 0;
     c->unfinished = (l0 ? n0 : 0) + (l1 ? n1 : 0);
+    c->ever_res.assign(c->nblocks, 0);
     c->pending_hops = n0 * l0 + n1 * l1;
     c->w_valid = false;
     return GW_OK;
@@ -1718,6 +1722,19 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
         for (uint32_t b = 0; b < c->nblocks; ++b)
             if (c->hblk[b].nedges >> 32) l.variant = 0;
     }
+    // the lane kernel steps walkers whose previous block is not resident until a step needs
+    // that block's list (it stops there and waits for it); the other variants read both rows
+    // at pickup, so their worklist keeps walkers with both blocks resident.  Such walkers
+    // exist only once a block the walk has had resident is gone (walkers start with prev =
+    // cur; a step leaves prev in a resident block): before that, the lane kernel instance
+    // without those checks runs (C3 16 GB: its step kernel is ~4% faster)
+    bool gone = false;
+    if (c->ever_res.size() != c->nblocks) c->ever_res.assign(c->nblocks, 0);
+    for (uint32_t b = 0; b < c->nblocks; ++b) {
+        if (c->hblk[b].resident != kNotResident) c->ever_res[b] = 1;
+        else if (c->ever_res[b]) gone = true;
+    }
+    f.relax_prev = a.relax = (l.variant == 2 && gone) ? 1u : 0u;
     // persistent grid: as many workgroups as the CUs hold at once (occupancy), or the
     // caller's waves per CU
     const bool one_blk = c->nblocks == 1;

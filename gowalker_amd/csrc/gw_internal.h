@@ -18,6 +18,9 @@ constexpr uint32_t kHopMask = 0xFFFFFu;
 constexpr uint32_t kStatusShift = 20, kTypeShift = 24;
 constexpr uint32_t kMaxLength = kHopMask;
 enum : uint32_t { kActive = 0, kDoneLength = 1, kDoneTerm = 2, kDoneDeadEnd = 3, kDoneTrialCap = 4 };
+// meta bit: the walker's step stopped where it needed its previous vertex's list, whose block
+// was not resident (the lane kernel picks such walkers up again once that block is)
+constexpr uint32_t kWaitPrev = 1u << 25;
 // DevBlock::resident: 0 not readable, 1 in an HBM slot, 2 host-mapped (zero-copy reads)
 enum : uint32_t { kNotResident = 0, kInSlot = 1, kMapped = 2 };
 
@@ -75,6 +78,9 @@ struct WalkArgs {
     uint32_t zc_count;        // some block is host-mapped: count zero-copy reads (counters[7])
     uint32_t search_end;      // 0: membership by bisection; else interpolation, ids below it
     uint32_t search_win;      // probes read aligned 16-entry windows
+    uint32_t relax;           // lane kernel: the worklist holds walkers whose previous block is not
+                              // resident (FilterArgs::relax_prev): the instance that stops their
+                              // steps where they need it
     uint32_t* queue;          // work-fetch counter, zero at launch
     uint32_t chunk;           // walkers a wavefront takes per queue atomic (multiple of 64)
     unsigned long long* counters;  // steps, trials, probes, mix, ended, left
@@ -112,6 +118,10 @@ struct FilterArgs {
     uint32_t len0, len1;      // hop caps of walker types 0 / 1
     uint32_t* bucket_part;    // optional: part_rows x nblocks^2 per-workgroup rows of the bucket matrix
     uint32_t part_rows;
+    // 1: a walker whose current block is resident is runnable also when its previous one is
+    // not, unless it stopped for it before (kWaitPrev); for the lane kernel, which stops a
+    // step where the previous list is needed and lets the others run
+    uint32_t relax_prev;
 };
 
 // records a message for gw_last_error (gw_api.cpp)

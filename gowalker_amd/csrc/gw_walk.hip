@@ -1038,7 +1038,8 @@ enum : uint32_t {
     LS_GUESS, LS_MOVE, LS_STEP, LS_TRIAL, LS_END
 };
 
-constexpr uint32_t kLaneBisect = 1u, kLaneMv = 2u, kLaneMu = 4u;   // Lane::fl bits
+// Lane::fl bits; kLaneNoPrev: the previous vertex's block is not resident (its list unread)
+constexpr uint32_t kLaneBisect = 1u, kLaneMv = 2u, kLaneMu = 4u, kLaneNoPrev = 8u;
 
 struct Lane {
     uint32_t ph, w, cur, prev;
@@ -1097,7 +1098,9 @@ __device__ __forceinline__ bool lane_search(Lane& L, uint32_t n, uint32_t id_end
     return true;
 }
 
-template <bool ONE, uint32_t SM, bool LB, bool MIX>
+// RX: the worklist may hold walkers whose previous block is not resident (WalkArgs::relax):
+// their steps stop where they need the previous vertex's list
+template <bool ONE, uint32_t SM, bool LB, bool MIX, bool RX>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM & 2) ? 4 : 8))) walk_lane_kernel(WalkArgs a)
 {
     constexpr uint32_t kTrace = ONE ? 1u : (LB ? kLdsBlocks : kTraceLdsBlocks);
@@ -1183,8 +1186,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             const uint64_t* pp = (const uint64_t*)L.nu;
             r0 = pc[0];
             r1 = pc[1];
-            r2 = pp[0];
-            r3 = pp[1];
+            if (!RX || !(L.fl & kLaneNoPrev)) {
+                r2 = pp[0];
+                r3 = pp[1];
+            }
         } else if (L.ph == LS_ROW) {
             const uint64_t* pc = (const uint64_t*)L.addr;
             r0 = pc[0];
@@ -1219,13 +1224,22 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             const DevBlock& P = ONE ? one : blocks[L.bcp >> 16];
             L.addr = B.beg + (L.cur - B.start_vert);
             L.nu = (const uint32_t*)(P.beg + (L.prev - P.start_vert));
-            L.fl = (B.resident == kMapped ? kLaneMv : 0u) | (P.resident == kMapped ? kLaneMu : 0u);
+            // a walker picked up with its previous block not resident (the filter's relaxed
+            // pickup) reads only its current row; its step stops where it needs the other
+            L.fl = (B.resident == kMapped ? kLaneMv : 0u) | (P.resident == kMapped ? kLaneMu : 0u) |
+                   (RX && !P.resident ? kLaneNoPrev : 0u);
+            if (RX) L.meta &= ~kWaitPrev;
             L.ph = LS_ROWS;
             break;
         }
         case LS_ROWS:
             lane_row(ONE ? one : blocks[L.bcp & 0xFFFFu], r0, r1, L.nv, L.dv);
-            lane_row(ONE ? one : blocks[L.bcp >> 16], r2, r3, L.nu, L.du);
+            if (RX && (L.fl & kLaneNoPrev)) {
+                L.nu = nullptr;
+                L.du = 0;
+            } else {
+                lane_row(ONE ? one : blocks[L.bcp >> 16], r2, r3, L.nu, L.du);
+            }
             L.ph = LS_STEP;
             break;
         case LS_ROW:
@@ -1247,6 +1261,13 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
                 L.ph = LS_MOVE;
             } else if (L.ra > tp.m_hi) {
                 L.ph = LS_TRIAL;
+            } else if (RX && (L.fl & kLaneNoPrev)) {
+                // the class needs prev's list, which is not resident: the step stops here,
+                // its trials uncounted, and runs again from its first trial (same draws) once
+                // prev's block is resident
+                c.trials -= L.k + 1;
+                L.meta |= kWaitPrev;
+                L.ph = LS_END;
             } else if (lane_search<SM>(L, L.du, a.search_end)) {
                 ++c.probes;
                 if (a.zc_count && (L.fl & kLaneMu)) ++c.zc;
@@ -1354,7 +1375,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
             if (a.paths) a.paths[(uint64_t)L.w * a.path_stride + hop] = L.x;
             L.nu = L.nv;
             L.du = L.dv;
-            L.fl = (L.fl & ~kLaneMu) | ((L.fl & kLaneMv) ? kLaneMu : 0u);
+            L.fl = (L.fl & ~(kLaneMu | (RX ? kLaneNoPrev : 0u))) | ((L.fl & kLaneMv) ? kLaneMu : 0u);
             if (hop >= tp.length) {
                 L.meta |= kDoneLength << kStatusShift;
                 L.bcp = bc | (bc << 16);
@@ -1439,6 +1460,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((SM &
                         L.addr = L.nv + (((uint64_t)d.y * L.dv) >> 32);
                         if (a.zc_count && (L.fl & kLaneMv)) ++c.zc;
                         L.ph = LS_PICK;
+                    } else if (RX && (L.fl & kLaneNoPrev)) {
+                        // the proposal draws from prev's list, which is not resident: the
+                        // step stops here (as for a Node2vec class test above)
+                        c.trials -= L.k + 1;
+                        L.meta |= kWaitPrev;
+                        L.ph = LS_END;
                     } else {
                         L.addr = L.nu + (((uint64_t)d.y * L.du) >> 32);
                         if (a.zc_count && (L.fl & kLaneMu)) ++c.zc;
@@ -1574,9 +1601,11 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
                 const uint32_t bp = vp[k] == vc[k] ? bc
                                     : (lds_cuts ? find_block_lut(slut, lut_shift, svcut, a.nblocks, vp[k])
                                                 : find_block(a.vcut, a.nblocks, vp[k]));
-                if (a.worklist)
-                    take = lds_res ? ((res_bits[bc >> 5] >> (bc & 31)) & (res_bits[bp >> 5] >> (bp & 31)) & 1u) != 0
-                                   : a.blocks[bc].resident && a.blocks[bp].resident;
+                if (a.worklist) {
+                    const bool rc = lds_res ? ((res_bits[bc >> 5] >> (bc & 31)) & 1u) != 0 : a.blocks[bc].resident != 0;
+                    const bool rp = lds_res ? ((res_bits[bp >> 5] >> (bp & 31)) & 1u) != 0 : a.blocks[bp].resident != 0;
+                    take = rc && (rp || (a.relax_prev && !(m[k] & kWaitPrev)));
+                }
                 if (lds_hist) atomicAdd(hist + bp * a.nblocks + bc, 1u);
                 else if (a.bucket) atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
             }
@@ -1741,8 +1770,17 @@ hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s)
 template <bool ONE, uint32_t SM, bool LB, bool MIX>
 static void launch_shape(const WalkArgs& a, uint32_t variant, uint32_t grid, uint32_t block, hipStream_t s)
 {
-    if (variant == 2) hipLaunchKernelGGL((walk_lane_kernel<ONE, SM, LB, MIX>), dim3(grid), dim3(block), 0, s, a);
-    else hipLaunchKernelGGL((walk_kernel<ONE, SM, LB, MIX>), dim3(grid), dim3(block), 0, s, a);
+    if (variant == 2) {
+        if constexpr (!ONE) {
+            if (a.relax) {
+                hipLaunchKernelGGL((walk_lane_kernel<ONE, SM, LB, MIX, true>), dim3(grid), dim3(block), 0, s, a);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((walk_lane_kernel<ONE, SM, LB, MIX, false>), dim3(grid), dim3(block), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((walk_kernel<ONE, SM, LB, MIX>), dim3(grid), dim3(block), 0, s, a);
+    }
 }
 
 template <bool ONE, uint32_t SM, bool LB>
@@ -1788,10 +1826,10 @@ uint32_t walk_blocks_per_cu(bool one_block, uint32_t nblocks, uint32_t variant, 
         e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<true>, (int)block, 0)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_flat_kernel<false>, (int)block, 0);
     else if (variant == 2)
-        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<true, false, false, true>, (int)block, 0)
+        e = one_block ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<true, false, false, true, false>, (int)block, 0)
             : nblocks <= kLdsBlocks
-                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, true, true>, (int)block, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, false, true>, (int)block, 0);
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, true, true, false>, (int)block, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_lane_kernel<false, false, false, true, false>, (int)block, 0);
     else if (one_block)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, walk_kernel<true, false, false, true>, (int)block, 0);
     else if (nblocks <= kLdsBlocks)

@@ -241,7 +241,11 @@ gw_status gw_init_walkers(gw_ctx* ctx, uint64_t nwalkers, uint32_t walkers_per_v
 
 /* Runs every unfinished walker whose (prev, cur) blocks are both resident until it
  * finishes or steps into a non-resident block: replaces the engine.hpp:418-448 loop
- * with gpu_run/SOPR_gpu_run, prefix, insertglobal and walkswap (gpu/gpuwalk.hpp:193-372). */
+ * with gpu_run/SOPR_gpu_run, prefix, insertglobal and walkswap (gpu/gpuwalk.hpp:193-372).
+ * With the lane kernel (variant 2), once a block resident in an earlier run since
+ * gw_init_walkers is no longer resident, walkers whose current block is resident and
+ * previous one is not run too, until a step needs the previous vertex's list: that step
+ * stops unchanged, and the walker waits for the block (walks are the same either way). */
 gw_status gw_run(gw_ctx* ctx, const gw_launch* launch, gw_run_stats* stats);
 
 /* Launch-shape autotuner (no reference counterpart; the reference hard-codes 512 x 512,

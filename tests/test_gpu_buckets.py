@@ -142,3 +142,68 @@ def test_short_worklist_takes_rounds_with_the_same_walks(gpu, variant, window):
                 np.testing.assert_array_equal(x, y)
     for h in finals[1:]:
         np.testing.assert_array_equal(finals[0], h)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_walkers_whose_previous_block_was_evicted(gpu, mixed):
+    """After an eviction, walkers whose current block is resident and previous one is not are
+    run by the lane kernel (variant 2) until a step needs the previous vertex's list (a
+    Node2vec class test inside the membership band, an SOSR proposal from it); such a step
+    stops, unchanged, and the walker is not picked up again before that block is resident.
+    Variant 0 reads both rows at pickup and runs only walkers with both blocks resident.  With
+    the same resident sets in turn (half the blocks, then the other half twice, then all),
+    the lane kernel steps more in the second run, neither kernel finds work in the third, the
+    bucket matrix is the walker-state histogram after every run, and the finished walks,
+    steps and trials are the oracle's for both kernels."""
+    import _oracle as O
+
+    beg, csr = G.rmat(13, 40000, seed=8, undirected=True, permute=True)
+    n = len(beg) - 1
+    bl = Blocks(beg, csr, 16384)
+    nb = bl.nblocks
+    assert nb >= 6
+    t0 = dict(algo=NODE2VEC, length=20, p=0.5, q=2.0)
+    if mixed:
+        types = [t0, dict(algo=SOPR, length=20, alpha=0.2, term=0.15)]
+        cfg = O.walk_cfg(5, O.type_cfg(O.N2V, 20, p=0.5, q=2.0), O.type_cfg(O.SOPR, 20, alpha=0.2, term=0.15))
+        wpv, ntypes = 2, 2
+    else:
+        types = [t0]
+        cfg = O.walk_cfg(5, O.type_cfg(O.N2V, 20, p=0.5, q=2.0))
+        wpv, ntypes = 1, 1
+    nw = n * wpv
+    o = O.run(beg, csr, cfg, nw, wpv=wpv, ntypes=ntypes, state=False)
+    half = nb // 2
+    A, B = list(range(half)), list(range(half, nb))
+    out = {}
+    for variant, window in ((0, False), (2, True)):
+        with Context() as ctx:
+            ctx.set_blocks(bl)
+            ctx.set_walk(types, 5)
+            ctx.init_walkers(nw, wpv, outputs=OUT_HASH)
+            ctx.reserve_slots(max(len(A), len(B)))
+            runs = []
+            for it, resident in enumerate((A, B, B)):
+                if it < 2:
+                    for s in range(max(len(A), len(B))):
+                        ctx.evict(s)
+                    for s, b in enumerate(resident):
+                        ctx.upload(s, b)
+                    ctx.wait_uploads()
+                runs.append(ctx.run(buckets=True, variant=variant, window=window))
+                check(ctx, bl, "variant %d run %d" % (variant, it))
+            ctx.reserve_slots(nb)
+            for b in range(nb):
+                ctx.upload(b, b)
+            ctx.wait_uploads()
+            runs.append(ctx.run(buckets=True, variant=variant, window=window))
+            assert runs[-1]["walkers_left"] == 0
+            print("variant", variant, [(r["steps"], r["trials"], r["walkers_run"]) for r in runs], flush=True)
+            out[variant] = (runs, ctx.hashes())
+    for variant, (runs, h) in out.items():
+        np.testing.assert_array_equal(h, o["hashes"], err_msg="variant %d" % variant)
+        assert sum(r["steps"] for r in runs) == o["totals"]["steps"]
+        assert sum(r["trials"] for r in runs) == o["totals"]["trials"]
+        assert runs[2]["steps"] == 0 and runs[2]["walkers_run"] == 0   # nothing new is runnable
+    # the second run: the lane kernel also steps walkers that came from the evicted half
+    assert out[2][0][1]["steps"] > out[0][0][1]["steps"]
