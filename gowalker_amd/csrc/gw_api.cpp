@@ -163,7 +163,9 @@ struct gw_ctx {
     float* d_feat = nullptr;               // nblocks x kNumFeatures
     float* d_age = nullptr;
     uint32_t* d_misc = nullptr;            // [0] nwork, [1] queue
-    unsigned long long* d_cnt = nullptr;   // [0..5] run counters, [6] left, [7] zero-copy reads, [8] hops
+    // [0..5] run counters, [6] left, [7] zero-copy reads, [8] hops, [10..13] walkers and hops
+    // the run's list took out of the bucket matrix and put back
+    unsigned long long* d_cnt = nullptr;
     unsigned long long* h_cnt = nullptr;   // pinned readback
     unsigned long long* m_cnt = nullptr;   // its device-side (mapped) address
     uint32_t* m_bucket = nullptr;          // mapped address of h_bucket
@@ -650,10 +652,10 @@ gw_status gw_set_block_table(gw_ctx* c, const gw_block_desc* b, uint32_t nb)
     c->d_vcut = (uint32_t*)p;
     if ((st = dalloc(c, &p, sizeof(uint32_t) * (uint64_t)nb * nb, false)) != GW_OK) return st;
     c->d_bucket = (uint32_t*)p;
-    // the bucket pass's per-workgroup rows: table metadata like the matrix itself, outside
-    // the budget (at most 48 KB per CU: 12 MB on 256 CUs)
+    // the bucket passes' per-workgroup rows (two per CU for the run's worklist filter): table
+    // metadata like the matrix itself, outside the budget (at most 96 KB per CU: 24 MB on 256 CUs)
     if ((uint64_t)nb * nb <= kFilterLdsBuckets) {
-        if ((st = dalloc(c, &p, sizeof(uint32_t) * (uint64_t)nb * nb * c->ncu, false)) != GW_OK) return st;
+        if ((st = dalloc(c, &p, sizeof(uint32_t) * (uint64_t)nb * nb * c->ncu * 2, false)) != GW_OK) return st;
         c->d_bpart = (uint32_t*)p;
     }
     if ((st = dalloc(c, &p, sizeof(unsigned long long) * nb, false)) != GW_OK) return st;
@@ -1655,6 +1657,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
 
     gw_status st;
     if ((st = push_block_table(c, false)) != GW_OK) return st;
+    const bool w_was_valid = c->w_valid;
     c->w_valid = false;
     // The whole walker array is the work list: the kernel skips finished walkers and,
     // unless every block is readable, walkers whose (prev, cur) blocks are not both
@@ -1752,6 +1755,36 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     // the trace comes back with the counters, so gw_trace_read needs no device round trip
     const bool traced = a.block_steps != nullptr;
     const bool buckets = (l.flags & GW_RUN_BUCKETS) != 0;
+    // With a worklist and the matrix current when the run starts, the matrix is kept current
+    // instead of recounted over every walker: the filter takes the walkers it lists out of
+    // it, and a pass over that list after the step kernel puts the unfinished ones back
+    // with their new (prev, cur) blocks (C5, 268M walkers: the recount took ~1 ms a run,
+    // on the compute stream between the step kernel and the next schedule)
+    const bool delta = buckets && w_was_valid && !all_res && !cap;
+    FilterArgs g;
+    memset(&g, 0, sizeof g);
+    if (delta) {
+        f.bucket_sub = c->d_bucket;
+        f.taken = c->d_cnt + 10;
+        f.bucket_part = c->d_bpart;
+        f.part_rows = c->d_bpart ? 2 * c->ncu : 0;
+        g.blocks = c->d_blocks;
+        g.nblocks = c->nblocks;
+        g.nwalkers = nwork;   // the step kernel's bound on the list
+        g.meta = c->d_meta;
+        g.cur = c->d_cur;
+        g.prev = c->d_prev;
+        g.vcut = c->d_vcut;
+        g.bucket = c->d_bucket;
+        g.left = c->d_cnt + 12;
+        g.hops = c->d_cnt + 13;
+        g.len0 = c->dt[0].length;
+        g.len1 = c->dt[1].length;
+        g.bucket_part = c->d_bpart;
+        g.part_rows = c->d_bpart ? c->ncu : 0;
+        g.gather = c->d_work;
+        g.gather_n = c->d_misc;
+    }
     c->trace_valid = false;
     // the run's device work: the block table push, counter clears, filter, step kernel,
     // readbacks and the bucket pass -- launched one by one, or as a graph captured the
@@ -1763,6 +1796,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
                                   sizeof(DevBlock) / 4 * c->nblocks, c->s_compute));
         HIPCHK(c, launch_fill_u32((uint32_t*)c->d_cnt, 0, 16, c->s_compute));
         HIPCHK(c, launch_fill_u32(c->d_misc, 0, 4, c->s_compute));
+        if (delta) HIPCHK(c, launch_fill_u32((uint32_t*)(c->d_cnt + 10), 0, 8, c->s_compute));
         if (!all_res) HIPCHK(c, launch_filter(f, c->ncu * 2, c->s_compute));
         if (traced && round == 0) HIPCHK(c, launch_fill_u32((uint32_t*)c->d_trace, 0, 2ull * c->nblocks, c->s_compute));
         HIPCHK(c, graph ? launch_timestamp(c->m_ts, c->s_compute) : hipEventRecord(c->ev_k0, c->s_compute));
@@ -1774,6 +1808,14 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
         if (traced)
             HIPCHK(c, launch_copy_u32((uint32_t*)c->m_trace, (const uint32_t*)c->d_trace, 2ull * c->nblocks, c->s_compute));
         // bucket matrix of the walkers left, read back with the counters (one host wait)
+        if (delta) {
+            HIPCHK(c, graph ? launch_timestamp(c->m_ts + 2, c->s_compute) : hipEventRecord(c->ev_f0, c->s_compute));
+            HIPCHK(c, launch_filter(g, c->ncu, c->s_compute));
+            HIPCHK(c, graph ? launch_timestamp(c->m_ts + 3, c->s_compute) : hipEventRecord(c->ev_f1, c->s_compute));
+            HIPCHK(c, launch_copy_u32(c->m_bucket, c->d_bucket, (uint64_t)c->nblocks * c->nblocks, c->s_compute));
+            HIPCHK(c, launch_copy_u32((uint32_t*)(c->m_cnt + 11), (const uint32_t*)(c->d_cnt + 10), 8, c->s_compute));
+            return GW_OK;
+        }
         if (buckets) return enqueue_bucket_pass(c, graph);
         return GW_OK;
     };
@@ -1788,7 +1830,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
             sig.block = l.block;
             sig.variant = l.variant;
             sig.flags = (all_res ? 1u : 0u) | (traced ? 2u : 0u) | (buckets ? 4u : 0u) | (one_blk ? 8u : 0u) |
-                        (round ? 16u : 0u);
+                        (round ? 16u : 0u) | (delta ? 32u : 0u);
             sig.ncu = c->ncu;
             sig.d_bucket = c->d_bucket;
             sig.m_cnt = c->m_cnt;
@@ -1834,6 +1876,10 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
         s.filter_ms += fms;
         s.zc_touches += c->h_cnt[7];
         if (!buckets) c->unfinished -= std::min<uint64_t>(c->unfinished, c->h_cnt[4]);
+        if (delta) {   // listed walkers out ([11] count, [12] hops), unfinished ones back ([13], [14])
+            c->unfinished = c->unfinished - c->h_cnt[11] + c->h_cnt[13];
+            c->pending_hops = c->pending_hops - c->h_cnt[12] + c->h_cnt[14];
+        }
         // every walker of the list has run until it finished or left the resident blocks,
         // so the runnable count falls by the list's length each round
         if (all_res || (uint32_t)c->h_cnt[10] <= c->work_cap) break;
@@ -1842,7 +1888,7 @@ static gw_status run_impl(gw_ctx* c, gw_launch l, gw_run_stats* stats, uint64_t 
     c->trace_valid = traced;
     if (buckets) c->w_valid = true;
     for (uint32_t sl = 0; sl < c->nslots; ++sl) collect_copy_time(c, sl, false);
-    if (buckets) {
+    if (buckets && !delta) {
         c->unfinished = c->h_cnt[8];
         c->pending_hops = c->h_cnt[9];
     }

@@ -122,6 +122,15 @@ struct FilterArgs {
     // not, unless it stopped for it before (kWaitPrev); for the lane kernel, which stops a
     // step where the previous list is needed and lets the others run
     uint32_t relax_prev;
+    // optional: the bucket matrix as it stands before a run; walkers the filter writes into
+    // the worklist leave it (the pass over that worklist after the step kernel puts the
+    // unfinished ones back), and taken[0] / taken[1] count them and their remaining hops
+    uint32_t* bucket_sub;
+    unsigned long long* taken;
+    // optional: visit the walker ids gather[0 .. min(*gather_n, nwalkers)) instead of
+    // 0 .. nwalkers-1 (a worklist); bucket counts then add to the matrix as it stands
+    const uint32_t* gather;
+    const uint32_t* gather_n;
 };
 
 // records a message for gw_last_error (gw_api.cpp)

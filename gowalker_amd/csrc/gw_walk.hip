@@ -1539,6 +1539,14 @@ constexpr uint32_t kFilterLdsCuts = 2048;   // cut points staged in LDS (with a 
 // histogram leaves the workgroup as one row of plain stores into bucket_part, summed by
 // bucket_reduce_kernel: a device-scope atomic per bin and workgroup (1.5M of them on a
 // 54-block table) made the pass 2x slower and held up host-to-device copies meanwhile.
+// Two forms keep the matrix current across a run without a pass over all walkers: the
+// worklist filter with bucket_sub counts the walkers it writes into the worklist (their
+// pairs leave the matrix: bucket_reduce_kernel subtracts the rows), and the pass with
+// gather visits only that worklist after the step kernel (the pairs of the unfinished ones
+// come back: the rows are added).  GATHER and SUB are those forms (a.gather, a.bucket_sub),
+// as template arguments so that the plain pass keeps its registers (80 VGPRs; 103 with the
+// pairs that SUB holds until the worklist is written, 116 with every form in one kernel).
+template <bool GATHER, bool SUB>
 __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
 {
     extern __shared__ uint32_t hist[];
@@ -1548,8 +1556,10 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
     __shared__ uint32_t svcut[kFilterLdsCuts + 1];
     __shared__ uint16_t slut[kLutSize];
     const uint32_t nbb = a.nblocks * a.nblocks;
-    const bool lds_hist = a.bucket && nbb <= kFilterLdsBuckets;
+    const bool lds_hist = (a.bucket || SUB) && nbb <= kFilterLdsBuckets;
     const bool lds_res = a.worklist && a.nblocks <= kFilterLdsResident;
+    uint64_t nw = a.nwalkers;
+    if (GATHER && *a.gather_n < nw) nw = *a.gather_n;
     const bool lds_cuts = a.nblocks <= kFilterLdsCuts;
     uint32_t lut_shift = 0;
     if (lds_cuts) {
@@ -1571,22 +1581,27 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
     __syncthreads();
     const uint32_t lane = __lane_id(), wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
     const uint64_t tile = (uint64_t)blockDim.x * kFilterItems;
-    uint64_t left = 0, hops = 0;
-    for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < a.nwalkers; t0 += (uint64_t)gridDim.x * tile) {
-        uint32_t m[kFilterItems], vc[kFilterItems], vp[kFilterItems];
+    uint64_t left = 0, hops = 0, tk = 0, tk_hops = 0;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < nw; t0 += (uint64_t)gridDim.x * tile) {
+        uint32_t m[kFilterItems], vc[kFilterItems], vp[kFilterItems], id[kFilterItems];
 #pragma unroll
         for (uint32_t k = 0; k < kFilterItems; ++k) {
-            const uint64_t w = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
-            m[k] = w < a.nwalkers ? a.meta[w] : (kDoneLength << kStatusShift);
+            const uint64_t j = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
+            id[k] = j < nw ? (GATHER ? a.gather[j] : (uint32_t)j) : 0u;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kFilterItems; ++k) {
-            const uint64_t w = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
+            const uint64_t j = t0 + (uint64_t)k * blockDim.x + threadIdx.x;
+            m[k] = j < nw ? a.meta[id[k]] : (kDoneLength << kStatusShift);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kFilterItems; ++k) {
             const bool act = ((m[k] >> kStatusShift) & 0xFu) == kActive;
-            vc[k] = act ? a.cur[w] : 0u;
-            vp[k] = act ? a.prev[w] : 0u;
+            vc[k] = act ? a.cur[id[k]] : 0u;
+            vp[k] = act ? a.prev[id[k]] : 0u;
         }
         uint64_t masks[kFilterItems];
+        uint32_t pair[kFilterItems];
         uint32_t cnt = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kFilterItems; ++k) {
@@ -1606,8 +1621,11 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
                     const bool rp = lds_res ? ((res_bits[bp >> 5] >> (bp & 31)) & 1u) != 0 : a.blocks[bp].resident != 0;
                     take = rc && (rp || (a.relax_prev && !(m[k] & kWaitPrev)));
                 }
-                if (lds_hist) atomicAdd(hist + bp * a.nblocks + bc, 1u);
-                else if (a.bucket) atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
+                if (SUB) pair[k] = bp * a.nblocks + bc;
+                if (a.bucket) {
+                    if (lds_hist) atomicAdd(hist + bp * a.nblocks + bc, 1u);
+                    else atomicAdd(a.bucket + (uint64_t)bp * a.nblocks + bc, 1u);
+                }
             }
             masks[k] = __ballot(take);
             cnt += (uint32_t)__popcll(masks[k]);
@@ -1626,7 +1644,17 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
             for (uint32_t k = 0; k < kFilterItems; ++k) {
                 if ((masks[k] >> lane) & 1ull) {
                     const uint32_t at = off + lane_rank(masks[k]);
-                    if (at < a.work_cap) a.worklist[at] = (uint32_t)(t0 + (uint64_t)k * blockDim.x + threadIdx.x);
+                    if (at < a.work_cap) {   // (no gather with a worklist: the id is the index)
+                        a.worklist[at] = (uint32_t)(t0 + (uint64_t)k * blockDim.x + threadIdx.x);
+                        if (SUB) {
+                            if (lds_hist) atomicAdd(hist + pair[k], 1u);
+                            else atomicSub(a.bucket_sub + pair[k], 1u);
+                            const uint32_t len = ((m[k] >> kTypeShift) & 1u) ? a.len1 : a.len0;
+                            const uint32_t hop = m[k] & kHopMask;
+                            ++tk;
+                            tk_hops += len > hop ? len - hop : 0u;
+                        }
+                    }
                 }
                 off += (uint32_t)__popcll(masks[k]);
             }
@@ -1639,11 +1667,22 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
         hops = wave_sum(hops);
         if (lane == 0 && hops) atomicAdd(a.hops, (unsigned long long)hops);
     }
+    if (SUB) {
+        tk = wave_sum(tk);
+        tk_hops = wave_sum(tk_hops);
+        if (lane == 0 && tk) {
+            atomicAdd(a.taken, (unsigned long long)tk);
+            atomicAdd(a.taken + 1, (unsigned long long)tk_hops);
+        }
+    }
     if (lds_hist) {
         __syncthreads();
         if (a.bucket_part) {
             uint32_t* row = a.bucket_part + (uint64_t)blockIdx.x * nbb;
             for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x) row[i] = hist[i];
+        } else if (SUB) {
+            for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x)
+                if (hist[i]) atomicSub(a.bucket_sub + i, hist[i]);
         } else {
             for (uint32_t i = threadIdx.x; i < nbb; i += blockDim.x)
                 if (hist[i]) atomicAdd(a.bucket + i, hist[i]);
@@ -1652,9 +1691,9 @@ __global__ void __launch_bounds__(1024) filter_kernel(FilterArgs a)
 }
 
 // bucket[i] = the sum of the filter workgroups' rows of bucket_part (integer sums: the
-// same matrix in any order).
+// same matrix in any order); mode 1 adds the sum to bucket[i], mode 2 subtracts it.
 __global__ void __launch_bounds__(256) bucket_reduce_kernel(const uint32_t* __restrict__ part, uint32_t nrows,
-                                                            uint32_t nbb, uint32_t* __restrict__ bucket)
+                                                            uint32_t nbb, uint32_t* __restrict__ bucket, uint32_t mode)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nbb) return;
@@ -1667,7 +1706,8 @@ __global__ void __launch_bounds__(256) bucket_reduce_kernel(const uint32_t* __re
         s3 += part[(uint64_t)(r + 3) * nbb + i];
     }
     for (; r < nrows; ++r) s0 += part[(uint64_t)r * nbb + i];
-    bucket[i] = s0 + s1 + s2 + s3;
+    const uint32_t sum = s0 + s1 + s2 + s3;
+    bucket[i] = mode == 0 ? sum : (mode == 1 ? bucket[i] + sum : bucket[i] - sum);
 }
 
 // Per-block features of the learned scheduler, one thread per block b, from the bucket
@@ -1754,15 +1794,20 @@ hipError_t launch_filter(const FilterArgs& a, uint32_t grid, hipStream_t s)
     const uint64_t need = (a.nwalkers + kFilterThreads * kFilterItems - 1) / (kFilterThreads * kFilterItems);
     if (grid == 0 || grid > need) grid = (uint32_t)(need < 65536 ? need : 65536);
     const uint32_t nbb = a.nblocks * a.nblocks;
-    const bool lds_hist = a.bucket && nbb <= kFilterLdsBuckets;
+    const bool lds_hist = (a.bucket || a.bucket_sub) && nbb <= kFilterLdsBuckets;
     FilterArgs f = a;
     if (!lds_hist || a.part_rows == 0) f.bucket_part = nullptr;
     if (f.bucket_part && grid > a.part_rows) grid = a.part_rows;
     const size_t lds = lds_hist ? nbb * 4 : 0;
-    hipLaunchKernelGGL(filter_kernel, dim3(grid), dim3(kFilterThreads), lds, s, f);
+    if (f.gather)
+        hipLaunchKernelGGL((filter_kernel<true, false>), dim3(grid), dim3(kFilterThreads), lds, s, f);
+    else if (f.bucket_sub)
+        hipLaunchKernelGGL((filter_kernel<false, true>), dim3(grid), dim3(kFilterThreads), lds, s, f);
+    else
+        hipLaunchKernelGGL((filter_kernel<false, false>), dim3(grid), dim3(kFilterThreads), lds, s, f);
     if (f.bucket_part)
         hipLaunchKernelGGL(bucket_reduce_kernel, dim3((nbb + 255) / 256), dim3(256), 0, s, f.bucket_part, grid, nbb,
-                           f.bucket);
+                           f.bucket_sub ? f.bucket_sub : f.bucket, f.bucket_sub ? 2u : (f.gather ? 1u : 0u));
     return hipGetLastError();
 }
 

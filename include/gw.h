@@ -101,8 +101,11 @@ typedef struct {
 } gw_launch;
 
 enum {
-    GW_RUN_BUCKETS = 1u << 0, /* after the run, recount the (prev_blk, cur_blk) bucket matrix of
-                                 the walkers left, so gw_bucket_counts needs no extra pass */
+    GW_RUN_BUCKETS = 1u << 0, /* after the run, the (prev_blk, cur_blk) bucket matrix of the
+                                 walkers left, so gw_bucket_counts needs no extra pass: kept
+                                 current from the run's worklist when it was current before
+                                 (the filter takes the listed walkers out, a pass over the
+                                 list puts the unfinished ones back), else recounted */
     GW_RUN_TRACE = 1u << 1,   /* record steps per current block (gw_trace_read) */
     GW_RUN_INTERP = 1u << 2,  /* membership tests by guarded interpolation search instead of
                                  bisection: same walks, fewer reads on evenly spread ids
@@ -126,7 +129,8 @@ typedef struct {
     uint64_t walkers_ended;  /* walkers that finished during this run */
     uint64_t walkers_left;   /* unfinished walkers after this run */
     double kernel_ms;        /* step kernel time (events on the compute stream) */
-    double filter_ms;        /* bucket pass time (GW_RUN_BUCKETS) */
+    double filter_ms;        /* bucket pass time (GW_RUN_BUCKETS): the recount, or the pass
+                                over the worklist */
     uint64_t zc_touches;     /* reads that went to host-mapped blocks (row, candidate, probe);
                                 about 64 B of PCIe traffic each */
 } gw_run_stats;
